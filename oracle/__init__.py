@@ -1,0 +1,194 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes front end of ``sht_oracle.c``, a CPU restatement of the SHTOOLS
+Fortran transforms.  It mirrors the pyshtools f2py signatures and defaults
+(src/pyshtools.pyf:235-276,321-377,1895-1905) so parity tests read like the
+reference's own calls.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+CPU-baseline leg of ``bench.py`` may use it, and only as the checker.
+
+Every function is pinned by checks that do not retype its formulas
+(tests/test_oracle.py): mpmath closed forms of the associated Legendre
+functions in all four normalisations with and without the Condon-Shortley
+phase, mpmath DH weights and Gauss-Legendre nodes, brute-force direct
+summation of small expansions at every grid node, quadrature exactness,
+round trips, numpy.fft for the longitude DFT and the known answers of the
+reference's normalisation notebook.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBPATH = os.path.join(_HERE, "libsht_oracle.so")
+_lib = None
+
+
+class OracleError(Exception):
+    pass
+
+
+_MESSAGES = {
+    1: "Improper dimensions of input array.",
+    2: "Improper bounds for input variable.",
+    3: "Error allocating memory.",
+    4: "File IO error.",
+}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIBPATH):
+            build()
+        L = ctypes.CDLL(_LIBPATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        i = ctypes.c_int
+        L.or_dhaj.argtypes = [i, i, dp]
+        L.or_preglq.argtypes = [i, dp, dp]
+        L.or_glqgridcoord.argtypes = [i, i, dp, dp]
+        L.or_plm.argtypes = [i, i, i, ctypes.c_double, dp]
+        L.or_makegriddh.argtypes = [dp, i, i, i, i, i, i, i, i, dp, i, i]
+        L.or_shexpanddh.argtypes = [dp, i, i, i, i, i, i, dp, i, i]
+        L.or_makegridglq.argtypes = [dp, i, i, i, dp, i, i, i, i, i, dp, i, i]
+        L.or_shexpandglq.argtypes = [dp, i, i, i, dp, i, dp, i, i, i, i, dp, i, i]
+        L.or_r2c.argtypes = [i, dp, dp]
+        L.or_c2r.argtypes = [i, dp, dp]
+        lp = ctypes.POINTER(ctypes.c_longlong)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.or_roundtrip_dh_batch.argtypes = [i, ip, lp, lp, i, i, i, dp, dp, dp, i]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check(st):
+    if st != 0:
+        raise OracleError(_MESSAGES.get(st, "Unhandled Fortran 95 error."))
+
+
+def DHaj(n, extend=0):
+    aj = np.zeros(n + extend)
+    _check(lib().or_dhaj(n, extend, _p(aj)))
+    return aj
+
+
+def SHGLQ(lmax):
+    zero = np.zeros(lmax + 1)
+    w = np.zeros(lmax + 1)
+    _check(lib().or_preglq(lmax + 1, _p(zero), _p(w)))
+    return zero, w
+
+
+def GLQGridCoord(lmax, extend=0):
+    lat = np.zeros(lmax + 1)
+    lon = np.zeros(2 * lmax + 1 + extend)
+    _check(lib().or_glqgridcoord(lmax, extend, _p(lat), _p(lon)))
+    return lat, lon
+
+
+def plm(lmax, z, norm=1, csphase=1):
+    """P_lm(z) of the transforms' recursion, as an [l, m] array."""
+    p = np.zeros((lmax + 1, lmax + 1))
+    _check(lib().or_plm(lmax, norm, csphase, float(z), _p(p)))
+    return p
+
+
+def MakeGridDH(cilm, lmax=None, norm=1, sampling=1, csphase=1, lmax_calc=None, extend=0):
+    cilm = np.ascontiguousarray(cilm, dtype=np.float64)
+    if lmax is None:
+        lmax = cilm.shape[1] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    g0 = 2 * (lmax + 1) + extend
+    g1 = sampling * 2 * (lmax + 1) + extend
+    grid = np.zeros((g0, g1))
+    _check(lib().or_makegriddh(_p(cilm), cilm.shape[1], cilm.shape[2], lmax, norm, sampling,
+                               csphase, lmax_calc, extend, _p(grid), g0, g1))
+    return grid
+
+
+def SHExpandDH(griddh, norm=1, sampling=1, csphase=1, lmax_calc=None):
+    grid = np.ascontiguousarray(griddh, dtype=np.float64)
+    n = grid.shape[0]
+    if lmax_calc is None:
+        lmax_calc = n // 2 - 1
+    cilm = np.zeros((2, lmax_calc + 1, lmax_calc + 1))
+    _check(lib().or_shexpanddh(_p(grid), n, grid.shape[1], norm, sampling, csphase, lmax_calc,
+                               _p(cilm), lmax_calc + 1, lmax_calc + 1))
+    return cilm
+
+
+def MakeGridGLQ(cilm, zero, lmax=None, norm=1, csphase=1, lmax_calc=None, extend=0):
+    cilm = np.ascontiguousarray(cilm, dtype=np.float64)
+    zero = np.ascontiguousarray(zero, dtype=np.float64)
+    if lmax is None:
+        lmax = cilm.shape[1] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    g0, g1 = lmax + 1, 2 * lmax + 1 + extend
+    grid = np.zeros((g0, g1))
+    _check(lib().or_makegridglq(_p(cilm), cilm.shape[1], cilm.shape[2], lmax, _p(zero), zero.size,
+                                norm, csphase, lmax_calc, extend, _p(grid), g0, g1))
+    return grid
+
+
+def SHExpandGLQ(gridglq, w, zero, norm=1, csphase=1, lmax_calc=None):
+    grid = np.ascontiguousarray(gridglq, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    zero = np.ascontiguousarray(zero, dtype=np.float64)
+    lmax = grid.shape[0] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    cilm = np.zeros((2, lmax_calc + 1, lmax_calc + 1))
+    _check(lib().or_shexpandglq(_p(grid), grid.shape[0], grid.shape[1], lmax, _p(w), w.size,
+                                _p(zero), zero.size, norm, csphase, lmax_calc, _p(cilm),
+                                lmax_calc + 1, lmax_calc + 1))
+    return cilm
+
+
+def rfft(x):
+    """FFTW-style unnormalised r2c of a real vector."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = x.size
+    out = np.zeros(2 * (n // 2 + 1))
+    lib().or_r2c(n, _p(x), _p(out))
+    return out[0::2] + 1j * out[1::2]
+
+
+def irfft(c, n):
+    """FFTW-style unnormalised c2r to length n."""
+    c = np.ascontiguousarray(c, dtype=np.complex128)
+    ri = np.zeros(2 * (n // 2 + 1))
+    ri[0::2] = c.real[: n // 2 + 1]
+    ri[1::2] = c.imag[: n // 2 + 1]
+    out = np.zeros(n)
+    lib().or_c2r(n, _p(ri), _p(out))
+    return out
+
+
+def roundtrip_dh_batch(cilms, norm=1, sampling=2, csphase=1, nthreads=1):
+    """MakeGridDH -> SHExpandDH for a list of cilm arrays, OpenMP over fields."""
+    lmaxs = np.array([c.shape[1] - 1 for c in cilms], dtype=np.int32)
+    csz = np.array([2 * (L + 1) ** 2 for L in lmaxs], dtype=np.int64)
+    gsz = np.array([(2 * L + 2) * sampling * (2 * L + 2) for L in lmaxs], dtype=np.int64)
+    coff = np.concatenate([[0], np.cumsum(csz)[:-1]]).astype(np.int64)
+    goff = np.concatenate([[0], np.cumsum(gsz)[:-1]]).astype(np.int64)
+    cin = np.concatenate([np.ascontiguousarray(c, dtype=np.float64).ravel() for c in cilms])
+    grid = np.zeros(int(gsz.sum()))
+    out = np.zeros_like(cin)
+    st = lib().or_roundtrip_dh_batch(
+        len(cilms), lmaxs.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+        coff.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+        goff.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+        norm, sampling, csphase, _p(cin), _p(grid), _p(out), nthreads)
+    _check(st)
+    return [out[o:o + s].reshape(2, L + 1, L + 1) for o, s, L in zip(coff, csz, lmaxs)]

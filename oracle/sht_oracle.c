@@ -1,0 +1,921 @@
+/*
+ * ORACLE -- test infrastructure only.
+ *
+ * A plain-C restatement of the SHTOOLS Fortran 95 spherical-harmonic
+ * transforms (real, scalar, FP64) that the GPU library replaces.  It is the
+ * checker for parity tests, for `__graft_entry__.smoke()` and for the CPU
+ * baseline leg of `bench.py`; the shipped library never links, loads or calls
+ * it.  Every routine follows the reference's loop structure and arithmetic
+ * order so that it reproduces the reference's rounding closely:
+ *
+ *   or_dhaj            src/DHaj.f95:91-105
+ *   or_preglq          src/PreGLQ.f95:93-149 (x1=-1, x2=1 as src/SHGLQ.f95:182-192)
+ *   or_glqgridcoord    src/GLQGridCoord.f95:100-119
+ *   build_tables       src/MakeGridDH.f95:258-386 (identical in SHExpandDH,
+ *                      MakeGridGLQ, SHExpandGLQ)
+ *   or_plm             the transforms' own recursion for one z (tables above,
+ *                      seeds MakeGridDH.f95:542-603, scaling :228,:552)
+ *   or_makegriddh      src/MakeGridDH.f95:107-648
+ *   or_shexpanddh      src/SHExpandDH.f95:118-704
+ *   or_makegridglq     src/MakeGridGLQ.f95:192-678 (on-the-fly branch)
+ *   or_shexpandglq     src/SHExpandGLQ.f95:100-691 (on-the-fly branch)
+ *
+ * The longitude stage of the reference calls FFTW 3.3.x (not part of the
+ * reference tree): r2c X_m = sum_k x_k e^{-2 pi i m k/N}, c2r
+ * x_k = sum_m X_m e^{+2 pi i m k/N} over the Hermitian extension, both
+ * unnormalised (src/FFTW3.f95:212-218,265-271).  Here that DFT is computed by
+ * radix-2 Cooley-Tukey for powers of two and by Bluestein's chirp-z
+ * convolution otherwise.
+ *
+ * Arrays use the Python (C-order) layouts of the pyshtools API:
+ * cilm[2][d1][d2] with cilm[i][l][m], grid[nlat][nlon] with rings contiguous.
+ * Return value: the reference's exitstatus codes (0 ok, 1 dimensions,
+ * 2 bounds, 3 allocation, 5 other).
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef double complex cplx;
+
+/* ------------------------------------------------------------------ FFT */
+
+typedef struct {
+    int n;       /* transform length */
+    int m;       /* power-of-two work length (== n when n is a power of 2) */
+    cplx *tw;    /* exp(-2 pi i k / m), k < m/2 */
+    cplx *chirp; /* Bluestein: exp(-i pi (j^2 mod 2n) / n), j < n */
+    cplx *bhat;  /* Bluestein: FFT_m of the conjugate chirp, scaled by 1/m */
+    cplx *work;  /* m scratch */
+} fftplan;
+
+static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+static void fft_pow2(cplx *a, int n, const cplx *tw, int inverse)
+{
+    for (int i = 1, j = 0; i < n; i++) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) { cplx t = a[i]; a[i] = a[j]; a[j] = t; }
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        int half = len >> 1, step = n / len;
+        for (int k = 0; k < half; k++) {
+            cplx w = tw[k * step];
+            if (inverse) w = conj(w);
+            for (int i = k; i < n; i += len) {
+                cplx u = a[i], v = a[i + half] * w;
+                a[i] = u + v;
+                a[i + half] = u - v;
+            }
+        }
+    }
+}
+
+static fftplan *plan_create(int n)
+{
+    fftplan *p = (fftplan *)calloc(1, sizeof(fftplan));
+    if (!p) return NULL;
+    p->n = n;
+    if (is_pow2(n)) p->m = n;
+    else { p->m = 1; while (p->m < 2 * n - 1) p->m <<= 1; }
+    int m = p->m;
+    p->tw = (cplx *)malloc(sizeof(cplx) * (m / 2 + 1));
+    p->work = (cplx *)malloc(sizeof(cplx) * m);
+    for (int k = 0; k < m / 2 + 1; k++) {
+        double ang = -2.0 * M_PI * (double)k / (double)m;
+        p->tw[k] = cos(ang) + I * sin(ang);
+    }
+    if (m != n) {
+        p->chirp = (cplx *)malloc(sizeof(cplx) * n);
+        p->bhat = (cplx *)malloc(sizeof(cplx) * m);
+        for (int j = 0; j < n; j++) {
+            long long q = ((long long)j * j) % (2LL * n);
+            double ang = -M_PI * (double)q / (double)n;
+            p->chirp[j] = cos(ang) + I * sin(ang);
+        }
+        for (int j = 0; j < m; j++) p->bhat[j] = 0;
+        p->bhat[0] = conj(p->chirp[0]);
+        for (int j = 1; j < n; j++) {
+            p->bhat[j] = conj(p->chirp[j]);
+            p->bhat[m - j] = conj(p->chirp[j]);
+        }
+        fft_pow2(p->bhat, m, p->tw, 0);
+        for (int j = 0; j < m; j++) p->bhat[j] /= (double)m;
+    }
+    return p;
+}
+
+static void plan_destroy(fftplan *p)
+{
+    if (!p) return;
+    free(p->tw); free(p->work); free(p->chirp); free(p->bhat);
+    free(p);
+}
+
+/* In-place unnormalised DFT, e^{-} (forward) or e^{+} (inverse). */
+static void plan_exec(fftplan *p, cplx *x, int inverse)
+{
+    int n = p->n, m = p->m;
+    if (m == n) { fft_pow2(x, n, p->tw, inverse); return; }
+    cplx *a = p->work;
+    for (int j = 0; j < n; j++) a[j] = (inverse ? conj(x[j]) : x[j]) * p->chirp[j];
+    for (int j = n; j < m; j++) a[j] = 0;
+    fft_pow2(a, m, p->tw, 0);
+    for (int j = 0; j < m; j++) a[j] *= p->bhat[j];
+    fft_pow2(a, m, p->tw, 1);
+    for (int k = 0; k < n; k++) {
+        cplx v = a[k] * p->chirp[k];
+        x[k] = inverse ? conj(v) : v;
+    }
+}
+
+/* FFTW-style r2c: out[0..n/2] from real in[0..n-1]. */
+static void r2c(fftplan *p, const double *in, cplx *buf, cplx *out)
+{
+    int n = p->n;
+    for (int k = 0; k < n; k++) buf[k] = in[k];
+    plan_exec(p, buf, 0);
+    for (int k = 0; k <= n / 2; k++) out[k] = buf[k];
+}
+
+/* FFTW-style c2r: real out[0..n-1] from the Hermitian half in[0..n/2]. */
+static void c2r(fftplan *p, const cplx *in, cplx *buf, double *out)
+{
+    int n = p->n;
+    buf[0] = creal(in[0]);
+    for (int k = 1; k < (n + 1) / 2; k++) {
+        buf[k] = in[k];
+        buf[n - k] = conj(in[k]);
+    }
+    if (n % 2 == 0) buf[n / 2] = creal(in[n / 2]);
+    plan_exec(p, buf, 1);
+    for (int k = 0; k < n; k++) out[k] = creal(buf[k]);
+}
+
+/* Exported for tests of the longitude stage against numpy.fft. */
+int or_r2c(int n, const double *in, double *out_ri)
+{
+    fftplan *p = plan_create(n);
+    cplx *buf = (cplx *)malloc(sizeof(cplx) * n), *o = (cplx *)malloc(sizeof(cplx) * (n / 2 + 1));
+    r2c(p, in, buf, o);
+    for (int k = 0; k <= n / 2; k++) { out_ri[2 * k] = creal(o[k]); out_ri[2 * k + 1] = cimag(o[k]); }
+    free(buf); free(o); plan_destroy(p);
+    return 0;
+}
+
+int or_c2r(int n, const double *in_ri, double *out)
+{
+    fftplan *p = plan_create(n);
+    cplx *buf = (cplx *)malloc(sizeof(cplx) * n), *i2 = (cplx *)malloc(sizeof(cplx) * (n / 2 + 1));
+    for (int k = 0; k <= n / 2; k++) i2[k] = in_ri[2 * k] + I * in_ri[2 * k + 1];
+    c2r(p, i2, buf, out);
+    free(buf); free(i2); plan_destroy(p);
+    return 0;
+}
+
+/* ------------------------------------------------------ quadrature */
+
+/* src/DHaj.f95:91-105 */
+int or_dhaj(int n, int extend, double *aj)
+{
+    if (n % 2 != 0) return 2;
+    if (extend != 0 && extend != 1) return 2;
+    double pi = acos(-1.0);
+    for (int j = 0; j <= n - 1; j++) {
+        double sum1 = 0.0;
+        for (int l = 0; l <= n / 2 - 1; l++)
+            sum1 = sum1 + sin((double)(2 * l + 1) * pi * (double)j / (double)n) / (double)(2 * l + 1);
+        aj[j] = sum1 * sin(pi * (double)j / (double)n) * sqrt(8.0) / (double)n;
+    }
+    if (extend == 1) aj[n] = 0.0;
+    return 0;
+}
+
+/* src/PreGLQ.f95:93-149 with x1 = -1, x2 = 1. */
+int or_preglq(int n, double *zero, double *w)
+{
+    const int itermax = 1000;
+    const double eps = 1.0e-15;
+    double pi = acos(-1.0);
+    double xm = 0.0, xu = 1.0;   /* (x2+x1)/2, (x2-x1)/2 */
+    for (int i = 0; i < n; i++) { zero[i] = 0.0; w[i] = 0.0; }
+    int mh = (n + 1) / 2;
+    for (int i = 1; i <= mh; i++) {
+        int iter = 0;
+        double z = cos(pi * (i - 0.25) / (n + 0.5)), z1, p1, p2, p3, pp;
+        for (;;) {
+            iter++;
+            p1 = 1.0; p2 = 0.0;
+            for (int j = 1; j <= n; j++) {
+                p3 = p2; p2 = p1;
+                p1 = ((double)(2 * j - 1) * z * p2 - (double)(j - 1) * p3) / (double)j;
+            }
+            pp = (double)n * (z * p1 - p2) / (z * z - 1.0);
+            z1 = z;
+            z = z1 - p1 / pp;
+            if (fabs(z - z1) <= eps) break;
+            if (iter > itermax) return 5;
+        }
+        zero[i - 1] = xm + xu * z;
+        zero[n - i] = xm - xu * z;
+        w[i - 1] = 2.0 * xu / ((1.0 - z * z) * pp * pp);
+        w[n - i] = w[i - 1];
+    }
+    return 0;
+}
+
+/* src/GLQGridCoord.f95:100-119 */
+int or_glqgridcoord(int lmax, int extend, double *lat, double *lon)
+{
+    if (extend != 0 && extend != 1) return 2;
+    int nlat = lmax + 1, nlong = 2 * lmax + 1, nlong_out = nlong + extend;
+    double *zero = (double *)malloc(sizeof(double) * nlat), *w = (double *)malloc(sizeof(double) * nlat);
+    int st = or_preglq(nlat, zero, w);
+    if (st == 0) {
+        double pi = acos(-1.0);
+        for (int i = 0; i < nlong_out; i++) lon[i] = 360.0 * (double)i / (double)nlong;
+        for (int i = 0; i < nlat; i++) lat[i] = asin(zero[i]) * 180.0 / pi;
+    }
+    free(zero); free(w);
+    return st;
+}
+
+/* ------------------------------------------------- recursion tables */
+
+typedef struct {
+    int lc;        /* lmax_comp */
+    double *sqr;   /* sqr[k] = sqrt(k), k = 0..2lc+1 (index 0 unused) */
+    double *ff1;   /* ff1[l*(lc+1)+m] */
+    double *ff2;
+} tables;
+
+/* column-major like the reference's ff1(l+1,m+1): consecutive l contiguous */
+#define FF(t, a, l, m) ((t)->a[(size_t)(m) * ((t)->lc + 1) + (l)])
+
+/* src/MakeGridDH.f95:291-381 */
+static int build_tables(tables *t, int lc, int norm)
+{
+    t->lc = lc;
+    t->sqr = (double *)calloc((size_t)(2 * lc + 2), sizeof(double));
+    t->ff1 = (double *)calloc((size_t)(lc + 1) * (lc + 1), sizeof(double));
+    t->ff2 = (double *)calloc((size_t)(lc + 1) * (lc + 1), sizeof(double));
+    if (!t->sqr || !t->ff1 || !t->ff2) return 3;
+    double *sqr = t->sqr;
+    for (int l = 1; l <= 2 * lc + 1; l++) sqr[l] = sqrt((double)l);
+    int l, m;
+    switch (norm) {
+    case 1: case 4:
+        if (lc != 0) { FF(t, ff1, 1, 0) = sqr[3]; FF(t, ff2, 1, 0) = 0.0; }
+        for (l = 2; l <= lc; l++) {
+            FF(t, ff1, l, 0) = sqr[2 * l - 1] * sqr[2 * l + 1] / (double)l;
+            FF(t, ff2, l, 0) = (double)(l - 1) * sqr[2 * l + 1] / sqr[2 * l - 3] / (double)l;
+            for (m = 1; m <= l - 2; m++) {
+                FF(t, ff1, l, m) = sqr[2 * l + 1] * sqr[2 * l - 1] / sqr[l + m] / sqr[l - m];
+                FF(t, ff2, l, m) = sqr[2 * l + 1] * sqr[l - m - 1] * sqr[l + m - 1] / sqr[2 * l - 3] / sqr[l + m] / sqr[l - m];
+            }
+            m = l - 1;   /* value of the DO variable after the loop */
+            FF(t, ff1, l, l - 1) = sqr[2 * l + 1] * sqr[2 * l - 1] / sqr[l + m] / sqr[l - m];
+            FF(t, ff2, l, l - 1) = 0.0;
+        }
+        break;
+    case 2:
+        if (lc != 0) { FF(t, ff1, 1, 0) = 1.0; FF(t, ff2, 1, 0) = 0.0; }
+        for (l = 2; l <= lc; l++) {
+            FF(t, ff1, l, 0) = (double)(2 * l - 1) / (double)l;
+            FF(t, ff2, l, 0) = (double)(l - 1) / (double)l;
+            for (m = 1; m <= l - 2; m++) {
+                FF(t, ff1, l, m) = (double)(2 * l - 1) / sqr[l + m] / sqr[l - m];
+                FF(t, ff2, l, m) = sqr[l - m - 1] * sqr[l + m - 1] / sqr[l + m] / sqr[l - m];
+            }
+            m = l - 1;
+            FF(t, ff1, l, l - 1) = (double)(2 * l - 1) / sqr[l + m] / sqr[l - m];
+            FF(t, ff2, l, l - 1) = 0.0;
+        }
+        break;
+    case 3:
+        for (l = 1; l <= lc; l++) {
+            FF(t, ff1, l, 0) = (double)(2 * l - 1) / (double)l;
+            FF(t, ff2, l, 0) = (double)(l - 1) / (double)l;
+            for (m = 1; m <= l - 1; m++) {
+                FF(t, ff1, l, m) = (double)(2 * l - 1) / (double)(l - m);
+                FF(t, ff2, l, m) = (double)(l + m - 1) / (double)(l - m);
+            }
+        }
+        break;
+    }
+    return 0;
+}
+
+static void free_tables(tables *t) { free(t->sqr); free(t->ff1); free(t->ff2); }
+
+static inline int fsym(int l, int m) { return ((l - m) % 2 == 0) ? 1 : -1; }
+
+/* Seed of order 0: MakeGridDH.f95:516-519 */
+static double pm00(int norm) { return norm == 4 ? 1.0 / sqrt(4.0 * acos(-1.0)) : 1.0; }
+
+/* Scaled seed chain start: MakeGridDH.f95:542-546 */
+static double pmm_start(int norm, const double *sqr)
+{
+    double scalef = 1.0e-280, pi = acos(-1.0);
+    switch (norm) {
+    case 1: case 2: return sqr[2] * scalef;
+    case 3: return scalef;
+    default: return sqr[2] * scalef / sqrt(4.0 * pi);
+    }
+}
+
+/* One step of the seed chain for 1 <= m < lc: returns pm2 (the P_mm used in
+ * the sums) and updates pmm.  MakeGridDH.f95:554-564 */
+static double pmm_step(int norm, int phase, double *pmm, int m, const double *sqr)
+{
+    switch (norm) {
+    case 1: case 4:
+        *pmm = phase * *pmm * sqr[2 * m + 1] / sqr[2 * m];
+        return *pmm;
+    case 2:
+        *pmm = phase * *pmm * sqr[2 * m + 1] / sqr[2 * m];
+        return *pmm / sqr[2 * m + 1];
+    default:
+        *pmm = phase * *pmm * (double)(2 * m - 1);
+        return *pmm;
+    }
+}
+
+/* Last sectoral term m = lc (MakeGridDH.f95:595-603), times rescalem. */
+static double pmm_last(int norm, int phase, double pmm, int lc, const double *sqr, double rescalem)
+{
+    switch (norm) {
+    case 1: case 4: return phase * pmm * sqr[2 * lc + 1] / sqr[2 * lc] * rescalem;
+    case 2: return phase * pmm / sqr[2 * lc] * rescalem;
+    default: return phase * pmm * (double)(2 * lc - 1) * rescalem;
+    }
+}
+
+/* All P_lm(z), l,m <= lmax, with the transforms' recursion, seeds and
+ * 1e-280 scaling; p[l*(lmax+1)+m].  Used to pin the recursion against
+ * closed forms. */
+int or_plm(int lmax, int norm, int csphase, double z, double *p)
+{
+    if (norm < 1 || norm > 4 || (csphase != 1 && csphase != -1) || lmax < 0) return 2;
+    tables t;
+    int st = build_tables(&t, lmax, norm);
+    if (st) { free_tables(&t); return st; }
+    const double *sqr = t.sqr;
+    int L1 = lmax + 1;
+    for (int i = 0; i < L1 * L1; i++) p[i] = 0.0;
+    double u = sqrt((1.0 - z) * (1.0 + z));
+    double pm2 = pm00(norm), pm1, pv;
+    p[0] = pm2;
+    if (lmax == 0) { free_tables(&t); return 0; }
+    pm1 = FF(&t, ff1, 1, 0) * z * pm2;
+    p[1 * L1] = pm1;
+    for (int l = 2; l <= lmax; l++) {
+        pv = FF(&t, ff1, l, 0) * z * pm1 - FF(&t, ff2, l, 0) * pm2;
+        p[l * L1] = pv; pm2 = pm1; pm1 = pv;
+    }
+    double pmm = pmm_start(norm, sqr), rescalem = 1.0 / 1.0e-280;
+    for (int m = 1; m <= lmax - 1; m++) {
+        rescalem = rescalem * u;
+        pm2 = pmm_step(norm, csphase, &pmm, m, sqr);
+        p[m * L1 + m] = pm2 * rescalem;
+        pm1 = z * FF(&t, ff1, m + 1, m) * pm2;
+        p[(m + 1) * L1 + m] = pm1 * rescalem;
+        for (int l = m + 2; l <= lmax; l++) {
+            pv = z * FF(&t, ff1, l, m) * pm1 - FF(&t, ff2, l, m) * pm2;
+            pm2 = pm1; pm1 = pv;
+            p[l * L1 + m] = pv * rescalem;
+        }
+    }
+    rescalem = rescalem * u;
+    p[lmax * L1 + lmax] = pmm_last(norm, csphase, pmm, lmax, sqr, rescalem);
+    free_tables(&t);
+    return 0;
+}
+
+/* ------------------------------------------------------- synthesis */
+
+/* Coefficients in the reference's column-major order cf[m][l][i] (what f2py
+ * hands the Fortran: cilm(i,l,m) with i fastest), l, m <= lc. */
+static double *to_fortran_order(const double *cilm, int d1, int d2, int lc)
+{
+    size_t L1 = (size_t)lc + 1;
+    double *cf = (double *)calloc(2 * L1 * L1, sizeof(double));
+    if (!cf) return NULL;
+    for (int m = 0; m <= lc; m++)
+        for (int l = 0; l <= lc; l++)
+            for (int i = 0; i < 2; i++)
+                cf[(m * L1 + l) * 2 + i] = cilm[((size_t)i * d1 + l) * d2 + m];
+    return cf;
+}
+
+#define CF(i, l, m) cf[((size_t)(m) * (lc + 1) + (l)) * 2 + (i)]
+
+/* Per-ring Legendre sums for one ring (north: cr/ci) and its mirror
+ * (south: csr/csi); entry m>=1 is scaled by rescalem but NOT yet halved.
+ * (C - iS) * p is evaluated component-wise.  MakeGridDH.f95:510-609 /
+ * MakeGridGLQ.f95:548-642. */
+static void synth_ring(const tables *t, int norm, int phase, const double *cf,
+                       double z, double u, double *cr, double *ci, double *csr, double *csi,
+                       double *c0, double *c0s)
+{
+    const double *sqr = t->sqr;
+    int lc = t->lc;
+    double coef0 = 0.0, coef0s = 0.0, tempr, pm2, pm1, p, pmm, rescalem, tr, ti;
+    pm2 = pm00(norm);
+    tempr = CF(0, 0, 0) * pm2;
+    coef0 += tempr; coef0s += tempr;
+    pm1 = FF(t, ff1, 1, 0) * z * pm2;
+    tempr = CF(0, 1, 0) * pm1;
+    coef0 += tempr; coef0s -= tempr;
+    for (int l = 2; l <= lc; l++) {
+        p = FF(t, ff1, l, 0) * z * pm1 - FF(t, ff2, l, 0) * pm2;
+        tempr = CF(0, l, 0) * p;
+        coef0 += tempr;
+        coef0s += tempr * fsym(l, 0);
+        pm2 = pm1; pm1 = p;
+    }
+    pmm = pmm_start(norm, sqr);
+    rescalem = 1.0 / 1.0e-280;
+    for (int m = 1; m <= lc - 1; m++) {
+        double ar = 0.0, ai = 0.0, sr = 0.0, si = 0.0;
+        rescalem = rescalem * u;
+        pm2 = pmm_step(norm, phase, &pmm, m, sqr);
+        tr = CF(0, m, m) * pm2; ti = -CF(1, m, m) * pm2;
+        ar += tr; ai += ti; sr += tr; si += ti;
+        pm1 = z * FF(t, ff1, m + 1, m) * pm2;
+        tr = CF(0, m + 1, m) * pm1; ti = -CF(1, m + 1, m) * pm1;
+        ar += tr; ai += ti; sr -= tr; si -= ti;
+        for (int l = m + 2; l <= lc; l++) {
+            p = z * FF(t, ff1, l, m) * pm1 - FF(t, ff2, l, m) * pm2;
+            pm2 = pm1; pm1 = p;
+            tr = CF(0, l, m) * p; ti = -CF(1, l, m) * p;
+            ar += tr; ai += ti;
+            if ((l - m) % 2 == 0) { sr += tr; si += ti; } else { sr -= tr; si -= ti; }
+        }
+        cr[m] = ar * rescalem; ci[m] = ai * rescalem;
+        csr[m] = sr * rescalem; csi[m] = si * rescalem;
+    }
+    rescalem = rescalem * u;
+    pmm = pmm_last(norm, phase, pmm, lc, sqr, rescalem);
+    tr = CF(0, lc, lc) * pmm; ti = -CF(1, lc, lc) * pmm;
+    cr[lc] += tr; ci[lc] += ti; csr[lc] += tr; csi[lc] += ti;
+    *c0 = coef0; *c0s = coef0s;
+}
+
+/* Equator ring (z = 0, u = 1): MakeGridDH.f95:421-496; entry m>=1 is
+ * returned not yet scaled by rescalem/2. */
+static void synth_equator(const tables *t, int norm, int phase, const double *cf,
+                          double *cr, double *ci, double *c0, double *rescale_out)
+{
+    const double *sqr = t->sqr;
+    int lc = t->lc;
+    double coef0 = 0.0, pm2, p, pmm, rescalem;
+    pm2 = pm00(norm);
+    coef0 += CF(0, 0, 0) * pm2;
+    for (int l = 2; l <= lc; l += 2) {
+        p = -FF(t, ff2, l, 0) * pm2;
+        pm2 = p;
+        coef0 += CF(0, l, 0) * p;
+    }
+    pmm = pmm_start(norm, sqr);
+    rescalem = 1.0 / 1.0e-280;
+    for (int m = 1; m <= lc - 1; m++) {
+        pm2 = pmm_step(norm, phase, &pmm, m, sqr);
+        cr[m] += CF(0, m, m) * pm2; ci[m] += -CF(1, m, m) * pm2;
+        for (int l = m + 2; l <= lc; l += 2) {
+            p = -FF(t, ff2, l, m) * pm2;
+            cr[m] += CF(0, l, m) * p; ci[m] += -CF(1, l, m) * p;
+            pm2 = p;
+        }
+    }
+    pmm = pmm_last(norm, phase, pmm, lc, sqr, 1.0);
+    cr[lc] += CF(0, lc, lc) * pmm; ci[lc] += -CF(1, lc, lc) * pmm;
+    *c0 = coef0;
+    *rescale_out = rescalem;
+}
+
+static void pack_half(cplx *coef, const double *cr, const double *ci, int nmax, int ncl)
+{
+    for (int m = 0; m < ncl; m++) coef[m] = (m < nmax) ? (cr[m] + I * ci[m]) : 0.0;
+}
+
+int or_makegriddh(const double *cilm, int d1, int d2, int lmax, int norm, int sampling,
+                  int csphase, int lmax_calc, int extend, double *grid, int g0, int g1)
+{
+    int n = 2 * lmax + 2, nlong, nlat_out, nlong_out;
+    if (sampling == 1) nlong = n; else if (sampling == 2) nlong = 2 * n; else return 2;
+    if (extend == 0) { nlat_out = n; nlong_out = nlong; }
+    else if (extend == 1) { nlat_out = n + 1; nlong_out = nlong + 1; }
+    else return 2;
+    if (g0 < nlat_out || g1 < nlong_out) return 1;
+    if (norm < 1 || norm > 4) return 2;
+    if (csphase != -1 && csphase != 1) return 2;
+    if (lmax_calc > lmax) return 2;
+    int lc = lmax;
+    if (d2 - 1 < lc) lc = d2 - 1;
+    if (d1 - 1 < lc) lc = d1 - 1;
+    if (lmax_calc < lc) lc = lmax_calc;
+    double pi = acos(-1.0);
+#define G(i, k) grid[(size_t)(i) * g1 + (k)]
+    if (lc == 0) {
+        double pm2 = (norm == 4) ? 1.0 / sqrt(4.0 * pi) : 1.0;
+        for (int i = 0; i < nlat_out; i++)
+            for (int k = 0; k < nlong_out; k++) G(i, k) = cilm[0] * pm2;
+        return 0;
+    }
+    tables t;
+    int st = build_tables(&t, lc, norm);
+    double *cf = to_fortran_order(cilm, d1, d2, lc);
+    if (st || !cf) { free_tables(&t); free(cf); return 3; }
+    fftplan *plan = plan_create(nlong);
+    int ncl = nlong / 2 + 1;
+    cplx *coef = (cplx *)malloc(sizeof(cplx) * ncl);
+    cplx *buf = (cplx *)malloc(sizeof(cplx) * nlong);
+    double *row = (double *)malloc(sizeof(double) * nlong);
+    double *cr = (double *)malloc(sizeof(double) * 4 * (lmax + 2));
+    double *ci = cr + (lmax + 2), *csr = ci + (lmax + 2), *csi = csr + (lmax + 2);
+    int i_eq = n / 2 + 1;   /* 1-based, as in the reference */
+    double c0, c0s, rescalem;
+
+    /* equator: MakeGridDH.f95:421-502 */
+    for (int k = 0; k < 4 * (lmax + 2); k++) cr[k] = 0.0;
+    synth_equator(&t, norm, csphase, cf, cr, ci, &c0, &rescalem);
+    cr[0] = c0; ci[0] = 0.0;
+    for (int m = 1; m <= lmax; m++) { cr[m] = cr[m] * rescalem / 2.0; ci[m] = ci[m] * rescalem / 2.0; }
+    pack_half(coef, cr, ci, lmax + 1, ncl);
+    c2r(plan, coef, buf, row);
+    for (int k = 0; k < nlong; k++) G(i_eq - 1, k) = row[k];
+
+    /* ring pairs: MakeGridDH.f95:504-639 */
+    for (int i = 1; i <= i_eq - 1; i++) {
+        int i_s = 2 * i_eq - i;
+        double theta = pi * (double)(i - 1) / (double)n;
+        double z = cos(theta);
+        double u = sqrt((1.0 - z) * (1.0 + z));
+        for (int k = 0; k < 4 * (lmax + 2); k++) cr[k] = 0.0;
+        synth_ring(&t, norm, csphase, cf, z, u, cr, ci, csr, csi, &c0, &c0s);
+        cr[0] = c0; ci[0] = 0.0;
+        for (int m = 1; m <= lmax; m++) { cr[m] = cr[m] / 2.0; ci[m] = ci[m] / 2.0; }
+        pack_half(coef, cr, ci, lmax + 1, ncl);
+        c2r(plan, coef, buf, row);
+        for (int k = 0; k < nlong; k++) G(i - 1, k) = row[k];
+        if (!(i == 1 && extend == 0)) {
+            csr[0] = c0s; csi[0] = 0.0;
+            for (int m = 1; m <= lmax; m++) { csr[m] = csr[m] / 2.0; csi[m] = csi[m] / 2.0; }
+            pack_half(coef, csr, csi, lmax + 1, ncl);
+            c2r(plan, coef, buf, row);
+            for (int k = 0; k < nlong; k++) G(i_s - 1, k) = row[k];
+        }
+    }
+    if (extend == 1)
+        for (int i = 0; i < nlat_out; i++) G(i, nlong_out - 1) = G(i, 0);
+#undef G
+    free(coef); free(buf); free(row); free(cr); free(cf);
+    plan_destroy(plan);
+    free_tables(&t);
+    return 0;
+}
+
+int or_makegridglq(const double *cilm, int d1, int d2, int lmax, const double *zero, int nzero,
+                   int norm, int csphase, int lmax_calc, int extend, double *grid, int g0, int g1)
+{
+    int nlong = 2 * lmax + 1, nlat = lmax + 1, nlong_out;
+    if (extend == 0) nlong_out = nlong; else if (extend == 1) nlong_out = nlong + 1; else return 2;
+    if (g1 < nlong_out || g0 < nlat) return 1;
+    if (!zero) return 5;
+    if (nzero < lmax + 1) return 1;
+    if (norm < 1 || norm > 4) return 2;
+    if (csphase != -1 && csphase != 1) return 2;
+    if (lmax_calc > lmax) return 2;
+    int lc = lmax;
+    if (d2 - 1 < lc) lc = d2 - 1;
+    if (d1 - 1 < lc) lc = d1 - 1;
+    if (lmax_calc < lc) lc = lmax_calc;
+    double pi = acos(-1.0);
+#define G(i, k) grid[(size_t)(i) * g1 + (k)]
+    if (lc == 0) {
+        double pm2 = (norm == 4) ? 1.0 / sqrt(4.0 * pi) : 1.0;
+        for (int i = 0; i < nlat; i++)
+            for (int k = 0; k < nlong_out; k++) G(i, k) = cilm[0] * pm2;
+        return 0;
+    }
+    tables t;
+    int st = build_tables(&t, lc, norm);
+    double *cf = to_fortran_order(cilm, d1, d2, lc);
+    if (st || !cf) { free_tables(&t); free(cf); return 3; }
+    fftplan *plan = plan_create(nlong);
+    int ncl = nlong / 2 + 1;   /* == lmax + 1 */
+    cplx *coef = (cplx *)malloc(sizeof(cplx) * ncl);
+    cplx *buf = (cplx *)malloc(sizeof(cplx) * nlong);
+    double *row = (double *)malloc(sizeof(double) * nlong);
+    double *cr = (double *)malloc(sizeof(double) * 4 * (lmax + 2));
+    double *ci = cr + (lmax + 2), *csr = ci + (lmax + 2), *csi = csr + (lmax + 2);
+    for (int i = 1; i <= (nlat + 1) / 2; i++) {
+        double c0, c0s, rescalem;
+        for (int k = 0; k < 4 * (lmax + 2); k++) cr[k] = 0.0;
+        if (i == (nlat + 1) / 2 && nlat % 2 != 0) {
+            /* exact equator: MakeGridGLQ.f95:472-545 */
+            synth_equator(&t, norm, csphase, cf, cr, ci, &c0, &rescalem);
+            cr[0] = c0; ci[0] = 0.0;
+            for (int m = 1; m <= lc; m++) { cr[m] = cr[m] * rescalem / 2.0; ci[m] = ci[m] * rescalem / 2.0; }
+            pack_half(coef, cr, ci, lc + 1, ncl);
+            c2r(plan, coef, buf, row);
+            for (int k = 0; k < nlong; k++) G(i - 1, k) = row[k];
+        } else {
+            double z = zero[i - 1];
+            double u = sqrt((1.0 - z) * (1.0 + z));
+            int i_s = nlat + 1 - i;
+            synth_ring(&t, norm, csphase, cf, z, u, cr, ci, csr, csi, &c0, &c0s);
+            cr[0] = c0; ci[0] = 0.0;
+            for (int m = 1; m <= lc; m++) { cr[m] = cr[m] / 2.0; ci[m] = ci[m] / 2.0; }
+            pack_half(coef, cr, ci, lc + 1, ncl);
+            c2r(plan, coef, buf, row);
+            for (int k = 0; k < nlong; k++) G(i - 1, k) = row[k];
+            csr[0] = c0s; csi[0] = 0.0;
+            for (int m = 1; m <= lc; m++) { csr[m] = csr[m] / 2.0; csi[m] = csi[m] / 2.0; }
+            pack_half(coef, csr, csi, lc + 1, ncl);
+            c2r(plan, coef, buf, row);
+            for (int k = 0; k < nlong; k++) G(i_s - 1, k) = row[k];
+        }
+    }
+    if (extend == 1)
+        for (int i = 0; i < nlat; i++) G(i, nlong_out - 1) = G(i, 0);
+#undef G
+    free(coef); free(buf); free(row); free(cr); free(cf);
+    plan_destroy(plan);
+    free_tables(&t);
+    return 0;
+}
+
+/* -------------------------------------------------------- analysis */
+
+/* Accumulate one ring pair into cf (column-major [m][l][i]); f1/f2 are the
+ * weighted Fourier coefficients [i][m] of the north/south rings.
+ * SHExpandDH.f95:481-580 / SHExpandGLQ.f95:555-645. */
+static void anal_ring(const tables *t, int norm, int phase, double z, double u,
+                      double *f1, double *f2, int nf, double *cf)
+{
+    const double *sqr = t->sqr;
+    int lc = t->lc;
+#define F1(i, m) f1[(size_t)(i) * nf + (m)]
+#define F2(i, m) f2[(size_t)(i) * nf + (m)]
+    double pm2, pm1, p, pmm, rescalem, ffc[2][2];
+    pm2 = pm00(norm);
+    CF(0, 0, 0) += pm2 * (F1(0, 0) + F2(0, 0));
+    if (lc == 0) return;
+    pm1 = FF(t, ff1, 1, 0) * z * pm2;
+    CF(0, 1, 0) += pm1 * (F1(0, 0) - F2(0, 0));
+    ffc[0][0] = F1(0, 0) - F2(0, 0);   /* [.][0] : fsymsign = -1 */
+    ffc[0][1] = F1(0, 0) + F2(0, 0);   /* [.][1] : fsymsign = +1 */
+    for (int l = 2; l <= lc; l++) {
+        p = FF(t, ff1, l, 0) * z * pm1 - FF(t, ff2, l, 0) * pm2;
+        pm2 = pm1; pm1 = p;
+        CF(0, l, 0) += p * ffc[0][fsym(l, 0) > 0];
+    }
+    pmm = pmm_start(norm, sqr);
+    rescalem = 1.0 / 1.0e-280;
+    for (int m = 1; m <= lc - 1; m++) {
+        rescalem = rescalem * u;
+        pm2 = pmm_step(norm, phase, &pmm, m, sqr);
+        for (int i = 0; i < 2; i++) { F1(i, m) = F1(i, m) * rescalem; F2(i, m) = F2(i, m) * rescalem; }
+        for (int i = 0; i < 2; i++) CF(i, m, m) += pm2 * (F1(i, m) + F2(i, m));
+        pm1 = z * FF(t, ff1, m + 1, m) * pm2;
+        for (int i = 0; i < 2; i++) CF(i, m + 1, m) += pm1 * (F1(i, m) - F2(i, m));
+        for (int i = 0; i < 2; i++) {
+            ffc[i][0] = F1(i, m) - F2(i, m);
+            ffc[i][1] = F1(i, m) + F2(i, m);
+        }
+        for (int l = m + 2; l <= lc; l++) {
+            p = z * FF(t, ff1, l, m) * pm1 - FF(t, ff2, l, m) * pm2;
+            pm2 = pm1; pm1 = p;
+            int s = ((l - m) % 2 == 0);
+            CF(0, l, m) += p * ffc[0][s];
+            CF(1, l, m) += p * ffc[1][s];
+        }
+    }
+    rescalem = rescalem * u;
+    pmm = pmm_last(norm, phase, pmm, lc, sqr, rescalem);
+    for (int i = 0; i < 2; i++) CF(i, lc, lc) += pmm * (F1(i, lc) + F2(i, lc));
+#undef F1
+#undef F2
+}
+
+/* Equator: SHExpandDH.f95:584-664 / SHExpandGLQ.f95:470-553. */
+static void anal_equator(const tables *t, int norm, int phase, double *f1, int nf, double *cf)
+{
+    const double *sqr = t->sqr;
+    int lc = t->lc;
+#define F1(i, m) f1[(size_t)(i) * nf + (m)]
+    double pm2, p, pmm, rescalem;
+    pm2 = pm00(norm);
+    CF(0, 0, 0) += pm2 * F1(0, 0);
+    if (lc == 0) return;
+    for (int l = 2; l <= lc; l += 2) {
+        p = -FF(t, ff2, l, 0) * pm2;
+        pm2 = p;
+        CF(0, l, 0) += p * F1(0, 0);
+    }
+    pmm = pmm_start(norm, sqr);
+    rescalem = 1.0 / 1.0e-280;
+    for (int m = 1; m <= lc - 1; m++) {
+        pm2 = pmm_step(norm, phase, &pmm, m, sqr);
+        for (int i = 0; i < 2; i++) F1(i, m) = F1(i, m) * rescalem;
+        for (int i = 0; i < 2; i++) CF(i, m, m) += pm2 * F1(i, m);
+        for (int l = m + 2; l <= lc; l += 2) {
+            p = -FF(t, ff2, l, m) * pm2;
+            pm2 = p;
+            for (int i = 0; i < 2; i++) CF(i, l, m) += p * F1(i, m);
+        }
+    }
+    pmm = pmm_last(norm, phase, pmm, lc, sqr, rescalem);
+    for (int i = 0; i < 2; i++) CF(i, lc, lc) += pmm * F1(i, lc);
+#undef F1
+}
+
+/* Copy the column-major accumulator back to cilm[2][d1][d2] and apply the
+ * normalisation epilogue: SHExpandDH.f95:674-702. */
+static void anal_epilogue(int norm, int lc, const double *cf, double *cilm, int d1, int d2)
+{
+#define CI(i, l, m) cilm[((size_t)(i) * d1 + (l)) * d2 + (m)]
+    for (int l = 0; l <= lc; l++)
+        for (int m = 0; m <= l; m++)
+            for (int i = 0; i < 2; i++) CI(i, l, m) = CF(i, l, m);
+    double pi = acos(-1.0), prod;
+    switch (norm) {
+    case 1:
+        for (int l = 0; l <= lc; l++)
+            for (int m = 0; m <= l; m++)
+                for (int i = 0; i < 2; i++) CI(i, l, m) = CI(i, l, m) / (4 * pi);
+        break;
+    case 2:
+        for (int l = 0; l <= lc; l++)
+            for (int m = 0; m <= l; m++)
+                for (int i = 0; i < 2; i++) CI(i, l, m) = CI(i, l, m) * (2 * l + 1) / (4 * pi);
+        break;
+    case 3:
+        for (int l = 0; l <= lc; l++) {
+            prod = 4 * pi / (double)(2 * l + 1);
+            CI(0, l, 0) = CI(0, l, 0) / prod;
+            prod = prod / 2.0;
+            for (int m = 1; m <= l - 1; m++) {
+                prod = prod * (l + m) * (l - m + 1);
+                for (int i = 0; i < 2; i++) CI(i, l, m) = CI(i, l, m) / prod;
+            }
+            if (l != 0)
+                for (int i = 0; i < 2; i++) CI(i, l, l) = CI(i, l, l) / (prod * 2 * l);
+        }
+        break;
+    default:
+        break;
+    }
+#undef CI
+}
+
+int or_shexpanddh(const double *grid, int n, int g1, int norm, int sampling, int csphase,
+                  int lmax_calc, double *cilm, int d1, int d2)
+{
+    int lmax = n / 2 - 1, lc, nlong;
+    if (lmax_calc > lmax) return 2;
+    lc = lmax_calc < lmax ? lmax_calc : lmax;
+    if (sampling != 1 && sampling != 2) return 2;
+    if (n % 2 != 0) return 2;
+    if (d1 < lc + 1 || d2 < lc + 1) return 1;
+    nlong = (sampling == 1) ? n : 2 * n;
+    if (g1 < nlong) return 1;
+    if (csphase != -1 && csphase != 1) return 2;
+    if (norm < 1 || norm > 4) return 2;
+    double pi = acos(-1.0);
+    for (size_t k = 0; k < (size_t)2 * d1 * d2; k++) cilm[k] = 0.0;
+    double *aj = (double *)malloc(sizeof(double) * (n + 1));
+    or_dhaj(n, 0, aj);
+    for (int j = 0; j < n; j++) aj[j] = aj[j] * sqrt(4.0 * pi);
+    tables t;
+    int st = build_tables(&t, lc, norm);
+    double *cf = (double *)calloc((size_t)2 * (lc + 1) * (lc + 1), sizeof(double));
+    if (st || !cf) { free_tables(&t); free(aj); free(cf); return 3; }
+    fftplan *plan = plan_create(nlong);
+    cplx *buf = (cplx *)malloc(sizeof(cplx) * nlong);
+    cplx *cc = (cplx *)malloc(sizeof(cplx) * (nlong / 2 + 1));
+    int nf = n / 2 + 1;
+    double *f1 = (double *)calloc((size_t)2 * nf, sizeof(double));
+    double *f2 = (double *)calloc((size_t)2 * nf, sizeof(double));
+    int i_eq = n / 2 + 1;
+#define GR(i, k) grid[(size_t)(i) * g1 + (k)]
+    for (int i = 2; i <= i_eq - 1; i++) {
+        double theta = (i - 1) * pi / (double)n;
+        double z = cos(theta);
+        double u = sqrt((1.0 - z) * (1.0 + z));
+        r2c(plan, &GR(i - 1, 0), buf, cc);
+        for (int m = 0; m < n / 2; m++) {
+            f1[m] = sqrt(2 * pi) * aj[i - 1] * creal(cc[m]) / (double)nlong;
+            f1[nf + m] = -sqrt(2 * pi) * aj[i - 1] * cimag(cc[m]) / (double)nlong;
+        }
+        int i_s = 2 * i_eq - i;
+        r2c(plan, &GR(i_s - 1, 0), buf, cc);
+        for (int m = 0; m < n / 2; m++) {
+            f2[m] = sqrt(2 * pi) * aj[i_s - 1] * creal(cc[m]) / (double)nlong;
+            f2[nf + m] = -sqrt(2 * pi) * aj[i_s - 1] * cimag(cc[m]) / (double)nlong;
+        }
+        anal_ring(&t, norm, csphase, z, u, f1, f2, nf, cf);
+    }
+    {
+        int i = i_eq;
+        r2c(plan, &GR(i - 1, 0), buf, cc);
+        for (int m = 0; m < n / 2; m++) {
+            f1[m] = sqrt(2 * pi) * aj[i - 1] * creal(cc[m]) / (double)nlong;
+            f1[nf + m] = -sqrt(2 * pi) * aj[i - 1] * cimag(cc[m]) / (double)nlong;
+        }
+        anal_equator(&t, norm, csphase, f1, nf, cf);
+    }
+#undef GR
+    anal_epilogue(norm, lc, cf, cilm, d1, d2);
+    free(buf); free(cc); free(f1); free(f2); free(aj); free(cf);
+    plan_destroy(plan);
+    free_tables(&t);
+    return 0;
+}
+
+int or_shexpandglq(const double *grid, int g0, int g1, int lmax, const double *w, int nw,
+                   const double *zero, int nzero, int norm, int csphase, int lmax_calc,
+                   double *cilm, int d1, int d2)
+{
+    int lc, nlong = 2 * lmax + 1, nlat = lmax + 1;
+    if (lmax_calc > lmax) return 2;
+    lc = lmax_calc < lmax ? lmax_calc : lmax;
+    if (d1 < lc + 1 || d2 < lc + 1) return 1;
+    if (g1 < 2 * lmax + 1 || g0 < lmax + 1) return 1;
+    if (nw < lmax + 1) return 1;
+    if (!zero) return 5;
+    if (nzero < lmax + 1) return 1;
+    if (norm < 1 || norm > 4) return 2;
+    if (csphase != -1 && csphase != 1) return 2;
+    double pi = acos(-1.0);
+    for (size_t k = 0; k < (size_t)2 * d1 * d2; k++) cilm[k] = 0.0;
+    tables t;
+    int st = build_tables(&t, lc, norm);
+    double *cf = (double *)calloc((size_t)2 * (lc + 1) * (lc + 1), sizeof(double));
+    if (st || !cf) { free_tables(&t); free(cf); return 3; }
+    fftplan *plan = plan_create(nlong);
+    cplx *buf = (cplx *)malloc(sizeof(cplx) * nlong);
+    cplx *cc = (cplx *)malloc(sizeof(cplx) * (nlong / 2 + 1));
+    int nf = lmax + 1;
+    double *f1 = (double *)calloc((size_t)2 * nf, sizeof(double));
+    double *f2 = (double *)calloc((size_t)2 * nf, sizeof(double));
+#define GR(i, k) grid[(size_t)(i) * g1 + (k)]
+    for (int i = 1; i <= (nlat + 1) / 2; i++) {
+        r2c(plan, &GR(i - 1, 0), buf, cc);
+        for (int m = 0; m <= lmax; m++) {
+            f1[m] = 2 * w[i - 1] * pi * creal(cc[m]) / (double)nlong;
+            f1[nf + m] = -2 * w[i - 1] * pi * cimag(cc[m]) / (double)nlong;
+        }
+        if (i == (nlat + 1) / 2 && nlat % 2 != 0) {
+            anal_equator(&t, norm, csphase, f1, nf, cf);
+        } else {
+            double z = zero[i - 1];
+            double u = sqrt((1.0 - z) * (1.0 + z));
+            int i_s = nlat + 1 - i;
+            r2c(plan, &GR(i_s - 1, 0), buf, cc);
+            for (int m = 0; m <= lmax; m++) {
+                f2[m] = 2 * w[i_s - 1] * pi * creal(cc[m]) / (double)nlong;
+                f2[nf + m] = -2 * w[i_s - 1] * pi * cimag(cc[m]) / (double)nlong;
+            }
+            anal_ring(&t, norm, csphase, z, u, f1, f2, nf, cf);
+        }
+    }
+#undef GR
+    anal_epilogue(norm, lc, cf, cilm, d1, d2);
+    free(buf); free(cc); free(f1); free(f2); free(cf);
+    plan_destroy(plan);
+    free_tables(&t);
+    return 0;
+}
+
+/* ------------------------------------------- batched CPU round trips */
+
+/* One MakeGridDH + SHExpandDH round trip per field, fields spread over the
+ * OpenMP threads with each reference-style call single-threaded.  Inputs are
+ * packed back to back: field f has cilm at cilm + coff[f] (shape
+ * [2][L+1][L+1]), grid scratch at grid + goff[f] and output at out + coff[f].
+ * Returns the first nonzero status. */
+int or_roundtrip_dh_batch(int nfields, const int *lmax, const long long *coff, const long long *goff,
+                          int norm, int sampling, int csphase, const double *cilm, double *grid,
+                          double *out, int nthreads)
+{
+    int status = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(| : status)
+#endif
+    for (int f = 0; f < nfields; f++) {
+        int L = lmax[f], n = 2 * L + 2, nlong = sampling * n;
+        int s = or_makegriddh(cilm + coff[f], L + 1, L + 1, L, norm, sampling, csphase, L, 0,
+                              grid + goff[f], n, nlong);
+        if (!s) s = or_shexpanddh(grid + goff[f], n, nlong, norm, sampling, csphase, L,
+                                  out + coff[f], L + 1, L + 1);
+        status |= s;
+    }
+    (void)nthreads;
+    return status;
+}

@@ -1,0 +1,234 @@
+"""Pins the oracle (the CPU restatement of the SHTOOLS Fortran transforms).
+
+Each check comes from mathematics the oracle does not contain: mpmath
+closed forms and brute-force sums (tests/golden/make_golden.py), numpy's
+independent FFT and Gauss-Legendre routines, quadrature identities, round
+trips, and the known answers of the reference's normalisation notebook
+(examples/notebooks/spherical-harmonic-normalizations.ipynb cells 9 and 11).
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+
+def rand_cilm(rng, L, slope=-2.0):
+    c = rng.standard_normal((2, L + 1, L + 1))
+    l = np.arange(L + 1)
+    c *= np.sqrt((l + 1.0) ** slope / (2 * l + 1.0))[None, :, None]
+    for k in range(L + 1):
+        c[:, k, k + 1:] = 0.0
+    c[1, :, 0] = 0.0
+    return c
+
+
+def nrel(a, b):
+    return np.linalg.norm(np.ravel(a - b)) / max(np.linalg.norm(np.ravel(b)), 1e-300)
+
+
+# ---------------------------------------------------------------- Legendre
+
+@pytest.mark.parametrize("norm", [1, 2, 3, 4])
+@pytest.mark.parametrize("cs", [1, -1])
+def test_plm_matches_closed_form(golden, norm, cs):
+    L = int(golden["plm_lmax"])
+    ref = golden[f"plm_n{norm}_cs{cs}"]
+    for iz, z in enumerate(golden["plm_z"]):
+        p = oracle.plm(L, z, norm=norm, csphase=cs)
+        for l in range(L + 1):
+            scale = max(np.abs(ref[iz, l, : l + 1]).max(), 1e-300)
+            err = np.abs(p[l, : l + 1] - ref[iz, l, : l + 1]).max() / scale
+            assert err < 2e-14, (norm, cs, z, l, err)
+
+
+def test_plm_high_degree_scaling(golden):
+    for l, m, z, norm, val in golden["plmhi"]:
+        l, m, norm = int(l), int(m), int(norm)
+        p = oracle.plm(l, z, norm=norm)
+        got = p[l, m]
+        assert abs(got - val) <= 1e-11 * abs(val) + 1e-300, (l, m, z, norm, got, val)
+
+
+def test_plm_pole_values():
+    # P_l0(+-1) = (+-1)^l sqrt(2l+1) (4pi norm) and P_lm(+-1) = 0 for m > 0
+    L = 60
+    for z, s in ((1.0, 1.0), (-1.0, -1.0)):
+        p = oracle.plm(L, z, norm=1)
+        l = np.arange(L + 1)
+        np.testing.assert_allclose(p[:, 0], s ** l * np.sqrt(2 * l + 1.0), rtol=1e-14)
+        assert np.all(p[:, 1:] == 0.0)
+
+
+# ---------------------------------------------------------------- quadrature
+
+@pytest.mark.parametrize("n", [2, 4, 6, 8, 16, 64, 256])
+def test_dhaj_quadrature_exact(n):
+    aj = oracle.DHaj(n)
+    assert aj[0] == 0.0
+    theta = np.pi * np.arange(n) / n
+    x = np.cos(theta)
+    for k in range(n):
+        exact = np.sqrt(2.0) / (k + 1) if k % 2 == 0 else 0.0
+        assert abs(np.sum(aj * x ** k) - exact) < 2e-14, (n, k)
+    ext = oracle.DHaj(n, extend=1)
+    assert ext.size == n + 1 and ext[-1] == 0.0
+    np.testing.assert_array_equal(ext[:n], aj)
+
+
+def test_dhaj_rejects_odd_n():
+    with pytest.raises(oracle.OracleError, match="Improper bounds"):
+        oracle.DHaj(7)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 16, 33, 64])
+def test_glq_nodes_weights_mpmath(golden, n):
+    z, w = oracle.SHGLQ(n - 1)
+    np.testing.assert_allclose(z, golden[f"glq_x_{n}"], rtol=0, atol=2e-15)
+    np.testing.assert_allclose(w, golden[f"glq_w_{n}"], rtol=2e-13, atol=0)
+
+
+@pytest.mark.parametrize("n", [7, 50, 100])
+def test_glq_matches_numpy_and_integrates_exactly(n):
+    z, w = oracle.SHGLQ(n - 1)
+    xn, wn = np.polynomial.legendre.leggauss(n)
+    np.testing.assert_allclose(z[::-1], xn, atol=5e-15)
+    np.testing.assert_allclose(w[::-1], wn, rtol=0, atol=1e-14)
+    assert abs(w.sum() - 2.0) < 1e-13
+    for k in range(2 * n):
+        exact = 2.0 / (k + 1) if k % 2 == 0 else 0.0
+        assert abs(np.sum(w * z ** k) - exact) < 1e-13
+
+
+def test_glqgridcoord():
+    lat, lon = oracle.GLQGridCoord(10, extend=1)
+    z, _ = oracle.SHGLQ(10)
+    np.testing.assert_allclose(lat, np.degrees(np.arcsin(z)), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(lon, 360.0 * np.arange(22) / 21, atol=1e-12)
+
+
+# ---------------------------------------------------------------- longitude DFT
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 12, 17, 64, 97, 128, 250, 2047, 4096, 4084])
+def test_fft_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n)
+    X = oracle.rfft(x)
+    np.testing.assert_allclose(X, np.fft.rfft(x), rtol=0, atol=1e-12 * np.sqrt(n))
+    c = np.fft.rfft(x)
+    y = oracle.irfft(c, n)
+    np.testing.assert_allclose(y / n, x, atol=1e-13)
+
+
+# ---------------------------------------------------------------- synthesis
+
+@pytest.mark.parametrize("norm", [1, 2, 3, 4])
+@pytest.mark.parametrize("cs", [1, -1])
+@pytest.mark.parametrize("sampling", [1, 2])
+def test_makegriddh_brute_force(golden, norm, cs, sampling):
+    cilm = golden["bf_cilm"]
+    ref = golden[f"bf_dh_n{norm}_cs{cs}_s{sampling}"]
+    g = oracle.MakeGridDH(cilm, norm=norm, sampling=sampling, csphase=cs, extend=1)
+    assert g.shape == ref.shape
+    assert np.abs(g - ref).max() < 1e-13 * np.abs(ref).max()
+    g0 = oracle.MakeGridDH(cilm, norm=norm, sampling=sampling, csphase=cs)
+    np.testing.assert_array_equal(g0, g[:-1, :-1])
+
+
+@pytest.mark.parametrize("Lg", [5, 6])
+@pytest.mark.parametrize("norm", [1, 4])
+@pytest.mark.parametrize("cs", [1, -1])
+def test_makegridglq_brute_force(golden, Lg, norm, cs):
+    cilm = golden[f"bf_glq_cilm_{Lg}"]
+    ref = golden[f"bf_glq{Lg}_n{norm}_cs{cs}"]
+    z, _ = oracle.SHGLQ(Lg)
+    g = oracle.MakeGridGLQ(cilm, z, norm=norm, csphase=cs, extend=1)
+    assert g.shape == ref.shape
+    assert np.abs(g - ref).max() < 1e-13 * np.abs(ref).max()
+
+
+def test_lmax_calc_and_padding():
+    rng = np.random.default_rng(3)
+    c = rand_cilm(rng, 12)
+    g1 = oracle.MakeGridDH(c, lmax=20, lmax_calc=8, sampling=2)
+    c8 = np.zeros((2, 21, 21))
+    c8[:, :9, :9] = c[:, :9, :9]
+    g2 = oracle.MakeGridDH(c8, sampling=2)
+    assert g1.shape == (42, 84)
+    np.testing.assert_allclose(g1, g2, atol=1e-14)
+
+
+# ---------------------------------------------------------------- analysis
+
+@pytest.mark.parametrize("L", [0, 1, 2, 3, 7, 31, 63, 127])
+@pytest.mark.parametrize("norm", [1, 2, 3, 4])
+def test_roundtrip_dh(L, norm):
+    if norm == 3 and L > 15:
+        pytest.skip("unnormalised recursion is stable only to degree ~15 (MakeGridDH.f95:17-18)")
+    rng = np.random.default_rng(100 + L)
+    c = rand_cilm(rng, L)
+    for sampling in (1, 2):
+        for cs in (1, -1):
+            g = oracle.MakeGridDH(c, norm=norm, sampling=sampling, csphase=cs)
+            c2 = oracle.SHExpandDH(g, norm=norm, sampling=sampling, csphase=cs)
+            # unnormalised functions span many decades already at L = 7-15
+            assert nrel(c2, c) < (1e-13 if norm != 3 else 1e-11), (L, norm, sampling, cs)
+
+
+@pytest.mark.parametrize("L", [0, 1, 2, 3, 6, 7, 31, 64, 127])
+@pytest.mark.parametrize("norm", [1, 2, 3, 4])
+def test_roundtrip_glq(L, norm):
+    if norm == 3 and L > 15:
+        pytest.skip("unnormalised recursion is stable only to degree ~15")
+    rng = np.random.default_rng(200 + L)
+    c = rand_cilm(rng, L)
+    z, w = oracle.SHGLQ(L)
+    for cs in (1, -1):
+        g = oracle.MakeGridGLQ(c, z, norm=norm, csphase=cs)
+        c2 = oracle.SHExpandGLQ(g, w, z, norm=norm, csphase=cs)
+        assert nrel(c2, c) < (1e-13 if norm != 3 else 1e-11), (L, norm, cs)
+
+
+def test_analysis_lmax_calc_truncates():
+    rng = np.random.default_rng(5)
+    c = rand_cilm(rng, 20)
+    g = oracle.MakeGridDH(c, sampling=2)
+    c5 = oracle.SHExpandDH(g, sampling=2, lmax_calc=5)
+    assert c5.shape == (2, 6, 6)
+    assert nrel(c5, c[:, :6, :6]) < 1e-13
+
+
+def test_known_answers_normalisation_notebook():
+    # examples/notebooks/spherical-harmonic-normalizations.ipynb cells 4, 9, 11:
+    # a single 4pi-normalised Y_{5,2} at lmax=100 has N = 4 pi under both
+    # quadratures (printed 12.566370614359176 and 12.566370614359178).
+    lmax = 100
+    c = np.zeros((2, lmax + 1, lmax + 1))
+    c[0, 5, 2] = 1.0
+    z, w = oracle.SHGLQ(lmax)
+    g = oracle.MakeGridGLQ(c, z, extend=1)
+    nlon = g.shape[1] - 1
+    N = ((g[:, :nlon] ** 2) * w[:, None]).sum() * (2.0 * np.pi / nlon)
+    assert abs(N - 12.566370614359176) < 4e-15 * 12.6
+    g = oracle.MakeGridDH(c, extend=1)
+    n = g.shape[0] - 1
+    aj = oracle.DHaj(n)
+    N = ((g[:n, :n] ** 2) * aj[:, None]).sum() * 2.0 * np.sqrt(2.0) * np.pi / n
+    assert abs(N - 12.566370614359178) < 4e-15 * 12.6
+
+
+# ---------------------------------------------------------------- errors
+
+def test_error_codes_match_reference():
+    c = np.zeros((2, 5, 5))
+    with pytest.raises(oracle.OracleError, match="Improper bounds"):
+        oracle.MakeGridDH(c, sampling=3)
+    with pytest.raises(oracle.OracleError, match="Improper bounds"):
+        oracle.MakeGridDH(c, norm=5)
+    with pytest.raises(oracle.OracleError, match="Improper bounds"):
+        oracle.MakeGridDH(c, csphase=0)
+    with pytest.raises(oracle.OracleError, match="Improper bounds"):
+        oracle.MakeGridDH(c, lmax_calc=9)
+    with pytest.raises(oracle.OracleError, match="Improper bounds"):
+        oracle.SHExpandDH(np.zeros((7, 14)), sampling=2)
+    with pytest.raises(oracle.OracleError, match="Improper dimensions"):
+        oracle.SHExpandDH(np.zeros((8, 10)), sampling=2)

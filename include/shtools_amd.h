@@ -1,0 +1,109 @@
+/*
+ * shtools_amd -- MI355X-native real spherical-harmonic transforms (FP64).
+ *
+ * C ABI of libshtools_amd.so.  Each transform entry point replaces one
+ * SHTOOLS routine behind the pyshtools backend plugin API
+ * (pyshtools/backends/backends.py:26-31 registers backend modules whose
+ * MakeGridDH/SHExpandDH/MakeGridGLQ/SHExpandGLQ are injected into
+ * pyshtools.expand by pyshtools/expand/__init__.py:91-110).
+ *
+ * Conventions (all entry points):
+ *   - arrays are C-order (row-major) device pointers: cilm[2][d1][d2] with
+ *     cilm[i][l][m] (i = 0: C_lm, i = 1: S_lm), grid[nlat][nlon_alloc] with
+ *     latitude rings contiguous;
+ *   - work is enqueued on `stream` (a hipStream_t, NULL = default stream) and
+ *     the call returns before it completes;
+ *   - the return value is the reference's exitstatus: 0 ok, 1 improper array
+ *     dimensions, 2 improper bounds of an input variable, 3 allocation
+ *     failure, 5 other (e.g. neither zero nor plx given, no GPU);
+ *   - norm 1 = 4pi (geodesy), 2 = Schmidt, 3 = unnormalised, 4 = orthonormal;
+ *     csphase 1 omits and -1 applies the Condon-Shortley phase (-1)^m;
+ *   - calls are thread-safe; host-side tables are cached per (norm, csphase,
+ *     degree) and per grid.
+ */
+#ifndef SHTOOLS_AMD_H
+#define SHTOOLS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* MakeGridDH: src/MakeGridDH.f95:1-2 (C ABI include/shtools.h:104-116;
+ * f2py src/pyshtools.pyf:256-276).  Output grid has
+ * 2(lmax+1)+extend rows and sampling*2(lmax+1)+extend columns; grid_d0/d1
+ * are the allocated dimensions (must be at least that). */
+int shtg_makegrid_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax, int norm,
+                     int sampling, int csphase, int lmax_calc, int extend,
+                     double *grid, int grid_d0, int grid_d1, void *stream);
+
+/* SHExpandDH: src/SHExpandDH.f95:1-2 (include/shtools.h:92-102;
+ * src/pyshtools.pyf:235-254).  n = grid_d0 must be even; the grid has at
+ * least n (sampling 1) or 2n (sampling 2) columns.  cilm receives
+ * (2, lmax_calc+1, lmax_calc+1) coefficients in a [2][cilm_d1][cilm_d2]
+ * array that is fully overwritten (zero where m > l). */
+int shtg_shexpand_dh(const double *grid, int grid_d0, int grid_d1, int norm, int sampling,
+                     int csphase, int lmax_calc, double *cilm, int cilm_d1, int cilm_d2,
+                     void *stream);
+
+/* MakeGridGLQ, on-the-fly branch: src/MakeGridGLQ.f95:1-2,467-669
+ * (include/shtools.h:161-172; src/pyshtools.pyf:357-377).  `zero` is a HOST
+ * array of nzero >= lmax+1 Gauss-Legendre nodes (from shtg_shglq). */
+int shtg_makegrid_glq(const double *cilm, int cilm_d1, int cilm_d2, int lmax,
+                      const double *zero, int nzero, int norm, int csphase, int lmax_calc,
+                      int extend, double *grid, int grid_d0, int grid_d1, void *stream);
+
+/* SHExpandGLQ, on-the-fly branch: src/SHExpandGLQ.f95:1-2,445-649
+ * (include/shtools.h:150-159; src/pyshtools.pyf:334-355).  lmax =
+ * grid_d0 - 1; `w` and `zero` are HOST arrays of >= lmax+1 weights and
+ * nodes. */
+int shtg_shexpand_glq(const double *grid, int grid_d0, int grid_d1, const double *w, int nw,
+                      const double *zero, int nzero, int norm, int csphase, int lmax_calc,
+                      double *cilm, int cilm_d1, int cilm_d2, void *stream);
+
+/* Host helpers. */
+/* SHGLQ nodes and weights (include/shtools.h:142-148; src/SHGLQ.f95:187-192 ->
+ * src/PreGLQ.f95:93-149):
+ * zero[lmax+1] descending from ~+1, w[lmax+1]. */
+int shtg_shglq(int lmax, double *zero, double *w);
+/* DHaj (include/shtools.h:1034; src/DHaj.f95:91-105): aj[n+extend]
+ * Driscoll-Healy weights. */
+int shtg_dhaj(int n, int extend, double *aj);
+/* GLQGridCoord (include/shtools.h:198; src/GLQGridCoord.f95:100-119):
+ * lat[lmax+1], lon[2lmax+1+extend]. */
+int shtg_glqgridcoord(int lmax, int extend, double *lat, double *lon);
+
+/* Associated Legendre functions P_lm(z), l, m <= lmax, into the host array
+ * p[(lmax+1)*(lmax+1)] (row l, column m), computed on the host with the same
+ * tables, seeds, extended-range scaling and recursion as the GPU kernels;
+ * norm/csphase as above (the reference's PlmBar, PlmSchmidt, PLegendreA and
+ * PlmON: src/PlmBar.f95:153-288, include/shtools.h:20-90). */
+int shtg_plm(int lmax, int norm, int csphase, double z, double *p);
+
+/* Per-kernel timing with HIP events recorded on each launch stream (no
+ * counterpart in the reference; used by the benchmark).  While enabled,
+ * every launch is bracketed by two events; collect() waits for them and
+ * returns per kernel id the summed milliseconds and launch counts, then
+ * forgets them. */
+enum {
+    SHTG_K_PACK = 0,
+    SHTG_K_SYNTH_LEG = 1,
+    SHTG_K_SYNTH_FFT = 2,
+    SHTG_K_ANAL_FFT = 3,
+    SHTG_K_ANAL_LEG = 4,
+    SHTG_K_UNPACK = 5,
+    SHTG_K_COUNT = 16
+};
+int shtg_profile_enable(int on);
+int shtg_profile_collect(double *ms, long long *count, int nk);
+
+/* Library status. */
+int shtg_device_count(void);
+const char *shtg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SHTOOLS_AMD_H */

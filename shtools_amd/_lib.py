@@ -1,0 +1,48 @@
+"""ctypes binding of libshtools_amd.so (the C ABI of include/shtools_amd.h).
+
+There is no fallback: if the library is missing or no HIP device is
+visible, calls raise.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libshtools_amd.so")
+_lib = None
+
+_c_int = ctypes.c_int
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+SIGNATURES = {
+    "shtg_makegrid_dh": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _vp],
+    "shtg_shexpand_dh": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _vp],
+    "shtg_makegrid_glq": [_vp, _c_int, _c_int, _c_int, _dp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int,
+                          _c_int, _vp],
+    "shtg_shexpand_glq": [_vp, _c_int, _c_int, _dp, _c_int, _dp, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int,
+                          _vp],
+    "shtg_shglq": [_c_int, _dp, _dp],
+    "shtg_dhaj": [_c_int, _c_int, _dp],
+    "shtg_glqgridcoord": [_c_int, _c_int, _dp, _dp],
+    "shtg_plm": [_c_int, _c_int, _c_int, ctypes.c_double, _dp],
+    "shtg_profile_enable": [_c_int],
+    "shtg_profile_collect": [_dp, ctypes.POINTER(ctypes.c_longlong), _c_int],
+    "shtg_device_count": [],
+    "shtg_version": [],
+}
+
+
+def lib():
+    """Load the library (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise OSError(f"{LIBPATH} is missing: build it with `python -m shtools_amd.build`")
+        L = ctypes.CDLL(LIBPATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        L.shtg_version.restype = ctypes.c_char_p
+        _lib = L
+    return _lib

@@ -1,0 +1,289 @@
+"""pyshtools backend module running the transforms on an MI355X.
+
+Drop-in for the names that ``pyshtools.backends`` injects into
+``pyshtools.expand`` (pyshtools/expand/__init__.py:91-110), with the f2py
+signatures, defaults and errors of the Fortran backend
+(src/pyshtools.pyf:235-276,321-377,424-438,1895-1905;
+pyshtools/backends/shtools.py:189-229):
+
+    MakeGridDH(cilm, lmax=cilm.shape[1]-1, norm=1, sampling=1, csphase=1,
+               lmax_calc=lmax, extend=0) -> griddh
+    SHExpandDH(griddh, norm=1, sampling=1, csphase=1, lmax_calc=n//2-1) -> cilm
+    MakeGridGLQ(cilm, zero, lmax=cilm.shape[1]-1, norm=1, csphase=1,
+                lmax_calc=lmax, extend=0) -> gridglq
+    SHExpandGLQ(gridglq, w, zero, norm=1, csphase=1,
+                lmax_calc=gridglq.shape[0]-1) -> cilm
+    SHGLQ(lmax) -> (zero, w);  GLQGridCoord(lmax, extend=0) -> (lat, lon);
+    DHaj(n, extend=0) -> aj
+
+Array arguments may be numpy arrays (copied to the GPU and back, like f2py
+copies between C and Fortran order) or float64 torch tensors on a HIP device
+(used in place; the result is a tensor on the same device, produced
+asynchronously on the current stream).  Nonzero exit codes raise
+``SHToolsError`` with the reference's messages.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+__all__ = [
+    "SHToolsError", "MakeGridDH", "SHExpandDH", "MakeGridGLQ", "SHExpandGLQ", "SHGLQ", "GLQGridCoord", "DHaj",
+    "PlmBar", "PlmSchmidt", "PlmON", "PLegendreA",
+    "SHExpandDHC", "MakeGridDHC", "SHExpandGLQC", "MakeGridGLQC", "MakeGradientDH", "SHRotateRealCoef",
+]
+
+
+class SHToolsError(Exception):
+    pass
+
+
+def _status_message(status):
+    # pyshtools/backends/shtools.py:197-208
+    if status == 1:
+        return "Improper dimensions of input array."
+    if status == 2:
+        return "Improper bounds for input variable."
+    if status == 3:
+        return "Error allocating memory."
+    if status == 4:
+        return "File IO error."
+    return "Unhandled Fortran 95 error."
+
+
+def _check(status):
+    if status != 0:
+        raise SHToolsError(_status_message(status))
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _is_tensor(x):
+    try:
+        torch = _torch()
+    except ImportError:  # pragma: no cover
+        return False
+    return isinstance(x, torch.Tensor)
+
+
+def _device_of(*arrays):
+    torch = _torch()
+    for a in arrays:
+        if _is_tensor(a) and a.is_cuda:
+            return a.device
+    if not torch.cuda.is_available():
+        raise RuntimeError("shtools_amd needs a HIP device (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _to_dev(x, device):
+    torch = _torch()
+    if _is_tensor(x):
+        t = x.to(device=device, dtype=torch.float64)
+    else:
+        t = torch.as_tensor(np.ascontiguousarray(x, dtype=np.float64), device=device)
+    return t.contiguous()
+
+
+def _host_f64(x):
+    if _is_tensor(x):
+        x = x.detach().to("cpu").numpy()
+    return np.ascontiguousarray(x, dtype=np.float64).ravel()
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _stream(device):
+    return ctypes.c_void_p(_torch().cuda.current_stream(device).cuda_stream)
+
+
+def _finish(out, as_tensor):
+    if as_tensor:
+        return out
+    return out.cpu().numpy()
+
+
+# ------------------------------------------------------------ DH
+
+def MakeGridDH(cilm, lmax=None, norm=1, sampling=1, csphase=1, lmax_calc=None, extend=0):
+    """Evaluate real coefficients on a Driscoll-Healy grid (src/MakeGridDH.f95)."""
+    torch = _torch()
+    as_tensor = _is_tensor(cilm)
+    dev = _device_of(cilm)
+    c = _to_dev(cilm, dev)
+    if c.dim() != 3 or c.shape[0] < 2:
+        raise SHToolsError(_status_message(1))
+    if lmax is None:
+        lmax = c.shape[1] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    if sampling not in (1, 2) or extend not in (0, 1) or lmax < 0:
+        raise SHToolsError(_status_message(2))
+    g0 = 2 * (lmax + 1) + extend
+    g1 = sampling * 2 * (lmax + 1) + extend
+    grid = torch.empty((g0, g1), dtype=torch.float64, device=dev)
+    with torch.cuda.device(dev):
+        st = _lib.lib().shtg_makegrid_dh(c.data_ptr(), c.shape[1], c.shape[2], int(lmax), int(norm), int(sampling),
+                                         int(csphase), int(lmax_calc), int(extend), grid.data_ptr(), g0, g1,
+                                         _stream(dev))
+    _check(st)
+    return _finish(grid, as_tensor)
+
+
+def SHExpandDH(griddh, norm=1, sampling=1, csphase=1, lmax_calc=None):
+    """Expand a Driscoll-Healy grid into real coefficients (src/SHExpandDH.f95)."""
+    torch = _torch()
+    as_tensor = _is_tensor(griddh)
+    dev = _device_of(griddh)
+    g = _to_dev(griddh, dev)
+    if g.dim() != 2:
+        raise SHToolsError(_status_message(1))
+    n = g.shape[0]
+    if lmax_calc is None:
+        lmax_calc = n // 2 - 1
+    if lmax_calc < 0:
+        raise SHToolsError(_status_message(2))
+    cilm = torch.empty((2, lmax_calc + 1, lmax_calc + 1), dtype=torch.float64, device=dev)
+    with torch.cuda.device(dev):
+        st = _lib.lib().shtg_shexpand_dh(g.data_ptr(), g.shape[0], g.shape[1], int(norm), int(sampling),
+                                         int(csphase), int(lmax_calc), cilm.data_ptr(), lmax_calc + 1,
+                                         lmax_calc + 1, _stream(dev))
+    _check(st)
+    return _finish(cilm, as_tensor)
+
+
+# ------------------------------------------------------------ GLQ
+
+def MakeGridGLQ(cilm, zero, lmax=None, norm=1, csphase=1, lmax_calc=None, extend=0):
+    """Evaluate real coefficients on a Gauss-Legendre grid (src/MakeGridGLQ.f95)."""
+    torch = _torch()
+    as_tensor = _is_tensor(cilm)
+    dev = _device_of(cilm)
+    c = _to_dev(cilm, dev)
+    if c.dim() != 3 or c.shape[0] < 2:
+        raise SHToolsError(_status_message(1))
+    z = _host_f64(zero)
+    if lmax is None:
+        lmax = c.shape[1] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    if extend not in (0, 1) or lmax < 0:
+        raise SHToolsError(_status_message(2))
+    g0, g1 = lmax + 1, 2 * lmax + 1 + extend
+    grid = torch.empty((g0, g1), dtype=torch.float64, device=dev)
+    with torch.cuda.device(dev):
+        st = _lib.lib().shtg_makegrid_glq(c.data_ptr(), c.shape[1], c.shape[2], int(lmax), _dptr(z), z.size,
+                                          int(norm), int(csphase), int(lmax_calc), int(extend), grid.data_ptr(),
+                                          g0, g1, _stream(dev))
+    _check(st)
+    return _finish(grid, as_tensor)
+
+
+def SHExpandGLQ(gridglq, w, zero, norm=1, csphase=1, lmax_calc=None):
+    """Expand a Gauss-Legendre grid into real coefficients (src/SHExpandGLQ.f95)."""
+    torch = _torch()
+    as_tensor = _is_tensor(gridglq)
+    dev = _device_of(gridglq)
+    g = _to_dev(gridglq, dev)
+    if g.dim() != 2:
+        raise SHToolsError(_status_message(1))
+    wq = _host_f64(w)
+    z = _host_f64(zero)
+    lmax = g.shape[0] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    if lmax_calc < 0:
+        raise SHToolsError(_status_message(2))
+    cilm = torch.empty((2, lmax_calc + 1, lmax_calc + 1), dtype=torch.float64, device=dev)
+    with torch.cuda.device(dev):
+        st = _lib.lib().shtg_shexpand_glq(g.data_ptr(), g.shape[0], g.shape[1], _dptr(wq), wq.size, _dptr(z),
+                                          z.size, int(norm), int(csphase), int(lmax_calc), cilm.data_ptr(),
+                                          lmax_calc + 1, lmax_calc + 1, _stream(dev))
+    _check(st)
+    return _finish(cilm, as_tensor)
+
+
+# ------------------------------------------------------------ host helpers
+
+def SHGLQ(lmax):
+    """Gauss-Legendre nodes (descending) and weights for degree lmax (src/SHGLQ.f95)."""
+    zero = np.zeros(lmax + 1)
+    w = np.zeros(lmax + 1)
+    _check(_lib.lib().shtg_shglq(int(lmax), _dptr(zero), _dptr(w)))
+    return zero, w
+
+
+def GLQGridCoord(lmax, extend=0):
+    """Latitudes and longitudes (degrees) of the GLQ grid (src/GLQGridCoord.f95)."""
+    if extend not in (0, 1):
+        raise SHToolsError(_status_message(2))
+    lat = np.zeros(lmax + 1)
+    lon = np.zeros(2 * lmax + 1 + extend)
+    _check(_lib.lib().shtg_glqgridcoord(int(lmax), int(extend), _dptr(lat), _dptr(lon)))
+    return lat, lon
+
+
+def DHaj(n, extend=0):
+    """Driscoll-Healy quadrature weights (src/DHaj.f95)."""
+    if extend not in (0, 1):
+        raise SHToolsError(_status_message(2))
+    aj = np.zeros(n + extend)
+    _check(_lib.lib().shtg_dhaj(int(n), int(extend), _dptr(aj)))
+    return aj
+
+
+def _plm_packed(lmax, z, norm, csphase):
+    p = np.zeros((lmax + 1, lmax + 1))
+    _check(_lib.lib().shtg_plm(int(lmax), int(norm), int(csphase), float(z), _dptr(p)))
+    il, im = np.tril_indices(lmax + 1)
+    return p[il, im]   # packed index l(l+1)/2 + m (src/PlmIndex.f95:41)
+
+
+def PlmBar(lmax, z, csphase=1, cnorm=0):
+    """4pi-normalised P_lm(z), packed (src/PlmBar.f95); cnorm=1 is not supported."""
+    if cnorm != 0:
+        raise NotImplementedError("PlmBar(cnorm=1) (complex normalisation) is not provided by shtools_amd")
+    return _plm_packed(lmax, z, 1, csphase)
+
+
+def PlmSchmidt(lmax, z, csphase=1, cnorm=0):
+    """Schmidt semi-normalised P_lm(z), packed (src/PlmSchmidt.f95)."""
+    if cnorm != 0:
+        raise NotImplementedError("PlmSchmidt(cnorm=1) is not provided by shtools_amd")
+    return _plm_packed(lmax, z, 2, csphase)
+
+
+def PLegendreA(lmax, z, csphase=1):
+    """Unnormalised P_lm(z), packed (src/PLegendreA.f95)."""
+    return _plm_packed(lmax, z, 3, csphase)
+
+
+def PlmON(lmax, z, csphase=1, cnorm=0):
+    """Orthonormalised P_lm(z), packed (src/PlmON.f95)."""
+    if cnorm != 0:
+        raise NotImplementedError("PlmON(cnorm=1) is not provided by shtools_amd")
+    return _plm_packed(lmax, z, 4, csphase)
+
+
+# ------------------------------------------------------------ names injected but not on this path
+
+def _not_provided(name):
+    def f(*args, **kwargs):
+        raise NotImplementedError(f"{name} is not provided by the shtools_amd backend "
+                                  "(real scalar DH/GLQ transforms only); select the 'shtools' backend for it")
+    f.__name__ = name
+    return f
+
+
+SHExpandDHC = _not_provided("SHExpandDHC")
+MakeGridDHC = _not_provided("MakeGridDHC")
+SHExpandGLQC = _not_provided("SHExpandGLQC")
+MakeGridGLQC = _not_provided("MakeGridGLQC")
+MakeGradientDH = _not_provided("MakeGradientDH")
+SHRotateRealCoef = _not_provided("SHRotateRealCoef")

@@ -1,0 +1,579 @@
+// C ABI of libshtools_amd.so (declared in include/shtools_amd.h): argument
+// checks with the reference's exit codes, cached device tables and grid
+// geometry, and the kernel launch sequences of the plain transform path.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "../../include/shtools_amd.h"
+#include "fftplan.hpp"
+#include "kernels_plain.hpp"
+#include "tables.hpp"
+
+namespace shtg {
+
+enum { ST_OK = 0, ST_DIMS = 1, ST_BOUNDS = 2, ST_ALLOC = 3, ST_OTHER = 5 };
+
+// Largest LDS footprint a ring-FFT workgroup may use (gfx950: 160 KiB).
+constexpr size_t kMaxLds = 160 * 1024;
+
+static std::mutex g_mu;
+
+#define HIP_OK(x) (hipSuccess == (x))
+
+template <class T>
+static T *dev_upload(const std::vector<T> &v)
+{
+    T *p = nullptr;
+    if (v.empty()) return nullptr;
+    if (!HIP_OK(hipMalloc(&p, v.size() * sizeof(T)))) return nullptr;
+    if (!HIP_OK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice))) {
+        (void)hipFree(p);
+        return nullptr;
+    }
+    return p;
+}
+
+// ------------------------------------------------------------ caches
+
+struct DevRecurrence {
+    int L = -1;
+    double *alpha = nullptr, *s = nullptr, *post = nullptr;
+};
+struct DevSeeds {
+    int L = -1;
+    double *seed = nullptr;
+};
+
+// Grid geometry: ring-pair slots and the ring FFT plan.
+struct Geometry {
+    int P = 0;          // ring-pair slots
+    int nlat = 0;       // rows of the (non-extended) grid
+    int nlong = 0;      // FFT length
+    double *z = nullptr, *u = nullptr;
+    double2 *w = nullptr;  // analysis weights (north, south)
+    int2 *rows = nullptr;
+    FftDev fft{};
+    bool ok = false;
+};
+
+static std::map<std::tuple<int, int>, DevRecurrence> g_rec;            // (dev, norm)
+static std::map<std::tuple<int, int, int>, DevSeeds> g_seeds;          // (dev, norm, csphase)
+static std::map<std::tuple<int, int, int>, std::unique_ptr<Geometry>> g_dh;      // (dev, L, sampling)
+static std::map<std::tuple<int, int, uint64_t>, std::unique_ptr<Geometry>> g_glq;  // (dev, L, hash)
+
+static bool kernels_configured = false;
+
+static void configure_kernels()
+{
+    if (kernels_configured) return;
+    (void)hipFuncSetAttribute((const void *)k_synth_fft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+    (void)hipFuncSetAttribute((const void *)k_anal_fft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+    kernels_configured = true;
+}
+
+static const DevRecurrence *get_recurrence(int dev, int norm, int L)
+{
+    auto &r = g_rec[{dev, norm}];
+    if (r.L >= L) return &r;
+    int Lb = L < 64 ? 64 : L;
+    if (r.L >= 0) {
+        (void)hipFree(r.alpha);
+        (void)hipFree(r.s);
+        (void)hipFree(r.post);
+        r = DevRecurrence{};
+    }
+    RecurrenceTables t = build_recurrence(norm, Lb);
+    r.alpha = dev_upload(t.alpha);
+    r.s = dev_upload(t.s);
+    r.post = dev_upload(t.post);
+    if (!r.alpha || !r.s || !r.post) { r = DevRecurrence{}; return nullptr; }
+    r.L = Lb;
+    return &r;
+}
+
+static const DevSeeds *get_seeds(int dev, int norm, int csphase, int L)
+{
+    auto &r = g_seeds[{dev, norm, csphase}];
+    if (r.L >= L) return &r;
+    int Lb = L < 64 ? 64 : L;
+    if (r.L >= 0) { (void)hipFree(r.seed); r = DevSeeds{}; }
+    r.seed = dev_upload(build_seeds(norm, csphase, Lb));
+    if (!r.seed) { r = DevSeeds{}; return nullptr; }
+    r.L = Lb;
+    return &r;
+}
+
+static bool upload_fft(const FftPlanHost &h, FftDev &d)
+{
+    if ((size_t)h.m * sizeof(double2) > kMaxLds) return false;
+    std::memset(&d, 0, sizeof(d));
+    d.n = h.n;
+    d.m = h.m;
+    d.nst = h.nst;
+    d.blue = h.blue;
+    for (int i = 0; i < h.nst; ++i) { d.radix[i] = h.radix[i]; d.stride[i] = h.stride[i]; }
+    d.tw = reinterpret_cast<const double2 *>(dev_upload(h.tw));
+    d.perm = dev_upload(h.perm);
+    if (!d.tw || !d.perm) return false;
+    if (h.blue) {
+        d.chirp = reinterpret_cast<const double2 *>(dev_upload(h.chirp));
+        d.bhat = reinterpret_cast<const double2 *>(dev_upload(h.bhat));
+        if (!d.chirp || !d.bhat) return false;
+    }
+    return true;
+}
+
+static bool finish_geometry(Geometry &g, const std::vector<double> &z, const std::vector<double> &u,
+                            const std::vector<double> &w, const std::vector<int> &rows)
+{
+    g.z = dev_upload(z);
+    g.u = dev_upload(u);
+    g.w = reinterpret_cast<double2 *>(dev_upload(w));
+    g.rows = reinterpret_cast<int2 *>(dev_upload(rows));
+    if (!g.z || !g.u || !g.w || !g.rows) return false;
+    FftPlanHost fp = make_fft_plan(g.nlong);
+    if (!upload_fft(fp, g.fft)) return false;
+    g.ok = true;
+    return true;
+}
+
+// DH geometry (src/MakeGridDH.f95:504-510, src/SHExpandDH.f95:287-294,463-479):
+// slot s = 0 .. n/2 pairs row s (theta = pi s / n) with row n - s; slot 0 is
+// the north pole with the south pole (row n, present only with extend),
+// slot n/2 is the equator (unpaired).
+static Geometry *get_dh(int dev, int L, int sampling)
+{
+    auto &slot = g_dh[{dev, L, sampling}];
+    if (slot) return slot->ok ? slot.get() : nullptr;
+    slot.reset(new Geometry());
+    Geometry &g = *slot;
+    const int n = 2 * L + 2;
+    const double pi = std::acos(-1.0);
+    g.nlat = n;
+    g.nlong = sampling * n;
+    g.P = n / 2 + 1;
+    std::vector<double> z(g.P), u(g.P), w(2 * g.P);
+    std::vector<int> rows(2 * g.P);
+    std::vector<double> aj;
+    dh_weights(n, aj);
+    const double c = std::sqrt(2 * pi) * std::sqrt(4.0 * pi) / (double)g.nlong;
+    for (int s = 0; s < g.P; ++s) {
+        double zz;
+        if (s == n / 2) zz = 0.0;
+        else zz = std::cos(pi * (double)s / (double)n);
+        z[s] = zz;
+        u[s] = (s == n / 2) ? 1.0 : std::sqrt((1.0 - zz) * (1.0 + zz));
+        rows[2 * s] = s;
+        rows[2 * s + 1] = (s == n / 2) ? -1 : n - s;
+        w[2 * s] = c * aj[s];
+        w[2 * s + 1] = (s == 0 || s == n / 2) ? 0.0 : c * aj[n - s];
+    }
+    if (!finish_geometry(g, z, u, w, rows)) return nullptr;
+    return &g;
+}
+
+static uint64_t fnv1a(const void *p, size_t nbytes, uint64_t h = 1469598103934665603ULL)
+{
+    const unsigned char *b = (const unsigned char *)p;
+    for (size_t i = 0; i < nbytes; ++i) { h ^= b[i]; h *= 1099511628211ULL; }
+    return h;
+}
+
+// GLQ geometry (src/MakeGridGLQ.f95:467-480,546-553): slot s pairs row s
+// (z = zero(s)) with row nlat-1-s; for odd nlat the middle row is an exact
+// equator (z = 0).  Weights 2 pi w / nlong (src/SHExpandGLQ.f95:535-547).
+static Geometry *get_glq(int dev, int L, const double *zero, const double *wq)
+{
+    const int nlat = L + 1;
+    uint64_t h = fnv1a(zero, sizeof(double) * nlat);
+    h = wq ? fnv1a(wq, sizeof(double) * nlat, h) : fnv1a("nw", 2, h);
+    auto &slot = g_glq[{dev, L, h}];
+    if (slot) return slot->ok ? slot.get() : nullptr;
+    slot.reset(new Geometry());
+    Geometry &g = *slot;
+    const double pi = std::acos(-1.0);
+    g.nlat = nlat;
+    g.nlong = 2 * L + 1;
+    g.P = (nlat + 1) / 2;
+    std::vector<double> z(g.P), u(g.P), w(2 * g.P);
+    std::vector<int> rows(2 * g.P);
+    for (int s = 0; s < g.P; ++s) {
+        const bool eq = (nlat % 2 != 0) && (s == g.P - 1);
+        const double zz = eq ? 0.0 : zero[s];
+        z[s] = zz;
+        u[s] = eq ? 1.0 : std::sqrt((1.0 - zz) * (1.0 + zz));
+        rows[2 * s] = s;
+        rows[2 * s + 1] = eq ? -1 : nlat - 1 - s;
+        w[2 * s] = wq ? 2.0 * pi * wq[s] / (double)g.nlong : 0.0;
+        w[2 * s + 1] = (wq && !eq) ? 2.0 * pi * wq[nlat - 1 - s] / (double)g.nlong : 0.0;
+    }
+    if (!finish_geometry(g, z, u, w, rows)) return nullptr;
+    return &g;
+}
+
+static int current_device()
+{
+    int d = 0;
+    if (!HIP_OK(hipGetDevice(&d))) return -1;
+    return d;
+}
+
+static int grid_blocks(long long work)
+{
+    long long b = (work + kThreads - 1) / kThreads;
+    if (b < 1) b = 1;
+    if (b > 65535) b = 65535;
+    return (int)b;
+}
+
+// ------------------------------------------------------------ profiling
+
+// When enabled, every kernel launch is bracketed by HIP events recorded on
+// the launch stream; shtg_profile_collect() reduces them per kernel id.
+static bool g_prof = false;
+struct ProfRec {
+    int kid;
+    hipEvent_t a, b;
+};
+static std::vector<ProfRec> g_prof_recs;
+
+struct ProfScope {
+    int kid;
+    hipStream_t st;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(int k, hipStream_t s) : kid(k), st(s)
+    {
+        if (!g_prof) return;
+        if (!HIP_OK(hipEventCreate(&a)) || !HIP_OK(hipEventCreate(&b))) { a = b = nullptr; return; }
+        (void)hipEventRecord(a, st);
+    }
+    void end()
+    {
+        if (!g_prof || !a) return;
+        (void)hipEventRecord(b, st);
+        g_prof_recs.push_back({kid, a, b});
+        a = b = nullptr;
+    }
+    ~ProfScope() { end(); }
+};
+
+// ------------------------------------------------------------ launchers
+
+static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const DevSeeds &sd, const double *cilm,
+                            int d1, int d2, int Lc, int nrows_out, int ext_col, double *grid, long long ld,
+                            hipStream_t st)
+{
+    const size_t npack = (size_t)tri_size(Lc);
+    const size_t nf = (size_t)(Lc + 1) * g.P;
+    void *ws = nullptr;
+    const size_t bytes = npack * sizeof(double2) + nf * sizeof(double4);
+    if (!HIP_OK(hipMallocAsync(&ws, bytes, st))) return ST_ALLOC;
+    double2 *cpack = (double2 *)ws;
+    double4 *fbuf = (double4 *)((char *)ws + npack * sizeof(double2));
+
+    {
+        ProfScope ps(SHTG_K_PACK, st);
+        hipLaunchKernelGGL(k_pack_synth, dim3(grid_blocks((long long)npack)), dim3(kThreads), 0, st, cilm, d1, d2,
+                           rt.s, rt.L, cpack, Lc);
+    }
+    {
+        ProfScope ps(SHTG_K_SYNTH_LEG, st);
+        const int K = 2;
+        const int nrb = (g.P + K * kThreads - 1) / (K * kThreads);
+        hipLaunchKernelGGL(k_synth_leg<K>, dim3((Lc + 1) * nrb), dim3(kThreads), 0, st, rt.alpha, rt.L, sd.seed,
+                           g.z, g.u, cpack, Lc, g.P, nrb, fbuf);
+    }
+    {
+        ProfScope ps(SHTG_K_SYNTH_FFT, st);
+        const size_t lds = (size_t)g.fft.m * sizeof(double2);
+        hipLaunchKernelGGL(k_synth_fft, dim3(g.P), dim3(kThreads), lds, st, g.fft, fbuf, g.P, Lc, g.rows,
+                           nrows_out, grid, ld, ext_col);
+    }
+    if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
+    (void)hipFreeAsync(ws, st);
+    return ST_OK;
+}
+
+static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const DevSeeds &sd, const double *grid,
+                           long long ld, int Lc, double *cilm, int d1, int d2, hipStream_t st)
+{
+    const size_t npack = (size_t)tri_size(Lc);
+    const size_t nf = (size_t)(Lc + 1) * g.P;
+    void *ws = nullptr;
+    const size_t bytes = npack * sizeof(double2) + nf * sizeof(double4);
+    if (!HIP_OK(hipMallocAsync(&ws, bytes, st))) return ST_ALLOC;
+    double2 *cbuf = (double2 *)ws;
+    double4 *gbuf = (double4 *)((char *)ws + npack * sizeof(double2));
+
+    {
+        ProfScope ps(SHTG_K_ANAL_FFT, st);
+        const size_t lds = (size_t)g.fft.m * sizeof(double2);
+        hipLaunchKernelGGL(k_anal_fft, dim3(g.P), dim3(kThreads), lds, st, g.fft, grid, ld, g.rows, g.nlat, g.w,
+                           g.P, Lc, gbuf);
+    }
+    const int need = (g.P + kThreads - 1) / kThreads;
+    const dim3 gl(Lc + 1), bl(kThreads);
+    ProfScope ps_leg(SHTG_K_ANAL_LEG, st);
+    if (need <= 1) hipLaunchKernelGGL(k_anal_leg<1>, gl, bl, 0, st, rt.alpha, rt.L, sd.seed, g.z, g.u, gbuf, Lc, g.P, cbuf);
+    else if (need <= 2) hipLaunchKernelGGL(k_anal_leg<2>, gl, bl, 0, st, rt.alpha, rt.L, sd.seed, g.z, g.u, gbuf, Lc, g.P, cbuf);
+    else if (need <= 4) hipLaunchKernelGGL(k_anal_leg<4>, gl, bl, 0, st, rt.alpha, rt.L, sd.seed, g.z, g.u, gbuf, Lc, g.P, cbuf);
+    else if (need <= 8) hipLaunchKernelGGL(k_anal_leg<8>, gl, bl, 0, st, rt.alpha, rt.L, sd.seed, g.z, g.u, gbuf, Lc, g.P, cbuf);
+    else if (need <= 12) hipLaunchKernelGGL(k_anal_leg<12>, gl, bl, 0, st, rt.alpha, rt.L, sd.seed, g.z, g.u, gbuf, Lc, g.P, cbuf);
+    else { (void)hipFreeAsync(ws, st); return ST_OTHER; }
+    ps_leg.end();
+    const long long total = 2LL * d1 * d2;
+    ProfScope ps_un(SHTG_K_UNPACK, st);
+    hipLaunchKernelGGL(k_unpack_anal, dim3(grid_blocks(total)), dim3(kThreads), 0, st, cbuf, rt.post, rt.L, Lc, cilm,
+                       d1, d2);
+    if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
+    (void)hipFreeAsync(ws, st);
+    return ST_OK;
+}
+
+}  // namespace shtg
+
+using namespace shtg;
+
+extern "C" {
+
+int shtg_makegrid_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax, int norm, int sampling, int csphase,
+                     int lmax_calc, int extend, double *grid, int grid_d0, int grid_d1, void *stream)
+{
+    // argument checks in the order of src/MakeGridDH.f95:107-251
+    if (lmax < 0) return ST_BOUNDS;
+    const int n = 2 * lmax + 2;
+    int nlong;
+    if (sampling == 1) nlong = n;
+    else if (sampling == 2) nlong = 2 * n;
+    else return ST_BOUNDS;
+    if (extend != 0 && extend != 1) return ST_BOUNDS;
+    const int nlat_out = n + extend, nlong_out = nlong + extend;
+    if (grid_d0 < nlat_out || grid_d1 < nlong_out) return ST_DIMS;
+    if (norm < 1 || norm > 4) return ST_BOUNDS;
+    if (csphase != 1 && csphase != -1) return ST_BOUNDS;
+    if (lmax_calc > lmax) return ST_BOUNDS;
+    if (lmax_calc < 0 || cilm_d1 < 1 || cilm_d2 < 1) return ST_BOUNDS;
+    int Lc = lmax;
+    if (cilm_d2 - 1 < Lc) Lc = cilm_d2 - 1;
+    if (cilm_d1 - 1 < Lc) Lc = cilm_d1 - 1;
+    if (lmax_calc < Lc) Lc = lmax_calc;
+    if (!cilm || !grid) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    const int dev = current_device();
+    if (dev < 0) return ST_OTHER;
+    configure_kernels();
+    const Geometry *g = get_dh(dev, lmax, sampling);
+    const DevRecurrence *rt = get_recurrence(dev, norm, Lc);
+    const DevSeeds *sd = get_seeds(dev, norm, csphase, Lc);
+    if (!g || !rt || !sd) return ST_ALLOC;
+    return launch_synthesis(*g, *rt, *sd, cilm, cilm_d1, cilm_d2, Lc, nlat_out, extend, grid, grid_d1,
+                            (hipStream_t)stream);
+}
+
+int shtg_shexpand_dh(const double *grid, int grid_d0, int grid_d1, int norm, int sampling, int csphase,
+                     int lmax_calc, double *cilm, int cilm_d1, int cilm_d2, void *stream)
+{
+    // argument checks in the order of src/SHExpandDH.f95:118-270
+    const int n = grid_d0;
+    const int lmax = n / 2 - 1;
+    if (lmax_calc > lmax) return ST_BOUNDS;
+    if (lmax_calc < 0) return ST_BOUNDS;
+    const int Lc = lmax_calc < lmax ? lmax_calc : lmax;
+    if (sampling != 1 && sampling != 2) return ST_BOUNDS;
+    if (n % 2 != 0 || n < 2) return ST_BOUNDS;
+    if (cilm_d1 < Lc + 1 || cilm_d2 < Lc + 1) return ST_DIMS;
+    const int nlong = sampling * n;
+    if (grid_d1 < nlong) return ST_DIMS;
+    if (csphase != 1 && csphase != -1) return ST_BOUNDS;
+    if (norm < 1 || norm > 4) return ST_BOUNDS;
+    if (!cilm || !grid) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    const int dev = current_device();
+    if (dev < 0) return ST_OTHER;
+    configure_kernels();
+    const Geometry *g = get_dh(dev, lmax, sampling);
+    const DevRecurrence *rt = get_recurrence(dev, norm, Lc);
+    const DevSeeds *sd = get_seeds(dev, norm, csphase, Lc);
+    if (!g || !rt || !sd) return ST_ALLOC;
+    return launch_analysis(*g, *rt, *sd, grid, grid_d1, Lc, cilm, cilm_d1, cilm_d2, (hipStream_t)stream);
+}
+
+int shtg_makegrid_glq(const double *cilm, int cilm_d1, int cilm_d2, int lmax, const double *zero, int nzero,
+                      int norm, int csphase, int lmax_calc, int extend, double *grid, int grid_d0, int grid_d1,
+                      void *stream)
+{
+    // argument checks in the order of src/MakeGridGLQ.f95:108-270
+    if (lmax < 0) return ST_BOUNDS;
+    const int nlong = 2 * lmax + 1, nlat = lmax + 1;
+    if (extend != 0 && extend != 1) return ST_BOUNDS;
+    const int nlong_out = nlong + extend;
+    if (grid_d1 < nlong_out || grid_d0 < nlat) return ST_DIMS;
+    if (!zero) return ST_OTHER;
+    if (nzero < lmax + 1) return ST_DIMS;
+    if (norm < 1 || norm > 4) return ST_BOUNDS;
+    if (csphase != 1 && csphase != -1) return ST_BOUNDS;
+    if (lmax_calc > lmax) return ST_BOUNDS;
+    if (lmax_calc < 0 || cilm_d1 < 1 || cilm_d2 < 1) return ST_BOUNDS;
+    int Lc = lmax;
+    if (cilm_d2 - 1 < Lc) Lc = cilm_d2 - 1;
+    if (cilm_d1 - 1 < Lc) Lc = cilm_d1 - 1;
+    if (lmax_calc < Lc) Lc = lmax_calc;
+    if (!cilm || !grid) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    const int dev = current_device();
+    if (dev < 0) return ST_OTHER;
+    configure_kernels();
+    const Geometry *g = get_glq(dev, lmax, zero, nullptr);
+    const DevRecurrence *rt = get_recurrence(dev, norm, Lc);
+    const DevSeeds *sd = get_seeds(dev, norm, csphase, Lc);
+    if (!g || !rt || !sd) return ST_ALLOC;
+    return launch_synthesis(*g, *rt, *sd, cilm, cilm_d1, cilm_d2, Lc, nlat, extend, grid, grid_d1,
+                            (hipStream_t)stream);
+}
+
+int shtg_shexpand_glq(const double *grid, int grid_d0, int grid_d1, const double *w, int nw, const double *zero,
+                      int nzero, int norm, int csphase, int lmax_calc, double *cilm, int cilm_d1, int cilm_d2,
+                      void *stream)
+{
+    // argument checks in the order of src/SHExpandGLQ.f95:100-250
+    const int lmax = grid_d0 - 1;
+    if (lmax < 0) return ST_DIMS;
+    if (lmax_calc > lmax) return ST_BOUNDS;
+    if (lmax_calc < 0) return ST_BOUNDS;
+    const int Lc = lmax_calc < lmax ? lmax_calc : lmax;
+    if (cilm_d1 < Lc + 1 || cilm_d2 < Lc + 1) return ST_DIMS;
+    if (grid_d1 < 2 * lmax + 1 || grid_d0 < lmax + 1) return ST_DIMS;
+    if (!w || nw < lmax + 1) return ST_DIMS;
+    if (!zero) return ST_OTHER;
+    if (nzero < lmax + 1) return ST_DIMS;
+    if (norm < 1 || norm > 4) return ST_BOUNDS;
+    if (csphase != 1 && csphase != -1) return ST_BOUNDS;
+    if (!cilm || !grid) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    const int dev = current_device();
+    if (dev < 0) return ST_OTHER;
+    configure_kernels();
+    const Geometry *g = get_glq(dev, lmax, zero, w);
+    const DevRecurrence *rt = get_recurrence(dev, norm, Lc);
+    const DevSeeds *sd = get_seeds(dev, norm, csphase, Lc);
+    if (!g || !rt || !sd) return ST_ALLOC;
+    return launch_analysis(*g, *rt, *sd, grid, grid_d1, Lc, cilm, cilm_d1, cilm_d2, (hipStream_t)stream);
+}
+
+int shtg_shglq(int lmax, double *zero, double *w)
+{
+    if (lmax < 0) return ST_BOUNDS;
+    if (!zero || !w) return ST_DIMS;
+    return glq_nodes(lmax + 1, zero, w);
+}
+
+int shtg_dhaj(int n, int extend, double *aj)
+{
+    if (extend != 0 && extend != 1) return ST_BOUNDS;
+    if (n % 2 != 0 || n < 0) return ST_BOUNDS;
+    if (!aj) return ST_DIMS;
+    std::vector<double> v;
+    dh_weights(n, v);
+    for (int j = 0; j < n; ++j) aj[j] = v[j];
+    if (extend) aj[n] = 0.0;
+    return ST_OK;
+}
+
+int shtg_glqgridcoord(int lmax, int extend, double *lat, double *lon)
+{
+    if (lmax < 0) return ST_BOUNDS;
+    if (extend != 0 && extend != 1) return ST_BOUNDS;
+    if (!lat || !lon) return ST_DIMS;
+    const int nlat = lmax + 1, nlong = 2 * lmax + 1;
+    std::vector<double> zero(nlat), w(nlat);
+    int st = glq_nodes(nlat, zero.data(), w.data());
+    if (st) return st;
+    const double pi = std::acos(-1.0);
+    for (int i = 0; i < nlong + extend; ++i) lon[i] = 360.0 * (double)i / (double)nlong;
+    for (int i = 0; i < nlat; ++i) lat[i] = std::asin(zero[i]) * 180.0 / pi;
+    return ST_OK;
+}
+
+// P_lm(z) for all l, m <= lmax into p[l*(lmax+1)+m], evaluated on the host
+// with exactly the tables, seeds, rescaled recursion and extended-range
+// powers that the GPU kernels use (the reference's PlmBar/PlmSchmidt/
+// PLegendreA/PlmON family for the transforms' normalisations).
+int shtg_plm(int lmax, int norm, int csphase, double z, double *p)
+{
+    if (lmax < 0 || norm < 1 || norm > 4 || (csphase != 1 && csphase != -1)) return ST_BOUNDS;
+    if (!p) return ST_DIMS;
+    RecurrenceTables t = build_recurrence(norm, lmax);
+    std::vector<double> seed = build_seeds(norm, csphase, lmax);
+    const double u = std::sqrt((1.0 - z) * (1.0 + z));
+    for (int i = 0; i < (lmax + 1) * (lmax + 1); ++i) p[i] = 0.0;
+    for (int m = 0; m <= lmax; ++m) {
+        double r = 1.0;
+        if (m > 0) {
+            if (u == 0.0) r = 0.0;
+            else {
+                int e;
+                double f = std::frexp(u, &e), rr = 1.0, b = f;
+                int re = 0, be = 0, k = m;
+                while (k) {
+                    if (k & 1) { int q; rr = std::frexp(rr * b, &q); re += be + q; }
+                    k >>= 1;
+                    if (k) { int q; b = std::frexp(b * b, &q); be = 2 * be + q; }
+                }
+                r = std::ldexp(rr * 1.0e280, re + e * m);
+            }
+        }
+        const double zz = (r == 0.0) ? 0.0 : z;
+        const int64_t o = tri_off(m, lmax);
+        double q1 = 0.0, q2 = -seed[m];
+        for (int l = m; l <= lmax; ++l) {
+            const double q = (t.alpha[o + l - m] * zz) * q1 - q2;
+            q2 = q1;
+            q1 = q;
+            p[(size_t)l * (lmax + 1) + m] = (r == 0.0) ? 0.0 : t.s[o + l - m] * q * r;
+        }
+    }
+    return ST_OK;
+}
+
+int shtg_device_count(void)
+{
+    int n = 0;
+    if (!HIP_OK(hipGetDeviceCount(&n))) return 0;
+    return n;
+}
+
+const char *shtg_version(void) { return "shtools_amd 0.1 (gfx950)"; }
+
+int shtg_profile_enable(int on)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_prof = on != 0;
+    return ST_OK;
+}
+
+int shtg_profile_collect(double *ms, long long *count, int nk)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (int k = 0; k < nk; ++k) { ms[k] = 0.0; count[k] = 0; }
+    int st = ST_OK;
+    for (auto &r : g_prof_recs) {
+        float t = 0.0f;
+        if (!HIP_OK(hipEventSynchronize(r.b)) || !HIP_OK(hipEventElapsedTime(&t, r.a, r.b))) st = ST_OTHER;
+        if (r.kid >= 0 && r.kid < nk) { ms[r.kid] += t; count[r.kid] += 1; }
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    g_prof_recs.clear();
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,209 @@
+// Host-side tables for the MI355X spherical-harmonic transforms.
+//
+// Recursion constants follow the reference's table code (identical in all
+// four transforms, e.g. src/MakeGridDH.f95:291-381), re-expressed for the
+// GPU recursion
+//     q_l = (alpha_l z) q_{l-1} - q_{l-2},   P_lm = s_lm q_l,
+// which divides the reference's three-term recursion
+//     P_l = z ff1(l,m) P_{l-1} - ff2(l,m) P_{l-2}
+// by s_l = ff2(l,m) s_{l-2} (s_m = s_{m+1} = 1), so that alpha_l =
+// ff1(l,m) s_{l-1} / s_l.  Each step then costs one multiply and one FMA.
+// The s factors are folded into the coefficients before synthesis and into
+// the normalisation epilogue after analysis.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+namespace shtg {
+
+// Offset of order m in a triangular [m][l-m] table with maximum degree L.
+inline int64_t tri_off(int m, int L) { return (int64_t)m * (L + 1) - (int64_t)m * (m - 1) / 2; }
+inline int64_t tri_size(int L) { return (int64_t)(L + 1) * (L + 2) / 2; }
+
+struct RecurrenceTables {
+    int L = -1;
+    std::vector<double> alpha;  // tri: alpha_{m+j}; entry j = 0 is 0
+    std::vector<double> s;      // tri: s_{m+j}
+    std::vector<double> post;   // tri: s_{m+j} * (analysis epilogue factor)
+};
+
+// ff1/ff2 of the reference for one order m, l = m..L (index l).
+// src/MakeGridDH.f95:321-381 (norm 1,4 / 2 / 3 branches).
+inline void ref_ff_column(int norm, int m, int L, const std::vector<double> &sqr,
+                          std::vector<double> &ff1, std::vector<double> &ff2)
+{
+    ff1.assign(L + 2, 0.0);
+    ff2.assign(L + 2, 0.0);
+    for (int l = m + 1; l <= L; ++l) {
+        double f1 = 0.0, f2 = 0.0;
+        if (norm == 1 || norm == 4) {
+            if (m == 0) {
+                if (l == 1) { f1 = sqr[3]; f2 = 0.0; }
+                else {
+                    f1 = sqr[2 * l - 1] * sqr[2 * l + 1] / (double)l;
+                    f2 = (double)(l - 1) * sqr[2 * l + 1] / sqr[2 * l - 3] / (double)l;
+                }
+            } else if (m <= l - 2) {
+                f1 = sqr[2 * l + 1] * sqr[2 * l - 1] / sqr[l + m] / sqr[l - m];
+                f2 = sqr[2 * l + 1] * sqr[l - m - 1] * sqr[l + m - 1] / sqr[2 * l - 3] / sqr[l + m] / sqr[l - m];
+            } else {  // m = l - 1
+                f1 = sqr[2 * l + 1] * sqr[2 * l - 1] / sqr[l + m] / sqr[l - m];
+                f2 = 0.0;
+            }
+        } else if (norm == 2) {
+            if (m == 0) {
+                if (l == 1) { f1 = 1.0; f2 = 0.0; }
+                else { f1 = (double)(2 * l - 1) / (double)l; f2 = (double)(l - 1) / (double)l; }
+            } else if (m <= l - 2) {
+                f1 = (double)(2 * l - 1) / sqr[l + m] / sqr[l - m];
+                f2 = sqr[l - m - 1] * sqr[l + m - 1] / sqr[l + m] / sqr[l - m];
+            } else {
+                f1 = (double)(2 * l - 1) / sqr[l + m] / sqr[l - m];
+                f2 = 0.0;
+            }
+        } else {  // norm 3
+            if (m == 0) { f1 = (double)(2 * l - 1) / (double)l; f2 = (double)(l - 1) / (double)l; }
+            else { f1 = (double)(2 * l - 1) / (double)(l - m); f2 = (double)(l + m - 1) / (double)(l - m); }
+        }
+        ff1[l] = f1;
+        ff2[l] = f2;
+    }
+}
+
+// Analysis epilogue factor per (l, m): src/SHExpandDH.f95:674-702.
+inline void epilogue_factors(int norm, int L, std::vector<double> &epi /* [l*(L+1)+m] */)
+{
+    const double pi = std::acos(-1.0);
+    epi.assign((size_t)(L + 1) * (L + 1), 0.0);
+    for (int l = 0; l <= L; ++l) {
+        if (norm == 1) {
+            for (int m = 0; m <= l; ++m) epi[(size_t)l * (L + 1) + m] = 1.0 / (4 * pi);
+        } else if (norm == 2) {
+            for (int m = 0; m <= l; ++m) epi[(size_t)l * (L + 1) + m] = (double)(2 * l + 1) / (4 * pi);
+        } else if (norm == 3) {
+            double prod = 4 * pi / (double)(2 * l + 1);
+            epi[(size_t)l * (L + 1)] = 1.0 / prod;
+            prod = prod / 2.0;
+            for (int m = 1; m <= l - 1; ++m) {
+                prod = prod * (l + m) * (l - m + 1);
+                epi[(size_t)l * (L + 1) + m] = 1.0 / prod;
+            }
+            if (l != 0) epi[(size_t)l * (L + 1) + l] = 1.0 / (prod * 2 * l);
+        } else {
+            for (int m = 0; m <= l; ++m) epi[(size_t)l * (L + 1) + m] = 1.0;
+        }
+    }
+}
+
+inline RecurrenceTables build_recurrence(int norm, int L)
+{
+    RecurrenceTables t;
+    t.L = L;
+    size_t n = (size_t)tri_size(L);
+    t.alpha.assign(n, 0.0);
+    t.s.assign(n, 0.0);
+    t.post.assign(n, 0.0);
+    std::vector<double> sqr(2 * L + 4, 0.0);
+    for (int k = 1; k < (int)sqr.size(); ++k) sqr[k] = std::sqrt((double)k);
+    std::vector<double> epi;
+    epilogue_factors(norm, L, epi);
+    std::vector<double> ff1, ff2;
+    for (int m = 0; m <= L; ++m) {
+        ref_ff_column(norm, m, L, sqr, ff1, ff2);
+        int64_t o = tri_off(m, L);
+        double s_lm2 = 1.0, s_lm1 = 1.0;  // s_{l-2}, s_{l-1}
+        for (int l = m; l <= L; ++l) {
+            double s, a;
+            if (l == m) { s = 1.0; a = 0.0; }
+            else if (l == m + 1) { s = 1.0; a = ff1[l]; }
+            else { s = ff2[l] * s_lm2; a = ff1[l] * s_lm1 / s; }
+            t.alpha[o + (l - m)] = a;
+            t.s[o + (l - m)] = s;
+            t.post[o + (l - m)] = s * epi[(size_t)l * (L + 1) + m];
+            s_lm2 = s_lm1;
+            s_lm1 = s;
+        }
+    }
+    return t;
+}
+
+// Seeds q_m of each order: P_00 for m = 0 (MakeGridDH.f95:516-519) and the
+// 1e-280-scaled sectoral chain for m >= 1 (MakeGridDH.f95:542-564).
+inline std::vector<double> build_seeds(int norm, int csphase, int L)
+{
+    const double pi = std::acos(-1.0), scalef = 1.0e-280;
+    std::vector<double> seed(L + 1, 0.0);
+    std::vector<double> sqr(2 * L + 4, 0.0);
+    for (int k = 1; k < (int)sqr.size(); ++k) sqr[k] = std::sqrt((double)k);
+    seed[0] = (norm == 4) ? 1.0 / std::sqrt(4.0 * pi) : 1.0;
+    double pmm;
+    switch (norm) {
+    case 1: case 2: pmm = sqr[2] * scalef; break;
+    case 3: pmm = scalef; break;
+    default: pmm = sqr[2] * scalef / std::sqrt(4.0 * pi); break;
+    }
+    for (int m = 1; m <= L; ++m) {
+        if (norm == 1 || norm == 4) {
+            pmm = csphase * pmm * sqr[2 * m + 1] / sqr[2 * m];
+            seed[m] = pmm;
+        } else if (norm == 2) {
+            pmm = csphase * pmm * sqr[2 * m + 1] / sqr[2 * m];
+            seed[m] = pmm / sqr[2 * m + 1];
+        } else {
+            pmm = csphase * pmm * (double)(2 * m - 1);
+            seed[m] = pmm;
+        }
+    }
+    return seed;
+}
+
+// src/DHaj.f95:91-101
+inline void dh_weights(int n, std::vector<double> &aj)
+{
+    const double pi = std::acos(-1.0);
+    aj.assign(n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        double sum1 = 0.0;
+        for (int l = 0; l <= n / 2 - 1; ++l)
+            sum1 = sum1 + std::sin((double)(2 * l + 1) * pi * (double)j / (double)n) / (double)(2 * l + 1);
+        aj[j] = sum1 * std::sin(pi * (double)j / (double)n) * std::sqrt(8.0) / (double)n;
+    }
+}
+
+// Gauss-Legendre nodes and weights on [-1, 1]: src/PreGLQ.f95:93-149.
+// Returns 0, or 5 when Newton's method does not converge.
+inline int glq_nodes(int n, double *zero, double *w)
+{
+    const int itermax = 1000;
+    const double eps = 1.0e-15, pi = std::acos(-1.0);
+    for (int i = 0; i < n; ++i) { zero[i] = 0.0; w[i] = 0.0; }
+    int half = (n + 1) / 2;
+    for (int i = 1; i <= half; ++i) {
+        int iter = 0;
+        double z = std::cos(pi * (i - 0.25) / (n + 0.5)), z1, p1, p2, p3, pp;
+        for (;;) {
+            ++iter;
+            p1 = 1.0;
+            p2 = 0.0;
+            for (int j = 1; j <= n; ++j) {
+                p3 = p2;
+                p2 = p1;
+                p1 = ((double)(2 * j - 1) * z * p2 - (double)(j - 1) * p3) / (double)j;
+            }
+            pp = (double)n * (z * p1 - p2) / (z * z - 1.0);
+            z1 = z;
+            z = z1 - p1 / pp;
+            if (std::fabs(z - z1) <= eps) break;
+            if (iter > itermax) return 5;
+        }
+        zero[i - 1] = z;
+        zero[n - i] = -z;
+        w[i - 1] = 2.0 / ((1.0 - z * z) * pp * pp);
+        w[n - i] = w[i - 1];
+    }
+    return 0;
+}
+
+}  // namespace shtg

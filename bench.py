@@ -30,6 +30,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nfields", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--loop", action="store_true", help="time the per-field loop of the plain kernels instead")
+    ap.add_argument("--no-loop", action="store_true", help="skip the per-field loop comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     return ap.parse_args()
 
@@ -54,12 +56,35 @@ def main():
     Ls = [L for L, _ in spec]
     print(f"[bench] {len(Ls)} fields, lmax {min(Ls)}..{max(Ls)}", file=sys.stderr, flush=True)
 
-    def step():
-        outs = []
+    from shtools_amd.batch import DHBatch
+    batch = DHBatch(Ls, sampling=2, norm=1, csphase=1, device=dev)
+    grids = [torch.empty(batch.grid_shape(i), dtype=torch.float64, device=dev) for i in range(len(Ls))]
+    outs = [torch.empty(batch.cilm_shape(i), dtype=torch.float64, device=dev) for i in range(len(Ls))]
+
+    def loop_step():
+        res = []
         for c in cilm:
             g = B.MakeGridDH(c, sampling=2)
-            outs.append(B.SHExpandDH(g, sampling=2))
+            res.append(B.SHExpandDH(g, sampling=2))
+        return res
+
+    def batch_step():
+        batch.makegrid(cilm, out=grids)
+        batch.expand(grids, out=outs)
         return outs
+
+    step = loop_step if args.loop else batch_step
+
+    # the per-field loop of the plain kernels, for the batched-vs-loop ratio
+    loop_rate = None
+    if not args.loop and not args.no_loop:
+        loop_step()
+        torch.cuda.synchronize()
+        l0 = time.perf_counter()
+        loop_outs = loop_step()
+        torch.cuda.synchronize()
+        loop_rate = len(cilm) / (time.perf_counter() - l0)
+        print(f"[bench] per-field loop: {loop_rate:.1f} fields/s", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
         outs = step()
@@ -138,13 +163,16 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{len(Ls)} DH2 fields, lmax~U{{63..1023}} seed 3, slope~U[-4,-2], norm 4pi, "
-                               "synthesis+analysis per step, per-field loop of the plain kernels",
+                               "synthesis+analysis per step, "
+                               + ("per-field loop of the plain kernels" if args.loop else "batched launches"),
                    "fields": len(Ls), "sampling": 2, "norm": 1, "csphase": 1},
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": (achieved / PEAK_FP64_TFLOPS) if achieved else None,
                      "traffic": None},
         "model_tflops": total_flops * args.steps / (t1 - t0) / 1e12,
         "roundtrip_err": rt,
+        "loop_fields_per_s": loop_rate,
+        "batched_vs_loop": (value / loop_rate) if loop_rate else None,
         "kernels_ms": {k: v[0] for k, v in kern.items()},
         "cpu_baseline": cpu,
     }

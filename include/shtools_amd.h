@@ -62,6 +62,25 @@ int shtg_shexpand_glq(const double *grid, int grid_d0, int grid_d1, const double
                       const double *zero, int nzero, int norm, int csphase, int lmax_calc,
                       double *cilm, int cilm_d1, int cilm_d2, void *stream);
 
+/* Batched DH transforms of many fields of uneven degree (no counterpart in
+ * the reference, which transforms one field per call; semantics per field
+ * are those of MakeGridDH/SHExpandDH with lmax_calc = lmax, extend = 0).
+ * A plan fixes the degrees, sampling, norm and csphase; each call enqueues a
+ * handful of launches covering every field, with work items ordered longest
+ * first.  cilm[i] points to a device [2][lmax[i]+1][lmax[i]+1] array and
+ * grid[i] to a device [2lmax[i]+2][sampling(2lmax[i]+2)] array; the arrays of
+ * pointers are host arrays (uploaded when they change).  Calls on one plan
+ * must not overlap in time on different streams (they share its
+ * workspace). */
+typedef struct shtg_batch_plan shtg_batch_plan;
+int shtg_batch_create_dh(int nfields, const int *lmax, int sampling, int norm, int csphase,
+                         shtg_batch_plan **plan);
+int shtg_batch_makegrid_dh(shtg_batch_plan *plan, const double *const *cilm, double *const *grid,
+                           void *stream);
+int shtg_batch_shexpand_dh(shtg_batch_plan *plan, const double *const *grid, double *const *cilm,
+                           void *stream);
+int shtg_batch_destroy(shtg_batch_plan *plan);
+
 /* Host helpers. */
 /* SHGLQ nodes and weights (include/shtools.h:142-148; src/SHGLQ.f95:187-192 ->
  * src/PreGLQ.f95:93-149):

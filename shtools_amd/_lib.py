@@ -25,6 +25,10 @@ SIGNATURES = {
     "shtg_dhaj": [_c_int, _c_int, _dp],
     "shtg_glqgridcoord": [_c_int, _c_int, _dp, _dp],
     "shtg_plm": [_c_int, _c_int, _c_int, ctypes.c_double, _dp],
+    "shtg_batch_create_dh": [_c_int, ctypes.POINTER(_c_int), _c_int, _c_int, _c_int, ctypes.POINTER(_vp)],
+    "shtg_batch_makegrid_dh": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp],
+    "shtg_batch_shexpand_dh": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp],
+    "shtg_batch_destroy": [_vp],
     "shtg_profile_enable": [_c_int],
     "shtg_profile_collect": [_dp, ctypes.POINTER(ctypes.c_longlong), _c_int],
     "shtg_device_count": [],

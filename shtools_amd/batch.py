@@ -1,0 +1,89 @@
+"""Batched Driscoll-Healy transforms of many fields of uneven degree.
+
+One plan per set of degrees; every call covers all fields with a handful of
+launches (pack, Legendre, ring FFTs grouped by size, unpack) whose work
+items are ordered longest first.  Per field the results are those of
+``backend.MakeGridDH(c, sampling=s, norm=n, csphase=p)`` and
+``backend.SHExpandDH(g, sampling=s, norm=n, csphase=p)``.
+"""
+import ctypes
+
+from . import _lib
+from .backend import SHToolsError, _status_message
+
+
+def _ptr_array(tensors):
+    arr = (ctypes.c_void_p * len(tensors))()
+    for i, t in enumerate(tensors):
+        arr[i] = t.data_ptr()
+    return arr
+
+
+class DHBatch:
+    def __init__(self, lmax, sampling=2, norm=1, csphase=1, device=None):
+        import torch
+        self.torch = torch
+        self.lmax = [int(L) for L in lmax]
+        self.sampling, self.norm, self.csphase = int(sampling), int(norm), int(csphase)
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        arr = (ctypes.c_int * len(self.lmax))(*self.lmax)
+        plan = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            st = _lib.lib().shtg_batch_create_dh(len(self.lmax), arr, self.sampling, self.norm, self.csphase,
+                                                 ctypes.byref(plan))
+        if st != 0:
+            raise SHToolsError(_status_message(st))
+        self._plan = plan
+
+    def __del__(self):
+        plan = getattr(self, "_plan", None)
+        if plan:
+            _lib.lib().shtg_batch_destroy(plan)
+            self._plan = None
+
+    def grid_shape(self, i):
+        n = 2 * self.lmax[i] + 2
+        return (n, self.sampling * n)
+
+    def cilm_shape(self, i):
+        L = self.lmax[i]
+        return (2, L + 1, L + 1)
+
+    def _check(self, tensors, shapes):
+        torch = self.torch
+        if len(tensors) != len(self.lmax):
+            raise SHToolsError(_status_message(1))
+        for t, shp in zip(tensors, shapes):
+            if (t.dtype != torch.float64 or not t.is_contiguous() or tuple(t.shape) != shp
+                    or t.device != self.device):
+                raise SHToolsError(_status_message(1))
+
+    def makegrid(self, cilms, out=None):
+        """Synthesis of every field; returns the list of grids (``out`` if given)."""
+        torch = self.torch
+        self._check(cilms, [self.cilm_shape(i) for i in range(len(self.lmax))])
+        if out is None:
+            out = [torch.empty(self.grid_shape(i), dtype=torch.float64, device=self.device)
+                   for i in range(len(self.lmax))]
+        self._check(out, [self.grid_shape(i) for i in range(len(self.lmax))])
+        stream = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        with torch.cuda.device(self.device):
+            st = _lib.lib().shtg_batch_makegrid_dh(self._plan, _ptr_array(cilms), _ptr_array(out), stream)
+        if st != 0:
+            raise SHToolsError(_status_message(st))
+        return out
+
+    def expand(self, grids, out=None):
+        """Analysis of every field; returns the list of coefficient arrays."""
+        torch = self.torch
+        self._check(grids, [self.grid_shape(i) for i in range(len(self.lmax))])
+        if out is None:
+            out = [torch.empty(self.cilm_shape(i), dtype=torch.float64, device=self.device)
+                   for i in range(len(self.lmax))]
+        self._check(out, [self.cilm_shape(i) for i in range(len(self.lmax))])
+        stream = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        with torch.cuda.device(self.device):
+            st = _lib.lib().shtg_batch_shexpand_dh(self._plan, _ptr_array(grids), _ptr_array(out), stream)
+        if st != 0:
+            raise SHToolsError(_status_message(st))
+        return out

@@ -7,6 +7,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -15,7 +17,7 @@
 
 #include "../../include/shtools_amd.h"
 #include "fftplan.hpp"
-#include "kernels_plain.hpp"
+#include "kernels.hpp"
 #include "tables.hpp"
 
 namespace shtg {
@@ -282,15 +284,13 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
 
     {
         ProfScope ps(SHTG_K_PACK, st);
-        hipLaunchKernelGGL(k_pack_synth, dim3(grid_blocks((long long)npack)), dim3(kThreads), 0, st, cilm, d1, d2,
-                           rt.s, rt.L, cpack, Lc);
+        hipLaunchKernelGGL(k_pack_synth, dim3(Lc + 1), dim3(kThreads), 0, st, cilm, d1, d2, rt.s, rt.L, cpack, Lc);
     }
     {
         ProfScope ps(SHTG_K_SYNTH_LEG, st);
-        const int K = 2;
-        const int nrb = (g.P + K * kThreads - 1) / (K * kThreads);
-        hipLaunchKernelGGL(k_synth_leg<K>, dim3((Lc + 1) * nrb), dim3(kThreads), 0, st, rt.alpha, rt.L, sd.seed,
-                           g.z, g.u, cpack, Lc, g.P, nrb, fbuf);
+        const int nrb = (g.P + kSynthK * kThreads - 1) / (kSynthK * kThreads);
+        hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nrb), dim3(kThreads), 0, st, rt.alpha, rt.L, sd.seed, g.z,
+                           g.u, cpack, Lc, g.P, nrb, fbuf);
     }
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
@@ -320,16 +320,11 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
         hipLaunchKernelGGL(k_anal_fft, dim3(g.P), dim3(kThreads), lds, st, g.fft, grid, ld, g.rows, g.nlat, g.w,
                            g.P, Lc, gbuf);
     }
-    const int need = (g.P + kThreads - 1) / kThreads;
-    const dim3 gl(Lc + 1), bl(kThreads);
-    ProfScope ps_leg(SHTG_K_ANAL_LEG, st);
-    if (need <= 1) hipLaunchKernelGGL(k_anal_leg<1>, gl, bl, 0, st, rt.alpha, rt.L, sd.seed, g.z, g.u, gbuf, Lc, g.P, cbuf);
-    else if (need <= 2) hipLaunchKernelGGL(k_anal_leg<2>, gl, bl, 0, st, rt.alpha, rt.L, sd.seed, g.z, g.u, gbuf, Lc, g.P, cbuf);
-    else if (need <= 4) hipLaunchKernelGGL(k_anal_leg<4>, gl, bl, 0, st, rt.alpha, rt.L, sd.seed, g.z, g.u, gbuf, Lc, g.P, cbuf);
-    else if (need <= 8) hipLaunchKernelGGL(k_anal_leg<8>, gl, bl, 0, st, rt.alpha, rt.L, sd.seed, g.z, g.u, gbuf, Lc, g.P, cbuf);
-    else if (need <= 12) hipLaunchKernelGGL(k_anal_leg<12>, gl, bl, 0, st, rt.alpha, rt.L, sd.seed, g.z, g.u, gbuf, Lc, g.P, cbuf);
-    else { (void)hipFreeAsync(ws, st); return ST_OTHER; }
-    ps_leg.end();
+    {
+        ProfScope ps(SHTG_K_ANAL_LEG, st);
+        hipLaunchKernelGGL(k_anal_leg, dim3(Lc + 1), dim3(kThreads), 0, st, rt.alpha, rt.L, sd.seed, g.z, g.u, gbuf,
+                           Lc, g.P, cbuf);
+    }
     const long long total = 2LL * d1 * d2;
     ProfScope ps_un(SHTG_K_UNPACK, st);
     hipLaunchKernelGGL(k_unpack_anal, dim3(grid_blocks(total)), dim3(kThreads), 0, st, cbuf, rt.post, rt.L, Lc, cilm,
@@ -337,6 +332,294 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
     if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
     (void)hipFreeAsync(ws, st);
     return ST_OK;
+}
+
+// ------------------------------------------------------------ batched plans
+
+// A batch of DH fields of uneven degree, transformed by one launch per stage
+// (pack, Legendre, FFT grouped by LDS size, unpack) over work items sorted
+// longest first.  Tables, ring geometry, FFT plans, item lists and
+// workspace are built once per plan.
+struct FftGroup {
+    int2 *items = nullptr;
+    int nitems = 0;
+    size_t lds = 0;
+};
+
+struct BatchPlan {
+    int dev = -1, nf = 0, sampling = 2, norm = 1, csphase = 1, Lmax = 0;
+    std::vector<int> L;
+    BField *fields = nullptr;
+    int2 *items_pack = nullptr;
+    int n_pack = 0;
+    int4 *items_sleg = nullptr;
+    int n_sleg = 0;
+    int2 *items_aleg = nullptr;
+    int n_aleg = 0;
+    int2 *items_unpack = nullptr;
+    int n_unpack = 0;
+    std::vector<FftGroup> fft;
+    double *z = nullptr, *u = nullptr;
+    double2 *w = nullptr;
+    int2 *rows = nullptr;
+    FftDev *plans = nullptr;
+    std::vector<FftDev> hplans;
+    double2 *cbuf = nullptr;  // packed coefficients (synthesis) / sums (analysis)
+    double4 *fbuf = nullptr;  // per-order ring sums
+    const double **ptrs = nullptr;  // device: [4][nf] bound field pointers
+    std::vector<const void *> bound[4];
+    BTables T{};
+};
+
+template <class T>
+static T *dev_upload_n(const T *v, size_t n)
+{
+    std::vector<T> tmp(v, v + n);
+    return dev_upload(tmp);
+}
+
+static void batch_free(BatchPlan *b)
+{
+    if (!b) return;
+    (void)hipFree(b->fields);
+    (void)hipFree(b->items_pack);
+    (void)hipFree(b->items_sleg);
+    (void)hipFree(b->items_aleg);
+    (void)hipFree(b->items_unpack);
+    for (auto &g : b->fft) (void)hipFree(g.items);
+    (void)hipFree(b->z);
+    (void)hipFree(b->u);
+    (void)hipFree(b->w);
+    (void)hipFree(b->rows);
+    (void)hipFree(b->plans);
+    for (auto &p : b->hplans) {
+        (void)hipFree((void *)p.tw);
+        (void)hipFree((void *)p.perm);
+        (void)hipFree((void *)p.chirp);
+        (void)hipFree((void *)p.bhat);
+    }
+    (void)hipFree(b->cbuf);
+    (void)hipFree(b->fbuf);
+    (void)hipFree(b->ptrs);
+    delete b;
+}
+
+static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm, int csphase, BatchPlan **out)
+{
+    if (nfields <= 0 || !lmax || !out) return ST_DIMS;
+    if (sampling != 1 && sampling != 2) return ST_BOUNDS;
+    if (norm < 1 || norm > 4 || (csphase != 1 && csphase != -1)) return ST_BOUNDS;
+    int Lmax = 0;
+    for (int i = 0; i < nfields; ++i) {
+        if (lmax[i] < 0) return ST_BOUNDS;
+        Lmax = std::max(Lmax, lmax[i]);
+    }
+    const int dev = current_device();
+    if (dev < 0) return ST_OTHER;
+    configure_kernels();
+    std::unique_ptr<BatchPlan, void (*)(BatchPlan *)> b(new BatchPlan(), batch_free);
+    b->dev = dev;
+    b->nf = nfields;
+    b->sampling = sampling;
+    b->norm = norm;
+    b->csphase = csphase;
+    b->Lmax = Lmax;
+    b->L.assign(lmax, lmax + nfields);
+    const DevRecurrence *rt = get_recurrence(dev, norm, Lmax);
+    const DevSeeds *sd = get_seeds(dev, norm, csphase, Lmax);
+    if (!rt || !sd) return ST_ALLOC;
+
+    // ring geometry and FFT plans per distinct degree
+    const double pi = std::acos(-1.0);
+    std::map<int, std::pair<int, int>> geo_of;  // L -> (geo offset, plan index)
+    std::vector<double> z, u, w;
+    std::vector<int> rows;
+    std::vector<size_t> plan_lds;
+    for (int i = 0; i < nfields; ++i) {
+        const int L = lmax[i];
+        if (geo_of.count(L)) continue;
+        const int n = 2 * L + 2, nlong = sampling * n, P = n / 2 + 1;
+        const int off = (int)z.size();
+        std::vector<double> aj;
+        dh_weights(n, aj);
+        const double c = std::sqrt(2 * pi) * std::sqrt(4.0 * pi) / (double)nlong;
+        for (int s = 0; s < P; ++s) {
+            const double zz = (s == n / 2) ? 0.0 : std::cos(pi * (double)s / (double)n);
+            z.push_back(zz);
+            u.push_back((s == n / 2) ? 1.0 : std::sqrt((1.0 - zz) * (1.0 + zz)));
+            rows.push_back(s);
+            rows.push_back((s == n / 2) ? -1 : n - s);
+            w.push_back(c * aj[s]);
+            w.push_back((s == 0 || s == n / 2) ? 0.0 : c * aj[n - s]);
+        }
+        FftPlanHost fp = make_fft_plan(nlong);
+        FftDev fd;
+        if (!upload_fft(fp, fd)) return ST_OTHER;
+        b->hplans.push_back(fd);
+        plan_lds.push_back((size_t)fp.m * sizeof(double2));
+        geo_of[L] = {off, (int)b->hplans.size() - 1};
+    }
+    b->z = dev_upload(z);
+    b->u = dev_upload(u);
+    b->w = reinterpret_cast<double2 *>(dev_upload(w));
+    b->rows = reinterpret_cast<int2 *>(dev_upload(rows));
+    b->plans = dev_upload(b->hplans);
+    if (!b->z || !b->u || !b->w || !b->rows || !b->plans) return ST_ALLOC;
+
+    // fields and workspace offsets
+    std::vector<BField> fields(nfields);
+    long long coff = 0, foff = 0;
+    for (int i = 0; i < nfields; ++i) {
+        const int L = lmax[i], n = 2 * L + 2;
+        BField &f = fields[i];
+        f.L = L;
+        f.P = n / 2 + 1;
+        f.nlat = n;
+        f.nlong = sampling * n;
+        f.geo = geo_of[L].first;
+        f.fft = geo_of[L].second;
+        f.coff = coff;
+        f.foff = foff;
+        coff += tri_size(L);
+        foff += (long long)(L + 1) * f.P;
+    }
+    b->fields = dev_upload(fields);
+    if (!b->fields) return ST_ALLOC;
+    if (!HIP_OK(hipMalloc(&b->cbuf, (size_t)coff * sizeof(double2)))) return ST_ALLOC;
+    if (!HIP_OK(hipMalloc(&b->fbuf, (size_t)foff * sizeof(double4)))) return ST_ALLOC;
+    if (!HIP_OK(hipMalloc(&b->ptrs, 4 * (size_t)nfields * sizeof(void *)))) return ST_ALLOC;
+    (void)hipMemset(b->ptrs, 0, 4 * (size_t)nfields * sizeof(void *));
+
+    // work items, longest first
+    struct W { double w; int a, b, c; };
+    std::vector<W> wp, ws, wa, wu;
+    for (int i = 0; i < nfields; ++i) {
+        const BField &f = fields[i];
+        for (int m = 0; m <= f.L; ++m) {
+            wp.push_back({(double)(f.L - m + 1), i, m, 0});
+            const int span = kSynthK * kThreads;
+            for (int s0 = 0; s0 < f.P; s0 += span)
+                ws.push_back({(double)(f.L - m + 1) * std::min(span, f.P - s0), i, m, s0});
+            wa.push_back({(double)(f.L - m + 1) * f.P, i, m, 0});
+        }
+        for (int l = 0; l <= f.L; ++l) wu.push_back({(double)f.L, i, l, 0});
+    }
+    auto by_work = [](const W &x, const W &y) { return x.w > y.w; };
+    std::stable_sort(wp.begin(), wp.end(), by_work);
+    std::stable_sort(ws.begin(), ws.end(), by_work);
+    std::stable_sort(wa.begin(), wa.end(), by_work);
+    std::vector<int2> ip, ia, iu;
+    std::vector<int4> is;
+    for (auto &x : wp) ip.push_back(make_int2(x.a, x.b));
+    for (auto &x : ws) is.push_back(make_int4(x.a, x.b, x.c, 0));
+    for (auto &x : wa) ia.push_back(make_int2(x.a, x.b));
+    for (auto &x : wu) iu.push_back(make_int2(x.a, x.b));
+    b->items_pack = dev_upload(ip);
+    b->items_sleg = dev_upload(is);
+    b->items_aleg = dev_upload(ia);
+    b->items_unpack = dev_upload(iu);
+    b->n_pack = (int)ip.size();
+    b->n_sleg = (int)is.size();
+    b->n_aleg = (int)ia.size();
+    b->n_unpack = (int)iu.size();
+    if (!b->items_pack || !b->items_sleg || !b->items_aleg || !b->items_unpack) return ST_ALLOC;
+
+    // FFT items grouped by LDS size class (larger first)
+    std::map<size_t, std::vector<W>, std::greater<size_t>> groups;
+    for (int i = 0; i < nfields; ++i) {
+        const BField &f = fields[i];
+        const size_t need = plan_lds[f.fft];
+        size_t cls = 16 * 1024;
+        while (cls < need) cls *= 2;
+        if (cls > kMaxLds) cls = kMaxLds;
+        for (int s = 0; s < f.P; ++s) groups[cls].push_back({(double)need, i, s, 0});
+    }
+    for (auto &kv : groups) {
+        std::stable_sort(kv.second.begin(), kv.second.end(), by_work);
+        std::vector<int2> it;
+        for (auto &x : kv.second) it.push_back(make_int2(x.a, x.b));
+        FftGroup g;
+        g.items = dev_upload(it);
+        g.nitems = (int)it.size();
+        g.lds = kv.first;
+        if (!g.items) return ST_ALLOC;
+        b->fft.push_back(g);
+    }
+    b->T.alpha = rt->alpha;
+    b->T.s = rt->s;
+    b->T.post = rt->post;
+    b->T.seed = sd->seed;
+    b->T.Lt = rt->L;
+    b->T.z = b->z;
+    b->T.u = b->u;
+    b->T.w = b->w;
+    b->T.rows = b->rows;
+    b->T.plans = b->plans;
+    for (auto &v : b->bound) v.assign(nfields, nullptr);
+    *out = b.release();
+    return ST_OK;
+}
+
+// Upload field pointers of slot k (0: synthesis in, 1: synthesis out,
+// 2: analysis in, 3: analysis out) when they differ from the bound ones.
+static int batch_bind(BatchPlan *b, int k, const void *const *p)
+{
+    bool same = true;
+    for (int i = 0; i < b->nf; ++i) {
+        if (!p[i]) return ST_DIMS;
+        if (b->bound[k][i] != p[i]) same = false;
+    }
+    if (same) return ST_OK;
+    if (!HIP_OK(hipMemcpy(b->ptrs + (size_t)k * b->nf, p, sizeof(void *) * b->nf, hipMemcpyHostToDevice)))
+        return ST_OTHER;
+    for (int i = 0; i < b->nf; ++i) b->bound[k][i] = p[i];
+    return ST_OK;
+}
+
+static int batch_synthesis(BatchPlan *b, hipStream_t st)
+{
+    const double *const *cin = b->ptrs;
+    double *const *gout = (double *const *)(b->ptrs + b->nf);
+    {
+        ProfScope ps(SHTG_K_PACK, st);
+        hipLaunchKernelGGL(kb_pack, dim3(b->n_pack), dim3(kThreads), 0, st, b->fields, b->items_pack, b->T, cin,
+                           b->cbuf);
+    }
+    {
+        ProfScope ps(SHTG_K_SYNTH_LEG, st);
+        hipLaunchKernelGGL(kb_synth_leg, dim3(b->n_sleg), dim3(kThreads), 0, st, b->fields, b->items_sleg, b->T,
+                           b->cbuf, b->fbuf);
+    }
+    {
+        ProfScope ps(SHTG_K_SYNTH_FFT, st);
+        for (auto &g : b->fft)
+            hipLaunchKernelGGL(kb_synth_fft, dim3(g.nitems), dim3(kThreads), g.lds, st, b->fields, g.items, b->T,
+                               b->fbuf, gout);
+    }
+    return HIP_OK(hipGetLastError()) ? ST_OK : ST_OTHER;
+}
+
+static int batch_analysis(BatchPlan *b, hipStream_t st)
+{
+    const double *const *gin = b->ptrs + 2 * (size_t)b->nf;
+    double *const *cout = (double *const *)(b->ptrs + 3 * (size_t)b->nf);
+    {
+        ProfScope ps(SHTG_K_ANAL_FFT, st);
+        for (auto &g : b->fft)
+            hipLaunchKernelGGL(kb_anal_fft, dim3(g.nitems), dim3(kThreads), g.lds, st, b->fields, g.items, b->T, gin,
+                               b->fbuf);
+    }
+    {
+        ProfScope ps(SHTG_K_ANAL_LEG, st);
+        hipLaunchKernelGGL(kb_anal_leg, dim3(b->n_aleg), dim3(kThreads), 0, st, b->fields, b->items_aleg, b->T,
+                           b->fbuf, b->cbuf);
+    }
+    {
+        ProfScope ps(SHTG_K_UNPACK, st);
+        hipLaunchKernelGGL(kb_unpack, dim3(b->n_unpack), dim3(kThreads), 0, st, b->fields, b->items_unpack, b->T,
+                           b->cbuf, cout);
+    }
+    return HIP_OK(hipGetLastError()) ? ST_OK : ST_OTHER;
 }
 
 }  // namespace shtg
@@ -552,6 +835,45 @@ int shtg_device_count(void)
 }
 
 const char *shtg_version(void) { return "shtools_amd 0.1 (gfx950)"; }
+
+int shtg_batch_create_dh(int nfields, const int *lmax, int sampling, int norm, int csphase,
+                         shtg_batch_plan **plan)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    BatchPlan *b = nullptr;
+    int st = batch_create_dh(nfields, lmax, sampling, norm, csphase, &b);
+    if (plan) *plan = reinterpret_cast<shtg_batch_plan *>(b);
+    return st;
+}
+
+int shtg_batch_destroy(shtg_batch_plan *plan)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    batch_free(reinterpret_cast<BatchPlan *>(plan));
+    return ST_OK;
+}
+
+int shtg_batch_makegrid_dh(shtg_batch_plan *plan, const double *const *cilm, double *const *grid, void *stream)
+{
+    if (!plan || !cilm || !grid) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    int st = batch_bind(b, 0, (const void *const *)cilm);
+    if (!st) st = batch_bind(b, 1, (const void *const *)grid);
+    if (st) return st;
+    return batch_synthesis(b, (hipStream_t)stream);
+}
+
+int shtg_batch_shexpand_dh(shtg_batch_plan *plan, const double *const *grid, double *const *cilm, void *stream)
+{
+    if (!plan || !cilm || !grid) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    int st = batch_bind(b, 2, (const void *const *)grid);
+    if (!st) st = batch_bind(b, 3, (const void *const *)cilm);
+    if (st) return st;
+    return batch_analysis(b, (hipStream_t)stream);
+}
 
 int shtg_profile_enable(int on)
 {

@@ -13,6 +13,21 @@ def rand_cilm(rng, L, slope=-2.0):
     return c
 
 
+def for_norm(c, norm):
+    """Re-express 4pi-normalised coefficients for the unnormalised functions
+    (norm 3), C_unnorm = C_4pi sqrt((2-d_m0)(2l+1)(l-m)!/(l+m)!), so that the
+    field and its grid keep an O(1) range; other norms are returned as is."""
+    if norm != 3:
+        return c
+    from math import lgamma
+    L = c.shape[1] - 1
+    f = np.zeros((L + 1, L + 1))
+    for l in range(L + 1):
+        for m in range(l + 1):
+            f[l, m] = np.sqrt((1.0 if m == 0 else 2.0) * (2 * l + 1) * np.exp(lgamma(l - m + 1) - lgamma(l + m + 1)))
+    return c * f[None]
+
+
 def grid_err(a, b):
     """max |a - b| / max |b| (the grid parity measure)."""
     a = np.asarray(a)

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import coef_err, grid_err, rand_cilm
+from helpers import coef_err, for_norm, grid_err, rand_cilm
 
 pytestmark = pytest.mark.gpu
 
@@ -38,7 +38,7 @@ DH_CASES = [(L, norm, cs, s, e)
 def test_dh_small_all_options(B, L, norm, cs, sampling, extend):
     print(f"case DH L={L} norm={norm} cs={cs} s={sampling} e={extend}", flush=True)
     rng = np.random.default_rng(1000 + L)
-    c = rand_cilm(rng, L)
+    c = for_norm(rand_cilm(rng, L), norm)
     g_ref = oracle.MakeGridDH(c, norm=norm, sampling=sampling, csphase=cs, extend=extend)
     g = B.MakeGridDH(c, norm=norm, sampling=sampling, csphase=cs, extend=extend)
     assert g.shape == g_ref.shape
@@ -58,7 +58,7 @@ GLQ_CASES = [(L, norm, cs, e) for L in (0, 1, 2, 5, 6, 31, 64) for norm in (1, 2
 def test_glq_small_all_options(B, L, norm, cs, extend):
     print(f"case GLQ L={L} norm={norm} cs={cs} e={extend}", flush=True)
     rng = np.random.default_rng(2000 + L)
-    c = rand_cilm(rng, L)
+    c = for_norm(rand_cilm(rng, L), norm)
     z, w = oracle.SHGLQ(L)
     g_ref = oracle.MakeGridGLQ(c, z, norm=norm, csphase=cs, extend=extend)
     g = B.MakeGridGLQ(c, z, norm=norm, csphase=cs, extend=extend)

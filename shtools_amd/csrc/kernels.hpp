@@ -33,10 +33,12 @@
 
 namespace shtg {
 
-constexpr int kThreads = 256;
-constexpr int kSynthK = 2;   // ring pairs per lane in synthesis
-constexpr int kAnalK = 4;    // ring pairs per lane and pass in analysis
-constexpr int kAnalLC = 8;   // degrees per reduction chunk in analysis
+constexpr int kThreads = 256;  // ring-FFT and generic workgroups
+constexpr int kWave = 64;      // Legendre workgroups are single waves
+constexpr int kSynthKMax = 4;  // ring pairs per lane in synthesis (1, 2 or 4)
+constexpr int kAnalK = 8;      // ring pairs per lane and pass in analysis
+constexpr int kAnalLC = 8;     // degrees per reduction chunk in analysis
+constexpr int kRedStride = 65; // LDS row stride (doubles) of the analysis reduction
 
 // ====================================================================== tiles
 
@@ -49,31 +51,31 @@ __device__ __forceinline__ double2 pack_one(const double *__restrict__ cilm, int
     return make_double2(c * s, sn * s);
 }
 
-// Synthesis sums of order m for the ring pairs slot0 + k*kThreads + tid.
-// al: recursion row of order m (al[j] = alpha_{m+j}); cp: packed
-// coefficients of order m; frow: fbuf row of order m.
+// The reference's rescalem for order m at a ring with sin(theta) = u
+// (1 for m = 0, where the recursion is unscaled: src/MakeGridDH.f95:516-540).
+__device__ __forceinline__ double order_rescale(double u, int m) { return m == 0 ? 1.0 : scaled_upow(u, m); }
+
+// Synthesis sums of order m for the ring pairs slot0 + k*64 + lane (one
+// wave).  al: recursion row of order m (al[j] = alpha_{m+j}); cp: packed
+// coefficients of order m; frow: fbuf row of order m.  The sums are stored
+// without the order's rescale factor, which the ring FFT applies.
 template <int K>
 __device__ __forceinline__ void synth_leg_tile(const double *__restrict__ al, double sd,
-                                               const double *__restrict__ ringz, const double *__restrict__ ringu,
-                                               const double2 *__restrict__ cp, int jmax, int m, int P, int slot0,
-                                               double4 *__restrict__ frow)
+                                               const double *__restrict__ ringz, const double2 *__restrict__ cp,
+                                               int jmax, int P, int slot0, double4 *__restrict__ frow)
 {
-    double z[K], q1[K], q2[K], r[K], eC[K], eS[K], oC[K], oS[K];
-    int slot[K];
+    const int lane = threadIdx.x & (kWave - 1);
+    double z[K], q1[K], q2[K], eC[K], eS[K], oC[K], oS[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        slot[k] = slot0 + k * kThreads + threadIdx.x;
-        const bool ok = slot[k] < P;
-        double zz = ok ? ringz[slot[k]] : 0.0;
-        const double uu = ok ? ringu[slot[k]] : 1.0;
-        r[k] = (m == 0) ? 1.0 : scaled_upow(uu, m);
-        if (r[k] == 0.0) zz = 0.0;  // keeps the unused recursion bounded
-        z[k] = zz;
+        const int s = slot0 + k * kWave + lane;
+        z[k] = (s < P) ? ringz[s] : 0.0;
         q1[k] = 0.0;
         q2[k] = -sd;  // the first step then yields q_m = seed
         eC[k] = eS[k] = oC[k] = oS[k] = 0.0;
     }
     int j = 0;
+#pragma unroll 2
     for (; j + 1 <= jmax; j += 2) {
         const double a0 = al[j], a1 = al[j + 1];
         const double2 c0 = cp[j], c1 = cp[j + 1];
@@ -101,44 +103,53 @@ __device__ __forceinline__ void synth_leg_tile(const double *__restrict__ al, do
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        if (slot[k] < P) {
-            double4 o = make_double4(0.0, 0.0, 0.0, 0.0);
-            if (r[k] != 0.0)
-                o = make_double4((eC[k] + oC[k]) * r[k], (eS[k] + oS[k]) * r[k], (eC[k] - oC[k]) * r[k],
-                                 (eS[k] - oS[k]) * r[k]);
-            frow[slot[k]] = o;
-        }
+        const int s = slot0 + k * kWave + lane;
+        if (s < P) frow[s] = make_double4(eC[k] + oC[k], eS[k] + oS[k], eC[k] - oC[k], eS[k] - oS[k]);
     }
 }
 
-// Analysis sums of order m over all P ring pairs: out[j] = sum over slots of
-// q_{m+j}(z) * (E or O by parity of j), written as (C, S) pairs.  Passes of
-// kAnalK * kThreads slots; the first pass stores, later passes accumulate.
-// red: kThreads * 2 * kAnalLC doubles of LDS.
+template <int K>
+__device__ __forceinline__ void synth_leg_dispatch(int kk, const double *__restrict__ al, double sd,
+                                                   const double *__restrict__ ringz,
+                                                   const double2 *__restrict__ cp, int jmax, int P, int slot0,
+                                                   double4 *__restrict__ frow)
+{
+    if (kk <= 1) synth_leg_tile<1>(al, sd, ringz, cp, jmax, P, slot0, frow);
+    else if (kk == 2) synth_leg_tile<2>(al, sd, ringz, cp, jmax, P, slot0, frow);
+    else synth_leg_tile<K>(al, sd, ringz, cp, jmax, P, slot0, frow);
+}
+
+// Analysis sums of order m over ring pairs [s0, P) (one wave):
+// out[j] = sum over slots of q_{m+j}(z) * (E or O by parity of j), as (C, S)
+// pairs.  grow holds the order's (E_C, E_S, O_C, O_S) already multiplied by
+// the order's rescale factor.  Passes of kAnalK * 64 slots; the first pass
+// stores, later passes accumulate.  Each chunk of kAnalLC degrees is reduced
+// over the wave through red (16 x kRedStride doubles of LDS).
 __device__ void anal_leg_tile(const double *__restrict__ al, double sd, const double *__restrict__ ringz,
-                              const double *__restrict__ ringu, const double4 *__restrict__ grow, int jmax, int m,
-                              int P, double2 *__restrict__ out, double *red)
+                              const double4 *__restrict__ grow, int jmax, int s0, int P,
+                              double2 *__restrict__ out, double *red)
 {
     constexpr int K = kAnalK, LC = kAnalLC;
-    for (int p0 = 0; p0 < P; p0 += K * kThreads) {
+    const int lane = threadIdx.x & (kWave - 1);
+    for (int p0 = s0; p0 < P; p0 += K * kWave) {
         double z[K], q1[K], q2[K], eC[K], eS[K], oC[K], oS[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const int s = p0 + k * kThreads + threadIdx.x;
-            const bool ok = s < P;
-            double zz = ok ? ringz[s] : 0.0;
-            const double uu = ok ? ringu[s] : 1.0;
-            const double r = (m == 0) ? 1.0 : scaled_upow(uu, m);
-            double4 g = ok ? grow[s] : make_double4(0.0, 0.0, 0.0, 0.0);
-            if (r == 0.0 || !ok) {
-                zz = 0.0;
-                g = make_double4(0.0, 0.0, 0.0, 0.0);
+            const int s = p0 + k * kWave + lane;
+            double4 g = make_double4(0.0, 0.0, 0.0, 0.0);
+            double zz = 0.0;
+            if (s < P) {
+                g = grow[s];
+                zz = ringz[s];
             }
+            // a ring whose data vanish (rescale underflow, zero weight) keeps
+            // a bounded recursion so that 0 * q stays 0
+            if (g.x == 0.0 && g.y == 0.0 && g.z == 0.0 && g.w == 0.0) zz = 0.0;
             z[k] = zz;
-            eC[k] = g.x * r;
-            eS[k] = g.y * r;
-            oC[k] = g.z * r;
-            oS[k] = g.w * r;
+            eC[k] = g.x;
+            eS[k] = g.y;
+            oC[k] = g.z;
+            oS[k] = g.w;
             q1[k] = 0.0;
             q2[k] = -sd;
         }
@@ -166,52 +177,54 @@ __device__ void anal_leg_tile(const double *__restrict__ al, double sd, const do
                     }
                 }
             }
+            // transpose through LDS: lane (v, part) sums 16 lanes of value v
 #pragma unroll
-            for (int v = 0; v < 2 * LC; ++v) red[v * kThreads + threadIdx.x] = acc[v >> 1][v & 1];
+            for (int v = 0; v < 2 * LC; ++v) red[v * kRedStride + lane] = acc[v >> 1][v & 1];
             __syncthreads();
-            {
-                const int v = threadIdx.x >> 4, part = threadIdx.x & 15;
-                double sum = 0.0;
+            const int v = lane & 15, part = lane >> 4;
+            double sum = 0.0;
 #pragma unroll
-                for (int i = 0; i < 16; ++i) sum += red[v * kThreads + part * 16 + i];
-                sum += __shfl_xor(sum, 8, 16);
-                sum += __shfl_xor(sum, 4, 16);
-                sum += __shfl_xor(sum, 2, 16);
-                sum += __shfl_xor(sum, 1, 16);
-                const int jj = v >> 1;
-                if (part == 0 && j0 + jj <= jmax) {
-                    double *o = reinterpret_cast<double *>(out + j0 + jj) + (v & 1);
-                    *o = (p0 == 0) ? sum : *o + sum;
-                }
+            for (int i = 0; i < 16; ++i) sum += red[v * kRedStride + part * 16 + i];
+            sum += __shfl_xor(sum, 16);
+            sum += __shfl_xor(sum, 32);
+            const int jj = v >> 1;
+            if (part == 0 && j0 + jj <= jmax) {
+                double *o = reinterpret_cast<double *>(out + j0 + jj) + (v & 1);
+                *o = (p0 == s0) ? sum : *o + sum;
             }
             __syncthreads();
         }
     }
 }
 
-// Ring-pair synthesis FFT for slot s: fbuf holds rows of order m at stride P.
-__device__ void synth_fft_tile(const FftDev &F, const double4 *__restrict__ fbuf, int P, int Lc, int s, int2 rw,
-                               int nrows, double *__restrict__ grid, long long ld, int ext_col, double2 *lds)
+// Ring-pair synthesis FFT for slot s: fbuf holds rows of order m at stride P
+// (unscaled sums); u = sin(theta) of the slot gives the orders' rescale.
+__device__ void synth_fft_tile(const FftDev &F, const double4 *__restrict__ fbuf, int P, int Lc, int s, double u,
+                               int2 rw, int nrows, double *__restrict__ grid, long long ld, int ext_col,
+                               double2 *lds)
 {
     const int n = F.n;
-    for (int k = threadIdx.x; k < F.m; k += blockDim.x) {
-        double2 zc = make_double2(0.0, 0.0);
-        if (k < n) {
-            // Z_k = X_k + i Y_k with X (north) and Y (south) Hermitian:
-            // X_0 = C_N, X_k = (C_N - i S_N)/2 and X_{n-k} = conj(X_k).
-            if (k == 0) {
-                const double4 f = fbuf[s];
-                zc = make_double2(f.x, -f.z);
-            } else if (k <= Lc) {
-                const double4 f = fbuf[(size_t)k * P + s];
-                zc = make_double2(0.5 * (f.x + f.w), -0.5 * (f.z - f.y));
-            } else if (k >= n - Lc) {
-                const double4 f = fbuf[(size_t)(n - k) * P + s];
-                zc = make_double2(0.5 * (f.x - f.w), -0.5 * (f.y + f.z));
-            }
-            zc = fft_prep_in(F, k, zc);  // conj(Z_k): inverse via conj(F(conj Z))
+    for (int k = threadIdx.x; k < F.m; k += blockDim.x) lds[k] = make_double2(0.0, 0.0);
+    __syncthreads();
+    for (int k = threadIdx.x; k <= Lc; k += blockDim.x) {
+        // Z_k = X_k + i Y_k with X (north) and Y (south) Hermitian:
+        // X_0 = C_N, X_k = (C_N - i S_N)/2 and X_{n-k} = conj(X_k); stored
+        // conjugated (inverse DFT via conj(F(conj Z))).
+        const double r = order_rescale(u, k);
+        double4 f = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (r != 0.0) {
+            f = fbuf[(size_t)k * P + s];
+            f.x *= r;
+            f.y *= r;
+            f.z *= r;
+            f.w *= r;
         }
-        lds[k] = zc;
+        if (k == 0) {
+            lds[0] = fft_prep_in(F, 0, make_double2(f.x, -f.z));
+        } else {
+            lds[k] = fft_prep_in(F, k, make_double2(0.5 * (f.x + f.w), -0.5 * (f.z - f.y)));
+            lds[n - k] = fft_prep_in(F, n - k, make_double2(0.5 * (f.x - f.w), -0.5 * (f.y + f.z)));
+        }
     }
     __syncthreads();
     fft_forward(lds, F);
@@ -229,9 +242,11 @@ __device__ void synth_fft_tile(const FftDev &F, const double4 *__restrict__ fbuf
 }
 
 // Ring-pair analysis FFT for slot s; w = (w_north, w_south) include the
-// quadrature weight and the reference's constant factors.
+// quadrature weight and the reference's constant factors; the outputs carry
+// the order's rescale factor for the ring (sin(theta) = u).
 __device__ void anal_fft_tile(const FftDev &F, const double *__restrict__ grid, long long ld, int2 rw, int nrows,
-                              double2 w, int P, int Lc, int s, double4 *__restrict__ gbuf, double2 *lds)
+                              double2 w, double u, int P, int Lc, int s, double4 *__restrict__ gbuf,
+                              double2 *lds)
 {
     const int n = F.n;
     const bool north = rw.x >= 0 && rw.x < nrows;
@@ -254,7 +269,9 @@ __device__ void anal_fft_tile(const FftDev &F, const double *__restrict__ grid, 
         const double xr = 0.5 * (zm.x + zn.x), xi = 0.5 * (zm.y - zn.y);
         const double yr = 0.5 * (zm.y + zn.y), yi = 0.5 * (zn.x - zm.x);
         const double f1c = wn * xr, f1s = -wn * xi, f2c = ws * yr, f2s = -ws * yi;
-        gbuf[(size_t)mm * P + s] = make_double4(f1c + f2c, f1s + f2s, f1c - f2c, f1s - f2s);
+        const double r = order_rescale(u, mm);
+        gbuf[(size_t)mm * P + s] =
+            make_double4((f1c + f2c) * r, (f1s + f2s) * r, (f1c - f2c) * r, (f1s - f2s) * r);
     }
 }
 
@@ -270,49 +287,52 @@ __global__ __launch_bounds__(kThreads) void k_pack_synth(const double *__restric
     for (int j = threadIdx.x; j <= Lc - m; j += blockDim.x) o[j] = pack_one(cilm, d1, d2, m + j, m, s[j]);
 }
 
-__global__ __launch_bounds__(kThreads) void k_synth_leg(const double *__restrict__ alpha, int Lt,
-                                                        const double *__restrict__ seed,
-                                                        const double *__restrict__ ringz,
-                                                        const double *__restrict__ ringu,
-                                                        const double2 *__restrict__ cpack, int Lc, int P,
-                                                        int nrb, double4 *__restrict__ fbuf)
+// grid: (Lc+1) * nsb single-wave workgroups, order m = blockIdx / nsb, slot
+// block of kk*64 ring pairs.
+__global__ __launch_bounds__(kWave) void k_synth_leg(const double *__restrict__ alpha, int Lt,
+                                                     const double *__restrict__ seed,
+                                                     const double *__restrict__ ringz,
+                                                     const double2 *__restrict__ cpack, int Lc, int P, int nsb,
+                                                     int kk, double4 *__restrict__ fbuf)
 {
-    const int m = blockIdx.x / nrb;
-    const int rb = blockIdx.x - m * nrb;
-    synth_leg_tile<kSynthK>(alpha + tri_offd(m, Lt), seed[m], ringz, ringu, cpack + tri_offd(m, Lc), Lc - m, m, P,
-                            rb * kSynthK * kThreads, fbuf + (size_t)m * P);
+    const int m = blockIdx.x / nsb;
+    const int sb = blockIdx.x - m * nsb;
+    synth_leg_dispatch<kSynthKMax>(kk, alpha + tri_offd(m, Lt), seed[m], ringz, cpack + tri_offd(m, Lc), Lc - m,
+                                   P, sb * kk * kWave, fbuf + (size_t)m * P);
 }
 
 // rows[s] = (north row, south row or -1); rows >= nrows are skipped.
 __global__ __launch_bounds__(kThreads) void k_synth_fft(FftDev F, const double4 *__restrict__ fbuf, int P, int Lc,
+                                                        const double *__restrict__ ringu,
                                                         const int2 *__restrict__ rows, int nrows,
                                                         double *__restrict__ grid, long long ld, int ext_col)
 {
     extern __shared__ double2 lds[];
     const int s = blockIdx.x;
-    synth_fft_tile(F, fbuf, P, Lc, s, rows[s], nrows, grid, ld, ext_col, lds);
+    synth_fft_tile(F, fbuf, P, Lc, s, ringu[s], rows[s], nrows, grid, ld, ext_col, lds);
 }
 
 __global__ __launch_bounds__(kThreads) void k_anal_fft(FftDev F, const double *__restrict__ grid, long long ld,
+                                                       const double *__restrict__ ringu,
                                                        const int2 *__restrict__ rows, int nrows,
                                                        const double2 *__restrict__ wts, int P, int Lc,
                                                        double4 *__restrict__ gbuf)
 {
     extern __shared__ double2 lds[];
     const int s = blockIdx.x;
-    anal_fft_tile(F, grid, ld, rows[s], nrows, wts[s], P, Lc, s, gbuf, lds);
+    anal_fft_tile(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, Lc, s, gbuf, lds);
 }
 
-__global__ __launch_bounds__(kThreads) void k_anal_leg(const double *__restrict__ alpha, int Lt,
-                                                       const double *__restrict__ seed,
-                                                       const double *__restrict__ ringz,
-                                                       const double *__restrict__ ringu,
-                                                       const double4 *__restrict__ gbuf, int Lc, int P,
-                                                       double2 *__restrict__ cbuf)
+// one single-wave workgroup per order m over ring pairs [s0, P)
+__global__ __launch_bounds__(kWave) void k_anal_leg(const double *__restrict__ alpha, int Lt,
+                                                    const double *__restrict__ seed,
+                                                    const double *__restrict__ ringz,
+                                                    const double4 *__restrict__ gbuf, int Lc, int s0, int P,
+                                                    double2 *__restrict__ cbuf)
 {
-    __shared__ double red[2 * kAnalLC * kThreads];
+    __shared__ double red[2 * kAnalLC * kRedStride];
     const int m = blockIdx.x;
-    anal_leg_tile(alpha + tri_offd(m, Lt), seed[m], ringz, ringu, gbuf + (size_t)m * P, Lc - m, m, P,
+    anal_leg_tile(alpha + tri_offd(m, Lt), seed[m], ringz, gbuf + (size_t)m * P, Lc - m, s0, P,
                   cbuf + tri_offd(m, Lc), red);
 }
 
@@ -378,56 +398,77 @@ __global__ __launch_bounds__(kThreads) void kb_pack(const BField *__restrict__ f
     for (int j = threadIdx.x; j <= L - m; j += blockDim.x) o[j] = pack_one(c, L + 1, L + 1, m + j, m, s[j]);
 }
 
-// items: (field, m, first slot)
-__global__ __launch_bounds__(kThreads) void kb_synth_leg(const BField *__restrict__ fields,
-                                                         const int4 *__restrict__ items, BTables T,
-                                                         const double2 *__restrict__ cpack,
-                                                         double4 *__restrict__ fbuf)
+// items: (field, m, first slot, ring pairs per lane)
+__global__ __launch_bounds__(kWave) void kb_synth_leg(const BField *__restrict__ fields,
+                                                      const int4 *__restrict__ items, BTables T,
+                                                      const double2 *__restrict__ cpack, double4 *__restrict__ fbuf)
 {
     const int4 it = items[blockIdx.x];
     const BField f = fields[it.x];
     const int m = it.y;
-    synth_leg_tile<kSynthK>(T.alpha + tri_offd(m, T.Lt), T.seed[m], T.z + f.geo, T.u + f.geo,
-                            cpack + f.coff + tri_offd(m, f.L), f.L - m, m, f.P, it.z,
-                            fbuf + f.foff + (size_t)m * f.P);
+    synth_leg_dispatch<kSynthKMax>(it.w, T.alpha + tri_offd(m, T.Lt), T.seed[m], T.z + f.geo,
+                                   cpack + f.coff + tri_offd(m, f.L), f.L - m, f.P, it.z,
+                                   fbuf + f.foff + (size_t)m * f.P);
 }
 
-// items: (field, slot); all fields of a launch share its LDS size.
+// Ring FFT items (field, first slot): each workgroup transforms kFftSlots
+// consecutive ring pairs in turn.  Workgroups are mapped so that groups of 4
+// consecutive items run on the same XCD (shared L2 lines of fbuf/gbuf).
+constexpr int kFftSlots = 2;
+
+__device__ __forceinline__ int xcd_item(int b, int n)
+{
+    const int x = b & 7, t = b >> 3;
+    const int it = (t >> 2) * 32 + x * 4 + (t & 3);
+    return it < n ? it : -1;
+}
+
 __global__ __launch_bounds__(kThreads) void kb_synth_fft(const BField *__restrict__ fields,
-                                                         const int2 *__restrict__ items, BTables T,
+                                                         const int2 *__restrict__ items, int nitems, BTables T,
                                                          const double4 *__restrict__ fbuf, double *const *grid)
 {
     extern __shared__ double2 lds[];
-    const int2 it = items[blockIdx.x];
+    const int id = xcd_item(blockIdx.x, nitems);
+    if (id < 0) return;
+    const int2 it = items[id];
     const BField f = fields[it.x];
     const FftDev F = T.plans[f.fft];
-    synth_fft_tile(F, fbuf + f.foff, f.P, f.L, it.y, T.rows[f.geo + it.y], f.nlat, grid[it.x], f.nlong, 0, lds);
+    for (int s = it.y; s < it.y + kFftSlots && s < f.P; ++s) {
+        synth_fft_tile(F, fbuf + f.foff, f.P, f.L, s, T.u[f.geo + s], T.rows[f.geo + s], f.nlat, grid[it.x],
+                       f.nlong, 0, lds);
+        __syncthreads();
+    }
 }
 
 __global__ __launch_bounds__(kThreads) void kb_anal_fft(const BField *__restrict__ fields,
-                                                        const int2 *__restrict__ items, BTables T,
+                                                        const int2 *__restrict__ items, int nitems, BTables T,
                                                         const double *const *grid, double4 *__restrict__ gbuf)
 {
     extern __shared__ double2 lds[];
-    const int2 it = items[blockIdx.x];
+    const int id = xcd_item(blockIdx.x, nitems);
+    if (id < 0) return;
+    const int2 it = items[id];
     const BField f = fields[it.x];
     const FftDev F = T.plans[f.fft];
-    anal_fft_tile(F, grid[it.x], f.nlong, T.rows[f.geo + it.y], f.nlat, T.w[f.geo + it.y], f.P, f.L, it.y,
-                  gbuf + f.foff, lds);
+    for (int s = it.y; s < it.y + kFftSlots && s < f.P; ++s) {
+        anal_fft_tile(F, grid[it.x], f.nlong, T.rows[f.geo + s], f.nlat, T.w[f.geo + s], T.u[f.geo + s], f.P, f.L,
+                      s, gbuf + f.foff, lds);
+        __syncthreads();
+    }
 }
 
-// items: (field, m)
-__global__ __launch_bounds__(kThreads) void kb_anal_leg(const BField *__restrict__ fields,
-                                                        const int2 *__restrict__ items, BTables T,
-                                                        const double4 *__restrict__ gbuf,
-                                                        double2 *__restrict__ cbuf)
+// items: (field, m); single-wave workgroups.  The DH pole slot (0) carries
+// no analysis weight and is skipped.
+__global__ __launch_bounds__(kWave) void kb_anal_leg(const BField *__restrict__ fields,
+                                                     const int2 *__restrict__ items, BTables T,
+                                                     const double4 *__restrict__ gbuf, double2 *__restrict__ cbuf)
 {
-    __shared__ double red[2 * kAnalLC * kThreads];
+    __shared__ double red[2 * kAnalLC * kRedStride];
     const int2 it = items[blockIdx.x];
     const BField f = fields[it.x];
     const int m = it.y;
-    anal_leg_tile(T.alpha + tri_offd(m, T.Lt), T.seed[m], T.z + f.geo, T.u + f.geo,
-                  gbuf + f.foff + (size_t)m * f.P, f.L - m, m, f.P, cbuf + f.coff + tri_offd(m, f.L), red);
+    anal_leg_tile(T.alpha + tri_offd(m, T.Lt), T.seed[m], T.z + f.geo, gbuf + f.foff + (size_t)m * f.P, f.L - m,
+                  1, f.P, cbuf + f.coff + tri_offd(m, f.L), red);
 }
 
 // items: (field, l): writes row l of C and S of the field's [2][L+1][L+1].

@@ -106,8 +106,13 @@ template <int R, bool DIT>
 __device__ void fft_stage(double2 *x, int m, int S, const double2 *__restrict__ tw)
 {
     const int Lsub = R * S, step = m / Lsub, nb = m / R;
+    const float invS = 1.0f / (float)S;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-        const int grp = b / S, i = b - grp * S, base = grp * Lsub + i;
+        // grp = b / S without an integer division (exact after correction)
+        int grp = (int)((float)b * invS);
+        if (grp * S > b) --grp;
+        if ((grp + 1) * S <= b) ++grp;
+        const int i = b - grp * S, base = grp * Lsub + i;
         double2 v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) v[r] = x[base + r * S];

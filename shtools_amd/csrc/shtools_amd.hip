@@ -60,6 +60,7 @@ struct Geometry {
     int P = 0;          // ring-pair slots
     int nlat = 0;       // rows of the (non-extended) grid
     int nlong = 0;      // FFT length
+    int s0 = 0;         // first slot with analysis weight (DH: 1, the pole carries none)
     double *z = nullptr, *u = nullptr;
     double2 *w = nullptr;  // analysis weights (north, south)
     int2 *rows = nullptr;
@@ -163,6 +164,7 @@ static Geometry *get_dh(int dev, int L, int sampling)
     g.nlat = n;
     g.nlong = sampling * n;
     g.P = n / 2 + 1;
+    g.s0 = 1;
     std::vector<double> z(g.P), u(g.P), w(2 * g.P);
     std::vector<int> rows(2 * g.P);
     std::vector<double> aj;
@@ -229,6 +231,14 @@ static int current_device()
     return d;
 }
 
+// ring pairs per lane of a synthesis wave for P slots: 1, 2 or 4
+static int synth_lanes_per_slot(int P)
+{
+    if (P <= kWave) return 1;
+    if (P <= 2 * kWave) return 2;
+    return kSynthKMax;
+}
+
 static int grid_blocks(long long work)
 {
     long long b = (work + kThreads - 1) / kThreads;
@@ -288,14 +298,15 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
     }
     {
         ProfScope ps(SHTG_K_SYNTH_LEG, st);
-        const int nrb = (g.P + kSynthK * kThreads - 1) / (kSynthK * kThreads);
-        hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nrb), dim3(kThreads), 0, st, rt.alpha, rt.L, sd.seed, g.z,
-                           g.u, cpack, Lc, g.P, nrb, fbuf);
+        const int kk = synth_lanes_per_slot(g.P);
+        const int nsb = (g.P + kk * kWave - 1) / (kk * kWave);
+        hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nsb), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, cpack,
+                           Lc, g.P, nsb, kk, fbuf);
     }
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
         const size_t lds = (size_t)g.fft.m * sizeof(double2);
-        hipLaunchKernelGGL(k_synth_fft, dim3(g.P), dim3(kThreads), lds, st, g.fft, fbuf, g.P, Lc, g.rows,
+        hipLaunchKernelGGL(k_synth_fft, dim3(g.P), dim3(kThreads), lds, st, g.fft, fbuf, g.P, Lc, g.u, g.rows,
                            nrows_out, grid, ld, ext_col);
     }
     if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
@@ -317,13 +328,13 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
     {
         ProfScope ps(SHTG_K_ANAL_FFT, st);
         const size_t lds = (size_t)g.fft.m * sizeof(double2);
-        hipLaunchKernelGGL(k_anal_fft, dim3(g.P), dim3(kThreads), lds, st, g.fft, grid, ld, g.rows, g.nlat, g.w,
-                           g.P, Lc, gbuf);
+        hipLaunchKernelGGL(k_anal_fft, dim3(g.P), dim3(kThreads), lds, st, g.fft, grid, ld, g.u, g.rows, g.nlat,
+                           g.w, g.P, Lc, gbuf);
     }
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);
-        hipLaunchKernelGGL(k_anal_leg, dim3(Lc + 1), dim3(kThreads), 0, st, rt.alpha, rt.L, sd.seed, g.z, g.u, gbuf,
-                           Lc, g.P, cbuf);
+        hipLaunchKernelGGL(k_anal_leg, dim3(Lc + 1), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gbuf, Lc,
+                           g.s0, g.P, cbuf);
     }
     const long long total = 2LL * d1 * d2;
     ProfScope ps_un(SHTG_K_UNPACK, st);
@@ -497,9 +508,9 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
         const BField &f = fields[i];
         for (int m = 0; m <= f.L; ++m) {
             wp.push_back({(double)(f.L - m + 1), i, m, 0});
-            const int span = kSynthK * kThreads;
+            const int kk = synth_lanes_per_slot(f.P), span = kk * kWave;
             for (int s0 = 0; s0 < f.P; s0 += span)
-                ws.push_back({(double)(f.L - m + 1) * std::min(span, f.P - s0), i, m, s0});
+                ws.push_back({(double)(f.L - m + 1) * std::min(span, f.P - s0), i, m, s0 + (kk << 24)});
             wa.push_back({(double)(f.L - m + 1) * f.P, i, m, 0});
         }
         for (int l = 0; l <= f.L; ++l) wu.push_back({(double)f.L, i, l, 0});
@@ -511,7 +522,7 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
     std::vector<int2> ip, ia, iu;
     std::vector<int4> is;
     for (auto &x : wp) ip.push_back(make_int2(x.a, x.b));
-    for (auto &x : ws) is.push_back(make_int4(x.a, x.b, x.c, 0));
+    for (auto &x : ws) is.push_back(make_int4(x.a, x.b, x.c & 0xFFFFFF, x.c >> 24));
     for (auto &x : wa) ia.push_back(make_int2(x.a, x.b));
     for (auto &x : wu) iu.push_back(make_int2(x.a, x.b));
     b->items_pack = dev_upload(ip);
@@ -532,7 +543,7 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
         size_t cls = 16 * 1024;
         while (cls < need) cls *= 2;
         if (cls > kMaxLds) cls = kMaxLds;
-        for (int s = 0; s < f.P; ++s) groups[cls].push_back({(double)need, i, s, 0});
+        for (int s = 0; s < f.P; s += kFftSlots) groups[cls].push_back({(double)need, i, s, 0});
     }
     for (auto &kv : groups) {
         std::stable_sort(kv.second.begin(), kv.second.end(), by_work);
@@ -576,6 +587,9 @@ static int batch_bind(BatchPlan *b, int k, const void *const *p)
     return ST_OK;
 }
 
+// workgroups covering n items under xcd_item's mapping (multiple of 32)
+static int xcd_blocks(int n) { return ((n + 31) / 32) * 32; }
+
 static int batch_synthesis(BatchPlan *b, hipStream_t st)
 {
     const double *const *cin = b->ptrs;
@@ -587,14 +601,14 @@ static int batch_synthesis(BatchPlan *b, hipStream_t st)
     }
     {
         ProfScope ps(SHTG_K_SYNTH_LEG, st);
-        hipLaunchKernelGGL(kb_synth_leg, dim3(b->n_sleg), dim3(kThreads), 0, st, b->fields, b->items_sleg, b->T,
+        hipLaunchKernelGGL(kb_synth_leg, dim3(b->n_sleg), dim3(kWave), 0, st, b->fields, b->items_sleg, b->T,
                            b->cbuf, b->fbuf);
     }
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
         for (auto &g : b->fft)
-            hipLaunchKernelGGL(kb_synth_fft, dim3(g.nitems), dim3(kThreads), g.lds, st, b->fields, g.items, b->T,
-                               b->fbuf, gout);
+            hipLaunchKernelGGL(kb_synth_fft, dim3(xcd_blocks(g.nitems)), dim3(kThreads), g.lds, st, b->fields,
+                               g.items, g.nitems, b->T, b->fbuf, gout);
     }
     return HIP_OK(hipGetLastError()) ? ST_OK : ST_OTHER;
 }
@@ -606,12 +620,12 @@ static int batch_analysis(BatchPlan *b, hipStream_t st)
     {
         ProfScope ps(SHTG_K_ANAL_FFT, st);
         for (auto &g : b->fft)
-            hipLaunchKernelGGL(kb_anal_fft, dim3(g.nitems), dim3(kThreads), g.lds, st, b->fields, g.items, b->T, gin,
-                               b->fbuf);
+            hipLaunchKernelGGL(kb_anal_fft, dim3(xcd_blocks(g.nitems)), dim3(kThreads), g.lds, st, b->fields,
+                               g.items, g.nitems, b->T, gin, b->fbuf);
     }
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);
-        hipLaunchKernelGGL(kb_anal_leg, dim3(b->n_aleg), dim3(kThreads), 0, st, b->fields, b->items_aleg, b->T,
+        hipLaunchKernelGGL(kb_anal_leg, dim3(b->n_aleg), dim3(kWave), 0, st, b->fields, b->items_aleg, b->T,
                            b->fbuf, b->cbuf);
     }
     {

@@ -2,7 +2,7 @@
 //
 // Every ring (pair) transform is a complex DFT of length n, forward
 // X_k = sum_j x_j e^{-2 pi i jk/n}; inverse transforms are taken as
-// conj(F(conj(.))).  When n factors into radices {4, 2, 3, 5, 7, 11, 13} the
+// conj(F(conj(.))).  When n factors into radices {8, 4, 2, 3, 5, 7} the
 // kernel runs in-place decimation-in-frequency stages on the length-n array
 // and reads results through a digit-reversal table.  Otherwise the DFT is
 // computed by Bluestein's chirp-z method on a 5-smooth work length m >= 2n-1:
@@ -34,8 +34,10 @@ inline bool factor_smooth(int m, std::vector<int> &rad)
 {
     rad.clear();
     int r = m;
-    while (r % 4 == 0) { rad.push_back(4); r /= 4; }
-    static const int primes[] = {2, 3, 5, 7, 11, 13};
+    while (r % 8 == 0) { rad.push_back(8); r /= 8; }
+    if (r % 4 == 0) { rad.push_back(4); r /= 4; }
+    if (r % 2 == 0) { rad.push_back(2); r /= 2; }
+    static const int primes[] = {3, 5, 7};
     for (int p : primes)
         while (r % p == 0) { rad.push_back(p); r /= p; }
     return r == 1;

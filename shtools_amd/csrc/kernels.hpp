@@ -33,10 +33,11 @@
 
 namespace shtg {
 
-constexpr int kThreads = 256;  // ring-FFT and generic workgroups
+constexpr int kThreads = 256;  // generic workgroups
+constexpr int kFftMaxThreads = 512;  // ring-FFT workgroups: 256 or 512 threads
 constexpr int kWave = 64;      // Legendre workgroups are single waves
 constexpr int kSynthKMax = 4;  // ring pairs per lane in synthesis (1, 2 or 4)
-constexpr int kAnalK = 8;      // ring pairs per lane and pass in analysis
+constexpr int kAnalKMax = 8;   // ring pairs per lane and pass in analysis (1, 2, 4 or 8)
 constexpr int kAnalLC = 8;     // degrees per reduction chunk in analysis
 constexpr int kRedStride = 65; // LDS row stride (doubles) of the analysis reduction
 
@@ -122,14 +123,15 @@ __device__ __forceinline__ void synth_leg_dispatch(int kk, const double *__restr
 // Analysis sums of order m over ring pairs [s0, P) (one wave):
 // out[j] = sum over slots of q_{m+j}(z) * (E or O by parity of j), as (C, S)
 // pairs.  grow holds the order's (E_C, E_S, O_C, O_S) already multiplied by
-// the order's rescale factor.  Passes of kAnalK * 64 slots; the first pass
+// the order's rescale factor.  Passes of K * 64 slots; the first pass
 // stores, later passes accumulate.  Each chunk of kAnalLC degrees is reduced
 // over the wave through red (16 x kRedStride doubles of LDS).
-__device__ void anal_leg_tile(const double *__restrict__ al, double sd, const double *__restrict__ ringz,
-                              const double4 *__restrict__ grow, int jmax, int s0, int P,
-                              double2 *__restrict__ out, double *red)
+template <int K>
+__device__ __forceinline__ void anal_leg_tile(const double *__restrict__ al, double sd,
+                                              const double *__restrict__ ringz, const double4 *__restrict__ grow,
+                                              int jmax, int s0, int P, double2 *__restrict__ out, double *red)
 {
-    constexpr int K = kAnalK, LC = kAnalLC;
+    constexpr int LC = kAnalLC;
     const int lane = threadIdx.x & (kWave - 1);
     for (int p0 = s0; p0 < P; p0 += K * kWave) {
         double z[K], q1[K], q2[K], eC[K], eS[K], oC[K], oS[K];
@@ -199,7 +201,7 @@ __device__ void anal_leg_tile(const double *__restrict__ al, double sd, const do
 
 // Ring-pair synthesis FFT for slot s: fbuf holds rows of order m at stride P
 // (unscaled sums); u = sin(theta) of the slot gives the orders' rescale.
-__device__ void synth_fft_tile(const FftDev &F, const double4 *__restrict__ fbuf, int P, int Lc, int s, double u,
+__device__ __forceinline__ void synth_fft_tile(const FftDev &F, const double4 *__restrict__ fbuf, int P, int Lc, int s, double u,
                                int2 rw, int nrows, double *__restrict__ grid, long long ld, int ext_col,
                                double2 *lds)
 {
@@ -244,7 +246,7 @@ __device__ void synth_fft_tile(const FftDev &F, const double4 *__restrict__ fbuf
 // Ring-pair analysis FFT for slot s; w = (w_north, w_south) include the
 // quadrature weight and the reference's constant factors; the outputs carry
 // the order's rescale factor for the ring (sin(theta) = u).
-__device__ void anal_fft_tile(const FftDev &F, const double *__restrict__ grid, long long ld, int2 rw, int nrows,
+__device__ __forceinline__ void anal_fft_tile(const FftDev &F, const double *__restrict__ grid, long long ld, int2 rw, int nrows,
                               double2 w, double u, int P, int Lc, int s, double4 *__restrict__ gbuf,
                               double2 *lds)
 {
@@ -302,7 +304,7 @@ __global__ __launch_bounds__(kWave) void k_synth_leg(const double *__restrict__ 
 }
 
 // rows[s] = (north row, south row or -1); rows >= nrows are skipped.
-__global__ __launch_bounds__(kThreads) void k_synth_fft(FftDev F, const double4 *__restrict__ fbuf, int P, int Lc,
+__global__ __launch_bounds__(kFftMaxThreads) void k_synth_fft(FftDev F, const double4 *__restrict__ fbuf, int P, int Lc,
                                                         const double *__restrict__ ringu,
                                                         const int2 *__restrict__ rows, int nrows,
                                                         double *__restrict__ grid, long long ld, int ext_col)
@@ -312,7 +314,7 @@ __global__ __launch_bounds__(kThreads) void k_synth_fft(FftDev F, const double4 
     synth_fft_tile(F, fbuf, P, Lc, s, ringu[s], rows[s], nrows, grid, ld, ext_col, lds);
 }
 
-__global__ __launch_bounds__(kThreads) void k_anal_fft(FftDev F, const double *__restrict__ grid, long long ld,
+__global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft(FftDev F, const double *__restrict__ grid, long long ld,
                                                        const double *__restrict__ ringu,
                                                        const int2 *__restrict__ rows, int nrows,
                                                        const double2 *__restrict__ wts, int P, int Lc,
@@ -324,6 +326,7 @@ __global__ __launch_bounds__(kThreads) void k_anal_fft(FftDev F, const double *_
 }
 
 // one single-wave workgroup per order m over ring pairs [s0, P)
+template <int K>
 __global__ __launch_bounds__(kWave) void k_anal_leg(const double *__restrict__ alpha, int Lt,
                                                     const double *__restrict__ seed,
                                                     const double *__restrict__ ringz,
@@ -332,8 +335,8 @@ __global__ __launch_bounds__(kWave) void k_anal_leg(const double *__restrict__ a
 {
     __shared__ double red[2 * kAnalLC * kRedStride];
     const int m = blockIdx.x;
-    anal_leg_tile(alpha + tri_offd(m, Lt), seed[m], ringz, gbuf + (size_t)m * P, Lc - m, s0, P,
-                  cbuf + tri_offd(m, Lc), red);
+    anal_leg_tile<K>(alpha + tri_offd(m, Lt), seed[m], ringz, gbuf + (size_t)m * P, Lc - m, s0, P,
+                     cbuf + tri_offd(m, Lc), red);
 }
 
 // cilm[i][l][m] = cbuf[tri(m) + l - m].i * post(l, m) for m <= l <= Lc,
@@ -423,7 +426,7 @@ __device__ __forceinline__ int xcd_item(int b, int n)
     return it < n ? it : -1;
 }
 
-__global__ __launch_bounds__(kThreads) void kb_synth_fft(const BField *__restrict__ fields,
+__global__ __launch_bounds__(kFftMaxThreads) void kb_synth_fft(const BField *__restrict__ fields,
                                                          const int2 *__restrict__ items, int nitems, BTables T,
                                                          const double4 *__restrict__ fbuf, double *const *grid)
 {
@@ -432,7 +435,7 @@ __global__ __launch_bounds__(kThreads) void kb_synth_fft(const BField *__restric
     if (id < 0) return;
     const int2 it = items[id];
     const BField f = fields[it.x];
-    const FftDev F = T.plans[f.fft];
+    const FftDev &F = T.plans[f.fft];
     for (int s = it.y; s < it.y + kFftSlots && s < f.P; ++s) {
         synth_fft_tile(F, fbuf + f.foff, f.P, f.L, s, T.u[f.geo + s], T.rows[f.geo + s], f.nlat, grid[it.x],
                        f.nlong, 0, lds);
@@ -440,7 +443,7 @@ __global__ __launch_bounds__(kThreads) void kb_synth_fft(const BField *__restric
     }
 }
 
-__global__ __launch_bounds__(kThreads) void kb_anal_fft(const BField *__restrict__ fields,
+__global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft(const BField *__restrict__ fields,
                                                         const int2 *__restrict__ items, int nitems, BTables T,
                                                         const double *const *grid, double4 *__restrict__ gbuf)
 {
@@ -449,7 +452,7 @@ __global__ __launch_bounds__(kThreads) void kb_anal_fft(const BField *__restrict
     if (id < 0) return;
     const int2 it = items[id];
     const BField f = fields[it.x];
-    const FftDev F = T.plans[f.fft];
+    const FftDev &F = T.plans[f.fft];
     for (int s = it.y; s < it.y + kFftSlots && s < f.P; ++s) {
         anal_fft_tile(F, grid[it.x], f.nlong, T.rows[f.geo + s], f.nlat, T.w[f.geo + s], T.u[f.geo + s], f.P, f.L,
                       s, gbuf + f.foff, lds);
@@ -457,8 +460,9 @@ __global__ __launch_bounds__(kThreads) void kb_anal_fft(const BField *__restrict
     }
 }
 
-// items: (field, m); single-wave workgroups.  The DH pole slot (0) carries
-// no analysis weight and is skipped.
+// items: (field, m); single-wave workgroups, one launch per ring pairs per
+// lane K.  The DH pole slot (0) carries no analysis weight and is skipped.
+template <int K>
 __global__ __launch_bounds__(kWave) void kb_anal_leg(const BField *__restrict__ fields,
                                                      const int2 *__restrict__ items, BTables T,
                                                      const double4 *__restrict__ gbuf, double2 *__restrict__ cbuf)
@@ -467,8 +471,8 @@ __global__ __launch_bounds__(kWave) void kb_anal_leg(const BField *__restrict__ 
     const int2 it = items[blockIdx.x];
     const BField f = fields[it.x];
     const int m = it.y;
-    anal_leg_tile(T.alpha + tri_offd(m, T.Lt), T.seed[m], T.z + f.geo, gbuf + f.foff + (size_t)m * f.P, f.L - m,
-                  1, f.P, cbuf + f.coff + tri_offd(m, f.L), red);
+    anal_leg_tile<K>(T.alpha + tri_offd(m, T.Lt), T.seed[m], T.z + f.geo, gbuf + f.foff + (size_t)m * f.P,
+                     f.L - m, 1, f.P, cbuf + f.coff + tri_offd(m, f.L), red);
 }
 
 // items: (field, l): writes row l of C and S of the field's [2][L+1][L+1].

@@ -64,47 +64,80 @@ struct FftDev {
     const double2 *bhat;   // [m]
 };
 
-// y_k = sum_j v_j e^{-2 pi i jk/R}
+// Fully unrolled forward DFTs, y_k = sum_j v_j e^{-2 pi i jk/R}, in place.
+__device__ __forceinline__ void dft2(double2 &a, double2 &b)
+{
+    const double2 t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+__device__ __forceinline__ void dft4(double2 &v0, double2 &v1, double2 &v2, double2 &v3)
+{
+    const double2 a = cadd(v0, v2), b = csub(v0, v2);
+    const double2 c = cadd(v1, v3), d = csub(v1, v3);
+    v0 = cadd(a, c);
+    v2 = csub(a, c);
+    v1 = make_double2(b.x + d.y, b.y - d.x);  // b - i d
+    v3 = make_double2(b.x - d.y, b.y + d.x);  // b + i d
+}
+
 template <int R>
-__device__ __forceinline__ void dft_small(double2 (&v)[R], const double2 *__restrict__ tw, int m)
+__device__ __forceinline__ void dft_unrolled(double2 (&v)[R])
 {
     if constexpr (R == 2) {
-        double2 a = v[0], b = v[1];
-        v[0] = cadd(a, b);
-        v[1] = csub(a, b);
+        dft2(v[0], v[1]);
+    } else if constexpr (R == 3) {
+        constexpr double c = -0.5, s = -0.86602540378443864676;  // e^{-2 pi i/3}
+        const double2 t = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+        const double2 m = make_double2(v[0].x + c * t.x, v[0].y + c * t.y);
+        v[0] = cadd(v[0], t);
+        v[1] = make_double2(m.x - s * d.y, m.y + s * d.x);
+        v[2] = make_double2(m.x + s * d.y, m.y - s * d.x);
     } else if constexpr (R == 4) {
-        double2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
-        double2 c = cadd(v[1], v[3]), d = csub(v[1], v[3]);
-        // d * (-i) = (d.y, -d.x)
-        v[0] = cadd(a, c);
-        v[2] = csub(a, c);
-        v[1] = make_double2(b.x + d.y, b.y - d.x);
-        v[3] = make_double2(b.x - d.y, b.y + d.x);
-    } else {
-        double2 w[R];
-        const int st = m / R;
-#pragma unroll
-        for (int q = 0; q < R; ++q) w[q] = tw[q * st];
-        double2 y[R];
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-            double2 acc = v[0];
-#pragma unroll
-            for (int j = 1; j < R; ++j) {
-                double2 c = w[(j * k) % R];
-                acc.x += v[j].x * c.x - v[j].y * c.y;
-                acc.y += v[j].x * c.y + v[j].y * c.x;
-            }
-            y[k] = acc;
-        }
-#pragma unroll
-        for (int k = 0; k < R; ++k) v[k] = y[k];
+        dft4(v[0], v[1], v[2], v[3]);
+    } else if constexpr (R == 5) {
+        constexpr double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+        constexpr double s1 = -0.95105651629515357212, s2 = -0.58778525229247312917;
+        const double2 t1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]);
+        const double2 t2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
+        const double2 m1 = make_double2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+        const double2 m2 = make_double2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+        // i * (s1 d1 + s2 d2) and i * (s2 d1 - s1 d2)
+        const double2 n1 = make_double2(-(s1 * d1.y + s2 * d2.y), s1 * d1.x + s2 * d2.x);
+        const double2 n2 = make_double2(-(s2 * d1.y - s1 * d2.y), s2 * d1.x - s1 * d2.x);
+        v[0] = cadd(v[0], cadd(t1, t2));
+        v[1] = cadd(m1, n1);
+        v[4] = csub(m1, n1);
+        v[2] = cadd(m2, n2);
+        v[3] = csub(m2, n2);
+    } else if constexpr (R == 8) {
+        constexpr double h = 0.70710678118654752440;
+        double2 b0 = v[0], b1 = v[1], b2 = v[2], b3 = v[3];
+        double2 c0 = v[4], c1 = v[5], c2 = v[6], c3 = v[7];
+        dft2(b0, c0);
+        dft2(b1, c1);
+        dft2(b2, c2);
+        dft2(b3, c3);
+        // c_j *= e^{-i pi j/4}
+        c1 = make_double2(h * (c1.x + c1.y), h * (c1.y - c1.x));
+        c2 = make_double2(c2.y, -c2.x);
+        c3 = make_double2(h * (c3.y - c3.x), -h * (c3.x + c3.y));
+        dft4(b0, b1, b2, b3);
+        dft4(c0, c1, c2, c3);
+        v[0] = b0; v[2] = b1; v[4] = b2; v[6] = b3;
+        v[1] = c0; v[3] = c1; v[5] = c2; v[7] = c3;
     }
 }
 
+// One in-place stage of radix R over x[0..m): butterflies over elements
+// base + r*S; DIF applies twiddles after the butterfly, DIT (the transposed
+// stage) before it.  Radices 2, 3, 4, 5 and 8 are unrolled; other primes use
+// a rolled loop over outputs that stores each output as it is formed.
 template <int R, bool DIT>
-__device__ void fft_stage(double2 *x, int m, int S, const double2 *__restrict__ tw)
+__device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double2 *__restrict__ tw)
 {
+    constexpr bool kUnrolled = (R == 2 || R == 3 || R == 4 || R == 5 || R == 8);
     const int Lsub = R * S, step = m / Lsub, nb = m / R;
     const float invS = 1.0f / (float)S;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -119,19 +152,39 @@ __device__ void fft_stage(double2 *x, int m, int S, const double2 *__restrict__ 
         if constexpr (DIT) {
 #pragma unroll
             for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[i * r * step]);
-            dft_small<R>(v, tw, m);
-        } else {
-            dft_small<R>(v, tw, m);
-#pragma unroll
-            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[i * r * step]);
         }
+        if constexpr (kUnrolled) {
+            dft_unrolled<R>(v);
+            if constexpr (!DIT) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) x[base + r * S] = v[r];
+                for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[i * r * step]);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) x[base + r * S] = v[r];
+        } else {
+            const int st = m / R;
+            for (int k = 0; k < R; ++k) {
+                double2 acc = v[0];
+                int jk = 0;
+#pragma unroll
+                for (int j = 1; j < R; ++j) {
+                    jk += k;
+                    if (jk >= R) jk -= R;
+                    const double2 c = tw[jk * st];
+                    acc.x += v[j].x * c.x - v[j].y * c.y;
+                    acc.y += v[j].x * c.y + v[j].y * c.x;
+                }
+                if constexpr (!DIT) {
+                    if (k > 0) acc = cmul(acc, tw[i * k * step]);
+                }
+                x[base + k * S] = acc;
+            }
+        }
     }
 }
 
 template <bool DIT>
-__device__ void fft_run_stage(double2 *x, const FftDev &P, int st)
+__device__ __forceinline__ void fft_run_stage(double2 *x, const FftDev &P, int st)
 {
     const int S = P.stride[st];
     switch (P.radix[st]) {
@@ -139,9 +192,8 @@ __device__ void fft_run_stage(double2 *x, const FftDev &P, int st)
     case 3: fft_stage<3, DIT>(x, P.m, S, P.tw); break;
     case 4: fft_stage<4, DIT>(x, P.m, S, P.tw); break;
     case 5: fft_stage<5, DIT>(x, P.m, S, P.tw); break;
+    case 8: fft_stage<8, DIT>(x, P.m, S, P.tw); break;
     case 7: fft_stage<7, DIT>(x, P.m, S, P.tw); break;
-    case 11: fft_stage<11, DIT>(x, P.m, S, P.tw); break;
-    case 13: fft_stage<13, DIT>(x, P.m, S, P.tw); break;
     default: break;
     }
 }
@@ -154,7 +206,7 @@ __device__ __forceinline__ double2 fft_prep_in(const FftDev &P, int j, double2 v
 
 // In-place forward DFT of the n values in x[0..n) (x has P.m slots; for
 // Bluestein the caller has zeroed x[n..m)).  Ends with a barrier.
-__device__ void fft_forward(double2 *x, const FftDev &P)
+__device__ __forceinline__ void fft_forward(double2 *x, const FftDev &P)
 {
     for (int st = 0; st < P.nst; ++st) {
         fft_run_stage<false>(x, P, st);

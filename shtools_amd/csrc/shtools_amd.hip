@@ -231,6 +231,20 @@ static int current_device()
     return d;
 }
 
+// ring pairs per lane and pass of an analysis wave for P slots: 1, 2, 4, 8
+static int anal_lanes_per_slot(int P)
+{
+    int k = 1;
+    while (k < kAnalKMax && k * kWave < P) k *= 2;
+    return k;
+}
+
+// threads of a ring-FFT workgroup for a work length of m points
+static int fft_threads(int m)
+{
+    return m <= 1024 ? 256 : kFftMaxThreads;
+}
+
 // ring pairs per lane of a synthesis wave for P slots: 1, 2 or 4
 static int synth_lanes_per_slot(int P)
 {
@@ -306,7 +320,7 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
         const size_t lds = (size_t)g.fft.m * sizeof(double2);
-        hipLaunchKernelGGL(k_synth_fft, dim3(g.P), dim3(kThreads), lds, st, g.fft, fbuf, g.P, Lc, g.u, g.rows,
+        hipLaunchKernelGGL(k_synth_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, fbuf, g.P, Lc, g.u, g.rows,
                            nrows_out, grid, ld, ext_col);
     }
     if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
@@ -328,13 +342,15 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
     {
         ProfScope ps(SHTG_K_ANAL_FFT, st);
         const size_t lds = (size_t)g.fft.m * sizeof(double2);
-        hipLaunchKernelGGL(k_anal_fft, dim3(g.P), dim3(kThreads), lds, st, g.fft, grid, ld, g.u, g.rows, g.nlat,
+        hipLaunchKernelGGL(k_anal_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, grid, ld, g.u, g.rows, g.nlat,
                            g.w, g.P, Lc, gbuf);
     }
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);
-        hipLaunchKernelGGL(k_anal_leg, dim3(Lc + 1), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gbuf, Lc,
-                           g.s0, g.P, cbuf);
+        const int K = anal_lanes_per_slot(g.P - g.s0);
+        auto kern = K == 1 ? k_anal_leg<1> : K == 2 ? k_anal_leg<2> : K == 4 ? k_anal_leg<4> : k_anal_leg<kAnalKMax>;
+        hipLaunchKernelGGL(kern, dim3(Lc + 1), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gbuf, Lc, g.s0,
+                           g.P, cbuf);
     }
     const long long total = 2LL * d1 * d2;
     ProfScope ps_un(SHTG_K_UNPACK, st);
@@ -355,6 +371,7 @@ struct FftGroup {
     int2 *items = nullptr;
     int nitems = 0;
     size_t lds = 0;
+    int threads = 256;
 };
 
 struct BatchPlan {
@@ -365,8 +382,8 @@ struct BatchPlan {
     int n_pack = 0;
     int4 *items_sleg = nullptr;
     int n_sleg = 0;
-    int2 *items_aleg = nullptr;
-    int n_aleg = 0;
+    int2 *items_aleg[4] = {nullptr, nullptr, nullptr, nullptr};  // by K = 8, 4, 2, 1
+    int n_aleg[4] = {0, 0, 0, 0};
     int2 *items_unpack = nullptr;
     int n_unpack = 0;
     std::vector<FftGroup> fft;
@@ -395,7 +412,7 @@ static void batch_free(BatchPlan *b)
     (void)hipFree(b->fields);
     (void)hipFree(b->items_pack);
     (void)hipFree(b->items_sleg);
-    (void)hipFree(b->items_aleg);
+    for (auto p : b->items_aleg) (void)hipFree(p);
     (void)hipFree(b->items_unpack);
     for (auto &g : b->fft) (void)hipFree(g.items);
     (void)hipFree(b->z);
@@ -511,7 +528,7 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
             const int kk = synth_lanes_per_slot(f.P), span = kk * kWave;
             for (int s0 = 0; s0 < f.P; s0 += span)
                 ws.push_back({(double)(f.L - m + 1) * std::min(span, f.P - s0), i, m, s0 + (kk << 24)});
-            wa.push_back({(double)(f.L - m + 1) * f.P, i, m, 0});
+            wa.push_back({(double)(f.L - m + 1) * f.P, i, m, anal_lanes_per_slot(f.P - 1)});
         }
         for (int l = 0; l <= f.L; ++l) wu.push_back({(double)f.L, i, l, 0});
     }
@@ -519,21 +536,23 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
     std::stable_sort(wp.begin(), wp.end(), by_work);
     std::stable_sort(ws.begin(), ws.end(), by_work);
     std::stable_sort(wa.begin(), wa.end(), by_work);
-    std::vector<int2> ip, ia, iu;
+    std::vector<int2> ip, iu, ia[4];
     std::vector<int4> is;
     for (auto &x : wp) ip.push_back(make_int2(x.a, x.b));
     for (auto &x : ws) is.push_back(make_int4(x.a, x.b, x.c & 0xFFFFFF, x.c >> 24));
-    for (auto &x : wa) ia.push_back(make_int2(x.a, x.b));
+    for (auto &x : wa) ia[x.c == 8 ? 0 : x.c == 4 ? 1 : x.c == 2 ? 2 : 3].push_back(make_int2(x.a, x.b));
     for (auto &x : wu) iu.push_back(make_int2(x.a, x.b));
     b->items_pack = dev_upload(ip);
     b->items_sleg = dev_upload(is);
-    b->items_aleg = dev_upload(ia);
+    for (int k = 0; k < 4; ++k) {
+        b->n_aleg[k] = (int)ia[k].size();
+        if (b->n_aleg[k] && !(b->items_aleg[k] = dev_upload(ia[k]))) return ST_ALLOC;
+    }
     b->items_unpack = dev_upload(iu);
     b->n_pack = (int)ip.size();
     b->n_sleg = (int)is.size();
-    b->n_aleg = (int)ia.size();
     b->n_unpack = (int)iu.size();
-    if (!b->items_pack || !b->items_sleg || !b->items_aleg || !b->items_unpack) return ST_ALLOC;
+    if (!b->items_pack || !b->items_sleg || !b->items_unpack) return ST_ALLOC;
 
     // FFT items grouped by LDS size class (larger first)
     std::map<size_t, std::vector<W>, std::greater<size_t>> groups;
@@ -553,6 +572,7 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
         g.items = dev_upload(it);
         g.nitems = (int)it.size();
         g.lds = kv.first;
+        g.threads = fft_threads((int)(kv.first / sizeof(double2)));
         if (!g.items) return ST_ALLOC;
         b->fft.push_back(g);
     }
@@ -607,7 +627,7 @@ static int batch_synthesis(BatchPlan *b, hipStream_t st)
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
         for (auto &g : b->fft)
-            hipLaunchKernelGGL(kb_synth_fft, dim3(xcd_blocks(g.nitems)), dim3(kThreads), g.lds, st, b->fields,
+            hipLaunchKernelGGL(kb_synth_fft, dim3(xcd_blocks(g.nitems)), dim3(g.threads), g.lds, st, b->fields,
                                g.items, g.nitems, b->T, b->fbuf, gout);
     }
     return HIP_OK(hipGetLastError()) ? ST_OK : ST_OTHER;
@@ -620,13 +640,23 @@ static int batch_analysis(BatchPlan *b, hipStream_t st)
     {
         ProfScope ps(SHTG_K_ANAL_FFT, st);
         for (auto &g : b->fft)
-            hipLaunchKernelGGL(kb_anal_fft, dim3(xcd_blocks(g.nitems)), dim3(kThreads), g.lds, st, b->fields,
+            hipLaunchKernelGGL(kb_anal_fft, dim3(xcd_blocks(g.nitems)), dim3(g.threads), g.lds, st, b->fields,
                                g.items, g.nitems, b->T, gin, b->fbuf);
     }
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);
-        hipLaunchKernelGGL(kb_anal_leg, dim3(b->n_aleg), dim3(kWave), 0, st, b->fields, b->items_aleg, b->T,
-                           b->fbuf, b->cbuf);
+        if (b->n_aleg[0])
+            hipLaunchKernelGGL(kb_anal_leg<8>, dim3(b->n_aleg[0]), dim3(kWave), 0, st, b->fields, b->items_aleg[0],
+                               b->T, b->fbuf, b->cbuf);
+        if (b->n_aleg[1])
+            hipLaunchKernelGGL(kb_anal_leg<4>, dim3(b->n_aleg[1]), dim3(kWave), 0, st, b->fields, b->items_aleg[1],
+                               b->T, b->fbuf, b->cbuf);
+        if (b->n_aleg[2])
+            hipLaunchKernelGGL(kb_anal_leg<2>, dim3(b->n_aleg[2]), dim3(kWave), 0, st, b->fields, b->items_aleg[2],
+                               b->T, b->fbuf, b->cbuf);
+        if (b->n_aleg[3])
+            hipLaunchKernelGGL(kb_anal_leg<1>, dim3(b->n_aleg[3]), dim3(kWave), 0, st, b->fields, b->items_aleg[3],
+                               b->T, b->fbuf, b->cbuf);
     }
     {
         ProfScope ps(SHTG_K_UNPACK, st);

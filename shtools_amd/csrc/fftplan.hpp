@@ -24,7 +24,9 @@ struct FftPlanHost {
     int n = 0, m = 0, nst = 0, blue = 0;
     int radix[kMaxStages] = {0};
     int stride[kMaxStages] = {0};
+    int twoff[kMaxStages] = {0};  // offset of each stage's twiddles in twst
     std::vector<double> tw;     // 2*m: e^{-2 pi i k/m}
+    std::vector<double> twst;   // per stage: e^{-2 pi i i / (R S)}, i < S
     std::vector<int> perm;      // m: DIF output position of natural index k
     std::vector<double> chirp;  // 2*n (Bluestein)
     std::vector<double> bhat;   // 2*m (Bluestein), DIF output order, /m
@@ -110,6 +112,17 @@ inline void set_stages(FftPlanHost &p, const std::vector<int> &rad)
         p.tw[2 * k] = (double)w.real();
         p.tw[2 * k + 1] = (double)w.imag();
     }
+    p.twst.clear();
+    for (int s = 0; s < p.nst; ++s) {
+        p.twoff[s] = (int)(p.twst.size() / 2);
+        const int Lsub = p.radix[s] * p.stride[s];
+        for (int i = 0; i < p.stride[s]; ++i) {
+            auto w = root_ld(i, Lsub);
+            p.twst.push_back((double)w.real());
+            p.twst.push_back((double)w.imag());
+        }
+    }
+    if (p.twst.empty()) p.twst.assign(2, 0.0);
 }
 
 inline FftPlanHost make_fft_plan(int n)

@@ -206,7 +206,7 @@ __device__ __forceinline__ void synth_fft_tile(const FftDev &F, const double4 *_
                                double2 *lds)
 {
     const int n = F.n;
-    for (int k = threadIdx.x; k < F.m; k += blockDim.x) lds[k] = make_double2(0.0, 0.0);
+    for (int k = threadIdx.x; k < F.m; k += blockDim.x) lds[fpad(k)] = make_double2(0.0, 0.0);
     __syncthreads();
     for (int k = threadIdx.x; k <= Lc; k += blockDim.x) {
         // Z_k = X_k + i Y_k with X (north) and Y (south) Hermitian:
@@ -224,8 +224,8 @@ __device__ __forceinline__ void synth_fft_tile(const FftDev &F, const double4 *_
         if (k == 0) {
             lds[0] = fft_prep_in(F, 0, make_double2(f.x, -f.z));
         } else {
-            lds[k] = fft_prep_in(F, k, make_double2(0.5 * (f.x + f.w), -0.5 * (f.z - f.y)));
-            lds[n - k] = fft_prep_in(F, n - k, make_double2(0.5 * (f.x - f.w), -0.5 * (f.y + f.z)));
+            lds[fpad(k)] = fft_prep_in(F, k, make_double2(0.5 * (f.x + f.w), -0.5 * (f.z - f.y)));
+            lds[fpad(n - k)] = fft_prep_in(F, n - k, make_double2(0.5 * (f.x - f.w), -0.5 * (f.y + f.z)));
         }
     }
     __syncthreads();
@@ -260,7 +260,7 @@ __device__ __forceinline__ void anal_fft_tile(const FftDev &F, const double *__r
             v.y = south ? grid[rw.y * ld + k] : 0.0;
             v = fft_prep_in(F, k, v);
         }
-        lds[k] = v;
+        lds[fpad(k)] = v;
     }
     __syncthreads();
     fft_forward(lds, F);

@@ -58,11 +58,18 @@ struct FftDev {
     int n, m, nst, blue;
     int radix[kMaxStages];
     int stride[kMaxStages];
+    int twoff[kMaxStages];
     const double2 *tw;     // [m]
+    const double2 *twst;   // per-stage twiddles (w_{RS}^i, i < S)
     const int *perm;       // [m]
     const double2 *chirp;  // [n]
     const double2 *bhat;   // [m]
 };
+
+// LDS slot of FFT element e: one pad slot per 16 elements keeps the strided
+// accesses of every power-of-two stage free of bank conflicts.
+__device__ __forceinline__ int fpad(int e) { return e + (e >> 4); }
+__host__ __device__ __forceinline__ int fft_lds_slots(int m) { return m + (m >> 4) + 1; }
 
 // Fully unrolled forward DFTs, y_k = sum_j v_j e^{-2 pi i jk/R}, in place.
 __device__ __forceinline__ void dft2(double2 &a, double2 &b)
@@ -135,7 +142,8 @@ __device__ __forceinline__ void dft_unrolled(double2 (&v)[R])
 // stage) before it.  Radices 2, 3, 4, 5 and 8 are unrolled; other primes use
 // a rolled loop over outputs that stores each output as it is formed.
 template <int R, bool DIT>
-__device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double2 *__restrict__ tw)
+__device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double2 *__restrict__ tw,
+                                          const double2 *__restrict__ tws)
 {
     constexpr bool kUnrolled = (R == 2 || R == 3 || R == 4 || R == 5 || R == 8);
     const int Lsub = R * S, step = m / Lsub, nb = m / R;
@@ -148,20 +156,30 @@ __device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double
         const int i = b - grp * S, base = grp * Lsub + i;
         double2 v[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = x[base + r * S];
-        if constexpr (DIT) {
-#pragma unroll
-            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[i * r * step]);
-        }
+        for (int r = 0; r < R; ++r) v[r] = x[fpad(base + r * S)];
         if constexpr (kUnrolled) {
+            // twiddles w^r, w = e^{-2 pi i i/Lsub}, by a short product chain
+            double2 w[R];
+            w[0] = make_double2(1.0, 0.0);
+            if constexpr (R > 1) w[1] = tws[i];
+#pragma unroll
+            for (int r = 2; r < R; ++r) w[r] = cmul(w[r - 1], w[1]);
+            if constexpr (DIT) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r]);
+            }
             dft_unrolled<R>(v);
             if constexpr (!DIT) {
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[i * r * step]);
+                for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r]);
             }
 #pragma unroll
-            for (int r = 0; r < R; ++r) x[base + r * S] = v[r];
+            for (int r = 0; r < R; ++r) x[fpad(base + r * S)] = v[r];
         } else {
+            if constexpr (DIT) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[i * r * step]);
+            }
             const int st = m / R;
             for (int k = 0; k < R; ++k) {
                 double2 acc = v[0];
@@ -177,7 +195,7 @@ __device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double
                 if constexpr (!DIT) {
                     if (k > 0) acc = cmul(acc, tw[i * k * step]);
                 }
-                x[base + k * S] = acc;
+                x[fpad(base + k * S)] = acc;
             }
         }
     }
@@ -187,18 +205,20 @@ template <bool DIT>
 __device__ __forceinline__ void fft_run_stage(double2 *x, const FftDev &P, int st)
 {
     const int S = P.stride[st];
+    const double2 *tws = P.twst + P.twoff[st];
     switch (P.radix[st]) {
-    case 2: fft_stage<2, DIT>(x, P.m, S, P.tw); break;
-    case 3: fft_stage<3, DIT>(x, P.m, S, P.tw); break;
-    case 4: fft_stage<4, DIT>(x, P.m, S, P.tw); break;
-    case 5: fft_stage<5, DIT>(x, P.m, S, P.tw); break;
-    case 8: fft_stage<8, DIT>(x, P.m, S, P.tw); break;
-    case 7: fft_stage<7, DIT>(x, P.m, S, P.tw); break;
+    case 2: fft_stage<2, DIT>(x, P.m, S, P.tw, tws); break;
+    case 3: fft_stage<3, DIT>(x, P.m, S, P.tw, tws); break;
+    case 4: fft_stage<4, DIT>(x, P.m, S, P.tw, tws); break;
+    case 5: fft_stage<5, DIT>(x, P.m, S, P.tw, tws); break;
+    case 8: fft_stage<8, DIT>(x, P.m, S, P.tw, tws); break;
+    case 7: fft_stage<7, DIT>(x, P.m, S, P.tw, tws); break;
     default: break;
     }
 }
 
-// Value that the caller stores at natural index j < n before fft_forward.
+// Value that the caller stores at natural index j < n (LDS slot fpad(j))
+// before fft_forward.
 __device__ __forceinline__ double2 fft_prep_in(const FftDev &P, int j, double2 v)
 {
     return P.blue ? cmul(v, P.chirp[j]) : v;
@@ -213,7 +233,7 @@ __device__ __forceinline__ void fft_forward(double2 *x, const FftDev &P)
         __syncthreads();
     }
     if (P.blue) {
-        for (int q = threadIdx.x; q < P.m; q += blockDim.x) x[q] = conjd(cmul(x[q], P.bhat[q]));
+        for (int q = threadIdx.x; q < P.m; q += blockDim.x) x[fpad(q)] = conjd(cmul(x[fpad(q)], P.bhat[q]));
         __syncthreads();
         for (int st = P.nst - 1; st >= 0; --st) {
             fft_run_stage<true>(x, P, st);
@@ -225,8 +245,8 @@ __device__ __forceinline__ void fft_forward(double2 *x, const FftDev &P)
 // X_k after fft_forward.
 __device__ __forceinline__ double2 fft_get(const double2 *x, const FftDev &P, int k)
 {
-    if (P.blue) return cmul(conjd(x[k]), P.chirp[k]);
-    return x[P.perm[k]];
+    if (P.blue) return cmul(conjd(x[fpad(k)]), P.chirp[k]);
+    return x[fpad(P.perm[k])];
 }
 
 }  // namespace shtg

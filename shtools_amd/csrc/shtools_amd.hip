@@ -117,14 +117,20 @@ static const DevSeeds *get_seeds(int dev, int norm, int csphase, int L)
 
 static bool upload_fft(const FftPlanHost &h, FftDev &d)
 {
-    if ((size_t)h.m * sizeof(double2) > kMaxLds) return false;
+    if ((size_t)fft_lds_slots(h.m) * sizeof(double2) > kMaxLds) return false;
     std::memset(&d, 0, sizeof(d));
     d.n = h.n;
     d.m = h.m;
     d.nst = h.nst;
     d.blue = h.blue;
-    for (int i = 0; i < h.nst; ++i) { d.radix[i] = h.radix[i]; d.stride[i] = h.stride[i]; }
+    for (int i = 0; i < h.nst; ++i) {
+        d.radix[i] = h.radix[i];
+        d.stride[i] = h.stride[i];
+        d.twoff[i] = h.twoff[i];
+    }
     d.tw = reinterpret_cast<const double2 *>(dev_upload(h.tw));
+    d.twst = reinterpret_cast<const double2 *>(dev_upload(h.twst));
+    if (!d.twst) return false;
     d.perm = dev_upload(h.perm);
     if (!d.tw || !d.perm) return false;
     if (h.blue) {
@@ -319,7 +325,7 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
     }
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
-        const size_t lds = (size_t)g.fft.m * sizeof(double2);
+        const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         hipLaunchKernelGGL(k_synth_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, fbuf, g.P, Lc, g.u, g.rows,
                            nrows_out, grid, ld, ext_col);
     }
@@ -341,7 +347,7 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
 
     {
         ProfScope ps(SHTG_K_ANAL_FFT, st);
-        const size_t lds = (size_t)g.fft.m * sizeof(double2);
+        const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         hipLaunchKernelGGL(k_anal_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, grid, ld, g.u, g.rows, g.nlat,
                            g.w, g.P, Lc, gbuf);
     }
@@ -422,6 +428,7 @@ static void batch_free(BatchPlan *b)
     (void)hipFree(b->plans);
     for (auto &p : b->hplans) {
         (void)hipFree((void *)p.tw);
+        (void)hipFree((void *)p.twst);
         (void)hipFree((void *)p.perm);
         (void)hipFree((void *)p.chirp);
         (void)hipFree((void *)p.bhat);
@@ -484,7 +491,7 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
         FftDev fd;
         if (!upload_fft(fp, fd)) return ST_OTHER;
         b->hplans.push_back(fd);
-        plan_lds.push_back((size_t)fp.m * sizeof(double2));
+        plan_lds.push_back((size_t)fft_lds_slots(fp.m) * sizeof(double2));
         geo_of[L] = {off, (int)b->hplans.size() - 1};
     }
     b->z = dev_upload(z);

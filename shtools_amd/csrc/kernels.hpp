@@ -34,7 +34,7 @@
 namespace shtg {
 
 constexpr int kThreads = 256;  // generic workgroups
-constexpr int kFftMaxThreads = 512;  // ring-FFT workgroups: 256 or 512 threads
+constexpr int kFftMaxThreads = 1024;  // ring-FFT workgroups: 256, 512 or 1024 threads
 constexpr int kWave = 64;      // Legendre workgroups are single waves
 constexpr int kSynthKMax = 4;  // ring pairs per lane in synthesis (1, 2 or 4)
 constexpr int kAnalKMax = 8;   // ring pairs per lane and pass in analysis (1, 2, 4 or 8)

@@ -248,7 +248,9 @@ static int anal_lanes_per_slot(int P)
 // threads of a ring-FFT workgroup for a work length of m points
 static int fft_threads(int m)
 {
-    return m <= 1024 ? 256 : kFftMaxThreads;
+    if (m <= 1024) return 256;
+    if (m <= 4096) return 512;
+    return kFftMaxThreads;
 }
 
 // ring pairs per lane of a synthesis wave for P slots: 1, 2 or 4
@@ -561,15 +563,18 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
     b->n_unpack = (int)iu.size();
     if (!b->items_pack || !b->items_sleg || !b->items_unpack) return ST_ALLOC;
 
-    // FFT items grouped by LDS size class (larger first)
-    std::map<size_t, std::vector<W>, std::greater<size_t>> groups;
+    // FFT items grouped by how many workgroups fit the LDS of a CU (the
+    // group's LDS is its largest need), then by threads per workgroup
+    std::map<std::pair<int, int>, std::vector<W>> groups;  // (-(WGs per CU), -threads)
+    std::map<std::pair<int, int>, size_t> group_lds;
     for (int i = 0; i < nfields; ++i) {
         const BField &f = fields[i];
         const size_t need = plan_lds[f.fft];
-        size_t cls = 16 * 1024;
-        while (cls < need) cls *= 2;
-        if (cls > kMaxLds) cls = kMaxLds;
-        for (int s = 0; s < f.P; s += kFftSlots) groups[cls].push_back({(double)need, i, s, 0});
+        const int per_cu = std::max(1, (int)(kMaxLds / need));
+        const int thr = fft_threads(b->hplans[f.fft].m);
+        const auto key = std::make_pair(-std::min(per_cu, 4), -thr);
+        group_lds[key] = std::max(group_lds[key], need);
+        for (int s = 0; s < f.P; s += kFftSlots) groups[key].push_back({(double)need, i, s, 0});
     }
     for (auto &kv : groups) {
         std::stable_sort(kv.second.begin(), kv.second.end(), by_work);
@@ -578,8 +583,8 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
         FftGroup g;
         g.items = dev_upload(it);
         g.nitems = (int)it.size();
-        g.lds = kv.first;
-        g.threads = fft_threads((int)(kv.first / sizeof(double2)));
+        g.lds = group_lds[kv.first];
+        g.threads = -kv.first.second;
         if (!g.items) return ST_ALLOC;
         b->fft.push_back(g);
     }

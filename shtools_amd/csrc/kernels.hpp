@@ -208,24 +208,34 @@ __device__ __forceinline__ void synth_fft_tile(const FftDev &F, const double4 *_
     const int n = F.n;
     for (int k = threadIdx.x; k < F.m; k += blockDim.x) lds[fpad(k)] = make_double2(0.0, 0.0);
     __syncthreads();
-    for (int k = threadIdx.x; k <= Lc; k += blockDim.x) {
-        // Z_k = X_k + i Y_k with X (north) and Y (south) Hermitian:
-        // X_0 = C_N, X_k = (C_N - i S_N)/2 and X_{n-k} = conj(X_k); stored
-        // conjugated (inverse DFT via conj(F(conj Z))).
-        const double r = order_rescale(u, k);
-        double4 f = make_double4(0.0, 0.0, 0.0, 0.0);
-        if (r != 0.0) {
-            f = fbuf[(size_t)k * P + s];
-            f.x *= r;
-            f.y *= r;
-            f.z *= r;
-            f.w *= r;
-        }
-        if (k == 0) {
-            lds[0] = fft_prep_in(F, 0, make_double2(f.x, -f.z));
-        } else {
-            lds[fpad(k)] = fft_prep_in(F, k, make_double2(0.5 * (f.x + f.w), -0.5 * (f.z - f.y)));
-            lds[fpad(n - k)] = fft_prep_in(F, n - k, make_double2(0.5 * (f.x - f.w), -0.5 * (f.y + f.z)));
+    // each thread takes 4 consecutive orders: one extended-range power, then
+    // products with u (as the reference's iterated rescalem)
+    for (int k0 = 4 * threadIdx.x; k0 <= Lc; k0 += 4 * blockDim.x) {
+        double r = 1.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = k0 + j;
+            if (k > Lc) break;
+            if (k == 0) r = 1.0;
+            else if (j == 0 || k == 1) r = scaled_upow(u, k);
+            else r *= u;
+            // Z_k = X_k + i Y_k with X (north) and Y (south) Hermitian:
+            // X_0 = C_N, X_k = (C_N - i S_N)/2 and X_{n-k} = conj(X_k); stored
+            // conjugated (inverse DFT via conj(F(conj Z))).
+            double4 f = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (r != 0.0) {
+                f = fbuf[(size_t)k * P + s];
+                f.x *= r;
+                f.y *= r;
+                f.z *= r;
+                f.w *= r;
+            }
+            if (k == 0) {
+                lds[0] = fft_prep_in(F, 0, make_double2(f.x, -f.z));
+            } else {
+                lds[fpad(k)] = fft_prep_in(F, k, make_double2(0.5 * (f.x + f.w), -0.5 * (f.z - f.y)));
+                lds[fpad(n - k)] = fft_prep_in(F, n - k, make_double2(0.5 * (f.x - f.w), -0.5 * (f.y + f.z)));
+            }
         }
     }
     __syncthreads();
@@ -265,15 +275,23 @@ __device__ __forceinline__ void anal_fft_tile(const FftDev &F, const double *__r
     __syncthreads();
     fft_forward(lds, F);
     const double wn = north ? w.x : 0.0, ws = south ? w.y : 0.0;
-    for (int mm = threadIdx.x; mm <= Lc; mm += blockDim.x) {
-        const double2 zm = fft_get(lds, F, mm);
-        const double2 zn = fft_get(lds, F, mm == 0 ? 0 : n - mm);
-        const double xr = 0.5 * (zm.x + zn.x), xi = 0.5 * (zm.y - zn.y);
-        const double yr = 0.5 * (zm.y + zn.y), yi = 0.5 * (zn.x - zm.x);
-        const double f1c = wn * xr, f1s = -wn * xi, f2c = ws * yr, f2s = -ws * yi;
-        const double r = order_rescale(u, mm);
-        gbuf[(size_t)mm * P + s] =
-            make_double4((f1c + f2c) * r, (f1s + f2s) * r, (f1c - f2c) * r, (f1s - f2s) * r);
+    for (int k0 = 4 * threadIdx.x; k0 <= Lc; k0 += 4 * blockDim.x) {
+        double r = 1.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int mm = k0 + j;
+            if (mm > Lc) break;
+            if (mm == 0) r = 1.0;
+            else if (j == 0 || mm == 1) r = scaled_upow(u, mm);
+            else r *= u;
+            const double2 zm = fft_get(lds, F, mm);
+            const double2 zn = fft_get(lds, F, mm == 0 ? 0 : n - mm);
+            const double xr = 0.5 * (zm.x + zn.x), xi = 0.5 * (zm.y - zn.y);
+            const double yr = 0.5 * (zm.y + zn.y), yi = 0.5 * (zn.x - zm.x);
+            const double f1c = wn * xr, f1s = -wn * xi, f2c = ws * yr, f2s = -ws * yi;
+            gbuf[(size_t)mm * P + s] =
+                make_double4((f1c + f2c) * r, (f1s + f2s) * r, (f1c - f2c) * r, (f1s - f2s) * r);
+        }
     }
 }
 
@@ -414,10 +432,9 @@ __global__ __launch_bounds__(kWave) void kb_synth_leg(const BField *__restrict__
                                    fbuf + f.foff + (size_t)m * f.P);
 }
 
-// Ring FFT items (field, first slot): each workgroup transforms kFftSlots
+// Ring FFT items (field, first slot): each workgroup transforms nslots
 // consecutive ring pairs in turn.  Workgroups are mapped so that groups of 4
 // consecutive items run on the same XCD (shared L2 lines of fbuf/gbuf).
-constexpr int kFftSlots = 2;
 
 __device__ __forceinline__ int xcd_item(int b, int n)
 {
@@ -427,8 +444,9 @@ __device__ __forceinline__ int xcd_item(int b, int n)
 }
 
 __global__ __launch_bounds__(kFftMaxThreads) void kb_synth_fft(const BField *__restrict__ fields,
-                                                         const int2 *__restrict__ items, int nitems, BTables T,
-                                                         const double4 *__restrict__ fbuf, double *const *grid)
+                                                         const int2 *__restrict__ items, int nitems, int nslots,
+                                                         BTables T, const double4 *__restrict__ fbuf,
+                                                         double *const *grid)
 {
     extern __shared__ double2 lds[];
     const int id = xcd_item(blockIdx.x, nitems);
@@ -436,7 +454,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_synth_fft(const BField *__r
     const int2 it = items[id];
     const BField f = fields[it.x];
     const FftDev &F = T.plans[f.fft];
-    for (int s = it.y; s < it.y + kFftSlots && s < f.P; ++s) {
+    for (int s = it.y; s < it.y + nslots && s < f.P; ++s) {
         synth_fft_tile(F, fbuf + f.foff, f.P, f.L, s, T.u[f.geo + s], T.rows[f.geo + s], f.nlat, grid[it.x],
                        f.nlong, 0, lds);
         __syncthreads();
@@ -444,8 +462,9 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_synth_fft(const BField *__r
 }
 
 __global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft(const BField *__restrict__ fields,
-                                                        const int2 *__restrict__ items, int nitems, BTables T,
-                                                        const double *const *grid, double4 *__restrict__ gbuf)
+                                                        const int2 *__restrict__ items, int nitems, int nslots,
+                                                        BTables T, const double *const *grid,
+                                                        double4 *__restrict__ gbuf)
 {
     extern __shared__ double2 lds[];
     const int id = xcd_item(blockIdx.x, nitems);
@@ -453,7 +472,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft(const BField *__re
     const int2 it = items[id];
     const BField f = fields[it.x];
     const FftDev &F = T.plans[f.fft];
-    for (int s = it.y; s < it.y + kFftSlots && s < f.P; ++s) {
+    for (int s = it.y; s < it.y + nslots && s < f.P; ++s) {
         anal_fft_tile(F, grid[it.x], f.nlong, T.rows[f.geo + s], f.nlat, T.w[f.geo + s], T.u[f.geo + s], f.P, f.L,
                       s, gbuf + f.foff, lds);
         __syncthreads();

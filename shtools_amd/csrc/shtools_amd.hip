@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <algorithm>
 #include <functional>
@@ -245,12 +246,30 @@ static int anal_lanes_per_slot(int P)
     return k;
 }
 
-// threads of a ring-FFT workgroup for a work length of m points
+// Tuning of the ring-FFT launches: threads per workgroup by work length and
+// ring pairs per batched workgroup.  SHTG_FFT_MAXTHREADS (256/512/1024) and
+// SHTG_FFT_SLOTS override the defaults.
+static int env_int(const char *name, int dflt)
+{
+    const char *v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    int x = std::atoi(v);
+    return x > 0 ? x : dflt;
+}
+
 static int fft_threads(int m)
 {
-    if (m <= 1024) return 256;
-    if (m <= 4096) return 512;
-    return kFftMaxThreads;
+    static const int cap = std::min(kFftMaxThreads, env_int("SHTG_FFT_MAXTHREADS", kFftMaxThreads));
+    int t = 256;
+    if (m > 1024) t = 512;
+    if (m > 4096) t = 1024;
+    return std::min(t, cap);
+}
+
+static int fft_slots_per_wg()
+{
+    static const int v = std::max(1, std::min(16, env_int("SHTG_FFT_SLOTS", 2)));
+    return v;
 }
 
 // ring pairs per lane of a synthesis wave for P slots: 1, 2 or 4
@@ -395,6 +414,7 @@ struct BatchPlan {
     int2 *items_unpack = nullptr;
     int n_unpack = 0;
     std::vector<FftGroup> fft;
+    int fft_slots = 2;
     double *z = nullptr, *u = nullptr;
     double2 *w = nullptr;
     int2 *rows = nullptr;
@@ -563,6 +583,7 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
     b->n_unpack = (int)iu.size();
     if (!b->items_pack || !b->items_sleg || !b->items_unpack) return ST_ALLOC;
 
+    b->fft_slots = fft_slots_per_wg();
     // FFT items grouped by how many workgroups fit the LDS of a CU (the
     // group's LDS is its largest need), then by threads per workgroup
     std::map<std::pair<int, int>, std::vector<W>> groups;  // (-(WGs per CU), -threads)
@@ -574,7 +595,7 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
         const int thr = fft_threads(b->hplans[f.fft].m);
         const auto key = std::make_pair(-std::min(per_cu, 4), -thr);
         group_lds[key] = std::max(group_lds[key], need);
-        for (int s = 0; s < f.P; s += kFftSlots) groups[key].push_back({(double)need, i, s, 0});
+        for (int s = 0; s < f.P; s += b->fft_slots) groups[key].push_back({(double)need, i, s, 0});
     }
     for (auto &kv : groups) {
         std::stable_sort(kv.second.begin(), kv.second.end(), by_work);
@@ -640,7 +661,7 @@ static int batch_synthesis(BatchPlan *b, hipStream_t st)
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
         for (auto &g : b->fft)
             hipLaunchKernelGGL(kb_synth_fft, dim3(xcd_blocks(g.nitems)), dim3(g.threads), g.lds, st, b->fields,
-                               g.items, g.nitems, b->T, b->fbuf, gout);
+                               g.items, g.nitems, b->fft_slots, b->T, b->fbuf, gout);
     }
     return HIP_OK(hipGetLastError()) ? ST_OK : ST_OTHER;
 }
@@ -653,7 +674,7 @@ static int batch_analysis(BatchPlan *b, hipStream_t st)
         ProfScope ps(SHTG_K_ANAL_FFT, st);
         for (auto &g : b->fft)
             hipLaunchKernelGGL(kb_anal_fft, dim3(xcd_blocks(g.nitems)), dim3(g.threads), g.lds, st, b->fields,
-                               g.items, g.nitems, b->T, gin, b->fbuf);
+                               g.items, g.nitems, b->fft_slots, b->T, gin, b->fbuf);
     }
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);

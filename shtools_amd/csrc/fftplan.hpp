@@ -22,6 +22,7 @@ constexpr int kMaxStages = 24;
 
 struct FftPlanHost {
     int n = 0, m = 0, nst = 0, blue = 0;
+    int lc = -1;                // pruning: orders |k| <= lc only (-1: none)
     int radix[kMaxStages] = {0};
     int stride[kMaxStages] = {0};
     int twoff[kMaxStages] = {0};  // offset of each stage's twiddles in twst
@@ -30,6 +31,7 @@ struct FftPlanHost {
     std::vector<int> perm;      // m: DIF output position of natural index k
     std::vector<double> chirp;  // 2*n (Bluestein)
     std::vector<double> bhat;   // 2*m (Bluestein), DIF output order, /m
+    std::vector<double> bhat2;  // pruned plans: analysis spectrum (bhat is synthesis)
 };
 
 inline bool factor_smooth(int m, std::vector<int> &rad)
@@ -125,7 +127,36 @@ inline void set_stages(FftPlanHost &p, const std::vector<int> &rad)
     if (p.twst.empty()) p.twst.assign(2, 0.0);
 }
 
-inline FftPlanHost make_fft_plan(int n)
+// Spectrum (DIF order, scaled by 1/m) of the conjugate chirp placed at the
+// positions p of signed lags t: p = t for t in [0, tpos], p = m + t for
+// t in [-tneg, -1].
+inline std::vector<double> chirp_spectrum(const FftPlanHost &p, int tpos, int tneg)
+{
+    const long double pi = 3.14159265358979323846264338327950288L;
+    const int n = p.n;
+    auto wconj = [&](long long t) {
+        long long q = ((t % (2LL * n)) + 2LL * n) % (2LL * n);
+        q = (q * q) % (2LL * n);
+        long double a = pi * (long double)q / (long double)n;
+        return std::complex<long double>(std::cos(a), std::sin(a));
+    };
+    std::vector<std::complex<long double>> b(p.m, 0);
+    for (int t = 0; t <= tpos; ++t) b[t] = wconj(t);
+    for (int t = 1; t <= tneg; ++t) b[p.m - t] = wconj(-t);
+    host_dif(b, p);
+    std::vector<double> out(2 * (size_t)p.m);
+    for (int q = 0; q < p.m; ++q) {
+        out[2 * q] = (double)(b[q].real() / p.m);
+        out[2 * q + 1] = (double)(b[q].imag() / p.m);
+    }
+    return out;
+}
+
+// Plan for length n.  lc >= 0 (even n only) declares that the synthesis
+// input and the analysis output occupy only the orders |k| <= lc; Bluestein
+// then needs m >= n + 2 lc instead of 2n - 1 (signed indices: negative
+// orders sit at the top of the work array).
+inline FftPlanHost make_fft_plan(int n, int lc = -1)
 {
     FftPlanHost p;
     p.n = n;
@@ -141,30 +172,24 @@ inline FftPlanHost make_fft_plan(int n)
         return p;
     }
     p.blue = 1;
-    p.m = next_5smooth(2 * n - 1);
+    const bool prune = lc >= 0 && (n % 2 == 0) && n + 2 * lc < 2 * n - 1;
+    p.lc = prune ? lc : -1;
+    p.m = next_5smooth(prune ? n + 2 * lc : 2 * n - 1);
     factor_smooth(p.m, rad);
     set_stages(p, rad);
     const long double pi = 3.14159265358979323846264338327950288L;
-    std::vector<std::complex<long double>> ch(n);
     p.chirp.assign(2 * (size_t)n, 0.0);
     for (int j = 0; j < n; ++j) {
         long long q = ((long long)j * j) % (2LL * n);
         long double a = -pi * (long double)q / (long double)n;
-        ch[j] = {std::cos(a), std::sin(a)};
-        p.chirp[2 * j] = (double)ch[j].real();
-        p.chirp[2 * j + 1] = (double)ch[j].imag();
+        p.chirp[2 * j] = (double)std::cos(a);
+        p.chirp[2 * j + 1] = (double)std::sin(a);
     }
-    std::vector<std::complex<long double>> b(p.m, 0);
-    b[0] = std::conj(ch[0]);
-    for (int j = 1; j < n; ++j) {
-        b[j] = std::conj(ch[j]);
-        b[p.m - j] = std::conj(ch[j]);
-    }
-    host_dif(b, p);  // now in DIF order
-    p.bhat.assign(2 * (size_t)p.m, 0.0);
-    for (int q = 0; q < p.m; ++q) {
-        p.bhat[2 * q] = (double)(b[q].real() / p.m);
-        p.bhat[2 * q + 1] = (double)(b[q].imag() / p.m);
+    if (prune) {
+        p.bhat = chirp_spectrum(p, n - 1 + lc, lc);   // synthesis: inputs |j| <= lc, outputs [0, n)
+        p.bhat2 = chirp_spectrum(p, lc, n - 1 + lc);  // analysis: inputs [0, n), outputs |k| <= lc
+    } else {
+        p.bhat = chirp_spectrum(p, n - 1, n - 1);
     }
     return p;
 }

@@ -234,16 +234,17 @@ __device__ __forceinline__ void synth_fft_tile(const FftDev &F, const double4 *_
                 lds[0] = fft_prep_in(F, 0, make_double2(f.x, -f.z));
             } else {
                 lds[fpad(k)] = fft_prep_in(F, k, make_double2(0.5 * (f.x + f.w), -0.5 * (f.z - f.y)));
-                lds[fpad(n - k)] = fft_prep_in(F, n - k, make_double2(0.5 * (f.x - f.w), -0.5 * (f.y + f.z)));
+                lds[fpad(fft_pos(F, n - k))] =
+                    fft_prep_in(F, n - k, make_double2(0.5 * (f.x - f.w), -0.5 * (f.y + f.z)));
             }
         }
     }
     __syncthreads();
-    fft_forward(lds, F);
+    fft_forward(lds, F, F.bhat);
     const bool north = rw.x >= 0 && rw.x < nrows;
     const bool south = rw.y >= 0 && rw.y < nrows;
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        const double2 v = fft_get(lds, F, j);
+        const double2 v = fft_get(lds, F, j, j);
         if (north) grid[rw.x * ld + j] = v.x;
         if (south) grid[rw.y * ld + j] = -v.y;
         if (ext_col && j == 0) {
@@ -273,7 +274,7 @@ __device__ __forceinline__ void anal_fft_tile(const FftDev &F, const double *__r
         lds[fpad(k)] = v;
     }
     __syncthreads();
-    fft_forward(lds, F);
+    fft_forward(lds, F, F.bhat2);
     const double wn = north ? w.x : 0.0, ws = south ? w.y : 0.0;
     for (int k0 = 4 * threadIdx.x; k0 <= Lc; k0 += 4 * blockDim.x) {
         double r = 1.0;
@@ -284,8 +285,9 @@ __device__ __forceinline__ void anal_fft_tile(const FftDev &F, const double *__r
             if (mm == 0) r = 1.0;
             else if (j == 0 || mm == 1) r = scaled_upow(u, mm);
             else r *= u;
-            const double2 zm = fft_get(lds, F, mm);
-            const double2 zn = fft_get(lds, F, mm == 0 ? 0 : n - mm);
+            const double2 zm = fft_get(lds, F, mm, mm);
+            const int nm = mm == 0 ? 0 : n - mm;
+            const double2 zn = fft_get(lds, F, nm, fft_pos(F, nm));
             const double xr = 0.5 * (zm.x + zn.x), xi = 0.5 * (zm.y - zn.y);
             const double yr = 0.5 * (zm.y + zn.y), yi = 0.5 * (zn.x - zm.x);
             const double f1c = wn * xr, f1s = -wn * xi, f2c = ws * yr, f2s = -ws * yi;

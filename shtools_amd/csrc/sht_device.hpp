@@ -56,6 +56,7 @@ __device__ __forceinline__ double scaled_upow(double u, int m)
 
 struct FftDev {
     int n, m, nst, blue;
+    int lc;                // pruned Bluestein: orders |k| <= lc (-1: unpruned)
     int radix[kMaxStages];
     int stride[kMaxStages];
     int twoff[kMaxStages];
@@ -63,8 +64,16 @@ struct FftDev {
     const double2 *twst;   // per-stage twiddles (w_{RS}^i, i < S)
     const int *perm;       // [m]
     const double2 *chirp;  // [n]
-    const double2 *bhat;   // [m]
+    const double2 *bhat;   // [m] (pruned plans: synthesis spectrum)
+    const double2 *bhat2;  // [m] pruned plans: analysis spectrum (else = bhat)
 };
+
+// Work-array position of signed order k (|k| <= lc) of a pruned plan, or of
+// natural index k otherwise.
+__device__ __forceinline__ int fft_pos(const FftDev &P, int k)
+{
+    return (P.lc >= 0 && k > P.lc) ? k - P.n + P.m : k;
+}
 
 // LDS slot of FFT element e: one pad slot per 16 elements keeps the strided
 // accesses of every power-of-two stage free of bank conflicts.
@@ -224,16 +233,17 @@ __device__ __forceinline__ double2 fft_prep_in(const FftDev &P, int j, double2 v
     return P.blue ? cmul(v, P.chirp[j]) : v;
 }
 
-// In-place forward DFT of the n values in x[0..n) (x has P.m slots; for
-// Bluestein the caller has zeroed x[n..m)).  Ends with a barrier.
-__device__ __forceinline__ void fft_forward(double2 *x, const FftDev &P)
+// In-place forward DFT of the values stored in x (x has P.m slots; for
+// Bluestein the caller has zeroed the unused slots).  bhat selects the chirp
+// spectrum (P.bhat for synthesis, P.bhat2 for analysis).  Ends with a barrier.
+__device__ __forceinline__ void fft_forward(double2 *x, const FftDev &P, const double2 *__restrict__ bhat)
 {
     for (int st = 0; st < P.nst; ++st) {
         fft_run_stage<false>(x, P, st);
         __syncthreads();
     }
     if (P.blue) {
-        for (int q = threadIdx.x; q < P.m; q += blockDim.x) x[fpad(q)] = conjd(cmul(x[fpad(q)], P.bhat[q]));
+        for (int q = threadIdx.x; q < P.m; q += blockDim.x) x[fpad(q)] = conjd(cmul(x[fpad(q)], bhat[q]));
         __syncthreads();
         for (int st = P.nst - 1; st >= 0; --st) {
             fft_run_stage<true>(x, P, st);
@@ -242,10 +252,11 @@ __device__ __forceinline__ void fft_forward(double2 *x, const FftDev &P)
     }
 }
 
-// X_k after fft_forward.
-__device__ __forceinline__ double2 fft_get(const double2 *x, const FftDev &P, int k)
+// X_k after fft_forward; for Bluestein, pos is the work-array position
+// holding order k (fft_pos for signed orders of pruned plans).
+__device__ __forceinline__ double2 fft_get(const double2 *x, const FftDev &P, int k, int pos)
 {
-    if (P.blue) return cmul(conjd(x[fpad(k)]), P.chirp[k]);
+    if (P.blue) return cmul(conjd(x[fpad(pos)]), P.chirp[k]);
     return x[fpad(P.perm[k])];
 }
 

@@ -124,6 +124,7 @@ static bool upload_fft(const FftPlanHost &h, FftDev &d)
     d.m = h.m;
     d.nst = h.nst;
     d.blue = h.blue;
+    d.lc = h.lc;
     for (int i = 0; i < h.nst; ++i) {
         d.radix[i] = h.radix[i];
         d.stride[i] = h.stride[i];
@@ -137,20 +138,21 @@ static bool upload_fft(const FftPlanHost &h, FftDev &d)
     if (h.blue) {
         d.chirp = reinterpret_cast<const double2 *>(dev_upload(h.chirp));
         d.bhat = reinterpret_cast<const double2 *>(dev_upload(h.bhat));
-        if (!d.chirp || !d.bhat) return false;
+        d.bhat2 = h.bhat2.empty() ? d.bhat : reinterpret_cast<const double2 *>(dev_upload(h.bhat2));
+        if (!d.chirp || !d.bhat || !d.bhat2) return false;
     }
     return true;
 }
 
 static bool finish_geometry(Geometry &g, const std::vector<double> &z, const std::vector<double> &u,
-                            const std::vector<double> &w, const std::vector<int> &rows)
+                            const std::vector<double> &w, const std::vector<int> &rows, int lc)
 {
     g.z = dev_upload(z);
     g.u = dev_upload(u);
     g.w = reinterpret_cast<double2 *>(dev_upload(w));
     g.rows = reinterpret_cast<int2 *>(dev_upload(rows));
     if (!g.z || !g.u || !g.w || !g.rows) return false;
-    FftPlanHost fp = make_fft_plan(g.nlong);
+    FftPlanHost fp = make_fft_plan(g.nlong, lc);
     if (!upload_fft(fp, g.fft)) return false;
     g.ok = true;
     return true;
@@ -188,7 +190,7 @@ static Geometry *get_dh(int dev, int L, int sampling)
         w[2 * s] = c * aj[s];
         w[2 * s + 1] = (s == 0 || s == n / 2) ? 0.0 : c * aj[n - s];
     }
-    if (!finish_geometry(g, z, u, w, rows)) return nullptr;
+    if (!finish_geometry(g, z, u, w, rows, L)) return nullptr;
     return &g;
 }
 
@@ -227,7 +229,7 @@ static Geometry *get_glq(int dev, int L, const double *zero, const double *wq)
         w[2 * s] = wq ? 2.0 * pi * wq[s] / (double)g.nlong : 0.0;
         w[2 * s + 1] = (wq && !eq) ? 2.0 * pi * wq[nlat - 1 - s] / (double)g.nlong : 0.0;
     }
-    if (!finish_geometry(g, z, u, w, rows)) return nullptr;
+    if (!finish_geometry(g, z, u, w, rows, -1)) return nullptr;
     return &g;
 }
 
@@ -453,6 +455,7 @@ static void batch_free(BatchPlan *b)
         (void)hipFree((void *)p.twst);
         (void)hipFree((void *)p.perm);
         (void)hipFree((void *)p.chirp);
+        if (p.bhat2 != p.bhat) (void)hipFree((void *)p.bhat2);
         (void)hipFree((void *)p.bhat);
     }
     (void)hipFree(b->cbuf);
@@ -509,7 +512,7 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
             w.push_back(c * aj[s]);
             w.push_back((s == 0 || s == n / 2) ? 0.0 : c * aj[n - s]);
         }
-        FftPlanHost fp = make_fft_plan(nlong);
+        FftPlanHost fp = make_fft_plan(nlong, L);
         FftDev fd;
         if (!upload_fft(fp, fd)) return ST_OTHER;
         b->hplans.push_back(fd);

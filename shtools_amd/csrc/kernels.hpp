@@ -155,30 +155,10 @@ __device__ __forceinline__ void anal_leg_tile(const double *__restrict__ al, dou
             q1[k] = 0.0;
             q2[k] = -sd;
         }
-        for (int j0 = 0; j0 <= jmax; j0 += LC) {
-            double acc[LC][2];
-#pragma unroll
-            for (int jj = 0; jj < LC; ++jj) acc[jj][0] = acc[jj][1] = 0.0;
-#pragma unroll
-            for (int jj = 0; jj < LC; ++jj) {
-                const int j = j0 + jj;
-                if (j <= jmax) {
-                    const double a = al[j];
-#pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        const double q = (a * z[k]) * q1[k] - q2[k];
-                        q2[k] = q1[k];
-                        q1[k] = q;
-                        if (jj % 2 == 0) {
-                            acc[jj][0] += q * eC[k];
-                            acc[jj][1] += q * eS[k];
-                        } else {
-                            acc[jj][0] += q * oC[k];
-                            acc[jj][1] += q * oS[k];
-                        }
-                    }
-                }
-            }
+        // Each chunk of LC degrees: K recursion steps per lane and degree, the
+        // first ring's product initialising the accumulators.  Full chunks
+        // run without guards so the recursion state rotates in registers.
+        auto reduce_chunk = [&](double (&acc)[LC][2], int j0) {
             // transpose through LDS: lane (v, part) sums 16 lanes of value v
 #pragma unroll
             for (int v = 0; v < 2 * LC; ++v) red[v * kRedStride + lane] = acc[v >> 1][v & 1];
@@ -195,6 +175,50 @@ __device__ __forceinline__ void anal_leg_tile(const double *__restrict__ al, dou
                 *o = (p0 == s0) ? sum : *o + sum;
             }
             __syncthreads();
+        };
+        int j0 = 0;
+        for (; j0 + LC - 1 <= jmax; j0 += LC) {
+            double acc[LC][2];
+#pragma unroll
+            for (int jj = 0; jj < LC; ++jj) {
+                const double a = al[j0 + jj];
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const double q = (a * z[k]) * q1[k] - q2[k];
+                    q2[k] = q1[k];
+                    q1[k] = q;
+                    const double ec = (jj % 2 == 0) ? eC[k] : oC[k];
+                    const double es = (jj % 2 == 0) ? eS[k] : oS[k];
+                    if (k == 0) {
+                        acc[jj][0] = q * ec;
+                        acc[jj][1] = q * es;
+                    } else {
+                        acc[jj][0] += q * ec;
+                        acc[jj][1] += q * es;
+                    }
+                }
+            }
+            reduce_chunk(acc, j0);
+        }
+        if (j0 <= jmax) {
+            double acc[LC][2];
+#pragma unroll
+            for (int jj = 0; jj < LC; ++jj) acc[jj][0] = acc[jj][1] = 0.0;
+#pragma unroll
+            for (int jj = 0; jj < LC; ++jj) {
+                if (j0 + jj <= jmax) {
+                    const double a = al[j0 + jj];
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        const double q = (a * z[k]) * q1[k] - q2[k];
+                        q2[k] = q1[k];
+                        q1[k] = q;
+                        acc[jj][0] += q * ((jj % 2 == 0) ? eC[k] : oC[k]);
+                        acc[jj][1] += q * ((jj % 2 == 0) ? eS[k] : oS[k]);
+                    }
+                }
+            }
+            reduce_chunk(acc, j0);
         }
     }
 }

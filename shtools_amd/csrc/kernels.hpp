@@ -3,7 +3,8 @@
 // kernels built from the same tiles.
 //
 // Synthesis (MakeGridDH / MakeGridGLQ):
-//   pack        cilm[2][d1][d2] -> cpack[m][l-m] = (C s_lm, S s_lm)
+//   pack        cilm[2][d1][d2] -> cpack[m][l-m] = (C s_lm, S s_lm) (LDS-tiled
+//               transpose; S_l0 is ignored as in src/MakeGridDH.f95:523-538)
 //   synth_leg   one workgroup per (order m, block of ring pairs); lanes are
 //               ring pairs, the recursion over l runs in registers with
 //               wave-uniform (scalar) table and coefficient loads; north and
@@ -43,13 +44,62 @@ constexpr int kRedStride = 65; // LDS row stride (doubles) of the analysis reduc
 
 // ====================================================================== tiles
 
-// One packed coefficient: C_lm s_lm, S_lm s_lm (S_l0 ignored as in
-// src/MakeGridDH.f95:523-538: only cilm(1,l,1) enters the m = 0 sum).
-__device__ __forceinline__ double2 pack_one(const double *__restrict__ cilm, int d1, int d2, int l, int m, double s)
+// Coefficient tiles of kTL degrees x kTM orders, transposed through LDS so
+// that both the [l][m] arrays and the per-order packed arrays are accessed
+// contiguously.
+constexpr int kTM = 32, kTL = 64;
+typedef double2 CoefTile[kTL][kTM + 1];
+
+// cpack[tri(m) + l - m] = (C s, S s) for the tile's m <= l <= L.
+__device__ __forceinline__ void pack_tile(const double *__restrict__ cilm, int d1, int d2, int L,
+                                          const double *__restrict__ s_tab, int Lt, double2 *__restrict__ cpack,
+                                          int m0, int l0, CoefTile &t)
 {
-    const double c = cilm[(size_t)l * d2 + m];
-    const double sn = (m == 0) ? 0.0 : cilm[((size_t)d1 + l) * d2 + m];
-    return make_double2(c * s, sn * s);
+    for (int e = threadIdx.x; e < kTL * kTM; e += blockDim.x) {
+        const int r = e / kTM, c = e - r * kTM, l = l0 + r, m = m0 + c;
+        double2 v = make_double2(0.0, 0.0);
+        if (l <= L && m <= l) {
+            v.x = cilm[(size_t)l * d2 + m];
+            v.y = (m == 0) ? 0.0 : cilm[((size_t)d1 + l) * d2 + m];
+        }
+        t[r][c] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kTL * kTM; e += blockDim.x) {
+        const int c = e / kTL, r = e - c * kTL, l = l0 + r, m = m0 + c;
+        if (l <= L && m <= l) {
+            const double s = s_tab[tri_offd(m, Lt) + (l - m)];
+            const double2 v = t[r][c];
+            cpack[tri_offd(m, L) + (l - m)] = make_double2(v.x * s, v.y * s);
+        }
+    }
+}
+
+// cilm rows [l0, l0+kTL) x columns [m0, m0+kTM) (within d1 x d2): packed
+// sums times (s_lm * normalisation) for m <= l <= L, zero elsewhere and S_l0 = 0.
+__device__ __forceinline__ void unpack_tile(const double2 *__restrict__ cbuf, const double *__restrict__ post,
+                                            int Lt, int L, double *__restrict__ cilm, int d1, int d2, int m0, int l0,
+                                            CoefTile &t)
+{
+    for (int e = threadIdx.x; e < kTL * kTM; e += blockDim.x) {
+        const int c = e / kTL, r = e - c * kTL, l = l0 + r, m = m0 + c;
+        double2 v = make_double2(0.0, 0.0);
+        if (l <= L && m <= l) {
+            const double p = post[tri_offd(m, Lt) + (l - m)];
+            const double2 x = cbuf[tri_offd(m, L) + (l - m)];
+            v = make_double2(x.x * p, m == 0 ? 0.0 : x.y * p);
+        }
+        t[r][c] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kTL * kTM; e += blockDim.x) {
+        const int r = e / kTM, c = e - r * kTM, l = l0 + r, m = m0 + c;
+        if (l < d1 && m < d2) {
+            const double2 v = t[r][c];
+            cilm[(size_t)l * d2 + m] = v.x;
+            cilm[((size_t)d1 + l) * d2 + m] = v.y;
+        }
+    }
 }
 
 // The reference's rescalem for order m at a ring with sin(theta) = u
@@ -323,14 +373,16 @@ __device__ __forceinline__ void anal_fft_tile(const FftDev &F, const double *__r
 
 // =============================================================== plain kernels
 
+// grid: ntm x ntl tiles (blockIdx.x = tile_l * ntm + tile_m)
 __global__ __launch_bounds__(kThreads) void k_pack_synth(const double *__restrict__ cilm, int d1, int d2,
                                                          const double *__restrict__ s_tab, int Lt,
-                                                         double2 *__restrict__ cpack, int Lc)
+                                                         double2 *__restrict__ cpack, int Lc, int ntm)
 {
-    const int m = blockIdx.x;
-    const double *__restrict__ s = s_tab + tri_offd(m, Lt);
-    double2 *__restrict__ o = cpack + tri_offd(m, Lc);
-    for (int j = threadIdx.x; j <= Lc - m; j += blockDim.x) o[j] = pack_one(cilm, d1, d2, m + j, m, s[j]);
+    __shared__ CoefTile t;
+    const int tl = blockIdx.x / ntm, tm = blockIdx.x - tl * ntm;
+    const int m0 = tm * kTM, l0 = tl * kTL;
+    if (l0 + kTL - 1 < m0) return;  // above the diagonal: nothing to pack
+    pack_tile(cilm, d1, d2, Lc, s_tab, Lt, cpack, m0, l0, t);
 }
 
 // grid: (Lc+1) * nsb single-wave workgroups, order m = blockIdx / nsb, slot
@@ -383,29 +435,14 @@ __global__ __launch_bounds__(kWave) void k_anal_leg(const double *__restrict__ a
                      cbuf + tri_offd(m, Lc), red);
 }
 
-// cilm[i][l][m] = cbuf[tri(m) + l - m].i * post(l, m) for m <= l <= Lc,
-// zero elsewhere (including S_l0), over the whole [2][d1][d2] array.
-__device__ __forceinline__ double unpack_one(const double2 *__restrict__ cbuf, const double *__restrict__ post,
-                                             int Lt, int Lc, int i, int l, int m)
-{
-    if (l > Lc || m > l || (i == 1 && m == 0)) return 0.0;
-    const double2 c = cbuf[tri_offd(m, Lc) + (l - m)];
-    return (i == 0 ? c.x : c.y) * post[tri_offd(m, Lt) + (l - m)];
-}
-
+// grid: ntm x ntl tiles over the whole d1 x d2 array
 __global__ __launch_bounds__(kThreads) void k_unpack_anal(const double2 *__restrict__ cbuf,
                                                           const double *__restrict__ post, int Lt, int Lc,
-                                                          double *__restrict__ cilm, int d1, int d2)
+                                                          double *__restrict__ cilm, int d1, int d2, int ntm)
 {
-    const long long total = 2LL * d1 * d2;
-    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (long long)gridDim.x * blockDim.x) {
-        const int m = (int)(idx % d2);
-        const long long t = idx / d2;
-        const int l = (int)(t % d1);
-        const int i = (int)(t / d1);
-        cilm[idx] = unpack_one(cbuf, post, Lt, Lc, i, l, m);
-    }
+    __shared__ CoefTile t;
+    const int tl = blockIdx.x / ntm, tm = blockIdx.x - tl * ntm;
+    unpack_tile(cbuf, post, Lt, Lc, cilm, d1, d2, tm * kTM, tl * kTL, t);
 }
 
 // ============================================================= batched kernels
@@ -431,18 +468,15 @@ struct BTables {
     const FftDev *plans;
 };
 
-// items: (field, m)
-__global__ __launch_bounds__(kThreads) void kb_pack(const BField *__restrict__ fields, const int2 *__restrict__ items,
+// items: (field, first order, first degree) of a coefficient tile
+__global__ __launch_bounds__(kThreads) void kb_pack(const BField *__restrict__ fields, const int4 *__restrict__ items,
                                                     BTables T, const double *const *__restrict__ cilm,
                                                     double2 *__restrict__ cpack)
 {
-    const int2 it = items[blockIdx.x];
+    __shared__ CoefTile t;
+    const int4 it = items[blockIdx.x];
     const BField f = fields[it.x];
-    const int m = it.y, L = f.L;
-    const double *__restrict__ s = T.s + tri_offd(m, T.Lt);
-    const double *__restrict__ c = cilm[it.x];
-    double2 *__restrict__ o = cpack + f.coff + tri_offd(m, L);
-    for (int j = threadIdx.x; j <= L - m; j += blockDim.x) o[j] = pack_one(c, L + 1, L + 1, m + j, m, s[j]);
+    pack_tile(cilm[it.x], f.L + 1, f.L + 1, f.L, T.s, T.Lt, cpack + f.coff, it.y, it.z, t);
 }
 
 // items: (field, m, first slot, ring pairs per lane)
@@ -520,19 +554,16 @@ __global__ __launch_bounds__(kWave) void kb_anal_leg(const BField *__restrict__ 
                      f.L - m, 1, f.P, cbuf + f.coff + tri_offd(m, f.L), red);
 }
 
-// items: (field, l): writes row l of C and S of the field's [2][L+1][L+1].
+// items: (field, first order, first degree) of a tile of the field's
+// [2][L+1][L+1] output
 __global__ __launch_bounds__(kThreads) void kb_unpack(const BField *__restrict__ fields,
-                                                      const int2 *__restrict__ items, BTables T,
+                                                      const int4 *__restrict__ items, BTables T,
                                                       const double2 *__restrict__ cbuf, double *const *cilm)
 {
-    const int2 it = items[blockIdx.x];
+    __shared__ CoefTile t;
+    const int4 it = items[blockIdx.x];
     const BField f = fields[it.x];
-    const int l = it.y, L = f.L;
-    double *__restrict__ c = cilm[it.x];
-    for (int k = threadIdx.x; k < 2 * (L + 1); k += blockDim.x) {
-        const int i = k / (L + 1), m = k - i * (L + 1);
-        c[((size_t)i * (L + 1) + l) * (L + 1) + m] = unpack_one(cbuf + f.coff, T.post, T.Lt, L, i, l, m);
-    }
+    unpack_tile(cbuf + f.coff, T.post, T.Lt, f.L, cilm[it.x], f.L + 1, f.L + 1, it.y, it.z, t);
 }
 
 }  // namespace shtg

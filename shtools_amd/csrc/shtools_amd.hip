@@ -337,7 +337,9 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
 
     {
         ProfScope ps(SHTG_K_PACK, st);
-        hipLaunchKernelGGL(k_pack_synth, dim3(Lc + 1), dim3(kThreads), 0, st, cilm, d1, d2, rt.s, rt.L, cpack, Lc);
+        const int ntm = (Lc + kTM) / kTM, ntl = (Lc + kTL) / kTL;
+        hipLaunchKernelGGL(k_pack_synth, dim3(ntm * ntl), dim3(kThreads), 0, st, cilm, d1, d2, rt.s, rt.L, cpack, Lc,
+                           ntm);
     }
     {
         ProfScope ps(SHTG_K_SYNTH_LEG, st);
@@ -381,10 +383,10 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
         hipLaunchKernelGGL(kern, dim3(Lc + 1), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gbuf, Lc, g.s0,
                            g.P, cbuf);
     }
-    const long long total = 2LL * d1 * d2;
     ProfScope ps_un(SHTG_K_UNPACK, st);
-    hipLaunchKernelGGL(k_unpack_anal, dim3(grid_blocks(total)), dim3(kThreads), 0, st, cbuf, rt.post, rt.L, Lc, cilm,
-                       d1, d2);
+    const int ntm = (d2 + kTM - 1) / kTM, ntl = (d1 + kTL - 1) / kTL;
+    hipLaunchKernelGGL(k_unpack_anal, dim3(ntm * ntl), dim3(kThreads), 0, st, cbuf, rt.post, rt.L, Lc, cilm, d1, d2,
+                       ntm);
     if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
     (void)hipFreeAsync(ws, st);
     return ST_OK;
@@ -407,13 +409,13 @@ struct BatchPlan {
     int dev = -1, nf = 0, sampling = 2, norm = 1, csphase = 1, Lmax = 0;
     std::vector<int> L;
     BField *fields = nullptr;
-    int2 *items_pack = nullptr;
+    int4 *items_pack = nullptr;
     int n_pack = 0;
     int4 *items_sleg = nullptr;
     int n_sleg = 0;
     int2 *items_aleg[4] = {nullptr, nullptr, nullptr, nullptr};  // by K = 8, 4, 2, 1
     int n_aleg[4] = {0, 0, 0, 0};
-    int2 *items_unpack = nullptr;
+    int4 *items_unpack = nullptr;
     int n_unpack = 0;
     std::vector<FftGroup> fft;
     int fft_slots = 2;
@@ -555,25 +557,28 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
     std::vector<W> wp, ws, wa, wu;
     for (int i = 0; i < nfields; ++i) {
         const BField &f = fields[i];
+        for (int m0 = 0; m0 <= f.L; m0 += kTM)
+            for (int l0 = 0; l0 <= f.L; l0 += kTL) {
+                wu.push_back({(double)f.L, i, m0, l0});
+                if (l0 + kTL - 1 >= m0) wp.push_back({(double)f.L, i, m0, l0});
+            }
         for (int m = 0; m <= f.L; ++m) {
-            wp.push_back({(double)(f.L - m + 1), i, m, 0});
             const int kk = synth_lanes_per_slot(f.P), span = kk * kWave;
             for (int s0 = 0; s0 < f.P; s0 += span)
                 ws.push_back({(double)(f.L - m + 1) * std::min(span, f.P - s0), i, m, s0 + (kk << 24)});
             wa.push_back({(double)(f.L - m + 1) * f.P, i, m, anal_lanes_per_slot(f.P - 1)});
         }
-        for (int l = 0; l <= f.L; ++l) wu.push_back({(double)f.L, i, l, 0});
     }
     auto by_work = [](const W &x, const W &y) { return x.w > y.w; };
     std::stable_sort(wp.begin(), wp.end(), by_work);
     std::stable_sort(ws.begin(), ws.end(), by_work);
     std::stable_sort(wa.begin(), wa.end(), by_work);
-    std::vector<int2> ip, iu, ia[4];
-    std::vector<int4> is;
-    for (auto &x : wp) ip.push_back(make_int2(x.a, x.b));
+    std::vector<int4> ip, iu, is;
+    std::vector<int2> ia[4];
+    for (auto &x : wp) ip.push_back(make_int4(x.a, x.b, x.c, 0));
     for (auto &x : ws) is.push_back(make_int4(x.a, x.b, x.c & 0xFFFFFF, x.c >> 24));
     for (auto &x : wa) ia[x.c == 8 ? 0 : x.c == 4 ? 1 : x.c == 2 ? 2 : 3].push_back(make_int2(x.a, x.b));
-    for (auto &x : wu) iu.push_back(make_int2(x.a, x.b));
+    for (auto &x : wu) iu.push_back(make_int4(x.a, x.b, x.c, 0));
     b->items_pack = dev_upload(ip);
     b->items_sleg = dev_upload(is);
     for (int k = 0; k < 4; ++k) {

@@ -163,16 +163,25 @@ __device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double
         if (grp * S > b) --grp;
         if ((grp + 1) * S <= b) ++grp;
         const int i = b - grp * S, base = grp * Lsub + i;
+        // LDS slots of the R elements: linear in r when S is a multiple of 16
+        int slot[R];
+        if ((S & 15) == 0) {
+            const int b0 = fpad(base), ds = S + (S >> 4);
+#pragma unroll
+            for (int r = 0; r < R; ++r) slot[r] = b0 + r * ds;
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) slot[r] = fpad(base + r * S);
+        }
         double2 v[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = x[fpad(base + r * S)];
+        for (int r = 0; r < R; ++r) v[r] = x[slot[r]];
         if constexpr (kUnrolled) {
-            // twiddles w^r, w = e^{-2 pi i i/Lsub}, by a short product chain
+            // twiddles w^r (w = e^{-2 pi i i/Lsub}) from the stage's table
             double2 w[R];
-            w[0] = make_double2(1.0, 0.0);
-            if constexpr (R > 1) w[1] = tws[i];
+            const double2 *tw_i = tws + (size_t)i * (R - 1);
 #pragma unroll
-            for (int r = 2; r < R; ++r) w[r] = cmul(w[r - 1], w[1]);
+            for (int r = 1; r < R; ++r) w[r] = tw_i[r - 1];
             if constexpr (DIT) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r]);
@@ -183,7 +192,7 @@ __device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double
                 for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r]);
             }
 #pragma unroll
-            for (int r = 0; r < R; ++r) x[fpad(base + r * S)] = v[r];
+            for (int r = 0; r < R; ++r) x[slot[r]] = v[r];
         } else {
             if constexpr (DIT) {
 #pragma unroll

@@ -27,7 +27,7 @@ struct FftPlanHost {
     int stride[kMaxStages] = {0};
     int twoff[kMaxStages] = {0};  // offset of each stage's twiddles in twst
     std::vector<double> tw;     // 2*m: e^{-2 pi i k/m}
-    std::vector<double> twst;   // per stage: w^r, w = e^{-2 pi i i / (R S)}, i < S, 1 <= r < R
+    std::vector<double> twst;   // per stage: e^{-2 pi i i / (R S)}, i < S
     std::vector<int> perm;      // m: DIF output position of natural index k
     std::vector<double> chirp;  // 2*n (Bluestein)
     std::vector<double> bhat;   // 2*m (Bluestein), DIF output order, /m
@@ -117,13 +117,12 @@ inline void set_stages(FftPlanHost &p, const std::vector<int> &rad)
     p.twst.clear();
     for (int s = 0; s < p.nst; ++s) {
         p.twoff[s] = (int)(p.twst.size() / 2);
-        const int R = p.radix[s], Lsub = R * p.stride[s];
-        for (int i = 0; i < p.stride[s]; ++i)
-            for (int r = 1; r < R; ++r) {
-                auto w = root_ld((long long)i * r, Lsub);
-                p.twst.push_back((double)w.real());
-                p.twst.push_back((double)w.imag());
-            }
+        const int Lsub = p.radix[s] * p.stride[s];
+        for (int i = 0; i < p.stride[s]; ++i) {
+            auto w = root_ld(i, Lsub);
+            p.twst.push_back((double)w.real());
+            p.twst.push_back((double)w.imag());
+        }
     }
     if (p.twst.empty()) p.twst.assign(2, 0.0);
 }

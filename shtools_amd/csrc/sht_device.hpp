@@ -177,11 +177,16 @@ __device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double
 #pragma unroll
         for (int r = 0; r < R; ++r) v[r] = x[slot[r]];
         if constexpr (kUnrolled) {
-            // twiddles w^r (w = e^{-2 pi i i/Lsub}) from the stage's table
+            // twiddles w^r, w = e^{-2 pi i i/Lsub}: a product tree of depth 3
             double2 w[R];
-            const double2 *tw_i = tws + (size_t)i * (R - 1);
-#pragma unroll
-            for (int r = 1; r < R; ++r) w[r] = tw_i[r - 1];
+            w[0] = make_double2(1.0, 0.0);
+            if constexpr (R > 1) w[1] = tws[i];
+            if constexpr (R > 2) w[2] = cmul(w[1], w[1]);
+            if constexpr (R > 3) w[3] = cmul(w[2], w[1]);
+            if constexpr (R > 4) w[4] = cmul(w[2], w[2]);
+            if constexpr (R > 5) w[5] = cmul(w[4], w[1]);
+            if constexpr (R > 6) w[6] = cmul(w[4], w[2]);
+            if constexpr (R > 7) w[7] = cmul(w[4], w[3]);
             if constexpr (DIT) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r]);

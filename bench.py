@@ -19,8 +19,31 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
+# profiling ids of the library's stages and the batched kernels behind them
 KERNEL_NAMES = {0: "k_pack_synth", 1: "k_synth_leg", 2: "k_synth_fft", 3: "k_anal_fft", 4: "k_anal_leg",
                 5: "k_unpack_anal"}
+BATCH_KERNELS = {"k_pack_synth": "kb_pack", "k_synth_leg": "kb_synth_leg", "k_synth_fft": "kb_synth_fft",
+                 "k_anal_fft": "kb_anal_fft", "k_anal_leg": "kb_anal_leg", "k_unpack_anal": "kb_unpack"}
+CALIB_BYTES = 1 << 30     # plain device-to-device copy used to scale the HBM counters
+
+
+def stage_model(L, sampling=2):
+    """Algorithmic work of each stage for one DH field of degree L, one
+    direction: (FP64 flops, HBM bytes).  Legendre stages are priced by the
+    recursion + accumulation flops; FFT stages by the grid they write (read)
+    plus the order-major ring sums they read (write); pack/unpack by the
+    coefficient array and the packed triangle."""
+    from shtools_amd.synthetic import dh_flops
+    n = 2 * L + 2
+    nlon = sampling * n
+    P = n // 2 + 1
+    leg, fft = dh_flops(L, sampling)
+    tri = (L + 1) * (L + 2) // 2
+    pack_b = 8.0 * 2 * (L + 1) ** 2 + 16.0 * tri
+    fft_b = 8.0 * n * nlon + 32.0 * P * (L + 1)
+    leg_b = 16.0 * tri + 32.0 * P * (L + 1)
+    return {"k_pack_synth": (0.0, pack_b), "k_synth_leg": (leg, leg_b), "k_synth_fft": (fft, fft_b),
+            "k_anal_fft": (fft, fft_b), "k_anal_leg": (leg, leg_b), "k_unpack_anal": (0.0, pack_b)}
 
 
 def parse():
@@ -32,8 +55,98 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--loop", action="store_true", help="time the per-field loop of the plain kernels instead")
     ap.add_argument("--no-loop", action="store_true", help="skip the per-field loop comparison")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure HBM traffic per kernel")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
+    ap.add_argument("--counter-pass", action="store_true",
+                    help="(internal) one untimed step then a calibration copy, for a rocprofv3 --pmc run")
     return ap.parse_args()
+
+
+def make_workload(nfields, dev):
+    import torch
+    from shtools_amd.synthetic import mixed_lmax_workload, power_law_cilm
+    spec, rng = mixed_lmax_workload(nfields=nfields)
+    host_cilm = [power_law_cilm(rng, L, s) for L, s in spec]
+    cilm = [torch.from_numpy(c).to(dev) for c in host_cilm]
+    return [L for L, _ in spec], host_cilm, cilm
+
+
+def counter_pass(args):
+    """Run under rocprofv3 --pmc: one batched step, then a plain copy of
+    CALIB_BYTES whose counter readings scale the others to bytes."""
+    import torch
+    from shtools_amd.batch import DHBatch
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    Ls, _, cilm = make_workload(args.nfields, dev)
+    batch = DHBatch(Ls, sampling=2, norm=1, csphase=1, device=dev)
+    grids = batch.makegrid(cilm)
+    batch.expand(grids)
+    src = torch.ones(CALIB_BYTES // 8, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+
+
+def run_counter(counter, nfields, timeout=240):
+    """One rocprofv3 --pmc pass of the counter pass; returns {kernel name:
+    summed KiB} and the reading of the last dispatch (the calibration copy)."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        raise RuntimeError("rocprofv3 not found")
+    out = tempfile.mkdtemp(prefix="shtg_pmc_")
+    cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "run", "--",
+           sys.executable, os.path.abspath(__file__), "--counter-pass", "--nfields", str(nfields)]
+    r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"rocprofv3 --pmc {counter} exited with {r.returncode}: {r.stderr[-400:]}")
+    path = None
+    for dp, _, fs in os.walk(out):
+        for f in fs:
+            if f.endswith("counter_collection.csv"):
+                path = os.path.join(dp, f)
+    if path is None:
+        raise RuntimeError(f"rocprofv3 --pmc {counter} wrote no counter_collection.csv")
+    per_kernel, last = {}, (-1, 0.0)
+    with open(path) as fh:
+        for row in csv.DictReader(fh):
+            if row["Counter_Name"] != counter:
+                continue
+            v = float(row["Counter_Value"])
+            name = row["Kernel_Name"]
+            per_kernel[name] = per_kernel.get(name, 0.0) + v
+            d = int(row["Dispatch_Id"])
+            if d > last[0]:
+                last = (d, v)
+    shutil.rmtree(out, ignore_errors=True)
+    return per_kernel, last[1]
+
+
+def measure_traffic(nfields):
+    """HBM bytes per step of each batched kernel from FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch), each scaled by the reading of a plain copy
+    of CALIB_BYTES; None if the profiler is unavailable."""
+    res = {}
+    try:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            per_kernel, calib = run_counter(counter, nfields)
+            if calib <= 0:
+                raise RuntimeError(f"{counter}: calibration copy read {calib}")
+            scale = CALIB_BYTES / calib
+            res[counter] = {"scale_bytes_per_unit": scale, "calib_kib": calib, "kernels": {}}
+            for stage, kname in BATCH_KERNELS.items():
+                tot = sum(v for k, v in per_kernel.items() if k.startswith(f"shtg::{kname}") or
+                          k.startswith(f"void shtg::{kname}"))
+                res[counter]["kernels"][stage] = tot * scale
+    except Exception as e:  # reported in the JSON line, never fatal
+        return None, str(e)
+    return res, None
 
 
 def main():
@@ -41,22 +154,31 @@ def main():
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.counter_pass:
+        counter_pass(args)
+        return
+
+    # counter passes first, in child processes, before this one touches the GPU
+    traffic, traffic_err = None, "skipped (--no-traffic)"
+    if not args.no_traffic and not args.loop:
+        t0 = time.perf_counter()
+        traffic, traffic_err = measure_traffic(args.nfields)
+        print(f"[bench] counter passes: {time.perf_counter() - t0:.1f} s"
+              + (f" ({traffic_err})" if traffic_err else ""), file=sys.stderr, flush=True)
+
     import numpy as np
     import torch
 
     from shtools_amd import _lib
     from shtools_amd import backend as B
-    from shtools_amd.synthetic import dh_flops, mixed_lmax_workload, power_law_cilm
+    from shtools_amd.batch import DHBatch
+    from shtools_amd.synthetic import dh_flops
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    spec, rng = mixed_lmax_workload(nfields=args.nfields)
-    host_cilm = [power_law_cilm(rng, L, s) for L, s in spec]
-    cilm = [torch.from_numpy(c).to(dev) for c in host_cilm]
-    Ls = [L for L, _ in spec]
+    Ls, host_cilm, cilm = make_workload(args.nfields, dev)
     print(f"[bench] {len(Ls)} fields, lmax {min(Ls)}..{max(Ls)}", file=sys.stderr, flush=True)
 
-    from shtools_amd.batch import DHBatch
     batch = DHBatch(Ls, sampling=2, norm=1, csphase=1, device=dev)
     grids = [torch.empty(batch.grid_shape(i), dtype=torch.float64, device=dev) for i in range(len(Ls))]
     outs = [torch.empty(batch.cilm_shape(i), dtype=torch.float64, device=dev) for i in range(len(Ls))]
@@ -81,29 +203,27 @@ def main():
         loop_step()
         torch.cuda.synchronize()
         l0 = time.perf_counter()
-        loop_outs = loop_step()
+        loop_step()
         torch.cuda.synchronize()
         loop_rate = len(cilm) / (time.perf_counter() - l0)
         print(f"[bench] per-field loop: {loop_rate:.1f} fields/s", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
-        outs = step()
-    torch.cuda.synchronize()
-    # parity sanity on the warm-up outputs (round trip, norm-wise)
-    rt = max(float(torch.linalg.norm(o - c) / torch.linalg.norm(c)) for o, c in zip(outs, cilm))
-    print(f"[bench] warmup done, worst round-trip err {rt:.2e}", file=sys.stderr, flush=True)
-
+        step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step()
+        res = step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = len(cilm) * args.steps / (t1 - t0)
     print(f"[bench] timed {args.steps} steps: {ms_per_step:.2f} ms/step", file=sys.stderr, flush=True)
+    # parity sanity on the last step's outputs (round trip, norm-wise)
+    rt = max(float(torch.linalg.norm(o - c) / torch.linalg.norm(c)) for o, c in zip(res, cilm))
+    print(f"[bench] worst round-trip err {rt:.2e}", file=sys.stderr, flush=True)
 
-    # per-kernel durations with HIP events on the launch stream
+    # per-stage durations with HIP events recorded on the launch stream
     lib = _lib.lib()
     lib.shtg_profile_enable(1)
     step()
@@ -113,16 +233,39 @@ def main():
     cnt = (ctypes.c_longlong * nk)()
     lib.shtg_profile_collect(ms, cnt, nk)
     lib.shtg_profile_enable(0)
-    kern = {KERNEL_NAMES.get(k, str(k)): (ms[k], cnt[k]) for k in range(nk) if cnt[k]}
+    kern = {KERNEL_NAMES.get(k, str(k)): ms[k] for k in range(nk) if cnt[k]}
+    print("[bench] stage ms: " + ", ".join(f"{k}={v:.2f}" for k, v in kern.items()), file=sys.stderr, flush=True)
+
+    # algorithmic work per stage over the batch, and the roofline of each
+    work = {}
+    for L in Ls:
+        for k, (f, b) in stage_model(L).items():
+            w = work.setdefault(k, [0.0, 0.0])
+            w[0] += f
+            w[1] += b
+    stages = {}
+    for k, t in kern.items():
+        f, b = work.get(k, (0.0, 0.0))
+        bound = "mfma" if k in ("k_synth_leg", "k_anal_leg") else "hbm"
+        if bound == "mfma":
+            ach, peak, unit = f / (t * 1e-3) / 1e12, PEAK_FP64_TFLOPS, "TFLOP/s"
+        else:
+            ach, peak, unit = b / (t * 1e-3) / 1e9, PEAK_HBM_GBS, "GB/s"
+        st = {"ms": t, "bound": bound, "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak,
+              "algo_flops": f, "algo_bytes": b}
+        if traffic:
+            st["hbm_bytes"] = (traffic["FETCH_SIZE"]["kernels"][k] + traffic["WRITE_SIZE"]["kernels"][k])
+        stages[k] = st
+    dom = max(stages, key=lambda k: stages[k]["ms"])
+    d = stages[dom]
+    roofline = {"bound": d["bound"], "achieved": d["achieved"], "peak": d["peak"], "unit": d["unit"],
+                "frac": d["frac"], "traffic": d.get("hbm_bytes"), "kernel": BATCH_KERNELS.get(dom, dom),
+                "launch": "one step's dispatches of the stage (FFT stages: one per LDS class)"}
+    if traffic_err:
+        roofline["traffic_note"] = traffic_err
     leg_flops = sum(dh_flops(L)[0] for L in Ls)
     fft_flops = sum(dh_flops(L)[1] for L in Ls)
-    algo = {"k_synth_leg": leg_flops, "k_anal_leg": leg_flops, "k_synth_fft": fft_flops, "k_anal_fft": fft_flops}
-    dom = max(kern, key=lambda k: kern[k][0])
-    dom_ms, dom_n = kern[dom]
-    achieved = algo.get(dom, 0.0) / (dom_ms * 1e-3) / 1e12 if dom in algo else None
     total_flops = 2 * (leg_flops + fft_flops)
-    print("[bench] kernels (ms total, launches): " + ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in kern.items()),
-          file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -166,14 +309,12 @@ def main():
                                "synthesis+analysis per step, "
                                + ("per-field loop of the plain kernels" if args.loop else "batched launches"),
                    "fields": len(Ls), "sampling": 2, "norm": 1, "csphase": 1},
-        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": (achieved / PEAK_FP64_TFLOPS) if achieved else None,
-                     "traffic": None},
+        "roofline": roofline,
         "model_tflops": total_flops * args.steps / (t1 - t0) / 1e12,
         "roundtrip_err": rt,
         "loop_fields_per_s": loop_rate,
         "batched_vs_loop": (value / loop_rate) if loop_rate else None,
-        "kernels_ms": {k: v[0] for k, v in kern.items()},
+        "stages": stages,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

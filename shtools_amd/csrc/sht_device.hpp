@@ -155,27 +155,24 @@ __device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double
                                           const double2 *__restrict__ tws)
 {
     constexpr bool kUnrolled = (R == 2 || R == 3 || R == 4 || R == 5 || R == 8);
-    const int Lsub = R * S, step = m / Lsub, nb = m / R;
-    const float invS = 1.0f / (float)S;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-        // grp = b / S without an integer division (exact after correction)
-        int grp = (int)((float)b * invS);
+    const int Lsub = R * S, step = m / Lsub, nb = m / R, nt = blockDim.x;
+    // butterfly b = grp * S + i starts at element base = grp * Lsub + i; the
+    // thread's first (grp, i) come from one division, later ones by stepping
+    // b by nt = dg * S + di
+    int i, base;
+    {
+        const int b = threadIdx.x;
+        int grp = (int)((float)b * (1.0f / (float)S));
         if (grp * S > b) --grp;
         if ((grp + 1) * S <= b) ++grp;
-        const int i = b - grp * S, base = grp * Lsub + i;
-        // LDS slots of the R elements: linear in r when S is a multiple of 16
-        int slot[R];
-        if ((S & 15) == 0) {
-            const int b0 = fpad(base), ds = S + (S >> 4);
-#pragma unroll
-            for (int r = 0; r < R; ++r) slot[r] = b0 + r * ds;
-        } else {
-#pragma unroll
-            for (int r = 0; r < R; ++r) slot[r] = fpad(base + r * S);
-        }
+        i = b - grp * S;
+        base = grp * Lsub + i;
+    }
+    const int dg = nt / S, di = nt - dg * S, dbase = dg * Lsub + di, wrap = Lsub - S;
+    for (int b = threadIdx.x; b < nb; b += nt) {
         double2 v[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = x[slot[r]];
+        for (int r = 0; r < R; ++r) v[r] = x[fpad(base + r * S)];
         if constexpr (kUnrolled) {
             // twiddles w^r, w = e^{-2 pi i i/Lsub}: a product tree of depth 3
             double2 w[R];
@@ -197,7 +194,7 @@ __device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double
                 for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r]);
             }
 #pragma unroll
-            for (int r = 0; r < R; ++r) x[slot[r]] = v[r];
+            for (int r = 0; r < R; ++r) x[fpad(base + r * S)] = v[r];
         } else {
             if constexpr (DIT) {
 #pragma unroll
@@ -220,6 +217,12 @@ __device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double
                 }
                 x[fpad(base + k * S)] = acc;
             }
+        }
+        i += di;
+        base += dbase;
+        if (i >= S) {
+            i -= S;
+            base += wrap;
         }
     }
 }

@@ -62,6 +62,34 @@ int shtg_shexpand_glq(const double *grid, int grid_d0, int grid_d1, const double
                       const double *zero, int nzero, int norm, int csphase, int lmax_calc,
                       double *cilm, int cilm_d1, int cilm_d2, void *stream);
 
+/* Adjoints (transposes) of the four transforms, for gradients through them
+ * (no counterpart in the reference; the differentiable path of the
+ * transform API).  Each takes the argument list of its forward call with
+ * the roles of input and output exchanged, and applies the exact transpose
+ * of the linear map that call computes:
+ *   makegrid adjoint:  cilm_bar = S^T grid_bar, S the synthesis
+ *                      (MakeGridDH.f95:504-643 / MakeGridGLQ.f95:467-669),
+ *                      including the extended south-pole row and repeated
+ *                      column; entries of cilm_bar the synthesis does not
+ *                      read (l > lmax_calc, m > l, S_l0) are zero;
+ *   shexpand adjoint:  grid_bar = A^T cilm_bar, A the analysis
+ *                      (SHExpandDH.f95:463-702 / SHExpandGLQ.f95:445-689);
+ *                      grid entries the analysis does not read are zero.
+ * The kernels recompute the Legendre functions as the forward ones do. */
+int shtg_makegrid_dh_adjoint(const double *grid_bar, int grid_d0, int grid_d1, int lmax, int norm,
+                             int sampling, int csphase, int lmax_calc, int extend,
+                             double *cilm_bar, int cilm_d1, int cilm_d2, void *stream);
+int shtg_shexpand_dh_adjoint(const double *cilm_bar, int cilm_d1, int cilm_d2, int norm, int sampling,
+                             int csphase, int lmax_calc, double *grid_bar, int grid_d0, int grid_d1,
+                             void *stream);
+int shtg_makegrid_glq_adjoint(const double *grid_bar, int grid_d0, int grid_d1, int lmax,
+                              const double *zero, int nzero, int norm, int csphase, int lmax_calc,
+                              int extend, double *cilm_bar, int cilm_d1, int cilm_d2, void *stream);
+int shtg_shexpand_glq_adjoint(const double *cilm_bar, int cilm_d1, int cilm_d2, const double *w,
+                              int nw, const double *zero, int nzero, int norm, int csphase,
+                              int lmax_calc, double *grid_bar, int grid_d0, int grid_d1,
+                              void *stream);
+
 /* Batched DH transforms of many fields of uneven degree (no counterpart in
  * the reference, which transforms one field per call; semantics per field
  * are those of MakeGridDH/SHExpandDH with lmax_calc = lmax, extend = 0).
@@ -79,6 +107,12 @@ int shtg_batch_makegrid_dh(shtg_batch_plan *plan, const double *const *cilm, dou
                            void *stream);
 int shtg_batch_shexpand_dh(shtg_batch_plan *plan, const double *const *grid, double *const *cilm,
                            void *stream);
+/* Adjoints of the batched transforms (per field as shtg_makegrid_dh_adjoint
+ * and shtg_shexpand_dh_adjoint). */
+int shtg_batch_makegrid_dh_adjoint(shtg_batch_plan *plan, const double *const *grid_bar,
+                                   double *const *cilm_bar, void *stream);
+int shtg_batch_shexpand_dh_adjoint(shtg_batch_plan *plan, const double *const *cilm_bar,
+                                   double *const *grid_bar, void *stream);
 int shtg_batch_destroy(shtg_batch_plan *plan);
 
 /* Host helpers. */

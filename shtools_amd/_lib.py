@@ -21,6 +21,13 @@ SIGNATURES = {
                           _c_int, _vp],
     "shtg_shexpand_glq": [_vp, _c_int, _c_int, _dp, _c_int, _dp, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int,
                           _vp],
+    "shtg_makegrid_dh_adjoint": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int,
+                                 _c_int, _vp],
+    "shtg_shexpand_dh_adjoint": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _vp],
+    "shtg_makegrid_glq_adjoint": [_vp, _c_int, _c_int, _c_int, _dp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp,
+                                  _c_int, _c_int, _vp],
+    "shtg_shexpand_glq_adjoint": [_vp, _c_int, _c_int, _dp, _c_int, _dp, _c_int, _c_int, _c_int, _c_int, _vp,
+                                  _c_int, _c_int, _vp],
     "shtg_shglq": [_c_int, _dp, _dp],
     "shtg_dhaj": [_c_int, _c_int, _dp],
     "shtg_glqgridcoord": [_c_int, _c_int, _dp, _dp],
@@ -28,6 +35,8 @@ SIGNATURES = {
     "shtg_batch_create_dh": [_c_int, ctypes.POINTER(_c_int), _c_int, _c_int, _c_int, ctypes.POINTER(_vp)],
     "shtg_batch_makegrid_dh": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp],
     "shtg_batch_shexpand_dh": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp],
+    "shtg_batch_makegrid_dh_adjoint": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp],
+    "shtg_batch_shexpand_dh_adjoint": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp],
     "shtg_batch_destroy": [_vp],
     "shtg_profile_enable": [_c_int],
     "shtg_profile_collect": [_dp, ctypes.POINTER(ctypes.c_longlong), _c_int],

@@ -32,6 +32,7 @@ __all__ = [
     "SHToolsError", "MakeGridDH", "SHExpandDH", "MakeGridGLQ", "SHExpandGLQ", "SHGLQ", "GLQGridCoord", "DHaj",
     "PlmBar", "PlmSchmidt", "PlmON", "PLegendreA",
     "SHExpandDHC", "MakeGridDHC", "SHExpandGLQC", "MakeGridGLQC", "MakeGradientDH", "SHRotateRealCoef",
+    "MakeGridDH_adjoint", "SHExpandDH_adjoint", "MakeGridGLQ_adjoint", "SHExpandGLQ_adjoint",
 ]
 
 
@@ -287,3 +288,89 @@ SHExpandGLQC = _not_provided("SHExpandGLQC")
 MakeGridGLQC = _not_provided("MakeGridGLQC")
 MakeGradientDH = _not_provided("MakeGradientDH")
 SHRotateRealCoef = _not_provided("SHRotateRealCoef")
+
+
+# ------------------------------------------------------------ adjoints (device tensors only)
+
+def _dev_tensor(x, name):
+    torch = _torch()
+    if not (_is_tensor(x) and x.is_cuda and x.dtype == torch.float64):
+        raise TypeError(f"{name} must be a float64 tensor on a HIP device")
+    return x.contiguous()
+
+
+def MakeGridDH_adjoint(grid_bar, cilm_shape, lmax=None, norm=1, sampling=1, csphase=1, lmax_calc=None, extend=0):
+    """Transpose of MakeGridDH: cilm_bar (shape cilm_shape) = S^T grid_bar,
+    for the same arguments as the forward call that read a cilm of that
+    shape."""
+    torch = _torch()
+    g = _dev_tensor(grid_bar, "grid_bar")
+    d0, d1, d2 = (int(x) for x in cilm_shape)
+    if lmax is None:
+        lmax = d1 - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    cbar = torch.empty((d0, d1, d2), dtype=torch.float64, device=g.device)
+    if d0 > 2:
+        cbar[2:].zero_()
+    with torch.cuda.device(g.device):
+        st = _lib.lib().shtg_makegrid_dh_adjoint(g.data_ptr(), g.shape[0], g.shape[1], int(lmax), int(norm),
+                                                 int(sampling), int(csphase), int(lmax_calc), int(extend),
+                                                 cbar.data_ptr(), d1, d2, _stream(g.device))
+    _check(st)
+    return cbar
+
+
+def SHExpandDH_adjoint(cilm_bar, grid_shape, norm=1, sampling=1, csphase=1, lmax_calc=None):
+    """Transpose of SHExpandDH: grid_bar (shape grid_shape) = A^T cilm_bar."""
+    torch = _torch()
+    c = _dev_tensor(cilm_bar, "cilm_bar")
+    g0, g1 = (int(x) for x in grid_shape)
+    if lmax_calc is None:
+        lmax_calc = g0 // 2 - 1
+    gbar = torch.empty((g0, g1), dtype=torch.float64, device=c.device)
+    with torch.cuda.device(c.device):
+        st = _lib.lib().shtg_shexpand_dh_adjoint(c.data_ptr(), c.shape[1], c.shape[2], int(norm), int(sampling),
+                                                 int(csphase), int(lmax_calc), gbar.data_ptr(), g0, g1,
+                                                 _stream(c.device))
+    _check(st)
+    return gbar
+
+
+def MakeGridGLQ_adjoint(grid_bar, cilm_shape, zero, lmax=None, norm=1, csphase=1, lmax_calc=None, extend=0):
+    """Transpose of MakeGridGLQ."""
+    torch = _torch()
+    g = _dev_tensor(grid_bar, "grid_bar")
+    z = _host_f64(zero)
+    d0, d1, d2 = (int(x) for x in cilm_shape)
+    if lmax is None:
+        lmax = d1 - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    cbar = torch.empty((d0, d1, d2), dtype=torch.float64, device=g.device)
+    if d0 > 2:
+        cbar[2:].zero_()
+    with torch.cuda.device(g.device):
+        st = _lib.lib().shtg_makegrid_glq_adjoint(g.data_ptr(), g.shape[0], g.shape[1], int(lmax), _dptr(z), z.size,
+                                                  int(norm), int(csphase), int(lmax_calc), int(extend),
+                                                  cbar.data_ptr(), d1, d2, _stream(g.device))
+    _check(st)
+    return cbar
+
+
+def SHExpandGLQ_adjoint(cilm_bar, grid_shape, w, zero, norm=1, csphase=1, lmax_calc=None):
+    """Transpose of SHExpandGLQ."""
+    torch = _torch()
+    c = _dev_tensor(cilm_bar, "cilm_bar")
+    wq = _host_f64(w)
+    z = _host_f64(zero)
+    g0, g1 = (int(x) for x in grid_shape)
+    if lmax_calc is None:
+        lmax_calc = g0 - 1
+    gbar = torch.empty((g0, g1), dtype=torch.float64, device=c.device)
+    with torch.cuda.device(c.device):
+        st = _lib.lib().shtg_shexpand_glq_adjoint(c.data_ptr(), c.shape[1], c.shape[2], _dptr(wq), wq.size,
+                                                  _dptr(z), z.size, int(norm), int(csphase), int(lmax_calc),
+                                                  gbar.data_ptr(), g0, g1, _stream(c.device))
+    _check(st)
+    return gbar

@@ -87,3 +87,35 @@ class DHBatch:
         if st != 0:
             raise SHToolsError(_status_message(st))
         return out
+
+    def makegrid_adjoint(self, grid_bars, out=None):
+        """Transpose of makegrid per field: cilm_bar[i] = S_i^T grid_bar[i]."""
+        torch = self.torch
+        self._check(grid_bars, [self.grid_shape(i) for i in range(len(self.lmax))])
+        if out is None:
+            out = [torch.empty(self.cilm_shape(i), dtype=torch.float64, device=self.device)
+                   for i in range(len(self.lmax))]
+        self._check(out, [self.cilm_shape(i) for i in range(len(self.lmax))])
+        stream = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        with torch.cuda.device(self.device):
+            st = _lib.lib().shtg_batch_makegrid_dh_adjoint(self._plan, _ptr_array(grid_bars), _ptr_array(out),
+                                                           stream)
+        if st != 0:
+            raise SHToolsError(_status_message(st))
+        return out
+
+    def expand_adjoint(self, cilm_bars, out=None):
+        """Transpose of expand per field: grid_bar[i] = A_i^T cilm_bar[i]."""
+        torch = self.torch
+        self._check(cilm_bars, [self.cilm_shape(i) for i in range(len(self.lmax))])
+        if out is None:
+            out = [torch.empty(self.grid_shape(i), dtype=torch.float64, device=self.device)
+                   for i in range(len(self.lmax))]
+        self._check(out, [self.grid_shape(i) for i in range(len(self.lmax))])
+        stream = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        with torch.cuda.device(self.device):
+            st = _lib.lib().shtg_batch_shexpand_dh_adjoint(self._plan, _ptr_array(cilm_bars), _ptr_array(out),
+                                                           stream)
+        if st != 0:
+            raise SHToolsError(_status_message(st))
+        return out

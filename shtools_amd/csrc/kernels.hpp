@@ -274,10 +274,12 @@ __device__ __forceinline__ void anal_leg_tile(const double *__restrict__ al, dou
 }
 
 // Ring-pair synthesis FFT for slot s: fbuf holds rows of order m at stride P
-// (unscaled sums); u = sin(theta) of the slot gives the orders' rescale.
+// (unscaled sums); u = sin(theta) of the slot gives the orders' rescale; the
+// north and south rows are scaled by rwt (1 for synthesis, the quadrature
+// weights for the adjoint of analysis).
 __device__ __forceinline__ void synth_fft_tile(const FftDev &F, const double4 *__restrict__ fbuf, int P, int Lc, int s, double u,
                                int2 rw, int nrows, double *__restrict__ grid, long long ld, int ext_col,
-                               double2 *lds)
+                               double2 rwt, double2 *lds)
 {
     const int n = F.n;
     for (int k = threadIdx.x; k < F.m; k += blockDim.x) lds[fpad(k)] = make_double2(0.0, 0.0);
@@ -319,21 +321,23 @@ __device__ __forceinline__ void synth_fft_tile(const FftDev &F, const double4 *_
     const bool south = rw.y >= 0 && rw.y < nrows;
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
         const double2 v = fft_get(lds, F, j, j);
-        if (north) grid[rw.x * ld + j] = v.x;
-        if (south) grid[rw.y * ld + j] = -v.y;
+        const double gn = v.x * rwt.x, gs = -v.y * rwt.y;
+        if (north) grid[rw.x * ld + j] = gn;
+        if (south) grid[rw.y * ld + j] = gs;
         if (ext_col && j == 0) {
-            if (north) grid[rw.x * ld + n] = v.x;
-            if (south) grid[rw.y * ld + n] = -v.y;
+            if (north) grid[rw.x * ld + n] = gn;
+            if (south) grid[rw.y * ld + n] = gs;
         }
     }
 }
 
 // Ring-pair analysis FFT for slot s; w = (w_north, w_south) include the
 // quadrature weight and the reference's constant factors; the outputs carry
-// the order's rescale factor for the ring (sin(theta) = u).
+// the order's rescale factor for the ring (sin(theta) = u).  ext_col adds
+// column n to column 0 (the adjoint of a grid's repeated first column).
 __device__ __forceinline__ void anal_fft_tile(const FftDev &F, const double *__restrict__ grid, long long ld, int2 rw, int nrows,
                               double2 w, double u, int P, int Lc, int s, double4 *__restrict__ gbuf,
-                              double2 *lds)
+                              int ext_col, double2 *lds)
 {
     const int n = F.n;
     const bool north = rw.x >= 0 && rw.x < nrows;
@@ -343,6 +347,10 @@ __device__ __forceinline__ void anal_fft_tile(const FftDev &F, const double *__r
         if (k < n) {
             v.x = north ? grid[rw.x * ld + k] : 0.0;
             v.y = south ? grid[rw.y * ld + k] : 0.0;
+            if (ext_col && k == 0) {
+                if (north) v.x += grid[rw.x * ld + n];
+                if (south) v.y += grid[rw.y * ld + n];
+            }
             v = fft_prep_in(F, k, v);
         }
         lds[fpad(k)] = v;
@@ -399,26 +407,29 @@ __global__ __launch_bounds__(kWave) void k_synth_leg(const double *__restrict__ 
                                    P, sb * kk * kWave, fbuf + (size_t)m * P);
 }
 
-// rows[s] = (north row, south row or -1); rows >= nrows are skipped.
+// rows[s] = (north row, south row or -1); rows >= nrows are skipped;
+// roww[s]: row weights (nullptr: 1).
 __global__ __launch_bounds__(kFftMaxThreads) void k_synth_fft(FftDev F, const double4 *__restrict__ fbuf, int P, int Lc,
                                                         const double *__restrict__ ringu,
                                                         const int2 *__restrict__ rows, int nrows,
-                                                        double *__restrict__ grid, long long ld, int ext_col)
+                                                        double *__restrict__ grid, long long ld, int ext_col,
+                                                        const double2 *__restrict__ roww)
 {
     extern __shared__ double2 lds[];
     const int s = blockIdx.x;
-    synth_fft_tile(F, fbuf, P, Lc, s, ringu[s], rows[s], nrows, grid, ld, ext_col, lds);
+    synth_fft_tile(F, fbuf, P, Lc, s, ringu[s], rows[s], nrows, grid, ld, ext_col,
+                   roww ? roww[s] : make_double2(1.0, 1.0), lds);
 }
 
 __global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft(FftDev F, const double *__restrict__ grid, long long ld,
                                                        const double *__restrict__ ringu,
                                                        const int2 *__restrict__ rows, int nrows,
                                                        const double2 *__restrict__ wts, int P, int Lc,
-                                                       double4 *__restrict__ gbuf)
+                                                       double4 *__restrict__ gbuf, int ext_col)
 {
     extern __shared__ double2 lds[];
     const int s = blockIdx.x;
-    anal_fft_tile(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, Lc, s, gbuf, lds);
+    anal_fft_tile(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, Lc, s, gbuf, ext_col, lds);
 }
 
 // one single-wave workgroup per order m over ring pairs [s0, P)
@@ -445,6 +456,19 @@ __global__ __launch_bounds__(kThreads) void k_unpack_anal(const double2 *__restr
     unpack_tile(cbuf, post, Lt, Lc, cilm, d1, d2, tm * kTM, tl * kTL, t);
 }
 
+// Zero the part of a rows x cols array (leading dimension ld) outside its
+// leading r0 x c0 block.
+__global__ __launch_bounds__(kThreads) void k_zero_outside(double *__restrict__ a, int rows, int cols, long long ld,
+                                                           int r0, int c0)
+{
+    const long long total = (long long)rows * cols;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int r = (int)(e / cols), c = (int)(e - (long long)r * cols);
+        if (r >= r0 || c >= c0) a[r * ld + c] = 0.0;
+    }
+}
+
 // ============================================================= batched kernels
 
 // Per-field description of a batch (all fields: lmax_calc = lmax).
@@ -460,10 +484,12 @@ struct BField {
 };
 
 struct BTables {
-    const double *alpha, *s, *post, *seed;
+    const double *alpha, *s, *post, *seed;  // s: pack scale, post: unpack scale
     int Lt;
     const double *z, *u;
-    const double2 *w;
+    const double2 *w;     // analysis ring weights per slot
+    const double2 *roww;  // synthesis row weights per slot (nullptr: 1)
+    int s0;               // first slot of the analysis sums
     const int2 *rows;
     const FftDev *plans;
 };
@@ -516,7 +542,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_synth_fft(const BField *__r
     const FftDev &F = T.plans[f.fft];
     for (int s = it.y; s < it.y + nslots && s < f.P; ++s) {
         synth_fft_tile(F, fbuf + f.foff, f.P, f.L, s, T.u[f.geo + s], T.rows[f.geo + s], f.nlat, grid[it.x],
-                       f.nlong, 0, lds);
+                       f.nlong, 0, T.roww ? T.roww[f.geo + s] : make_double2(1.0, 1.0), lds);
         __syncthreads();
     }
 }
@@ -534,13 +560,14 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft(const BField *__re
     const FftDev &F = T.plans[f.fft];
     for (int s = it.y; s < it.y + nslots && s < f.P; ++s) {
         anal_fft_tile(F, grid[it.x], f.nlong, T.rows[f.geo + s], f.nlat, T.w[f.geo + s], T.u[f.geo + s], f.P, f.L,
-                      s, gbuf + f.foff, lds);
+                      s, gbuf + f.foff, 0, lds);
         __syncthreads();
     }
 }
 
 // items: (field, m); single-wave workgroups, one launch per ring pairs per
-// lane K.  The DH pole slot (0) carries no analysis weight and is skipped.
+// lane K.  Slots below T.s0 are skipped (the DH pole slot carries no
+// analysis weight; the adjoint of synthesis reads it).
 template <int K>
 __global__ __launch_bounds__(kWave) void kb_anal_leg(const BField *__restrict__ fields,
                                                      const int2 *__restrict__ items, BTables T,
@@ -551,7 +578,7 @@ __global__ __launch_bounds__(kWave) void kb_anal_leg(const BField *__restrict__ 
     const BField f = fields[it.x];
     const int m = it.y;
     anal_leg_tile<K>(T.alpha + tri_offd(m, T.Lt), T.seed[m], T.z + f.geo, gbuf + f.foff + (size_t)m * f.P,
-                     f.L - m, 1, f.P, cbuf + f.coff + tri_offd(m, f.L), red);
+                     f.L - m, T.s0, f.P, cbuf + f.coff + tri_offd(m, f.L), red);
 }
 
 // items: (field, first order, first degree) of a tile of the field's

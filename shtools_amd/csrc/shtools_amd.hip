@@ -64,6 +64,7 @@ struct Geometry {
     int s0 = 0;         // first slot with analysis weight (DH: 1, the pole carries none)
     double *z = nullptr, *u = nullptr;
     double2 *w = nullptr;  // analysis weights (north, south)
+    double2 *ones = nullptr;  // unit weights (adjoint of synthesis)
     int2 *rows = nullptr;
     FftDev fft{};
     bool ok = false;
@@ -151,7 +152,8 @@ static bool finish_geometry(Geometry &g, const std::vector<double> &z, const std
     g.u = dev_upload(u);
     g.w = reinterpret_cast<double2 *>(dev_upload(w));
     g.rows = reinterpret_cast<int2 *>(dev_upload(rows));
-    if (!g.z || !g.u || !g.w || !g.rows) return false;
+    g.ones = reinterpret_cast<double2 *>(dev_upload(std::vector<double>(w.size(), 1.0)));
+    if (!g.z || !g.u || !g.w || !g.rows || !g.ones) return false;
     FftPlanHost fp = make_fft_plan(g.nlong, lc);
     if (!upload_fft(fp, g.fft)) return false;
     g.ok = true;
@@ -290,6 +292,14 @@ static int grid_blocks(long long work)
     return (int)b;
 }
 
+// Zero what lies outside the leading r0 x c0 block of a rows x cols array.
+static void zero_outside(double *a, int rows, int cols, long long ld, int r0, int c0, hipStream_t st)
+{
+    if (r0 >= rows && c0 >= cols) return;
+    hipLaunchKernelGGL(k_zero_outside, dim3(grid_blocks((long long)rows * cols)), dim3(kThreads), 0, st, a, rows,
+                       cols, ld, r0, c0);
+}
+
 // ------------------------------------------------------------ profiling
 
 // When enabled, every kernel launch is bracketed by HIP events recorded on
@@ -323,10 +333,14 @@ struct ProfScope {
 
 // ------------------------------------------------------------ launchers
 
+// Synthesis of rows [0, nrows_out) (+ the repeated column with ext_col).
+// ctab scales the coefficients before the Legendre sums (rt.s; rt.post for
+// the adjoint of analysis) and roww the output rows (nullptr: 1).
 static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const DevSeeds &sd, const double *cilm,
                             int d1, int d2, int Lc, int nrows_out, int ext_col, double *grid, long long ld,
-                            hipStream_t st)
+                            hipStream_t st, const double *ctab = nullptr, const double2 *roww = nullptr)
 {
+    if (!ctab) ctab = rt.s;
     const size_t npack = (size_t)tri_size(Lc);
     const size_t nf = (size_t)(Lc + 1) * g.P;
     void *ws = nullptr;
@@ -338,7 +352,7 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
     {
         ProfScope ps(SHTG_K_PACK, st);
         const int ntm = (Lc + kTM) / kTM, ntl = (Lc + kTL) / kTL;
-        hipLaunchKernelGGL(k_pack_synth, dim3(ntm * ntl), dim3(kThreads), 0, st, cilm, d1, d2, rt.s, rt.L, cpack, Lc,
+        hipLaunchKernelGGL(k_pack_synth, dim3(ntm * ntl), dim3(kThreads), 0, st, cilm, d1, d2, ctab, rt.L, cpack, Lc,
                            ntm);
     }
     {
@@ -352,16 +366,25 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         hipLaunchKernelGGL(k_synth_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, fbuf, g.P, Lc, g.u, g.rows,
-                           nrows_out, grid, ld, ext_col);
+                           nrows_out, grid, ld, ext_col, roww);
     }
     if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
     (void)hipFreeAsync(ws, st);
     return ST_OK;
 }
 
+// Analysis of rows [0, g.nlat).  The adjoint of synthesis passes unit
+// weights w, all slots (s0 = 0), the recursion's s as ptab (no
+// normalisation epilogue), the rows it wrote (nrows) and ext_col.
 static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const DevSeeds &sd, const double *grid,
-                           long long ld, int Lc, double *cilm, int d1, int d2, hipStream_t st)
+                           long long ld, int Lc, double *cilm, int d1, int d2, hipStream_t st,
+                           const double2 *w = nullptr, int s0 = -1, const double *ptab = nullptr, int nrows = -1,
+                           int ext_col = 0)
 {
+    if (!w) w = g.w;
+    if (s0 < 0) s0 = g.s0;
+    if (!ptab) ptab = rt.post;
+    if (nrows < 0) nrows = g.nlat;
     const size_t npack = (size_t)tri_size(Lc);
     const size_t nf = (size_t)(Lc + 1) * g.P;
     void *ws = nullptr;
@@ -373,19 +396,19 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
     {
         ProfScope ps(SHTG_K_ANAL_FFT, st);
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
-        hipLaunchKernelGGL(k_anal_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, grid, ld, g.u, g.rows, g.nlat,
-                           g.w, g.P, Lc, gbuf);
+        hipLaunchKernelGGL(k_anal_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, grid, ld, g.u, g.rows, nrows,
+                           w, g.P, Lc, gbuf, ext_col);
     }
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);
-        const int K = anal_lanes_per_slot(g.P - g.s0);
+        const int K = anal_lanes_per_slot(g.P - s0);
         auto kern = K == 1 ? k_anal_leg<1> : K == 2 ? k_anal_leg<2> : K == 4 ? k_anal_leg<4> : k_anal_leg<kAnalKMax>;
-        hipLaunchKernelGGL(kern, dim3(Lc + 1), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gbuf, Lc, g.s0,
+        hipLaunchKernelGGL(kern, dim3(Lc + 1), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gbuf, Lc, s0,
                            g.P, cbuf);
     }
     ProfScope ps_un(SHTG_K_UNPACK, st);
     const int ntm = (d2 + kTM - 1) / kTM, ntl = (d1 + kTL - 1) / kTL;
-    hipLaunchKernelGGL(k_unpack_anal, dim3(ntm * ntl), dim3(kThreads), 0, st, cbuf, rt.post, rt.L, Lc, cilm, d1, d2,
+    hipLaunchKernelGGL(k_unpack_anal, dim3(ntm * ntl), dim3(kThreads), 0, st, cbuf, ptab, rt.L, Lc, cilm, d1, d2,
                        ntm);
     if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
     (void)hipFreeAsync(ws, st);
@@ -420,7 +443,7 @@ struct BatchPlan {
     std::vector<FftGroup> fft;
     int fft_slots = 2;
     double *z = nullptr, *u = nullptr;
-    double2 *w = nullptr;
+    double2 *w = nullptr, *ones = nullptr;
     int2 *rows = nullptr;
     FftDev *plans = nullptr;
     std::vector<FftDev> hplans;
@@ -450,6 +473,7 @@ static void batch_free(BatchPlan *b)
     (void)hipFree(b->z);
     (void)hipFree(b->u);
     (void)hipFree(b->w);
+    (void)hipFree(b->ones);
     (void)hipFree(b->rows);
     (void)hipFree(b->plans);
     for (auto &p : b->hplans) {
@@ -524,9 +548,10 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
     b->z = dev_upload(z);
     b->u = dev_upload(u);
     b->w = reinterpret_cast<double2 *>(dev_upload(w));
+    b->ones = reinterpret_cast<double2 *>(dev_upload(std::vector<double>(w.size(), 1.0)));
     b->rows = reinterpret_cast<int2 *>(dev_upload(rows));
     b->plans = dev_upload(b->hplans);
-    if (!b->z || !b->u || !b->w || !b->rows || !b->plans) return ST_ALLOC;
+    if (!b->z || !b->u || !b->w || !b->ones || !b->rows || !b->plans) return ST_ALLOC;
 
     // fields and workspace offsets
     std::vector<BField> fields(nfields);
@@ -625,6 +650,8 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
     b->T.z = b->z;
     b->T.u = b->u;
     b->T.w = b->w;
+    b->T.roww = nullptr;
+    b->T.s0 = 1;
     b->T.rows = b->rows;
     b->T.plans = b->plans;
     for (auto &v : b->bound) v.assign(nfields, nullptr);
@@ -651,30 +678,34 @@ static int batch_bind(BatchPlan *b, int k, const void *const *p)
 // workgroups covering n items under xcd_item's mapping (multiple of 32)
 static int xcd_blocks(int n) { return ((n + 31) / 32) * 32; }
 
-static int batch_synthesis(BatchPlan *b, hipStream_t st)
+// Synthesis of every field with tables T (b->T; the adjoint of analysis
+// scales the coefficients by post and the rows by the analysis weights).
+static int batch_synthesis(BatchPlan *b, hipStream_t st, const BTables &T)
 {
     const double *const *cin = b->ptrs;
     double *const *gout = (double *const *)(b->ptrs + b->nf);
     {
         ProfScope ps(SHTG_K_PACK, st);
-        hipLaunchKernelGGL(kb_pack, dim3(b->n_pack), dim3(kThreads), 0, st, b->fields, b->items_pack, b->T, cin,
+        hipLaunchKernelGGL(kb_pack, dim3(b->n_pack), dim3(kThreads), 0, st, b->fields, b->items_pack, T, cin,
                            b->cbuf);
     }
     {
         ProfScope ps(SHTG_K_SYNTH_LEG, st);
-        hipLaunchKernelGGL(kb_synth_leg, dim3(b->n_sleg), dim3(kWave), 0, st, b->fields, b->items_sleg, b->T,
+        hipLaunchKernelGGL(kb_synth_leg, dim3(b->n_sleg), dim3(kWave), 0, st, b->fields, b->items_sleg, T,
                            b->cbuf, b->fbuf);
     }
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
         for (auto &g : b->fft)
             hipLaunchKernelGGL(kb_synth_fft, dim3(xcd_blocks(g.nitems)), dim3(g.threads), g.lds, st, b->fields,
-                               g.items, g.nitems, b->fft_slots, b->T, b->fbuf, gout);
+                               g.items, g.nitems, b->fft_slots, T, b->fbuf, gout);
     }
     return HIP_OK(hipGetLastError()) ? ST_OK : ST_OTHER;
 }
 
-static int batch_analysis(BatchPlan *b, hipStream_t st)
+// Analysis of every field with tables T (b->T; the adjoint of synthesis
+// uses unit weights, every slot and the recursion's s as epilogue).
+static int batch_analysis(BatchPlan *b, hipStream_t st, const BTables &T)
 {
     const double *const *gin = b->ptrs + 2 * (size_t)b->nf;
     double *const *cout = (double *const *)(b->ptrs + 3 * (size_t)b->nf);
@@ -682,41 +713,45 @@ static int batch_analysis(BatchPlan *b, hipStream_t st)
         ProfScope ps(SHTG_K_ANAL_FFT, st);
         for (auto &g : b->fft)
             hipLaunchKernelGGL(kb_anal_fft, dim3(xcd_blocks(g.nitems)), dim3(g.threads), g.lds, st, b->fields,
-                               g.items, g.nitems, b->fft_slots, b->T, gin, b->fbuf);
+                               g.items, g.nitems, b->fft_slots, T, gin, b->fbuf);
     }
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);
         if (b->n_aleg[0])
             hipLaunchKernelGGL(kb_anal_leg<8>, dim3(b->n_aleg[0]), dim3(kWave), 0, st, b->fields, b->items_aleg[0],
-                               b->T, b->fbuf, b->cbuf);
+                               T, b->fbuf, b->cbuf);
         if (b->n_aleg[1])
             hipLaunchKernelGGL(kb_anal_leg<4>, dim3(b->n_aleg[1]), dim3(kWave), 0, st, b->fields, b->items_aleg[1],
-                               b->T, b->fbuf, b->cbuf);
+                               T, b->fbuf, b->cbuf);
         if (b->n_aleg[2])
             hipLaunchKernelGGL(kb_anal_leg<2>, dim3(b->n_aleg[2]), dim3(kWave), 0, st, b->fields, b->items_aleg[2],
-                               b->T, b->fbuf, b->cbuf);
+                               T, b->fbuf, b->cbuf);
         if (b->n_aleg[3])
             hipLaunchKernelGGL(kb_anal_leg<1>, dim3(b->n_aleg[3]), dim3(kWave), 0, st, b->fields, b->items_aleg[3],
-                               b->T, b->fbuf, b->cbuf);
+                               T, b->fbuf, b->cbuf);
     }
     {
         ProfScope ps(SHTG_K_UNPACK, st);
-        hipLaunchKernelGGL(kb_unpack, dim3(b->n_unpack), dim3(kThreads), 0, st, b->fields, b->items_unpack, b->T,
+        hipLaunchKernelGGL(kb_unpack, dim3(b->n_unpack), dim3(kThreads), 0, st, b->fields, b->items_unpack, T,
                            b->cbuf, cout);
     }
     return HIP_OK(hipGetLastError()) ? ST_OK : ST_OTHER;
 }
 
-}  // namespace shtg
+// ------------------------------------------------------------ argument checks
 
-using namespace shtg;
+// Shapes of one plain transform after the reference's argument checks.
+struct Shape {
+    int lmax = 0;      // grid degree
+    int Lc = 0;        // degrees computed
+    int nrows = 0;     // grid rows written by synthesis (with extend)
+    int ext = 0;       // extend
+};
 
-extern "C" {
-
-int shtg_makegrid_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax, int norm, int sampling, int csphase,
-                     int lmax_calc, int extend, double *grid, int grid_d0, int grid_d1, void *stream)
+// src/MakeGridDH.f95:107-251
+static int check_makegrid_dh(int cilm_d1, int cilm_d2, int lmax, int norm, int sampling, int csphase,
+                             int lmax_calc, int extend, int grid_d0, int grid_d1, Shape &sh)
 {
-    // argument checks in the order of src/MakeGridDH.f95:107-251
     if (lmax < 0) return ST_BOUNDS;
     const int n = 2 * lmax + 2;
     int nlong;
@@ -734,23 +769,17 @@ int shtg_makegrid_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax, int
     if (cilm_d2 - 1 < Lc) Lc = cilm_d2 - 1;
     if (cilm_d1 - 1 < Lc) Lc = cilm_d1 - 1;
     if (lmax_calc < Lc) Lc = lmax_calc;
-    if (!cilm || !grid) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
-    const int dev = current_device();
-    if (dev < 0) return ST_OTHER;
-    configure_kernels();
-    const Geometry *g = get_dh(dev, lmax, sampling);
-    const DevRecurrence *rt = get_recurrence(dev, norm, Lc);
-    const DevSeeds *sd = get_seeds(dev, norm, csphase, Lc);
-    if (!g || !rt || !sd) return ST_ALLOC;
-    return launch_synthesis(*g, *rt, *sd, cilm, cilm_d1, cilm_d2, Lc, nlat_out, extend, grid, grid_d1,
-                            (hipStream_t)stream);
+    sh.lmax = lmax;
+    sh.Lc = Lc;
+    sh.nrows = nlat_out;
+    sh.ext = extend;
+    return ST_OK;
 }
 
-int shtg_shexpand_dh(const double *grid, int grid_d0, int grid_d1, int norm, int sampling, int csphase,
-                     int lmax_calc, double *cilm, int cilm_d1, int cilm_d2, void *stream)
+// src/SHExpandDH.f95:118-270
+static int check_shexpand_dh(int grid_d0, int grid_d1, int norm, int sampling, int csphase, int lmax_calc,
+                             int cilm_d1, int cilm_d2, Shape &sh)
 {
-    // argument checks in the order of src/SHExpandDH.f95:118-270
     const int n = grid_d0;
     const int lmax = n / 2 - 1;
     if (lmax_calc > lmax) return ST_BOUNDS;
@@ -763,23 +792,16 @@ int shtg_shexpand_dh(const double *grid, int grid_d0, int grid_d1, int norm, int
     if (grid_d1 < nlong) return ST_DIMS;
     if (csphase != 1 && csphase != -1) return ST_BOUNDS;
     if (norm < 1 || norm > 4) return ST_BOUNDS;
-    if (!cilm || !grid) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
-    const int dev = current_device();
-    if (dev < 0) return ST_OTHER;
-    configure_kernels();
-    const Geometry *g = get_dh(dev, lmax, sampling);
-    const DevRecurrence *rt = get_recurrence(dev, norm, Lc);
-    const DevSeeds *sd = get_seeds(dev, norm, csphase, Lc);
-    if (!g || !rt || !sd) return ST_ALLOC;
-    return launch_analysis(*g, *rt, *sd, grid, grid_d1, Lc, cilm, cilm_d1, cilm_d2, (hipStream_t)stream);
+    sh.lmax = lmax;
+    sh.Lc = Lc;
+    sh.nrows = n;
+    return ST_OK;
 }
 
-int shtg_makegrid_glq(const double *cilm, int cilm_d1, int cilm_d2, int lmax, const double *zero, int nzero,
-                      int norm, int csphase, int lmax_calc, int extend, double *grid, int grid_d0, int grid_d1,
-                      void *stream)
+// src/MakeGridGLQ.f95:108-270
+static int check_makegrid_glq(int cilm_d1, int cilm_d2, int lmax, int nzero, const double *zero, int norm,
+                              int csphase, int lmax_calc, int extend, int grid_d0, int grid_d1, Shape &sh)
 {
-    // argument checks in the order of src/MakeGridGLQ.f95:108-270
     if (lmax < 0) return ST_BOUNDS;
     const int nlong = 2 * lmax + 1, nlat = lmax + 1;
     if (extend != 0 && extend != 1) return ST_BOUNDS;
@@ -795,24 +817,17 @@ int shtg_makegrid_glq(const double *cilm, int cilm_d1, int cilm_d2, int lmax, co
     if (cilm_d2 - 1 < Lc) Lc = cilm_d2 - 1;
     if (cilm_d1 - 1 < Lc) Lc = cilm_d1 - 1;
     if (lmax_calc < Lc) Lc = lmax_calc;
-    if (!cilm || !grid) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
-    const int dev = current_device();
-    if (dev < 0) return ST_OTHER;
-    configure_kernels();
-    const Geometry *g = get_glq(dev, lmax, zero, nullptr);
-    const DevRecurrence *rt = get_recurrence(dev, norm, Lc);
-    const DevSeeds *sd = get_seeds(dev, norm, csphase, Lc);
-    if (!g || !rt || !sd) return ST_ALLOC;
-    return launch_synthesis(*g, *rt, *sd, cilm, cilm_d1, cilm_d2, Lc, nlat, extend, grid, grid_d1,
-                            (hipStream_t)stream);
+    sh.lmax = lmax;
+    sh.Lc = Lc;
+    sh.nrows = nlat;
+    sh.ext = extend;
+    return ST_OK;
 }
 
-int shtg_shexpand_glq(const double *grid, int grid_d0, int grid_d1, const double *w, int nw, const double *zero,
-                      int nzero, int norm, int csphase, int lmax_calc, double *cilm, int cilm_d1, int cilm_d2,
-                      void *stream)
+// src/SHExpandGLQ.f95:100-250
+static int check_shexpand_glq(int grid_d0, int grid_d1, const double *w, int nw, const double *zero, int nzero,
+                              int norm, int csphase, int lmax_calc, int cilm_d1, int cilm_d2, Shape &sh)
 {
-    // argument checks in the order of src/SHExpandGLQ.f95:100-250
     const int lmax = grid_d0 - 1;
     if (lmax < 0) return ST_DIMS;
     if (lmax_calc > lmax) return ST_BOUNDS;
@@ -825,16 +840,174 @@ int shtg_shexpand_glq(const double *grid, int grid_d0, int grid_d1, const double
     if (nzero < lmax + 1) return ST_DIMS;
     if (norm < 1 || norm > 4) return ST_BOUNDS;
     if (csphase != 1 && csphase != -1) return ST_BOUNDS;
-    if (!cilm || !grid) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
+    sh.lmax = lmax;
+    sh.Lc = Lc;
+    sh.nrows = lmax + 1;
+    return ST_OK;
+}
+
+// Device tables of one call, under g_mu.
+struct CallTables {
+    const Geometry *g = nullptr;
+    const DevRecurrence *rt = nullptr;
+    const DevSeeds *sd = nullptr;
+};
+
+static int get_tables(bool dh, int lmax, int sampling, const double *zero, const double *wq, int norm,
+                      int csphase, int Lc, CallTables &t)
+{
     const int dev = current_device();
     if (dev < 0) return ST_OTHER;
     configure_kernels();
-    const Geometry *g = get_glq(dev, lmax, zero, w);
-    const DevRecurrence *rt = get_recurrence(dev, norm, Lc);
-    const DevSeeds *sd = get_seeds(dev, norm, csphase, Lc);
-    if (!g || !rt || !sd) return ST_ALLOC;
-    return launch_analysis(*g, *rt, *sd, grid, grid_d1, Lc, cilm, cilm_d1, cilm_d2, (hipStream_t)stream);
+    t.g = dh ? get_dh(dev, lmax, sampling) : get_glq(dev, lmax, zero, wq);
+    t.rt = get_recurrence(dev, norm, Lc);
+    t.sd = get_seeds(dev, norm, csphase, Lc);
+    if (!t.g || !t.rt || !t.sd) return ST_ALLOC;
+    return ST_OK;
+}
+
+// Adjoint of synthesis: cilm_bar = S^T grid_bar over the rows synthesis
+// writes (with extend the south pole row and the repeated column), all
+// slots with unit weight and the raw Legendre functions (no epilogue).
+static int adjoint_of_synthesis(const CallTables &t, const Shape &sh, const double *grid_bar, long long ld,
+                                double *cilm_bar, int d1, int d2, hipStream_t st)
+{
+    return launch_analysis(*t.g, *t.rt, *t.sd, grid_bar, ld, sh.Lc, cilm_bar, d1, d2, st, t.g->ones, 0, t.rt->s,
+                           sh.nrows, sh.ext);
+}
+
+// Adjoint of analysis: grid_bar = A^T cilm_bar = W S (post * cilm_bar) on
+// the rows analysis reads; the rest of the array is zeroed.
+static int adjoint_of_analysis(const CallTables &t, const Shape &sh, const double *cilm_bar, int d1, int d2,
+                               double *grid_bar, int gd0, int gd1, hipStream_t st)
+{
+    const Geometry &g = *t.g;
+    zero_outside(grid_bar, gd0, gd1, gd1, g.nlat, g.nlong, st);
+    return launch_synthesis(g, *t.rt, *t.sd, cilm_bar, d1, d2, sh.Lc, g.nlat, 0, grid_bar, gd1, st, t.rt->post,
+                            g.w);
+}
+
+}  // namespace shtg
+
+using namespace shtg;
+
+extern "C" {
+
+int shtg_makegrid_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax, int norm, int sampling, int csphase,
+                     int lmax_calc, int extend, double *grid, int grid_d0, int grid_d1, void *stream)
+{
+    Shape sh;
+    int st = check_makegrid_dh(cilm_d1, cilm_d2, lmax, norm, sampling, csphase, lmax_calc, extend, grid_d0, grid_d1,
+                               sh);
+    if (st) return st;
+    if (!cilm || !grid) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    CallTables t;
+    if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t))) return st;
+    return launch_synthesis(*t.g, *t.rt, *t.sd, cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend, grid, grid_d1,
+                            (hipStream_t)stream);
+}
+
+int shtg_shexpand_dh(const double *grid, int grid_d0, int grid_d1, int norm, int sampling, int csphase,
+                     int lmax_calc, double *cilm, int cilm_d1, int cilm_d2, void *stream)
+{
+    Shape sh;
+    int st = check_shexpand_dh(grid_d0, grid_d1, norm, sampling, csphase, lmax_calc, cilm_d1, cilm_d2, sh);
+    if (st) return st;
+    if (!cilm || !grid) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    CallTables t;
+    if ((st = get_tables(true, sh.lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t))) return st;
+    return launch_analysis(*t.g, *t.rt, *t.sd, grid, grid_d1, sh.Lc, cilm, cilm_d1, cilm_d2, (hipStream_t)stream);
+}
+
+int shtg_makegrid_glq(const double *cilm, int cilm_d1, int cilm_d2, int lmax, const double *zero, int nzero,
+                      int norm, int csphase, int lmax_calc, int extend, double *grid, int grid_d0, int grid_d1,
+                      void *stream)
+{
+    Shape sh;
+    int st = check_makegrid_glq(cilm_d1, cilm_d2, lmax, nzero, zero, norm, csphase, lmax_calc, extend, grid_d0,
+                                grid_d1, sh);
+    if (st) return st;
+    if (!cilm || !grid) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    CallTables t;
+    if ((st = get_tables(false, lmax, 0, zero, nullptr, norm, csphase, sh.Lc, t))) return st;
+    return launch_synthesis(*t.g, *t.rt, *t.sd, cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend, grid, grid_d1,
+                            (hipStream_t)stream);
+}
+
+int shtg_shexpand_glq(const double *grid, int grid_d0, int grid_d1, const double *w, int nw, const double *zero,
+                      int nzero, int norm, int csphase, int lmax_calc, double *cilm, int cilm_d1, int cilm_d2,
+                      void *stream)
+{
+    Shape sh;
+    int st = check_shexpand_glq(grid_d0, grid_d1, w, nw, zero, nzero, norm, csphase, lmax_calc, cilm_d1, cilm_d2,
+                                sh);
+    if (st) return st;
+    if (!cilm || !grid) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    CallTables t;
+    if ((st = get_tables(false, sh.lmax, 0, zero, w, norm, csphase, sh.Lc, t))) return st;
+    return launch_analysis(*t.g, *t.rt, *t.sd, grid, grid_d1, sh.Lc, cilm, cilm_d1, cilm_d2, (hipStream_t)stream);
+}
+
+int shtg_makegrid_dh_adjoint(const double *grid_bar, int grid_d0, int grid_d1, int lmax, int norm, int sampling,
+                             int csphase, int lmax_calc, int extend, double *cilm_bar, int cilm_d1, int cilm_d2,
+                             void *stream)
+{
+    Shape sh;
+    int st = check_makegrid_dh(cilm_d1, cilm_d2, lmax, norm, sampling, csphase, lmax_calc, extend, grid_d0, grid_d1,
+                               sh);
+    if (st) return st;
+    if (!cilm_bar || !grid_bar) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    CallTables t;
+    if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t))) return st;
+    return adjoint_of_synthesis(t, sh, grid_bar, grid_d1, cilm_bar, cilm_d1, cilm_d2, (hipStream_t)stream);
+}
+
+int shtg_shexpand_dh_adjoint(const double *cilm_bar, int cilm_d1, int cilm_d2, int norm, int sampling, int csphase,
+                             int lmax_calc, double *grid_bar, int grid_d0, int grid_d1, void *stream)
+{
+    Shape sh;
+    int st = check_shexpand_dh(grid_d0, grid_d1, norm, sampling, csphase, lmax_calc, cilm_d1, cilm_d2, sh);
+    if (st) return st;
+    if (!cilm_bar || !grid_bar) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    CallTables t;
+    if ((st = get_tables(true, sh.lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t))) return st;
+    return adjoint_of_analysis(t, sh, cilm_bar, cilm_d1, cilm_d2, grid_bar, grid_d0, grid_d1, (hipStream_t)stream);
+}
+
+int shtg_makegrid_glq_adjoint(const double *grid_bar, int grid_d0, int grid_d1, int lmax, const double *zero,
+                              int nzero, int norm, int csphase, int lmax_calc, int extend, double *cilm_bar,
+                              int cilm_d1, int cilm_d2, void *stream)
+{
+    Shape sh;
+    int st = check_makegrid_glq(cilm_d1, cilm_d2, lmax, nzero, zero, norm, csphase, lmax_calc, extend, grid_d0,
+                                grid_d1, sh);
+    if (st) return st;
+    if (!cilm_bar || !grid_bar) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    CallTables t;
+    if ((st = get_tables(false, lmax, 0, zero, nullptr, norm, csphase, sh.Lc, t))) return st;
+    return adjoint_of_synthesis(t, sh, grid_bar, grid_d1, cilm_bar, cilm_d1, cilm_d2, (hipStream_t)stream);
+}
+
+int shtg_shexpand_glq_adjoint(const double *cilm_bar, int cilm_d1, int cilm_d2, const double *w, int nw,
+                              const double *zero, int nzero, int norm, int csphase, int lmax_calc, double *grid_bar,
+                              int grid_d0, int grid_d1, void *stream)
+{
+    Shape sh;
+    int st = check_shexpand_glq(grid_d0, grid_d1, w, nw, zero, nzero, norm, csphase, lmax_calc, cilm_d1, cilm_d2,
+                                sh);
+    if (st) return st;
+    if (!cilm_bar || !grid_bar) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    CallTables t;
+    if ((st = get_tables(false, sh.lmax, 0, zero, w, norm, csphase, sh.Lc, t))) return st;
+    return adjoint_of_analysis(t, sh, cilm_bar, cilm_d1, cilm_d2, grid_bar, grid_d0, grid_d1, (hipStream_t)stream);
 }
 
 int shtg_shglq(int lmax, double *zero, double *w)
@@ -946,7 +1119,7 @@ int shtg_batch_makegrid_dh(shtg_batch_plan *plan, const double *const *cilm, dou
     int st = batch_bind(b, 0, (const void *const *)cilm);
     if (!st) st = batch_bind(b, 1, (const void *const *)grid);
     if (st) return st;
-    return batch_synthesis(b, (hipStream_t)stream);
+    return batch_synthesis(b, (hipStream_t)stream, b->T);
 }
 
 int shtg_batch_shexpand_dh(shtg_batch_plan *plan, const double *const *grid, double *const *cilm, void *stream)
@@ -957,7 +1130,38 @@ int shtg_batch_shexpand_dh(shtg_batch_plan *plan, const double *const *grid, dou
     int st = batch_bind(b, 2, (const void *const *)grid);
     if (!st) st = batch_bind(b, 3, (const void *const *)cilm);
     if (st) return st;
-    return batch_analysis(b, (hipStream_t)stream);
+    return batch_analysis(b, (hipStream_t)stream, b->T);
+}
+
+int shtg_batch_makegrid_dh_adjoint(shtg_batch_plan *plan, const double *const *grid_bar,
+                                   double *const *cilm_bar, void *stream)
+{
+    if (!plan || !cilm_bar || !grid_bar) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    int st = batch_bind(b, 2, (const void *const *)grid_bar);
+    if (!st) st = batch_bind(b, 3, (const void *const *)cilm_bar);
+    if (st) return st;
+    BTables T = b->T;
+    T.w = b->ones;
+    T.s0 = 0;
+    T.post = T.s;
+    return batch_analysis(b, (hipStream_t)stream, T);
+}
+
+int shtg_batch_shexpand_dh_adjoint(shtg_batch_plan *plan, const double *const *cilm_bar,
+                                   double *const *grid_bar, void *stream)
+{
+    if (!plan || !cilm_bar || !grid_bar) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    int st = batch_bind(b, 0, (const void *const *)cilm_bar);
+    if (!st) st = batch_bind(b, 1, (const void *const *)grid_bar);
+    if (st) return st;
+    BTables T = b->T;
+    T.s = T.post;
+    T.roww = b->w;
+    return batch_synthesis(b, (hipStream_t)stream, T);
 }
 
 int shtg_profile_enable(int on)

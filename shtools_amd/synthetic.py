@@ -46,3 +46,24 @@ def dh_flops(lmax, sampling=2):
 def transform_bytes(lmax, nlat, nlon):
     """Algorithmic HBM bytes of one direction: coefficients once, grid once."""
     return 8.0 * (2 * (lmax + 1) ** 2 + nlat * nlon)
+
+
+def lsq_fit_workload(batch, seed=4, noise=0.01):
+    """The differentiated least-squares workload on a DHBatch plan: per field a
+    truth with power spectrum (l+1)^-3 (Var C_lm = (l+1)^-3/(2l+1)), data =
+    MakeGridDH(truth) + Gaussian noise of sigma = noise * grid RMS, and the
+    per-ring DH weights w_lat = DHaj(n).  Returns (truths, data, w_lat) as
+    device tensors (w_lat for the first field's grid)."""
+    import torch
+    from . import backend
+    rng = np.random.default_rng(seed)
+    dev = batch.device
+    truth = [torch.from_numpy(power_law_cilm(rng, L, -3.0)).to(dev) for L in batch.lmax]
+    grids = batch.makegrid(truth)
+    data = []
+    for g in grids:
+        rms = float(torch.sqrt((g * g).mean()))
+        data.append(g + noise * rms * torch.from_numpy(rng.standard_normal(tuple(g.shape))).to(dev))
+    n = 2 * batch.lmax[0] + 2
+    w = torch.from_numpy(backend.DHaj(n)).to(dev)
+    return truth, data, w

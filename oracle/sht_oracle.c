@@ -549,27 +549,43 @@ int or_makegriddh(const double *cilm, int d1, int d2, int lmax, int norm, int sa
     c2r(plan, coef, buf, row);
     for (int k = 0; k < nlong; k++) G(i_eq - 1, k) = row[k];
 
-    /* ring pairs: MakeGridDH.f95:504-639 */
-    for (int i = 1; i <= i_eq - 1; i++) {
-        int i_s = 2 * i_eq - i;
-        double theta = pi * (double)(i - 1) / (double)n;
-        double z = cos(theta);
-        double u = sqrt((1.0 - z) * (1.0 + z));
-        for (int k = 0; k < 4 * (lmax + 2); k++) cr[k] = 0.0;
-        synth_ring(&t, norm, csphase, cf, z, u, cr, ci, csr, csi, &c0, &c0s);
-        cr[0] = c0; ci[0] = 0.0;
-        for (int m = 1; m <= lmax; m++) { cr[m] = cr[m] / 2.0; ci[m] = ci[m] / 2.0; }
-        pack_half(coef, cr, ci, lmax + 1, ncl);
-        c2r(plan, coef, buf, row);
-        for (int k = 0; k < nlong; k++) G(i - 1, k) = row[k];
-        if (!(i == 1 && extend == 0)) {
-            csr[0] = c0s; csi[0] = 0.0;
-            for (int m = 1; m <= lmax; m++) { csr[m] = csr[m] / 2.0; csi[m] = csi[m] / 2.0; }
-            pack_half(coef, csr, csi, lmax + 1, ncl);
-            c2r(plan, coef, buf, row);
-            for (int k = 0; k < nlong; k++) G(i_s - 1, k) = row[k];
+    /* ring pairs: MakeGridDH.f95:504-639.  Rings are independent, so they
+     * are spread over OpenMP threads (each with its own FFT plan and
+     * buffers); the arithmetic of every ring is unchanged. */
+#pragma omp parallel
+    {
+        fftplan *tplan = plan_create(nlong);
+        cplx *tcoef = (cplx *)malloc(sizeof(cplx) * ncl);
+        cplx *tbuf = (cplx *)malloc(sizeof(cplx) * nlong);
+        double *trow = (double *)malloc(sizeof(double) * nlong);
+        double *tcr = (double *)malloc(sizeof(double) * 4 * (lmax + 2));
+        double *tci = tcr + (lmax + 2), *tcsr = tci + (lmax + 2), *tcsi = tcsr + (lmax + 2);
+#pragma omp for schedule(dynamic, 1)
+        for (int i = 1; i <= i_eq - 1; i++) {
+            int i_s = 2 * i_eq - i;
+            double theta = pi * (double)(i - 1) / (double)n;
+            double z = cos(theta);
+            double u = sqrt((1.0 - z) * (1.0 + z));
+            double d0, d0s;
+            for (int k = 0; k < 4 * (lmax + 2); k++) tcr[k] = 0.0;
+            synth_ring(&t, norm, csphase, cf, z, u, tcr, tci, tcsr, tcsi, &d0, &d0s);
+            tcr[0] = d0; tci[0] = 0.0;
+            for (int m = 1; m <= lmax; m++) { tcr[m] = tcr[m] / 2.0; tci[m] = tci[m] / 2.0; }
+            pack_half(tcoef, tcr, tci, lmax + 1, ncl);
+            c2r(tplan, tcoef, tbuf, trow);
+            for (int k = 0; k < nlong; k++) G(i - 1, k) = trow[k];
+            if (!(i == 1 && extend == 0)) {
+                tcsr[0] = d0s; tcsi[0] = 0.0;
+                for (int m = 1; m <= lmax; m++) { tcsr[m] = tcsr[m] / 2.0; tcsi[m] = tcsi[m] / 2.0; }
+                pack_half(tcoef, tcsr, tcsi, lmax + 1, ncl);
+                c2r(tplan, tcoef, tbuf, trow);
+                for (int k = 0; k < nlong; k++) G(i_s - 1, k) = trow[k];
+            }
         }
+        free(tcoef); free(tbuf); free(trow); free(tcr);
+        plan_destroy(tplan);
     }
+    (void)c0s;
     if (extend == 1)
         for (int i = 0; i < nlat_out; i++) G(i, nlong_out - 1) = G(i, 0);
 #undef G
@@ -613,6 +629,18 @@ int or_makegridglq(const double *cilm, int d1, int d2, int lmax, const double *z
     double *row = (double *)malloc(sizeof(double) * nlong);
     double *cr = (double *)malloc(sizeof(double) * 4 * (lmax + 2));
     double *ci = cr + (lmax + 2), *csr = ci + (lmax + 2), *csi = csr + (lmax + 2);
+    free(coef); free(buf); free(row); free(cr);
+    plan_destroy(plan);
+    /* rings spread over OpenMP threads as in or_makegriddh */
+#pragma omp parallel
+    {
+    fftplan *plan = plan_create(nlong);
+    cplx *coef = (cplx *)malloc(sizeof(cplx) * ncl);
+    cplx *buf = (cplx *)malloc(sizeof(cplx) * nlong);
+    double *row = (double *)malloc(sizeof(double) * nlong);
+    double *cr = (double *)malloc(sizeof(double) * 4 * (lmax + 2));
+    double *ci = cr + (lmax + 2), *csr = ci + (lmax + 2), *csi = csr + (lmax + 2);
+#pragma omp for schedule(dynamic, 1)
     for (int i = 1; i <= (nlat + 1) / 2; i++) {
         double c0, c0s, rescalem;
         for (int k = 0; k < 4 * (lmax + 2); k++) cr[k] = 0.0;
@@ -641,11 +669,13 @@ int or_makegridglq(const double *cilm, int d1, int d2, int lmax, const double *z
             for (int k = 0; k < nlong; k++) G(i_s - 1, k) = row[k];
         }
     }
+    free(coef); free(buf); free(row); free(cr);
+    plan_destroy(plan);
+    }
     if (extend == 1)
         for (int i = 0; i < nlat; i++) G(i, nlong_out - 1) = G(i, 0);
 #undef G
-    free(coef); free(buf); free(row); free(cr); free(cf);
-    plan_destroy(plan);
+    free(cf);
     free_tables(&t);
     return 0;
 }

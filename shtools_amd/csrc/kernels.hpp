@@ -432,6 +432,38 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft(FftDev F, const dou
     anal_fft_tile(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, Lc, s, gbuf, ext_col, lds);
 }
 
+// Rings whose FFT work array does not fit the LDS of a CU: the same tiles
+// on a per-workgroup slice (ws_stride double2) of a global workspace, each
+// workgroup taking slots blockIdx.x, blockIdx.x + gridDim.x, ...
+__global__ __launch_bounds__(kFftMaxThreads) void k_synth_fft_ws(FftDev F, const double4 *__restrict__ fbuf, int P,
+                                                           int Lc, const double *__restrict__ ringu,
+                                                           const int2 *__restrict__ rows, int nrows,
+                                                           double *__restrict__ grid, long long ld, int ext_col,
+                                                           const double2 *__restrict__ roww, double2 *ws,
+                                                           long long ws_stride)
+{
+    double2 *x = ws + blockIdx.x * ws_stride;
+    for (int s = blockIdx.x; s < P; s += gridDim.x) {
+        synth_fft_tile(F, fbuf, P, Lc, s, ringu[s], rows[s], nrows, grid, ld, ext_col,
+                       roww ? roww[s] : make_double2(1.0, 1.0), x);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft_ws(FftDev F, const double *__restrict__ grid,
+                                                          long long ld, const double *__restrict__ ringu,
+                                                          const int2 *__restrict__ rows, int nrows,
+                                                          const double2 *__restrict__ wts, int P, int Lc,
+                                                          double4 *__restrict__ gbuf, int ext_col, double2 *ws,
+                                                          long long ws_stride)
+{
+    double2 *x = ws + blockIdx.x * ws_stride;
+    for (int s = blockIdx.x; s < P; s += gridDim.x) {
+        anal_fft_tile(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, Lc, s, gbuf, ext_col, x);
+        __syncthreads();
+    }
+}
+
 // one single-wave workgroup per order m over ring pairs [s0, P)
 template <int K>
 __global__ __launch_bounds__(kWave) void k_anal_leg(const double *__restrict__ alpha, int Lt,
@@ -562,6 +594,45 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft(const BField *__re
         anal_fft_tile(F, grid[it.x], f.nlong, T.rows[f.geo + s], f.nlat, T.w[f.geo + s], T.u[f.geo + s], f.P, f.L,
                       s, gbuf + f.foff, 0, lds);
         __syncthreads();
+    }
+}
+
+// Batched ring FFTs on a global workspace (fields whose work array exceeds
+// the LDS): workgroup b takes items b, b + gridDim.x, ...
+__global__ __launch_bounds__(kFftMaxThreads) void kb_synth_fft_ws(const BField *__restrict__ fields,
+                                                            const int2 *__restrict__ items, int nitems, int nslots,
+                                                            BTables T, const double4 *__restrict__ fbuf,
+                                                            double *const *grid, double2 *ws, long long ws_stride)
+{
+    double2 *x = ws + blockIdx.x * ws_stride;
+    for (int id = blockIdx.x; id < nitems; id += gridDim.x) {
+        const int2 it = items[id];
+        const BField f = fields[it.x];
+        const FftDev &F = T.plans[f.fft];
+        for (int s = it.y; s < it.y + nslots && s < f.P; ++s) {
+            synth_fft_tile(F, fbuf + f.foff, f.P, f.L, s, T.u[f.geo + s], T.rows[f.geo + s], f.nlat, grid[it.x],
+                           f.nlong, 0, T.roww ? T.roww[f.geo + s] : make_double2(1.0, 1.0), x);
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft_ws(const BField *__restrict__ fields,
+                                                           const int2 *__restrict__ items, int nitems, int nslots,
+                                                           BTables T, const double *const *grid,
+                                                           double4 *__restrict__ gbuf, double2 *ws,
+                                                           long long ws_stride)
+{
+    double2 *x = ws + blockIdx.x * ws_stride;
+    for (int id = blockIdx.x; id < nitems; id += gridDim.x) {
+        const int2 it = items[id];
+        const BField f = fields[it.x];
+        const FftDev &F = T.plans[f.fft];
+        for (int s = it.y; s < it.y + nslots && s < f.P; ++s) {
+            anal_fft_tile(F, grid[it.x], f.nlong, T.rows[f.geo + s], f.nlat, T.w[f.geo + s], T.u[f.geo + s], f.P,
+                          f.L, s, gbuf + f.foff, 0, x);
+            __syncthreads();
+        }
     }
 }
 

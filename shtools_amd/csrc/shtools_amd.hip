@@ -85,15 +85,27 @@ static void configure_kernels()
     kernels_configured = true;
 }
 
+// Tables outgrown by a larger degree stay allocated until the process ends:
+// work already enqueued and batch plans may still read them.  Degrees grow
+// geometrically, which bounds what is kept.
+static std::vector<void *> g_retired;
+
+static int grown_degree(int L, int old)
+{
+    int Lb = std::max(L, 64);
+    if (old >= 0) Lb = std::max(Lb, old + old / 2);
+    return Lb;
+}
+
 static const DevRecurrence *get_recurrence(int dev, int norm, int L)
 {
     auto &r = g_rec[{dev, norm}];
     if (r.L >= L) return &r;
-    int Lb = L < 64 ? 64 : L;
+    const int Lb = grown_degree(L, r.L);
     if (r.L >= 0) {
-        (void)hipFree(r.alpha);
-        (void)hipFree(r.s);
-        (void)hipFree(r.post);
+        g_retired.push_back(r.alpha);
+        g_retired.push_back(r.s);
+        g_retired.push_back(r.post);
         r = DevRecurrence{};
     }
     RecurrenceTables t = build_recurrence(norm, Lb);
@@ -109,17 +121,22 @@ static const DevSeeds *get_seeds(int dev, int norm, int csphase, int L)
 {
     auto &r = g_seeds[{dev, norm, csphase}];
     if (r.L >= L) return &r;
-    int Lb = L < 64 ? 64 : L;
-    if (r.L >= 0) { (void)hipFree(r.seed); r = DevSeeds{}; }
+    const int Lb = grown_degree(L, r.L);
+    if (r.L >= 0) { g_retired.push_back(r.seed); r = DevSeeds{}; }
     r.seed = dev_upload(build_seeds(norm, csphase, Lb));
     if (!r.seed) { r = DevSeeds{}; return nullptr; }
     r.L = Lb;
     return &r;
 }
 
+// Ring FFTs run in LDS when their work array fits a CU's LDS, otherwise on
+// a global workspace of kWsGroups slices.
+constexpr int kWsGroups = 512;
+static bool fft_in_lds(int m) { return (size_t)fft_lds_slots(m) * sizeof(double2) <= kMaxLds; }
+static long long ws_stride(int m) { return ((long long)fft_lds_slots(m) + 15) / 16 * 16; }
+
 static bool upload_fft(const FftPlanHost &h, FftDev &d)
 {
-    if ((size_t)fft_lds_slots(h.m) * sizeof(double2) > kMaxLds) return false;
     std::memset(&d, 0, sizeof(d));
     d.n = h.n;
     d.m = h.m;
@@ -343,11 +360,15 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
     if (!ctab) ctab = rt.s;
     const size_t npack = (size_t)tri_size(Lc);
     const size_t nf = (size_t)(Lc + 1) * g.P;
+    const bool lds_fft = fft_in_lds(g.fft.m);
+    const int nwg = std::min(g.P, kWsGroups);
+    const size_t nws = lds_fft ? 0 : (size_t)nwg * ws_stride(g.fft.m);
     void *ws = nullptr;
-    const size_t bytes = npack * sizeof(double2) + nf * sizeof(double4);
+    const size_t bytes = npack * sizeof(double2) + nf * sizeof(double4) + nws * sizeof(double2);
     if (!HIP_OK(hipMallocAsync(&ws, bytes, st))) return ST_ALLOC;
     double2 *cpack = (double2 *)ws;
     double4 *fbuf = (double4 *)((char *)ws + npack * sizeof(double2));
+    double2 *fws = (double2 *)((char *)fbuf + nf * sizeof(double4));
 
     {
         ProfScope ps(SHTG_K_PACK, st);
@@ -365,8 +386,12 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
-        hipLaunchKernelGGL(k_synth_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, fbuf, g.P, Lc, g.u, g.rows,
-                           nrows_out, grid, ld, ext_col, roww);
+        if (lds_fft)
+            hipLaunchKernelGGL(k_synth_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, fbuf, g.P, Lc, g.u,
+                               g.rows, nrows_out, grid, ld, ext_col, roww);
+        else
+            hipLaunchKernelGGL(k_synth_fft_ws, dim3(nwg), dim3(kFftMaxThreads), 0, st, g.fft, fbuf, g.P, Lc, g.u,
+                               g.rows, nrows_out, grid, ld, ext_col, roww, fws, ws_stride(g.fft.m));
     }
     if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
     (void)hipFreeAsync(ws, st);
@@ -387,17 +412,25 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
     if (nrows < 0) nrows = g.nlat;
     const size_t npack = (size_t)tri_size(Lc);
     const size_t nf = (size_t)(Lc + 1) * g.P;
+    const bool lds_fft = fft_in_lds(g.fft.m);
+    const int nwg = std::min(g.P, kWsGroups);
+    const size_t nws = lds_fft ? 0 : (size_t)nwg * ws_stride(g.fft.m);
     void *ws = nullptr;
-    const size_t bytes = npack * sizeof(double2) + nf * sizeof(double4);
+    const size_t bytes = npack * sizeof(double2) + nf * sizeof(double4) + nws * sizeof(double2);
     if (!HIP_OK(hipMallocAsync(&ws, bytes, st))) return ST_ALLOC;
     double2 *cbuf = (double2 *)ws;
     double4 *gbuf = (double4 *)((char *)ws + npack * sizeof(double2));
+    double2 *fws = (double2 *)((char *)gbuf + nf * sizeof(double4));
 
     {
         ProfScope ps(SHTG_K_ANAL_FFT, st);
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
-        hipLaunchKernelGGL(k_anal_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, grid, ld, g.u, g.rows, nrows,
-                           w, g.P, Lc, gbuf, ext_col);
+        if (lds_fft)
+            hipLaunchKernelGGL(k_anal_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, grid, ld, g.u, g.rows,
+                               nrows, w, g.P, Lc, gbuf, ext_col);
+        else
+            hipLaunchKernelGGL(k_anal_fft_ws, dim3(nwg), dim3(kFftMaxThreads), 0, st, g.fft, grid, ld, g.u, g.rows,
+                               nrows, w, g.P, Lc, gbuf, ext_col, fws, ws_stride(g.fft.m));
     }
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);
@@ -441,6 +474,9 @@ struct BatchPlan {
     int4 *items_unpack = nullptr;
     int n_unpack = 0;
     std::vector<FftGroup> fft;
+    FftGroup fft_ws;              // fields whose ring FFTs run on the global workspace
+    double2 *fws = nullptr;       // that workspace: kWsGroups slices of fws_stride
+    long long fws_stride = 0;
     int fft_slots = 2;
     double *z = nullptr, *u = nullptr;
     double2 *w = nullptr, *ones = nullptr;
@@ -470,6 +506,8 @@ static void batch_free(BatchPlan *b)
     for (auto p : b->items_aleg) (void)hipFree(p);
     (void)hipFree(b->items_unpack);
     for (auto &g : b->fft) (void)hipFree(g.items);
+    (void)hipFree(b->fft_ws.items);
+    (void)hipFree(b->fws);
     (void)hipFree(b->z);
     (void)hipFree(b->u);
     (void)hipFree(b->w);
@@ -621,9 +659,15 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
     // group's LDS is its largest need), then by threads per workgroup
     std::map<std::pair<int, int>, std::vector<W>> groups;  // (-(WGs per CU), -threads)
     std::map<std::pair<int, int>, size_t> group_lds;
+    std::vector<W> wsi;
     for (int i = 0; i < nfields; ++i) {
         const BField &f = fields[i];
         const size_t need = plan_lds[f.fft];
+        if (!fft_in_lds(b->hplans[f.fft].m)) {
+            for (int s = 0; s < f.P; s += b->fft_slots) wsi.push_back({(double)need, i, s, 0});
+            b->fws_stride = std::max(b->fws_stride, ws_stride(b->hplans[f.fft].m));
+            continue;
+        }
         const int per_cu = std::max(1, (int)(kMaxLds / need));
         const int thr = fft_threads(b->hplans[f.fft].m);
         const auto key = std::make_pair(-std::min(per_cu, 4), -thr);
@@ -641,6 +685,16 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
         g.threads = -kv.first.second;
         if (!g.items) return ST_ALLOC;
         b->fft.push_back(g);
+    }
+    if (!wsi.empty()) {
+        std::stable_sort(wsi.begin(), wsi.end(), by_work);
+        std::vector<int2> it;
+        for (auto &x : wsi) it.push_back(make_int2(x.a, x.b));
+        b->fft_ws.items = dev_upload(it);
+        b->fft_ws.nitems = (int)it.size();
+        b->fft_ws.threads = kFftMaxThreads;
+        if (!b->fft_ws.items) return ST_ALLOC;
+        if (!HIP_OK(hipMalloc(&b->fws, (size_t)kWsGroups * b->fws_stride * sizeof(double2)))) return ST_ALLOC;
     }
     b->T.alpha = rt->alpha;
     b->T.s = rt->s;
@@ -699,6 +753,10 @@ static int batch_synthesis(BatchPlan *b, hipStream_t st, const BTables &T)
         for (auto &g : b->fft)
             hipLaunchKernelGGL(kb_synth_fft, dim3(xcd_blocks(g.nitems)), dim3(g.threads), g.lds, st, b->fields,
                                g.items, g.nitems, b->fft_slots, T, b->fbuf, gout);
+        if (b->fft_ws.nitems)
+            hipLaunchKernelGGL(kb_synth_fft_ws, dim3(std::min(b->fft_ws.nitems, kWsGroups)), dim3(kFftMaxThreads), 0,
+                               st, b->fields, b->fft_ws.items, b->fft_ws.nitems, b->fft_slots, T, b->fbuf, gout,
+                               b->fws, b->fws_stride);
     }
     return HIP_OK(hipGetLastError()) ? ST_OK : ST_OTHER;
 }
@@ -714,6 +772,10 @@ static int batch_analysis(BatchPlan *b, hipStream_t st, const BTables &T)
         for (auto &g : b->fft)
             hipLaunchKernelGGL(kb_anal_fft, dim3(xcd_blocks(g.nitems)), dim3(g.threads), g.lds, st, b->fields,
                                g.items, g.nitems, b->fft_slots, T, gin, b->fbuf);
+        if (b->fft_ws.nitems)
+            hipLaunchKernelGGL(kb_anal_fft_ws, dim3(std::min(b->fft_ws.nitems, kWsGroups)), dim3(kFftMaxThreads), 0,
+                               st, b->fields, b->fft_ws.items, b->fft_ws.nitems, b->fft_slots, T, gin, b->fbuf,
+                               b->fws, b->fws_stride);
     }
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);

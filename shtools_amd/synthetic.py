@@ -67,3 +67,30 @@ def lsq_fit_workload(batch, seed=4, noise=0.01):
     n = 2 * batch.lmax[0] + 2
     w = torch.from_numpy(backend.DHaj(n)).to(dev)
     return truth, data, w
+
+
+def kaula_cilm(rng, lmax, amp=1e-5):
+    """Kaula-like spectrum: per-coefficient RMS amp / l^2 for l >= 2, zero for
+    degrees 0 and 1 (the geodesy convention); S_l0 = 0, zero for m > l."""
+    c = rng.standard_normal((2, lmax + 1, lmax + 1))
+    l = np.arange(lmax + 1, dtype=np.float64)
+    rms = np.zeros(lmax + 1)
+    rms[2:] = amp / l[2:] ** 2
+    c *= rms[None, :, None]
+    iu = np.triu_indices(lmax + 1, 1)
+    c[:, iu[0], iu[1]] = 0.0
+    c[1, :, 0] = 0.0
+    return c
+
+
+def upward_continuation(lmax, r_over_R=1.01):
+    """Per-degree filter (l+1) (R/r)^(l+2) of the high-degree workload."""
+    l = np.arange(lmax + 1, dtype=np.float64)
+    return (l + 1.0) * (1.0 / r_over_R) ** (l + 2.0)
+
+
+def high_degree_workload(nfields=8, lmin=2047, lmax=2799, seed=5):
+    """Degrees ~ U{lmin..lmax} (seeded) and Kaula-spectrum coefficients."""
+    rng = np.random.default_rng(seed)
+    Ls = [int(L) for L in rng.integers(lmin, lmax + 1, size=nfields)]
+    return Ls, [kaula_cilm(rng, L) for L in Ls]

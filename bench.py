@@ -263,6 +263,10 @@ def main():
                 "launch": "one step's dispatches of the stage (FFT stages: one per LDS class)"}
     if traffic_err:
         roofline["traffic_note"] = traffic_err
+    if traffic:
+        roofline["traffic_counters"] = {
+            c: {"kib_of_1GiB_copy": traffic[c]["calib_kib"], "bytes_per_kib": traffic[c]["scale_bytes_per_unit"],
+                "bytes": traffic[c]["kernels"][dom]} for c in ("FETCH_SIZE", "WRITE_SIZE")}
     leg_flops = sum(dh_flops(L)[0] for L in Ls)
     fft_flops = sum(dh_flops(L)[1] for L in Ls)
     total_flops = 2 * (leg_flops + fft_flops)

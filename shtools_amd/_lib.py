@@ -7,7 +7,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(_HERE, "libshtools_amd.so")
+# SHTOOLS_AMD_LIB selects another build of the library (same C ABI).
+LIBPATH = os.environ.get("SHTOOLS_AMD_LIB") or os.path.join(_HERE, "libshtools_amd.so")
 _lib = None
 
 _c_int = ctypes.c_int

@@ -75,10 +75,13 @@ __device__ __forceinline__ int fft_pos(const FftDev &P, int k)
     return (P.lc >= 0 && k > P.lc) ? k - P.n + P.m : k;
 }
 
-// LDS slot of FFT element e: one pad slot per 16 elements keeps the strided
-// accesses of every power-of-two stage free of bank conflicts.
-__device__ __forceinline__ int fpad(int e) { return e + (e >> 4); }
-__host__ __device__ __forceinline__ int fft_lds_slots(int m) { return m + (m >> 4) + 1; }
+// LDS slot of FFT element e (the one place that fixes the layout).  The
+// array is unpadded: the stages run the radix-8/4/2 factors first and the odd
+// radices last, so the late, small-stride stages access the LDS at odd
+// strides, which spread over the banks; padding (one slot per 16 elements)
+// broke those patterns and measured 6% slower.
+__device__ __forceinline__ int fpad(int e) { return e; }
+__host__ __device__ __forceinline__ int fft_lds_slots(int m) { return m + 1; }
 
 // Fully unrolled forward DFTs, y_k = sum_j v_j e^{-2 pi i jk/R}, in place.
 __device__ __forceinline__ void dft2(double2 &a, double2 &b)

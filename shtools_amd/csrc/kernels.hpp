@@ -273,49 +273,92 @@ __device__ __forceinline__ void anal_leg_tile(const double *__restrict__ al, dou
     }
 }
 
-// Ring-pair synthesis FFT for slot s: fbuf holds rows of order m at stride P
-// (unscaled sums); u = sin(theta) of the slot gives the orders' rescale; the
-// north and south rows are scaled by rwt (1 for synthesis, the quadrature
-// weights for the adjoint of analysis).
-__device__ __forceinline__ void synth_fft_tile(const FftDev &F, const double4 *__restrict__ fbuf, int P, int Lc, int s, double u,
-                               int2 rw, int nrows, double *__restrict__ grid, long long ld, int ext_col,
-                               double2 rwt, double2 *lds)
+// ---------------------------------------------------------------- ring FFTs
+//
+// A ring-pair FFT is split into load (global inputs into registers), fill
+// (all m work slots of the LDS array written once: data and zeros, no
+// barrier inside), and run (the DFT and the global outputs), so that a
+// workgroup transforming several slots can load the next slot's inputs
+// while the current one is in its FFT stages.
+
+// Synthesis inputs of one thread: ring sums of orders 2*threadIdx.x and
+// 2*threadIdx.x + 1 (orders beyond 2 * blockDim.x are read in the fill).
+constexpr int kSynthPre = 2;
+struct SynthRing {
+    double4 f[kSynthPre];
+};
+
+__device__ __forceinline__ void synth_fft_load(const double4 *__restrict__ fbuf, int P, int Lc, int s,
+                                               SynthRing &in)
 {
-    const int n = F.n;
-    for (int k = threadIdx.x; k < F.m; k += blockDim.x) lds[fpad(k)] = make_double2(0.0, 0.0);
-    __syncthreads();
-    // each thread takes 4 consecutive orders: one extended-range power, then
-    // products with u (as the reference's iterated rescalem)
-    for (int k0 = 4 * threadIdx.x; k0 <= Lc; k0 += 4 * blockDim.x) {
+    const int k0 = kSynthPre * threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kSynthPre; ++j) {
+        const int k = k0 + j;
+        in.f[j] = (k <= Lc) ? fbuf[(size_t)k * P + s] : make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+}
+
+// Hermitian fill of slot s (fbuf rows of order m at stride P hold unscaled
+// sums; u = sin(theta) of the slot gives the orders' rescale).
+__device__ __forceinline__ void synth_fft_fill(const FftDev &F, const SynthRing &in,
+                                               const double4 *__restrict__ fbuf, int P, int Lc, int s, double u,
+                                               double2 *lds)
+{
+    const int n = F.n, m = F.m;
+    // data occupy work positions [0, Lc] and [top - Lc, top): negative orders
+    // sit at the top of the array in pruned plans, at n - k otherwise
+    const int top = (F.lc >= 0) ? m : n;
+    for (int q = threadIdx.x; q < m; q += blockDim.x)
+        if ((q > Lc && q < top - Lc) || q >= top) lds[fpad(q)] = make_double2(0.0, 0.0);
+    // each thread takes kSynthPre consecutive orders: one extended-range
+    // power, then products with u (as the reference's iterated rescalem)
+    auto put = [&](int k, double r, double4 f) {
+        // Z_k = X_k + i Y_k with X (north) and Y (south) Hermitian:
+        // X_0 = C_N, X_k = (C_N - i S_N)/2 and X_{n-k} = conj(X_k); stored
+        // conjugated (inverse DFT via conj(F(conj Z))).
+        f.x *= r;
+        f.y *= r;
+        f.z *= r;
+        f.w *= r;
+        if (k == 0) {
+            lds[0] = fft_prep_in(F, 0, make_double2(f.x, -f.z));
+        } else {
+            lds[fpad(k)] = fft_prep_in(F, k, make_double2(0.5 * (f.x + f.w), -0.5 * (f.z - f.y)));
+            lds[fpad(fft_pos(F, n - k))] = fft_prep_in(F, n - k, make_double2(0.5 * (f.x - f.w), -0.5 * (f.y + f.z)));
+        }
+    };
+    {
+        const int k0 = kSynthPre * threadIdx.x;
         double r = 1.0;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < kSynthPre; ++j) {
             const int k = k0 + j;
-            if (k > Lc) break;
             if (k == 0) r = 1.0;
             else if (j == 0 || k == 1) r = scaled_upow(u, k);
             else r *= u;
-            // Z_k = X_k + i Y_k with X (north) and Y (south) Hermitian:
-            // X_0 = C_N, X_k = (C_N - i S_N)/2 and X_{n-k} = conj(X_k); stored
-            // conjugated (inverse DFT via conj(F(conj Z))).
-            double4 f = make_double4(0.0, 0.0, 0.0, 0.0);
-            if (r != 0.0) {
-                f = fbuf[(size_t)k * P + s];
-                f.x *= r;
-                f.y *= r;
-                f.z *= r;
-                f.w *= r;
-            }
-            if (k == 0) {
-                lds[0] = fft_prep_in(F, 0, make_double2(f.x, -f.z));
-            } else {
-                lds[fpad(k)] = fft_prep_in(F, k, make_double2(0.5 * (f.x + f.w), -0.5 * (f.z - f.y)));
-                lds[fpad(fft_pos(F, n - k))] =
-                    fft_prep_in(F, n - k, make_double2(0.5 * (f.x - f.w), -0.5 * (f.y + f.z)));
-            }
+            if (k <= Lc) put(k, r, in.f[j]);
         }
     }
-    __syncthreads();
+    for (int k0 = kSynthPre * (threadIdx.x + blockDim.x); k0 <= Lc; k0 += kSynthPre * blockDim.x) {
+        double r = 1.0;
+#pragma unroll
+        for (int j = 0; j < kSynthPre; ++j) {
+            const int k = k0 + j;
+            if (j == 0) r = scaled_upow(u, k);
+            else r *= u;
+            if (k <= Lc) put(k, r, fbuf[(size_t)k * P + s]);
+        }
+    }
+}
+
+// DFT of the filled array and the rows out; the north and south rows are
+// scaled by rwt (1 for synthesis, the quadrature weights for the adjoint of
+// analysis).
+__device__ __forceinline__ void synth_fft_run(const FftDev &F, int2 rw, int nrows, double *__restrict__ grid,
+                                              long long ld, int ext_col, double2 rwt, double2 *lds)
+{
+    const int n = F.n;
     fft_forward(lds, F, F.bhat);
     const bool north = rw.x >= 0 && rw.x < nrows;
     const bool south = rw.y >= 0 && rw.y < nrows;
@@ -331,31 +374,77 @@ __device__ __forceinline__ void synth_fft_tile(const FftDev &F, const double4 *_
     }
 }
 
-// Ring-pair analysis FFT for slot s; w = (w_north, w_south) include the
-// quadrature weight and the reference's constant factors; the outputs carry
-// the order's rescale factor for the ring (sin(theta) = u).  ext_col adds
-// column n to column 0 (the adjoint of a grid's repeated first column).
-__device__ __forceinline__ void anal_fft_tile(const FftDev &F, const double *__restrict__ grid, long long ld, int2 rw, int nrows,
-                              double2 w, double u, int P, int Lc, int s, double4 *__restrict__ gbuf,
-                              int ext_col, double2 *lds)
+__device__ __forceinline__ void synth_fft_tile(const FftDev &F, const double4 *__restrict__ fbuf, int P, int Lc,
+                                               int s, double u, int2 rw, int nrows, double *__restrict__ grid,
+                                               long long ld, int ext_col, double2 rwt, double2 *lds)
+{
+    SynthRing in;
+    synth_fft_load(fbuf, P, Lc, s, in);
+    synth_fft_fill(F, in, fbuf, P, Lc, s, u, lds);
+    __syncthreads();
+    synth_fft_run(F, rw, nrows, grid, ld, ext_col, rwt, lds);
+}
+
+// Analysis inputs of one thread: ring values at columns threadIdx.x +
+// j * blockDim.x, j < kAnalPre (later columns are read in the fill).
+constexpr int kAnalPre = 4;
+struct AnalRing {
+    double2 v[kAnalPre];
+};
+
+// (north, south) values of column k; ext_col adds column n to column 0 (the
+// adjoint of a grid's repeated first column).
+__device__ __forceinline__ double2 anal_column(const double *__restrict__ grid, long long ld, int2 rw, bool north,
+                                               bool south, int n, int ext_col, int k)
+{
+    double2 v;
+    v.x = north ? grid[rw.x * ld + k] : 0.0;
+    v.y = south ? grid[rw.y * ld + k] : 0.0;
+    if (ext_col && k == 0) {
+        if (north) v.x += grid[rw.x * ld + n];
+        if (south) v.y += grid[rw.y * ld + n];
+    }
+    return v;
+}
+
+__device__ __forceinline__ void anal_fft_load(const FftDev &F, const double *__restrict__ grid, long long ld, int2 rw,
+                                              int nrows, int ext_col, AnalRing &in)
 {
     const int n = F.n;
     const bool north = rw.x >= 0 && rw.x < nrows;
     const bool south = rw.y >= 0 && rw.y < nrows;
-    for (int k = threadIdx.x; k < F.m; k += blockDim.x) {
-        double2 v = make_double2(0.0, 0.0);
-        if (k < n) {
-            v.x = north ? grid[rw.x * ld + k] : 0.0;
-            v.y = south ? grid[rw.y * ld + k] : 0.0;
-            if (ext_col && k == 0) {
-                if (north) v.x += grid[rw.x * ld + n];
-                if (south) v.y += grid[rw.y * ld + n];
-            }
-            v = fft_prep_in(F, k, v);
-        }
-        lds[fpad(k)] = v;
+#pragma unroll
+    for (int j = 0; j < kAnalPre; ++j) {
+        const int k = threadIdx.x + j * blockDim.x;
+        in.v[j] = (k < n) ? anal_column(grid, ld, rw, north, south, n, ext_col, k) : make_double2(0.0, 0.0);
     }
-    __syncthreads();
+}
+
+__device__ __forceinline__ void anal_fft_fill(const FftDev &F, const AnalRing &in, const double *__restrict__ grid,
+                                              long long ld, int2 rw, int nrows, int ext_col, double2 *lds)
+{
+    const int n = F.n, m = F.m, nt = blockDim.x;
+#pragma unroll
+    for (int j = 0; j < kAnalPre; ++j) {
+        const int q = threadIdx.x + j * nt;
+        if (q < m) lds[fpad(q)] = (q < n) ? fft_prep_in(F, q, in.v[j]) : make_double2(0.0, 0.0);
+    }
+    const bool north = rw.x >= 0 && rw.x < nrows;
+    const bool south = rw.y >= 0 && rw.y < nrows;
+    for (int q = threadIdx.x + kAnalPre * nt; q < m; q += nt)
+        lds[fpad(q)] = (q < n) ? fft_prep_in(F, q, anal_column(grid, ld, rw, north, south, n, ext_col, q))
+                               : make_double2(0.0, 0.0);
+}
+
+// DFT of the filled array; w = (w_north, w_south) include the quadrature
+// weight and the reference's constant factors; the outputs carry the
+// order's rescale factor for the ring (sin(theta) = u).
+__device__ __forceinline__ void anal_fft_run(const FftDev &F, int2 rw, int nrows, double2 w, double u, int P, int Lc,
+                                             int s, double4 *__restrict__ gbuf, double2 *lds)
+{
+    const int n = F.n;
+    const bool north = rw.x >= 0 && rw.x < nrows;
+    const bool south = rw.y >= 0 && rw.y < nrows;
     fft_forward(lds, F, F.bhat2);
     const double wn = north ? w.x : 0.0, ws = south ? w.y : 0.0;
     for (int k0 = 4 * threadIdx.x; k0 <= Lc; k0 += 4 * blockDim.x) {
@@ -377,6 +466,17 @@ __device__ __forceinline__ void anal_fft_tile(const FftDev &F, const double *__r
                 make_double4((f1c + f2c) * r, (f1s + f2s) * r, (f1c - f2c) * r, (f1s - f2s) * r);
         }
     }
+}
+
+__device__ __forceinline__ void anal_fft_tile(const FftDev &F, const double *__restrict__ grid, long long ld,
+                                              int2 rw, int nrows, double2 w, double u, int P, int Lc, int s,
+                                              double4 *__restrict__ gbuf, int ext_col, double2 *lds)
+{
+    AnalRing in;
+    anal_fft_load(F, grid, ld, rw, nrows, ext_col, in);
+    anal_fft_fill(F, in, grid, ld, rw, nrows, ext_col, lds);
+    __syncthreads();
+    anal_fft_run(F, rw, nrows, w, u, P, Lc, s, gbuf, lds);
 }
 
 // =============================================================== plain kernels
@@ -572,9 +672,18 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_synth_fft(const BField *__r
     const int2 it = items[id];
     const BField f = fields[it.x];
     const FftDev &F = T.plans[f.fft];
-    for (int s = it.y; s < it.y + nslots && s < f.P; ++s) {
-        synth_fft_tile(F, fbuf + f.foff, f.P, f.L, s, T.u[f.geo + s], T.rows[f.geo + s], f.nlat, grid[it.x],
-                       f.nlong, 0, T.roww ? T.roww[f.geo + s] : make_double2(1.0, 1.0), lds);
+    const double4 *fb = fbuf + f.foff;
+    const int send = min(it.y + nslots, f.P);
+    // the next slot's ring sums are loaded while the current slot is in its
+    // FFT stages
+    SynthRing in;
+    synth_fft_load(fb, f.P, f.L, it.y, in);
+    for (int s = it.y; s < send; ++s) {
+        synth_fft_fill(F, in, fb, f.P, f.L, s, T.u[f.geo + s], lds);
+        __syncthreads();
+        if (s + 1 < send) synth_fft_load(fb, f.P, f.L, s + 1, in);
+        synth_fft_run(F, T.rows[f.geo + s], f.nlat, grid[it.x], f.nlong, 0,
+                      T.roww ? T.roww[f.geo + s] : make_double2(1.0, 1.0), lds);
         __syncthreads();
     }
 }
@@ -590,9 +699,18 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft(const BField *__re
     const int2 it = items[id];
     const BField f = fields[it.x];
     const FftDev &F = T.plans[f.fft];
-    for (int s = it.y; s < it.y + nslots && s < f.P; ++s) {
-        anal_fft_tile(F, grid[it.x], f.nlong, T.rows[f.geo + s], f.nlat, T.w[f.geo + s], T.u[f.geo + s], f.P, f.L,
-                      s, gbuf + f.foff, 0, lds);
+    const double *g = grid[it.x];
+    const int send = min(it.y + nslots, f.P);
+    // the next slot's rows are loaded while the current slot is in its FFT
+    // stages
+    AnalRing in;
+    anal_fft_load(F, g, f.nlong, T.rows[f.geo + it.y], f.nlat, 0, in);
+    for (int s = it.y; s < send; ++s) {
+        const int2 rw = T.rows[f.geo + s];
+        anal_fft_fill(F, in, g, f.nlong, rw, f.nlat, 0, lds);
+        __syncthreads();
+        if (s + 1 < send) anal_fft_load(F, g, f.nlong, T.rows[f.geo + s + 1], f.nlat, 0, in);
+        anal_fft_run(F, rw, f.nlat, T.w[f.geo + s], T.u[f.geo + s], f.P, f.L, s, gbuf + f.foff, lds);
         __syncthreads();
     }
 }

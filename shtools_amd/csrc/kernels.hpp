@@ -37,8 +37,8 @@ namespace shtg {
 constexpr int kThreads = 256;  // generic workgroups
 constexpr int kFftMaxThreads = 1024;  // ring-FFT workgroups: 256, 512 or 1024 threads
 constexpr int kWave = 64;      // Legendre workgroups are single waves
-constexpr int kSynthKMax = 4;  // ring pairs per lane in synthesis (1, 2 or 4)
-constexpr int kAnalKMax = 8;   // ring pairs per lane and pass in analysis (1, 2, 4 or 8)
+constexpr int kSynthKMax = 4;  // ring pairs per lane in synthesis (1 .. 4)
+constexpr int kAnalKMax = 8;   // ring pairs per lane and pass in analysis (1 .. 8)
 constexpr int kAnalLC = 8;     // degrees per reduction chunk in analysis
 constexpr int kRedStride = 65; // LDS row stride (doubles) of the analysis reduction
 
@@ -165,9 +165,11 @@ __device__ __forceinline__ void synth_leg_dispatch(int kk, const double *__restr
                                                    const double2 *__restrict__ cp, int jmax, int P, int slot0,
                                                    double4 *__restrict__ frow)
 {
+    static_assert(K == 4, "synthesis tiles are instantiated for 1 .. 4 ring pairs per lane");
     if (kk <= 1) synth_leg_tile<1>(al, sd, ringz, cp, jmax, P, slot0, frow);
     else if (kk == 2) synth_leg_tile<2>(al, sd, ringz, cp, jmax, P, slot0, frow);
-    else synth_leg_tile<K>(al, sd, ringz, cp, jmax, P, slot0, frow);
+    else if (kk == 3) synth_leg_tile<3>(al, sd, ringz, cp, jmax, P, slot0, frow);
+    else synth_leg_tile<4>(al, sd, ringz, cp, jmax, P, slot0, frow);
 }
 
 // Analysis sums of order m over ring pairs [s0, P) (one wave):
@@ -494,17 +496,20 @@ __global__ __launch_bounds__(kThreads) void k_pack_synth(const double *__restric
 }
 
 // grid: (Lc+1) * nsb single-wave workgroups, order m = blockIdx / nsb, slot
-// block of kk*64 ring pairs.
+// block sb of kSynthKMax*64 ring pairs; the last block takes as many ring
+// pairs per lane as its slots need.
 __global__ __launch_bounds__(kWave) void k_synth_leg(const double *__restrict__ alpha, int Lt,
                                                      const double *__restrict__ seed,
                                                      const double *__restrict__ ringz,
                                                      const double2 *__restrict__ cpack, int Lc, int P, int nsb,
-                                                     int kk, double4 *__restrict__ fbuf)
+                                                     double4 *__restrict__ fbuf)
 {
     const int m = blockIdx.x / nsb;
     const int sb = blockIdx.x - m * nsb;
+    const int slot0 = sb * kSynthKMax * kWave;
+    const int kk = min(kSynthKMax, (P - slot0 + kWave - 1) / kWave);
     synth_leg_dispatch<kSynthKMax>(kk, alpha + tri_offd(m, Lt), seed[m], ringz, cpack + tri_offd(m, Lc), Lc - m,
-                                   P, sb * kk * kWave, fbuf + (size_t)m * P);
+                                   P, slot0, fbuf + (size_t)m * P);
 }
 
 // rows[s] = (north row, south row or -1); rows >= nrows are skipped;

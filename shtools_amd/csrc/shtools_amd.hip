@@ -259,12 +259,30 @@ static int current_device()
     return d;
 }
 
-// ring pairs per lane and pass of an analysis wave for P slots: 1, 2, 4, 8
+// Ring pairs per lane of an analysis wave for P slots (1 .. kAnalKMax): the
+// fewest passes of at most kAnalKMax * 64 slots, with lanes packed as tightly
+// as those passes allow.
 static int anal_lanes_per_slot(int P)
 {
-    int k = 1;
-    while (k < kAnalKMax && k * kWave < P) k *= 2;
-    return k;
+    const int per = kAnalKMax * kWave;
+    const int np = std::max(1, (P + per - 1) / per);
+    return std::max(1, (P + np * kWave - 1) / (np * kWave));
+}
+
+// the analysis kernels by ring pairs per lane
+typedef decltype(&k_anal_leg<1>) AnalLegKernel;
+typedef decltype(&kb_anal_leg<1>) BatchAnalLegKernel;
+static AnalLegKernel anal_leg_kernel(int K)
+{
+    static const AnalLegKernel k[kAnalKMax] = {k_anal_leg<1>, k_anal_leg<2>, k_anal_leg<3>, k_anal_leg<4>,
+                                               k_anal_leg<5>, k_anal_leg<6>, k_anal_leg<7>, k_anal_leg<8>};
+    return k[std::min(std::max(K, 1), kAnalKMax) - 1];
+}
+static BatchAnalLegKernel batch_anal_leg_kernel(int K)
+{
+    static const BatchAnalLegKernel k[kAnalKMax] = {kb_anal_leg<1>, kb_anal_leg<2>, kb_anal_leg<3>, kb_anal_leg<4>,
+                                                    kb_anal_leg<5>, kb_anal_leg<6>, kb_anal_leg<7>, kb_anal_leg<8>};
+    return k[std::min(std::max(K, 1), kAnalKMax) - 1];
 }
 
 // Tuning of the ring-FFT launches: threads per workgroup by work length and
@@ -291,14 +309,6 @@ static int fft_slots_per_wg()
 {
     static const int v = std::max(1, std::min(16, env_int("SHTG_FFT_SLOTS", 2)));
     return v;
-}
-
-// ring pairs per lane of a synthesis wave for P slots: 1, 2 or 4
-static int synth_lanes_per_slot(int P)
-{
-    if (P <= kWave) return 1;
-    if (P <= 2 * kWave) return 2;
-    return kSynthKMax;
 }
 
 static int grid_blocks(long long work)
@@ -378,10 +388,9 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
     }
     {
         ProfScope ps(SHTG_K_SYNTH_LEG, st);
-        const int kk = synth_lanes_per_slot(g.P);
-        const int nsb = (g.P + kk * kWave - 1) / (kk * kWave);
+        const int nsb = (g.P + kSynthKMax * kWave - 1) / (kSynthKMax * kWave);
         hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nsb), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, cpack,
-                           Lc, g.P, nsb, kk, fbuf);
+                           Lc, g.P, nsb, fbuf);
     }
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
@@ -434,8 +443,7 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
     }
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);
-        const int K = anal_lanes_per_slot(g.P - s0);
-        auto kern = K == 1 ? k_anal_leg<1> : K == 2 ? k_anal_leg<2> : K == 4 ? k_anal_leg<4> : k_anal_leg<kAnalKMax>;
+        const AnalLegKernel kern = anal_leg_kernel(anal_lanes_per_slot(g.P - s0));
         hipLaunchKernelGGL(kern, dim3(Lc + 1), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gbuf, Lc, s0,
                            g.P, cbuf);
     }
@@ -469,8 +477,8 @@ struct BatchPlan {
     int n_pack = 0;
     int4 *items_sleg = nullptr;
     int n_sleg = 0;
-    int2 *items_aleg[4] = {nullptr, nullptr, nullptr, nullptr};  // by K = 8, 4, 2, 1
-    int n_aleg[4] = {0, 0, 0, 0};
+    int2 *items_aleg[kAnalKMax] = {};  // by ring pairs per lane K = index + 1
+    int n_aleg[kAnalKMax] = {};
     int4 *items_unpack = nullptr;
     int n_unpack = 0;
     std::vector<FftGroup> fft;
@@ -626,9 +634,11 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
                 if (l0 + kTL - 1 >= m0) wp.push_back({(double)f.L, i, m0, l0});
             }
         for (int m = 0; m <= f.L; ++m) {
-            const int kk = synth_lanes_per_slot(f.P), span = kk * kWave;
-            for (int s0 = 0; s0 < f.P; s0 += span)
-                ws.push_back({(double)(f.L - m + 1) * std::min(span, f.P - s0), i, m, s0 + (kk << 24)});
+            // blocks of kSynthKMax*64 slots; the last one packs its lanes tightly
+            for (int s0 = 0; s0 < f.P; s0 += kSynthKMax * kWave) {
+                const int kk = std::min(kSynthKMax, (f.P - s0 + kWave - 1) / kWave);
+                ws.push_back({(double)(f.L - m + 1) * std::min(kk * kWave, f.P - s0), i, m, s0 + (kk << 24)});
+            }
             wa.push_back({(double)(f.L - m + 1) * f.P, i, m, anal_lanes_per_slot(f.P - 1)});
         }
     }
@@ -637,14 +647,14 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
     std::stable_sort(ws.begin(), ws.end(), by_work);
     std::stable_sort(wa.begin(), wa.end(), by_work);
     std::vector<int4> ip, iu, is;
-    std::vector<int2> ia[4];
+    std::vector<int2> ia[kAnalKMax];
     for (auto &x : wp) ip.push_back(make_int4(x.a, x.b, x.c, 0));
     for (auto &x : ws) is.push_back(make_int4(x.a, x.b, x.c & 0xFFFFFF, x.c >> 24));
-    for (auto &x : wa) ia[x.c == 8 ? 0 : x.c == 4 ? 1 : x.c == 2 ? 2 : 3].push_back(make_int2(x.a, x.b));
+    for (auto &x : wa) ia[x.c - 1].push_back(make_int2(x.a, x.b));
     for (auto &x : wu) iu.push_back(make_int4(x.a, x.b, x.c, 0));
     b->items_pack = dev_upload(ip);
     b->items_sleg = dev_upload(is);
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < kAnalKMax; ++k) {
         b->n_aleg[k] = (int)ia[k].size();
         if (b->n_aleg[k] && !(b->items_aleg[k] = dev_upload(ia[k]))) return ST_ALLOC;
     }
@@ -779,18 +789,10 @@ static int batch_analysis(BatchPlan *b, hipStream_t st, const BTables &T)
     }
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);
-        if (b->n_aleg[0])
-            hipLaunchKernelGGL(kb_anal_leg<8>, dim3(b->n_aleg[0]), dim3(kWave), 0, st, b->fields, b->items_aleg[0],
-                               T, b->fbuf, b->cbuf);
-        if (b->n_aleg[1])
-            hipLaunchKernelGGL(kb_anal_leg<4>, dim3(b->n_aleg[1]), dim3(kWave), 0, st, b->fields, b->items_aleg[1],
-                               T, b->fbuf, b->cbuf);
-        if (b->n_aleg[2])
-            hipLaunchKernelGGL(kb_anal_leg<2>, dim3(b->n_aleg[2]), dim3(kWave), 0, st, b->fields, b->items_aleg[2],
-                               T, b->fbuf, b->cbuf);
-        if (b->n_aleg[3])
-            hipLaunchKernelGGL(kb_anal_leg<1>, dim3(b->n_aleg[3]), dim3(kWave), 0, st, b->fields, b->items_aleg[3],
-                               T, b->fbuf, b->cbuf);
+        for (int k = 0; k < kAnalKMax; ++k)
+            if (b->n_aleg[k])
+                hipLaunchKernelGGL(batch_anal_leg_kernel(k + 1), dim3(b->n_aleg[k]), dim3(kWave), 0, st, b->fields,
+                                   b->items_aleg[k], T, b->fbuf, b->cbuf);
     }
     {
         ProfScope ps(SHTG_K_UNPACK, st);

@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure HBM traffic per kernel")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
+    ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of HIP-graph replays")
+    ap.add_argument("--no-tune", action="store_true", help="skip the ring-FFT launch-shape autotuner")
     ap.add_argument("--counter-pass", action="store_true",
                     help="(internal) one untimed step then a calibration copy, for a rocprofv3 --pmc run")
     return ap.parse_args()
@@ -196,6 +198,12 @@ def main():
         return outs
 
     step = loop_step if args.loop else batch_step
+    shapes = None
+    if not args.loop and not args.no_tune:
+        t0 = time.perf_counter()
+        shapes = batch.tune(cilm, grids, outs)
+        print(f"[bench] tuned ring-FFT workgroups in {time.perf_counter() - t0:.1f} s: {shapes}",
+              file=sys.stderr, flush=True)
 
     # the per-field loop of the plain kernels, for the batched-vs-loop ratio
     loop_rate = None
@@ -211,9 +219,19 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    timed = step
+    if not args.loop and not args.no_graph:
+        graph = batch.capture(cilm, grids, outs)
+
+        def timed():
+            graph.replay()
+            return outs
+    for _ in range(args.warmup):
+        timed()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        res = step()
+        res = timed()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
@@ -311,7 +329,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{len(Ls)} DH2 fields, lmax~U{{63..1023}} seed 3, slope~U[-4,-2], norm 4pi, "
                                "synthesis+analysis per step, "
-                               + ("per-field loop of the plain kernels" if args.loop else "batched launches"),
+                               + ("per-field loop of the plain kernels" if args.loop else
+                                  "batched launches" + ("" if args.no_graph else " replayed as one HIP graph")),
                    "fields": len(Ls), "sampling": 2, "norm": 1, "csphase": 1},
         "roofline": roofline,
         "model_tflops": total_flops * args.steps / (t1 - t0) / 1e12,
@@ -319,6 +338,7 @@ def main():
         "loop_fields_per_s": loop_rate,
         "batched_vs_loop": (value / loop_rate) if loop_rate else None,
         "stages": stages,
+        "fft_workgroups": shapes,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -114,6 +114,17 @@ int shtg_batch_makegrid_dh_adjoint(shtg_batch_plan *plan, const double *const *g
 int shtg_batch_shexpand_dh_adjoint(shtg_batch_plan *plan, const double *const *cilm_bar,
                                    double *const *grid_bar, void *stream);
 int shtg_batch_destroy(shtg_batch_plan *plan);
+/* Launch-shape autotuner (no counterpart in the reference): times each
+ * ring-FFT group of the plan at 256, 512 and 1024 threads per workgroup
+ * (best of `reps` runs, synthesis and analysis separately) on the arrays
+ * bound by the plan's last makegrid and shexpand calls, and keeps the
+ * fastest.  Synchronises `stream`; overwrites those grids and the plan's
+ * workspace, so rerun the transforms afterwards.  Returns 5 if the plan has
+ * not been called in both directions yet. */
+int shtg_batch_tune(shtg_batch_plan *plan, int reps, void *stream);
+/* Workgroup sizes of the plan's ring-FFT groups (up to n of them); returns
+ * the number of groups. */
+int shtg_batch_get_shapes(shtg_batch_plan *plan, int *threads_synth, int *threads_anal, int n);
 
 /* Host helpers. */
 /* SHGLQ nodes and weights (include/shtools.h:142-148; src/SHGLQ.f95:187-192 ->

@@ -119,3 +119,44 @@ class DHBatch:
         if st != 0:
             raise SHToolsError(_status_message(st))
         return out
+
+    def capture(self, cilms, grids, outs, warmup=1):
+        """Capture one synthesis + analysis of the batch (cilms -> grids ->
+        outs) in a HIP graph and return it (``graph.replay()`` reruns the whole
+        step on the current stream).  The calls are made once eagerly first,
+        which binds the plan to these tensors; the graph holds no memset and
+        no allocation, and must be replayed with the same tensors alive."""
+        torch = self.torch
+        for _ in range(max(1, warmup)):
+            self.makegrid(cilms, out=grids)
+            self.expand(grids, out=outs)
+        torch.cuda.synchronize(self.device)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            self.makegrid(cilms, out=grids)
+            self.expand(grids, out=outs)
+        return graph
+
+    def tune(self, cilms, grids, outs, reps=3):
+        """Autotune the ring-FFT workgroup sizes of this plan on these arrays
+        (timed on the device), then recompute grids and outs.  Returns the
+        chosen (synthesis, analysis) threads per FFT group."""
+        torch = self.torch
+        self.makegrid(cilms, out=grids)
+        self.expand(grids, out=outs)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        with torch.cuda.device(self.device):
+            st = _lib.lib().shtg_batch_tune(self._plan, int(reps), stream)
+        if st != 0:
+            raise SHToolsError(_status_message(st))
+        self.makegrid(cilms, out=grids)
+        self.expand(grids, out=outs)
+        return self.shapes()
+
+    def shapes(self):
+        """(synthesis, analysis) threads per workgroup of each ring-FFT group."""
+        n = 64
+        ts = (ctypes.c_int * n)()
+        ta = (ctypes.c_int * n)()
+        ng = _lib.lib().shtg_batch_get_shapes(self._plan, ts, ta, n)
+        return [(ts[i], ta[i]) for i in range(min(ng, n))]

@@ -466,7 +466,8 @@ struct FftGroup {
     int2 *items = nullptr;
     int nitems = 0;
     size_t lds = 0;
-    int threads = 256;
+    int threads = 256;       // synthesis workgroup size (heuristic or tuned)
+    int threads_anal = 256;  // analysis workgroup size
 };
 
 struct BatchPlan {
@@ -692,7 +693,7 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
         g.items = dev_upload(it);
         g.nitems = (int)it.size();
         g.lds = group_lds[kv.first];
-        g.threads = -kv.first.second;
+        g.threads = g.threads_anal = -kv.first.second;
         if (!g.items) return ST_ALLOC;
         b->fft.push_back(g);
     }
@@ -780,7 +781,7 @@ static int batch_analysis(BatchPlan *b, hipStream_t st, const BTables &T)
     {
         ProfScope ps(SHTG_K_ANAL_FFT, st);
         for (auto &g : b->fft)
-            hipLaunchKernelGGL(kb_anal_fft, dim3(xcd_blocks(g.nitems)), dim3(g.threads), g.lds, st, b->fields,
+            hipLaunchKernelGGL(kb_anal_fft, dim3(xcd_blocks(g.nitems)), dim3(g.threads_anal), g.lds, st, b->fields,
                                g.items, g.nitems, b->fft_slots, T, gin, b->fbuf);
         if (b->fft_ws.nitems)
             hipLaunchKernelGGL(kb_anal_fft_ws, dim3(std::min(b->fft_ws.nitems, kWsGroups)), dim3(kFftMaxThreads), 0,
@@ -800,6 +801,61 @@ static int batch_analysis(BatchPlan *b, hipStream_t st, const BTables &T)
                            b->cbuf, cout);
     }
     return HIP_OK(hipGetLastError()) ? ST_OK : ST_OTHER;
+}
+
+// ------------------------------------------------------------ autotuning
+
+// Launch-shape autotuner of a batch plan: for every ring-FFT group, times the
+// synthesis and the analysis kernels at 256, 512 and 1024 threads per
+// workgroup (the group's items and LDS are unchanged) on the plan's bound
+// arrays, and keeps the fastest of each.  The runs overwrite the bound grids
+// and the plan's workspace; the caller reruns its transforms afterwards.
+static int batch_tune(BatchPlan *b, hipStream_t st, int reps)
+{
+    const double *const *gin = b->ptrs + 2 * (size_t)b->nf;
+    double *const *gout = (double *const *)(b->ptrs + b->nf);
+    for (int i = 0; i < b->nf; ++i)
+        if (!b->bound[1][i] || !b->bound[2][i]) return ST_OTHER;  // needs a prior synthesis and analysis
+    hipEvent_t e0, e1;
+    if (!HIP_OK(hipEventCreate(&e0)) || !HIP_OK(hipEventCreate(&e1))) return ST_OTHER;
+    int st_out = ST_OK;
+    for (auto &g : b->fft) {
+        for (int dir = 0; dir < 2; ++dir) {
+            float best = 1e30f;
+            int best_t = dir ? g.threads_anal : g.threads;
+            for (int t : {256, 512, 1024}) {
+                float tmin = 1e30f;
+                for (int r = 0; r < reps; ++r) {
+                    (void)hipEventRecord(e0, st);
+                    if (dir == 0)
+                        hipLaunchKernelGGL(kb_synth_fft, dim3(xcd_blocks(g.nitems)), dim3(t), g.lds, st, b->fields,
+                                           g.items, g.nitems, b->fft_slots, b->T, b->fbuf, gout);
+                    else
+                        hipLaunchKernelGGL(kb_anal_fft, dim3(xcd_blocks(g.nitems)), dim3(t), g.lds, st, b->fields,
+                                           g.items, g.nitems, b->fft_slots, b->T, gin, b->fbuf);
+                    (void)hipEventRecord(e1, st);
+                    float ms = 0.0f;
+                    if (!HIP_OK(hipGetLastError()) || !HIP_OK(hipEventSynchronize(e1)) ||
+                        !HIP_OK(hipEventElapsedTime(&ms, e0, e1))) {
+                        st_out = ST_OTHER;
+                        break;
+                    }
+                    tmin = std::min(tmin, ms);
+                }
+                if (st_out) break;
+                if (tmin < best) {
+                    best = tmin;
+                    best_t = t;
+                }
+            }
+            if (st_out) break;
+            (dir ? g.threads_anal : g.threads) = best_t;
+        }
+        if (st_out) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st_out;
 }
 
 // ------------------------------------------------------------ argument checks
@@ -1226,6 +1282,26 @@ int shtg_batch_shexpand_dh_adjoint(shtg_batch_plan *plan, const double *const *c
     T.s = T.post;
     T.roww = b->w;
     return batch_synthesis(b, (hipStream_t)stream, T);
+}
+
+int shtg_batch_tune(shtg_batch_plan *plan, int reps, void *stream)
+{
+    if (!plan || reps < 1) return ST_BOUNDS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    return batch_tune(reinterpret_cast<BatchPlan *>(plan), (hipStream_t)stream, reps);
+}
+
+int shtg_batch_get_shapes(shtg_batch_plan *plan, int *threads_synth, int *threads_anal, int n)
+{
+    if (!plan) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    const int ng = (int)b->fft.size();
+    for (int i = 0; i < n && i < ng; ++i) {
+        if (threads_synth) threads_synth[i] = b->fft[i].threads;
+        if (threads_anal) threads_anal[i] = b->fft[i].threads_anal;
+    }
+    return ng;
 }
 
 int shtg_profile_enable(int on)

@@ -1,0 +1,62 @@
+"""HIP-graph capture of a batched synthesis + analysis: every replay (with
+eager work and synchronisations in between) equals the eager result
+bitwise."""
+import numpy as np
+import pytest
+
+from helpers import rand_cilm
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batched_step_graph_replays(tmp_path):
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from shtools_amd.batch import DHBatch
+    rng = np.random.default_rng(21)
+    Ls = [17, 64, 5, 100, 64, 33]
+    batch = DHBatch(Ls, sampling=2, norm=1, csphase=1)
+    cilms = [torch.from_numpy(rand_cilm(rng, L)).cuda() for L in Ls]
+    grids = [torch.empty(batch.grid_shape(i), dtype=torch.float64, device="cuda") for i in range(len(Ls))]
+    outs = [torch.empty(batch.cilm_shape(i), dtype=torch.float64, device="cuda") for i in range(len(Ls))]
+    ref_g = [g.clone() for g in batch.makegrid(cilms)]
+    ref_c = [c.clone() for c in batch.expand(ref_g)]
+    graph = batch.capture(cilms, grids, outs)
+    for rep in range(4):
+        for g, c in zip(grids, outs):
+            g.fill_(np.nan)
+            c.fill_(np.nan)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        for i in range(len(Ls)):
+            assert torch.equal(grids[i], ref_g[i]), (rep, i)
+            assert torch.equal(outs[i], ref_c[i]), (rep, i)
+        # eager work between replays
+        _ = [x * 2.0 for x in cilms]
+        torch.cuda.synchronize()
+
+
+def test_batch_tune_keeps_results():
+    """Autotuning changes only workgroup sizes: results stay bitwise equal."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from shtools_amd.batch import DHBatch
+    rng = np.random.default_rng(22)
+    Ls = [300, 31, 700, 129, 64]
+    batch = DHBatch(Ls, sampling=2, norm=2, csphase=-1)
+    cilms = [torch.from_numpy(rand_cilm(rng, L)).cuda() for L in Ls]
+    ref_g = [g.clone() for g in batch.makegrid(cilms)]
+    ref_c = [c.clone() for c in batch.expand(ref_g)]
+    grids = [torch.empty_like(g) for g in ref_g]
+    outs = [torch.empty_like(c) for c in ref_c]
+    shapes = batch.tune(cilms, grids, outs, reps=2)
+    assert shapes and all(t in (256, 512, 1024) for pair in shapes for t in pair)
+    for i in range(len(Ls)):
+        assert torch.equal(grids[i], ref_g[i])
+        assert torch.equal(outs[i], ref_c[i])
+    g2 = batch.makegrid(cilms)
+    for i in range(len(Ls)):
+        assert torch.equal(g2[i], ref_g[i])

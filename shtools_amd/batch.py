@@ -37,9 +37,13 @@ class DHBatch:
 
     def __del__(self):
         plan = getattr(self, "_plan", None)
-        if plan:
-            _lib.lib().shtg_batch_destroy(plan)
-            self._plan = None
+        self._plan = None
+        # at interpreter shutdown the module globals may already be gone
+        if plan and _lib is not None and getattr(_lib, "lib", None) is not None:
+            try:
+                _lib.lib().shtg_batch_destroy(plan)
+            except Exception:
+                pass
 
     def grid_shape(self, i):
         n = 2 * self.lmax[i] + 2

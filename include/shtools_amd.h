@@ -62,6 +62,27 @@ int shtg_shexpand_glq(const double *grid, int grid_d0, int grid_d1, const double
                       const double *zero, int nzero, int norm, int csphase, int lmax_calc,
                       double *cilm, int cilm_d1, int cilm_d2, void *stream);
 
+/* Complex transforms: MakeGridDHC src/MakeGridDHC.f95:1-2 (f2py
+ * src/pyshtools.pyf:299-319), SHExpandDHC src/SHExpandDHC.f95:1-2
+ * (src/pyshtools.pyf:278-297), MakeGridGLQC src/MakeGridGLQC.f95:1-2
+ * (src/pyshtools.pyf:402-422), SHExpandGLQC src/SHExpandGLQC.f95:1-2
+ * (src/pyshtools.pyf:379-400), on-the-fly branches.  Arguments as for the
+ * real transforms; cilm is a complex [2][d1][d2] array ([0] orders +m, [1]
+ * orders -m) and the grid a complex [d0][d1] array, both as interleaved
+ * (re, im) doubles (the layout of numpy/torch complex128). */
+int shtg_makegrid_dhc(const double *cilm, int cilm_d1, int cilm_d2, int lmax, int norm,
+                      int sampling, int csphase, int lmax_calc, int extend,
+                      double *grid, int grid_d0, int grid_d1, void *stream);
+int shtg_shexpand_dhc(const double *grid, int grid_d0, int grid_d1, int norm, int sampling,
+                      int csphase, int lmax_calc, double *cilm, int cilm_d1, int cilm_d2,
+                      void *stream);
+int shtg_makegrid_glqc(const double *cilm, int cilm_d1, int cilm_d2, int lmax,
+                       const double *zero, int nzero, int norm, int csphase, int lmax_calc,
+                       int extend, double *grid, int grid_d0, int grid_d1, void *stream);
+int shtg_shexpand_glqc(const double *grid, int grid_d0, int grid_d1, const double *w, int nw,
+                       const double *zero, int nzero, int norm, int csphase, int lmax_calc,
+                       double *cilm, int cilm_d1, int cilm_d2, void *stream);
+
 /* Adjoints (transposes) of the four transforms, for gradients through them
  * (no counterpart in the reference; the differentiable path of the
  * transform API).  Each takes the argument list of its forward call with

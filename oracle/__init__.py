@@ -12,7 +12,11 @@ functions in all four normalisations with and without the Condon-Shortley
 phase, mpmath DH weights and Gauss-Legendre nodes, brute-force direct
 summation of small expansions at every grid node, quadrature exactness,
 round trips, numpy.fft for the longitude DFT and the known answers of the
-reference's normalisation notebook.
+reference's normalisation notebook.  The complex transforms (MakeGridDHC,
+SHExpandDHC, MakeGridGLQC, SHExpandGLQC) are pinned through the real ones:
+for coefficients of a real field, c_lm = (C_lm - i S_lm)/2 and
+c_l,-m = (-1)^m (C_lm + i S_lm)/2 (c_l0 = C_l0), and the complex maps are
+complex-linear, which fixes them completely.
 """
 import ctypes
 import os
@@ -57,6 +61,10 @@ def lib():
         L.or_shexpanddh.argtypes = [dp, i, i, i, i, i, i, dp, i, i]
         L.or_makegridglq.argtypes = [dp, i, i, i, dp, i, i, i, i, i, dp, i, i]
         L.or_shexpandglq.argtypes = [dp, i, i, i, dp, i, dp, i, i, i, i, dp, i, i]
+        L.or_makegriddhc.argtypes = [dp, i, i, i, i, i, i, i, i, dp, i, i]
+        L.or_shexpanddhc.argtypes = [dp, i, i, i, i, i, i, dp, i, i]
+        L.or_makegridglqc.argtypes = [dp, i, i, i, dp, i, i, i, i, i, dp, i, i]
+        L.or_shexpandglqc.argtypes = [dp, i, i, i, dp, i, dp, i, i, i, i, dp, i, i]
         L.or_r2c.argtypes = [i, dp, dp]
         L.or_c2r.argtypes = [i, dp, dp]
         lp = ctypes.POINTER(ctypes.c_longlong)
@@ -152,6 +160,59 @@ def SHExpandGLQ(gridglq, w, zero, norm=1, csphase=1, lmax_calc=None):
     _check(lib().or_shexpandglq(_p(grid), grid.shape[0], grid.shape[1], lmax, _p(w), w.size,
                                 _p(zero), zero.size, norm, csphase, lmax_calc, _p(cilm),
                                 lmax_calc + 1, lmax_calc + 1))
+    return cilm
+
+
+def MakeGridDHC(cilm, lmax=None, norm=1, sampling=1, csphase=1, lmax_calc=None, extend=0):
+    cilm = np.ascontiguousarray(cilm, dtype=np.complex128)
+    if lmax is None:
+        lmax = cilm.shape[1] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    g0 = 2 * (lmax + 1) + extend
+    g1 = sampling * 2 * (lmax + 1) + extend
+    grid = np.zeros((g0, g1), dtype=np.complex128)
+    _check(lib().or_makegriddhc(_p(cilm), cilm.shape[1], cilm.shape[2], lmax, norm, sampling,
+                                csphase, lmax_calc, extend, _p(grid), g0, g1))
+    return grid
+
+
+def SHExpandDHC(griddh, norm=1, sampling=1, csphase=1, lmax_calc=None):
+    grid = np.ascontiguousarray(griddh, dtype=np.complex128)
+    n = grid.shape[0]
+    if lmax_calc is None:
+        lmax_calc = n // 2 - 1
+    cilm = np.zeros((2, lmax_calc + 1, lmax_calc + 1), dtype=np.complex128)
+    _check(lib().or_shexpanddhc(_p(grid), n, grid.shape[1], norm, sampling, csphase, lmax_calc,
+                                _p(cilm), lmax_calc + 1, lmax_calc + 1))
+    return cilm
+
+
+def MakeGridGLQC(cilm, zero, lmax=None, norm=1, csphase=1, lmax_calc=None, extend=0):
+    cilm = np.ascontiguousarray(cilm, dtype=np.complex128)
+    zero = np.ascontiguousarray(zero, dtype=np.float64)
+    if lmax is None:
+        lmax = cilm.shape[1] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    g0, g1 = lmax + 1, 2 * lmax + 1 + extend
+    grid = np.zeros((g0, g1), dtype=np.complex128)
+    _check(lib().or_makegridglqc(_p(cilm), cilm.shape[1], cilm.shape[2], lmax, _p(zero), zero.size,
+                                 norm, csphase, lmax_calc, extend, _p(grid), g0, g1))
+    return grid
+
+
+def SHExpandGLQC(gridglq, w, zero, norm=1, csphase=1, lmax_calc=None):
+    grid = np.ascontiguousarray(gridglq, dtype=np.complex128)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    zero = np.ascontiguousarray(zero, dtype=np.float64)
+    lmax = grid.shape[0] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    cilm = np.zeros((2, lmax_calc + 1, lmax_calc + 1), dtype=np.complex128)
+    _check(lib().or_shexpandglqc(_p(grid), grid.shape[0], grid.shape[1], lmax, _p(w), w.size,
+                                 _p(zero), zero.size, norm, csphase, lmax_calc, _p(cilm),
+                                 lmax_calc + 1, lmax_calc + 1))
     return cilm
 
 

@@ -344,6 +344,14 @@ static double pmm_step(int norm, int phase, double *pmm, int m, const double *sq
     }
 }
 
+/* Scaled seed chain start of the complex transforms, whose functions carry
+ * no sqrt(2) for m > 0: MakeGridDHC.f95:485-489, SHExpandDHC.f95:495-498. */
+static double pmm_start_c(int norm)
+{
+    double scalef = 1.0e-280, pi = acos(-1.0);
+    return norm == 4 ? scalef / sqrt(4.0 * pi) : scalef;
+}
+
 /* Last sectoral term m = lc (MakeGridDH.f95:595-603), times rescalem. */
 static double pmm_last(int norm, int phase, double pmm, int lc, const double *sqr, double rescalem)
 {
@@ -918,6 +926,478 @@ int or_shexpandglq(const double *grid, int g0, int g1, int lmax, const double *w
 #undef GR
     anal_epilogue(norm, lc, cf, cilm, d1, d2);
     free(buf); free(cc); free(f1); free(f2); free(cf);
+    plan_destroy(plan);
+    free_tables(&t);
+    return 0;
+}
+
+/* ------------------------------------------------ complex transforms */
+
+/* Complex coefficients cilm[2][d1][d2] (complex, [0] positive orders C_lm,
+ * [1] negative orders C_l,-m) in the reference's column-major order
+ * cc[m][l][i], l, m <= lc. */
+static cplx *to_fortran_order_c(const cplx *cilm, int d1, int d2, int lc)
+{
+    size_t L1 = (size_t)lc + 1;
+    cplx *cc = (cplx *)calloc(2 * L1 * L1, sizeof(cplx));
+    if (!cc) return NULL;
+    for (int m = 0; m <= lc; m++)
+        for (int l = 0; l <= lc; l++)
+            for (int i = 0; i < 2; i++)
+                cc[(m * L1 + l) * 2 + i] = cilm[((size_t)i * d1 + l) * d2 + m];
+    return cc;
+}
+
+#define CCF(i, l, m) cc[((size_t)(m) * (lc + 1) + (l)) * 2 + (i)]
+static inline double msign(int m) { return (m % 2 == 0) ? 1.0 : -1.0; }
+
+/* One ring (coef) and its mirror (coefs) of the complex synthesis:
+ * coef[m] = sum_l C_lm P_lm, coef[nlong-m] = (-1)^m sum_l C_l,-m P_lm.
+ * MakeGridDHC.f95:531-609 / MakeGridGLQC.f95:556-668. */
+static void synth_ring_c(const tables *t, int norm, int phase, const cplx *cc, double z, double u,
+                         cplx *coef, cplx *coefs, int nlong)
+{
+    const double *sqr = t->sqr;
+    int lc = t->lc;
+    double pm2, pm1, p, pmm, rescalem;
+    cplx tempc;
+    for (int k = 0; k < nlong; k++) { coef[k] = 0; coefs[k] = 0; }
+    pm2 = pm00(norm);
+    tempc = CCF(0, 0, 0) * pm2;
+    coef[0] += tempc; coefs[0] += tempc;
+    pm1 = FF(t, ff1, 1, 0) * z * pm2;
+    tempc = CCF(0, 1, 0) * pm1;
+    coef[0] += tempc; coefs[0] -= tempc;
+    for (int l = 2; l <= lc; l++) {
+        p = FF(t, ff1, l, 0) * z * pm1 - FF(t, ff2, l, 0) * pm2;
+        tempc = CCF(0, l, 0) * p;
+        coef[0] += tempc; coefs[0] += tempc * fsym(l, 0);
+        pm2 = pm1; pm1 = p;
+    }
+    pmm = pmm_start_c(norm);
+    rescalem = 1.0 / 1.0e-280;
+    for (int m = 1; m <= lc - 1; m++) {
+        rescalem = rescalem * u;
+        pm2 = pmm_step(norm, phase, &pmm, m, sqr);
+        tempc = CCF(0, m, m) * pm2; coef[m] += tempc; coefs[m] += tempc;
+        tempc = CCF(1, m, m) * pm2; coef[nlong - m] += tempc; coefs[nlong - m] += tempc;
+        pm1 = z * FF(t, ff1, m + 1, m) * pm2;
+        tempc = CCF(0, m + 1, m) * pm1; coef[m] += tempc; coefs[m] -= tempc;
+        tempc = CCF(1, m + 1, m) * pm1; coef[nlong - m] += tempc; coefs[nlong - m] -= tempc;
+        for (int l = m + 2; l <= lc; l++) {
+            p = z * FF(t, ff1, l, m) * pm1 - FF(t, ff2, l, m) * pm2;
+            pm2 = pm1; pm1 = p;
+            tempc = CCF(0, l, m) * p; coef[m] += tempc; coefs[m] += tempc * fsym(l, m);
+            tempc = CCF(1, l, m) * p; coef[nlong - m] += tempc; coefs[nlong - m] += tempc * fsym(l, m);
+        }
+        coef[m] *= rescalem; coefs[m] *= rescalem;
+        coef[nlong - m] *= rescalem * msign(m); coefs[nlong - m] *= rescalem * msign(m);
+    }
+    rescalem = rescalem * u;
+    pmm = pmm_last(norm, phase, pmm, lc, sqr, rescalem);
+    tempc = CCF(0, lc, lc) * pmm; coef[lc] += tempc; coefs[lc] += tempc;
+    tempc = CCF(1, lc, lc) * pmm * msign(lc); coef[nlong - lc] += tempc; coefs[nlong - lc] += tempc;
+}
+
+/* Equator (z = 0, u = 1): MakeGridDHC.f95:452-529 / MakeGridGLQC.f95:473-553. */
+static void synth_equator_c(const tables *t, int norm, int phase, const cplx *cc, cplx *coef, int nlong)
+{
+    const double *sqr = t->sqr;
+    int lc = t->lc;
+    double pm2, p, pmm, rescalem;
+    for (int k = 0; k < nlong; k++) coef[k] = 0;
+    pm2 = pm00(norm);
+    coef[0] += CCF(0, 0, 0) * pm2;
+    for (int l = 2; l <= lc; l += 2) {
+        p = -FF(t, ff2, l, 0) * pm2;
+        pm2 = p;
+        coef[0] += CCF(0, l, 0) * p;
+    }
+    pmm = pmm_start_c(norm);
+    rescalem = 1.0 / 1.0e-280;
+    for (int m = 1; m <= lc - 1; m++) {
+        pm2 = pmm_step(norm, phase, &pmm, m, sqr);
+        coef[m] += CCF(0, m, m) * pm2;
+        coef[nlong - m] += CCF(1, m, m) * pm2;
+        for (int l = m + 2; l <= lc; l += 2) {
+            p = -FF(t, ff2, l, m) * pm2;
+            coef[m] += CCF(0, l, m) * p;
+            coef[nlong - m] += CCF(1, l, m) * p;
+            pm2 = p;
+        }
+        coef[m] *= rescalem;
+        coef[nlong - m] *= rescalem * msign(m);
+    }
+    pmm = pmm_last(norm, phase, pmm, lc, sqr, rescalem);
+    coef[lc] += CCF(0, lc, lc) * pmm;
+    coef[nlong - lc] += CCF(1, lc, lc) * pmm * msign(lc);
+}
+
+static int lc_of_makegrid(int d1, int d2, int lmax, int lmax_calc)
+{
+    int lc = lmax;
+    if (d2 - 1 < lc) lc = d2 - 1;
+    if (d1 - 1 < lc) lc = d1 - 1;
+    if (lmax_calc < lc) lc = lmax_calc;
+    return lc;
+}
+
+/* src/MakeGridDHC.f95:1-643: grid[g0][g1] complex (interleaved doubles). */
+int or_makegriddhc(const double *cilm_ri, int d1, int d2, int lmax, int norm, int sampling,
+                   int csphase, int lmax_calc, int extend, double *grid_ri, int g0, int g1)
+{
+    const cplx *cilm = (const cplx *)cilm_ri;
+    cplx *grid = (cplx *)grid_ri;
+    int n = 2 * lmax + 2, nlong, nlat_out, nlong_out;
+    if (sampling == 1) nlong = n; else if (sampling == 2) nlong = 2 * n; else return 2;
+    if (extend == 0) { nlat_out = n; nlong_out = nlong; }
+    else if (extend == 1) { nlat_out = n + 1; nlong_out = nlong + 1; }
+    else return 2;
+    if (g0 < nlat_out || g1 < nlong_out) return 1;
+    if (norm < 1 || norm > 4) return 2;
+    if (csphase != -1 && csphase != 1) return 2;
+    if (lmax_calc > lmax) return 2;
+    int lc = lc_of_makegrid(d1, d2, lmax, lmax_calc);
+#define G(i, k) grid[(size_t)(i) * g1 + (k)]
+    if (lc == 0) {
+        double pm2 = pm00(norm);
+        for (int i = 0; i < nlat_out; i++)
+            for (int k = 0; k < nlong_out; k++) G(i, k) = cilm[0] * pm2;
+        return 0;
+    }
+    tables t;
+    int st = build_tables(&t, lc, norm);
+    cplx *cc = to_fortran_order_c(cilm, d1, d2, lc);
+    if (st || !cc) { free_tables(&t); free(cc); return 3; }
+    int i_eq = n / 2 + 1;
+    {
+        fftplan *plan = plan_create(nlong);
+        cplx *coef = (cplx *)malloc(sizeof(cplx) * nlong);
+        synth_equator_c(&t, norm, csphase, cc, coef, nlong);
+        plan_exec(plan, coef, 1);
+        for (int k = 0; k < nlong; k++) G(i_eq - 1, k) = coef[k];
+        free(coef);
+        plan_destroy(plan);
+    }
+    const double pi = acos(-1.0);
+#pragma omp parallel
+    {
+        fftplan *plan = plan_create(nlong);
+        cplx *coef = (cplx *)malloc(sizeof(cplx) * nlong);
+        cplx *coefs = (cplx *)malloc(sizeof(cplx) * nlong);
+#pragma omp for schedule(dynamic, 1)
+        for (int i = 1; i <= i_eq - 1; i++) {
+            int i_s = 2 * i_eq - i;
+            double theta = pi * (double)(i - 1) / (double)n;
+            double z = cos(theta);
+            double u = sqrt((1.0 - z) * (1.0 + z));
+            synth_ring_c(&t, norm, csphase, cc, z, u, coef, coefs, nlong);
+            plan_exec(plan, coef, 1);
+            for (int k = 0; k < nlong; k++) G(i - 1, k) = coef[k];
+            if (!(i == 1 && extend == 0)) {
+                plan_exec(plan, coefs, 1);
+                for (int k = 0; k < nlong; k++) G(i_s - 1, k) = coefs[k];
+            }
+        }
+        free(coef); free(coefs);
+        plan_destroy(plan);
+    }
+    if (extend == 1)
+        for (int i = 0; i < nlat_out; i++) G(i, nlong_out - 1) = G(i, 0);
+#undef G
+    free(cc);
+    free_tables(&t);
+    return 0;
+}
+
+/* src/MakeGridGLQC.f95:1-688, on-the-fly branch (:471-678). */
+int or_makegridglqc(const double *cilm_ri, int d1, int d2, int lmax, const double *zero, int nzero,
+                    int norm, int csphase, int lmax_calc, int extend, double *grid_ri, int g0, int g1)
+{
+    const cplx *cilm = (const cplx *)cilm_ri;
+    cplx *grid = (cplx *)grid_ri;
+    int nlong = 2 * lmax + 1, nlat = lmax + 1, nlong_out;
+    if (extend == 0) nlong_out = nlong; else if (extend == 1) nlong_out = nlong + 1; else return 2;
+    if (g1 < nlong_out || g0 < nlat) return 1;
+    if (!zero) return 5;
+    if (nzero < lmax + 1) return 1;
+    if (norm < 1 || norm > 4) return 2;
+    if (csphase != -1 && csphase != 1) return 2;
+    if (lmax_calc > lmax) return 2;
+    int lc = lc_of_makegrid(d1, d2, lmax, lmax_calc);
+#define G(i, k) grid[(size_t)(i) * g1 + (k)]
+    if (lc == 0) {
+        double pm2 = pm00(norm);
+        for (int i = 0; i < nlat; i++)
+            for (int k = 0; k < nlong_out; k++) G(i, k) = cilm[0] * pm2;
+        return 0;
+    }
+    tables t;
+    int st = build_tables(&t, lc, norm);
+    cplx *cc = to_fortran_order_c(cilm, d1, d2, lc);
+    if (st || !cc) { free_tables(&t); free(cc); return 3; }
+#pragma omp parallel
+    {
+        fftplan *plan = plan_create(nlong);
+        cplx *coef = (cplx *)malloc(sizeof(cplx) * nlong);
+        cplx *coefs = (cplx *)malloc(sizeof(cplx) * nlong);
+#pragma omp for schedule(dynamic, 1)
+        for (int i = 1; i <= (nlat + 1) / 2; i++) {
+            if (i == (nlat + 1) / 2 && nlat % 2 != 0) {
+                synth_equator_c(&t, norm, csphase, cc, coef, nlong);
+                plan_exec(plan, coef, 1);
+                for (int k = 0; k < nlong; k++) G(i - 1, k) = coef[k];
+            } else {
+                double z = zero[i - 1];
+                double u = sqrt((1.0 - z) * (1.0 + z));
+                int i_s = nlat + 1 - i;
+                synth_ring_c(&t, norm, csphase, cc, z, u, coef, coefs, nlong);
+                plan_exec(plan, coef, 1);
+                for (int k = 0; k < nlong; k++) G(i - 1, k) = coef[k];
+                plan_exec(plan, coefs, 1);
+                for (int k = 0; k < nlong; k++) G(i_s - 1, k) = coefs[k];
+            }
+        }
+        free(coef); free(coefs);
+        plan_destroy(plan);
+    }
+    if (extend == 1)
+        for (int i = 0; i < nlat; i++) G(i, nlong_out - 1) = G(i, 0);
+#undef G
+    free(cc);
+    free_tables(&t);
+    return 0;
+}
+
+/* Accumulate one ring pair of the complex analysis into cc ([m][l][i]);
+ * f1/f2 are the weighted DFTs of the north/south rings (length nlong,
+ * negative orders at nlong - m).  SHExpandDHC.f95:585-664 /
+ * SHExpandGLQC.f95:556-656. */
+static void anal_ring_c(const tables *t, int norm, int phase, double z, double u, cplx *f1, cplx *f2,
+                        int nlong, cplx *cc)
+{
+    const double *sqr = t->sqr;
+    int lc = t->lc;
+    double pm2, pm1, p, pmm, rescalem;
+    cplx ffc1[2], ffc2[2];
+    pm2 = pm00(norm);
+    CCF(0, 0, 0) += pm2 * (f1[0] + f2[0]);
+    if (lc == 0) return;
+    pm1 = FF(t, ff1, 1, 0) * z * pm2;
+    CCF(0, 1, 0) += pm1 * (f1[0] - f2[0]);
+    ffc1[0] = f1[0] - f2[0];
+    ffc1[1] = f1[0] + f2[0];
+    for (int l = 2; l <= lc; l++) {
+        p = FF(t, ff1, l, 0) * z * pm1 - FF(t, ff2, l, 0) * pm2;
+        pm2 = pm1; pm1 = p;
+        CCF(0, l, 0) += p * ffc1[fsym(l, 0) > 0];
+    }
+    pmm = pmm_start_c(norm);
+    rescalem = 1.0 / 1.0e-280;
+    for (int m = 1; m <= lc - 1; m++) {
+        rescalem = rescalem * u;
+        pm2 = pmm_step(norm, phase, &pmm, m, sqr);
+        f1[m] *= rescalem;
+        f1[nlong - m] *= rescalem * msign(m);
+        f2[m] *= rescalem;
+        f2[nlong - m] *= rescalem * msign(m);
+        CCF(0, m, m) += pm2 * (f1[m] + f2[m]);
+        CCF(1, m, m) += pm2 * (f1[nlong - m] + f2[nlong - m]);
+        pm1 = z * FF(t, ff1, m + 1, m) * pm2;
+        CCF(0, m + 1, m) += pm1 * (f1[m] - f2[m]);
+        CCF(1, m + 1, m) += pm1 * (f1[nlong - m] - f2[nlong - m]);
+        ffc1[0] = f1[m] - f2[m];
+        ffc1[1] = f1[m] + f2[m];
+        ffc2[0] = f1[nlong - m] - f2[nlong - m];
+        ffc2[1] = f1[nlong - m] + f2[nlong - m];
+        for (int l = m + 2; l <= lc; l++) {
+            p = z * FF(t, ff1, l, m) * pm1 - FF(t, ff2, l, m) * pm2;
+            pm2 = pm1; pm1 = p;
+            int s = fsym(l, m) > 0;
+            CCF(0, l, m) += p * ffc1[s];
+            CCF(1, l, m) += p * ffc2[s];
+        }
+    }
+    rescalem = rescalem * u;
+    pmm = pmm_last(norm, phase, pmm, lc, sqr, rescalem);
+    CCF(0, lc, lc) += pmm * (f1[lc] + f2[lc]);
+    CCF(1, lc, lc) += pmm * (f1[nlong - lc] + f2[nlong - lc]) * msign(lc);
+}
+
+/* Equator of the complex analysis: SHExpandDHC.f95:469-550. */
+static void anal_equator_c(const tables *t, int norm, int phase, cplx *f1, int nlong, cplx *cc)
+{
+    const double *sqr = t->sqr;
+    int lc = t->lc;
+    double pm2, p, pmm, rescalem;
+    pm2 = pm00(norm);
+    CCF(0, 0, 0) += pm2 * f1[0];
+    if (lc == 0) return;
+    for (int l = 2; l <= lc; l += 2) {
+        p = -FF(t, ff2, l, 0) * pm2;
+        pm2 = p;
+        CCF(0, l, 0) += p * f1[0];
+    }
+    pmm = pmm_start_c(norm);
+    rescalem = 1.0 / 1.0e-280;
+    for (int m = 1; m <= lc - 1; m++) {
+        pm2 = pmm_step(norm, phase, &pmm, m, sqr);
+        f1[m] *= rescalem;
+        f1[nlong - m] *= rescalem * msign(m);
+        CCF(0, m, m) += pm2 * f1[m];
+        CCF(1, m, m) += pm2 * f1[nlong - m];
+        for (int l = m + 2; l <= lc; l += 2) {
+            p = -FF(t, ff2, l, m) * pm2;
+            pm2 = p;
+            CCF(0, l, m) += p * f1[m];
+            CCF(1, l, m) += p * f1[nlong - m];
+        }
+    }
+    pmm = pmm_last(norm, phase, pmm, lc, sqr, rescalem);
+    CCF(0, lc, lc) += pmm * f1[lc];
+    CCF(1, lc, lc) += msign(lc) * pmm * f1[nlong - lc];
+}
+
+/* Normalisation epilogue of the complex analysis: SHExpandDHC.f95:681-723. */
+static void anal_epilogue_c(int norm, int lc, const cplx *cc, cplx *cilm, int d1, int d2)
+{
+#define CI(i, l, m) cilm[((size_t)(i) * d1 + (l)) * d2 + (m)]
+    for (int l = 0; l <= lc; l++)
+        for (int m = 0; m <= l; m++)
+            for (int i = 0; i < 2; i++) CI(i, l, m) = CCF(i, l, m);
+    double pi = acos(-1.0), prod;
+    switch (norm) {
+    case 1:
+        for (int l = 0; l <= lc; l++)
+            for (int m = 0; m <= l; m++)
+                for (int i = 0; i < 2; i++) CI(i, l, m) = CI(i, l, m) / (4 * pi);
+        break;
+    case 2:
+        for (int l = 0; l <= lc; l++)
+            for (int m = 0; m <= l; m++)
+                for (int i = 0; i < 2; i++) CI(i, l, m) = CI(i, l, m) * (2 * l + 1) / (4 * pi);
+        break;
+    case 3:
+        for (int l = 0; l <= lc; l++) {
+            prod = 4 * pi / (double)(2 * l + 1);
+            CI(0, l, 0) = CI(0, l, 0) / prod;
+            for (int m = 1; m <= l - 1; m++) {
+                prod = prod * (l + m) * (l - m + 1);
+                for (int i = 0; i < 2; i++) CI(i, l, m) = CI(i, l, m) / prod;
+            }
+            if (l != 0)
+                for (int i = 0; i < 2; i++) CI(i, l, l) = CI(i, l, l) / (prod * 2 * l);
+        }
+        break;
+    default: break;
+    }
+#undef CI
+}
+
+/* src/SHExpandDHC.f95:1-725: grid[n][g1] complex; cilm complex [2][d1][d2]. */
+int or_shexpanddhc(const double *grid_ri, int n, int g1, int norm, int sampling, int csphase,
+                   int lmax_calc, double *cilm_ri, int d1, int d2)
+{
+    const cplx *grid = (const cplx *)grid_ri;
+    cplx *cilm = (cplx *)cilm_ri;
+    int lmax = n / 2 - 1, lc, nlong;
+    if (lmax_calc > lmax) return 2;
+    lc = lmax_calc < lmax ? lmax_calc : lmax;
+    if (sampling != 1 && sampling != 2) return 2;
+    if (n % 2 != 0) return 2;
+    if (d1 < lc + 1 || d2 < lc + 1) return 1;
+    nlong = (sampling == 1) ? n : 2 * n;
+    if (g1 < nlong) return 1;
+    if (csphase != -1 && csphase != 1) return 2;
+    if (norm < 1 || norm > 4) return 2;
+    double pi = acos(-1.0);
+    for (size_t k = 0; k < (size_t)2 * d1 * d2; k++) cilm[k] = 0;
+    double *aj = (double *)malloc(sizeof(double) * (n + 1));
+    or_dhaj(n, 0, aj);
+    for (int j = 0; j < n; j++) aj[j] = aj[j] * sqrt(4.0 * pi);
+    tables t;
+    int st = build_tables(&t, lc, norm);
+    cplx *cc = (cplx *)calloc((size_t)2 * (lc + 1) * (lc + 1), sizeof(cplx));
+    if (st || !cc) { free_tables(&t); free(aj); free(cc); return 3; }
+    fftplan *plan = plan_create(nlong);
+    cplx *f1 = (cplx *)malloc(sizeof(cplx) * nlong);
+    cplx *f2 = (cplx *)malloc(sizeof(cplx) * nlong);
+    int i_eq = n / 2 + 1;
+#define GR(i, k) grid[(size_t)(i) * g1 + (k)]
+    {   /* equator first (SHExpandDHC.f95:469-550) */
+        int i = i_eq;
+        for (int k = 0; k < nlong; k++) f1[k] = GR(i - 1, k);
+        plan_exec(plan, f1, 0);
+        for (int k = 0; k < nlong; k++) f1[k] = f1[k] * sqrt(2 * pi) * aj[i - 1] / (double)nlong;
+        anal_equator_c(&t, norm, csphase, f1, nlong, cc);
+    }
+    for (int i = 2; i <= i_eq - 1; i++) {
+        double theta = (i - 1) * pi / (double)n;
+        double z = cos(theta);
+        double u = sqrt((1.0 - z) * (1.0 + z));
+        int i_s = 2 * i_eq - i;
+        for (int k = 0; k < nlong; k++) f1[k] = GR(i - 1, k);
+        plan_exec(plan, f1, 0);
+        for (int k = 0; k < nlong; k++) f1[k] = f1[k] * sqrt(2 * pi) * aj[i - 1] / (double)nlong;
+        for (int k = 0; k < nlong; k++) f2[k] = GR(i_s - 1, k);
+        plan_exec(plan, f2, 0);
+        for (int k = 0; k < nlong; k++) f2[k] = f2[k] * sqrt(2 * pi) * aj[i_s - 1] / (double)nlong;
+        anal_ring_c(&t, norm, csphase, z, u, f1, f2, nlong, cc);
+    }
+#undef GR
+    anal_epilogue_c(norm, lc, cc, cilm, d1, d2);
+    free(f1); free(f2); free(aj); free(cc);
+    plan_destroy(plan);
+    free_tables(&t);
+    return 0;
+}
+
+/* src/SHExpandGLQC.f95:1-708, on-the-fly branch (:450-660). */
+int or_shexpandglqc(const double *grid_ri, int g0, int g1, int lmax, const double *w, int nw,
+                    const double *zero, int nzero, int norm, int csphase, int lmax_calc,
+                    double *cilm_ri, int d1, int d2)
+{
+    const cplx *grid = (const cplx *)grid_ri;
+    cplx *cilm = (cplx *)cilm_ri;
+    int lc, nlong = 2 * lmax + 1, nlat = lmax + 1;
+    if (lmax_calc > lmax) return 2;
+    lc = lmax_calc < lmax ? lmax_calc : lmax;
+    if (d1 < lc + 1 || d2 < lc + 1) return 1;
+    if (g1 < 2 * lmax + 1 || g0 < lmax + 1) return 1;
+    if (nw < lmax + 1) return 1;
+    if (!zero) return 5;
+    if (nzero < lmax + 1) return 1;
+    if (norm < 1 || norm > 4) return 2;
+    if (csphase != -1 && csphase != 1) return 2;
+    double pi = acos(-1.0);
+    for (size_t k = 0; k < (size_t)2 * d1 * d2; k++) cilm[k] = 0;
+    tables t;
+    int st = build_tables(&t, lc, norm);
+    cplx *cc = (cplx *)calloc((size_t)2 * (lc + 1) * (lc + 1), sizeof(cplx));
+    if (st || !cc) { free_tables(&t); free(cc); return 3; }
+    fftplan *plan = plan_create(nlong);
+    cplx *f1 = (cplx *)malloc(sizeof(cplx) * nlong);
+    cplx *f2 = (cplx *)malloc(sizeof(cplx) * nlong);
+#define GR(i, k) grid[(size_t)(i) * g1 + (k)]
+    for (int i = 1; i <= (nlat + 1) / 2; i++) {
+        for (int k = 0; k < nlong; k++) f1[k] = GR(i - 1, k);
+        plan_exec(plan, f1, 0);
+        for (int k = 0; k < nlong; k++) f1[k] = 2 * w[i - 1] * pi * f1[k] / (double)nlong;
+        if (i == (nlat + 1) / 2 && nlat % 2 != 0) {
+            anal_equator_c(&t, norm, csphase, f1, nlong, cc);
+        } else {
+            double z = zero[i - 1];
+            double u = sqrt((1.0 - z) * (1.0 + z));
+            int i_s = nlat + 1 - i;
+            for (int k = 0; k < nlong; k++) f2[k] = GR(i_s - 1, k);
+            plan_exec(plan, f2, 0);
+            for (int k = 0; k < nlong; k++) f2[k] = 2 * w[i_s - 1] * pi * f2[k] / (double)nlong;
+            anal_ring_c(&t, norm, csphase, z, u, f1, f2, nlong, cc);
+        }
+    }
+#undef GR
+    anal_epilogue_c(norm, lc, cc, cilm, d1, d2);
+    free(f1); free(f2); free(cc);
     plan_destroy(plan);
     free_tables(&t);
     return 0;

@@ -22,6 +22,13 @@ SIGNATURES = {
                           _c_int, _vp],
     "shtg_shexpand_glq": [_vp, _c_int, _c_int, _dp, _c_int, _dp, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int,
                           _vp],
+    "shtg_makegrid_dhc": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int,
+                          _vp],
+    "shtg_shexpand_dhc": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _vp],
+    "shtg_makegrid_glqc": [_vp, _c_int, _c_int, _c_int, _dp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int,
+                           _c_int, _vp],
+    "shtg_shexpand_glqc": [_vp, _c_int, _c_int, _dp, _c_int, _dp, _c_int, _c_int, _c_int, _c_int, _vp, _c_int,
+                           _c_int, _vp],
     "shtg_makegrid_dh_adjoint": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int,
                                  _c_int, _vp],
     "shtg_shexpand_dh_adjoint": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _vp],

@@ -15,9 +15,12 @@ pyshtools/backends/shtools.py:189-229):
                 lmax_calc=gridglq.shape[0]-1) -> cilm
     SHGLQ(lmax) -> (zero, w);  GLQGridCoord(lmax, extend=0) -> (lat, lon);
     DHaj(n, extend=0) -> aj
+    MakeGridDHC, SHExpandDHC, MakeGridGLQC, SHExpandGLQC: the complex
+    transforms with the same arguments (src/pyshtools.pyf:278-319,379-422)
 
 Array arguments may be numpy arrays (copied to the GPU and back, like f2py
-copies between C and Fortran order) or float64 torch tensors on a HIP device
+copies between C and Fortran order) or float64 (complex128 for the complex
+transforms) torch tensors on a HIP device
 (used in place; the result is a tensor on the same device, produced
 asynchronously on the current stream).  Nonzero exit codes raise
 ``SHToolsError`` with the reference's messages.
@@ -210,6 +213,114 @@ def SHExpandGLQ(gridglq, w, zero, norm=1, csphase=1, lmax_calc=None):
     return _finish(cilm, as_tensor)
 
 
+# ------------------------------------------------------------ complex transforms
+
+def _to_dev_c(x, device):
+    torch = _torch()
+    if _is_tensor(x):
+        t = x.to(device=device, dtype=torch.complex128)
+    else:
+        t = torch.as_tensor(np.ascontiguousarray(x, dtype=np.complex128), device=device)
+    return t.contiguous()
+
+
+def MakeGridDHC(cilm, lmax=None, norm=1, sampling=1, csphase=1, lmax_calc=None, extend=0):
+    """Evaluate complex coefficients on a Driscoll-Healy grid (src/MakeGridDHC.f95).
+    cilm[0] holds the orders +m, cilm[1] the orders -m."""
+    torch = _torch()
+    as_tensor = _is_tensor(cilm)
+    dev = _device_of(cilm)
+    c = _to_dev_c(cilm, dev)
+    if c.dim() != 3 or c.shape[0] < 2:
+        raise SHToolsError(_status_message(1))
+    if lmax is None:
+        lmax = c.shape[1] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    if sampling not in (1, 2) or extend not in (0, 1) or lmax < 0:
+        raise SHToolsError(_status_message(2))
+    g0 = 2 * (lmax + 1) + extend
+    g1 = sampling * 2 * (lmax + 1) + extend
+    grid = torch.empty((g0, g1), dtype=torch.complex128, device=dev)
+    with torch.cuda.device(dev):
+        st = _lib.lib().shtg_makegrid_dhc(c.data_ptr(), c.shape[1], c.shape[2], int(lmax), int(norm), int(sampling),
+                                          int(csphase), int(lmax_calc), int(extend), grid.data_ptr(), g0, g1,
+                                          _stream(dev))
+    _check(st)
+    return _finish(grid, as_tensor)
+
+
+def SHExpandDHC(griddh, norm=1, sampling=1, csphase=1, lmax_calc=None):
+    """Expand a complex Driscoll-Healy grid into complex coefficients (src/SHExpandDHC.f95)."""
+    torch = _torch()
+    as_tensor = _is_tensor(griddh)
+    dev = _device_of(griddh)
+    g = _to_dev_c(griddh, dev)
+    if g.dim() != 2:
+        raise SHToolsError(_status_message(1))
+    n = g.shape[0]
+    if lmax_calc is None:
+        lmax_calc = n // 2 - 1
+    if lmax_calc < 0:
+        raise SHToolsError(_status_message(2))
+    cilm = torch.empty((2, lmax_calc + 1, lmax_calc + 1), dtype=torch.complex128, device=dev)
+    with torch.cuda.device(dev):
+        st = _lib.lib().shtg_shexpand_dhc(g.data_ptr(), g.shape[0], g.shape[1], int(norm), int(sampling),
+                                          int(csphase), int(lmax_calc), cilm.data_ptr(), lmax_calc + 1,
+                                          lmax_calc + 1, _stream(dev))
+    _check(st)
+    return _finish(cilm, as_tensor)
+
+
+def MakeGridGLQC(cilm, zero, lmax=None, norm=1, csphase=1, lmax_calc=None, extend=0):
+    """Evaluate complex coefficients on a Gauss-Legendre grid (src/MakeGridGLQC.f95)."""
+    torch = _torch()
+    as_tensor = _is_tensor(cilm)
+    dev = _device_of(cilm)
+    c = _to_dev_c(cilm, dev)
+    if c.dim() != 3 or c.shape[0] < 2:
+        raise SHToolsError(_status_message(1))
+    z = _host_f64(zero)
+    if lmax is None:
+        lmax = c.shape[1] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    if extend not in (0, 1) or lmax < 0:
+        raise SHToolsError(_status_message(2))
+    g0, g1 = lmax + 1, 2 * lmax + 1 + extend
+    grid = torch.empty((g0, g1), dtype=torch.complex128, device=dev)
+    with torch.cuda.device(dev):
+        st = _lib.lib().shtg_makegrid_glqc(c.data_ptr(), c.shape[1], c.shape[2], int(lmax), _dptr(z), z.size,
+                                           int(norm), int(csphase), int(lmax_calc), int(extend), grid.data_ptr(),
+                                           g0, g1, _stream(dev))
+    _check(st)
+    return _finish(grid, as_tensor)
+
+
+def SHExpandGLQC(gridglq, w, zero, norm=1, csphase=1, lmax_calc=None):
+    """Expand a complex Gauss-Legendre grid into complex coefficients (src/SHExpandGLQC.f95)."""
+    torch = _torch()
+    as_tensor = _is_tensor(gridglq)
+    dev = _device_of(gridglq)
+    g = _to_dev_c(gridglq, dev)
+    if g.dim() != 2:
+        raise SHToolsError(_status_message(1))
+    wq = _host_f64(w)
+    z = _host_f64(zero)
+    lmax = g.shape[0] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    if lmax_calc < 0:
+        raise SHToolsError(_status_message(2))
+    cilm = torch.empty((2, lmax_calc + 1, lmax_calc + 1), dtype=torch.complex128, device=dev)
+    with torch.cuda.device(dev):
+        st = _lib.lib().shtg_shexpand_glqc(g.data_ptr(), g.shape[0], g.shape[1], _dptr(wq), wq.size, _dptr(z),
+                                           z.size, int(norm), int(csphase), int(lmax_calc), cilm.data_ptr(),
+                                           lmax_calc + 1, lmax_calc + 1, _stream(dev))
+    _check(st)
+    return _finish(cilm, as_tensor)
+
+
 # ------------------------------------------------------------ host helpers
 
 def SHGLQ(lmax):
@@ -282,10 +393,6 @@ def _not_provided(name):
     return f
 
 
-SHExpandDHC = _not_provided("SHExpandDHC")
-MakeGridDHC = _not_provided("MakeGridDHC")
-SHExpandGLQC = _not_provided("SHExpandGLQC")
-MakeGridGLQC = _not_provided("MakeGridGLQC")
 MakeGradientDH = _not_provided("MakeGradientDH")
 SHRotateRealCoef = _not_provided("SHRotateRealCoef")
 

@@ -606,6 +606,202 @@ __global__ __launch_bounds__(kThreads) void k_zero_outside(double *__restrict__ 
     }
 }
 
+// ============================================================ complex kernels
+//
+// MakeGridDHC / SHExpandDHC / MakeGridGLQC / SHExpandGLQC
+// (src/MakeGridDHC.f95:531-609, src/SHExpandDHC.f95:552-723 and the GLQ
+// counterparts): coefficients cilm[2][d1][d2] complex ([0] orders +m, [1]
+// orders -m, the latter entering with (-1)^m), rings complex.  The Legendre
+// sums reuse the real kernels once per sign of m with (Re, Im) in place of
+// (C, S); the ring FFTs are full complex DFTs, one ring at a time.
+
+// cpack[tri(m) + l - m] = plane[l][m] * s_lm for the tile's m <= l <= L (zero
+// at m = 0 when zero_m0: the negative-order plane has no m = 0 entry).
+__device__ __forceinline__ void pack_tile_c(const double2 *__restrict__ plane, int d2, int L,
+                                            const double *__restrict__ s_tab, int Lt, double2 *__restrict__ cpack,
+                                            int m0, int l0, bool zero_m0, CoefTile &t)
+{
+    for (int e = threadIdx.x; e < kTL * kTM; e += blockDim.x) {
+        const int r = e / kTM, c = e - r * kTM, l = l0 + r, m = m0 + c;
+        double2 v = make_double2(0.0, 0.0);
+        if (l <= L && m <= l && !(zero_m0 && m == 0)) v = plane[(size_t)l * d2 + m];
+        t[r][c] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kTL * kTM; e += blockDim.x) {
+        const int c = e / kTL, r = e - c * kTL, l = l0 + r, m = m0 + c;
+        if (l <= L && m <= l) {
+            const double sv = s_tab[tri_offd(m, Lt) + (l - m)];
+            const double2 v = t[r][c];
+            cpack[tri_offd(m, L) + (l - m)] = make_double2(v.x * sv, v.y * sv);
+        }
+    }
+    __syncthreads();
+}
+
+// plane[l][m] (within d1 x d2) = cbuf * post for m <= l <= L, zero elsewhere
+// (and at m = 0 when zero_m0).
+__device__ __forceinline__ void unpack_tile_c(const double2 *__restrict__ cbuf, const double *__restrict__ post,
+                                              int Lt, int L, double2 *__restrict__ plane, int d1, int d2, int m0,
+                                              int l0, bool zero_m0, CoefTile &t)
+{
+    for (int e = threadIdx.x; e < kTL * kTM; e += blockDim.x) {
+        const int c = e / kTL, r = e - c * kTL, l = l0 + r, m = m0 + c;
+        double2 v = make_double2(0.0, 0.0);
+        if (l <= L && m <= l && !(zero_m0 && m == 0)) {
+            const double pv = post[tri_offd(m, Lt) + (l - m)];
+            const double2 x = cbuf[tri_offd(m, L) + (l - m)];
+            v = make_double2(x.x * pv, x.y * pv);
+        }
+        t[r][c] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kTL * kTM; e += blockDim.x) {
+        const int r = e / kTM, c = e - r * kTM, l = l0 + r, m = m0 + c;
+        if (l < d1 && m < d2) plane[(size_t)l * d2 + m] = t[r][c];
+    }
+    __syncthreads();
+}
+
+// grid: ntm x ntl tiles; both planes
+__global__ __launch_bounds__(kThreads) void k_pack_synth_c(const double2 *__restrict__ cilm, int d1, int d2,
+                                                           const double *__restrict__ s_tab, int Lt,
+                                                           double2 *__restrict__ cpack_p,
+                                                           double2 *__restrict__ cpack_m, int Lc, int ntm)
+{
+    __shared__ CoefTile t;
+    const int tl = blockIdx.x / ntm, tm = blockIdx.x - tl * ntm;
+    const int m0 = tm * kTM, l0 = tl * kTL;
+    if (l0 + kTL - 1 < m0) return;
+    pack_tile_c(cilm, d2, Lc, s_tab, Lt, cpack_p, m0, l0, false, t);
+    pack_tile_c(cilm + (size_t)d1 * d2, d2, Lc, s_tab, Lt, cpack_m, m0, l0, true, t);
+}
+
+__global__ __launch_bounds__(kThreads) void k_unpack_anal_c(const double2 *__restrict__ cbuf_p,
+                                                            const double2 *__restrict__ cbuf_m,
+                                                            const double *__restrict__ post, int Lt, int Lc,
+                                                            double2 *__restrict__ cilm, int d1, int d2, int ntm)
+{
+    __shared__ CoefTile t;
+    const int tl = blockIdx.x / ntm, tm = blockIdx.x - tl * ntm;
+    unpack_tile_c(cbuf_p, post, Lt, Lc, cilm, d1, d2, tm * kTM, tl * kTL, false, t);
+    unpack_tile_c(cbuf_m, post, Lt, Lc, cilm + (size_t)d1 * d2, d1, d2, tm * kTM, tl * kTL, true, t);
+}
+
+__device__ __forceinline__ double msign(int m) { return (m & 1) ? -1.0 : 1.0; }
+
+// Complex synthesis of slot s: for the north (ring 0) and south (ring 1)
+// row, X_k = F_k and X_{n-k} = (-1)^k G_k (F, G the sums of orders +k, -k,
+// times the order's rescale), inverse DFT via conj(DFT(conj X)).
+__device__ __forceinline__ void synth_fft_c_tile(const FftDev &F, const double4 *__restrict__ fb_p,
+                                                 const double4 *__restrict__ fb_m, int P, int Lc, int s, double u,
+                                                 int2 rw, int nrows, double2 *__restrict__ grid, long long ld,
+                                                 int ext_col, double2 *x)
+{
+    const int n = F.n, m = F.m;
+    const int top = (F.lc >= 0) ? m : n;
+    for (int ring = 0; ring < 2; ++ring) {
+        const int row = ring ? rw.y : rw.x;
+        if (row < 0 || row >= nrows) continue;
+        for (int q = threadIdx.x; q < m; q += blockDim.x)
+            if ((q > Lc && q < top - Lc) || q >= top) x[fpad(q)] = make_double2(0.0, 0.0);
+        for (int k = threadIdx.x; k <= Lc; k += blockDim.x) {
+            const double r = order_rescale(u, k);
+            const double4 fp = fb_p[(size_t)k * P + s];
+            const double2 a = ring ? make_double2(fp.z * r, fp.w * r) : make_double2(fp.x * r, fp.y * r);
+            x[fpad(k)] = fft_prep_in(F, k, conjd(a));
+            if (k > 0) {
+                const double4 fm = fb_m[(size_t)k * P + s];
+                const double g = r * msign(k);
+                const double2 b = ring ? make_double2(fm.z * g, fm.w * g) : make_double2(fm.x * g, fm.y * g);
+                x[fpad(fft_pos(F, n - k))] = fft_prep_in(F, n - k, conjd(b));
+            }
+        }
+        __syncthreads();
+        fft_forward(x, F, F.bhat);
+        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+            const double2 v = conjd(fft_get(x, F, j, j));
+            grid[row * ld + j] = v;
+            if (ext_col && j == 0) grid[row * ld + n] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// Complex analysis of slot s: DFT of each ring, weighted, orders +k and -k
+// (the latter with (-1)^k), times the order's rescale; the north pass
+// stores (f, f), the south pass turns them into (f_N + f_S, f_N - f_S).
+__device__ __forceinline__ void anal_fft_c_tile(const FftDev &F, const double2 *__restrict__ grid, long long ld,
+                                                int2 rw, int nrows, double2 w, double u, int P, int Lc, int s,
+                                                double4 *__restrict__ gb_p, double4 *__restrict__ gb_m, double2 *x)
+{
+    const int n = F.n, m = F.m;
+    for (int ring = 0; ring < 2; ++ring) {
+        const int row = ring ? rw.y : rw.x;
+        if (row < 0 || row >= nrows) continue;
+        const double wr = ring ? w.y : w.x;
+        for (int q = threadIdx.x; q < m; q += blockDim.x)
+            x[fpad(q)] = (q < n) ? fft_prep_in(F, q, grid[row * ld + q]) : make_double2(0.0, 0.0);
+        __syncthreads();
+        fft_forward(x, F, F.bhat2);
+        for (int k = threadIdx.x; k <= Lc; k += blockDim.x) {
+            const double r = order_rescale(u, k) * wr;
+            const double2 zp = fft_get(x, F, k, k);
+            const double2 fp = make_double2(zp.x * r, zp.y * r);
+            double2 fm = make_double2(0.0, 0.0);
+            if (k > 0) {
+                const double2 zm = fft_get(x, F, n - k, fft_pos(F, n - k));
+                const double g = r * msign(k);
+                fm = make_double2(zm.x * g, zm.y * g);
+            }
+            double4 &op = gb_p[(size_t)k * P + s];
+            double4 &om = gb_m[(size_t)k * P + s];
+            if (ring == 0) {
+                op = make_double4(fp.x, fp.y, fp.x, fp.y);
+                om = make_double4(fm.x, fm.y, fm.x, fm.y);
+            } else {
+                const double4 a = op, b = om;
+                op = make_double4(a.x + fp.x, a.y + fp.y, a.x - fp.x, a.y - fp.y);
+                om = make_double4(b.x + fm.x, b.y + fm.y, b.x - fm.x, b.y - fm.y);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// One workgroup per slot (LDS) or per slice of a global workspace (WS).
+template <bool WS>
+__global__ __launch_bounds__(kFftMaxThreads) void k_synth_fft_c(FftDev F, const double4 *__restrict__ fb_p,
+                                                          const double4 *__restrict__ fb_m, int P, int Lc,
+                                                          const double *__restrict__ ringu,
+                                                          const int2 *__restrict__ rows, int nrows,
+                                                          double2 *__restrict__ grid, long long ld, int ext_col,
+                                                          double2 *ws, long long ws_stride)
+{
+    extern __shared__ double2 lds[];
+    double2 *x;
+    if constexpr (WS) x = ws + blockIdx.x * ws_stride;
+    else x = lds;
+    for (int s = blockIdx.x; s < P; s += gridDim.x)
+        synth_fft_c_tile(F, fb_p, fb_m, P, Lc, s, ringu[s], rows[s], nrows, grid, ld, ext_col, x);
+}
+
+template <bool WS>
+__global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft_c(FftDev F, const double2 *__restrict__ grid,
+                                                         long long ld, const double *__restrict__ ringu,
+                                                         const int2 *__restrict__ rows, int nrows,
+                                                         const double2 *__restrict__ wts, int P, int Lc,
+                                                         double4 *__restrict__ gb_p, double4 *__restrict__ gb_m,
+                                                         double2 *ws, long long ws_stride)
+{
+    extern __shared__ double2 lds[];
+    double2 *x;
+    if constexpr (WS) x = ws + blockIdx.x * ws_stride;
+    else x = lds;
+    for (int s = blockIdx.x; s < P; s += gridDim.x)
+        anal_fft_c_tile(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, Lc, s, gb_p, gb_m, x);
+}
+
 // ============================================================= batched kernels
 
 // Per-field description of a batch (all fields: lmax_calc = lmax).

@@ -49,7 +49,7 @@ static T *dev_upload(const std::vector<T> &v)
 
 struct DevRecurrence {
     int L = -1;
-    double *alpha = nullptr, *s = nullptr, *post = nullptr;
+    double *alpha = nullptr, *s = nullptr, *post = nullptr, *post_c = nullptr;
 };
 struct DevSeeds {
     int L = -1;
@@ -106,24 +106,26 @@ static const DevRecurrence *get_recurrence(int dev, int norm, int L)
         g_retired.push_back(r.alpha);
         g_retired.push_back(r.s);
         g_retired.push_back(r.post);
+        g_retired.push_back(r.post_c);
         r = DevRecurrence{};
     }
     RecurrenceTables t = build_recurrence(norm, Lb);
     r.alpha = dev_upload(t.alpha);
     r.s = dev_upload(t.s);
     r.post = dev_upload(t.post);
-    if (!r.alpha || !r.s || !r.post) { r = DevRecurrence{}; return nullptr; }
+    r.post_c = dev_upload(t.post_c);
+    if (!r.alpha || !r.s || !r.post || !r.post_c) { r = DevRecurrence{}; return nullptr; }
     r.L = Lb;
     return &r;
 }
 
-static const DevSeeds *get_seeds(int dev, int norm, int csphase, int L)
+static const DevSeeds *get_seeds(int dev, int norm, int csphase, int L, bool cplx = false)
 {
-    auto &r = g_seeds[{dev, norm, csphase}];
+    auto &r = g_seeds[{dev, norm + (cplx ? 8 : 0), csphase}];
     if (r.L >= L) return &r;
     const int Lb = grown_degree(L, r.L);
     if (r.L >= 0) { g_retired.push_back(r.seed); r = DevSeeds{}; }
-    r.seed = dev_upload(build_seeds(norm, csphase, Lb));
+    r.seed = dev_upload(build_seeds(norm, csphase, Lb, cplx));
     if (!r.seed) { r = DevSeeds{}; return nullptr; }
     r.L = Lb;
     return &r;
@@ -451,6 +453,96 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
     const int ntm = (d2 + kTM - 1) / kTM, ntl = (d1 + kTL - 1) / kTL;
     hipLaunchKernelGGL(k_unpack_anal, dim3(ntm * ntl), dim3(kThreads), 0, st, cbuf, ptab, rt.L, Lc, cilm, d1, d2,
                        ntm);
+    if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
+    (void)hipFreeAsync(ws, st);
+    return ST_OK;
+}
+
+// Complex synthesis (MakeGridDHC / MakeGridGLQC): pack both coefficient
+// planes, the real Legendre kernel once per sign of m, complex ring FFTs.
+static int launch_synthesis_c(const Geometry &g, const DevRecurrence &rt, const DevSeeds &sd, const double2 *cilm,
+                              int d1, int d2, int Lc, int nrows_out, int ext_col, double2 *grid, long long ld,
+                              hipStream_t st)
+{
+    const size_t npack = (size_t)tri_size(Lc);
+    const size_t nf = (size_t)(Lc + 1) * g.P;
+    const bool lds_fft = fft_in_lds(g.fft.m);
+    const int nwg = std::min(g.P, kWsGroups);
+    const size_t nws = lds_fft ? 0 : (size_t)nwg * ws_stride(g.fft.m);
+    void *ws = nullptr;
+    const size_t bytes = 2 * npack * sizeof(double2) + 2 * nf * sizeof(double4) + nws * sizeof(double2);
+    if (!HIP_OK(hipMallocAsync(&ws, bytes, st))) return ST_ALLOC;
+    double2 *cp_p = (double2 *)ws, *cp_m = cp_p + npack;
+    double4 *fb_p = (double4 *)(cp_m + npack), *fb_m = fb_p + nf;
+    double2 *fws = (double2 *)(fb_m + nf);
+    {
+        ProfScope ps(SHTG_K_PACK, st);
+        const int ntm = (Lc + kTM) / kTM, ntl = (Lc + kTL) / kTL;
+        hipLaunchKernelGGL(k_pack_synth_c, dim3(ntm * ntl), dim3(kThreads), 0, st, cilm, d1, d2, rt.s, rt.L, cp_p,
+                           cp_m, Lc, ntm);
+    }
+    {
+        ProfScope ps(SHTG_K_SYNTH_LEG, st);
+        const int nsb = (g.P + kSynthKMax * kWave - 1) / (kSynthKMax * kWave);
+        hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nsb), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, cp_p,
+                           Lc, g.P, nsb, fb_p);
+        hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nsb), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, cp_m,
+                           Lc, g.P, nsb, fb_m);
+    }
+    {
+        ProfScope ps(SHTG_K_SYNTH_FFT, st);
+        const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
+        if (lds_fft)
+            hipLaunchKernelGGL(k_synth_fft_c<false>, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, fb_p, fb_m,
+                               g.P, Lc, g.u, g.rows, nrows_out, grid, ld, ext_col, (double2 *)nullptr, 0LL);
+        else
+            hipLaunchKernelGGL(k_synth_fft_c<true>, dim3(nwg), dim3(kFftMaxThreads), 0, st, g.fft, fb_p, fb_m, g.P, Lc,
+                               g.u, g.rows, nrows_out, grid, ld, ext_col, fws, ws_stride(g.fft.m));
+    }
+    if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
+    (void)hipFreeAsync(ws, st);
+    return ST_OK;
+}
+
+// Complex analysis (SHExpandDHC / SHExpandGLQC).
+static int launch_analysis_c(const Geometry &g, const DevRecurrence &rt, const DevSeeds &sd, const double2 *grid,
+                             long long ld, int Lc, double2 *cilm, int d1, int d2, hipStream_t st)
+{
+    const size_t npack = (size_t)tri_size(Lc);
+    const size_t nf = (size_t)(Lc + 1) * g.P;
+    const bool lds_fft = fft_in_lds(g.fft.m);
+    const int nwg = std::min(g.P, kWsGroups);
+    const size_t nws = lds_fft ? 0 : (size_t)nwg * ws_stride(g.fft.m);
+    void *ws = nullptr;
+    const size_t bytes = 2 * npack * sizeof(double2) + 2 * nf * sizeof(double4) + nws * sizeof(double2);
+    if (!HIP_OK(hipMallocAsync(&ws, bytes, st))) return ST_ALLOC;
+    double2 *cb_p = (double2 *)ws, *cb_m = cb_p + npack;
+    double4 *gb_p = (double4 *)(cb_m + npack), *gb_m = gb_p + nf;
+    double2 *fws = (double2 *)(gb_m + nf);
+    {
+        ProfScope ps(SHTG_K_ANAL_FFT, st);
+        const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
+        if (lds_fft)
+            hipLaunchKernelGGL(k_anal_fft_c<false>, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, grid, ld,
+                               g.u, g.rows, g.nlat, g.w, g.P, Lc, gb_p, gb_m, (double2 *)nullptr, 0LL);
+        else
+            hipLaunchKernelGGL(k_anal_fft_c<true>, dim3(nwg), dim3(kFftMaxThreads), 0, st, g.fft, grid, ld, g.u,
+                               g.rows, g.nlat, g.w, g.P, Lc, gb_p, gb_m, fws, ws_stride(g.fft.m));
+    }
+    {
+        ProfScope ps(SHTG_K_ANAL_LEG, st);
+        const AnalLegKernel kern = anal_leg_kernel(anal_lanes_per_slot(g.P - g.s0));
+        hipLaunchKernelGGL(kern, dim3(Lc + 1), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gb_p, Lc, g.s0, g.P,
+                           cb_p);
+        hipLaunchKernelGGL(kern, dim3(Lc + 1), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gb_m, Lc, g.s0, g.P,
+                           cb_m);
+    }
+    {
+        ProfScope ps(SHTG_K_UNPACK, st);
+        const int ntm = (d2 + kTM - 1) / kTM, ntl = (d1 + kTL - 1) / kTL;
+        hipLaunchKernelGGL(k_unpack_anal_c, dim3(ntm * ntl), dim3(kThreads), 0, st, cb_p, cb_m, rt.post_c, rt.L,
+                           Lc, cilm, d1, d2, ntm);
+    }
     if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
     (void)hipFreeAsync(ws, st);
     return ST_OK;
@@ -974,14 +1066,14 @@ struct CallTables {
 };
 
 static int get_tables(bool dh, int lmax, int sampling, const double *zero, const double *wq, int norm,
-                      int csphase, int Lc, CallTables &t)
+                      int csphase, int Lc, CallTables &t, bool cplx = false)
 {
     const int dev = current_device();
     if (dev < 0) return ST_OTHER;
     configure_kernels();
     t.g = dh ? get_dh(dev, lmax, sampling) : get_glq(dev, lmax, zero, wq);
     t.rt = get_recurrence(dev, norm, Lc);
-    t.sd = get_seeds(dev, norm, csphase, Lc);
+    t.sd = get_seeds(dev, norm, csphase, Lc, cplx);
     if (!t.g || !t.rt || !t.sd) return ST_ALLOC;
     return ST_OK;
 }
@@ -1128,6 +1220,67 @@ int shtg_shexpand_glq_adjoint(const double *cilm_bar, int cilm_d1, int cilm_d2, 
     CallTables t;
     if ((st = get_tables(false, sh.lmax, 0, zero, w, norm, csphase, sh.Lc, t))) return st;
     return adjoint_of_analysis(t, sh, cilm_bar, cilm_d1, cilm_d2, grid_bar, grid_d0, grid_d1, (hipStream_t)stream);
+}
+
+int shtg_makegrid_dhc(const double *cilm, int cilm_d1, int cilm_d2, int lmax, int norm, int sampling, int csphase,
+                      int lmax_calc, int extend, double *grid, int grid_d0, int grid_d1, void *stream)
+{
+    Shape sh;
+    int st = check_makegrid_dh(cilm_d1, cilm_d2, lmax, norm, sampling, csphase, lmax_calc, extend, grid_d0, grid_d1,
+                               sh);
+    if (st) return st;
+    if (!cilm || !grid) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    CallTables t;
+    if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t, true))) return st;
+    return launch_synthesis_c(*t.g, *t.rt, *t.sd, (const double2 *)cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend,
+                              (double2 *)grid, grid_d1, (hipStream_t)stream);
+}
+
+int shtg_shexpand_dhc(const double *grid, int grid_d0, int grid_d1, int norm, int sampling, int csphase,
+                      int lmax_calc, double *cilm, int cilm_d1, int cilm_d2, void *stream)
+{
+    Shape sh;
+    int st = check_shexpand_dh(grid_d0, grid_d1, norm, sampling, csphase, lmax_calc, cilm_d1, cilm_d2, sh);
+    if (st) return st;
+    if (!cilm || !grid) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    CallTables t;
+    if ((st = get_tables(true, sh.lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t, true))) return st;
+    return launch_analysis_c(*t.g, *t.rt, *t.sd, (const double2 *)grid, grid_d1, sh.Lc, (double2 *)cilm, cilm_d1,
+                             cilm_d2, (hipStream_t)stream);
+}
+
+int shtg_makegrid_glqc(const double *cilm, int cilm_d1, int cilm_d2, int lmax, const double *zero, int nzero,
+                       int norm, int csphase, int lmax_calc, int extend, double *grid, int grid_d0, int grid_d1,
+                       void *stream)
+{
+    Shape sh;
+    int st = check_makegrid_glq(cilm_d1, cilm_d2, lmax, nzero, zero, norm, csphase, lmax_calc, extend, grid_d0,
+                                grid_d1, sh);
+    if (st) return st;
+    if (!cilm || !grid) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    CallTables t;
+    if ((st = get_tables(false, lmax, 0, zero, nullptr, norm, csphase, sh.Lc, t, true))) return st;
+    return launch_synthesis_c(*t.g, *t.rt, *t.sd, (const double2 *)cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend,
+                              (double2 *)grid, grid_d1, (hipStream_t)stream);
+}
+
+int shtg_shexpand_glqc(const double *grid, int grid_d0, int grid_d1, const double *w, int nw, const double *zero,
+                       int nzero, int norm, int csphase, int lmax_calc, double *cilm, int cilm_d1, int cilm_d2,
+                       void *stream)
+{
+    Shape sh;
+    int st = check_shexpand_glq(grid_d0, grid_d1, w, nw, zero, nzero, norm, csphase, lmax_calc, cilm_d1, cilm_d2,
+                                sh);
+    if (st) return st;
+    if (!cilm || !grid) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    CallTables t;
+    if ((st = get_tables(false, sh.lmax, 0, zero, w, norm, csphase, sh.Lc, t, true))) return st;
+    return launch_analysis_c(*t.g, *t.rt, *t.sd, (const double2 *)grid, grid_d1, sh.Lc, (double2 *)cilm, cilm_d1,
+                             cilm_d2, (hipStream_t)stream);
 }
 
 int shtg_shglq(int lmax, double *zero, double *w)

@@ -27,6 +27,7 @@ struct RecurrenceTables {
     std::vector<double> alpha;  // tri: alpha_{m+j}; entry j = 0 is 0
     std::vector<double> s;      // tri: s_{m+j}
     std::vector<double> post;   // tri: s_{m+j} * (analysis epilogue factor)
+    std::vector<double> post_c; // tri: the same for the complex analysis
 };
 
 // ff1/ff2 of the reference for one order m, l = m..L (index l).
@@ -72,8 +73,10 @@ inline void ref_ff_column(int norm, int m, int L, const std::vector<double> &sqr
     }
 }
 
-// Analysis epilogue factor per (l, m): src/SHExpandDH.f95:674-702.
-inline void epilogue_factors(int norm, int L, std::vector<double> &epi /* [l*(L+1)+m] */)
+// Analysis epilogue factor per (l, m): src/SHExpandDH.f95:674-702.  The
+// complex analysis (src/SHExpandDHC.f95:681-723) omits the halving of the
+// unnormalised (norm 3) product for m >= 1.
+inline void epilogue_factors(int norm, int L, std::vector<double> &epi /* [l*(L+1)+m] */, bool cplx = false)
 {
     const double pi = std::acos(-1.0);
     epi.assign((size_t)(L + 1) * (L + 1), 0.0);
@@ -85,7 +88,7 @@ inline void epilogue_factors(int norm, int L, std::vector<double> &epi /* [l*(L+
         } else if (norm == 3) {
             double prod = 4 * pi / (double)(2 * l + 1);
             epi[(size_t)l * (L + 1)] = 1.0 / prod;
-            prod = prod / 2.0;
+            if (!cplx) prod = prod / 2.0;
             for (int m = 1; m <= l - 1; ++m) {
                 prod = prod * (l + m) * (l - m + 1);
                 epi[(size_t)l * (L + 1) + m] = 1.0 / prod;
@@ -105,10 +108,12 @@ inline RecurrenceTables build_recurrence(int norm, int L)
     t.alpha.assign(n, 0.0);
     t.s.assign(n, 0.0);
     t.post.assign(n, 0.0);
+    t.post_c.assign(n, 0.0);
     std::vector<double> sqr(2 * L + 4, 0.0);
     for (int k = 1; k < (int)sqr.size(); ++k) sqr[k] = std::sqrt((double)k);
-    std::vector<double> epi;
+    std::vector<double> epi, epi_c;
     epilogue_factors(norm, L, epi);
+    epilogue_factors(norm, L, epi_c, true);
     std::vector<double> ff1, ff2;
     for (int m = 0; m <= L; ++m) {
         ref_ff_column(norm, m, L, sqr, ff1, ff2);
@@ -122,6 +127,7 @@ inline RecurrenceTables build_recurrence(int norm, int L)
             t.alpha[o + (l - m)] = a;
             t.s[o + (l - m)] = s;
             t.post[o + (l - m)] = s * epi[(size_t)l * (L + 1) + m];
+            t.post_c[o + (l - m)] = s * epi_c[(size_t)l * (L + 1) + m];
             s_lm2 = s_lm1;
             s_lm1 = s;
         }
@@ -130,8 +136,11 @@ inline RecurrenceTables build_recurrence(int norm, int L)
 }
 
 // Seeds q_m of each order: P_00 for m = 0 (MakeGridDH.f95:516-519) and the
-// 1e-280-scaled sectoral chain for m >= 1 (MakeGridDH.f95:542-564).
-inline std::vector<double> build_seeds(int norm, int csphase, int L)
+// 1e-280-scaled sectoral chain for m >= 1 (MakeGridDH.f95:542-564).  The
+// complex transforms' functions carry no sqrt(2) for m >= 1: their chain
+// starts at 1e-280 (norms 1, 2, 3) or 1e-280/sqrt(4 pi) (norm 4)
+// (MakeGridDHC.f95:485-489).
+inline std::vector<double> build_seeds(int norm, int csphase, int L, bool cplx = false)
 {
     const double pi = std::acos(-1.0), scalef = 1.0e-280;
     std::vector<double> seed(L + 1, 0.0);
@@ -140,9 +149,9 @@ inline std::vector<double> build_seeds(int norm, int csphase, int L)
     seed[0] = (norm == 4) ? 1.0 / std::sqrt(4.0 * pi) : 1.0;
     double pmm;
     switch (norm) {
-    case 1: case 2: pmm = sqr[2] * scalef; break;
+    case 1: case 2: pmm = cplx ? scalef : sqr[2] * scalef; break;
     case 3: pmm = scalef; break;
-    default: pmm = sqr[2] * scalef / std::sqrt(4.0 * pi); break;
+    default: pmm = (cplx ? scalef : sqr[2] * scalef) / std::sqrt(4.0 * pi); break;
     }
     for (int m = 1; m <= L; ++m) {
         if (norm == 1 || norm == 4) {

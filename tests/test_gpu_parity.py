@@ -118,7 +118,7 @@ def test_errors_match_reference(B):
     with pytest.raises(B.SHToolsError, match="Improper dimensions of input array."):
         B.SHExpandDH(np.zeros((8, 10)), sampling=2)
     with pytest.raises(NotImplementedError):
-        B.MakeGridDHC(c)
+        B.SHRotateRealCoef(c, None, None)
 
 
 @pytest.mark.parametrize("L,sampling", [(127, 2), (255, 1), (300, 2)])

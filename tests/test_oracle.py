@@ -233,3 +233,68 @@ def test_error_codes_match_reference():
         oracle.SHExpandDH(np.zeros((7, 14)), sampling=2)
     with pytest.raises(oracle.OracleError, match="Improper dimensions"):
         oracle.SHExpandDH(np.zeros((8, 10)), sampling=2)
+
+
+# ---------------------------------------------------------------- complex transforms
+
+from helpers import coef_err, for_norm, grid_err, rand_cilm, real_to_complex  # noqa: E402
+
+CPLX_DH = [(L, norm, cs, s, e) for L in (0, 1, 2, 5, 12) for norm in (1, 2, 3, 4) for cs in (1, -1)
+           for s in (1, 2) for e in (0, 1)]
+
+
+@pytest.mark.parametrize("L,norm,cs,sampling,extend", CPLX_DH)
+def test_makegriddhc_is_complex_linear_extension_of_real(L, norm, cs, sampling, extend):
+    """MakeGridDHC of the complex forms of real fields A, B (coefficients
+    a + i b) equals MakeGridDH(A) + i MakeGridDH(B); complex-linearity makes
+    this fix the map.  The analysis of a real grid gives the complex form of
+    its real coefficients, and SHExpandDHC inverts MakeGridDHC."""
+    rng = np.random.default_rng(300 + L * 7 + norm)
+    A = for_norm(rand_cilm(rng, L), norm)
+    B = for_norm(rand_cilm(rng, L), norm)
+    c = real_to_complex(A, norm) + 1j * real_to_complex(B, norm)
+    kw = dict(norm=norm, csphase=cs, sampling=sampling)
+    g = oracle.MakeGridDHC(c, extend=extend, **kw)
+    ref = oracle.MakeGridDH(A, extend=extend, **kw) + 1j * oracle.MakeGridDH(B, extend=extend, **kw)
+    assert g.shape == ref.shape
+    assert grid_err(g, ref) <= 1e-13
+    gin = g[: g.shape[0] - extend, : g.shape[1] - extend]
+    tol = 1e-13 if norm != 3 else 1e-9
+    assert coef_err(oracle.SHExpandDHC(gin, **kw), c) <= tol
+    ga = oracle.MakeGridDH(A, **kw)
+    assert coef_err(oracle.SHExpandDHC(ga, **kw), real_to_complex(A, norm)) <= tol
+
+
+CPLX_GLQ = [(L, norm, cs, e) for L in (0, 1, 2, 5, 6, 13) for norm in (1, 2, 3, 4) for cs in (1, -1) for e in (0, 1)]
+
+
+@pytest.mark.parametrize("L,norm,cs,extend", CPLX_GLQ)
+def test_makegridglqc_is_complex_linear_extension_of_real(L, norm, cs, extend):
+    rng = np.random.default_rng(400 + L * 7 + norm)
+    A = for_norm(rand_cilm(rng, L), norm)
+    B = for_norm(rand_cilm(rng, L), norm)
+    c = real_to_complex(A, norm) + 1j * real_to_complex(B, norm)
+    zero, w = oracle.SHGLQ(L)
+    kw = dict(norm=norm, csphase=cs)
+    g = oracle.MakeGridGLQC(c, zero, extend=extend, **kw)
+    ref = oracle.MakeGridGLQ(A, zero, extend=extend, **kw) + 1j * oracle.MakeGridGLQ(B, zero, extend=extend, **kw)
+    assert grid_err(g, ref) <= 1e-13
+    tol = 1e-13 if norm != 3 else 1e-9
+    assert coef_err(oracle.SHExpandGLQC(g[:, : g.shape[1] - extend], w, zero, **kw), c) <= tol
+
+
+def test_complex_lmax_calc_and_padding():
+    rng = np.random.default_rng(77)
+    L = 9
+    A = rand_cilm(rng, L)
+    c = real_to_complex(A, 1)
+    g1 = oracle.MakeGridDHC(c, lmax=12, lmax_calc=6, sampling=2)
+    A6 = A.copy()
+    A6[:, 7:, :] = 0.0
+    g2 = oracle.MakeGridDH(A6, lmax=12, sampling=2)
+    assert grid_err(g1, g2) <= 1e-13
+    c6 = oracle.SHExpandDHC(oracle.MakeGridDH(A, sampling=2), sampling=2, lmax_calc=6)
+    assert c6.shape == (2, 7, 7)
+    assert coef_err(c6, real_to_complex(A, 1)[:, :7, :7]) <= 1e-13
+    with pytest.raises(oracle.OracleError):
+        oracle.MakeGridDHC(c, lmax_calc=L + 1)

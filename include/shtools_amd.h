@@ -83,6 +83,16 @@ int shtg_shexpand_glqc(const double *grid, int grid_d0, int grid_d1, const doubl
                        const double *zero, int nzero, int norm, int csphase, int lmax_calc,
                        double *cilm, int cilm_d1, int cilm_d2, void *stream);
 
+/* MakeGradientDH: src/MakeGradientDH.f95:1-658 (f2py src/pyshtools.pyf:
+ * 2021-2043).  Horizontal gradient of a 4pi-normalised real field without
+ * the Condon-Shortley phase: theta = (1/r) dF/dtheta and phi =
+ * (1/(r sin theta)) dF/dphi on DH grids of 2(lmax+1)+extend rows and
+ * sampling*2(lmax+1)+extend columns (both [grid_d0][grid_d1]); the pole rows
+ * are zero. */
+int shtg_makegradient_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax, int sampling,
+                         int lmax_calc, int extend, double radius, double *theta, double *phi,
+                         int grid_d0, int grid_d1, void *stream);
+
 /* Adjoints (transposes) of the four transforms, for gradients through them
  * (no counterpart in the reference; the differentiable path of the
  * transform API).  Each takes the argument list of its forward call with

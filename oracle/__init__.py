@@ -65,6 +65,7 @@ def lib():
         L.or_shexpanddhc.argtypes = [dp, i, i, i, i, i, i, dp, i, i]
         L.or_makegridglqc.argtypes = [dp, i, i, i, dp, i, i, i, i, i, dp, i, i]
         L.or_shexpandglqc.argtypes = [dp, i, i, i, dp, i, dp, i, i, i, i, dp, i, i]
+        L.or_makegradientdh.argtypes = [dp, i, i, i, i, i, i, ctypes.c_double, dp, dp, i, i]
         L.or_r2c.argtypes = [i, dp, dp]
         L.or_c2r.argtypes = [i, dp, dp]
         lp = ctypes.POINTER(ctypes.c_longlong)
@@ -214,6 +215,21 @@ def SHExpandGLQC(gridglq, w, zero, norm=1, csphase=1, lmax_calc=None):
                                  _p(zero), zero.size, norm, csphase, lmax_calc, _p(cilm),
                                  lmax_calc + 1, lmax_calc + 1))
     return cilm
+
+
+def MakeGradientDH(cilm, lmax=None, sampling=2, lmax_calc=None, extend=0, radius=1.0):
+    cilm = np.ascontiguousarray(cilm, dtype=np.float64)
+    if lmax is None:
+        lmax = cilm.shape[1] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    g0 = 2 * (lmax + 1) + extend
+    g1 = sampling * 2 * (lmax + 1) + extend
+    theta = np.zeros((g0, g1))
+    phi = np.zeros((g0, g1))
+    _check(lib().or_makegradientdh(_p(cilm), cilm.shape[1], cilm.shape[2], lmax, sampling, lmax_calc, extend,
+                                   float(radius), _p(theta), _p(phi), g0, g1))
+    return theta, phi
 
 
 def rfft(x):

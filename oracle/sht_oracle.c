@@ -1403,6 +1403,194 @@ int or_shexpandglqc(const double *grid_ri, int g0, int g1, int lmax, const doubl
     return 0;
 }
 
+/* ------------------------------------------------------ gradient */
+
+/* Horizontal gradient of a 4pi-normalised real field (no Condon-Shortley
+ * phase) on a DH grid: theta = (1/r) dF/dtheta, phi = (1/(r sin theta))
+ * dF/dphi, with dP_lm/dz evaluated term by term as
+ * (sqrt((2l+1)(l^2-m^2)/(2l-1)) P_l-1,m - l z P_lm) / u^2.
+ * src/MakeGradientDH.f95:1-658. */
+int or_makegradientdh(const double *cilm, int d1, int d2, int lmax, int sampling, int lmax_calc,
+                      int extend, double radius, double *theta_g, double *phi_g, int g0, int g1)
+{
+    int n = 2 * lmax + 2, nlong, nlat_out, nlong_out;
+    if (sampling == 1) nlong = n; else if (sampling == 2) nlong = 2 * n; else return 2;
+    if (extend == 0) { nlat_out = n; nlong_out = nlong; }
+    else if (extend == 1) { nlat_out = n + 1; nlong_out = nlong + 1; }
+    else return 2;
+    if (g0 < nlat_out || g1 < nlong_out) return 1;
+    if (lmax_calc > lmax) return 2;
+    int lc = lc_of_makegrid(d1, d2, lmax, lmax_calc);
+    double pi = acos(-1.0), scalef = 1.0e-280, r = radius;
+#define TG(i, k) theta_g[(size_t)(i) * g1 + (k)]
+#define PG(i, k) phi_g[(size_t)(i) * g1 + (k)]
+    tables t;
+    int st = build_tables(&t, lc, 1);
+    double *cf = to_fortran_order(cilm, d1, d2, lc);
+    if (st || !cf) { free_tables(&t); free(cf); return 3; }
+    const double *sqr = t.sqr;
+    int ncl = nlong / 2 + 1, i_eq = n / 2 + 1;
+    fftplan *plan = plan_create(nlong);
+    cplx *coef = (cplx *)malloc(sizeof(cplx) * ncl);
+    cplx *buf = (cplx *)malloc(sizeof(cplx) * nlong);
+    double *row = (double *)malloc(sizeof(double) * nlong);
+    cplx *ct = (cplx *)malloc(sizeof(cplx) * (lmax + 2)), *cts = (cplx *)malloc(sizeof(cplx) * (lmax + 2));
+    cplx *cp = (cplx *)malloc(sizeof(cplx) * (lmax + 2)), *cps = (cplx *)malloc(sizeof(cplx) * (lmax + 2));
+    /* c2r of (c0, c[1..lmax]/2) with zeros above lmax */
+#define EMIT(c0v, carr, dest, i, fac, divsin)                                            \
+    do {                                                                                 \
+        for (int k = 0; k < ncl; k++) coef[k] = 0;                                       \
+        coef[0] = (c0v);                                                                 \
+        for (int k = 1; k <= lmax && k < ncl; k++) coef[k] = (carr)[k] / 2.0;            \
+        c2r(plan, coef, buf, row);                                                       \
+        for (int k = 0; k < nlong; k++)                                                  \
+            dest(i, k) = (divsin) ? row[k] / r / (fac) : -(fac) * row[k] / r;            \
+    } while (0)
+    double pm2, pm1, p, dpl, pmm, rescalem, tempr, ct0, cts0, cp0, cps0;
+    cplx tempc;
+    /* equator: MakeGradientDH.f95:322-430 */
+    {
+        double theta = pi / 2.0, z = 0.0;
+        for (int k = 0; k < lmax + 2; k++) { ct[k] = 0; cp[k] = 0; }
+        ct0 = 0.0; cp0 = 0.0;
+        pm2 = 1.0;
+        pm1 = 0.0;
+        if (lc != 0) {
+            pm1 = 0.0;
+            dpl = FF(&t, ff1, 1, 0);
+            ct0 = ct0 + CF(0, 1, 0) * dpl;
+        }
+        for (int l = 2; l <= lc; l++) {
+            p = -FF(&t, ff2, l, 0) * pm2;
+            dpl = l * (sqr[2 * l + 1] / sqr[2 * l - 1] * pm1);
+            ct0 = ct0 + CF(0, l, 0) * dpl;
+            pm2 = pm1;
+            pm1 = p;
+        }
+        pmm = sqr[2] * scalef;
+        rescalem = 1.0 / scalef;
+        for (int m = 1; m <= lc - 1; m++) {
+            pmm = pmm * sqr[2 * m + 1] / sqr[2 * m];
+            pm2 = pmm;
+            cp[m] += (CF(1, m, m) + I * CF(0, m, m)) * pm2 * m;
+            pm1 = 0.0;
+            dpl = sqr[2 * m + 3] * pmm;
+            ct[m] += (CF(0, m + 1, m) - I * CF(1, m + 1, m)) * dpl;
+            for (int l = m + 2; l <= lc; l++) {
+                p = -FF(&t, ff2, l, m) * pm2;
+                cp[m] += (CF(1, l, m) + I * CF(0, l, m)) * p * m;
+                dpl = sqr[2 * l + 1] * sqr[l - m] * sqr[l + m] / sqr[2 * l - 1] * pm1;
+                ct[m] += (CF(0, l, m) - I * CF(1, l, m)) * dpl;
+                pm2 = pm1;
+                pm1 = p;
+            }
+            cp[m] *= rescalem;
+            ct[m] *= rescalem;
+        }
+        if (lc != 0) {
+            pmm = pmm * sqr[2 * lc + 1] / sqr[2 * lc] * rescalem;
+            cp[lc] += (CF(1, lc, lc) + I * CF(0, lc, lc)) * pmm * lc;
+            dpl = -lc * z * pmm / 1.0;
+            ct[lc] += (CF(0, lc, lc) - I * CF(1, lc, lc)) * dpl;
+        }
+        EMIT(ct0, ct, TG, i_eq - 1, sin(theta), 0);
+        EMIT(cp0, cp, PG, i_eq - 1, sin(theta), 1);
+    }
+    /* ring pairs: MakeGradientDH.f95:432-640 */
+    for (int i = 1; i <= i_eq - 1; i++) {
+        int i_s = 2 * i_eq - i;
+        double theta = pi * (double)(i - 1) / (double)n;
+        double z = cos(theta);
+        double u = sqrt((1.0 - z) * (1.0 + z));
+        double u2 = u * u;
+        for (int k = 0; k < lmax + 2; k++) { ct[k] = 0; cts[k] = 0; cp[k] = 0; cps[k] = 0; }
+        ct0 = cts0 = cp0 = cps0 = 0.0;
+        pm2 = 1.0;
+        pm1 = 0.0;
+        if (lc != 0) {
+            pm1 = FF(&t, ff1, 1, 0) * z;
+            dpl = FF(&t, ff1, 1, 0);
+            tempr = CF(0, 1, 0) * dpl;
+            ct0 = ct0 + tempr;
+            cts0 = cts0 + tempr;
+        }
+        for (int l = 2; l <= lc; l++) {
+            p = FF(&t, ff1, l, 0) * z * pm1 - FF(&t, ff2, l, 0) * pm2;
+            dpl = l * (sqr[2 * l + 1] / sqr[2 * l - 1] * pm1 - z * p) / u2;
+            tempr = CF(0, l, 0) * dpl;
+            ct0 = ct0 + tempr;
+            cts0 = cts0 - tempr * fsym(l, 0);
+            pm2 = pm1;
+            pm1 = p;
+        }
+        pmm = sqr[2] * scalef;
+        rescalem = 1.0 / scalef;
+        for (int m = 1; m <= lc - 1; m++) {
+            rescalem = rescalem * u;
+            pmm = pmm * sqr[2 * m + 1] / sqr[2 * m];
+            pm2 = pmm;
+            tempc = (CF(1, m, m) + I * CF(0, m, m)) * pm2 * m;
+            cp[m] += tempc; cps[m] += tempc;
+            dpl = -m * z * pm2 / u2;
+            tempc = (CF(0, m, m) - I * CF(1, m, m)) * dpl;
+            ct[m] += tempc; cts[m] -= tempc;
+            pm1 = z * FF(&t, ff1, m + 1, m) * pm2;
+            tempc = (CF(1, m + 1, m) + I * CF(0, m + 1, m)) * pm1 * m;
+            cp[m] += tempc; cps[m] -= tempc;
+            dpl = (sqr[2 * m + 3] * pmm - z * (m + 1) * pm1) / u2;
+            tempc = (CF(0, m + 1, m) - I * CF(1, m + 1, m)) * dpl;
+            ct[m] += tempc; cts[m] += tempc;
+            for (int l = m + 2; l <= lc; l++) {
+                p = z * FF(&t, ff1, l, m) * pm1 - FF(&t, ff2, l, m) * pm2;
+                tempc = (CF(1, l, m) + I * CF(0, l, m)) * p * m;
+                cp[m] += tempc; cps[m] += tempc * fsym(l, m);
+                dpl = (sqr[2 * l + 1] * sqr[l - m] * sqr[l + m] / sqr[2 * l - 1] * pm1 - z * l * p) / u2;
+                tempc = (CF(0, l, m) - I * CF(1, l, m)) * dpl;
+                ct[m] += tempc; cts[m] -= tempc * fsym(l, m);
+                pm2 = pm1;
+                pm1 = p;
+            }
+            ct[m] *= rescalem; cts[m] *= rescalem;
+            cp[m] *= rescalem; cps[m] *= rescalem;
+        }
+        if (lc != 0) {
+            rescalem = rescalem * u;
+            pmm = pmm * sqr[2 * lc + 1] / sqr[2 * lc] * rescalem;
+            tempc = (CF(1, lc, lc) + I * CF(0, lc, lc)) * pmm * lc;
+            cp[lc] += tempc; cps[lc] += tempc;
+            dpl = -lc * z * pmm / u2;
+            tempc = (CF(0, lc, lc) - I * CF(1, lc, lc)) * dpl;
+            ct[lc] += tempc; cts[lc] -= tempc;
+        }
+        if (i == 1) {
+            for (int k = 0; k < nlong; k++) { TG(0, k) = 0.0; PG(0, k) = 0.0; }
+        } else {
+            EMIT(ct0, ct, TG, i - 1, sin(theta), 0);
+            EMIT(cp0, cp, PG, i - 1, sin(theta), 1);
+        }
+        if (!(i == 1 && extend == 0)) {
+            if (i == 1) {
+                for (int k = 0; k < nlong; k++) { TG(i_s - 1, k) = 0.0; PG(i_s - 1, k) = 0.0; }
+            } else {
+                EMIT(cts0, cts, TG, i_s - 1, sin(theta), 0);
+                EMIT(cps0, cps, PG, i_s - 1, sin(theta), 1);
+            }
+        }
+    }
+#undef EMIT
+    if (extend == 1)
+        for (int i = 0; i < nlat_out; i++) {
+            TG(i, nlong_out - 1) = TG(i, 0);
+            PG(i, nlong_out - 1) = PG(i, 0);
+        }
+#undef TG
+#undef PG
+    free(coef); free(buf); free(row); free(ct); free(cts); free(cp); free(cps); free(cf);
+    plan_destroy(plan);
+    free_tables(&t);
+    return 0;
+}
+
 /* ------------------------------------------- batched CPU round trips */
 
 /* One MakeGridDH + SHExpandDH round trip per field, fields spread over the

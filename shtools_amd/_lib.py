@@ -29,6 +29,8 @@ SIGNATURES = {
                            _c_int, _vp],
     "shtg_shexpand_glqc": [_vp, _c_int, _c_int, _dp, _c_int, _dp, _c_int, _c_int, _c_int, _c_int, _vp, _c_int,
                            _c_int, _vp],
+    "shtg_makegradient_dh": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, ctypes.c_double, _vp, _vp, _c_int,
+                             _c_int, _vp],
     "shtg_makegrid_dh_adjoint": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int,
                                  _c_int, _vp],
     "shtg_shexpand_dh_adjoint": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _vp],

@@ -17,6 +17,8 @@ pyshtools/backends/shtools.py:189-229):
     DHaj(n, extend=0) -> aj
     MakeGridDHC, SHExpandDHC, MakeGridGLQC, SHExpandGLQC: the complex
     transforms with the same arguments (src/pyshtools.pyf:278-319,379-422)
+    MakeGradientDH(cilm, lmax, sampling=2, lmax_calc, extend=0, radius=1.0)
+        -> (theta, phi)
 
 Array arguments may be numpy arrays (copied to the GPU and back, like f2py
 copies between C and Fortran order) or float64 (complex128 for the complex
@@ -321,6 +323,35 @@ def SHExpandGLQC(gridglq, w, zero, norm=1, csphase=1, lmax_calc=None):
     return _finish(cilm, as_tensor)
 
 
+# ------------------------------------------------------------ gradient
+
+def MakeGradientDH(cilm, lmax=None, sampling=2, lmax_calc=None, extend=0, radius=1.0):
+    """Horizontal gradient (theta, phi components) of a 4pi-normalised real
+    field on DH grids (src/MakeGradientDH.f95; f2py src/pyshtools.pyf:2021-2043)."""
+    torch = _torch()
+    as_tensor = _is_tensor(cilm)
+    dev = _device_of(cilm)
+    c = _to_dev(cilm, dev)
+    if c.dim() != 3 or c.shape[0] < 2:
+        raise SHToolsError(_status_message(1))
+    if lmax is None:
+        lmax = c.shape[1] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    if sampling not in (1, 2) or extend not in (0, 1) or lmax < 0:
+        raise SHToolsError(_status_message(2))
+    g0 = 2 * (lmax + 1) + extend
+    g1 = sampling * 2 * (lmax + 1) + extend
+    theta = torch.empty((g0, g1), dtype=torch.float64, device=dev)
+    phi = torch.empty((g0, g1), dtype=torch.float64, device=dev)
+    with torch.cuda.device(dev):
+        st = _lib.lib().shtg_makegradient_dh(c.data_ptr(), c.shape[1], c.shape[2], int(lmax), int(sampling),
+                                             int(lmax_calc), int(extend), float(radius), theta.data_ptr(),
+                                             phi.data_ptr(), g0, g1, _stream(dev))
+    _check(st)
+    return _finish(theta, as_tensor), _finish(phi, as_tensor)
+
+
 # ------------------------------------------------------------ host helpers
 
 def SHGLQ(lmax):
@@ -393,7 +424,6 @@ def _not_provided(name):
     return f
 
 
-MakeGradientDH = _not_provided("MakeGradientDH")
 SHRotateRealCoef = _not_provided("SHRotateRealCoef")
 
 

@@ -802,6 +802,80 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft_c(FftDev F, const d
         anal_fft_c_tile(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, Lc, s, gb_p, gb_m, x);
 }
 
+// ================================================================== gradient
+//
+// MakeGradientDH (src/MakeGradientDH.f95): with dP_lm/dz = (a_lm P_l-1,m -
+// l z P_lm) / u^2, a_lm = sqrt((2l+1)(l^2-m^2)/(2l-1)), the theta component
+// (1/r) dF/dtheta = -(sin theta / (r u^2)) (G_A - z G_B) and the phi component
+// (1/(r sin theta)) dF/dphi = G_Phi / (r sin theta), where G_X is the 4pi
+// synthesis of A_lm = a_l+1,m c_l+1,m, B_lm = l c_lm and
+// Phi_lm = (m S_lm, -m C_lm).
+
+// out: [3][2][Lc+1][Lc+1] (A, B, Phi)
+__global__ __launch_bounds__(kThreads) void k_gradient_coefs(const double *__restrict__ cilm, int d1, int d2, int Lc,
+                                                             double *__restrict__ out)
+{
+    const int L1 = Lc + 1;
+    const size_t plane = (size_t)L1 * L1;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < L1 * L1; e += gridDim.x * blockDim.x) {
+        const int l = e / L1, m = e - l * L1;
+        double c = 0.0, sv = 0.0, cn = 0.0, sn = 0.0;
+        if (m <= l) {
+            c = cilm[(size_t)l * d2 + m];
+            sv = (m == 0) ? 0.0 : cilm[((size_t)d1 + l) * d2 + m];
+            if (l + 1 <= Lc) {
+                cn = cilm[(size_t)(l + 1) * d2 + m];
+                sn = (m == 0) ? 0.0 : cilm[((size_t)d1 + l + 1) * d2 + m];
+            }
+        }
+        const int lp = l + 1;
+        const double a = (m <= l && lp <= Lc)
+                             ? sqrt((double)(2 * lp + 1)) * sqrt((double)(lp - m)) * sqrt((double)(lp + m)) /
+                                   sqrt((double)(2 * lp - 1))
+                             : 0.0;
+        out[0 * plane + e] = a * cn;
+        out[1 * plane + e] = a * sn;
+        out[2 * plane + e] = l * c;
+        out[3 * plane + e] = l * sv;
+        out[4 * plane + e] = m * sv;
+        out[5 * plane + e] = -m * c;
+    }
+}
+
+// theta = -(sin t / (r u^2)) (theta - z gb) and phi = phi / (r sin t) row by
+// row (t = pi i / n, z = cos t, u^2 = (1 - z)(1 + z)); the pole rows are zero.
+__global__ __launch_bounds__(kThreads) void k_gradient_combine(double *__restrict__ theta, double *__restrict__ phi,
+                                                               const double *__restrict__ gb, int nrows, int ncols,
+                                                               long long ld, int n, double radius)
+{
+    const double pi = 3.14159265358979323846;
+    const long long total = (long long)nrows * ncols;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(e / ncols), k = (int)(e - (long long)i * ncols);
+        const long long o = (long long)i * ld + k;
+        if (i == 0 || i == n) {
+            theta[o] = 0.0;
+            phi[o] = 0.0;
+            continue;
+        }
+        double t, z, st;
+        if (2 * i == n) {
+            t = pi / 2.0;
+            z = 0.0;
+        } else {
+            const int in = (2 * i < n) ? i : n - i;  // the northern ring of the pair
+            t = pi * (double)in / (double)n;
+            z = cos(t);
+            if (in != i) z = -z;
+        }
+        st = sin(t);
+        const double u2 = (1.0 - z) * (1.0 + z);
+        theta[o] = -st * ((theta[o] - z * gb[(long long)i * ld + k]) / u2) / radius;
+        phi[o] = phi[o] / radius / st;
+    }
+}
+
 // ============================================================= batched kernels
 
 // Per-field description of a batch (all fields: lmax_calc = lmax).

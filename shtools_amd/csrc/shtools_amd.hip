@@ -1283,6 +1283,42 @@ int shtg_shexpand_glqc(const double *grid, int grid_d0, int grid_d1, const doubl
                              cilm_d2, (hipStream_t)stream);
 }
 
+int shtg_makegradient_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax, int sampling, int lmax_calc,
+                         int extend, double radius, double *theta, double *phi, int grid_d0, int grid_d1,
+                         void *stream)
+{
+    // argument checks in the order of src/MakeGradientDH.f95:104-226
+    Shape sh;
+    int st = check_makegrid_dh(cilm_d1, cilm_d2, lmax, 1, sampling, 1, lmax_calc, extend, grid_d0, grid_d1, sh);
+    if (st) return st;
+    if (!cilm || !theta || !phi) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    CallTables t;
+    if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, 1, 1, sh.Lc, t))) return st;
+    hipStream_t hs = (hipStream_t)stream;
+    const int L1 = sh.Lc + 1, n = 2 * lmax + 2;
+    const size_t plane = (size_t)L1 * L1;
+    void *ws = nullptr;
+    const size_t ngrid = (size_t)grid_d0 * grid_d1;
+    if (!HIP_OK(hipMallocAsync(&ws, (6 * plane + ngrid) * sizeof(double), hs))) return ST_ALLOC;
+    double *coefs = (double *)ws, *gb = coefs + 6 * plane;
+    hipLaunchKernelGGL(k_gradient_coefs, dim3(grid_blocks((long long)plane)), dim3(kThreads), 0, hs, cilm, cilm_d1,
+                       cilm_d2, sh.Lc, coefs);
+    st = launch_synthesis(*t.g, *t.rt, *t.sd, coefs, L1, L1, sh.Lc, sh.nrows, extend, theta, grid_d1, hs);
+    if (!st) st = launch_synthesis(*t.g, *t.rt, *t.sd, coefs + 2 * plane, L1, L1, sh.Lc, sh.nrows, extend, gb,
+                                   grid_d1, hs);
+    if (!st) st = launch_synthesis(*t.g, *t.rt, *t.sd, coefs + 4 * plane, L1, L1, sh.Lc, sh.nrows, extend, phi,
+                                   grid_d1, hs);
+    if (!st) {
+        const int ncols = sampling * n + extend;
+        hipLaunchKernelGGL(k_gradient_combine, dim3(grid_blocks((long long)sh.nrows * ncols)), dim3(kThreads), 0, hs,
+                           theta, phi, gb, sh.nrows, ncols, (long long)grid_d1, n, radius);
+        if (!HIP_OK(hipGetLastError())) st = ST_OTHER;
+    }
+    (void)hipFreeAsync(ws, hs);
+    return st;
+}
+
 int shtg_shglq(int lmax, double *zero, double *w)
 {
     if (lmax < 0) return ST_BOUNDS;

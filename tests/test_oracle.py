@@ -298,3 +298,66 @@ def test_complex_lmax_calc_and_padding():
     assert coef_err(c6, real_to_complex(A, 1)[:, :7, :7]) <= 1e-13
     with pytest.raises(oracle.OracleError):
         oracle.MakeGridDHC(c, lmax_calc=L + 1)
+
+
+# ---------------------------------------------------------------- gradient
+
+def _grad_brute(c, n, nlong, extend, radius, h=1e-5):
+    """Brute-force horizontal gradient at every DH node: d/dtheta by central
+    differences of the oracle's pinned P_lm(cos theta) (step h), d/dphi
+    analytically; poles zero as in the reference."""
+    L = c.shape[1] - 1
+    rows = n + extend
+    th = np.zeros((rows, nlong + extend))
+    ph = np.zeros((rows, nlong + extend))
+    phis = 2 * np.pi * np.arange(nlong + extend) / nlong
+    m = np.arange(L + 1)
+    cosm = np.cos(np.outer(m, phis))
+    sinm = np.sin(np.outer(m, phis))
+    for i in range(rows):
+        t = np.pi * i / n
+        if i == 0 or i == n:
+            continue
+        p0 = oracle.plm(L, np.cos(t))
+        dp = (oracle.plm(L, np.cos(t + h)) - oracle.plm(L, np.cos(t - h))) / (2 * h)
+        f_th = (c[0] * dp) @ cosm + (c[1] * dp) @ sinm
+        f_ph = (c[0] * p0 * m[None, :]) @ (-sinm) + (c[1] * p0 * m[None, :]) @ cosm
+        th[i] = f_th.sum(axis=0) / radius
+        ph[i] = f_ph.sum(axis=0) / radius / np.sin(t)
+    return th, ph
+
+
+@pytest.mark.parametrize("L,sampling,extend,radius", [(1, 2, 0, 1.0), (4, 1, 1, 2.5), (7, 2, 1, 1.0), (10, 2, 0, 6371.0)])
+def test_makegradientdh_vs_finite_differences(L, sampling, extend, radius):
+    rng = np.random.default_rng(900 + L)
+    c = rand_cilm(rng, L)
+    th, ph = oracle.MakeGradientDH(c, sampling=sampling, extend=extend, radius=radius)
+    n = 2 * L + 2
+    tb, pb = _grad_brute(c, n, sampling * n, extend, radius)
+    assert th.shape == tb.shape
+    assert grid_err(th, tb) <= 1e-8       # central-difference truncation
+    assert grid_err(ph, pb) <= 1e-12
+    assert np.all(th[0] == 0.0) and np.all(ph[0] == 0.0)
+
+
+def test_makegradientdh_closed_forms():
+    """Y_10 = sqrt(3) cos(theta): grad_theta = -sqrt(3) sin(theta), grad_phi = 0;
+    Y_11 = sqrt(3) sin(theta) cos(phi): grad_theta = sqrt(3) cos(theta) cos(phi),
+    grad_phi = -sqrt(3) sin(phi)."""
+    L, n = 3, 8
+    c = np.zeros((2, L + 1, L + 1))
+    c[0, 1, 0] = 1.0
+    th, ph = oracle.MakeGradientDH(c, sampling=2)
+    t = np.pi * np.arange(n) / n
+    p = 2 * np.pi * np.arange(2 * n) / (2 * n)
+    ref = -np.sqrt(3) * np.sin(t)[:, None] * np.ones_like(p)[None]
+    ref[0] = 0.0
+    assert np.abs(th - ref).max() <= 1e-14 and np.abs(ph).max() <= 1e-14
+    c[:] = 0.0
+    c[0, 1, 1] = 1.0
+    th, ph = oracle.MakeGradientDH(c, sampling=2)
+    rt = np.sqrt(3) * np.cos(t)[:, None] * np.cos(p)[None]
+    rp = -np.sqrt(3) * np.ones_like(t)[:, None] * np.sin(p)[None]
+    rt[0] = 0.0
+    rp[0] = 0.0
+    assert np.abs(th - rt).max() <= 1e-14 and np.abs(ph - rp).max() <= 1e-14

@@ -2,23 +2,35 @@
 //
 // Every ring (pair) transform is a complex DFT of length n, forward
 // X_k = sum_j x_j e^{-2 pi i jk/n}; inverse transforms are taken as
-// conj(F(conj(.))).  When n factors into radices {8, 4, 2, 3, 5, 7} the
-// kernel runs in-place decimation-in-frequency stages on the length-n array
-// and reads results through a digit-reversal table.  Otherwise the DFT is
-// computed by Bluestein's chirp-z method on a 5-smooth work length m >= 2n-1:
+// conj(F(conj(.))).  When n has no prime factor above 7 the kernel runs
+// in-place decimation-in-frequency stages (radices up to 8, see
+// factor_stages) on the length-n array and reads results through a
+// digit-reversal table.  Otherwise the DFT is computed by Bluestein's chirp-z
+// method on a 5-smooth work length m >= 2n-1 (bluestein_length):
 // the chirp-modulated input is transformed by DIF, multiplied by the
 // precomputed spectrum of the conjugate chirp (stored in DIF output order,
 // scaled by 1/m) and transformed back by the transposed (DIT) stages, which
 // consume digit-reversed input and leave natural order.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
+#include <functional>
+#include <iterator>
+#include <map>
 #include <vector>
 
 namespace shtg {
 
 constexpr int kMaxStages = 24;
+
+// The radices the device stages unroll in registers; radix 7 is the one
+// rolled radix.  Larger register radices cut LDS passes but raise the
+// kernels' register count above 128, which halves the waves per SIMD of
+// these latency-bound kernels and measured slower.  X(R) is applied to each.
+#define SHTG_FFT_RADICES(X) X(2) X(3) X(4) X(5) X(8)
+constexpr int kRegRadices[] = {2, 3, 4, 5, 8};
 
 struct FftPlanHost {
     int n = 0, m = 0, nst = 0, blue = 0;
@@ -34,28 +46,86 @@ struct FftPlanHost {
     std::vector<double> bhat2;  // pruned plans: analysis spectrum (bhat is synthesis)
 };
 
-inline bool factor_smooth(int m, std::vector<int> &rad)
+// Fewest-stage factorization of m over the register radices (and radix 7
+// when allow7).  Even radices run first, at the large strides; odd radices
+// run last, so that the stride-1 stage has an odd radix whenever m has an odd
+// factor (consecutive lanes then touch the LDS at odd multiples of 16 bytes,
+// which spread over the banks).  Among fewest-stage factorizations one with
+// an odd last radix is preferred.  Returns false when m has another prime.
+inline bool factor_stages(int m, bool allow7, std::vector<int> &rad)
 {
     rad.clear();
-    int r = m;
-    while (r % 8 == 0) { rad.push_back(8); r /= 8; }
-    if (r % 4 == 0) { rad.push_back(4); r /= 4; }
-    if (r % 2 == 0) { rad.push_back(2); r /= 2; }
-    static const int primes[] = {3, 5, 7};
-    for (int p : primes)
-        while (r % p == 0) { rad.push_back(p); r /= p; }
-    return r == 1;
+    if (m == 1) return true;
+    std::vector<int> radices(std::begin(kRegRadices), std::end(kRegRadices));
+    if (allow7) radices.push_back(7);
+    // best[d]: fewest stages for the cofactor d of m (memo over divisors)
+    std::map<int, int> best;
+    std::function<int(int)> stages = [&](int r) -> int {
+        if (r == 1) return 0;
+        auto it = best.find(r);
+        if (it != best.end()) return it->second;
+        int b = 1 << 20;
+        for (int R : radices)
+            if (r % R == 0) b = std::min(b, 1 + stages(r / R));
+        best[r] = b;
+        return b;
+    };
+    const int k = stages(m);
+    if (k >= (1 << 20)) return false;
+    // walk one optimal factorization, taking an odd radix last if one fits
+    std::vector<int> chosen;
+    std::function<bool(int, int, bool)> walk = [&](int r, int left, bool want_odd_last) -> bool {
+        if (r == 1) return left == 0;
+        for (int R : radices) {
+            if (r % R != 0 || stages(r / R) != left - 1) continue;
+            if (want_odd_last && left == 1 && R % 2 == 0) continue;
+            chosen.push_back(R);
+            if (walk(r / R, left - 1, want_odd_last)) return true;
+            chosen.pop_back();
+        }
+        return false;
+    };
+    if (!walk(m, k, m % 2 == 0 && (m & (m - 1)) != 0)) {
+        chosen.clear();
+        walk(m, k, false);
+    }
+    std::vector<int> even, odd;
+    for (int R : chosen) (R % 2 == 0 ? even : odd).push_back(R);
+    std::sort(even.rbegin(), even.rend());
+    std::sort(odd.rbegin(), odd.rend());
+    rad = even;
+    rad.insert(rad.end(), odd.begin(), odd.end());
+    return (int)rad.size() <= kMaxStages;
 }
 
-inline int next_5smooth(int v)
+inline bool is_5smooth(int m)
 {
-    for (int m = v;; ++m) {
-        int r = m;
-        while (r % 2 == 0) r /= 2;
-        while (r % 3 == 0) r /= 3;
-        while (r % 5 == 0) r /= 5;
-        if (r == 1) return m;
+    for (int p : {2, 3, 5})
+        while (m % p == 0) m /= p;
+    return m == 1;
+}
+
+// Bluestein work length: the 5-smooth m >= lo (up to 1.35 lo) with the
+// fewest LDS passes, m (2 stages - 1) (the last DIF and first DIT stage run
+// together in registers), plus one pass when the last radix is even (its
+// stride-1 accesses collide on the LDS banks).  Powers of two are skipped,
+// so that the last radix is odd (3 or 5), the two radices the device runs
+// the Bluestein turn for.
+inline int bluestein_length(int lo)
+{
+    int best_m = 0;
+    double best_cost = 0.0;
+    std::vector<int> rad;
+    for (int m = lo; m <= lo + lo / 3 + 8; ++m) {
+        if (!is_5smooth(m) || !factor_stages(m, false, rad)) continue;
+        if ((m & (m - 1)) == 0 || rad.back() % 2 == 0) continue;
+        const double cost = (double)m * (2.0 * rad.size() - 1.0);
+        if (best_m == 0 || cost < best_cost) {
+            best_m = m;
+            best_cost = cost;
+        }
     }
+    return best_m;
 }
 
 // e^{-2 pi i k / m} evaluated in long double.
@@ -68,20 +138,23 @@ inline std::complex<long double> root_ld(long long k, long long m)
 }
 
 // Reference host DIF (same stage structure as the device) in long double;
-// output left in DIF order.  Used to build the Bluestein spectrum.
+// output left in DIF order.  Used to build the Bluestein spectrum.  The
+// roots of unity come from one table of the m-th roots.
 inline void host_dif(std::vector<std::complex<long double>> &x, const FftPlanHost &p)
 {
-    int m = p.m;
+    const int m = p.m;
+    std::vector<std::complex<long double>> root(m);
+    for (int k = 0; k < m; ++k) root[k] = root_ld(k, m);
     for (int st = 0; st < p.nst; ++st) {
-        int R = p.radix[st], S = p.stride[st], Lsub = R * S, step = m / Lsub;
+        const int R = p.radix[st], S = p.stride[st], Lsub = R * S, step = m / Lsub, mr = m / R;
         std::vector<std::complex<long double>> v(R), y(R);
         for (int b = 0; b < m / R; ++b) {
-            int grp = b / S, i = b - grp * S, base = grp * Lsub + i;
+            const int grp = b / S, i = b - grp * S, base = grp * Lsub + i;
             for (int r = 0; r < R; ++r) v[r] = x[base + r * S];
             for (int k = 0; k < R; ++k) {
                 std::complex<long double> acc = 0;
-                for (int j = 0; j < R; ++j) acc += v[j] * root_ld((long long)j * k, R);
-                y[k] = acc * root_ld((long long)i * k * step, m);
+                for (int j = 0; j < R; ++j) acc += v[j] * root[(size_t)((j * k) % R) * mr];
+                y[k] = acc * root[(size_t)i * k * step % m];
             }
             for (int r = 0; r < R; ++r) x[base + r * S] = y[r];
         }
@@ -166,7 +239,7 @@ inline FftPlanHost make_fft_plan(int n, int lc = -1)
         set_stages(p, rad);
         return p;
     }
-    if (factor_smooth(n, rad) && (int)rad.size() <= kMaxStages) {
+    if (factor_stages(n, true, rad)) {
         p.m = n;
         set_stages(p, rad);
         return p;
@@ -174,8 +247,8 @@ inline FftPlanHost make_fft_plan(int n, int lc = -1)
     p.blue = 1;
     const bool prune = lc >= 0 && (n % 2 == 0) && n + 2 * lc < 2 * n - 1;
     p.lc = prune ? lc : -1;
-    p.m = next_5smooth(prune ? n + 2 * lc : 2 * n - 1);
-    factor_smooth(p.m, rad);
+    p.m = bluestein_length(prune ? n + 2 * lc : 2 * n - 1);
+    factor_stages(p.m, false, rad);
     set_stages(p, rad);
     const long double pi = 3.14159265358979323846264338327950288L;
     p.chirp.assign(2 * (size_t)n, 0.0);

@@ -35,7 +35,11 @@
 namespace shtg {
 
 constexpr int kThreads = 256;  // generic workgroups
-constexpr int kFftMaxThreads = 1024;  // ring-FFT workgroups: 256, 512 or 1024 threads
+// Ring-FFT workgroups have 256, 512 or 1024 threads.  The 1024-thread bound
+// caps the kernels at 128 VGPRs, i.e. up to 4 waves per SIMD: the ring FFTs
+// are latency-bound, so the stage radices stay small enough (up to 8) to
+// keep that occupancy.
+constexpr int kFftMaxThreads = 1024;
 constexpr int kWave = 64;      // Legendre workgroups are single waves
 constexpr int kSynthKMax = 4;  // ring pairs per lane in synthesis (1 .. 4)
 constexpr int kAnalKMax = 8;   // ring pairs per lane and pass in analysis (1 .. 8)

@@ -149,10 +149,29 @@ __device__ __forceinline__ void dft_unrolled(double2 (&v)[R])
     }
 }
 
+// v[r] *= w^r for r = 1..R-1 (R <= 8), w = e^{-2 pi i i/(R S)} of the
+// butterfly: the powers come from a product tree of depth 3.
+template <int R>
+__device__ __forceinline__ void apply_twiddles(double2 (&v)[R], double2 w1)
+{
+    double2 w[R];
+    w[0] = make_double2(1.0, 0.0);
+    if constexpr (R > 1) w[1] = w1;
+    if constexpr (R > 2) w[2] = cmul(w[1], w[1]);
+    if constexpr (R > 3) w[3] = cmul(w[2], w[1]);
+    if constexpr (R > 4) w[4] = cmul(w[2], w[2]);
+    if constexpr (R > 5) w[5] = cmul(w[4], w[1]);
+    if constexpr (R > 6) w[6] = cmul(w[4], w[2]);
+    if constexpr (R > 7) w[7] = cmul(w[4], w[3]);
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r]);
+}
+
 // One in-place stage of radix R over x[0..m): butterflies over elements
 // base + r*S; DIF applies twiddles after the butterfly, DIT (the transposed
-// stage) before it.  Radices 2, 3, 4, 5 and 8 are unrolled; other primes use
-// a rolled loop over outputs that stores each output as it is formed.
+// stage) before it.  Radices 2, 3, 4, 5 and 8 are unrolled in registers;
+// radix 7 uses a rolled loop over outputs that stores each output as it is
+// formed.
 template <int R, bool DIT>
 __device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double2 *__restrict__ tw,
                                           const double2 *__restrict__ tws)
@@ -177,25 +196,11 @@ __device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double
 #pragma unroll
         for (int r = 0; r < R; ++r) v[r] = x[fpad(base + r * S)];
         if constexpr (kUnrolled) {
-            // twiddles w^r, w = e^{-2 pi i i/Lsub}: a product tree of depth 3
-            double2 w[R];
-            w[0] = make_double2(1.0, 0.0);
-            if constexpr (R > 1) w[1] = tws[i];
-            if constexpr (R > 2) w[2] = cmul(w[1], w[1]);
-            if constexpr (R > 3) w[3] = cmul(w[2], w[1]);
-            if constexpr (R > 4) w[4] = cmul(w[2], w[2]);
-            if constexpr (R > 5) w[5] = cmul(w[4], w[1]);
-            if constexpr (R > 6) w[6] = cmul(w[4], w[2]);
-            if constexpr (R > 7) w[7] = cmul(w[4], w[3]);
-            if constexpr (DIT) {
-#pragma unroll
-                for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r]);
-            }
+            // twiddles w^r, w = e^{-2 pi i i/Lsub}
+            const double2 w1 = tws[i];
+            if constexpr (DIT) apply_twiddles<R>(v, w1);
             dft_unrolled<R>(v);
-            if constexpr (!DIT) {
-#pragma unroll
-                for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r]);
-            }
+            if constexpr (!DIT) apply_twiddles<R>(v, w1);
 #pragma unroll
             for (int r = 0; r < R; ++r) x[fpad(base + r * S)] = v[r];
         } else {
@@ -236,11 +241,10 @@ __device__ __forceinline__ void fft_run_stage(double2 *x, const FftDev &P, int s
     const int S = P.stride[st];
     const double2 *tws = P.twst + P.twoff[st];
     switch (P.radix[st]) {
-    case 2: fft_stage<2, DIT>(x, P.m, S, P.tw, tws); break;
-    case 3: fft_stage<3, DIT>(x, P.m, S, P.tw, tws); break;
-    case 4: fft_stage<4, DIT>(x, P.m, S, P.tw, tws); break;
-    case 5: fft_stage<5, DIT>(x, P.m, S, P.tw, tws); break;
-    case 8: fft_stage<8, DIT>(x, P.m, S, P.tw, tws); break;
+#define SHTG_STAGE_CASE(R) \
+    case R: fft_stage<R, DIT>(x, P.m, S, P.tw, tws); break;
+        SHTG_FFT_RADICES(SHTG_STAGE_CASE)
+#undef SHTG_STAGE_CASE
     case 7: fft_stage<7, DIT>(x, P.m, S, P.tw, tws); break;
     default: break;
     }
@@ -253,22 +257,60 @@ __device__ __forceinline__ double2 fft_prep_in(const FftDev &P, int j, double2 v
     return P.blue ? cmul(v, P.chirp[j]) : v;
 }
 
+// The turn of a Bluestein transform: the last DIF stage, the product with
+// the chirp spectrum (conjugated, so that the DIT stages compute the inverse)
+// and the first DIT stage.  The last stage has stride 1, so its twiddles are
+// all 1 and DIF and DIT touch the same R contiguous elements of one
+// butterfly: the three steps run in registers, with one LDS read and one
+// write per element.
+template <int R>
+__device__ __forceinline__ void fft_turn(double2 *x, int m, const double2 *__restrict__ bhat)
+{
+    for (int b = threadIdx.x; b < m / R; b += blockDim.x) {
+        const int base = b * R;
+        double2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = x[fpad(base + r)];
+        dft_unrolled<R>(v);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = conjd(cmul(v[r], bhat[base + r]));
+        dft_unrolled<R>(v);
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[fpad(base + r)] = v[r];
+    }
+}
+
+// Bluestein work lengths are 5-smooth and not powers of two
+// (bluestein_length), and factor_stages orders odd radices last, so the
+// last radix of a Bluestein plan is 3 or 5.
+__device__ __forceinline__ void fft_run_turn(double2 *x, const FftDev &P, const double2 *__restrict__ bhat)
+{
+    if (P.radix[P.nst - 1] == 3)
+        fft_turn<3>(x, P.m, bhat);
+    else
+        fft_turn<5>(x, P.m, bhat);
+}
+
 // In-place forward DFT of the values stored in x (x has P.m slots; for
 // Bluestein the caller has zeroed the unused slots).  bhat selects the chirp
 // spectrum (P.bhat for synthesis, P.bhat2 for analysis).  Ends with a barrier.
 __device__ __forceinline__ void fft_forward(double2 *x, const FftDev &P, const double2 *__restrict__ bhat)
 {
-    for (int st = 0; st < P.nst; ++st) {
+    const int last = P.nst - 1;
+    for (int st = 0; st < last; ++st) {
         fft_run_stage<false>(x, P, st);
         __syncthreads();
     }
     if (P.blue) {
-        for (int q = threadIdx.x; q < P.m; q += blockDim.x) x[fpad(q)] = conjd(cmul(x[fpad(q)], bhat[q]));
+        fft_run_turn(x, P, bhat);
         __syncthreads();
-        for (int st = P.nst - 1; st >= 0; --st) {
+        for (int st = last - 1; st >= 0; --st) {
             fft_run_stage<true>(x, P, st);
             __syncthreads();
         }
+    } else if (last >= 0) {
+        fft_run_stage<false>(x, P, last);
+        __syncthreads();
     }
 }
 

@@ -915,7 +915,7 @@ static int batch_tune(BatchPlan *b, hipStream_t st, int reps)
         for (int dir = 0; dir < 2; ++dir) {
             float best = 1e30f;
             int best_t = dir ? g.threads_anal : g.threads;
-            for (int t : {256, 512, 1024}) {
+            for (int t : {kFftMaxThreads / 4, kFftMaxThreads / 2, kFftMaxThreads}) {
                 float tmin = 1e30f;
                 for (int r = 0; r < reps; ++r) {
                     (void)hipEventRecord(e0, st);

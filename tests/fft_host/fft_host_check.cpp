@@ -1,0 +1,130 @@
+// CPU run of the library's device FFT code (sht_device.hpp compiled for the
+// host with hip/hip_runtime.h from this folder) against direct DFT sums.
+//
+// For every ring length given on the command line it builds the plan the
+// library builds (make_fft_plan, pruned for synthesis and analysis when
+// even), fills the work array the way the kernels do (fft_prep_in at
+// fft_pos), runs fft_forward and reads with fft_get.  Each output checked is
+// compared with the direct sum X_k = sum_j x_j e^{-2 pi i jk/n} evaluated in
+// long double; the line printed per length is
+//   n m blue nstages radices... max_rel_err
+// with the error relative to max |X|.
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <vector>
+
+#include "../../shtools_amd/csrc/sht_device.hpp"
+
+using namespace shtg;
+
+static FftDev to_dev(const FftPlanHost &h)
+{
+    FftDev d{};
+    d.n = h.n;
+    d.m = h.m;
+    d.nst = h.nst;
+    d.blue = h.blue;
+    d.lc = h.lc;
+    for (int s = 0; s < kMaxStages; ++s) {
+        d.radix[s] = h.radix[s];
+        d.stride[s] = h.stride[s];
+        d.twoff[s] = h.twoff[s];
+    }
+    d.tw = reinterpret_cast<const double2 *>(h.tw.data());
+    d.twst = reinterpret_cast<const double2 *>(h.twst.data());
+    d.perm = h.perm.data();
+    d.chirp = h.blue ? reinterpret_cast<const double2 *>(h.chirp.data()) : nullptr;
+    d.bhat = h.blue ? reinterpret_cast<const double2 *>(h.bhat.data()) : nullptr;
+    d.bhat2 = h.blue ? reinterpret_cast<const double2 *>((h.bhat2.empty() ? h.bhat : h.bhat2).data()) : nullptr;
+    return d;
+}
+
+// Direct DFT of the sparse input (index, value) at output k; root holds
+// the n-th roots of unity.
+static std::complex<long double> direct(const std::vector<std::pair<int, double2>> &in, int n, int k,
+                                        const std::vector<std::complex<long double>> &root)
+{
+    std::complex<long double> acc = 0;
+    for (const auto &e : in) {
+        const long long jk = ((long long)e.first * k) % n;
+        acc += std::complex<long double>(e.second.x, e.second.y) * root[jk];
+    }
+    return acc;
+}
+
+// mode 0: full input and output; 1: synthesis of a pruned plan (orders
+// |k| <= lc in, all n out); 2: analysis of a pruned plan (all n in, orders
+// |k| <= lc out).
+static double run_case(const FftPlanHost &h, int mode, std::mt19937_64 &rng, int nsample)
+{
+    const FftDev F = to_dev(h);
+    const int n = h.n;
+    std::normal_distribution<double> g;
+    std::vector<double2> x((size_t)h.m + 1, make_double2(0.0, 0.0));
+    std::vector<std::pair<int, double2>> in;
+    if (mode == 1) {
+        for (int k = -h.lc; k <= h.lc; ++k) {
+            const int j = (k + n) % n;
+            const double2 v = make_double2(g(rng), g(rng));
+            in.push_back({j, v});
+            x[fft_pos(F, j)] = fft_prep_in(F, j, v);
+        }
+    } else {
+        for (int j = 0; j < n; ++j) {
+            const double2 v = make_double2(g(rng), g(rng));
+            in.push_back({j, v});
+            x[j] = fft_prep_in(F, j, v);
+        }
+    }
+    fft_forward(x.data(), F, mode == 2 ? F.bhat2 : F.bhat);
+    std::vector<int> ks;
+    if (mode == 2) {
+        for (int k = -h.lc; k <= h.lc; ++k) ks.push_back((k + n) % n);
+    } else {
+        for (int k = 0; k < n; ++k) ks.push_back(k);
+    }
+    // sample the outputs (all of them for short rings)
+    if ((int)ks.size() > nsample) {
+        std::vector<int> pick;
+        std::uniform_int_distribution<int> u(0, (int)ks.size() - 1);
+        pick.push_back(ks.front());
+        pick.push_back(ks.back());
+        for (int t = 2; t < nsample; ++t) pick.push_back(ks[u(rng)]);
+        ks = pick;
+    }
+    std::vector<std::complex<long double>> root(n);
+    for (int k = 0; k < n; ++k) root[k] = root_ld(k, n);
+    double maxerr = 0.0, maxref = 0.0;
+    for (int k : ks) {
+        const int pos = (mode == 2) ? fft_pos(F, k) : k;
+        const double2 y = fft_get(x.data(), F, k, pos);
+        const auto r = direct(in, n, k, root);
+        maxerr = std::max(maxerr, (double)std::abs(std::complex<long double>(y.x, y.y) - r));
+        maxref = std::max(maxref, (double)std::abs(r));
+    }
+    return maxerr / maxref;
+}
+
+int main(int argc, char **argv)
+{
+    std::mt19937_64 rng(7);
+    const int nsample = 48;
+    for (int a = 1; a < argc; ++a) {
+        const int n = std::atoi(argv[a]);
+        const int lc = (n % 2 == 0) ? n / 4 - 1 : -1;  // the DH ring: lmax = n/4 - 1
+        FftPlanHost full = make_fft_plan(n);
+        double err = run_case(full, 0, rng, nsample);
+        if (n % 2 == 0) {
+            FftPlanHost pr = make_fft_plan(n, lc);
+            if (pr.lc >= 0) {
+                err = std::max(err, run_case(pr, 1, rng, nsample));
+                err = std::max(err, run_case(pr, 2, rng, nsample));
+            }
+        }
+        std::printf("%d %d %d %d", n, full.m, full.blue, full.nst);
+        for (int s = 0; s < full.nst; ++s) std::printf(" %d", full.radix[s]);
+        std::printf(" %.3e\n", err);
+    }
+    return 0;
+}

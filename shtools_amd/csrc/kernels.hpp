@@ -328,10 +328,12 @@ __device__ __forceinline__ void synth_fft_fill(const FftDev &F, const SynthRing 
         f.z *= r;
         f.w *= r;
         if (k == 0) {
-            lds[0] = fft_prep_in(F, 0, make_double2(f.x, -f.z));
+            lds[0] = make_double2(f.x, -f.z);  // chirp_0 = 1
         } else {
-            lds[fpad(k)] = fft_prep_in(F, k, make_double2(0.5 * (f.x + f.w), -0.5 * (f.z - f.y)));
-            lds[fpad(fft_pos(F, n - k))] = fft_prep_in(F, n - k, make_double2(0.5 * (f.x - f.w), -0.5 * (f.y + f.z)));
+            const double2 c = fft_chirp(F, k);
+            lds[fpad(k)] = fft_prep_in_c(F, make_double2(0.5 * (f.x + f.w), -0.5 * (f.z - f.y)), c);
+            lds[fpad(fft_pos(F, n - k))] =
+                fft_prep_in_c(F, make_double2(0.5 * (f.x - f.w), -0.5 * (f.y + f.z)), fft_chirp_mirror(F, c));
         }
     };
     {
@@ -368,14 +370,22 @@ __device__ __forceinline__ void synth_fft_run(const FftDev &F, int2 rw, int nrow
     fft_forward(lds, F, F.bhat);
     const bool north = rw.x >= 0 && rw.x < nrows;
     const bool south = rw.y >= 0 && rw.y < nrows;
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        const double2 v = fft_get(lds, F, j, j);
+    // columns j and n - j (1 <= j < n - j) together: one chirp read
+    for (int j = threadIdx.x; 2 * j <= n; j += blockDim.x) {
+        const double2 c = fft_chirp(F, j);
+        const double2 v = fft_get_c(lds, F, j, j, c);
         const double gn = v.x * rwt.x, gs = -v.y * rwt.y;
         if (north) grid[rw.x * ld + j] = gn;
         if (south) grid[rw.y * ld + j] = gs;
         if (ext_col && j == 0) {
             if (north) grid[rw.x * ld + n] = gn;
             if (south) grid[rw.y * ld + n] = gs;
+        }
+        const int j2 = n - j;
+        if (j > 0 && j2 > j) {
+            const double2 v2 = fft_get_c(lds, F, j2, j2, fft_chirp_mirror(F, c));
+            if (north) grid[rw.x * ld + j2] = v2.x * rwt.x;
+            if (south) grid[rw.y * ld + j2] = -v2.y * rwt.y;
         }
     }
 }
@@ -391,8 +401,9 @@ __device__ __forceinline__ void synth_fft_tile(const FftDev &F, const double4 *_
     synth_fft_run(F, rw, nrows, grid, ld, ext_col, rwt, lds);
 }
 
-// Analysis inputs of one thread: ring values at columns threadIdx.x +
-// j * blockDim.x, j < kAnalPre (later columns are read in the fill).
+// Analysis inputs of one thread: ring values at the column pairs q and
+// n - q, q = threadIdx.x + j * blockDim.x, j < kAnalPre / 2 (later columns
+// are read in the fill).
 constexpr int kAnalPre = 4;
 struct AnalRing {
     double2 v[kAnalPre];
@@ -420,9 +431,11 @@ __device__ __forceinline__ void anal_fft_load(const FftDev &F, const double *__r
     const bool north = rw.x >= 0 && rw.x < nrows;
     const bool south = rw.y >= 0 && rw.y < nrows;
 #pragma unroll
-    for (int j = 0; j < kAnalPre; ++j) {
-        const int k = threadIdx.x + j * blockDim.x;
-        in.v[j] = (k < n) ? anal_column(grid, ld, rw, north, south, n, ext_col, k) : make_double2(0.0, 0.0);
+    for (int j = 0; j < kAnalPre / 2; ++j) {
+        const int q = threadIdx.x + j * blockDim.x, q2 = n - q;
+        const bool lo = 2 * q <= n, hi = q > 0 && q2 > q;
+        in.v[2 * j] = lo ? anal_column(grid, ld, rw, north, south, n, ext_col, q) : make_double2(0.0, 0.0);
+        in.v[2 * j + 1] = hi ? anal_column(grid, ld, rw, north, south, n, ext_col, q2) : make_double2(0.0, 0.0);
     }
 }
 
@@ -430,16 +443,26 @@ __device__ __forceinline__ void anal_fft_fill(const FftDev &F, const AnalRing &i
                                               long long ld, int2 rw, int nrows, int ext_col, double2 *lds)
 {
     const int n = F.n, m = F.m, nt = blockDim.x;
+    // columns q and n - q (1 <= q < n - q) together: one chirp read
+    auto put = [&](int q, double2 a, double2 b) {
+        const double2 c = fft_chirp(F, q);
+        lds[fpad(q)] = fft_prep_in_c(F, a, c);
+        const int q2 = n - q;
+        if (q > 0 && q2 > q) lds[fpad(q2)] = fft_prep_in_c(F, b, fft_chirp_mirror(F, c));
+    };
 #pragma unroll
-    for (int j = 0; j < kAnalPre; ++j) {
+    for (int j = 0; j < kAnalPre / 2; ++j) {
         const int q = threadIdx.x + j * nt;
-        if (q < m) lds[fpad(q)] = (q < n) ? fft_prep_in(F, q, in.v[j]) : make_double2(0.0, 0.0);
+        if (2 * q <= n) put(q, in.v[2 * j], in.v[2 * j + 1]);
     }
     const bool north = rw.x >= 0 && rw.x < nrows;
     const bool south = rw.y >= 0 && rw.y < nrows;
-    for (int q = threadIdx.x + kAnalPre * nt; q < m; q += nt)
-        lds[fpad(q)] = (q < n) ? fft_prep_in(F, q, anal_column(grid, ld, rw, north, south, n, ext_col, q))
-                               : make_double2(0.0, 0.0);
+    for (int q = threadIdx.x + (kAnalPre / 2) * nt; 2 * q <= n; q += nt) {
+        const int q2 = n - q;
+        put(q, anal_column(grid, ld, rw, north, south, n, ext_col, q),
+            q2 > q ? anal_column(grid, ld, rw, north, south, n, ext_col, q2) : make_double2(0.0, 0.0));
+    }
+    for (int q = n + threadIdx.x; q < m; q += nt) lds[fpad(q)] = make_double2(0.0, 0.0);
 }
 
 // DFT of the filled array; w = (w_north, w_south) include the quadrature
@@ -462,9 +485,10 @@ __device__ __forceinline__ void anal_fft_run(const FftDev &F, int2 rw, int nrows
             if (mm == 0) r = 1.0;
             else if (j == 0 || mm == 1) r = scaled_upow(u, mm);
             else r *= u;
-            const double2 zm = fft_get(lds, F, mm, mm);
+            const double2 c = fft_chirp(F, mm);
+            const double2 zm = fft_get_c(lds, F, mm, mm, c);
             const int nm = mm == 0 ? 0 : n - mm;
-            const double2 zn = fft_get(lds, F, nm, fft_pos(F, nm));
+            const double2 zn = mm == 0 ? zm : fft_get_c(lds, F, nm, fft_pos(F, nm), fft_chirp_mirror(F, c));
             const double xr = 0.5 * (zm.x + zn.x), xi = 0.5 * (zm.y - zn.y);
             const double yr = 0.5 * (zm.y + zn.y), yi = 0.5 * (zn.x - zm.x);
             const double f1c = wn * xr, f1s = -wn * xi, f2c = ws * yr, f2s = -ws * yi;

@@ -10,9 +10,12 @@ namespace shtg {
 
 // --------------------------------------------------------------- helpers
 
+// Complex product with the fused multiply-adds written out, so that its
+// rounding does not depend on which product the compiler would contract in
+// a given kernel (the plain and the batched kernels then agree bit for bit).
 __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
@@ -257,6 +260,23 @@ __device__ __forceinline__ double2 fft_prep_in(const FftDev &P, int j, double2 v
     return P.blue ? cmul(v, P.chirp[j]) : v;
 }
 
+// The chirp e^{-i pi j^2/n} of natural index j (Bluestein plans).  Indices j
+// and n - j share it up to a sign, chirp_{n-j} = (-1)^n chirp_j, so callers
+// that handle both read the table once.
+__device__ __forceinline__ double2 fft_chirp(const FftDev &P, int j)
+{
+    return P.blue ? P.chirp[j] : make_double2(1.0, 0.0);
+}
+__device__ __forceinline__ double2 fft_chirp_mirror(const FftDev &P, double2 c)
+{
+    return (P.n & 1) ? make_double2(-c.x, -c.y) : c;
+}
+// fft_prep_in with the chirp of j given.
+__device__ __forceinline__ double2 fft_prep_in_c(const FftDev &P, double2 v, double2 c)
+{
+    return P.blue ? cmul(v, c) : v;
+}
+
 // The turn of a Bluestein transform: the last DIF stage, the product with
 // the chirp spectrum (conjugated, so that the DIT stages compute the inverse)
 // and the first DIT stage.  The last stage has stride 1, so its twiddles are
@@ -319,6 +339,13 @@ __device__ __forceinline__ void fft_forward(double2 *x, const FftDev &P, const d
 __device__ __forceinline__ double2 fft_get(const double2 *x, const FftDev &P, int k, int pos)
 {
     if (P.blue) return cmul(conjd(x[fpad(pos)]), P.chirp[k]);
+    return x[fpad(P.perm[k])];
+}
+
+// fft_get with the chirp c of k given (fft_chirp / fft_chirp_mirror).
+__device__ __forceinline__ double2 fft_get_c(const double2 *x, const FftDev &P, int k, int pos, double2 c)
+{
+    if (P.blue) return cmul(conjd(x[fpad(pos)]), c);
     return x[fpad(P.perm[k])];
 }
 

@@ -8,7 +8,8 @@
 // compared with the direct sum X_k = sum_j x_j e^{-2 pi i jk/n} evaluated in
 // long double; the line printed per length is
 //   n m blue nstages radices... max_rel_err
-// with the error relative to max |X|.
+// with the error relative to max |X| (1 when the chirp table breaks the
+// mirror identity chirp_{n-j} = (-1)^n chirp_j that the kernels rely on).
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -92,6 +93,13 @@ static double run_case(const FftPlanHost &h, int mode, std::mt19937_64 &rng, int
         pick.push_back(ks.back());
         for (int t = 2; t < nsample; ++t) pick.push_back(ks[u(rng)]);
         ks = pick;
+    }
+    // the mirror identity the kernels use: chirp_{n-j} = (-1)^n chirp_j
+    if (h.blue) {
+        for (int j = 1; j < n; ++j) {
+            const double2 a = fft_chirp(F, n - j), b = fft_chirp_mirror(F, fft_chirp(F, j));
+            if (std::fabs(a.x - b.x) + std::fabs(a.y - b.y) > 4e-16) return 1.0;
+        }
     }
     std::vector<std::complex<long double>> root(n);
     for (int k = 0; k < n; ++k) root[k] = root_ld(k, n);

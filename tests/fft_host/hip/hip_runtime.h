@@ -23,3 +23,4 @@ inline void __syncthreads() {}
 
 using std::frexp;
 using std::ldexp;
+using std::fma;

@@ -93,6 +93,23 @@ int shtg_makegradient_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax,
                          int lmax_calc, int extend, double radius, double *theta, double *phi,
                          int grid_d0, int grid_d1, void *stream);
 
+/* MakeGravGridDH: src/MakeGravGridDH.f95:1-1069 (f2py src/pyshtools.pyf:
+ * 1440-1477; Python wrapper src/PythonWrapper.f95:1690-1735, which always
+ * passes omega and pot).  Gravity vector (rad positive upward, theta, phi),
+ * its magnitude and the potential, in SI units, on a flattened ellipsoid
+ * (semi-major axis a, flattening f) from 4pi-normalised potential
+ * coefficients without the Condon-Shortley phase referenced to r0:
+ * V = GM/r sum (r0/r)^l C_lm Y_lm.  omega adds the centrifugal terms;
+ * normal_gravity = 1 removes the normal gravity (NormalGravity.f95) computed
+ * with normal_gravity_gm from the magnitude.  All five grids are
+ * [grid_d0][grid_d1] with 2(lmax+1)+extend rows and
+ * sampling*2(lmax+1)+extend columns. */
+int shtg_makegravgrid_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax, double gm, double r0,
+                         double a, double f, int sampling, int lmax_calc, double omega,
+                         int normal_gravity, double normal_gravity_gm, int extend, double *rad,
+                         double *theta, double *phi, double *total, double *pot, int grid_d0,
+                         int grid_d1, void *stream);
+
 /* Adjoints (transposes) of the four transforms, for gradients through them
  * (no counterpart in the reference; the differentiable path of the
  * transform API).  Each takes the argument list of its forward call with

@@ -66,6 +66,10 @@ def lib():
         L.or_makegridglqc.argtypes = [dp, i, i, i, dp, i, i, i, i, i, dp, i, i]
         L.or_shexpandglqc.argtypes = [dp, i, i, i, dp, i, dp, i, i, i, i, dp, i, i]
         L.or_makegradientdh.argtypes = [dp, i, i, i, i, i, i, ctypes.c_double, dp, dp, i, i]
+        d = ctypes.c_double
+        L.or_makegravgriddh.argtypes = [dp, i, i, i, d, d, d, d, i, i, d, i, d, i, dp, dp, dp, dp, dp, i, i]
+        L.or_normalgravity.argtypes = [d, d, d, d, d]
+        L.or_normalgravity.restype = d
         L.or_r2c.argtypes = [i, dp, dp]
         L.or_c2r.argtypes = [i, dp, dp]
         lp = ctypes.POINTER(ctypes.c_longlong)
@@ -230,6 +234,32 @@ def MakeGradientDH(cilm, lmax=None, sampling=2, lmax_calc=None, extend=0, radius
     _check(lib().or_makegradientdh(_p(cilm), cilm.shape[1], cilm.shape[2], lmax, sampling, lmax_calc, extend,
                                    float(radius), _p(theta), _p(phi), g0, g1))
     return theta, phi
+
+
+def MakeGravGridDH(cilm, gm, r0, lmax=None, a=None, f=0.0, sampling=2, lmax_calc=None, omega=0.0,
+                   normal_gravity=1, normal_gravity_gm=None, extend=0):
+    """(rad, theta, phi, total, pot) grids; argument order and defaults of
+    pyshtools' MakeGravGridDH (src/pyshtools.pyf:1440-1477)."""
+    cilm = np.ascontiguousarray(cilm, dtype=np.float64)
+    if lmax is None:
+        lmax = cilm.shape[1] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    if a is None:
+        a = r0
+    if normal_gravity_gm is None:
+        normal_gravity_gm = gm
+    g0 = 2 * (lmax + 1) + extend
+    g1 = sampling * 2 * (lmax + 1) + extend
+    out = [np.zeros((g0, g1)) for _ in range(5)]
+    _check(lib().or_makegravgriddh(_p(cilm), cilm.shape[1], cilm.shape[2], lmax, float(gm), float(r0), float(a),
+                                   float(f), sampling, lmax_calc, float(omega), normal_gravity,
+                                   float(normal_gravity_gm), extend, *[_p(x) for x in out], g0, g1))
+    return tuple(out)
+
+
+def NormalGravity(geocentric_lat, gm, omega, a, b):
+    return lib().or_normalgravity(float(geocentric_lat), float(gm), float(omega), float(a), float(b))
 
 
 def rfft(x):

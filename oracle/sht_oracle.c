@@ -1591,6 +1591,322 @@ int or_makegradientdh(const double *cilm, int d1, int d2, int lmax, int sampling
     return 0;
 }
 
+/* ------------------------------------------- gravity on an ellipsoid */
+
+/* Normal gravity of a rotating ellipsoid (Somigliana's closed form) at
+ * geocentric latitude lat (degrees).  src/NormalGravity.f95:1-139. */
+double or_normalgravity(double geocentric_lat, double gm, double omega, double a, double b)
+{
+    double pi = acos(-1.0);
+    if (a == b && omega == 0.0) return gm / (a * a);
+    if (a == b) {
+        double cl = cos(geocentric_lat * pi / 180.0);
+        double g2 = gm * gm / (a * a * a * a) + a * omega * omega * cl * cl * (a * omega * omega - 2.0 * gm / (a * a));
+        return sqrt(g2);
+    }
+    double m = omega * omega * a * a * b / gm;
+    double bigE = sqrt(a * a - b * b), ep = bigE / b;
+    double q0 = 0.5 * ((1.0 + 3.0 * (b / bigE) * (b / bigE)) * atan(bigE / b) - 3.0 * b / bigE);
+    double q0p = 3.0 * (1.0 + (b / bigE) * (b / bigE)) * (1.0 - b / bigE * atan(bigE / b)) - 1.0;
+    double ga = gm / (a * b) * (1.0 - m - m * ep * q0p / 6.0 / q0);
+    double gb = gm / (a * a) * (1.0 + m * ep * q0p / 3.0 / q0);
+    double gl = atan((a / b) * (a / b) * tan(geocentric_lat * pi / 180.0));
+    double c = cos(gl), sn = sin(gl);
+    double g = a * ga * c * c + b * gb * sn * sn;
+    return g / sqrt(a * a * c * c + b * b * sn * sn);
+}
+
+/* Gravity vector (radial, theta, phi), its magnitude and the potential on a
+ * flattened ellipsoid (semi-major axis a, flattening f) from 4pi-normalised
+ * potential coefficients without the Condon-Shortley phase:
+ * V = GM/r sum (r0/r)^l C_lm Y_lm, B = grad V, plus the centrifugal terms
+ * when omega != 0 and minus the normal gravity from the magnitude when
+ * normal_gravity = 1.  Restates src/MakeGravGridDH.f95:1-1069 ring by ring
+ * (equator :430-605, ring pairs :607-1000, centrifugal :1002-1024, magnitude
+ * and normal gravity :1026-1052, extend :1056-1066). */
+int or_makegravgriddh(const double *cilm, int d1, int d2, int lmax, double gm, double r0, double a, double f,
+                      int sampling, int lmax_calc, double omega, int normal_gravity, double normal_gravity_gm,
+                      int extend, double *rad_g, double *theta_g, double *phi_g, double *total_g, double *pot_g,
+                      int g0, int g1)
+{
+    int n = 2 * lmax + 2, nlong, nlat_out, nlong_out;
+    if (sampling == 1) nlong = n; else if (sampling == 2) nlong = 2 * n; else return 2;
+    if (extend == 0) { nlat_out = n; nlong_out = nlong; }
+    else if (extend == 1) { nlat_out = n + 1; nlong_out = nlong + 1; }
+    else return 2;
+    if (g0 < nlat_out || g1 < nlong_out) return 1;
+    if (normal_gravity != 0 && normal_gravity != 1) return 2;
+    if (lmax_calc > lmax) return 2;
+    int lc = lc_of_makegrid(d1, d2, lmax, lmax_calc);
+    double pi = acos(-1.0), scalef = 1.0e-280;
+#define RG(i, k) rad_g[(size_t)(i) * g1 + (k)]
+#define TG(i, k) theta_g[(size_t)(i) * g1 + (k)]
+#define PG(i, k) phi_g[(size_t)(i) * g1 + (k)]
+#define AG(i, k) total_g[(size_t)(i) * g1 + (k)]
+#define UG(i, k) pot_g[(size_t)(i) * g1 + (k)]
+    tables t;
+    int st = build_tables(&t, lc, 1);
+    double *cf = to_fortran_order(cilm, d1, d2, lc);
+    if (st || !cf) { free_tables(&t); free(cf); return 3; }
+    const double *sqr = t.sqr;
+    int ncl = nlong / 2 + 1, i_eq = n / 2 + 1;
+    fftplan *plan = plan_create(nlong);
+    cplx *coef = (cplx *)malloc(sizeof(cplx) * ncl);
+    cplx *buf = (cplx *)malloc(sizeof(cplx) * nlong);
+    double *row = (double *)malloc(sizeof(double) * nlong);
+    double *pf = (double *)malloc(sizeof(double) * (lmax + 2));
+    /* per ring: R, U, T, P sums of the north ring, then of the south ring */
+    cplx *cs[8];
+    for (int j = 0; j < 8; j++) cs[j] = (cplx *)malloc(sizeof(cplx) * (lmax + 2));
+    cplx *cr = cs[0], *cu = cs[1], *ct = cs[2], *cp = cs[3], *crs = cs[4], *cus = cs[5], *cts = cs[6], *cps = cs[7];
+    double c0[8];
+    /* c2r of (c0, c[1..lmax]/2), zeros above lmax, into row */
+#define SYN(c0v, carr)                                                                   \
+    do {                                                                                 \
+        for (int k = 0; k < ncl; k++) coef[k] = 0;                                       \
+        coef[0] = (c0v);                                                                 \
+        for (int k = 1; k <= lmax && k < ncl; k++) coef[k] = (carr)[k] / 2.0;            \
+        c2r(plan, coef, buf, row);                                                       \
+    } while (0)
+    double pm2, pm1, p, dpl, pmm, rescalem, tempr, r_ex;
+    cplx tempc, cmi, spc;
+    /* equator: MakeGravGridDH.f95:430-605 */
+    {
+        double theta = pi / 2.0, z = 0.0, u = 1.0;
+        r_ex = a;
+        for (int j = 0; j < 8; j++) { c0[j] = 0.0; for (int k = 0; k < lmax + 2; k++) cs[j][k] = 0; }
+        pm2 = 1.0;
+        c0[0] -= CF(0, 0, 0) * pm2;
+        c0[1] += CF(0, 0, 0) * pm2;
+        pm1 = 0.0;
+        if (lc != 0) {
+            pf[1] = r0 / r_ex;
+            for (int l = 2; l <= lc; l++) pf[l] = pf[l - 1] * r0 / r_ex;
+            c0[1] += CF(0, 1, 0) * pm1 * pf[1];
+            dpl = FF(&t, ff1, 1, 0);
+            c0[2] += CF(0, 1, 0) * dpl * pf[1];
+        }
+        for (int l = 2; l <= lc; l++) {
+            p = -FF(&t, ff2, l, 0) * pm2;
+            c0[0] += CF(0, l, 0) * p * (-(l + 1)) * pf[l];
+            c0[1] += CF(0, l, 0) * p * pf[l];
+            dpl = l * (sqr[2 * l + 1] / sqr[2 * l - 1] * pm1);
+            c0[2] += CF(0, l, 0) * dpl * pf[l];
+            pm2 = pm1;
+            pm1 = p;
+        }
+        pmm = sqr[2] * scalef;
+        rescalem = 1.0 / scalef;
+        for (int m = 1; m <= lc - 1; m++) {
+            pmm = pmm * sqr[2 * m + 1] / sqr[2 * m];
+            pm2 = pmm;
+            cmi = CF(0, m, m) - I * CF(1, m, m);
+            spc = CF(1, m, m) + I * CF(0, m, m);
+            cr[m] += cmi * pm2 * (-m - 1) * pf[m];
+            cu[m] += cmi * pm2 * pf[m];
+            cp[m] += spc * pm2 * pf[m] * m;
+            pm1 = 0.0;
+            dpl = sqr[2 * m + 3] * pmm;
+            ct[m] += (CF(0, m + 1, m) - I * CF(1, m + 1, m)) * dpl * pf[m + 1];
+            for (int l = m + 2; l <= lc; l++) {
+                p = -FF(&t, ff2, l, m) * pm2;
+                cmi = CF(0, l, m) - I * CF(1, l, m);
+                cr[m] += cmi * p * (-(l + 1)) * pf[l];
+                cu[m] += cmi * p * pf[l];
+                cp[m] += (CF(1, l, m) + I * CF(0, l, m)) * p * pf[l] * m;
+                dpl = sqr[2 * l + 1] * sqr[l - m] * sqr[l + m] / sqr[2 * l - 1] * pm1;
+                ct[m] += cmi * dpl * pf[l];
+                pm2 = pm1;
+                pm1 = p;
+            }
+            cr[m] *= rescalem; cu[m] *= rescalem; cp[m] *= rescalem; ct[m] *= rescalem;
+        }
+        if (lc != 0) {
+            pmm = pmm * sqr[2 * lc + 1] / sqr[2 * lc] * rescalem;
+            cmi = CF(0, lc, lc) - I * CF(1, lc, lc);
+            cr[lc] += cmi * pmm * (-lc - 1) * pf[lc];
+            cu[lc] += cmi * pmm * pf[lc];
+            cp[lc] += (CF(1, lc, lc) + I * CF(0, lc, lc)) * pmm * pf[lc] * lc;
+            dpl = -lc * z * pmm / (u * u);
+            ct[lc] += cmi * dpl * pf[lc];
+        }
+        int i = i_eq - 1;
+        SYN(c0[0], cr);
+        for (int k = 0; k < nlong; k++) RG(i, k) = row[k] * gm / (r_ex * r_ex);
+        SYN(c0[1], cu);
+        for (int k = 0; k < nlong; k++) UG(i, k) = row[k] * gm / r_ex;
+        SYN(c0[2], ct);
+        for (int k = 0; k < nlong; k++) TG(i, k) = -sin(theta) * row[k] * gm / (r_ex * r_ex);
+        SYN(c0[3], cp);
+        for (int k = 0; k < nlong; k++) PG(i, k) = row[k] * (gm / (r_ex * r_ex)) / sin(theta);
+    }
+    /* ring pairs: MakeGravGridDH.f95:607-1000 */
+    for (int i = 1; i <= i_eq - 1; i++) {
+        int i_s = 2 * i_eq - i;
+        double theta = pi * (double)(i - 1) / (double)n;
+        double z = cos(theta);
+        double u = sqrt((1.0 - z) * (1.0 + z));
+        double u2 = u * u;
+        double lat = pi / 2.0 - theta;
+        if (i == 1) {
+            r_ex = a * (1.0 - f);
+        } else {
+            r_ex = cos(lat) * cos(lat) + sin(lat) * sin(lat) / ((1.0 - f) * (1.0 - f));
+            r_ex = a * sqrt(1.0 / r_ex);
+        }
+        for (int j = 0; j < 8; j++) { c0[j] = 0.0; for (int k = 0; k < lmax + 2; k++) cs[j][k] = 0; }
+        pm2 = 1.0;
+        tempr = -CF(0, 0, 0) * pm2;
+        c0[0] += tempr; c0[4] += tempr;
+        tempr = CF(0, 0, 0) * pm2;
+        c0[1] += tempr; c0[5] += tempr;
+        pm1 = 0.0;
+        if (lc != 0) {
+            pf[1] = r0 / r_ex;
+            for (int l = 2; l <= lc; l++) pf[l] = pf[l - 1] * r0 / r_ex;
+            pm1 = FF(&t, ff1, 1, 0) * z;
+            tempr = CF(0, 1, 0) * pm1 * (-2) * pf[1];
+            c0[0] += tempr; c0[4] -= tempr;
+            tempr = CF(0, 1, 0) * pm1 * pf[1];
+            c0[1] += tempr; c0[5] -= tempr;
+            dpl = FF(&t, ff1, 1, 0);
+            tempr = CF(0, 1, 0) * dpl * pf[1];
+            c0[2] += tempr; c0[6] += tempr;
+        }
+        for (int l = 2; l <= lc; l++) {
+            p = FF(&t, ff1, l, 0) * z * pm1 - FF(&t, ff2, l, 0) * pm2;
+            tempr = CF(0, l, 0) * p * (-(l + 1)) * pf[l];
+            c0[0] += tempr; c0[4] += tempr * fsym(l, 0);
+            tempr = CF(0, l, 0) * p * pf[l];
+            c0[1] += tempr; c0[5] += tempr * fsym(l, 0);
+            dpl = l * (sqr[2 * l + 1] / sqr[2 * l - 1] * pm1 - z * p) / u2;
+            tempr = CF(0, l, 0) * dpl * pf[l];
+            c0[2] += tempr; c0[6] -= tempr * fsym(l, 0);
+            pm2 = pm1;
+            pm1 = p;
+        }
+        pmm = sqr[2] * scalef;
+        rescalem = 1.0 / scalef;
+        for (int m = 1; m <= lc - 1; m++) {
+            rescalem = rescalem * u;
+            pmm = pmm * sqr[2 * m + 1] / sqr[2 * m];
+            pm2 = pmm;
+            cmi = CF(0, m, m) - I * CF(1, m, m);
+            spc = CF(1, m, m) + I * CF(0, m, m);
+            tempc = cmi * pm2 * (-m - 1) * pf[m];
+            cr[m] += tempc; crs[m] += tempc;
+            tempc = cmi * pm2 * pf[m];
+            cu[m] += tempc; cus[m] += tempc;
+            tempc = spc * pm2 * pf[m] * m;
+            cp[m] += tempc; cps[m] += tempc;
+            dpl = -m * z * pm2 / u2;
+            tempc = cmi * dpl * pf[m];
+            ct[m] += tempc; cts[m] -= tempc;
+            pm1 = z * FF(&t, ff1, m + 1, m) * pm2;
+            cmi = CF(0, m + 1, m) - I * CF(1, m + 1, m);
+            spc = CF(1, m + 1, m) + I * CF(0, m + 1, m);
+            tempc = cmi * pm1 * (-m - 2) * pf[m + 1];
+            cr[m] += tempc; crs[m] -= tempc;
+            tempc = cmi * pm1 * pf[m + 1];
+            cu[m] += tempc; cus[m] -= tempc;
+            tempc = spc * pm1 * pf[m + 1] * m;
+            cp[m] += tempc; cps[m] -= tempc;
+            dpl = (sqr[2 * m + 3] * pmm - z * (m + 1) * pm1) / u2;
+            tempc = cmi * dpl * pf[m + 1];
+            ct[m] += tempc; cts[m] += tempc;
+            for (int l = m + 2; l <= lc; l++) {
+                p = z * FF(&t, ff1, l, m) * pm1 - FF(&t, ff2, l, m) * pm2;
+                cmi = CF(0, l, m) - I * CF(1, l, m);
+                tempc = cmi * p * (-(l + 1)) * pf[l];
+                cr[m] += tempc; crs[m] += tempc * fsym(l, m);
+                tempc = cmi * p * pf[l];
+                cu[m] += tempc; cus[m] += tempc * fsym(l, m);
+                tempc = (CF(1, l, m) + I * CF(0, l, m)) * p * pf[l] * m;
+                cp[m] += tempc; cps[m] += tempc * fsym(l, m);
+                dpl = (sqr[2 * l + 1] * sqr[l - m] * sqr[l + m] / sqr[2 * l - 1] * pm1 - z * l * p) / u2;
+                tempc = cmi * dpl * pf[l];
+                ct[m] += tempc; cts[m] -= tempc * fsym(l, m);
+                pm2 = pm1;
+                pm1 = p;
+            }
+            for (int j = 0; j < 8; j++) cs[j][m] *= rescalem;
+        }
+        if (lc != 0) {
+            rescalem = rescalem * u;
+            pmm = pmm * sqr[2 * lc + 1] / sqr[2 * lc] * rescalem;
+            cmi = CF(0, lc, lc) - I * CF(1, lc, lc);
+            tempc = cmi * pmm * (-lc - 1) * pf[lc];
+            cr[lc] += tempc; crs[lc] += tempc;
+            tempc = cmi * pmm * pf[lc];
+            cu[lc] += tempc; cus[lc] += tempc;
+            tempc = (CF(1, lc, lc) + I * CF(0, lc, lc)) * pmm * pf[lc] * lc;
+            cp[lc] += tempc; cps[lc] += tempc;
+            dpl = -lc * z * pmm / u2;
+            tempc = cmi * dpl * pf[lc];
+            ct[lc] += tempc; cts[lc] -= tempc;
+        }
+        double g2 = gm / (r_ex * r_ex);
+        for (int side = 0; side < 2; side++) {
+            if (side == 1 && i == 1 && extend == 0) break;
+            int row_i = side ? i_s - 1 : i - 1, o = side ? 4 : 0;
+            SYN(c0[o + 0], cs[o + 0]);
+            for (int k = 0; k < nlong; k++) RG(row_i, k) = row[k] * g2;
+            SYN(c0[o + 1], cs[o + 1]);
+            for (int k = 0; k < nlong; k++) UG(row_i, k) = row[k] * gm / r_ex;
+            if (i == 1) {
+                for (int k = 0; k < nlong; k++) { TG(row_i, k) = 0.0; PG(row_i, k) = 0.0; }
+            } else {
+                SYN(c0[o + 2], cs[o + 2]);
+                for (int k = 0; k < nlong; k++) TG(row_i, k) = -sin(theta) * row[k] * g2;
+                SYN(c0[o + 3], cs[o + 3]);
+                for (int k = 0; k < nlong; k++) PG(row_i, k) = row[k] * g2 / sin(theta);
+            }
+        }
+    }
+#undef SYN
+    /* centrifugal terms: MakeGravGridDH.f95:1002-1024 */
+    if (omega != 0.0) {
+        for (int i = 0; i < nlat_out; i++) {
+            double theta = pi * (double)i / (double)n, lat = pi / 2.0 - theta, tl = tan(lat);
+            double rr = a * sqrt((1.0 + tl * tl) / (1.0 + tl * tl / ((1.0 - f) * (1.0 - f))));
+            for (int k = 0; k < nlong; k++) {
+                RG(i, k) += rr * (sin(theta) * omega) * (sin(theta) * omega);
+                TG(i, k) += sin(theta) * cos(theta) * rr * omega * omega;
+                UG(i, k) += 0.5 * (rr * sin(theta) * omega) * (rr * sin(theta) * omega);
+            }
+        }
+    }
+    for (int i = 0; i < nlat_out; i++)
+        for (int k = 0; k < nlong; k++)
+            AG(i, k) = sqrt(RG(i, k) * RG(i, k) + PG(i, k) * PG(i, k) + TG(i, k) * TG(i, k));
+    if (normal_gravity == 1) {
+        double b = a * (1.0 - f);
+        for (int i = 0; i < nlat_out; i++) {
+            double theta = pi * (double)i / (double)n, lat = (pi / 2.0 - theta) * 180.0 / pi;
+            double ng = or_normalgravity(lat, normal_gravity_gm, omega, a, b);
+            for (int k = 0; k < nlong; k++) AG(i, k) -= ng;
+        }
+    }
+    if (extend == 1)
+        for (int i = 0; i < nlat_out; i++) {
+            RG(i, nlong_out - 1) = RG(i, 0);
+            TG(i, nlong_out - 1) = TG(i, 0);
+            PG(i, nlong_out - 1) = PG(i, 0);
+            AG(i, nlong_out - 1) = AG(i, 0);
+            UG(i, nlong_out - 1) = UG(i, 0);
+        }
+#undef RG
+#undef TG
+#undef PG
+#undef AG
+#undef UG
+    for (int j = 0; j < 8; j++) free(cs[j]);
+    free(coef); free(buf); free(row); free(pf); free(cf);
+    plan_destroy(plan);
+    free_tables(&t);
+    return 0;
+}
+
 /* ------------------------------------------- batched CPU round trips */
 
 /* One MakeGridDH + SHExpandDH round trip per field, fields spread over the

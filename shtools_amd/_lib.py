@@ -31,6 +31,9 @@ SIGNATURES = {
                            _c_int, _vp],
     "shtg_makegradient_dh": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, ctypes.c_double, _vp, _vp, _c_int,
                              _c_int, _vp],
+    "shtg_makegravgrid_dh": [_vp, _c_int, _c_int, _c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                             ctypes.c_double, _c_int, _c_int, ctypes.c_double, _c_int, ctypes.c_double, _c_int,
+                             _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp],
     "shtg_makegrid_dh_adjoint": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int,
                                  _c_int, _vp],
     "shtg_shexpand_dh_adjoint": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _vp],

@@ -352,6 +352,41 @@ def MakeGradientDH(cilm, lmax=None, sampling=2, lmax_calc=None, extend=0, radius
     return _finish(theta, as_tensor), _finish(phi, as_tensor)
 
 
+def MakeGravGridDH(cilm, gm, r0, lmax=None, a=None, f=0.0, sampling=2, lmax_calc=None, omega=0.0,
+                   normal_gravity=1, normal_gravity_gm=None, extend=0):
+    """Gravity (rad, theta, phi), its magnitude and the potential on a
+    flattened ellipsoid from 4pi-normalised potential coefficients
+    (src/MakeGravGridDH.f95; f2py src/pyshtools.pyf:1440-1477, whose
+    defaults these are: a = r0, f = 0, sampling 2, omega 0, normal gravity
+    removed).  Returns (rad, theta, phi, total, pot)."""
+    torch = _torch()
+    as_tensor = _is_tensor(cilm)
+    dev = _device_of(cilm)
+    c = _to_dev(cilm, dev)
+    if c.dim() != 3 or c.shape[0] < 2:
+        raise SHToolsError(_status_message(1))
+    if lmax is None:
+        lmax = c.shape[1] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    if a is None:
+        a = r0
+    if normal_gravity_gm is None:
+        normal_gravity_gm = gm
+    if sampling not in (1, 2) or extend not in (0, 1) or lmax < 0:
+        raise SHToolsError(_status_message(2))
+    g0 = 2 * (lmax + 1) + extend
+    g1 = sampling * 2 * (lmax + 1) + extend
+    outs = [torch.empty((g0, g1), dtype=torch.float64, device=dev) for _ in range(5)]
+    with torch.cuda.device(dev):
+        st = _lib.lib().shtg_makegravgrid_dh(c.data_ptr(), c.shape[1], c.shape[2], int(lmax), float(gm), float(r0),
+                                             float(a), float(f), int(sampling), int(lmax_calc), float(omega),
+                                             int(normal_gravity), float(normal_gravity_gm), int(extend),
+                                             *[o.data_ptr() for o in outs], g0, g1, _stream(dev))
+    _check(st)
+    return tuple(_finish(o, as_tensor) for o in outs)
+
+
 # ------------------------------------------------------------ host helpers
 
 def SHGLQ(lmax):

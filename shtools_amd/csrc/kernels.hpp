@@ -114,21 +114,31 @@ __device__ __forceinline__ double order_rescale(double u, int m) { return m == 0
 // wave).  al: recursion row of order m (al[j] = alpha_{m+j}); cp: packed
 // coefficients of order m; frow: fbuf row of order m.  The sums are stored
 // without the order's rescale factor, which the ring FFT applies.
-template <int K>
+//
+// kRho: sums of rho^(l-m) P_lm per ring (rho = ringr2[s]^(1/2), the radial
+// continuation factor r0/r of the ring): the rescaled recursion
+// q_l = alpha_l z q_{l-1} - q_{l-2} becomes, for rho^(l-m) q_l,
+// q_l = alpha_l (rho z) q_{l-1} - rho^2 q_{l-2}, with ringz holding rho z.
+template <int K, bool kRho>
 __device__ __forceinline__ void synth_leg_tile(const double *__restrict__ al, double sd,
-                                               const double *__restrict__ ringz, const double2 *__restrict__ cp,
-                                               int jmax, int P, int slot0, double4 *__restrict__ frow)
+                                               const double *__restrict__ ringz, const double *__restrict__ ringr2,
+                                               const double2 *__restrict__ cp, int jmax, int P, int slot0,
+                                               double4 *__restrict__ frow)
 {
     const int lane = threadIdx.x & (kWave - 1);
-    double z[K], q1[K], q2[K], eC[K], eS[K], oC[K], oS[K];
+    double z[K], q1[K], q2[K], eC[K], eS[K], oC[K], oS[K], r2[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int s = slot0 + k * kWave + lane;
         z[k] = (s < P) ? ringz[s] : 0.0;
+        r2[k] = (kRho && s < P) ? ringr2[s] : 1.0;
         q1[k] = 0.0;
-        q2[k] = -sd;  // the first step then yields q_m = seed
+        q2[k] = kRho ? -sd / r2[k] : -sd;  // the first step then yields q_m = seed
         eC[k] = eS[k] = oC[k] = oS[k] = 0.0;
     }
+    auto step = [&](double a, double z_, double qa, double qb, double r2_) {
+        return kRho ? (a * z_) * qa - r2_ * qb : (a * z_) * qa - qb;
+    };
     int j = 0;
 #pragma unroll 2
     for (; j + 1 <= jmax; j += 2) {
@@ -136,10 +146,10 @@ __device__ __forceinline__ void synth_leg_tile(const double *__restrict__ al, do
         const double2 c0 = cp[j], c1 = cp[j + 1];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const double q = (a0 * z[k]) * q1[k] - q2[k];
+            const double q = step(a0, z[k], q1[k], q2[k], r2[k]);
             eC[k] += q * c0.x;
             eS[k] += q * c0.y;
-            const double q3 = (a1 * z[k]) * q - q1[k];
+            const double q3 = step(a1, z[k], q, q1[k], r2[k]);
             oC[k] += q3 * c1.x;
             oS[k] += q3 * c1.y;
             q2[k] = q;
@@ -151,7 +161,7 @@ __device__ __forceinline__ void synth_leg_tile(const double *__restrict__ al, do
         const double2 c0 = cp[j];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const double q = (a0 * z[k]) * q1[k] - q2[k];
+            const double q = step(a0, z[k], q1[k], q2[k], r2[k]);
             eC[k] += q * c0.x;
             eS[k] += q * c0.y;
         }
@@ -163,17 +173,18 @@ __device__ __forceinline__ void synth_leg_tile(const double *__restrict__ al, do
     }
 }
 
-template <int K>
+template <int K, bool kRho = false>
 __device__ __forceinline__ void synth_leg_dispatch(int kk, const double *__restrict__ al, double sd,
                                                    const double *__restrict__ ringz,
                                                    const double2 *__restrict__ cp, int jmax, int P, int slot0,
-                                                   double4 *__restrict__ frow)
+                                                   double4 *__restrict__ frow,
+                                                   const double *__restrict__ ringr2 = nullptr)
 {
     static_assert(K == 4, "synthesis tiles are instantiated for 1 .. 4 ring pairs per lane");
-    if (kk <= 1) synth_leg_tile<1>(al, sd, ringz, cp, jmax, P, slot0, frow);
-    else if (kk == 2) synth_leg_tile<2>(al, sd, ringz, cp, jmax, P, slot0, frow);
-    else if (kk == 3) synth_leg_tile<3>(al, sd, ringz, cp, jmax, P, slot0, frow);
-    else synth_leg_tile<4>(al, sd, ringz, cp, jmax, P, slot0, frow);
+    if (kk <= 1) synth_leg_tile<1, kRho>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow);
+    else if (kk == 2) synth_leg_tile<2, kRho>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow);
+    else if (kk == 3) synth_leg_tile<3, kRho>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow);
+    else synth_leg_tile<4, kRho>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow);
 }
 
 // Analysis sums of order m over ring pairs [s0, P) (one wave):
@@ -526,18 +537,24 @@ __global__ __launch_bounds__(kThreads) void k_pack_synth(const double *__restric
 // grid: (Lc+1) * nsb single-wave workgroups, order m = blockIdx / nsb, slot
 // block sb of kSynthKMax*64 ring pairs; the last block takes as many ring
 // pairs per lane as its slots need.
+// ringr2: per-slot rho^2 of a radially continued synthesis (nullptr: none).
 __global__ __launch_bounds__(kWave) void k_synth_leg(const double *__restrict__ alpha, int Lt,
                                                      const double *__restrict__ seed,
                                                      const double *__restrict__ ringz,
                                                      const double2 *__restrict__ cpack, int Lc, int P, int nsb,
-                                                     double4 *__restrict__ fbuf)
+                                                     double4 *__restrict__ fbuf,
+                                                     const double *__restrict__ ringr2)
 {
     const int m = blockIdx.x / nsb;
     const int sb = blockIdx.x - m * nsb;
     const int slot0 = sb * kSynthKMax * kWave;
     const int kk = min(kSynthKMax, (P - slot0 + kWave - 1) / kWave);
-    synth_leg_dispatch<kSynthKMax>(kk, alpha + tri_offd(m, Lt), seed[m], ringz, cpack + tri_offd(m, Lc), Lc - m,
-                                   P, slot0, fbuf + (size_t)m * P);
+    if (ringr2)
+        synth_leg_dispatch<kSynthKMax, true>(kk, alpha + tri_offd(m, Lt), seed[m], ringz, cpack + tri_offd(m, Lc),
+                                             Lc - m, P, slot0, fbuf + (size_t)m * P, ringr2);
+    else
+        synth_leg_dispatch<kSynthKMax>(kk, alpha + tri_offd(m, Lt), seed[m], ringz, cpack + tri_offd(m, Lc), Lc - m,
+                                       P, slot0, fbuf + (size_t)m * P);
 }
 
 // rows[s] = (north row, south row or -1); rows >= nrows are skipped;
@@ -867,6 +884,51 @@ __global__ __launch_bounds__(kThreads) void k_gradient_coefs(const double *__res
         out[3 * plane + e] = l * sv;
         out[4 * plane + e] = m * sv;
         out[5 * plane + e] = -m * c;
+    }
+}
+
+// Per output row of a gravity grid on an ellipsoid: z of the row, sin of the
+// pair's northern colatitude, u^2, rho = r0/r and r of the row, the
+// centrifugal terms added to rad, theta and pot, the normal gravity taken
+// from the magnitude, and whether the row is a pole (theta and phi 0).
+struct GravRow {
+    double z, st, u2, rho, rex, crad, cth, cpot, ng, pole;
+};
+
+// In place: pot holds S(c), rad S(l c), theta S(A), phi S(Phi), the radially
+// continued syntheses of the potential coefficients (A_lm = a_l+1,m
+// c_l+1,m, Phi_lm = (m S_lm, -m C_lm)).  Then rad = -GM/r^2 (S(l c) + S(c)),
+// pot = GM/r S(c), theta = -sin t GM/r^2 (rho S(A) - z S(l c)) / u^2,
+// phi = GM/r^2 S(Phi) / sin t, plus the row's centrifugal terms, and
+// total = |(rad, theta, phi)| - normal gravity (src/MakeGravGridDH.f95:
+// 838-1052 term by term).
+__global__ __launch_bounds__(kThreads) void k_grav_combine(double *__restrict__ rad, double *__restrict__ theta,
+                                                           double *__restrict__ phi, double *__restrict__ total,
+                                                           double *__restrict__ pot,
+                                                           const GravRow *__restrict__ rowt, int nrows, int ncols,
+                                                           long long ld, double gm)
+{
+    const long long count = (long long)nrows * ncols;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < count;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(e / ncols), k = (int)(e - (long long)i * ncols);
+        const long long o = (long long)i * ld + k;
+        const GravRow R = rowt[i];
+        const double gu = pot[o], gb = rad[o], ga = theta[o], gp = phi[o];
+        const double r2 = R.rex * R.rex;
+        const double vr = -(gb + gu) * gm / r2 + R.crad;
+        const double vu = gu * gm / R.rex + R.cpot;
+        double vt = 0.0, vp = 0.0;
+        if (R.pole == 0.0) {
+            vt = -R.st * ((R.rho * ga - R.z * gb) / R.u2) * gm / r2;
+            vp = gp * (gm / r2) / R.st;
+        }
+        vt += R.cth;
+        rad[o] = vr;
+        theta[o] = vt;
+        phi[o] = vp;
+        pot[o] = vu;
+        total[o] = sqrt(vr * vr + vp * vp + vt * vt) - R.ng;
     }
 }
 

@@ -365,11 +365,19 @@ struct ProfScope {
 // Synthesis of rows [0, nrows_out) (+ the repeated column with ext_col).
 // ctab scales the coefficients before the Legendre sums (rt.s; rt.post for
 // the adjoint of analysis) and roww the output rows (nullptr: 1).
+// ring: per-slot (z, u, rho^2) of a radially continued synthesis, z and u
+// multiplied by rho (nullptr: the geometry's rings, no continuation).
+struct RhoRings {
+    const double *z = nullptr, *u = nullptr, *r2 = nullptr;
+};
+
 static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const DevSeeds &sd, const double *cilm,
                             int d1, int d2, int Lc, int nrows_out, int ext_col, double *grid, long long ld,
-                            hipStream_t st, const double *ctab = nullptr, const double2 *roww = nullptr)
+                            hipStream_t st, const double *ctab = nullptr, const double2 *roww = nullptr,
+                            const RhoRings &ring = RhoRings())
 {
     if (!ctab) ctab = rt.s;
+    const double *rz = ring.r2 ? ring.z : g.z, *ru = ring.r2 ? ring.u : g.u;
     const size_t npack = (size_t)tri_size(Lc);
     const size_t nf = (size_t)(Lc + 1) * g.P;
     const bool lds_fft = fft_in_lds(g.fft.m);
@@ -391,17 +399,17 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
     {
         ProfScope ps(SHTG_K_SYNTH_LEG, st);
         const int nsb = (g.P + kSynthKMax * kWave - 1) / (kSynthKMax * kWave);
-        hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nsb), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, cpack,
-                           Lc, g.P, nsb, fbuf);
+        hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nsb), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, rz, cpack,
+                           Lc, g.P, nsb, fbuf, ring.r2);
     }
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         if (lds_fft)
-            hipLaunchKernelGGL(k_synth_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, fbuf, g.P, Lc, g.u,
+            hipLaunchKernelGGL(k_synth_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, fbuf, g.P, Lc, ru,
                                g.rows, nrows_out, grid, ld, ext_col, roww);
         else
-            hipLaunchKernelGGL(k_synth_fft_ws, dim3(nwg), dim3(kFftMaxThreads), 0, st, g.fft, fbuf, g.P, Lc, g.u,
+            hipLaunchKernelGGL(k_synth_fft_ws, dim3(nwg), dim3(kFftMaxThreads), 0, st, g.fft, fbuf, g.P, Lc, ru,
                                g.rows, nrows_out, grid, ld, ext_col, roww, fws, ws_stride(g.fft.m));
     }
     if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
@@ -485,9 +493,9 @@ static int launch_synthesis_c(const Geometry &g, const DevRecurrence &rt, const 
         ProfScope ps(SHTG_K_SYNTH_LEG, st);
         const int nsb = (g.P + kSynthKMax * kWave - 1) / (kSynthKMax * kWave);
         hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nsb), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, cp_p,
-                           Lc, g.P, nsb, fb_p);
+                           Lc, g.P, nsb, fb_p, (const double *)nullptr);
         hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nsb), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, cp_m,
-                           Lc, g.P, nsb, fb_m);
+                           Lc, g.P, nsb, fb_m, (const double *)nullptr);
     }
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
@@ -1313,6 +1321,155 @@ int shtg_makegradient_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax,
         const int ncols = sampling * n + extend;
         hipLaunchKernelGGL(k_gradient_combine, dim3(grid_blocks((long long)sh.nrows * ncols)), dim3(kThreads), 0, hs,
                            theta, phi, gb, sh.nrows, ncols, (long long)grid_d1, n, radius);
+        if (!HIP_OK(hipGetLastError())) st = ST_OTHER;
+    }
+    (void)hipFreeAsync(ws, hs);
+    return st;
+}
+
+// ------------------------------------------------------------ gravity
+
+// NormalGravity of src/NormalGravity.f95:1-139 (Somigliana's closed form).
+static double normal_gravity(double geocentric_lat, double gm, double omega, double a, double b)
+{
+    const double pi = std::acos(-1.0);
+    if (a == b && omega == 0.0) return gm / (a * a);
+    if (a == b) {
+        const double cl = std::cos(geocentric_lat * pi / 180.0);
+        return std::sqrt(gm * gm / (a * a * a * a) + a * omega * omega * cl * cl * (a * omega * omega - 2.0 * gm / (a * a)));
+    }
+    const double m = omega * omega * a * a * b / gm;
+    const double bigE = std::sqrt(a * a - b * b), ep = bigE / b;
+    const double q0 = 0.5 * ((1.0 + 3.0 * (b / bigE) * (b / bigE)) * std::atan(bigE / b) - 3.0 * b / bigE);
+    const double q0p = 3.0 * (1.0 + (b / bigE) * (b / bigE)) * (1.0 - b / bigE * std::atan(bigE / b)) - 1.0;
+    const double ga = gm / (a * b) * (1.0 - m - m * ep * q0p / 6.0 / q0);
+    const double gb = gm / (a * a) * (1.0 + m * ep * q0p / 3.0 / q0);
+    const double gl = std::atan((a / b) * (a / b) * std::tan(geocentric_lat * pi / 180.0));
+    const double c = std::cos(gl), sn = std::sin(gl);
+    return (a * ga * c * c + b * gb * sn * sn) / std::sqrt(a * a * c * c + b * b * sn * sn);
+}
+
+// Ring and row tables of one gravity grid: per slot rho z, rho u and rho^2
+// with rho = r0 / r of the slot's ellipsoid radius, and the GravRow of each
+// output row.  Cached by every parameter they depend on.
+struct GravTables {
+    double *z = nullptr, *u = nullptr, *r2 = nullptr;
+    GravRow *rows = nullptr;
+    bool ok = false;
+};
+static std::map<std::vector<uint64_t>, std::unique_ptr<GravTables>> g_grav;
+
+static uint64_t dbits(double x)
+{
+    uint64_t b;
+    std::memcpy(&b, &x, sizeof b);
+    return b;
+}
+
+static const GravTables *get_grav(int dev, int lmax, int sampling, int extend, double r0, double a, double f,
+                                  double omega, int normal_grav, double gm_ng)
+{
+    const std::vector<uint64_t> key = {(uint64_t)dev, (uint64_t)lmax, (uint64_t)sampling, (uint64_t)extend,
+                                       dbits(r0), dbits(a), dbits(f), dbits(omega), (uint64_t)normal_grav,
+                                       dbits(gm_ng)};
+    auto &slot = g_grav[key];
+    if (slot) return slot->ok ? slot.get() : nullptr;
+    slot.reset(new GravTables());
+    GravTables &t = *slot;
+    const int n = 2 * lmax + 2, P = n / 2 + 1, nrows = n + extend;
+    const double pi = std::acos(-1.0);
+    // the ellipsoid radius of row pair s (src/MakeGravGridDH.f95:435,622-631)
+    auto r_of_slot = [&](int s) {
+        if (s == n / 2) return a;
+        if (s == 0) return a * (1.0 - f);
+        const double lat = pi / 2.0 - pi * (double)s / (double)n;
+        const double q = std::cos(lat) * std::cos(lat) + std::sin(lat) * std::sin(lat) / ((1.0 - f) * (1.0 - f));
+        return a * std::sqrt(1.0 / q);
+    };
+    std::vector<double> z(P), u(P), r2(P);
+    for (int s = 0; s < P; ++s) {
+        const double zz = (s == n / 2) ? 0.0 : std::cos(pi * (double)s / (double)n);
+        const double uu = (s == n / 2) ? 1.0 : std::sqrt((1.0 - zz) * (1.0 + zz));
+        const double rho = r0 / r_of_slot(s);
+        z[s] = rho * zz;
+        u[s] = rho * uu;
+        r2[s] = rho * rho;
+    }
+    std::vector<GravRow> rows(nrows);
+    const double b = a * (1.0 - f);
+    for (int i = 0; i < nrows; ++i) {
+        const int s = (2 * i <= n) ? i : n - i;  // the row pair's slot (northern row index)
+        const double tn = (s == n / 2) ? pi / 2.0 : pi * (double)s / (double)n;
+        GravRow &R = rows[i];
+        R.z = (s == n / 2) ? 0.0 : (i == s ? std::cos(tn) : -std::cos(tn));
+        R.st = std::sin(tn);
+        const double zn = (s == n / 2) ? 0.0 : std::cos(tn);
+        R.u2 = (s == n / 2) ? 1.0 : (1.0 - zn) * (1.0 + zn);
+        R.rex = r_of_slot(s);
+        R.rho = r0 / R.rex;
+        R.pole = (s == 0) ? 1.0 : 0.0;
+        // centrifugal terms and normal gravity at the row's own colatitude
+        // (src/MakeGravGridDH.f95:1004-1050)
+        const double th = pi * (double)i / (double)n, lat = pi / 2.0 - th, tl = std::tan(lat);
+        const double rr = a * std::sqrt((1.0 + tl * tl) / (1.0 + tl * tl / ((1.0 - f) * (1.0 - f))));
+        R.crad = R.cth = R.cpot = 0.0;
+        if (omega != 0.0) {
+            R.crad = rr * (std::sin(th) * omega) * (std::sin(th) * omega);
+            R.cth = std::sin(th) * std::cos(th) * rr * omega * omega;
+            R.cpot = 0.5 * (rr * std::sin(th) * omega) * (rr * std::sin(th) * omega);
+        }
+        R.ng = normal_grav ? normal_gravity(lat * 180.0 / pi, gm_ng, omega, a, b) : 0.0;
+    }
+    t.z = dev_upload(z);
+    t.u = dev_upload(u);
+    t.r2 = dev_upload(r2);
+    t.rows = dev_upload(rows);
+    t.ok = t.z && t.u && t.r2 && t.rows;
+    return t.ok ? &t : nullptr;
+}
+
+int shtg_makegravgrid_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax, double gm, double r0, double a,
+                         double f, int sampling, int lmax_calc, double omega, int normal_gravity_flag,
+                         double normal_gravity_gm, int extend, double *rad, double *theta, double *phi,
+                         double *total, double *pot, int grid_d0, int grid_d1, void *stream)
+{
+    // argument checks in the order of src/MakeGravGridDH.f95:140-300
+    Shape sh;
+    int st = check_makegrid_dh(cilm_d1, cilm_d2, lmax, 1, sampling, 1, lmax_calc, extend, grid_d0, grid_d1, sh);
+    if (st) return st;
+    if (normal_gravity_flag != 0 && normal_gravity_flag != 1) return ST_BOUNDS;
+    if (!cilm || !rad || !theta || !phi || !total || !pot) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    CallTables t;
+    if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, 1, 1, sh.Lc, t))) return st;
+    const GravTables *gt = get_grav(current_device(), lmax, sampling, extend, r0, a, f, omega, normal_gravity_flag,
+                                    normal_gravity_gm);
+    if (!gt) return ST_ALLOC;
+    hipStream_t hs = (hipStream_t)stream;
+    const int L1 = sh.Lc + 1, n = 2 * lmax + 2;
+    const size_t plane = (size_t)L1 * L1;
+    void *ws = nullptr;
+    if (!HIP_OK(hipMallocAsync(&ws, 6 * plane * sizeof(double), hs))) return ST_ALLOC;
+    double *coefs = (double *)ws;
+    hipLaunchKernelGGL(k_gradient_coefs, dim3(grid_blocks((long long)plane)), dim3(kThreads), 0, hs, cilm, cilm_d1,
+                       cilm_d2, sh.Lc, coefs);
+    RhoRings ring;
+    ring.z = gt->z;
+    ring.u = gt->u;
+    ring.r2 = gt->r2;
+    const Geometry &g = *t.g;
+    st = launch_synthesis(g, *t.rt, *t.sd, cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend, pot, grid_d1, hs,
+                          nullptr, nullptr, ring);
+    if (!st) st = launch_synthesis(g, *t.rt, *t.sd, coefs + 2 * plane, L1, L1, sh.Lc, sh.nrows, extend, rad, grid_d1,
+                                   hs, nullptr, nullptr, ring);
+    if (!st) st = launch_synthesis(g, *t.rt, *t.sd, coefs, L1, L1, sh.Lc, sh.nrows, extend, theta, grid_d1, hs,
+                                   nullptr, nullptr, ring);
+    if (!st) st = launch_synthesis(g, *t.rt, *t.sd, coefs + 4 * plane, L1, L1, sh.Lc, sh.nrows, extend, phi, grid_d1,
+                                   hs, nullptr, nullptr, ring);
+    if (!st) {
+        const int ncols = sampling * n + extend;
+        hipLaunchKernelGGL(k_grav_combine, dim3(grid_blocks((long long)sh.nrows * ncols)), dim3(kThreads), 0, hs, rad,
+                           theta, phi, total, pot, gt->rows, sh.nrows, ncols, (long long)grid_d1, gm);
         if (!HIP_OK(hipGetLastError())) st = ST_OTHER;
     }
     (void)hipFreeAsync(ws, hs);

@@ -110,6 +110,22 @@ int shtg_makegravgrid_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax,
                          double *theta, double *phi, double *total, double *pot, int grid_d0,
                          int grid_d1, void *stream);
 
+/* NormalGravity: src/NormalGravity.f95:1-139 (host).  Normal gravity of a
+ * rotating ellipsoid (semi-axes a >= b, GM, omega) at geocentric latitude
+ * geocentric_lat (degrees), by Somigliana's closed form; into *out. */
+int shtg_normal_gravity(double geocentric_lat, double gm, double omega, double a, double b, double *out);
+
+/* SHMultiply: src/SHMultiply.f95:1-368 (f2py src/pyshtools.pyf:596-620, which
+ * hides precomp = 0).  Coefficients [2][out_d1][out_d2] (at least
+ * lmax1+lmax2+1 square) of the pointwise product of the fields of the
+ * leading (lmax1+1)^2 and (lmax2+1)^2 blocks of cilm1 and cilm2: both
+ * synthesised on the Gauss-Legendre grid of degree lmax1+lmax2, multiplied,
+ * and expanded.  The reference's optional precomputed Legendre table (plx)
+ * has no counterpart: the GPU recomputes the functions in every pass. */
+int shtg_shmultiply(double *cilmout, int out_d1, int out_d2, const double *cilm1, int c1_d1, int c1_d2,
+                    int lmax1, const double *cilm2, int c2_d1, int c2_d2, int lmax2, int norm, int csphase,
+                    void *stream);
+
 /* Adjoints (transposes) of the four transforms, for gradients through them
  * (no counterpart in the reference; the differentiable path of the
  * transform API).  Each takes the argument list of its forward call with

@@ -34,6 +34,10 @@ SIGNATURES = {
     "shtg_makegravgrid_dh": [_vp, _c_int, _c_int, _c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                              ctypes.c_double, _c_int, _c_int, ctypes.c_double, _c_int, ctypes.c_double, _c_int,
                              _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp],
+    "shtg_normal_gravity": [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                            _dp],
+    "shtg_shmultiply": [_vp, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _c_int,
+                        _c_int, _vp],
     "shtg_makegrid_dh_adjoint": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int,
                                  _c_int, _vp],
     "shtg_shexpand_dh_adjoint": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _vp],

@@ -387,6 +387,42 @@ def MakeGravGridDH(cilm, gm, r0, lmax=None, a=None, f=0.0, sampling=2, lmax_calc
     return tuple(_finish(o, as_tensor) for o in outs)
 
 
+def NormalGravity(geocentric_lat, gm, omega, a, b):
+    """Normal gravity of a rotating ellipsoid (src/NormalGravity.f95)."""
+    out = ctypes.c_double(0.0)
+    _check(_lib.lib().shtg_normal_gravity(float(geocentric_lat), float(gm), float(omega), float(a), float(b),
+                                          ctypes.byref(out)))
+    return out.value
+
+
+# ------------------------------------------------------------ products
+
+def SHMultiply(sh1, sh2, lmax1=None, lmax2=None, norm=1, csphase=1):
+    """Coefficients of degree lmax1 + lmax2 of the product of two real fields
+    (src/SHMultiply.f95; f2py src/pyshtools.pyf:596-620)."""
+    torch = _torch()
+    as_tensor = _is_tensor(sh1) or _is_tensor(sh2)
+    dev = _device_of(sh1, sh2)
+    a = _to_dev(sh1, dev)
+    b = _to_dev(sh2, dev)
+    if a.dim() != 3 or b.dim() != 3 or a.shape[0] < 2 or b.shape[0] < 2:
+        raise SHToolsError(_status_message(1))
+    if lmax1 is None:
+        lmax1 = a.shape[1] - 1
+    if lmax2 is None:
+        lmax2 = b.shape[1] - 1
+    n = int(lmax1) + int(lmax2) + 1
+    if n < 1:
+        raise SHToolsError(_status_message(2))
+    out = torch.empty((2, n, n), dtype=torch.float64, device=dev)
+    with torch.cuda.device(dev):
+        st = _lib.lib().shtg_shmultiply(out.data_ptr(), n, n, a.data_ptr(), a.shape[1], a.shape[2], int(lmax1),
+                                        b.data_ptr(), b.shape[1], b.shape[2], int(lmax2), int(norm), int(csphase),
+                                        _stream(dev))
+    _check(st)
+    return _finish(out, as_tensor)
+
+
 # ------------------------------------------------------------ host helpers
 
 def SHGLQ(lmax):

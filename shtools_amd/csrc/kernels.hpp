@@ -887,6 +887,14 @@ __global__ __launch_bounds__(kThreads) void k_gradient_coefs(const double *__res
     }
 }
 
+// a[e] *= b[e] for e < n (the pointwise product of two grids).
+__global__ __launch_bounds__(kThreads) void k_mul_inplace(double *__restrict__ a, const double *__restrict__ b,
+                                                          long long n)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
+        a[e] *= b[e];
+}
+
 // Per output row of a gravity grid on an ellipsoid: z of the row, sin of the
 // pair's northern colatitude, u^2, rho = r0/r and r of the row, the
 // centrifugal terms added to rad, theta and pot, the normal gravity taken

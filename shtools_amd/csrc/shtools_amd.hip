@@ -1476,6 +1476,64 @@ int shtg_makegravgrid_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax,
     return st;
 }
 
+int shtg_normal_gravity(double geocentric_lat, double gm, double omega, double a, double b, double *out)
+{
+    if (!out) return ST_DIMS;
+    *out = normal_gravity(geocentric_lat, gm, omega, a, b);
+    return ST_OK;
+}
+
+// ------------------------------------------------------------ products
+
+// Gauss-Legendre nodes and weights of nlat points, cached (host).
+static const std::pair<std::vector<double>, std::vector<double>> *glq_cached(int nlat)
+{
+    static std::map<int, std::pair<std::vector<double>, std::vector<double>>> cache;
+    auto it = cache.find(nlat);
+    if (it != cache.end()) return &it->second;
+    std::vector<double> z(nlat), w(nlat);
+    if (glq_nodes(nlat, z.data(), w.data())) return nullptr;
+    return &(cache[nlat] = {std::move(z), std::move(w)});
+}
+
+int shtg_shmultiply(double *cilmout, int out_d1, int out_d2, const double *cilm1, int c1_d1, int c1_d2, int lmax1,
+                    const double *cilm2, int c2_d1, int c2_d2, int lmax2, int norm, int csphase, void *stream)
+{
+    // argument checks in the order of src/SHMultiply.f95:61-180
+    if (lmax1 < 0 || lmax2 < 0) return ST_BOUNDS;
+    if (c1_d1 < lmax1 + 1 || c1_d2 < lmax1 + 1) return ST_DIMS;
+    if (c2_d1 < lmax2 + 1 || c2_d2 < lmax2 + 1) return ST_DIMS;
+    const int lmaxout = lmax1 + lmax2;
+    if (out_d1 < lmaxout + 1 || out_d2 < lmaxout + 1) return ST_DIMS;
+    if (csphase != 1 && csphase != -1) return ST_BOUNDS;
+    if (norm < 1 || norm > 4) return ST_BOUNDS;
+    if (!cilmout || !cilm1 || !cilm2) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    const int nlat = lmaxout + 1, nlong = 2 * lmaxout + 1;
+    const auto *zw = glq_cached(nlat);
+    if (!zw) return ST_OTHER;
+    CallTables t;
+    int st = get_tables(false, lmaxout, 0, zw->first.data(), zw->second.data(), norm, csphase, lmaxout, t);
+    if (st) return st;
+    hipStream_t hs = (hipStream_t)stream;
+    const size_t ng = (size_t)nlat * nlong;
+    void *ws = nullptr;
+    if (!HIP_OK(hipMallocAsync(&ws, 2 * ng * sizeof(double), hs))) return ST_ALLOC;
+    double *g1 = (double *)ws, *g2 = g1 + ng;
+    // each factor synthesised on the product's grid from its leading
+    // (lmax+1)^2 block (src/SHMultiply.f95:330-350)
+    st = launch_synthesis(*t.g, *t.rt, *t.sd, cilm1, c1_d1, c1_d2, lmax1, nlat, 0, g1, nlong, hs);
+    if (!st) st = launch_synthesis(*t.g, *t.rt, *t.sd, cilm2, c2_d1, c2_d2, lmax2, nlat, 0, g2, nlong, hs);
+    if (!st) {
+        hipLaunchKernelGGL(k_mul_inplace, dim3(grid_blocks((long long)ng)), dim3(kThreads), 0, hs, g1, g2,
+                           (long long)ng);
+        if (!HIP_OK(hipGetLastError())) st = ST_OTHER;
+    }
+    if (!st) st = launch_analysis(*t.g, *t.rt, *t.sd, g1, nlong, lmaxout, cilmout, out_d1, out_d2, hs);
+    (void)hipFreeAsync(ws, hs);
+    return st;
+}
+
 int shtg_shglq(int lmax, double *zero, double *w)
 {
     if (lmax < 0) return ST_BOUNDS;

@@ -108,3 +108,12 @@ def test_argument_checks_before_any_device_work():
     assert lib.shtg_makegrid_glq(dummy, 5, 5, 4, dp(), 5, 1, 1, 4, 0, dummy, 5, 9, nul) == 5
     assert lib.shtg_makegrid_glq(dummy, 5, 5, 4, B._dptr(z), 4, 1, 1, 4, 0, dummy, 5, 9, nul) == 1
     assert lib.shtg_shexpand_glq(dummy, 5, 9, B._dptr(z), 4, B._dptr(z), 5, 1, 1, 4, dummy, 5, 5, nul) == 1
+
+
+def test_normal_gravity_matches_oracle():
+    """Host helper (no GPU): the library's NormalGravity equals the oracle's
+    restatement (pinned by WGS84 values in test_oracle_grav.py)."""
+    GM, a, b, w = 3.986004418e14, 6378137.0, 6356752.314245, 7.292115e-5
+    for lat in (-90.0, -45.5, 0.0, 12.25, 60.0, 90.0):
+        assert abs(B.NormalGravity(lat, GM, w, a, b) - oracle.NormalGravity(lat, GM, w, a, b)) <= 1e-14 * 9.8
+    assert B.NormalGravity(10.0, GM, 0.0, a, a) == GM / a ** 2

@@ -1,5 +1,5 @@
-"""Pins of the oracle's MakeGravGridDH and NormalGravity restatements
-(oracle/sht_oracle.c: or_makegravgriddh, or_normalgravity), independent of
+"""Pins of the oracle's MakeGravGridDH, NormalGravity and SHMultiply restatements
+(oracle/sht_oracle.c: or_makegravgriddh, or_normalgravity, or_shmultiply), independent of
 the restated code paths:
 
 * NormalGravity: the published WGS84 normal gravity at the equator and the
@@ -12,7 +12,10 @@ the restated code paths:
 * the centrifugal terms of a point mass;
 * random coefficients on a sphere (f = 0): pot, rad, theta and phi against
   the oracle's MakeGridDH and MakeGradientDH of the continued coefficients
-  (r0/a)^l C_lm, which are pinned on their own in test_oracle.py.
+  (r0/a)^l C_lm, which are pinned on their own in test_oracle.py;
+* SHMultiply: the square of P_10 in every normalisation (closed form), the
+  unit field as identity, exact commutation, and the product's DH synthesis
+  against the pointwise product of the factors' syntheses.
 """
 import numpy as np
 import pytest
@@ -146,3 +149,59 @@ def test_errors():
         oracle.MakeGravGridDH(c, GM, A, lmax_calc=9)
     with pytest.raises(Exception):
         oracle.MakeGravGridDH(c, GM, A, normal_gravity=2)
+
+
+# ------------------------------------------------------------ SHMultiply
+
+def test_shmultiply_zonal_square_closed_forms():
+    """(P_10)^2 in each normalisation: 4pi 3 z^2 = 1 + (2/sqrt 5) Pbar_20;
+    Schmidt and unnormalised z^2 = 1/3 + (2/3) P_20; orthonormal
+    (3/4pi) z^2 = (1/sqrt(4pi)) Y_00 + (1/(2pi)) / sqrt(5/(4pi)) Y_20."""
+    c = np.zeros((2, 2, 2))
+    c[0, 1, 0] = 1.0
+    want = {1: (1.0, 2 / np.sqrt(5.0)), 2: (1 / 3, 2 / 3), 3: (1 / 3, 2 / 3),
+            4: (1 / np.sqrt(4 * np.pi), (1 / (2 * np.pi)) / np.sqrt(5 / (4 * np.pi)))}
+    for norm, (c00, c20) in want.items():
+        o = oracle.SHMultiply(c, c, norm=norm)
+        assert o.shape == (2, 3, 3)
+        ref = np.zeros((2, 3, 3))
+        ref[0, 0, 0], ref[0, 2, 0] = c00, c20
+        assert np.abs(o - ref).max() <= 1e-14, norm
+
+
+def test_shmultiply_identity_and_commutation():
+    rng = np.random.default_rng(21)
+    f = rand_cilm(rng, 9)
+    one = np.zeros((2, 4, 4))
+    one[0, 0, 0] = 1.0
+    o = oracle.SHMultiply(one, f)
+    assert o.shape == (2, 13, 13)
+    assert np.abs(o[:, :10, :10] - f).max() <= 1e-13 * np.abs(f).max()
+    assert np.abs(o[:, 10:, :]).max() <= 1e-13 * np.abs(f).max()
+    g = rand_cilm(rng, 5)
+    np.testing.assert_array_equal(oracle.SHMultiply(f, g), oracle.SHMultiply(g, f))
+
+
+@pytest.mark.parametrize("norm,csphase", [(1, 1), (2, -1), (4, 1)])
+def test_shmultiply_is_the_pointwise_product(norm, csphase):
+    """On a DH grid of degree >= lmax1 + lmax2 the product's synthesis equals
+    the product of the syntheses (the oracle's MakeGridDH is pinned on its own)."""
+    rng = np.random.default_rng(22 + norm)
+    a, b = rand_cilm(rng, 7), rand_cilm(rng, 6)
+    o = oracle.SHMultiply(a, b, norm=norm, csphase=csphase)
+    L = 13
+    pad = lambda c: np.pad(c, ((0, 0), (0, L + 1 - c.shape[1]), (0, L + 1 - c.shape[2])))
+    ga = oracle.MakeGridDH(pad(a), sampling=2, norm=norm, csphase=csphase)
+    gb = oracle.MakeGridDH(pad(b), sampling=2, norm=norm, csphase=csphase)
+    go = oracle.MakeGridDH(pad(o), sampling=2, norm=norm, csphase=csphase)
+    assert np.abs(go - ga * gb).max() <= 1e-12 * np.abs(ga * gb).max()
+
+
+def test_shmultiply_errors():
+    c = np.zeros((2, 3, 3))
+    with pytest.raises(Exception):
+        oracle.SHMultiply(c, c, lmax1=5)
+    with pytest.raises(Exception):
+        oracle.SHMultiply(c, c, norm=5)
+    with pytest.raises(Exception):
+        oracle.SHMultiply(c, c, csphase=0)

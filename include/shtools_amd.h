@@ -126,6 +126,13 @@ int shtg_shmultiply(double *cilmout, int out_d1, int out_d2, const double *cilm1
                     int lmax1, const double *cilm2, int c2_d1, int c2_d2, int lmax2, int norm, int csphase,
                     void *stream);
 
+/* The polar skip of the DH transforms (host; no counterpart in the
+ * reference, which sums every term): kmax[s] for the ring-pair slots
+ * s = 0 .. lmax+1 (slot s holds rows s and 2 lmax + 2 - s, colatitude
+ * pi s / (2 lmax + 2)) is the largest order whose 4pi-normalised functions
+ * reach 1e-32 at some degree <= lmax; orders above it are not summed there. */
+int shtg_dh_polar_orders(int lmax, int *kmax);
+
 /* Adjoints (transposes) of the four transforms, for gradients through them
  * (no counterpart in the reference; the differentiable path of the
  * transform API).  Each takes the argument list of its forward call with

@@ -36,6 +36,7 @@ SIGNATURES = {
                              _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp],
     "shtg_normal_gravity": [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                             _dp],
+    "shtg_dh_polar_orders": [_c_int, ctypes.POINTER(ctypes.c_int)],
     "shtg_shmultiply": [_vp, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _c_int,
                         _c_int, _vp],
     "shtg_makegrid_dh_adjoint": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int,

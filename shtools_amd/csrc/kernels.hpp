@@ -298,6 +298,14 @@ __device__ __forceinline__ void anal_leg_tile(const double *__restrict__ al, dou
 // workgroup transforming several slots can load the next slot's inputs
 // while the current one is in its FFT stages.
 
+// The orders a ring slot carries: Lc, or fewer near the poles (kmax: per slot
+// the largest order whose Legendre functions are not negligible there;
+// nullptr: all).
+__device__ __forceinline__ int slot_orders(const int *__restrict__ kmax, int s, int Lc)
+{
+    return kmax ? min(Lc, kmax[s]) : Lc;
+}
+
 // Synthesis inputs of one thread: ring sums of orders 2*threadIdx.x and
 // 2*threadIdx.x + 1 (orders beyond 2 * blockDim.x are read in the fill).
 constexpr int kSynthPre = 2;
@@ -537,17 +545,20 @@ __global__ __launch_bounds__(kThreads) void k_pack_synth(const double *__restric
 // grid: (Lc+1) * nsb single-wave workgroups, order m = blockIdx / nsb, slot
 // block sb of kSynthKMax*64 ring pairs; the last block takes as many ring
 // pairs per lane as its slots need.
-// ringr2: per-slot rho^2 of a radially continued synthesis (nullptr: none).
+// ringr2: per-slot rho^2 of a radially continued synthesis (nullptr: none);
+// smin: per order the first slot with non-negligible functions (nullptr: 0).
 __global__ __launch_bounds__(kWave) void k_synth_leg(const double *__restrict__ alpha, int Lt,
                                                      const double *__restrict__ seed,
                                                      const double *__restrict__ ringz,
                                                      const double2 *__restrict__ cpack, int Lc, int P, int nsb,
                                                      double4 *__restrict__ fbuf,
-                                                     const double *__restrict__ ringr2)
+                                                     const double *__restrict__ ringr2,
+                                                     const int *__restrict__ smin)
 {
     const int m = blockIdx.x / nsb;
     const int sb = blockIdx.x - m * nsb;
-    const int slot0 = sb * kSynthKMax * kWave;
+    const int slot0 = (smin ? smin[m] : 0) + sb * kSynthKMax * kWave;
+    if (slot0 >= P) return;
     const int kk = min(kSynthKMax, (P - slot0 + kWave - 1) / kWave);
     if (ringr2)
         synth_leg_dispatch<kSynthKMax, true>(kk, alpha + tri_offd(m, Lt), seed[m], ringz, cpack + tri_offd(m, Lc),
@@ -563,11 +574,12 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_synth_fft(FftDev F, const do
                                                         const double *__restrict__ ringu,
                                                         const int2 *__restrict__ rows, int nrows,
                                                         double *__restrict__ grid, long long ld, int ext_col,
-                                                        const double2 *__restrict__ roww)
+                                                        const double2 *__restrict__ roww,
+                                                        const int *__restrict__ kmax)
 {
     extern __shared__ double2 lds[];
     const int s = blockIdx.x;
-    synth_fft_tile(F, fbuf, P, Lc, s, ringu[s], rows[s], nrows, grid, ld, ext_col,
+    synth_fft_tile(F, fbuf, P, slot_orders(kmax, s, Lc), s, ringu[s], rows[s], nrows, grid, ld, ext_col,
                    roww ? roww[s] : make_double2(1.0, 1.0), lds);
 }
 
@@ -575,11 +587,13 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft(FftDev F, const dou
                                                        const double *__restrict__ ringu,
                                                        const int2 *__restrict__ rows, int nrows,
                                                        const double2 *__restrict__ wts, int P, int Lc,
-                                                       double4 *__restrict__ gbuf, int ext_col)
+                                                       double4 *__restrict__ gbuf, int ext_col,
+                                                       const int *__restrict__ kmax)
 {
     extern __shared__ double2 lds[];
     const int s = blockIdx.x;
-    anal_fft_tile(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, Lc, s, gbuf, ext_col, lds);
+    anal_fft_tile(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, slot_orders(kmax, s, Lc), s, gbuf, ext_col,
+                  lds);
 }
 
 // Rings whose FFT work array does not fit the LDS of a CU: the same tiles
@@ -590,11 +604,11 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_synth_fft_ws(FftDev F, const
                                                            const int2 *__restrict__ rows, int nrows,
                                                            double *__restrict__ grid, long long ld, int ext_col,
                                                            const double2 *__restrict__ roww, double2 *ws,
-                                                           long long ws_stride)
+                                                           long long ws_stride, const int *__restrict__ kmax)
 {
     double2 *x = ws + blockIdx.x * ws_stride;
     for (int s = blockIdx.x; s < P; s += gridDim.x) {
-        synth_fft_tile(F, fbuf, P, Lc, s, ringu[s], rows[s], nrows, grid, ld, ext_col,
+        synth_fft_tile(F, fbuf, P, slot_orders(kmax, s, Lc), s, ringu[s], rows[s], nrows, grid, ld, ext_col,
                        roww ? roww[s] : make_double2(1.0, 1.0), x);
         __syncthreads();
     }
@@ -605,27 +619,29 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft_ws(FftDev F, const 
                                                           const int2 *__restrict__ rows, int nrows,
                                                           const double2 *__restrict__ wts, int P, int Lc,
                                                           double4 *__restrict__ gbuf, int ext_col, double2 *ws,
-                                                          long long ws_stride)
+                                                          long long ws_stride, const int *__restrict__ kmax)
 {
     double2 *x = ws + blockIdx.x * ws_stride;
     for (int s = blockIdx.x; s < P; s += gridDim.x) {
-        anal_fft_tile(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, Lc, s, gbuf, ext_col, x);
+        anal_fft_tile(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, slot_orders(kmax, s, Lc), s, gbuf, ext_col,
+                      x);
         __syncthreads();
     }
 }
 
-// one single-wave workgroup per order m over ring pairs [s0, P)
+// one single-wave workgroup per order m over ring pairs [s0, P), from the
+// order's first slot with non-negligible functions (smin; nullptr: s0)
 template <int K>
 __global__ __launch_bounds__(kWave) void k_anal_leg(const double *__restrict__ alpha, int Lt,
                                                     const double *__restrict__ seed,
                                                     const double *__restrict__ ringz,
                                                     const double4 *__restrict__ gbuf, int Lc, int s0, int P,
-                                                    double2 *__restrict__ cbuf)
+                                                    double2 *__restrict__ cbuf, const int *__restrict__ smin)
 {
     __shared__ double red[2 * kAnalLC * kRedStride];
     const int m = blockIdx.x;
-    anal_leg_tile<K>(alpha + tri_offd(m, Lt), seed[m], ringz, gbuf + (size_t)m * P, Lc - m, s0, P,
-                     cbuf + tri_offd(m, Lc), red);
+    anal_leg_tile<K>(alpha + tri_offd(m, Lt), seed[m], ringz, gbuf + (size_t)m * P, Lc - m,
+                     smin ? max(s0, smin[m]) : s0, P, cbuf + tri_offd(m, Lc), red);
 }
 
 // grid: ntm x ntl tiles over the whole d1 x d2 array
@@ -986,6 +1002,7 @@ struct BField {
     int fft;         // index of the field's FFT plan
     long long coff;  // offset of the field in cpack/cbuf (double2 units)
     long long foff;  // offset of the field in fbuf/gbuf (double4 units)
+    int sm;          // offset of the field's per-order first slots in T.smin
 };
 
 struct BTables {
@@ -997,6 +1014,8 @@ struct BTables {
     int s0;               // first slot of the analysis sums
     const int2 *rows;
     const FftDev *plans;
+    const int *kmax;      // per slot (indexed like z): orders with non-negligible functions
+    const int *smin;      // per field (BField::sm) and order: first such slot
 };
 
 // items: (field, first order, first degree) of a coefficient tile
@@ -1050,11 +1069,11 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_synth_fft(const BField *__r
     // the next slot's ring sums are loaded while the current slot is in its
     // FFT stages
     SynthRing in;
-    synth_fft_load(fb, f.P, f.L, it.y, in);
+    synth_fft_load(fb, f.P, slot_orders(T.kmax, f.geo + it.y, f.L), it.y, in);
     for (int s = it.y; s < send; ++s) {
-        synth_fft_fill(F, in, fb, f.P, f.L, s, T.u[f.geo + s], lds);
+        synth_fft_fill(F, in, fb, f.P, slot_orders(T.kmax, f.geo + s, f.L), s, T.u[f.geo + s], lds);
         __syncthreads();
-        if (s + 1 < send) synth_fft_load(fb, f.P, f.L, s + 1, in);
+        if (s + 1 < send) synth_fft_load(fb, f.P, slot_orders(T.kmax, f.geo + s + 1, f.L), s + 1, in);
         synth_fft_run(F, T.rows[f.geo + s], f.nlat, grid[it.x], f.nlong, 0,
                       T.roww ? T.roww[f.geo + s] : make_double2(1.0, 1.0), lds);
         __syncthreads();
@@ -1083,7 +1102,8 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft(const BField *__re
         anal_fft_fill(F, in, g, f.nlong, rw, f.nlat, 0, lds);
         __syncthreads();
         if (s + 1 < send) anal_fft_load(F, g, f.nlong, T.rows[f.geo + s + 1], f.nlat, 0, in);
-        anal_fft_run(F, rw, f.nlat, T.w[f.geo + s], T.u[f.geo + s], f.P, f.L, s, gbuf + f.foff, lds);
+        anal_fft_run(F, rw, f.nlat, T.w[f.geo + s], T.u[f.geo + s], f.P, slot_orders(T.kmax, f.geo + s, f.L), s,
+                     gbuf + f.foff, lds);
         __syncthreads();
     }
 }
@@ -1101,8 +1121,9 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_synth_fft_ws(const BField *
         const BField f = fields[it.x];
         const FftDev &F = T.plans[f.fft];
         for (int s = it.y; s < it.y + nslots && s < f.P; ++s) {
-            synth_fft_tile(F, fbuf + f.foff, f.P, f.L, s, T.u[f.geo + s], T.rows[f.geo + s], f.nlat, grid[it.x],
-                           f.nlong, 0, T.roww ? T.roww[f.geo + s] : make_double2(1.0, 1.0), x);
+            synth_fft_tile(F, fbuf + f.foff, f.P, slot_orders(T.kmax, f.geo + s, f.L), s, T.u[f.geo + s],
+                           T.rows[f.geo + s], f.nlat, grid[it.x], f.nlong, 0,
+                           T.roww ? T.roww[f.geo + s] : make_double2(1.0, 1.0), x);
             __syncthreads();
         }
     }
@@ -1121,7 +1142,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft_ws(const BField *_
         const FftDev &F = T.plans[f.fft];
         for (int s = it.y; s < it.y + nslots && s < f.P; ++s) {
             anal_fft_tile(F, grid[it.x], f.nlong, T.rows[f.geo + s], f.nlat, T.w[f.geo + s], T.u[f.geo + s], f.P,
-                          f.L, s, gbuf + f.foff, 0, x);
+                          slot_orders(T.kmax, f.geo + s, f.L), s, gbuf + f.foff, 0, x);
             __syncthreads();
         }
     }
@@ -1129,7 +1150,8 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft_ws(const BField *_
 
 // items: (field, m); single-wave workgroups, one launch per ring pairs per
 // lane K.  Slots below T.s0 are skipped (the DH pole slot carries no
-// analysis weight; the adjoint of synthesis reads it).
+// analysis weight; the adjoint of synthesis reads it), and so are the slots
+// below the order's first slot with non-negligible functions.
 template <int K>
 __global__ __launch_bounds__(kWave) void kb_anal_leg(const BField *__restrict__ fields,
                                                      const int2 *__restrict__ items, BTables T,
@@ -1140,7 +1162,8 @@ __global__ __launch_bounds__(kWave) void kb_anal_leg(const BField *__restrict__ 
     const BField f = fields[it.x];
     const int m = it.y;
     anal_leg_tile<K>(T.alpha + tri_offd(m, T.Lt), T.seed[m], T.z + f.geo, gbuf + f.foff + (size_t)m * f.P,
-                     f.L - m, T.s0, f.P, cbuf + f.coff + tri_offd(m, f.L), red);
+                     f.L - m, T.smin ? max(T.s0, T.smin[f.sm + m]) : T.s0, f.P, cbuf + f.coff + tri_offd(m, f.L),
+                     red);
 }
 
 // items: (field, first order, first degree) of a tile of the field's

@@ -66,6 +66,8 @@ struct Geometry {
     double2 *w = nullptr;  // analysis weights (north, south)
     double2 *ones = nullptr;  // unit weights (adjoint of synthesis)
     int2 *rows = nullptr;
+    int *kmax = nullptr;   // per slot: the orders with non-negligible functions (polar skip)
+    int *smin = nullptr;   // per order: the first such slot
     FftDev fft{};
     bool ok = false;
 };
@@ -164,15 +166,19 @@ static bool upload_fft(const FftPlanHost &h, FftDev &d)
     return true;
 }
 
+// lc: the FFT plan's pruning degree (-1: none); L: the grid's degree.
 static bool finish_geometry(Geometry &g, const std::vector<double> &z, const std::vector<double> &u,
-                            const std::vector<double> &w, const std::vector<int> &rows, int lc)
+                            const std::vector<double> &w, const std::vector<int> &rows, int lc, int L)
 {
     g.z = dev_upload(z);
     g.u = dev_upload(u);
     g.w = reinterpret_cast<double2 *>(dev_upload(w));
     g.rows = reinterpret_cast<int2 *>(dev_upload(rows));
     g.ones = reinterpret_cast<double2 *>(dev_upload(std::vector<double>(w.size(), 1.0)));
-    if (!g.z || !g.u || !g.w || !g.rows || !g.ones) return false;
+    const std::vector<int> km = polar_kmax(z, u, L);
+    g.kmax = dev_upload(km);
+    g.smin = dev_upload(polar_smin(km, L));
+    if (!g.z || !g.u || !g.w || !g.rows || !g.ones || !g.kmax || !g.smin) return false;
     FftPlanHost fp = make_fft_plan(g.nlong, lc);
     if (!upload_fft(fp, g.fft)) return false;
     g.ok = true;
@@ -211,7 +217,7 @@ static Geometry *get_dh(int dev, int L, int sampling)
         w[2 * s] = c * aj[s];
         w[2 * s + 1] = (s == 0 || s == n / 2) ? 0.0 : c * aj[n - s];
     }
-    if (!finish_geometry(g, z, u, w, rows, L)) return nullptr;
+    if (!finish_geometry(g, z, u, w, rows, L, L)) return nullptr;
     return &g;
 }
 
@@ -250,7 +256,7 @@ static Geometry *get_glq(int dev, int L, const double *zero, const double *wq)
         w[2 * s] = wq ? 2.0 * pi * wq[s] / (double)g.nlong : 0.0;
         w[2 * s + 1] = (wq && !eq) ? 2.0 * pi * wq[nlat - 1 - s] / (double)g.nlong : 0.0;
     }
-    if (!finish_geometry(g, z, u, w, rows, -1)) return nullptr;
+    if (!finish_geometry(g, z, u, w, rows, -1, L)) return nullptr;
     return &g;
 }
 
@@ -378,6 +384,9 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
 {
     if (!ctab) ctab = rt.s;
     const double *rz = ring.r2 ? ring.z : g.z, *ru = ring.r2 ? ring.u : g.u;
+    // the polar skip holds for the unit-radius functions only (rho^l may
+    // grow what it drops)
+    const int *kmax = ring.r2 ? nullptr : g.kmax, *smin = ring.r2 ? nullptr : g.smin;
     const size_t npack = (size_t)tri_size(Lc);
     const size_t nf = (size_t)(Lc + 1) * g.P;
     const bool lds_fft = fft_in_lds(g.fft.m);
@@ -400,17 +409,17 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
         ProfScope ps(SHTG_K_SYNTH_LEG, st);
         const int nsb = (g.P + kSynthKMax * kWave - 1) / (kSynthKMax * kWave);
         hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nsb), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, rz, cpack,
-                           Lc, g.P, nsb, fbuf, ring.r2);
+                           Lc, g.P, nsb, fbuf, ring.r2, smin);
     }
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         if (lds_fft)
             hipLaunchKernelGGL(k_synth_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, fbuf, g.P, Lc, ru,
-                               g.rows, nrows_out, grid, ld, ext_col, roww);
+                               g.rows, nrows_out, grid, ld, ext_col, roww, kmax);
         else
             hipLaunchKernelGGL(k_synth_fft_ws, dim3(nwg), dim3(kFftMaxThreads), 0, st, g.fft, fbuf, g.P, Lc, ru,
-                               g.rows, nrows_out, grid, ld, ext_col, roww, fws, ws_stride(g.fft.m));
+                               g.rows, nrows_out, grid, ld, ext_col, roww, fws, ws_stride(g.fft.m), kmax);
     }
     if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
     (void)hipFreeAsync(ws, st);
@@ -446,16 +455,16 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         if (lds_fft)
             hipLaunchKernelGGL(k_anal_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, grid, ld, g.u, g.rows,
-                               nrows, w, g.P, Lc, gbuf, ext_col);
+                               nrows, w, g.P, Lc, gbuf, ext_col, (const int *)g.kmax);
         else
             hipLaunchKernelGGL(k_anal_fft_ws, dim3(nwg), dim3(kFftMaxThreads), 0, st, g.fft, grid, ld, g.u, g.rows,
-                               nrows, w, g.P, Lc, gbuf, ext_col, fws, ws_stride(g.fft.m));
+                               nrows, w, g.P, Lc, gbuf, ext_col, fws, ws_stride(g.fft.m), (const int *)g.kmax);
     }
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);
         const AnalLegKernel kern = anal_leg_kernel(anal_lanes_per_slot(g.P - s0));
         hipLaunchKernelGGL(kern, dim3(Lc + 1), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gbuf, Lc, s0,
-                           g.P, cbuf);
+                           g.P, cbuf, (const int *)g.smin);
     }
     ProfScope ps_un(SHTG_K_UNPACK, st);
     const int ntm = (d2 + kTM - 1) / kTM, ntl = (d1 + kTL - 1) / kTL;
@@ -493,9 +502,9 @@ static int launch_synthesis_c(const Geometry &g, const DevRecurrence &rt, const 
         ProfScope ps(SHTG_K_SYNTH_LEG, st);
         const int nsb = (g.P + kSynthKMax * kWave - 1) / (kSynthKMax * kWave);
         hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nsb), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, cp_p,
-                           Lc, g.P, nsb, fb_p, (const double *)nullptr);
+                           Lc, g.P, nsb, fb_p, (const double *)nullptr, (const int *)nullptr);
         hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nsb), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, cp_m,
-                           Lc, g.P, nsb, fb_m, (const double *)nullptr);
+                           Lc, g.P, nsb, fb_m, (const double *)nullptr, (const int *)nullptr);
     }
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
@@ -541,9 +550,9 @@ static int launch_analysis_c(const Geometry &g, const DevRecurrence &rt, const D
         ProfScope ps(SHTG_K_ANAL_LEG, st);
         const AnalLegKernel kern = anal_leg_kernel(anal_lanes_per_slot(g.P - g.s0));
         hipLaunchKernelGGL(kern, dim3(Lc + 1), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gb_p, Lc, g.s0, g.P,
-                           cb_p);
+                           cb_p, (const int *)nullptr);
         hipLaunchKernelGGL(kern, dim3(Lc + 1), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gb_m, Lc, g.s0, g.P,
-                           cb_m);
+                           cb_m, (const int *)nullptr);
     }
     {
         ProfScope ps(SHTG_K_UNPACK, st);
@@ -590,6 +599,7 @@ struct BatchPlan {
     double *z = nullptr, *u = nullptr;
     double2 *w = nullptr, *ones = nullptr;
     int2 *rows = nullptr;
+    int *kmax = nullptr, *smin = nullptr;  // polar skip tables (per slot; per distinct degree and order)
     FftDev *plans = nullptr;
     std::vector<FftDev> hplans;
     double2 *cbuf = nullptr;  // packed coefficients (synthesis) / sums (analysis)
@@ -622,6 +632,8 @@ static void batch_free(BatchPlan *b)
     (void)hipFree(b->w);
     (void)hipFree(b->ones);
     (void)hipFree(b->rows);
+    (void)hipFree(b->kmax);
+    (void)hipFree(b->smin);
     (void)hipFree(b->plans);
     for (auto &p : b->hplans) {
         (void)hipFree((void *)p.tw);
@@ -665,8 +677,10 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
     // ring geometry and FFT plans per distinct degree
     const double pi = std::acos(-1.0);
     std::map<int, std::pair<int, int>> geo_of;  // L -> (geo offset, plan index)
+    std::map<int, int> sm_of;                   // L -> offset of its smin row
+    std::map<int, std::vector<int>> smin_of;    // L -> smin row
     std::vector<double> z, u, w;
-    std::vector<int> rows;
+    std::vector<int> rows, kmax, smin;
     std::vector<size_t> plan_lds;
     for (int i = 0; i < nfields; ++i) {
         const int L = lmax[i];
@@ -685,6 +699,12 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
             w.push_back(c * aj[s]);
             w.push_back((s == 0 || s == n / 2) ? 0.0 : c * aj[n - s]);
         }
+        const std::vector<double> zl(z.begin() + off, z.end()), ul(u.begin() + off, u.end());
+        const std::vector<int> km = polar_kmax(zl, ul, L);
+        kmax.insert(kmax.end(), km.begin(), km.end());
+        smin_of[L] = polar_smin(km, L);
+        sm_of[L] = (int)smin.size();
+        smin.insert(smin.end(), smin_of[L].begin(), smin_of[L].end());
         FftPlanHost fp = make_fft_plan(nlong, L);
         FftDev fd;
         if (!upload_fft(fp, fd)) return ST_OTHER;
@@ -697,8 +717,10 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
     b->w = reinterpret_cast<double2 *>(dev_upload(w));
     b->ones = reinterpret_cast<double2 *>(dev_upload(std::vector<double>(w.size(), 1.0)));
     b->rows = reinterpret_cast<int2 *>(dev_upload(rows));
+    b->kmax = dev_upload(kmax);
+    b->smin = dev_upload(smin);
     b->plans = dev_upload(b->hplans);
-    if (!b->z || !b->u || !b->w || !b->ones || !b->rows || !b->plans) return ST_ALLOC;
+    if (!b->z || !b->u || !b->w || !b->ones || !b->rows || !b->kmax || !b->smin || !b->plans) return ST_ALLOC;
 
     // fields and workspace offsets
     std::vector<BField> fields(nfields);
@@ -714,6 +736,7 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
         f.fft = geo_of[L].second;
         f.coff = coff;
         f.foff = foff;
+        f.sm = sm_of[L];
         coff += tri_size(L);
         foff += (long long)(L + 1) * f.P;
     }
@@ -734,13 +757,16 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
                 wu.push_back({(double)f.L, i, m0, l0});
                 if (l0 + kTL - 1 >= m0) wp.push_back({(double)f.L, i, m0, l0});
             }
+        const std::vector<int> &sm = smin_of[f.L];
         for (int m = 0; m <= f.L; ++m) {
-            // blocks of kSynthKMax*64 slots; the last one packs its lanes tightly
-            for (int s0 = 0; s0 < f.P; s0 += kSynthKMax * kWave) {
+            // blocks of kSynthKMax*64 slots from the order's first slot with
+            // non-negligible functions; the last one packs its lanes tightly
+            for (int s0 = sm[m]; s0 < f.P; s0 += kSynthKMax * kWave) {
                 const int kk = std::min(kSynthKMax, (f.P - s0 + kWave - 1) / kWave);
                 ws.push_back({(double)(f.L - m + 1) * std::min(kk * kWave, f.P - s0), i, m, s0 + (kk << 24)});
             }
-            wa.push_back({(double)(f.L - m + 1) * f.P, i, m, anal_lanes_per_slot(f.P - 1)});
+            const int sa = std::max(1, sm[m]);
+            wa.push_back({(double)(f.L - m + 1) * (f.P - sa), i, m, anal_lanes_per_slot(f.P - sa)});
         }
     }
     auto by_work = [](const W &x, const W &y) { return x.w > y.w; };
@@ -819,6 +845,8 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
     b->T.s0 = 1;
     b->T.rows = b->rows;
     b->T.plans = b->plans;
+    b->T.kmax = b->kmax;
+    b->T.smin = b->smin;
     for (auto &v : b->bound) v.assign(nfields, nullptr);
     *out = b.release();
     return ST_OK;
@@ -1532,6 +1560,22 @@ int shtg_shmultiply(double *cilmout, int out_d1, int out_d2, const double *cilm1
     if (!st) st = launch_analysis(*t.g, *t.rt, *t.sd, g1, nlong, lmaxout, cilmout, out_d1, out_d2, hs);
     (void)hipFreeAsync(ws, hs);
     return st;
+}
+
+int shtg_dh_polar_orders(int lmax, int *kmax)
+{
+    if (lmax < 0) return ST_BOUNDS;
+    if (!kmax) return ST_DIMS;
+    const int n = 2 * lmax + 2, P = n / 2 + 1;
+    const double pi = std::acos(-1.0);
+    std::vector<double> z(P), u(P);
+    for (int s = 0; s < P; ++s) {
+        z[s] = (s == n / 2) ? 0.0 : std::cos(pi * (double)s / (double)n);
+        u[s] = (s == n / 2) ? 1.0 : std::sqrt((1.0 - z[s]) * (1.0 + z[s]));
+    }
+    const std::vector<int> km = polar_kmax(z, u, lmax);
+    for (int s = 0; s < P; ++s) kmax[s] = km[s];
+    return ST_OK;
 }
 
 int shtg_shglq(int lmax, double *zero, double *w)

@@ -12,6 +12,7 @@
 // the normalisation epilogue after analysis.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -213,6 +214,89 @@ inline int glq_nodes(int n, double *zero, double *w)
         w[n - i] = w[i - 1];
     }
     return 0;
+}
+
+// ------------------------------------------------------------ polar skip
+//
+// Near the poles the associated Legendre functions of high order are
+// exponentially small at every degree: for a ring with sin(theta) = u, the
+// 4pi-normalised P_lm(cos theta), m <= l <= L, stay below kPolarEps when m is
+// well above L u.  Such (order, ring) pairs contribute nothing representable
+// to a field or to its coefficients (below 1e-32 of the coefficients'
+// magnitudes, times at most the number of degrees), so the transforms skip
+// them: the Legendre kernels do not run their recursion, the ring FFTs treat
+// their Fourier coefficients as zero.  The reference sums these terms; the
+// results differ from it far below any rounding the transforms incur.
+constexpr double kPolarEps = 1e-32;
+
+// kmax[s]: the largest order m <= L whose functions reach kPolarEps at ring
+// slot s for some degree l <= L (0 at the poles); non-decreasing in s for
+// slots ordered from the pole to the equator (|z| non-increasing), which
+// the caller guarantees.  A staircase walk over (s, m) with one recursion per
+// step: O((P + L) L) operations.  Without that ordering every slot keeps all
+// orders.
+inline std::vector<int> polar_kmax(const std::vector<double> &z, const std::vector<double> &u, int L)
+{
+    const int P = (int)z.size();
+    std::vector<int> km(P, L);
+    for (int s = 1; s < P; ++s)
+        if (std::fabs(z[s]) > std::fabs(z[s - 1])) return km;
+    // log of the sectoral value without u^m: P_mm = c_m u^m, c_1 = sqrt(3),
+    // c_m = c_m-1 sqrt((2m+1)/(2m))
+    std::vector<double> lc(L + 1, 0.0);
+    if (L >= 1) lc[1] = 0.5 * std::log(3.0);
+    for (int m = 2; m <= L; ++m) lc[m] = lc[m - 1] + 0.5 * std::log((2.0 * m + 1.0) / (2.0 * m));
+    const double leps = std::log(kPolarEps);
+    // does some P_lm(z_s), m <= l <= L, reach kPolarEps?
+    auto reaches = [&](int s, int m) {
+        if (m == 0) return true;
+        const double zz = z[s], uu = u[s];
+        if (uu <= 0.0) return false;
+        double off = lc[m] + m * std::log(uu);  // P = p exp(off)
+        if (off >= leps) return true;
+        // |p| >= thr reaches; recomputed from off (not rescaled) so that a
+        // threshold beyond the double range does not stick at infinity
+        auto threshold = [&]() { return leps - off > 700.0 ? HUGE_VAL : std::exp(leps - off); };
+        double thr = threshold();
+        double p2 = 0.0, p1 = 1.0;
+        for (int l = m + 1; l <= L; ++l) {
+            const double a = std::sqrt((2.0 * l + 1.0) * (2.0 * l - 1.0) / ((double)(l - m) * (double)(l + m)));
+            const double b = (l > m + 1) ? std::sqrt((2.0 * l + 1.0) * (double)(l - m - 1) * (double)(l + m - 1) /
+                                                     ((2.0 * l - 3.0) * (double)(l - m) * (double)(l + m)))
+                                         : 0.0;
+            const double p = a * zz * p1 - b * p2;
+            p2 = p1;
+            p1 = p;
+            if (std::fabs(p1) >= thr) return true;
+            if (std::fabs(p1) > 1e150) {
+                p1 *= 1e-150;
+                p2 *= 1e-150;
+                off += std::log(1e150);
+                thr = threshold();
+            }
+        }
+        return false;
+    };
+    int m = 0;
+    for (int s = 0; s < P; ++s) {
+        while (m < L && reaches(s, m + 1)) ++m;
+        km[s] = m;
+    }
+    return km;
+}
+
+// smin[m]: the first slot whose kmax reaches m (kmax non-decreasing), at most
+// P - 1 so that every order keeps a slot.
+inline std::vector<int> polar_smin(const std::vector<int> &kmax, int L)
+{
+    const int P = (int)kmax.size();
+    std::vector<int> sm(L + 1, P - 1);
+    int s = 0;
+    for (int m = 0; m <= L; ++m) {
+        while (s < P && kmax[s] < m) ++s;
+        sm[m] = std::min(s, P - 1);
+    }
+    return sm;
 }
 
 }  // namespace shtg

@@ -117,3 +117,31 @@ def test_normal_gravity_matches_oracle():
     for lat in (-90.0, -45.5, 0.0, 12.25, 60.0, 90.0):
         assert abs(B.NormalGravity(lat, GM, w, a, b) - oracle.NormalGravity(lat, GM, w, a, b)) <= 1e-14 * 9.8
     assert B.NormalGravity(10.0, GM, 0.0, a, a) == GM / a ** 2
+
+
+@pytest.mark.parametrize("L", [16, 200, 700, 2799])
+def test_polar_skip_is_exact_and_tight(L):
+    """The DH polar skip (host table, no GPU): at every ring slot, every
+    4pi-normalised P_lm with m above kmax stays below 1e-32 for all l <= L
+    (the oracle's P_lm, pinned to 60 digits near the poles at high degree in
+    test_oracle.py), and the order kmax itself reaches it (nothing more is
+    dropped than the bound allows)."""
+    P = L + 2
+    km = (ctypes.c_int * P)()
+    assert _lib.lib().shtg_dh_polar_orders(L, km) == 0
+    km = np.array(km[:])
+    assert km[0] == 0 and km[-1] == L and np.all(np.diff(km) >= 0)
+    n = 2 * L + 2
+    step = max(1, P // (60 if L < 1000 else 15))
+    for s in list(range(0, P, step)) + [P - 1]:
+        z = 0.0 if s == n // 2 else np.cos(np.pi * s / n)
+        p = oracle.plm(L, z)  # [l, m]
+        mx = np.abs(p).max(axis=0)      # max over l per order m
+        if km[s] < L:
+            assert mx[km[s] + 1:].max() < 1e-32, (s, km[s])
+        assert mx[km[s]] >= 1e-32, (s, km[s])
+    # the skip is substantial at high degree: the recursion work it removes
+    if L >= 700:
+        work = sum((L - m + 1) for s in range(P) for m in range(km[s] + 1, L + 1))
+        total = P * (L + 1) * (L + 2) / 2
+        assert work / total > 0.1

@@ -306,8 +306,10 @@ __device__ __forceinline__ int slot_orders(const int *__restrict__ kmax, int s, 
     return kmax ? min(Lc, kmax[s]) : Lc;
 }
 
-// Synthesis inputs of one thread: ring sums of orders 2*threadIdx.x and
-// 2*threadIdx.x + 1 (orders beyond 2 * blockDim.x are read in the fill).
+// Synthesis inputs of one thread: ring sums of orders threadIdx.x + j *
+// blockDim.x, j < kSynthPre (later orders are read in the fill).  Orders
+// interleave over the threads so that the fill's LDS writes of a wave touch
+// consecutive elements.
 constexpr int kSynthPre = 2;
 struct SynthRing {
     double4 f[kSynthPre];
@@ -316,10 +318,9 @@ struct SynthRing {
 __device__ __forceinline__ void synth_fft_load(const double4 *__restrict__ fbuf, int P, int Lc, int s,
                                                SynthRing &in)
 {
-    const int k0 = kSynthPre * threadIdx.x;
 #pragma unroll
     for (int j = 0; j < kSynthPre; ++j) {
-        const int k = k0 + j;
+        const int k = threadIdx.x + j * blockDim.x;
         in.f[j] = (k <= Lc) ? fbuf[(size_t)k * P + s] : make_double4(0.0, 0.0, 0.0, 0.0);
     }
 }
@@ -336,8 +337,8 @@ __device__ __forceinline__ void synth_fft_fill(const FftDev &F, const SynthRing 
     const int top = (F.lc >= 0) ? m : n;
     for (int q = threadIdx.x; q < m; q += blockDim.x)
         if ((q > Lc && q < top - Lc) || q >= top) lds[fpad(q)] = make_double2(0.0, 0.0);
-    // each thread takes kSynthPre consecutive orders: one extended-range
-    // power, then products with u (as the reference's iterated rescalem)
+    // order k carries the extended-range power 1e280 u^k (the reference's
+    // iterated rescalem)
     auto put = [&](int k, double r, double4 f) {
         // Z_k = X_k + i Y_k with X (north) and Y (south) Hermitian:
         // X_0 = C_N, X_k = (C_N - i S_N)/2 and X_{n-k} = conj(X_k); stored
@@ -355,28 +356,13 @@ __device__ __forceinline__ void synth_fft_fill(const FftDev &F, const SynthRing 
                 fft_prep_in_c(F, make_double2(0.5 * (f.x - f.w), -0.5 * (f.y + f.z)), fft_chirp_mirror(F, c));
         }
     };
-    {
-        const int k0 = kSynthPre * threadIdx.x;
-        double r = 1.0;
 #pragma unroll
-        for (int j = 0; j < kSynthPre; ++j) {
-            const int k = k0 + j;
-            if (k == 0) r = 1.0;
-            else if (j == 0 || k == 1) r = scaled_upow(u, k);
-            else r *= u;
-            if (k <= Lc) put(k, r, in.f[j]);
-        }
+    for (int j = 0; j < kSynthPre; ++j) {
+        const int k = threadIdx.x + j * blockDim.x;
+        if (k <= Lc) put(k, k == 0 ? 1.0 : scaled_upow(u, k), in.f[j]);
     }
-    for (int k0 = kSynthPre * (threadIdx.x + blockDim.x); k0 <= Lc; k0 += kSynthPre * blockDim.x) {
-        double r = 1.0;
-#pragma unroll
-        for (int j = 0; j < kSynthPre; ++j) {
-            const int k = k0 + j;
-            if (j == 0) r = scaled_upow(u, k);
-            else r *= u;
-            if (k <= Lc) put(k, r, fbuf[(size_t)k * P + s]);
-        }
-    }
+    for (int k = threadIdx.x + kSynthPre * blockDim.x; k <= Lc; k += blockDim.x)
+        put(k, scaled_upow(u, k), fbuf[(size_t)k * P + s]);
 }
 
 // DFT of the filled array and the rows out; the north and south rows are
@@ -495,25 +481,17 @@ __device__ __forceinline__ void anal_fft_run(const FftDev &F, int2 rw, int nrows
     const bool south = rw.y >= 0 && rw.y < nrows;
     fft_forward(lds, F, F.bhat2);
     const double wn = north ? w.x : 0.0, ws = south ? w.y : 0.0;
-    for (int k0 = 4 * threadIdx.x; k0 <= Lc; k0 += 4 * blockDim.x) {
-        double r = 1.0;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int mm = k0 + j;
-            if (mm > Lc) break;
-            if (mm == 0) r = 1.0;
-            else if (j == 0 || mm == 1) r = scaled_upow(u, mm);
-            else r *= u;
-            const double2 c = fft_chirp(F, mm);
-            const double2 zm = fft_get_c(lds, F, mm, mm, c);
-            const int nm = mm == 0 ? 0 : n - mm;
-            const double2 zn = mm == 0 ? zm : fft_get_c(lds, F, nm, fft_pos(F, nm), fft_chirp_mirror(F, c));
-            const double xr = 0.5 * (zm.x + zn.x), xi = 0.5 * (zm.y - zn.y);
-            const double yr = 0.5 * (zm.y + zn.y), yi = 0.5 * (zn.x - zm.x);
-            const double f1c = wn * xr, f1s = -wn * xi, f2c = ws * yr, f2s = -ws * yi;
-            gbuf[(size_t)mm * P + s] =
-                make_double4((f1c + f2c) * r, (f1s + f2s) * r, (f1c - f2c) * r, (f1s - f2s) * r);
-        }
+    // orders interleave over the threads (a wave reads consecutive elements)
+    for (int mm = threadIdx.x; mm <= Lc; mm += blockDim.x) {
+        const double r = (mm == 0) ? 1.0 : scaled_upow(u, mm);
+        const double2 c = fft_chirp(F, mm);
+        const double2 zm = fft_get_c(lds, F, mm, mm, c);
+        const int nm = mm == 0 ? 0 : n - mm;
+        const double2 zn = mm == 0 ? zm : fft_get_c(lds, F, nm, fft_pos(F, nm), fft_chirp_mirror(F, c));
+        const double xr = 0.5 * (zm.x + zn.x), xi = 0.5 * (zm.y - zn.y);
+        const double yr = 0.5 * (zm.y + zn.y), yi = 0.5 * (zn.x - zm.x);
+        const double f1c = wn * xr, f1s = -wn * xi, f2c = ws * yr, f2s = -ws * yi;
+        gbuf[(size_t)mm * P + s] = make_double4((f1c + f2c) * r, (f1s + f2s) * r, (f1c - f2c) * r, (f1s - f2s) * r);
     }
 }
 

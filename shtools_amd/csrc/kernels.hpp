@@ -136,8 +136,10 @@ __device__ __forceinline__ void synth_leg_tile(const double *__restrict__ al, do
         q2[k] = kRho ? -sd / r2[k] : -sd;  // the first step then yields q_m = seed
         eC[k] = eS[k] = oC[k] = oS[k] = 0.0;
     }
+    // explicit fused multiply-adds, so that every kernel built from this
+    // tile rounds alike whatever the compiler would contract
     auto step = [&](double a, double z_, double qa, double qb, double r2_) {
-        return kRho ? (a * z_) * qa - r2_ * qb : (a * z_) * qa - qb;
+        return kRho ? fma(a * z_, qa, -(r2_ * qb)) : fma(a * z_, qa, -qb);
     };
     int j = 0;
 #pragma unroll 2
@@ -147,11 +149,11 @@ __device__ __forceinline__ void synth_leg_tile(const double *__restrict__ al, do
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const double q = step(a0, z[k], q1[k], q2[k], r2[k]);
-            eC[k] += q * c0.x;
-            eS[k] += q * c0.y;
+            eC[k] = fma(q, c0.x, eC[k]);
+            eS[k] = fma(q, c0.y, eS[k]);
             const double q3 = step(a1, z[k], q, q1[k], r2[k]);
-            oC[k] += q3 * c1.x;
-            oS[k] += q3 * c1.y;
+            oC[k] = fma(q3, c1.x, oC[k]);
+            oS[k] = fma(q3, c1.y, oS[k]);
             q2[k] = q;
             q1[k] = q3;
         }
@@ -162,8 +164,8 @@ __device__ __forceinline__ void synth_leg_tile(const double *__restrict__ al, do
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const double q = step(a0, z[k], q1[k], q2[k], r2[k]);
-            eC[k] += q * c0.x;
-            eS[k] += q * c0.y;
+            eC[k] = fma(q, c0.x, eC[k]);
+            eS[k] = fma(q, c0.y, eS[k]);
         }
     }
 #pragma unroll
@@ -251,7 +253,8 @@ __device__ __forceinline__ void anal_leg_tile(const double *__restrict__ al, dou
                 const double a = al[j0 + jj];
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    const double q = (a * z[k]) * q1[k] - q2[k];
+                    // explicit fused multiply-adds, as in the synthesis tile
+                    const double q = fma(a * z[k], q1[k], -q2[k]);
                     q2[k] = q1[k];
                     q1[k] = q;
                     const double ec = (jj % 2 == 0) ? eC[k] : oC[k];
@@ -260,8 +263,8 @@ __device__ __forceinline__ void anal_leg_tile(const double *__restrict__ al, dou
                         acc[jj][0] = q * ec;
                         acc[jj][1] = q * es;
                     } else {
-                        acc[jj][0] += q * ec;
-                        acc[jj][1] += q * es;
+                        acc[jj][0] = fma(q, ec, acc[jj][0]);
+                        acc[jj][1] = fma(q, es, acc[jj][1]);
                     }
                 }
             }
@@ -277,11 +280,11 @@ __device__ __forceinline__ void anal_leg_tile(const double *__restrict__ al, dou
                     const double a = al[j0 + jj];
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
-                        const double q = (a * z[k]) * q1[k] - q2[k];
+                        const double q = fma(a * z[k], q1[k], -q2[k]);
                         q2[k] = q1[k];
                         q1[k] = q;
-                        acc[jj][0] += q * ((jj % 2 == 0) ? eC[k] : oC[k]);
-                        acc[jj][1] += q * ((jj % 2 == 0) ? eS[k] : oS[k]);
+                        acc[jj][0] = fma(q, (jj % 2 == 0) ? eC[k] : oC[k], acc[jj][0]);
+                        acc[jj][1] = fma(q, (jj % 2 == 0) ? eS[k] : oS[k], acc[jj][1]);
                     }
                 }
             }

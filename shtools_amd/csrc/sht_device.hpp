@@ -314,22 +314,23 @@ __device__ __forceinline__ void fft_run_turn(double2 *x, const FftDev &P, const 
 // In-place forward DFT of the values stored in x (x has P.m slots; for
 // Bluestein the caller has zeroed the unused slots).  bhat selects the chirp
 // spectrum (P.bhat for synthesis, P.bhat2 for analysis).  Ends with a barrier.
+//
+// One call site per stage direction (the last DIF stage of a plain plan runs
+// in the same loop as the others), which keeps the kernels' code and
+// register use down.
 __device__ __forceinline__ void fft_forward(double2 *x, const FftDev &P, const double2 *__restrict__ bhat)
 {
     const int last = P.nst - 1;
-    for (int st = 0; st < last; ++st) {
+    const int ndif = P.blue ? last : last + 1;  // Bluestein: the last stage is part of the turn
+    for (int st = 0; st < ndif; ++st) {
         fft_run_stage<false>(x, P, st);
         __syncthreads();
     }
-    if (P.blue) {
-        fft_run_turn(x, P, bhat);
-        __syncthreads();
-        for (int st = last - 1; st >= 0; --st) {
-            fft_run_stage<true>(x, P, st);
-            __syncthreads();
-        }
-    } else if (last >= 0) {
-        fft_run_stage<false>(x, P, last);
+    if (!P.blue) return;
+    fft_run_turn(x, P, bhat);
+    __syncthreads();
+    for (int st = last - 1; st >= 0; --st) {
+        fft_run_stage<true>(x, P, st);
         __syncthreads();
     }
 }

@@ -604,8 +604,16 @@ struct BatchPlan {
     std::vector<FftDev> hplans;
     double2 *cbuf = nullptr;  // packed coefficients (synthesis) / sums (analysis)
     double4 *fbuf = nullptr;  // per-order ring sums
-    const double **ptrs = nullptr;  // device: [4][nf] bound field pointers
-    std::vector<const void *> bound[4];
+    // Field pointers bound at slots 0..3 (synthesis in, synthesis out,
+    // analysis in, analysis out): a device table per slot and its host copy.
+    // A table a graph capture used is pinned: rebinding the slot afterwards
+    // uploads to a new table, so replays of that graph keep their pointers.
+    struct PtrSlot {
+        const void **dev = nullptr;
+        std::vector<const void *> host;
+        bool pinned = false;
+    } slot[4];
+    std::vector<void *> old_tables;  // pinned tables replaced by a rebind
     BTables T{};
 };
 
@@ -645,7 +653,8 @@ static void batch_free(BatchPlan *b)
     }
     (void)hipFree(b->cbuf);
     (void)hipFree(b->fbuf);
-    (void)hipFree(b->ptrs);
+    for (auto &t : b->slot) (void)hipFree((void *)t.dev);
+    for (auto p : b->old_tables) (void)hipFree(p);
     delete b;
 }
 
@@ -744,8 +753,6 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
     if (!b->fields) return ST_ALLOC;
     if (!HIP_OK(hipMalloc(&b->cbuf, (size_t)coff * sizeof(double2)))) return ST_ALLOC;
     if (!HIP_OK(hipMalloc(&b->fbuf, (size_t)foff * sizeof(double4)))) return ST_ALLOC;
-    if (!HIP_OK(hipMalloc(&b->ptrs, 4 * (size_t)nfields * sizeof(void *)))) return ST_ALLOC;
-    (void)hipMemset(b->ptrs, 0, 4 * (size_t)nfields * sizeof(void *));
 
     // work items, longest first
     struct W { double w; int a, b, c; };
@@ -847,24 +854,40 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
     b->T.plans = b->plans;
     b->T.kmax = b->kmax;
     b->T.smin = b->smin;
-    for (auto &v : b->bound) v.assign(nfields, nullptr);
     *out = b.release();
     return ST_OK;
 }
 
-// Upload field pointers of slot k (0: synthesis in, 1: synthesis out,
-// 2: analysis in, 3: analysis out) when they differ from the bound ones.
-static int batch_bind(BatchPlan *b, int k, const void *const *p)
+static bool capturing(hipStream_t st)
 {
-    bool same = true;
-    for (int i = 0; i < b->nf; ++i) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return HIP_OK(hipStreamIsCapturing(st, &cs)) && cs != hipStreamCaptureStatusNone;
+}
+
+// Bind field pointers p[0..nf) to slot k (0: synthesis in, 1: synthesis
+// out, 2: analysis in, 3: analysis out) for launches on stream st.  A new
+// list is uploaded (synchronously, so it cannot be bound under graph
+// capture: capture after an eager call with the same arrays); a list bound
+// under capture pins its table.
+static int batch_bind(BatchPlan *b, int k, const void *const *p, hipStream_t st)
+{
+    for (int i = 0; i < b->nf; ++i)
         if (!p[i]) return ST_DIMS;
-        if (b->bound[k][i] != p[i]) same = false;
+    BatchPlan::PtrSlot &t = b->slot[k];
+    const bool cap = capturing(st);
+    if (t.dev && std::equal(p, p + b->nf, t.host.begin())) {
+        if (cap) t.pinned = true;
+        return ST_OK;
     }
-    if (same) return ST_OK;
-    if (!HIP_OK(hipMemcpy(b->ptrs + (size_t)k * b->nf, p, sizeof(void *) * b->nf, hipMemcpyHostToDevice)))
-        return ST_OTHER;
-    for (int i = 0; i < b->nf; ++i) b->bound[k][i] = p[i];
+    if (cap) return ST_OTHER;
+    if (!t.dev || t.pinned) {
+        if (t.dev) b->old_tables.push_back((void *)t.dev);
+        t.dev = nullptr;
+        t.pinned = false;
+        if (!HIP_OK(hipMalloc((void **)&t.dev, sizeof(void *) * b->nf))) return ST_ALLOC;
+    }
+    if (!HIP_OK(hipMemcpy((void *)t.dev, p, sizeof(void *) * b->nf, hipMemcpyHostToDevice))) return ST_OTHER;
+    t.host.assign(p, p + b->nf);
     return ST_OK;
 }
 
@@ -875,8 +898,8 @@ static int xcd_blocks(int n) { return ((n + 31) / 32) * 32; }
 // scales the coefficients by post and the rows by the analysis weights).
 static int batch_synthesis(BatchPlan *b, hipStream_t st, const BTables &T)
 {
-    const double *const *cin = b->ptrs;
-    double *const *gout = (double *const *)(b->ptrs + b->nf);
+    const double *const *cin = (const double *const *)b->slot[0].dev;
+    double *const *gout = (double *const *)b->slot[1].dev;
     {
         ProfScope ps(SHTG_K_PACK, st);
         hipLaunchKernelGGL(kb_pack, dim3(b->n_pack), dim3(kThreads), 0, st, b->fields, b->items_pack, T, cin,
@@ -904,8 +927,8 @@ static int batch_synthesis(BatchPlan *b, hipStream_t st, const BTables &T)
 // uses unit weights, every slot and the recursion's s as epilogue).
 static int batch_analysis(BatchPlan *b, hipStream_t st, const BTables &T)
 {
-    const double *const *gin = b->ptrs + 2 * (size_t)b->nf;
-    double *const *cout = (double *const *)(b->ptrs + 3 * (size_t)b->nf);
+    const double *const *gin = (const double *const *)b->slot[2].dev;
+    double *const *cout = (double *const *)b->slot[3].dev;
     {
         ProfScope ps(SHTG_K_ANAL_FFT, st);
         for (auto &g : b->fft)
@@ -940,10 +963,9 @@ static int batch_analysis(BatchPlan *b, hipStream_t st, const BTables &T)
 // and the plan's workspace; the caller reruns its transforms afterwards.
 static int batch_tune(BatchPlan *b, hipStream_t st, int reps)
 {
-    const double *const *gin = b->ptrs + 2 * (size_t)b->nf;
-    double *const *gout = (double *const *)(b->ptrs + b->nf);
-    for (int i = 0; i < b->nf; ++i)
-        if (!b->bound[1][i] || !b->bound[2][i]) return ST_OTHER;  // needs a prior synthesis and analysis
+    const double *const *gin = (const double *const *)b->slot[2].dev;
+    double *const *gout = (double *const *)b->slot[1].dev;
+    if (!gin || !gout) return ST_OTHER;  // needs a prior synthesis and analysis
     hipEvent_t e0, e1;
     if (!HIP_OK(hipEventCreate(&e0)) || !HIP_OK(hipEventCreate(&e1))) return ST_OTHER;
     int st_out = ST_OK;
@@ -1684,8 +1706,8 @@ int shtg_batch_makegrid_dh(shtg_batch_plan *plan, const double *const *cilm, dou
     if (!plan || !cilm || !grid) return ST_DIMS;
     std::lock_guard<std::mutex> lk(g_mu);
     BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
-    int st = batch_bind(b, 0, (const void *const *)cilm);
-    if (!st) st = batch_bind(b, 1, (const void *const *)grid);
+    int st = batch_bind(b, 0, (const void *const *)cilm, (hipStream_t)stream);
+    if (!st) st = batch_bind(b, 1, (const void *const *)grid, (hipStream_t)stream);
     if (st) return st;
     return batch_synthesis(b, (hipStream_t)stream, b->T);
 }
@@ -1695,8 +1717,8 @@ int shtg_batch_shexpand_dh(shtg_batch_plan *plan, const double *const *grid, dou
     if (!plan || !cilm || !grid) return ST_DIMS;
     std::lock_guard<std::mutex> lk(g_mu);
     BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
-    int st = batch_bind(b, 2, (const void *const *)grid);
-    if (!st) st = batch_bind(b, 3, (const void *const *)cilm);
+    int st = batch_bind(b, 2, (const void *const *)grid, (hipStream_t)stream);
+    if (!st) st = batch_bind(b, 3, (const void *const *)cilm, (hipStream_t)stream);
     if (st) return st;
     return batch_analysis(b, (hipStream_t)stream, b->T);
 }
@@ -1707,8 +1729,8 @@ int shtg_batch_makegrid_dh_adjoint(shtg_batch_plan *plan, const double *const *g
     if (!plan || !cilm_bar || !grid_bar) return ST_DIMS;
     std::lock_guard<std::mutex> lk(g_mu);
     BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
-    int st = batch_bind(b, 2, (const void *const *)grid_bar);
-    if (!st) st = batch_bind(b, 3, (const void *const *)cilm_bar);
+    int st = batch_bind(b, 2, (const void *const *)grid_bar, (hipStream_t)stream);
+    if (!st) st = batch_bind(b, 3, (const void *const *)cilm_bar, (hipStream_t)stream);
     if (st) return st;
     BTables T = b->T;
     T.w = b->ones;
@@ -1723,8 +1745,8 @@ int shtg_batch_shexpand_dh_adjoint(shtg_batch_plan *plan, const double *const *c
     if (!plan || !cilm_bar || !grid_bar) return ST_DIMS;
     std::lock_guard<std::mutex> lk(g_mu);
     BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
-    int st = batch_bind(b, 0, (const void *const *)cilm_bar);
-    if (!st) st = batch_bind(b, 1, (const void *const *)grid_bar);
+    int st = batch_bind(b, 0, (const void *const *)cilm_bar, (hipStream_t)stream);
+    if (!st) st = batch_bind(b, 1, (const void *const *)grid_bar, (hipStream_t)stream);
     if (st) return st;
     BTables T = b->T;
     T.s = T.post;

@@ -56,3 +56,38 @@ def test_batch_repeated_calls_and_rebinding(torch):
         torch.cuda.synchronize()
         for c, o in zip(cs_host, back):
             assert coef_err(o.cpu().numpy(), c) <= 1e-12
+
+
+@pytest.mark.parametrize("norm,cs,sampling", [(1, 1, 2), (4, -1, 1)])
+def test_repeated_degrees_scales_and_zero_field_match_plain(torch, norm, cs, sampling):
+    """Many fields of equal degree (which share the plan's ring geometry and
+    FFT plan), fields differing in magnitude by many orders and an all-zero
+    field: every field equals the plain path bitwise, forward and adjoint."""
+    from shtools_amd import backend as B
+    from shtools_amd.batch import DHBatch
+    print(f"case repeated degrees norm={norm} cs={cs} s={sampling}", flush=True)
+    Ls = [33] * 2 + [70] * 3 + [5] * 4 + [129] * 5 + [16] * 8 + [48, 201, 201]
+    rng = np.random.default_rng(1234)
+    cs_host = [rand_cilm(rng, L) * 10.0 ** rng.integers(-8, 8) for L in Ls]
+    cs_host[3][...] = 0.0
+    cilm = [torch.from_numpy(c).cuda() for c in cs_host]
+    b = DHBatch(Ls, sampling=sampling, norm=norm, csphase=cs)
+    grids = b.makegrid(cilm)
+    back = b.expand(grids)
+    gbar = [torch.from_numpy(rng.standard_normal(tuple(g.shape))).cuda() for g in grids]
+    adj_s = b.makegrid_adjoint(gbar)
+    adj_a = b.expand_adjoint(cilm)
+    torch.cuda.synchronize()
+    for i, L in enumerate(Ls):
+        g_plain = B.MakeGridDH(cilm[i], sampling=sampling, norm=norm, csphase=cs)
+        c_plain = B.SHExpandDH(grids[i], sampling=sampling, norm=norm, csphase=cs)
+        s_plain = B.MakeGridDH_adjoint(gbar[i], tuple(cilm[i].shape), sampling=sampling, norm=norm, csphase=cs)
+        a_plain = B.SHExpandDH_adjoint(cilm[i], tuple(grids[i].shape), sampling=sampling, norm=norm, csphase=cs)
+        torch.cuda.synchronize()
+        assert torch.equal(g_plain, grids[i]), (i, L)
+        assert torch.equal(c_plain, back[i]), (i, L)
+        assert torch.equal(s_plain, adj_s[i]), (i, L)
+        assert torch.equal(a_plain, adj_a[i]), (i, L)
+    assert float(back[3].abs().max()) == 0.0
+    for i in (0, 5, 9, 22):
+        assert coef_err(back[i].cpu().numpy(), cs_host[i]) <= 1e-11

@@ -60,3 +60,38 @@ def test_batch_tune_keeps_results():
     g2 = batch.makegrid(cilms)
     for i in range(len(Ls)):
         assert torch.equal(g2[i], ref_g[i])
+
+
+def test_graph_keeps_its_arrays_after_rebinding():
+    """A captured step keeps reading and writing the arrays it was captured
+    with after the plan has been called eagerly on other arrays."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from shtools_amd.batch import DHBatch
+    rng = np.random.default_rng(23)
+    Ls = [40, 7, 200, 129, 64, 300]
+    batch = DHBatch(Ls, sampling=2, norm=1, csphase=1)
+    cilms = [torch.from_numpy(rand_cilm(rng, L)).cuda() for L in Ls]
+    other = [torch.from_numpy(rand_cilm(rng, L)).cuda() for L in Ls]
+    ref_g = [g.clone() for g in batch.makegrid(cilms)]
+    ref_c = [c.clone() for c in batch.expand(ref_g)]
+    grids = [torch.empty_like(g) for g in ref_g]
+    outs = [torch.empty_like(c) for c in ref_c]
+    graph = batch.capture(cilms, grids, outs)
+    for rep in range(3):
+        # eager calls on other arrays rebind every slot of the plan
+        og = batch.makegrid(other)
+        oc = batch.expand(og)
+        torch.cuda.synchronize()
+        for g, c in zip(grids, outs):
+            g.fill_(np.nan)
+            c.fill_(np.nan)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        for i in range(len(Ls)):
+            assert torch.equal(grids[i], ref_g[i]), (rep, i)
+            assert torch.equal(outs[i], ref_c[i]), (rep, i)
+            assert not torch.equal(og[i], ref_g[i])
+        del og, oc

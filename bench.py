@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of HIP-graph replays")
     ap.add_argument("--no-tune", action="store_true", help="skip the ring-FFT launch-shape autotuner")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the timings of the other workloads (GLQ 1023, DH1 2047, 16 fits, high degree)")
     ap.add_argument("--counter-pass", action="store_true",
                     help="(internal) one untimed step then a calibration copy, for a rocprofv3 --pmc run")
     return ap.parse_args()
@@ -149,6 +151,115 @@ def measure_traffic(nfields):
     except Exception as e:  # reported in the JSON line, never fatal
         return None, str(e)
     return res, None
+
+
+def _timed(fn, reps):
+    """Median wall time (ms) of reps calls of fn, each bracketed by device
+    synchronisations, after one untimed call."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append((time.perf_counter() - t0) * 1e3)
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def other_configs(dev, reps=5):
+    """The other workloads of the transform path, each timed on the GPU with
+    inputs resident (median of reps), priced by the same work model, with an
+    accuracy check of its own:
+      glq_L1023: MakeGridGLQ + SHExpandGLQ, lmax 1023, orthonormal, 1024 x 2047;
+      dh1_L2047: MakeGridDH + SHExpandDH, lmax 2047, sampling 1, orthonormal,
+                 4096 x 4096;
+      lsq_16fits_L511: one gradient of the 16-fit least-squares loss (DH2
+                 1024 x 2048, batched synthesis + batched adjoint);
+      high_degree_8fields: 8 fields lmax ~ U{2047..2799}, DH2, upward-
+                 continuation filter, batched synthesis + analysis."""
+    import numpy as np
+    import torch
+    from shtools_amd import backend as B
+    from shtools_amd.batch import DHBatch
+    from shtools_amd.synthetic import (dh_flops, high_degree_workload, lsq_fit_workload, power_law_cilm,
+                                       transform_flops, upward_continuation)
+    out = {}
+
+    def rec(name, ms, flops, extra):
+        out[name] = dict(ms=ms, model_tflops=flops / (ms * 1e-3) / 1e12,
+                         frac_fp64_peak=flops / (ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, **extra)
+        print(f"[bench] {name}: {ms:.2f} ms, {out[name]['model_tflops']:.1f} model TFLOP/s", file=sys.stderr,
+              flush=True)
+
+    rng = np.random.default_rng(11)
+    # Gauss-Legendre, lmax 1023, orthonormal
+    L = 1023
+    zero, w = B.SHGLQ(L)
+    c = torch.from_numpy(power_law_cilm(rng, L, -2.0)).to(dev)
+    state = {}
+
+    def glq():
+        state["g"] = B.MakeGridGLQ(c, zero, norm=4)
+        state["c"] = B.SHExpandGLQ(state["g"], w, zero, norm=4)
+    ms = _timed(glq, reps)
+    err = float(torch.linalg.norm(state["c"] - c) / torch.linalg.norm(c))
+    leg, fft = transform_flops(L, L + 1, 2 * L + 1)
+    rec("glq_L1023", ms, 2 * (leg + fft), {"grid": [L + 1, 2 * L + 1], "roundtrip_err": err})
+    # Driscoll-Healy sampling 1, lmax 2047, orthonormal
+    L = 2047
+    c = torch.from_numpy(power_law_cilm(rng, L, -2.0)).to(dev)
+
+    def dh1():
+        state["g"] = B.MakeGridDH(c, norm=4, sampling=1)
+        state["c"] = B.SHExpandDH(state["g"], norm=4, sampling=1)
+    ms = _timed(dh1, reps)
+    err = float(torch.linalg.norm(state["c"] - c) / torch.linalg.norm(c))
+    leg, fft = dh_flops(L, 1)
+    rec("dh1_L2047", ms, 2 * (leg + fft), {"grid": [2 * L + 2, 2 * L + 2], "roundtrip_err": err})
+    del state["g"], state["c"], c
+    # 16 least-squares fits at lmax 511: gradient = S^T (2 w (S c - d)) + 2e-6 l(l+1) c
+    L, nfits = 511, 16
+    batch = DHBatch([L] * nfits, sampling=2, norm=1, csphase=1, device=dev)
+    truth, data, wlat = lsq_fit_workload(batch, seed=4)
+    ll1 = torch.tensor([l * (l + 1.0) for l in range(L + 1)], dtype=torch.float64, device=dev)[None, :, None]
+    coef = [0.5 * t for t in truth]
+    grids = [torch.empty(batch.grid_shape(i), dtype=torch.float64, device=dev) for i in range(nfits)]
+    res = [torch.empty_like(g) for g in grids]
+    grads = [torch.empty_like(t) for t in truth]
+
+    def lsq_grad():
+        batch.makegrid(coef, out=grids)
+        for r, g, d in zip(res, grids, data):
+            torch.sub(g, d, out=r)
+            r.mul_(2.0 * wlat[:, None])
+        batch.makegrid_adjoint(res, out=grads)
+        for gr, cf in zip(grads, coef):
+            gr.add_(2e-6 * ll1 * cf)
+    ms = _timed(lsq_grad, reps)
+    leg, fft = dh_flops(L, 2)
+    rec("lsq_16fits_L511", ms, nfits * 2 * (leg + fft),
+        {"gradients_per_s": nfits / (ms * 1e-3), "note": "one gradient of all 16 fits: batched synthesis, "
+                                                         "weighted residual, batched adjoint (analysis kernels)"})
+    del batch, truth, data, coef, grids, res, grads
+    # high-degree batch: 8 fields lmax ~ U{2047..2799}, filtered synthesis + analysis
+    Ls, cs = high_degree_workload()
+    batch = DHBatch(Ls, sampling=2, norm=1, csphase=1, device=dev)
+    filt = [torch.from_numpy(upward_continuation(L)).to(dev)[None, :, None] for L in Ls]
+    cin = [torch.from_numpy(x).to(dev) * f for x, f in zip(cs, filt)]
+    grids = [torch.empty(batch.grid_shape(i), dtype=torch.float64, device=dev) for i in range(len(Ls))]
+    back = [torch.empty_like(x) for x in cin]
+
+    def high():
+        batch.makegrid(cin, out=grids)
+        batch.expand(grids, out=back)
+    ms = _timed(high, max(3, reps // 2))
+    err = max(float(torch.linalg.norm(b - x) / torch.linalg.norm(x)) for b, x in zip(back, cin))
+    rec("high_degree_8fields", ms, sum(2 * sum(dh_flops(L, 2)) for L in Ls),
+        {"lmax": Ls, "fields_per_s": len(Ls) / (ms * 1e-3), "roundtrip_err": err})
+    return out
 
 
 def main():
@@ -289,6 +400,10 @@ def main():
     fft_flops = sum(dh_flops(L)[1] for L in Ls)
     total_flops = 2 * (leg_flops + fft_flops)
 
+    configs = None
+    if not args.loop and not args.no_configs:
+        configs = other_configs(dev)
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -339,6 +454,7 @@ def main():
         "batched_vs_loop": (value / loop_rate) if loop_rate else None,
         "stages": stages,
         "fft_workgroups": shapes,
+        "other_configs": configs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

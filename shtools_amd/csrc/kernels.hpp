@@ -81,16 +81,22 @@ __device__ __forceinline__ void pack_tile(const double *__restrict__ cilm, int d
 
 // cilm rows [l0, l0+kTL) x columns [m0, m0+kTM) (within d1 x d2): packed
 // sums times (s_lm * normalisation) for m <= l <= L, zero elsewhere and S_l0 = 0.
+// The packed sums may come in nblk partial blocks cstride apart, added in
+// order.
 __device__ __forceinline__ void unpack_tile(const double2 *__restrict__ cbuf, const double *__restrict__ post,
                                             int Lt, int L, double *__restrict__ cilm, int d1, int d2, int m0, int l0,
-                                            CoefTile &t)
+                                            CoefTile &t, int nblk = 1, long long cstride = 0)
 {
     for (int e = threadIdx.x; e < kTL * kTM; e += blockDim.x) {
         const int c = e / kTL, r = e - c * kTL, l = l0 + r, m = m0 + c;
         double2 v = make_double2(0.0, 0.0);
         if (l <= L && m <= l) {
             const double p = post[tri_offd(m, Lt) + (l - m)];
-            const double2 x = cbuf[tri_offd(m, L) + (l - m)];
+            double2 x = cbuf[tri_offd(m, L) + (l - m)];
+            for (int b = 1; b < nblk; ++b) {
+                const double2 y = cbuf[b * cstride + tri_offd(m, L) + (l - m)];
+                x = make_double2(x.x + y.x, x.y + y.y);
+            }
             v = make_double2(x.x * p, m == 0 ? 0.0 : x.y * p);
         }
         t[r][c] = v;
@@ -612,27 +618,41 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft_ws(FftDev F, const 
 
 // one single-wave workgroup per order m over ring pairs [s0, P), from the
 // order's first slot with non-negligible functions (smin; nullptr: s0)
+//
+// nblk > 1 splits each order's ring pairs into blocks of bslots slots (a
+// multiple of K*64) from its first slot: workgroup m * nblk + b sums block b
+// into cbuf + b * cstride (zeros when the order has no slots there), and the
+// unpack adds the blocks in order.  Degrees far above the GPU's wave count
+// then no longer wait on one wave per order.
 template <int K>
 __global__ __launch_bounds__(kWave) void k_anal_leg(const double *__restrict__ alpha, int Lt,
                                                     const double *__restrict__ seed,
                                                     const double *__restrict__ ringz,
                                                     const double4 *__restrict__ gbuf, int Lc, int s0, int P,
-                                                    double2 *__restrict__ cbuf, const int *__restrict__ smin)
+                                                    double2 *__restrict__ cbuf, const int *__restrict__ smin,
+                                                    int nblk, int bslots, long long cstride)
 {
     __shared__ double red[2 * kAnalLC * kRedStride];
-    const int m = blockIdx.x;
-    anal_leg_tile<K>(alpha + tri_offd(m, Lt), seed[m], ringz, gbuf + (size_t)m * P, Lc - m,
-                     smin ? max(s0, smin[m]) : s0, P, cbuf + tri_offd(m, Lc), red);
+    const int m = blockIdx.x / nblk, b = blockIdx.x - m * nblk;
+    const int sa = (smin ? max(s0, smin[m]) : s0) + b * bslots;
+    const int se = nblk > 1 ? min(P, sa + bslots) : P;
+    double2 *out = cbuf + b * cstride + tri_offd(m, Lc);
+    if (sa >= se) {
+        for (int j = threadIdx.x; j <= Lc - m; j += kWave) out[j] = make_double2(0.0, 0.0);
+        return;
+    }
+    anal_leg_tile<K>(alpha + tri_offd(m, Lt), seed[m], ringz, gbuf + (size_t)m * P, Lc - m, sa, se, out, red);
 }
 
 // grid: ntm x ntl tiles over the whole d1 x d2 array
 __global__ __launch_bounds__(kThreads) void k_unpack_anal(const double2 *__restrict__ cbuf,
                                                           const double *__restrict__ post, int Lt, int Lc,
-                                                          double *__restrict__ cilm, int d1, int d2, int ntm)
+                                                          double *__restrict__ cilm, int d1, int d2, int ntm,
+                                                          int nblk, long long cstride)
 {
     __shared__ CoefTile t;
     const int tl = blockIdx.x / ntm, tm = blockIdx.x - tl * ntm;
-    unpack_tile(cbuf, post, Lt, Lc, cilm, d1, d2, tm * kTM, tl * kTL, t);
+    unpack_tile(cbuf, post, Lt, Lc, cilm, d1, d2, tm * kTM, tl * kTL, t, nblk, cstride);
 }
 
 // Zero the part of a rows x cols array (leading dimension ld) outside its

@@ -260,10 +260,25 @@ static Geometry *get_glq(int dev, int L, const double *zero, const double *wq)
     return &g;
 }
 
+// The plain transforms take their intermediates from the device's default
+// stream-ordered pool (hipMallocAsync).  Its release threshold is raised
+// once per device so that freed blocks stay in the pool across calls
+// instead of being unmapped at every synchronisation and mapped again by the
+// next call; the pool keeps at most kPoolKeep bytes that way.
+constexpr uint64_t kPoolKeep = 8ull << 30;
+
 static int current_device()
 {
+    static bool pool_done[64] = {};
     int d = 0;
     if (!HIP_OK(hipGetDevice(&d))) return -1;
+    if (d >= 0 && d < 64 && !pool_done[d]) {
+        pool_done[d] = true;
+        hipMemPool_t pool = nullptr;
+        uint64_t keep = kPoolKeep;
+        if (HIP_OK(hipDeviceGetDefaultMemPool(&pool, d)))
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     return d;
 }
 
@@ -275,6 +290,17 @@ static int anal_lanes_per_slot(int P)
     const int per = kAnalKMax * kWave;
     const int np = std::max(1, (P + per - 1) / per);
     return std::max(1, (P + np * kWave - 1) / (np * kWave));
+}
+
+// Slot blocks per order of a plain analysis whose waves take `passes`
+// passes of K*64 ring pairs: one (a wave per order) while the orders alone
+// give the GPU enough waves, else up to one block per pass, as many as bring
+// the launch to about kAnalWaves waves.
+constexpr int kAnalWaves = 8192;
+static int anal_blocks(int Lc, int passes)
+{
+    if (passes <= 1 || Lc + 1 >= kAnalWaves) return 1;
+    return std::max(1, std::min(passes, (kAnalWaves + Lc) / (Lc + 1)));
 }
 
 // the analysis kernels by ring pairs per lane
@@ -443,11 +469,17 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
     const bool lds_fft = fft_in_lds(g.fft.m);
     const int nwg = std::min(g.P, kWsGroups);
     const size_t nws = lds_fft ? 0 : (size_t)nwg * ws_stride(g.fft.m);
+    // the Legendre sums of each order in blocks of slots (one wave each) when
+    // one wave per order would leave the GPU short of waves
+    const int K = anal_lanes_per_slot(g.P - s0);
+    const int passes = (g.P - s0 + K * kWave - 1) / (K * kWave);
+    const int nblk = anal_blocks(Lc, passes);
+    const int bslots = (passes + nblk - 1) / nblk * K * kWave;
     void *ws = nullptr;
-    const size_t bytes = npack * sizeof(double2) + nf * sizeof(double4) + nws * sizeof(double2);
+    const size_t bytes = nblk * npack * sizeof(double2) + nf * sizeof(double4) + nws * sizeof(double2);
     if (!HIP_OK(hipMallocAsync(&ws, bytes, st))) return ST_ALLOC;
     double2 *cbuf = (double2 *)ws;
-    double4 *gbuf = (double4 *)((char *)ws + npack * sizeof(double2));
+    double4 *gbuf = (double4 *)((char *)ws + nblk * npack * sizeof(double2));
     double2 *fws = (double2 *)((char *)gbuf + nf * sizeof(double4));
 
     {
@@ -462,14 +494,13 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
     }
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);
-        const AnalLegKernel kern = anal_leg_kernel(anal_lanes_per_slot(g.P - s0));
-        hipLaunchKernelGGL(kern, dim3(Lc + 1), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gbuf, Lc, s0,
-                           g.P, cbuf, (const int *)g.smin);
+        hipLaunchKernelGGL(anal_leg_kernel(K), dim3((Lc + 1) * nblk), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed,
+                           g.z, gbuf, Lc, s0, g.P, cbuf, (const int *)g.smin, nblk, bslots, (long long)npack);
     }
     ProfScope ps_un(SHTG_K_UNPACK, st);
     const int ntm = (d2 + kTM - 1) / kTM, ntl = (d1 + kTL - 1) / kTL;
     hipLaunchKernelGGL(k_unpack_anal, dim3(ntm * ntl), dim3(kThreads), 0, st, cbuf, ptab, rt.L, Lc, cilm, d1, d2,
-                       ntm);
+                       ntm, nblk, (long long)npack);
     if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
     (void)hipFreeAsync(ws, st);
     return ST_OK;
@@ -550,9 +581,9 @@ static int launch_analysis_c(const Geometry &g, const DevRecurrence &rt, const D
         ProfScope ps(SHTG_K_ANAL_LEG, st);
         const AnalLegKernel kern = anal_leg_kernel(anal_lanes_per_slot(g.P - g.s0));
         hipLaunchKernelGGL(kern, dim3(Lc + 1), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gb_p, Lc, g.s0, g.P,
-                           cb_p, (const int *)nullptr);
+                           cb_p, (const int *)nullptr, 1, 0, 0LL);
         hipLaunchKernelGGL(kern, dim3(Lc + 1), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gb_m, Lc, g.s0, g.P,
-                           cb_m, (const int *)nullptr);
+                           cb_m, (const int *)nullptr, 1, 0, 0LL);
     }
     {
         ProfScope ps(SHTG_K_UNPACK, st);

@@ -377,11 +377,19 @@ __device__ __forceinline__ void synth_fft_fill(const FftDev &F, const SynthRing 
 // DFT of the filled array and the rows out; the north and south rows are
 // scaled by rwt (1 for synthesis, the quadrature weights for the adjoint of
 // analysis).
+//
+// chunk != nullptr: lds is a work array in global memory and the DFT runs
+// its inner stages in the LDS chunk (chunk_cap elements) when a group of the
+// plan fits there.
 __device__ __forceinline__ void synth_fft_run(const FftDev &F, int2 rw, int nrows, double *__restrict__ grid,
-                                              long long ld, int ext_col, double2 rwt, double2 *lds)
+                                              long long ld, int ext_col, double2 rwt, double2 *lds,
+                                              double2 *chunk = nullptr, int chunk_cap = 0)
 {
     const int n = F.n;
-    fft_forward(lds, F, F.bhat);
+    if (chunk && F.nst >= 2 && F.m / F.radix[0] <= chunk_cap)
+        fft_forward_chunked(lds, F, F.bhat, chunk, chunk_cap);
+    else
+        fft_forward(lds, F, F.bhat);
     const bool north = rw.x >= 0 && rw.x < nrows;
     const bool south = rw.y >= 0 && rw.y < nrows;
     // columns j and n - j (1 <= j < n - j) together: one chirp read
@@ -406,13 +414,14 @@ __device__ __forceinline__ void synth_fft_run(const FftDev &F, int2 rw, int nrow
 
 __device__ __forceinline__ void synth_fft_tile(const FftDev &F, const double4 *__restrict__ fbuf, int P, int Lc,
                                                int s, double u, int2 rw, int nrows, double *__restrict__ grid,
-                                               long long ld, int ext_col, double2 rwt, double2 *lds)
+                                               long long ld, int ext_col, double2 rwt, double2 *lds,
+                                               double2 *chunk = nullptr, int chunk_cap = 0)
 {
     SynthRing in;
     synth_fft_load(fbuf, P, Lc, s, in);
     synth_fft_fill(F, in, fbuf, P, Lc, s, u, lds);
     __syncthreads();
-    synth_fft_run(F, rw, nrows, grid, ld, ext_col, rwt, lds);
+    synth_fft_run(F, rw, nrows, grid, ld, ext_col, rwt, lds, chunk, chunk_cap);
 }
 
 // Analysis inputs of one thread: ring values at the column pairs q and
@@ -483,12 +492,16 @@ __device__ __forceinline__ void anal_fft_fill(const FftDev &F, const AnalRing &i
 // weight and the reference's constant factors; the outputs carry the
 // order's rescale factor for the ring (sin(theta) = u).
 __device__ __forceinline__ void anal_fft_run(const FftDev &F, int2 rw, int nrows, double2 w, double u, int P, int Lc,
-                                             int s, double4 *__restrict__ gbuf, double2 *lds)
+                                             int s, double4 *__restrict__ gbuf, double2 *lds,
+                                             double2 *chunk = nullptr, int chunk_cap = 0)
 {
     const int n = F.n;
     const bool north = rw.x >= 0 && rw.x < nrows;
     const bool south = rw.y >= 0 && rw.y < nrows;
-    fft_forward(lds, F, F.bhat2);
+    if (chunk && F.nst >= 2 && F.m / F.radix[0] <= chunk_cap)
+        fft_forward_chunked(lds, F, F.bhat2, chunk, chunk_cap);
+    else
+        fft_forward(lds, F, F.bhat2);
     const double wn = north ? w.x : 0.0, ws = south ? w.y : 0.0;
     // orders interleave over the threads (a wave reads consecutive elements)
     for (int mm = threadIdx.x; mm <= Lc; mm += blockDim.x) {
@@ -506,13 +519,14 @@ __device__ __forceinline__ void anal_fft_run(const FftDev &F, int2 rw, int nrows
 
 __device__ __forceinline__ void anal_fft_tile(const FftDev &F, const double *__restrict__ grid, long long ld,
                                               int2 rw, int nrows, double2 w, double u, int P, int Lc, int s,
-                                              double4 *__restrict__ gbuf, int ext_col, double2 *lds)
+                                              double4 *__restrict__ gbuf, int ext_col, double2 *lds,
+                                              double2 *chunk = nullptr, int chunk_cap = 0)
 {
     AnalRing in;
     anal_fft_load(F, grid, ld, rw, nrows, ext_col, in);
     anal_fft_fill(F, in, grid, ld, rw, nrows, ext_col, lds);
     __syncthreads();
-    anal_fft_run(F, rw, nrows, w, u, P, Lc, s, gbuf, lds);
+    anal_fft_run(F, rw, nrows, w, u, P, Lc, s, gbuf, lds, chunk, chunk_cap);
 }
 
 // =============================================================== plain kernels
@@ -591,12 +605,14 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_synth_fft_ws(FftDev F, const
                                                            const int2 *__restrict__ rows, int nrows,
                                                            double *__restrict__ grid, long long ld, int ext_col,
                                                            const double2 *__restrict__ roww, double2 *ws,
-                                                           long long ws_stride, const int *__restrict__ kmax)
+                                                           long long ws_stride, const int *__restrict__ kmax,
+                                                           int chunk_cap)
 {
+    extern __shared__ double2 chunk[];
     double2 *x = ws + blockIdx.x * ws_stride;
     for (int s = blockIdx.x; s < P; s += gridDim.x) {
         synth_fft_tile(F, fbuf, P, slot_orders(kmax, s, Lc), s, ringu[s], rows[s], nrows, grid, ld, ext_col,
-                       roww ? roww[s] : make_double2(1.0, 1.0), x);
+                       roww ? roww[s] : make_double2(1.0, 1.0), x, chunk, chunk_cap);
         __syncthreads();
     }
 }
@@ -606,12 +622,14 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft_ws(FftDev F, const 
                                                           const int2 *__restrict__ rows, int nrows,
                                                           const double2 *__restrict__ wts, int P, int Lc,
                                                           double4 *__restrict__ gbuf, int ext_col, double2 *ws,
-                                                          long long ws_stride, const int *__restrict__ kmax)
+                                                          long long ws_stride, const int *__restrict__ kmax,
+                                                          int chunk_cap)
 {
+    extern __shared__ double2 chunk[];
     double2 *x = ws + blockIdx.x * ws_stride;
     for (int s = blockIdx.x; s < P; s += gridDim.x) {
         anal_fft_tile(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, slot_orders(kmax, s, Lc), s, gbuf, ext_col,
-                      x);
+                      x, chunk, chunk_cap);
         __syncthreads();
     }
 }
@@ -1114,8 +1132,10 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft(const BField *__re
 __global__ __launch_bounds__(kFftMaxThreads) void kb_synth_fft_ws(const BField *__restrict__ fields,
                                                             const int2 *__restrict__ items, int nitems, int nslots,
                                                             BTables T, const double4 *__restrict__ fbuf,
-                                                            double *const *grid, double2 *ws, long long ws_stride)
+                                                            double *const *grid, double2 *ws, long long ws_stride,
+                                                            int chunk_cap)
 {
+    extern __shared__ double2 chunk[];
     double2 *x = ws + blockIdx.x * ws_stride;
     for (int id = blockIdx.x; id < nitems; id += gridDim.x) {
         const int2 it = items[id];
@@ -1124,7 +1144,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_synth_fft_ws(const BField *
         for (int s = it.y; s < it.y + nslots && s < f.P; ++s) {
             synth_fft_tile(F, fbuf + f.foff, f.P, slot_orders(T.kmax, f.geo + s, f.L), s, T.u[f.geo + s],
                            T.rows[f.geo + s], f.nlat, grid[it.x], f.nlong, 0,
-                           T.roww ? T.roww[f.geo + s] : make_double2(1.0, 1.0), x);
+                           T.roww ? T.roww[f.geo + s] : make_double2(1.0, 1.0), x, chunk, chunk_cap);
             __syncthreads();
         }
     }
@@ -1134,8 +1154,9 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft_ws(const BField *_
                                                            const int2 *__restrict__ items, int nitems, int nslots,
                                                            BTables T, const double *const *grid,
                                                            double4 *__restrict__ gbuf, double2 *ws,
-                                                           long long ws_stride)
+                                                           long long ws_stride, int chunk_cap)
 {
+    extern __shared__ double2 chunk[];
     double2 *x = ws + blockIdx.x * ws_stride;
     for (int id = blockIdx.x; id < nitems; id += gridDim.x) {
         const int2 it = items[id];
@@ -1143,7 +1164,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft_ws(const BField *_
         const FftDev &F = T.plans[f.fft];
         for (int s = it.y; s < it.y + nslots && s < f.P; ++s) {
             anal_fft_tile(F, grid[it.x], f.nlong, T.rows[f.geo + s], f.nlat, T.w[f.geo + s], T.u[f.geo + s], f.P,
-                          slot_orders(T.kmax, f.geo + s, f.L), s, gbuf + f.foff, 0, x);
+                          slot_orders(T.kmax, f.geo + s, f.L), s, gbuf + f.foff, 0, x, chunk, chunk_cap);
             __syncthreads();
         }
     }

@@ -175,12 +175,16 @@ __device__ __forceinline__ void apply_twiddles(double2 (&v)[R], double2 w1)
 // stage) before it.  Radices 2, 3, 4, 5 and 8 are unrolled in registers;
 // radix 7 uses a rolled loop over outputs that stores each output as it is
 // formed.
+//
+// x holds mloc elements (mloc = m, or a run of whole groups of R*S elements
+// of the m-element transform: the butterflies and twiddles are the same in
+// every group).
 template <int R, bool DIT>
 __device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double2 *__restrict__ tw,
-                                          const double2 *__restrict__ tws)
+                                          const double2 *__restrict__ tws, int mloc)
 {
     constexpr bool kUnrolled = (R == 2 || R == 3 || R == 4 || R == 5 || R == 8);
-    const int Lsub = R * S, step = m / Lsub, nb = m / R, nt = blockDim.x;
+    const int Lsub = R * S, step = m / Lsub, nb = mloc / R, nt = blockDim.x;
     // butterfly b = grp * S + i starts at element base = grp * Lsub + i; the
     // thread's first (grp, i) come from one division, later ones by stepping
     // b by nt = dg * S + di
@@ -239,16 +243,16 @@ __device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double
 }
 
 template <bool DIT>
-__device__ __forceinline__ void fft_run_stage(double2 *x, const FftDev &P, int st)
+__device__ __forceinline__ void fft_run_stage(double2 *x, const FftDev &P, int st, int mloc)
 {
     const int S = P.stride[st];
     const double2 *tws = P.twst + P.twoff[st];
     switch (P.radix[st]) {
 #define SHTG_STAGE_CASE(R) \
-    case R: fft_stage<R, DIT>(x, P.m, S, P.tw, tws); break;
+    case R: fft_stage<R, DIT>(x, P.m, S, P.tw, tws, mloc); break;
         SHTG_FFT_RADICES(SHTG_STAGE_CASE)
 #undef SHTG_STAGE_CASE
-    case 7: fft_stage<7, DIT>(x, P.m, S, P.tw, tws); break;
+    case 7: fft_stage<7, DIT>(x, P.m, S, P.tw, tws, mloc); break;
     default: break;
     }
 }
@@ -283,10 +287,12 @@ __device__ __forceinline__ double2 fft_prep_in_c(const FftDev &P, double2 v, dou
 // all 1 and DIF and DIT touch the same R contiguous elements of one
 // butterfly: the three steps run in registers, with one LDS read and one
 // write per element.
+//
+// x holds mloc elements; bhat is the spectrum at x's first element.
 template <int R>
-__device__ __forceinline__ void fft_turn(double2 *x, int m, const double2 *__restrict__ bhat)
+__device__ __forceinline__ void fft_turn(double2 *x, int mloc, const double2 *__restrict__ bhat)
 {
-    for (int b = threadIdx.x; b < m / R; b += blockDim.x) {
+    for (int b = threadIdx.x; b < mloc / R; b += blockDim.x) {
         const int base = b * R;
         double2 v[R];
 #pragma unroll
@@ -303,12 +309,12 @@ __device__ __forceinline__ void fft_turn(double2 *x, int m, const double2 *__res
 // Bluestein work lengths are 5-smooth and not powers of two
 // (bluestein_length), and factor_stages orders odd radices last, so the
 // last radix of a Bluestein plan is 3 or 5.
-__device__ __forceinline__ void fft_run_turn(double2 *x, const FftDev &P, const double2 *__restrict__ bhat)
+__device__ __forceinline__ void fft_run_turn(double2 *x, const FftDev &P, const double2 *__restrict__ bhat, int mloc)
 {
     if (P.radix[P.nst - 1] == 3)
-        fft_turn<3>(x, P.m, bhat);
+        fft_turn<3>(x, mloc, bhat);
     else
-        fft_turn<5>(x, P.m, bhat);
+        fft_turn<5>(x, mloc, bhat);
 }
 
 // In-place forward DFT of the values stored in x (x has P.m slots; for
@@ -323,14 +329,56 @@ __device__ __forceinline__ void fft_forward(double2 *x, const FftDev &P, const d
     const int last = P.nst - 1;
     const int ndif = P.blue ? last : last + 1;  // Bluestein: the last stage is part of the turn
     for (int st = 0; st < ndif; ++st) {
-        fft_run_stage<false>(x, P, st);
+        fft_run_stage<false>(x, P, st, P.m);
         __syncthreads();
     }
     if (!P.blue) return;
-    fft_run_turn(x, P, bhat);
+    fft_run_turn(x, P, bhat, P.m);
     __syncthreads();
     for (int st = last - 1; st >= 0; --st) {
-        fft_run_stage<true>(x, P, st);
+        fft_run_stage<true>(x, P, st, P.m);
+        __syncthreads();
+    }
+}
+
+// fft_forward for a work array x in global memory (rings whose array exceeds
+// the LDS).  After the first DIF stage the transform splits into radix[0]
+// independent groups of m / radix[0] consecutive elements, and every later
+// stage, the Bluestein turn and every DIT stage but the last stay inside a
+// group.  So only the first stage and its transpose run on x; the rest runs
+// in LDS on chunks of whole groups (lds holds lds_cap elements, at least one
+// group): load, all inner stages, store.  x then crosses the memory system
+// four times per transform instead of twice per stage.  Ends with a barrier.
+__device__ __forceinline__ void fft_forward_chunked(double2 *x, const FftDev &P, const double2 *__restrict__ bhat,
+                                                    double2 *lds, int lds_cap)
+{
+    const int last = P.nst - 1;
+    const int gsz = P.m / P.radix[0];
+    const int per = max(1, lds_cap / gsz) * gsz;  // elements per chunk
+    const int ndif = P.blue ? last : last + 1;
+    fft_run_stage<false>(x, P, 0, P.m);
+    __syncthreads();
+    for (int c0 = 0; c0 < P.m; c0 += per) {
+        const int len = min(per, P.m - c0);
+        for (int e = threadIdx.x; e < len; e += blockDim.x) lds[fpad(e)] = x[c0 + e];
+        __syncthreads();
+        for (int st = 1; st < ndif; ++st) {
+            fft_run_stage<false>(lds, P, st, len);
+            __syncthreads();
+        }
+        if (P.blue) {
+            fft_run_turn(lds, P, bhat + c0, len);
+            __syncthreads();
+            for (int st = last - 1; st >= 1; --st) {
+                fft_run_stage<true>(lds, P, st, len);
+                __syncthreads();
+            }
+        }
+        for (int e = threadIdx.x; e < len; e += blockDim.x) x[c0 + e] = lds[fpad(e)];
+        __syncthreads();
+    }
+    if (P.blue && last >= 1) {
+        fft_run_stage<true>(x, P, 0, P.m);
         __syncthreads();
     }
 }

@@ -25,8 +25,10 @@ namespace shtg {
 
 enum { ST_OK = 0, ST_DIMS = 1, ST_BOUNDS = 2, ST_ALLOC = 3, ST_OTHER = 5 };
 
-// Largest LDS footprint a ring-FFT workgroup may use (gfx950: 160 KiB).
+// Largest LDS footprint a ring-FFT workgroup may use (gfx950: 160 KiB), and
+// the elements of a workspace FFT's LDS chunk of that size.
 constexpr size_t kMaxLds = 160 * 1024;
+constexpr int kChunkCap = (int)(kMaxLds / sizeof(double2));
 
 static std::mutex g_mu;
 
@@ -84,6 +86,9 @@ static void configure_kernels()
     if (kernels_configured) return;
     (void)hipFuncSetAttribute((const void *)k_synth_fft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
     (void)hipFuncSetAttribute((const void *)k_anal_fft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+    for (const void *k : {(const void *)k_synth_fft_ws, (const void *)k_anal_fft_ws, (const void *)kb_synth_fft,
+                          (const void *)kb_anal_fft, (const void *)kb_synth_fft_ws, (const void *)kb_anal_fft_ws})
+        (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
     kernels_configured = true;
 }
 
@@ -293,13 +298,15 @@ static int anal_lanes_per_slot(int P)
 }
 
 // Slot blocks per order of a plain analysis whose waves take `passes`
-// passes of K*64 ring pairs: one (a wave per order) while the orders alone
-// give the GPU enough waves, else up to one block per pass, as many as bring
-// the launch to about kAnalWaves waves.
+// passes of K*64 ring pairs: one (a wave per order) for up to two passes or
+// when the orders alone give the GPU enough waves, else up to one block per
+// pass, as many as bring the launch to about kAnalWaves waves.  (With two
+// passes the split shortened single calls but slowed a loop of calls over
+// the degrees 63..1023, so it starts at three.)
 constexpr int kAnalWaves = 8192;
 static int anal_blocks(int Lc, int passes)
 {
-    if (passes <= 1 || Lc + 1 >= kAnalWaves) return 1;
+    if (passes <= 2 || Lc + 1 >= kAnalWaves) return 1;
     return std::max(1, std::min(passes, (kAnalWaves + Lc) / (Lc + 1)));
 }
 
@@ -444,8 +451,8 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
             hipLaunchKernelGGL(k_synth_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, fbuf, g.P, Lc, ru,
                                g.rows, nrows_out, grid, ld, ext_col, roww, kmax);
         else
-            hipLaunchKernelGGL(k_synth_fft_ws, dim3(nwg), dim3(kFftMaxThreads), 0, st, g.fft, fbuf, g.P, Lc, ru,
-                               g.rows, nrows_out, grid, ld, ext_col, roww, fws, ws_stride(g.fft.m), kmax);
+            hipLaunchKernelGGL(k_synth_fft_ws, dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft, fbuf, g.P, Lc, ru,
+                               g.rows, nrows_out, grid, ld, ext_col, roww, fws, ws_stride(g.fft.m), kmax, kChunkCap);
     }
     if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
     (void)hipFreeAsync(ws, st);
@@ -489,8 +496,9 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
             hipLaunchKernelGGL(k_anal_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, grid, ld, g.u, g.rows,
                                nrows, w, g.P, Lc, gbuf, ext_col, (const int *)g.kmax);
         else
-            hipLaunchKernelGGL(k_anal_fft_ws, dim3(nwg), dim3(kFftMaxThreads), 0, st, g.fft, grid, ld, g.u, g.rows,
-                               nrows, w, g.P, Lc, gbuf, ext_col, fws, ws_stride(g.fft.m), (const int *)g.kmax);
+            hipLaunchKernelGGL(k_anal_fft_ws, dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft, grid, ld, g.u, g.rows,
+                               nrows, w, g.P, Lc, gbuf, ext_col, fws, ws_stride(g.fft.m), (const int *)g.kmax,
+                               kChunkCap);
     }
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);
@@ -947,9 +955,9 @@ static int batch_synthesis(BatchPlan *b, hipStream_t st, const BTables &T)
             hipLaunchKernelGGL(kb_synth_fft, dim3(xcd_blocks(g.nitems)), dim3(g.threads), g.lds, st, b->fields,
                                g.items, g.nitems, b->fft_slots, T, b->fbuf, gout);
         if (b->fft_ws.nitems)
-            hipLaunchKernelGGL(kb_synth_fft_ws, dim3(std::min(b->fft_ws.nitems, kWsGroups)), dim3(kFftMaxThreads), 0,
-                               st, b->fields, b->fft_ws.items, b->fft_ws.nitems, b->fft_slots, T, b->fbuf, gout,
-                               b->fws, b->fws_stride);
+            hipLaunchKernelGGL(kb_synth_fft_ws, dim3(std::min(b->fft_ws.nitems, kWsGroups)), dim3(kFftMaxThreads),
+                               kMaxLds, st, b->fields, b->fft_ws.items, b->fft_ws.nitems, b->fft_slots, T, b->fbuf,
+                               gout, b->fws, b->fws_stride, kChunkCap);
     }
     return HIP_OK(hipGetLastError()) ? ST_OK : ST_OTHER;
 }
@@ -966,9 +974,9 @@ static int batch_analysis(BatchPlan *b, hipStream_t st, const BTables &T)
             hipLaunchKernelGGL(kb_anal_fft, dim3(xcd_blocks(g.nitems)), dim3(g.threads_anal), g.lds, st, b->fields,
                                g.items, g.nitems, b->fft_slots, T, gin, b->fbuf);
         if (b->fft_ws.nitems)
-            hipLaunchKernelGGL(kb_anal_fft_ws, dim3(std::min(b->fft_ws.nitems, kWsGroups)), dim3(kFftMaxThreads), 0,
-                               st, b->fields, b->fft_ws.items, b->fft_ws.nitems, b->fft_slots, T, gin, b->fbuf,
-                               b->fws, b->fws_stride);
+            hipLaunchKernelGGL(kb_anal_fft_ws, dim3(std::min(b->fft_ws.nitems, kWsGroups)), dim3(kFftMaxThreads),
+                               kMaxLds, st, b->fields, b->fft_ws.items, b->fft_ws.nitems, b->fft_slots, T, gin,
+                               b->fbuf, b->fws, b->fws_stride, kChunkCap);
     }
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);

@@ -9,7 +9,9 @@
 // long double; the line printed per length is
 //   n m blue nstages radices... max_rel_err
 // with the error relative to max |X| (1 when the chirp table breaks the
-// mirror identity chirp_{n-j} = (-1)^n chirp_j that the kernels rely on).
+// mirror identity chirp_{n-j} = (-1)^n chirp_j that the kernels rely on, 2
+// when fft_forward_chunked, the variant for work arrays outside the LDS,
+// differs from fft_forward in any bit).
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -78,7 +80,21 @@ static double run_case(const FftPlanHost &h, int mode, std::mt19937_64 &rng, int
             x[j] = fft_prep_in(F, j, v);
         }
     }
+    // the chunked variant (work array outside the LDS, inner stages on
+    // chunks of one and of three groups) must give the same bits
+    std::vector<std::vector<double2>> xc;
+    if (h.nst >= 2) {
+        const int gsz = h.m / h.radix[0];
+        for (int cap : {gsz, 3 * gsz}) {
+            std::vector<double2> y = x, lds((size_t)cap + 1);
+            fft_forward_chunked(y.data(), F, mode == 2 ? F.bhat2 : F.bhat, lds.data(), cap);
+            xc.push_back(y);
+        }
+    }
     fft_forward(x.data(), F, mode == 2 ? F.bhat2 : F.bhat);
+    for (const auto &y : xc)
+        for (int e = 0; e < h.m; ++e)
+            if (y[e].x != x[e].x || y[e].y != x[e].y) return 2.0;
     std::vector<int> ks;
     if (mode == 2) {
         for (int k = -h.lc; k <= h.lc; ++k) ks.push_back((k + n) % n);

@@ -3,6 +3,7 @@
 // with blockDim.x = 1 runs every butterfly of a stage in turn, and the
 // barrier is a no-op.  Used only by the CPU check of the FFT plans.
 #pragma once
+#include <algorithm>
 #include <cmath>
 
 #define __device__
@@ -24,3 +25,5 @@ inline void __syncthreads() {}
 using std::frexp;
 using std::ldexp;
 using std::fma;
+using std::max;
+using std::min;

@@ -776,8 +776,10 @@ __device__ __forceinline__ double msign(int m) { return (m & 1) ? -1.0 : 1.0; }
 __device__ __forceinline__ void synth_fft_c_tile(const FftDev &F, const double4 *__restrict__ fb_p,
                                                  const double4 *__restrict__ fb_m, int P, int Lc, int s, double u,
                                                  int2 rw, int nrows, double2 *__restrict__ grid, long long ld,
-                                                 int ext_col, double2 *x)
+                                                 int ext_col, double2 *x, double2 *chunk = nullptr,
+                                                 int chunk_cap = 0)
 {
+    const bool chunked = chunk && F.nst >= 2 && F.m / F.radix[0] <= chunk_cap;
     const int n = F.n, m = F.m;
     const int top = (F.lc >= 0) ? m : n;
     for (int ring = 0; ring < 2; ++ring) {
@@ -798,7 +800,10 @@ __device__ __forceinline__ void synth_fft_c_tile(const FftDev &F, const double4 
             }
         }
         __syncthreads();
-        fft_forward(x, F, F.bhat);
+        if (chunked)
+            fft_forward_chunked(x, F, F.bhat, chunk, chunk_cap);
+        else
+            fft_forward(x, F, F.bhat);
         for (int j = threadIdx.x; j < n; j += blockDim.x) {
             const double2 v = conjd(fft_get(x, F, j, j));
             grid[row * ld + j] = v;
@@ -813,8 +818,10 @@ __device__ __forceinline__ void synth_fft_c_tile(const FftDev &F, const double4 
 // stores (f, f), the south pass turns them into (f_N + f_S, f_N - f_S).
 __device__ __forceinline__ void anal_fft_c_tile(const FftDev &F, const double2 *__restrict__ grid, long long ld,
                                                 int2 rw, int nrows, double2 w, double u, int P, int Lc, int s,
-                                                double4 *__restrict__ gb_p, double4 *__restrict__ gb_m, double2 *x)
+                                                double4 *__restrict__ gb_p, double4 *__restrict__ gb_m, double2 *x,
+                                                double2 *chunk = nullptr, int chunk_cap = 0)
 {
+    const bool chunked = chunk && F.nst >= 2 && F.m / F.radix[0] <= chunk_cap;
     const int n = F.n, m = F.m;
     for (int ring = 0; ring < 2; ++ring) {
         const int row = ring ? rw.y : rw.x;
@@ -823,7 +830,10 @@ __device__ __forceinline__ void anal_fft_c_tile(const FftDev &F, const double2 *
         for (int q = threadIdx.x; q < m; q += blockDim.x)
             x[fpad(q)] = (q < n) ? fft_prep_in(F, q, grid[row * ld + q]) : make_double2(0.0, 0.0);
         __syncthreads();
-        fft_forward(x, F, F.bhat2);
+        if (chunked)
+            fft_forward_chunked(x, F, F.bhat2, chunk, chunk_cap);
+        else
+            fft_forward(x, F, F.bhat2);
         for (int k = threadIdx.x; k <= Lc; k += blockDim.x) {
             const double r = order_rescale(u, k) * wr;
             const double2 zp = fft_get(x, F, k, k);
@@ -856,14 +866,15 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_synth_fft_c(FftDev F, const 
                                                           const double *__restrict__ ringu,
                                                           const int2 *__restrict__ rows, int nrows,
                                                           double2 *__restrict__ grid, long long ld, int ext_col,
-                                                          double2 *ws, long long ws_stride)
+                                                          double2 *ws, long long ws_stride, int chunk_cap)
 {
     extern __shared__ double2 lds[];
     double2 *x;
     if constexpr (WS) x = ws + blockIdx.x * ws_stride;
     else x = lds;
     for (int s = blockIdx.x; s < P; s += gridDim.x)
-        synth_fft_c_tile(F, fb_p, fb_m, P, Lc, s, ringu[s], rows[s], nrows, grid, ld, ext_col, x);
+        synth_fft_c_tile(F, fb_p, fb_m, P, Lc, s, ringu[s], rows[s], nrows, grid, ld, ext_col, x,
+                         WS ? lds : nullptr, chunk_cap);
 }
 
 template <bool WS>
@@ -872,14 +883,15 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft_c(FftDev F, const d
                                                          const int2 *__restrict__ rows, int nrows,
                                                          const double2 *__restrict__ wts, int P, int Lc,
                                                          double4 *__restrict__ gb_p, double4 *__restrict__ gb_m,
-                                                         double2 *ws, long long ws_stride)
+                                                         double2 *ws, long long ws_stride, int chunk_cap)
 {
     extern __shared__ double2 lds[];
     double2 *x;
     if constexpr (WS) x = ws + blockIdx.x * ws_stride;
     else x = lds;
     for (int s = blockIdx.x; s < P; s += gridDim.x)
-        anal_fft_c_tile(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, Lc, s, gb_p, gb_m, x);
+        anal_fft_c_tile(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, Lc, s, gb_p, gb_m, x,
+                        WS ? lds : nullptr, chunk_cap);
 }
 
 // ================================================================== gradient

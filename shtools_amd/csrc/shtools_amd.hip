@@ -87,7 +87,9 @@ static void configure_kernels()
     (void)hipFuncSetAttribute((const void *)k_synth_fft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
     (void)hipFuncSetAttribute((const void *)k_anal_fft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
     for (const void *k : {(const void *)k_synth_fft_ws, (const void *)k_anal_fft_ws, (const void *)kb_synth_fft,
-                          (const void *)kb_anal_fft, (const void *)kb_synth_fft_ws, (const void *)kb_anal_fft_ws})
+                          (const void *)kb_anal_fft, (const void *)kb_synth_fft_ws, (const void *)kb_anal_fft_ws,
+                          (const void *)k_synth_fft_c<false>, (const void *)k_synth_fft_c<true>,
+                          (const void *)k_anal_fft_c<false>, (const void *)k_anal_fft_c<true>})
         (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
     kernels_configured = true;
 }
@@ -550,10 +552,11 @@ static int launch_synthesis_c(const Geometry &g, const DevRecurrence &rt, const 
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         if (lds_fft)
             hipLaunchKernelGGL(k_synth_fft_c<false>, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, fb_p, fb_m,
-                               g.P, Lc, g.u, g.rows, nrows_out, grid, ld, ext_col, (double2 *)nullptr, 0LL);
+                               g.P, Lc, g.u, g.rows, nrows_out, grid, ld, ext_col, (double2 *)nullptr, 0LL, 0);
         else
-            hipLaunchKernelGGL(k_synth_fft_c<true>, dim3(nwg), dim3(kFftMaxThreads), 0, st, g.fft, fb_p, fb_m, g.P, Lc,
-                               g.u, g.rows, nrows_out, grid, ld, ext_col, fws, ws_stride(g.fft.m));
+            hipLaunchKernelGGL(k_synth_fft_c<true>, dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft, fb_p, fb_m,
+                               g.P, Lc, g.u, g.rows, nrows_out, grid, ld, ext_col, fws, ws_stride(g.fft.m),
+                               kChunkCap);
     }
     if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
     (void)hipFreeAsync(ws, st);
@@ -580,10 +583,10 @@ static int launch_analysis_c(const Geometry &g, const DevRecurrence &rt, const D
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         if (lds_fft)
             hipLaunchKernelGGL(k_anal_fft_c<false>, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, grid, ld,
-                               g.u, g.rows, g.nlat, g.w, g.P, Lc, gb_p, gb_m, (double2 *)nullptr, 0LL);
+                               g.u, g.rows, g.nlat, g.w, g.P, Lc, gb_p, gb_m, (double2 *)nullptr, 0LL, 0);
         else
-            hipLaunchKernelGGL(k_anal_fft_c<true>, dim3(nwg), dim3(kFftMaxThreads), 0, st, g.fft, grid, ld, g.u,
-                               g.rows, g.nlat, g.w, g.P, Lc, gb_p, gb_m, fws, ws_stride(g.fft.m));
+            hipLaunchKernelGGL(k_anal_fft_c<true>, dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft, grid, ld,
+                               g.u, g.rows, g.nlat, g.w, g.P, Lc, gb_p, gb_m, fws, ws_stride(g.fft.m), kChunkCap);
     }
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);

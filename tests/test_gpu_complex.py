@@ -102,3 +102,25 @@ def test_glqc_large(B):
     assert grid_err(g.cpu().numpy(), g_ref) <= TOL
     c2 = B.SHExpandGLQC(g, w, zero, csphase=-1)
     assert coef_err(c2.cpu().numpy(), c) <= 1e-11
+
+
+def test_dhc_rings_beyond_lds(B):
+    """DH2 at lmax 1708: the complex ring's Bluestein work array (m >= 10252)
+    exceeds the LDS, so both directions run on the global workspace with LDS
+    chunks.  Checked through the real transforms (pinned against the oracle
+    at this degree): MakeGridDHC of the complex forms of real fields A, B is
+    MakeGridDH(A) + i MakeGridDH(B), and SHExpandDHC inverts it."""
+    import torch
+    from helpers import rand_cilm, real_to_complex
+    L = 1708
+    print(f"case DHC beyond LDS L={L}", flush=True)
+    rng = np.random.default_rng(L)
+    A, Bc = rand_cilm(rng, L), rand_cilm(rng, L)
+    c = torch.from_numpy(real_to_complex(A, 4) + 1j * real_to_complex(Bc, 4)).cuda()
+    g = B.MakeGridDHC(c, sampling=2, norm=4)
+    gA = B.MakeGridDH(torch.from_numpy(A).cuda(), sampling=2, norm=4)
+    gB = B.MakeGridDH(torch.from_numpy(Bc).cuda(), sampling=2, norm=4)
+    ref = torch.complex(gA, gB)
+    assert float((g - ref).abs().max() / ref.abs().max()) <= 1e-12
+    c2 = B.SHExpandDHC(g, sampling=2, norm=4)
+    assert float(torch.linalg.norm(c2 - c) / torch.linalg.norm(c)) <= 1e-11

@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="skip the ring-FFT launch-shape autotuner")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the timings of the other workloads (GLQ 1023, DH1 2047, 16 fits, high degree)")
+    ap.add_argument("--configs-only", action="store_true",
+                    help="time only the other workloads (for profiling them); prints their JSON")
     ap.add_argument("--counter-pass", action="store_true",
                     help="(internal) one untimed step then a calibration copy, for a rocprofv3 --pmc run")
     return ap.parse_args()
@@ -97,6 +99,14 @@ def counter_pass(args):
 def run_counter(counter, nfields, timeout=240):
     """One rocprofv3 --pmc pass of the counter pass; returns {kernel name:
     summed KiB} and the reading of the last dispatch (the calibration copy)."""
+    res = run_counters([counter], nfields, timeout)
+    return res[counter]
+
+
+def run_counters(counters, nfields, timeout=240):
+    """One rocprofv3 --pmc pass over the counter pass collecting several
+    counters (of blocks that fit one pass); returns {counter: ({kernel name:
+    summed value}, value of the last dispatch)}."""
     import csv
     import shutil
     import subprocess
@@ -105,31 +115,61 @@ def run_counter(counter, nfields, timeout=240):
     if exe is None:
         raise RuntimeError("rocprofv3 not found")
     out = tempfile.mkdtemp(prefix="shtg_pmc_")
-    cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "run", "--",
-           sys.executable, os.path.abspath(__file__), "--counter-pass", "--nfields", str(nfields)]
+    names = " ".join(counters)
+    cmd = [exe, "--pmc"] + list(counters) + ["--output-format", "csv", "-d", out, "-o", "run", "--",
+                                             sys.executable, os.path.abspath(__file__), "--counter-pass",
+                                             "--nfields", str(nfields)]
     r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True, timeout=timeout)
     if r.returncode != 0:
-        raise RuntimeError(f"rocprofv3 --pmc {counter} exited with {r.returncode}: {r.stderr[-400:]}")
+        raise RuntimeError(f"rocprofv3 --pmc {names} exited with {r.returncode}: {r.stderr[-400:]}")
     path = None
     for dp, _, fs in os.walk(out):
         for f in fs:
             if f.endswith("counter_collection.csv"):
                 path = os.path.join(dp, f)
     if path is None:
-        raise RuntimeError(f"rocprofv3 --pmc {counter} wrote no counter_collection.csv")
-    per_kernel, last = {}, (-1, 0.0)
+        raise RuntimeError(f"rocprofv3 --pmc {names} wrote no counter_collection.csv")
+    res = {c: ({}, (-1, 0.0)) for c in counters}
     with open(path) as fh:
         for row in csv.DictReader(fh):
-            if row["Counter_Name"] != counter:
+            c = row["Counter_Name"]
+            if c not in res:
                 continue
+            per_kernel, last = res[c]
             v = float(row["Counter_Value"])
             name = row["Kernel_Name"]
             per_kernel[name] = per_kernel.get(name, 0.0) + v
             d = int(row["Dispatch_Id"])
             if d > last[0]:
-                last = (d, v)
+                res[c] = (per_kernel, (d, v))
     shutil.rmtree(out, ignore_errors=True)
-    return per_kernel, last[1]
+    return {c: (pk, last[1]) for c, (pk, last) in res.items()}
+
+
+# FP64 VALU instruction counters (per wave instruction; 64 lanes each, an FMA
+# counting 2 flops): the counter-measured FP64 work of each kernel
+FP64_COUNTERS = ("SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_ADD_F64")
+
+
+def measure_fp64(nfields):
+    """FP64 flops per step of each batched stage from the VALU F64
+    instruction counters (64 lanes per wave instruction, 2 flops per FMA;
+    inactive lanes count as well, so this is an upper bound); None if the
+    profiler is unavailable."""
+    try:
+        res = run_counters(FP64_COUNTERS, nfields)
+    except Exception as e:  # reported in the JSON line, never fatal
+        return None, str(e)
+    out = {}
+    for stage, kname in BATCH_KERNELS.items():
+        tot = {}
+        for c in FP64_COUNTERS:
+            per_kernel = res[c][0]
+            tot[c] = sum(v for k, v in per_kernel.items() if k.startswith(f"shtg::{kname}") or
+                         k.startswith(f"void shtg::{kname}"))
+        out[stage] = 64.0 * (2.0 * tot["SQ_INSTS_VALU_FMA_F64"] + tot["SQ_INSTS_VALU_MUL_F64"] +
+                             tot["SQ_INSTS_VALU_ADD_F64"])
+    return out, None
 
 
 def measure_traffic(nfields):
@@ -270,14 +310,23 @@ def main():
     if args.counter_pass:
         counter_pass(args)
         return
+    if args.configs_only:
+        import torch
+        dev = torch.device("cuda", 0)
+        torch.cuda.set_device(dev)
+        print(json.dumps({"other_configs": other_configs(dev)}))
+        return
 
     # counter passes first, in child processes, before this one touches the GPU
     traffic, traffic_err = None, "skipped (--no-traffic)"
+    fp64 = None
     if not args.no_traffic and not args.loop:
         t0 = time.perf_counter()
         traffic, traffic_err = measure_traffic(args.nfields)
+        fp64, fp64_err = measure_fp64(args.nfields)
         print(f"[bench] counter passes: {time.perf_counter() - t0:.1f} s"
-              + (f" ({traffic_err})" if traffic_err else ""), file=sys.stderr, flush=True)
+              + (f" ({traffic_err})" if traffic_err else "") + (f" ({fp64_err})" if fp64_err else ""),
+              file=sys.stderr, flush=True)
 
     import numpy as np
     import torch
@@ -317,15 +366,18 @@ def main():
               file=sys.stderr, flush=True)
 
     # the per-field loop of the plain kernels, for the batched-vs-loop ratio
+    # (best of three passes after a warm-up pass: the loop is partly bound by
+    # the host's calls, whose time varies from pass to pass)
     loop_rate = None
     if not args.loop and not args.no_loop:
         loop_step()
         torch.cuda.synchronize()
-        l0 = time.perf_counter()
-        loop_step()
-        torch.cuda.synchronize()
-        loop_rate = len(cilm) / (time.perf_counter() - l0)
-        print(f"[bench] per-field loop: {loop_rate:.1f} fields/s", file=sys.stderr, flush=True)
+        for _ in range(3):
+            l0 = time.perf_counter()
+            loop_step()
+            torch.cuda.synchronize()
+            loop_rate = max(loop_rate or 0.0, len(cilm) / (time.perf_counter() - l0))
+        print(f"[bench] per-field loop: {loop_rate:.1f} fields/s (best of 3 passes)", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
         step()
@@ -384,6 +436,10 @@ def main():
               "algo_flops": f, "algo_bytes": b}
         if traffic:
             st["hbm_bytes"] = (traffic["FETCH_SIZE"]["kernels"][k] + traffic["WRITE_SIZE"]["kernels"][k])
+            st["hbm_gbs_counters"] = st["hbm_bytes"] / (t * 1e-3) / 1e9
+        if fp64:
+            st["fp64_flops_counters"] = fp64[k]
+            st["fp64_tflops_counters"] = fp64[k] / (t * 1e-3) / 1e12
         stages[k] = st
     dom = max(stages, key=lambda k: stages[k]["ms"])
     d = stages[dom]

@@ -274,6 +274,80 @@ static Geometry *get_glq(int dev, int L, const double *zero, const double *wq)
 // next call; the pool keeps at most kPoolKeep bytes that way.
 constexpr uint64_t kPoolKeep = 8ull << 30;
 
+// Intermediates of one call from the pool, in power-of-two sizes: calls of
+// other degrees then reuse the blocks earlier calls freed instead of adding
+// blocks of new sizes to the pool.
+static hipError_t pool_alloc(void **p, size_t bytes, hipStream_t st)
+{
+    size_t b = 4096;
+    while (b < bytes) b <<= 1;
+    return hipMallocAsync(p, b, st);
+}
+
+static bool capturing(hipStream_t st);
+static int current_device();
+
+// Intermediates of the plain launchers: one buffer per device, reused by
+// consecutive calls (the callers hold g_mu, so calls do not interleave on
+// the host).  A call on another stream than the buffer's last user first
+// makes its stream wait, on the device, for that user's work (an event
+// recorded at release).  A larger request replaces the buffer by one of the
+// next power-of-two size, the old one freed in stream order after its last
+// use.  Under graph capture the call takes its own block from the pool
+// instead, since a later growth would free memory the graph still uses.
+// (Allocating per call from the pool cost 0.1-0.5 ms of host time per call in
+// back-to-back calls.)
+struct WsCache {
+    void *buf = nullptr;
+    size_t size = 0;
+    hipStream_t last = nullptr;
+    hipEvent_t ev = nullptr;
+    bool used = false;
+};
+static std::map<int, WsCache> g_ws;
+
+struct WsLease {
+    void *p = nullptr;
+    bool pooled = false;
+};
+
+static int ws_acquire(WsLease &l, size_t bytes, hipStream_t st)
+{
+    if (capturing(st)) {
+        l.pooled = true;
+        return HIP_OK(pool_alloc(&l.p, bytes, st)) ? ST_OK : ST_ALLOC;
+    }
+    const int dev = current_device();
+    if (dev < 0) return ST_OTHER;
+    WsCache &c = g_ws[dev];
+    if (!c.ev && !HIP_OK(hipEventCreateWithFlags(&c.ev, hipEventDisableTiming))) return ST_OTHER;
+    if (c.used && c.last != st && !HIP_OK(hipStreamWaitEvent(st, c.ev, 0))) return ST_OTHER;
+    if (bytes > c.size) {
+        size_t b = 1 << 20;
+        while (b < bytes) b <<= 1;
+        void *nb = nullptr;
+        if (!HIP_OK(hipMallocAsync(&nb, b, st))) return ST_ALLOC;
+        if (c.buf) (void)hipFreeAsync(c.buf, st);
+        c.buf = nb;
+        c.size = b;
+    }
+    l.p = c.buf;
+    l.pooled = false;
+    return ST_OK;
+}
+
+static void ws_release(const WsLease &l, hipStream_t st)
+{
+    if (l.pooled) {
+        (void)hipFreeAsync(l.p, st);
+        return;
+    }
+    WsCache &c = g_ws[current_device()];
+    (void)hipEventRecord(c.ev, st);
+    c.last = st;
+    c.used = true;
+}
+
 static int current_device()
 {
     static bool pool_done[64] = {};
@@ -300,15 +374,13 @@ static int anal_lanes_per_slot(int P)
 }
 
 // Slot blocks per order of a plain analysis whose waves take `passes`
-// passes of K*64 ring pairs: one (a wave per order) for up to two passes or
+// passes of K*64 ring pairs: one (a wave per order) for a single pass or
 // when the orders alone give the GPU enough waves, else up to one block per
-// pass, as many as bring the launch to about kAnalWaves waves.  (With two
-// passes the split shortened single calls but slowed a loop of calls over
-// the degrees 63..1023, so it starts at three.)
+// pass, as many as bring the launch to about kAnalWaves waves.
 constexpr int kAnalWaves = 8192;
 static int anal_blocks(int Lc, int passes)
 {
-    if (passes <= 2 || Lc + 1 >= kAnalWaves) return 1;
+    if (passes <= 1 || Lc + 1 >= kAnalWaves) return 1;
     return std::max(1, std::min(passes, (kAnalWaves + Lc) / (Lc + 1)));
 }
 
@@ -427,9 +499,10 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
     const bool lds_fft = fft_in_lds(g.fft.m);
     const int nwg = std::min(g.P, kWsGroups);
     const size_t nws = lds_fft ? 0 : (size_t)nwg * ws_stride(g.fft.m);
-    void *ws = nullptr;
+    WsLease lease;
     const size_t bytes = npack * sizeof(double2) + nf * sizeof(double4) + nws * sizeof(double2);
-    if (!HIP_OK(hipMallocAsync(&ws, bytes, st))) return ST_ALLOC;
+    if (int rc = ws_acquire(lease, bytes, st)) return rc;
+    void *ws = lease.p;
     double2 *cpack = (double2 *)ws;
     double4 *fbuf = (double4 *)((char *)ws + npack * sizeof(double2));
     double2 *fws = (double2 *)((char *)fbuf + nf * sizeof(double4));
@@ -456,8 +529,9 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
             hipLaunchKernelGGL(k_synth_fft_ws, dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft, fbuf, g.P, Lc, ru,
                                g.rows, nrows_out, grid, ld, ext_col, roww, fws, ws_stride(g.fft.m), kmax, kChunkCap);
     }
-    if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
-    (void)hipFreeAsync(ws, st);
+    const bool ok = HIP_OK(hipGetLastError());
+    ws_release(lease, st);
+    if (!ok) return ST_OTHER;
     return ST_OK;
 }
 
@@ -484,9 +558,10 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
     const int passes = (g.P - s0 + K * kWave - 1) / (K * kWave);
     const int nblk = anal_blocks(Lc, passes);
     const int bslots = (passes + nblk - 1) / nblk * K * kWave;
-    void *ws = nullptr;
+    WsLease lease;
     const size_t bytes = nblk * npack * sizeof(double2) + nf * sizeof(double4) + nws * sizeof(double2);
-    if (!HIP_OK(hipMallocAsync(&ws, bytes, st))) return ST_ALLOC;
+    if (int rc = ws_acquire(lease, bytes, st)) return rc;
+    void *ws = lease.p;
     double2 *cbuf = (double2 *)ws;
     double4 *gbuf = (double4 *)((char *)ws + nblk * npack * sizeof(double2));
     double2 *fws = (double2 *)((char *)gbuf + nf * sizeof(double4));
@@ -511,8 +586,9 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
     const int ntm = (d2 + kTM - 1) / kTM, ntl = (d1 + kTL - 1) / kTL;
     hipLaunchKernelGGL(k_unpack_anal, dim3(ntm * ntl), dim3(kThreads), 0, st, cbuf, ptab, rt.L, Lc, cilm, d1, d2,
                        ntm, nblk, (long long)npack);
-    if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
-    (void)hipFreeAsync(ws, st);
+    const bool ok = HIP_OK(hipGetLastError());
+    ws_release(lease, st);
+    if (!ok) return ST_OTHER;
     return ST_OK;
 }
 
@@ -527,9 +603,10 @@ static int launch_synthesis_c(const Geometry &g, const DevRecurrence &rt, const 
     const bool lds_fft = fft_in_lds(g.fft.m);
     const int nwg = std::min(g.P, kWsGroups);
     const size_t nws = lds_fft ? 0 : (size_t)nwg * ws_stride(g.fft.m);
-    void *ws = nullptr;
+    WsLease lease;
     const size_t bytes = 2 * npack * sizeof(double2) + 2 * nf * sizeof(double4) + nws * sizeof(double2);
-    if (!HIP_OK(hipMallocAsync(&ws, bytes, st))) return ST_ALLOC;
+    if (int rc = ws_acquire(lease, bytes, st)) return rc;
+    void *ws = lease.p;
     double2 *cp_p = (double2 *)ws, *cp_m = cp_p + npack;
     double4 *fb_p = (double4 *)(cp_m + npack), *fb_m = fb_p + nf;
     double2 *fws = (double2 *)(fb_m + nf);
@@ -558,8 +635,9 @@ static int launch_synthesis_c(const Geometry &g, const DevRecurrence &rt, const 
                                g.P, Lc, g.u, g.rows, nrows_out, grid, ld, ext_col, fws, ws_stride(g.fft.m),
                                kChunkCap);
     }
-    if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
-    (void)hipFreeAsync(ws, st);
+    const bool ok = HIP_OK(hipGetLastError());
+    ws_release(lease, st);
+    if (!ok) return ST_OTHER;
     return ST_OK;
 }
 
@@ -572,9 +650,10 @@ static int launch_analysis_c(const Geometry &g, const DevRecurrence &rt, const D
     const bool lds_fft = fft_in_lds(g.fft.m);
     const int nwg = std::min(g.P, kWsGroups);
     const size_t nws = lds_fft ? 0 : (size_t)nwg * ws_stride(g.fft.m);
-    void *ws = nullptr;
+    WsLease lease;
     const size_t bytes = 2 * npack * sizeof(double2) + 2 * nf * sizeof(double4) + nws * sizeof(double2);
-    if (!HIP_OK(hipMallocAsync(&ws, bytes, st))) return ST_ALLOC;
+    if (int rc = ws_acquire(lease, bytes, st)) return rc;
+    void *ws = lease.p;
     double2 *cb_p = (double2 *)ws, *cb_m = cb_p + npack;
     double4 *gb_p = (double4 *)(cb_m + npack), *gb_m = gb_p + nf;
     double2 *fws = (double2 *)(gb_m + nf);
@@ -602,8 +681,9 @@ static int launch_analysis_c(const Geometry &g, const DevRecurrence &rt, const D
         hipLaunchKernelGGL(k_unpack_anal_c, dim3(ntm * ntl), dim3(kThreads), 0, st, cb_p, cb_m, rt.post_c, rt.L,
                            Lc, cilm, d1, d2, ntm);
     }
-    if (!HIP_OK(hipGetLastError())) { (void)hipFreeAsync(ws, st); return ST_OTHER; }
-    (void)hipFreeAsync(ws, st);
+    const bool ok = HIP_OK(hipGetLastError());
+    ws_release(lease, st);
+    if (!ok) return ST_OTHER;
     return ST_OK;
 }
 
@@ -1400,7 +1480,7 @@ int shtg_makegradient_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax,
     const size_t plane = (size_t)L1 * L1;
     void *ws = nullptr;
     const size_t ngrid = (size_t)grid_d0 * grid_d1;
-    if (!HIP_OK(hipMallocAsync(&ws, (6 * plane + ngrid) * sizeof(double), hs))) return ST_ALLOC;
+    if (!HIP_OK(pool_alloc(&ws, (6 * plane + ngrid) * sizeof(double), hs))) return ST_ALLOC;
     double *coefs = (double *)ws, *gb = coefs + 6 * plane;
     hipLaunchKernelGGL(k_gradient_coefs, dim3(grid_blocks((long long)plane)), dim3(kThreads), 0, hs, cilm, cilm_d1,
                        cilm_d2, sh.Lc, coefs);
@@ -1541,7 +1621,7 @@ int shtg_makegravgrid_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax,
     const int L1 = sh.Lc + 1, n = 2 * lmax + 2;
     const size_t plane = (size_t)L1 * L1;
     void *ws = nullptr;
-    if (!HIP_OK(hipMallocAsync(&ws, 6 * plane * sizeof(double), hs))) return ST_ALLOC;
+    if (!HIP_OK(pool_alloc(&ws, 6 * plane * sizeof(double), hs))) return ST_ALLOC;
     double *coefs = (double *)ws;
     hipLaunchKernelGGL(k_gradient_coefs, dim3(grid_blocks((long long)plane)), dim3(kThreads), 0, hs, cilm, cilm_d1,
                        cilm_d2, sh.Lc, coefs);
@@ -1610,7 +1690,7 @@ int shtg_shmultiply(double *cilmout, int out_d1, int out_d2, const double *cilm1
     hipStream_t hs = (hipStream_t)stream;
     const size_t ng = (size_t)nlat * nlong;
     void *ws = nullptr;
-    if (!HIP_OK(hipMallocAsync(&ws, 2 * ng * sizeof(double), hs))) return ST_ALLOC;
+    if (!HIP_OK(pool_alloc(&ws, 2 * ng * sizeof(double), hs))) return ST_ALLOC;
     double *g1 = (double *)ws, *g2 = g1 + ng;
     // each factor synthesised on the product's grid from its leading
     // (lmax+1)^2 block (src/SHMultiply.f95:330-350)

@@ -267,23 +267,30 @@ def other_configs(dev, reps=5):
     ll1 = torch.tensor([l * (l + 1.0) for l in range(L + 1)], dtype=torch.float64, device=dev)[None, :, None]
     coef = [0.5 * t for t in truth]
     grids = [torch.empty(batch.grid_shape(i), dtype=torch.float64, device=dev) for i in range(nfits)]
-    res = [torch.empty_like(g) for g in grids]
     grads = [torch.empty_like(t) for t in truth]
+    w2 = 2.0 * wlat[:, None]
+    reg = [2e-6 * ll1 * torch.ones_like(t) for t in truth]
 
     def lsq_grad():
         batch.makegrid(coef, out=grids)
-        for r, g, d in zip(res, grids, data):
-            torch.sub(g, d, out=r)
-            r.mul_(2.0 * wlat[:, None])
-        batch.makegrid_adjoint(res, out=grads)
-        for gr, cf in zip(grads, coef):
-            gr.add_(2e-6 * ll1 * cf)
+        torch._foreach_sub_(grids, data)          # residuals, in place of the grids
+        torch._foreach_mul_(grids, [w2] * nfits)  # 2 w (S c - d)
+        batch.makegrid_adjoint(grids, out=grads)
+        torch._foreach_addcmul_(grads, reg, coef)  # + 2e-6 l(l+1) c
     ms = _timed(lsq_grad, reps)
+    # the same gradient through torch.autograd (whose backward is the adjoint)
+    from shtools_amd import autograd as AG
+    cg = [c.clone().requires_grad_(True) for c in coef]
+    gg = AG.batch_make_grid_dh(batch, cg)
+    loss = sum((wlat[:, None] * (g - d) ** 2).sum() + 1e-6 * (ll1 * c * c).sum() for g, d, c in zip(gg, data, cg))
+    ref = torch.autograd.grad(loss, cg)
+    gerr = max(float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) for a, b in zip(grads, ref))
     leg, fft = dh_flops(L, 2)
     rec("lsq_16fits_L511", ms, nfits * 2 * (leg + fft),
-        {"gradients_per_s": nfits / (ms * 1e-3), "note": "one gradient of all 16 fits: batched synthesis, "
-                                                         "weighted residual, batched adjoint (analysis kernels)"})
-    del batch, truth, data, coef, grids, res, grads
+        {"gradients_per_s": nfits / (ms * 1e-3), "gradient_vs_autograd_err": gerr,
+         "note": "one gradient of all 16 fits: batched synthesis, weighted residual, batched adjoint "
+                 "(analysis kernels)"})
+    del batch, truth, data, coef, grids, grads, cg, gg, ref
     # high-degree batch: 8 fields lmax ~ U{2047..2799}, filtered synthesis + analysis
     Ls, cs = high_degree_workload()
     batch = DHBatch(Ls, sampling=2, norm=1, csphase=1, device=dev)

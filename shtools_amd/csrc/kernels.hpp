@@ -554,13 +554,13 @@ __global__ __launch_bounds__(kWave) void k_synth_leg(const double *__restrict__ 
                                                      const double2 *__restrict__ cpack, int Lc, int P, int nsb,
                                                      double4 *__restrict__ fbuf,
                                                      const double *__restrict__ ringr2,
-                                                     const int *__restrict__ smin)
+                                                     const int *__restrict__ smin, int kb)
 {
     const int m = blockIdx.x / nsb;
     const int sb = blockIdx.x - m * nsb;
-    const int slot0 = (smin ? smin[m] : 0) + sb * kSynthKMax * kWave;
+    const int slot0 = (smin ? smin[m] : 0) + sb * kb * kWave;
     if (slot0 >= P) return;
-    const int kk = min(kSynthKMax, (P - slot0 + kWave - 1) / kWave);
+    const int kk = min(kb, (P - slot0 + kWave - 1) / kWave);
     if (ringr2)
         synth_leg_dispatch<kSynthKMax, true>(kk, alpha + tri_offd(m, Lt), seed[m], ringz, cpack + tri_offd(m, Lc),
                                              Lc - m, P, slot0, fbuf + (size_t)m * P, ringr2);

@@ -366,22 +366,47 @@ static int current_device()
 // Ring pairs per lane of an analysis wave for P slots (1 .. kAnalKMax): the
 // fewest passes of at most kAnalKMax * 64 slots, with lanes packed as tightly
 // as those passes allow.
-static int anal_lanes_per_slot(int P)
+static int anal_lanes_per_slot(int P, int kmax = kAnalKMax)
 {
-    const int per = kAnalKMax * kWave;
+    const int per = kmax * kWave;
     const int np = std::max(1, (P + per - 1) / per);
     return std::max(1, (P + np * kWave - 1) / (np * kWave));
+}
+
+// Waves a plain transform's Legendre launch aims for (8 per SIMD), and the
+// fewest ring pairs per lane the plain kernels go down to for it.
+constexpr int kLegWaves = 8192;
+constexpr int kPlainSynthKMin = 1;
+constexpr int kPlainAnalKMin = 4;
+
+// Ring pairs per lane of a plain synthesis: kSynthKMax, or fewer when the
+// orders times the blocks of ring pairs would give the launch fewer than
+// kLegWaves waves.
+static int synth_lanes(int Lc, int P)
+{
+    for (int kb = kSynthKMax; kb > kPlainSynthKMin; --kb)
+        if ((long long)(Lc + 1) * ((P + kb * kWave - 1) / (kb * kWave)) >= kLegWaves) return kb;
+    return kPlainSynthKMin;
+}
+
+// Largest ring pairs per lane of a plain analysis over nslots slots: fewer
+// (down to kPlainAnalKMin) when the orders alone give fewer than kLegWaves
+// waves, so that the slots split into more blocks.
+static int plain_anal_kmax(int Lc, int nslots)
+{
+    const int want = (kLegWaves + Lc) / (Lc + 1);  // blocks per order wanted
+    const int k = (nslots + want * kWave - 1) / (want * kWave);
+    return std::max(kPlainAnalKMin, std::min(kAnalKMax, k));
 }
 
 // Slot blocks per order of a plain analysis whose waves take `passes`
 // passes of K*64 ring pairs: one (a wave per order) for a single pass or
 // when the orders alone give the GPU enough waves, else up to one block per
-// pass, as many as bring the launch to about kAnalWaves waves.
-constexpr int kAnalWaves = 8192;
+// pass, as many as bring the launch to about kLegWaves waves.
 static int anal_blocks(int Lc, int passes)
 {
-    if (passes <= 1 || Lc + 1 >= kAnalWaves) return 1;
-    return std::max(1, std::min(passes, (kAnalWaves + Lc) / (Lc + 1)));
+    if (passes <= 1 || Lc + 1 >= kLegWaves) return 1;
+    return std::max(1, std::min(passes, (kLegWaves + Lc) / (Lc + 1)));
 }
 
 // the analysis kernels by ring pairs per lane
@@ -515,9 +540,10 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
     }
     {
         ProfScope ps(SHTG_K_SYNTH_LEG, st);
-        const int nsb = (g.P + kSynthKMax * kWave - 1) / (kSynthKMax * kWave);
+        const int kb = synth_lanes(Lc, g.P);
+        const int nsb = (g.P + kb * kWave - 1) / (kb * kWave);
         hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nsb), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, rz, cpack,
-                           Lc, g.P, nsb, fbuf, ring.r2, smin);
+                           Lc, g.P, nsb, fbuf, ring.r2, smin, kb);
     }
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
@@ -554,7 +580,7 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
     const size_t nws = lds_fft ? 0 : (size_t)nwg * ws_stride(g.fft.m);
     // the Legendre sums of each order in blocks of slots (one wave each) when
     // one wave per order would leave the GPU short of waves
-    const int K = anal_lanes_per_slot(g.P - s0);
+    const int K = anal_lanes_per_slot(g.P - s0, plain_anal_kmax(Lc, g.P - s0));
     const int passes = (g.P - s0 + K * kWave - 1) / (K * kWave);
     const int nblk = anal_blocks(Lc, passes);
     const int bslots = (passes + nblk - 1) / nblk * K * kWave;
@@ -618,11 +644,12 @@ static int launch_synthesis_c(const Geometry &g, const DevRecurrence &rt, const 
     }
     {
         ProfScope ps(SHTG_K_SYNTH_LEG, st);
-        const int nsb = (g.P + kSynthKMax * kWave - 1) / (kSynthKMax * kWave);
+        const int kb = synth_lanes(Lc, g.P);
+        const int nsb = (g.P + kb * kWave - 1) / (kb * kWave);
         hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nsb), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, cp_p,
-                           Lc, g.P, nsb, fb_p, (const double *)nullptr, (const int *)nullptr);
+                           Lc, g.P, nsb, fb_p, (const double *)nullptr, (const int *)nullptr, kb);
         hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nsb), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, cp_m,
-                           Lc, g.P, nsb, fb_m, (const double *)nullptr, (const int *)nullptr);
+                           Lc, g.P, nsb, fb_m, (const double *)nullptr, (const int *)nullptr, kb);
     }
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);

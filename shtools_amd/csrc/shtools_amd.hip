@@ -289,9 +289,10 @@ static int current_device();
 
 // Intermediates of the plain launchers: one buffer per device, reused by
 // consecutive calls (the callers hold g_mu, so calls do not interleave on
-// the host).  A call on another stream than the buffer's last user first
-// makes its stream wait, on the device, for that user's work (an event
-// recorded at release).  A larger request replaces the buffer by one of the
+// the host).  Every call first makes its stream wait, on the device, for
+// the previous user's work (an event recorded at release; on the same
+// stream the wait is already met, and a stream handle reused after a
+// destroy is covered too).  A larger request replaces the buffer by one of the
 // next power-of-two size, the old one freed in stream order after its last
 // use.  Under graph capture the call takes its own block from the pool
 // instead, since a later growth would free memory the graph still uses.
@@ -300,7 +301,6 @@ static int current_device();
 struct WsCache {
     void *buf = nullptr;
     size_t size = 0;
-    hipStream_t last = nullptr;
     hipEvent_t ev = nullptr;
     bool used = false;
 };
@@ -321,7 +321,7 @@ static int ws_acquire(WsLease &l, size_t bytes, hipStream_t st)
     if (dev < 0) return ST_OTHER;
     WsCache &c = g_ws[dev];
     if (!c.ev && !HIP_OK(hipEventCreateWithFlags(&c.ev, hipEventDisableTiming))) return ST_OTHER;
-    if (c.used && c.last != st && !HIP_OK(hipStreamWaitEvent(st, c.ev, 0))) return ST_OTHER;
+    if (c.used && !HIP_OK(hipStreamWaitEvent(st, c.ev, 0))) return ST_OTHER;
     if (bytes > c.size) {
         size_t b = 1 << 20;
         while (b < bytes) b <<= 1;
@@ -344,7 +344,6 @@ static void ws_release(const WsLease &l, hipStream_t st)
     }
     WsCache &c = g_ws[current_device()];
     (void)hipEventRecord(c.ev, st);
-    c.last = st;
     c.used = true;
 }
 

@@ -51,7 +51,7 @@ constexpr int kRedStride = 65; // LDS row stride (doubles) of the analysis reduc
 // Coefficient tiles of kTL degrees x kTM orders, transposed through LDS so
 // that both the [l][m] arrays and the per-order packed arrays are accessed
 // contiguously.
-constexpr int kTM = 32, kTL = 64;
+constexpr int kTM = 64, kTL = 32;
 typedef double2 CoefTile[kTL][kTM + 1];
 
 // cpack[tri(m) + l - m] = (C s, S s) for the tile's m <= l <= L.

@@ -123,7 +123,13 @@ static const DevRecurrence *get_recurrence(int dev, int norm, int L)
     r.s = dev_upload(t.s);
     r.post = dev_upload(t.post);
     r.post_c = dev_upload(t.post_c);
-    if (!r.alpha || !r.s || !r.post || !r.post_c) { r = DevRecurrence{}; return nullptr; }
+    if (!r.alpha || !r.s || !r.post || !r.post_c) {
+        // nothing has been enqueued on these yet; a later call retries
+        for (void *q : {(void *)r.alpha, (void *)r.s, (void *)r.post, (void *)r.post_c})
+            if (q) (void)hipFree(q);
+        r = DevRecurrence{};
+        return nullptr;
+    }
     r.L = Lb;
     return &r;
 }
@@ -146,7 +152,27 @@ constexpr int kWsGroups = 512;
 static bool fft_in_lds(int m) { return (size_t)fft_lds_slots(m) * sizeof(double2) <= kMaxLds; }
 static long long ws_stride(int m) { return ((long long)fft_lds_slots(m) + 15) / 16 * 16; }
 
+static bool upload_fft_arrays(const FftPlanHost &h, FftDev &d);
+
+// Device arrays of an FFT plan that no work has been enqueued on.
+static void free_fft(FftDev &d)
+{
+    for (const void *q : {(const void *)d.tw, (const void *)d.twst, (const void *)d.perm, (const void *)d.chirp,
+                          (const void *)d.bhat})
+        if (q) (void)hipFree(const_cast<void *>(q));
+    if (d.bhat2 && d.bhat2 != d.bhat) (void)hipFree(const_cast<double2 *>(d.bhat2));
+    std::memset(&d, 0, sizeof(d));
+}
+
+// Plan upload; on failure nothing stays allocated and d is zeroed.
 static bool upload_fft(const FftPlanHost &h, FftDev &d)
+{
+    if (upload_fft_arrays(h, d)) return true;
+    free_fft(d);
+    return false;
+}
+
+static bool upload_fft_arrays(const FftPlanHost &h, FftDev &d)
 {
     std::memset(&d, 0, sizeof(d));
     d.n = h.n;
@@ -171,6 +197,17 @@ static bool upload_fft(const FftPlanHost &h, FftDev &d)
         if (!d.chirp || !d.bhat || !d.bhat2) return false;
     }
     return true;
+}
+
+// Device arrays of a geometry whose construction failed (nothing has been
+// enqueued on them yet).
+static void free_geometry(Geometry &g)
+{
+    for (const void *q : {(const void *)g.z, (const void *)g.u, (const void *)g.w, (const void *)g.rows,
+                          (const void *)g.ones, (const void *)g.kmax, (const void *)g.smin})
+        if (q) (void)hipFree(const_cast<void *>(q));
+    free_fft(g.fft);  // zeroed if upload_fft failed or never ran
+    g = Geometry{};
 }
 
 // lc: the FFT plan's pruning degree (-1: none); L: the grid's degree.
@@ -199,7 +236,7 @@ static bool finish_geometry(Geometry &g, const std::vector<double> &z, const std
 static Geometry *get_dh(int dev, int L, int sampling)
 {
     auto &slot = g_dh[{dev, L, sampling}];
-    if (slot) return slot->ok ? slot.get() : nullptr;
+    if (slot) return slot.get();
     slot.reset(new Geometry());
     Geometry &g = *slot;
     const int n = 2 * L + 2;
@@ -224,7 +261,11 @@ static Geometry *get_dh(int dev, int L, int sampling)
         w[2 * s] = c * aj[s];
         w[2 * s + 1] = (s == 0 || s == n / 2) ? 0.0 : c * aj[n - s];
     }
-    if (!finish_geometry(g, z, u, w, rows, L, L)) return nullptr;
+    if (!finish_geometry(g, z, u, w, rows, L, L)) {
+        free_geometry(g);
+        slot.reset();  // a later call retries
+        return nullptr;
+    }
     return &g;
 }
 
@@ -244,7 +285,7 @@ static Geometry *get_glq(int dev, int L, const double *zero, const double *wq)
     uint64_t h = fnv1a(zero, sizeof(double) * nlat);
     h = wq ? fnv1a(wq, sizeof(double) * nlat, h) : fnv1a("nw", 2, h);
     auto &slot = g_glq[{dev, L, h}];
-    if (slot) return slot->ok ? slot.get() : nullptr;
+    if (slot) return slot.get();
     slot.reset(new Geometry());
     Geometry &g = *slot;
     const double pi = std::acos(-1.0);
@@ -263,7 +304,11 @@ static Geometry *get_glq(int dev, int L, const double *zero, const double *wq)
         w[2 * s] = wq ? 2.0 * pi * wq[s] / (double)g.nlong : 0.0;
         w[2 * s + 1] = (wq && !eq) ? 2.0 * pi * wq[nlat - 1 - s] / (double)g.nlong : 0.0;
     }
-    if (!finish_geometry(g, z, u, w, rows, -1, L)) return nullptr;
+    if (!finish_geometry(g, z, u, w, rows, -1, L)) {
+        free_geometry(g);
+        slot.reset();  // a later call retries
+        return nullptr;
+    }
     return &g;
 }
 
@@ -791,14 +836,7 @@ static void batch_free(BatchPlan *b)
     (void)hipFree(b->kmax);
     (void)hipFree(b->smin);
     (void)hipFree(b->plans);
-    for (auto &p : b->hplans) {
-        (void)hipFree((void *)p.tw);
-        (void)hipFree((void *)p.twst);
-        (void)hipFree((void *)p.perm);
-        (void)hipFree((void *)p.chirp);
-        if (p.bhat2 != p.bhat) (void)hipFree((void *)p.bhat2);
-        (void)hipFree((void *)p.bhat);
-    }
+    for (auto &p : b->hplans) free_fft(p);
     (void)hipFree(b->cbuf);
     (void)hipFree(b->fbuf);
     for (auto &t : b->slot) (void)hipFree((void *)t.dev);

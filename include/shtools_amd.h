@@ -174,6 +174,10 @@ int shtg_shexpand_glq_adjoint(const double *cilm_bar, int cilm_d1, int cilm_d2, 
 typedef struct shtg_batch_plan shtg_batch_plan;
 int shtg_batch_create_dh(int nfields, const int *lmax, int sampling, int norm, int csphase,
                          shtg_batch_plan **plan);
+/* The same with a sampling (1 or 2) per field: grid[i] is then
+ * [2lmax[i]+2][sampling[i](2lmax[i]+2)]. */
+int shtg_batch_create_dh_fields(int nfields, const int *lmax, const int *sampling, int norm, int csphase,
+                                shtg_batch_plan **plan);
 int shtg_batch_makegrid_dh(shtg_batch_plan *plan, const double *const *cilm, double *const *grid,
                            void *stream);
 int shtg_batch_shexpand_dh(shtg_batch_plan *plan, const double *const *grid, double *const *cilm,

@@ -51,6 +51,8 @@ SIGNATURES = {
     "shtg_glqgridcoord": [_c_int, _c_int, _dp, _dp],
     "shtg_plm": [_c_int, _c_int, _c_int, ctypes.c_double, _dp],
     "shtg_batch_create_dh": [_c_int, ctypes.POINTER(_c_int), _c_int, _c_int, _c_int, ctypes.POINTER(_vp)],
+    "shtg_batch_create_dh_fields": [_c_int, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), _c_int, _c_int,
+                                    ctypes.POINTER(_vp)],
     "shtg_batch_makegrid_dh": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp],
     "shtg_batch_shexpand_dh": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp],
     "shtg_batch_makegrid_dh_adjoint": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp],

@@ -20,17 +20,27 @@ def _ptr_array(tensors):
 
 
 class DHBatch:
+    """Plan for the fields of degrees ``lmax``; ``sampling`` is one value for
+    every field or a sequence with one per field."""
+
     def __init__(self, lmax, sampling=2, norm=1, csphase=1, device=None):
         import torch
         self.torch = torch
         self.lmax = [int(L) for L in lmax]
-        self.sampling, self.norm, self.csphase = int(sampling), int(norm), int(csphase)
+        if isinstance(sampling, (list, tuple)):
+            if len(sampling) != len(self.lmax):
+                raise SHToolsError(_status_message(1))
+            self.sampling = [int(x) for x in sampling]
+        else:
+            self.sampling = [int(sampling)] * len(self.lmax)
+        self.norm, self.csphase = int(norm), int(csphase)
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         arr = (ctypes.c_int * len(self.lmax))(*self.lmax)
+        samp = (ctypes.c_int * len(self.lmax))(*self.sampling)
         plan = ctypes.c_void_p()
         with torch.cuda.device(self.device):
-            st = _lib.lib().shtg_batch_create_dh(len(self.lmax), arr, self.sampling, self.norm, self.csphase,
-                                                 ctypes.byref(plan))
+            st = _lib.lib().shtg_batch_create_dh_fields(len(self.lmax), arr, samp, self.norm, self.csphase,
+                                                        ctypes.byref(plan))
         if st != 0:
             raise SHToolsError(_status_message(st))
         self._plan = plan
@@ -47,7 +57,7 @@ class DHBatch:
 
     def grid_shape(self, i):
         n = 2 * self.lmax[i] + 2
-        return (n, self.sampling * n)
+        return (n, self.sampling[i] * n)
 
     def cilm_shape(self, i):
         L = self.lmax[i]

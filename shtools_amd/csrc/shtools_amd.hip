@@ -773,7 +773,7 @@ struct FftGroup {
 };
 
 struct BatchPlan {
-    int dev = -1, nf = 0, sampling = 2, norm = 1, csphase = 1, Lmax = 0;
+    int dev = -1, nf = 0, norm = 1, csphase = 1, Lmax = 0;
     std::vector<int> L;
     BField *fields = nullptr;
     int4 *items_pack = nullptr;
@@ -844,14 +844,15 @@ static void batch_free(BatchPlan *b)
     delete b;
 }
 
-static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm, int csphase, BatchPlan **out)
+// sampling: per field (1 or 2).
+static int batch_create_dh(int nfields, const int *lmax, const int *sampling, int norm, int csphase,
+                           BatchPlan **out)
 {
-    if (nfields <= 0 || !lmax || !out) return ST_DIMS;
-    if (sampling != 1 && sampling != 2) return ST_BOUNDS;
+    if (nfields <= 0 || !lmax || !sampling || !out) return ST_DIMS;
     if (norm < 1 || norm > 4 || (csphase != 1 && csphase != -1)) return ST_BOUNDS;
     int Lmax = 0;
     for (int i = 0; i < nfields; ++i) {
-        if (lmax[i] < 0) return ST_BOUNDS;
+        if (lmax[i] < 0 || (sampling[i] != 1 && sampling[i] != 2)) return ST_BOUNDS;
         Lmax = std::max(Lmax, lmax[i]);
     }
     const int dev = current_device();
@@ -860,7 +861,6 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
     std::unique_ptr<BatchPlan, void (*)(BatchPlan *)> b(new BatchPlan(), batch_free);
     b->dev = dev;
     b->nf = nfields;
-    b->sampling = sampling;
     b->norm = norm;
     b->csphase = csphase;
     b->Lmax = Lmax;
@@ -869,18 +869,21 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
     const DevSeeds *sd = get_seeds(dev, norm, csphase, Lmax);
     if (!rt || !sd) return ST_ALLOC;
 
-    // ring geometry and FFT plans per distinct degree
+    // ring geometry and FFT plans per distinct (degree, sampling)
     const double pi = std::acos(-1.0);
-    std::map<int, std::pair<int, int>> geo_of;  // L -> (geo offset, plan index)
-    std::map<int, int> sm_of;                   // L -> offset of its smin row
-    std::map<int, std::vector<int>> smin_of;    // L -> smin row
+    auto key_of = [&](int i) { return std::make_pair(lmax[i], sampling[i]); };
+    typedef std::pair<int, int> Key;
+    std::map<Key, std::pair<int, int>> geo_of;  // -> (geo offset, plan index)
+    std::map<Key, int> sm_of;                   // -> offset of its smin row
+    std::map<Key, std::vector<int>> smin_of;    // -> smin row
     std::vector<double> z, u, w;
     std::vector<int> rows, kmax, smin;
     std::vector<size_t> plan_lds;
     for (int i = 0; i < nfields; ++i) {
         const int L = lmax[i];
-        if (geo_of.count(L)) continue;
-        const int n = 2 * L + 2, nlong = sampling * n, P = n / 2 + 1;
+        const Key key = key_of(i);
+        if (geo_of.count(key)) continue;
+        const int n = 2 * L + 2, nlong = sampling[i] * n, P = n / 2 + 1;
         const int off = (int)z.size();
         std::vector<double> aj;
         dh_weights(n, aj);
@@ -897,15 +900,15 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
         const std::vector<double> zl(z.begin() + off, z.end()), ul(u.begin() + off, u.end());
         const std::vector<int> km = polar_kmax(zl, ul, L);
         kmax.insert(kmax.end(), km.begin(), km.end());
-        smin_of[L] = polar_smin(km, L);
-        sm_of[L] = (int)smin.size();
-        smin.insert(smin.end(), smin_of[L].begin(), smin_of[L].end());
+        smin_of[key] = polar_smin(km, L);
+        sm_of[key] = (int)smin.size();
+        smin.insert(smin.end(), smin_of[key].begin(), smin_of[key].end());
         FftPlanHost fp = make_fft_plan(nlong, L);
         FftDev fd;
         if (!upload_fft(fp, fd)) return ST_OTHER;
         b->hplans.push_back(fd);
         plan_lds.push_back((size_t)fft_lds_slots(fp.m) * sizeof(double2));
-        geo_of[L] = {off, (int)b->hplans.size() - 1};
+        geo_of[key] = {off, (int)b->hplans.size() - 1};
     }
     b->z = dev_upload(z);
     b->u = dev_upload(u);
@@ -926,12 +929,12 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
         f.L = L;
         f.P = n / 2 + 1;
         f.nlat = n;
-        f.nlong = sampling * n;
-        f.geo = geo_of[L].first;
-        f.fft = geo_of[L].second;
+        f.nlong = sampling[i] * n;
+        f.geo = geo_of[key_of(i)].first;
+        f.fft = geo_of[key_of(i)].second;
         f.coff = coff;
         f.foff = foff;
-        f.sm = sm_of[L];
+        f.sm = sm_of[key_of(i)];
         coff += tri_size(L);
         foff += (long long)(L + 1) * f.P;
     }
@@ -950,7 +953,7 @@ static int batch_create_dh(int nfields, const int *lmax, int sampling, int norm,
                 wu.push_back({(double)f.L, i, m0, l0});
                 if (l0 + kTL - 1 >= m0) wp.push_back({(double)f.L, i, m0, l0});
             }
-        const std::vector<int> &sm = smin_of[f.L];
+        const std::vector<int> &sm = smin_of[key_of(i)];
         for (int m = 0; m <= f.L; ++m) {
             // blocks of kSynthKMax*64 slots from the order's first slot with
             // non-negligible functions; the last one packs its lanes tightly
@@ -1872,6 +1875,17 @@ const char *shtg_version(void) { return "shtools_amd 0.1 (gfx950)"; }
 
 int shtg_batch_create_dh(int nfields, const int *lmax, int sampling, int norm, int csphase,
                          shtg_batch_plan **plan)
+{
+    if (nfields <= 0) {
+        if (plan) *plan = nullptr;
+        return ST_DIMS;
+    }
+    const std::vector<int> samp((size_t)nfields, sampling);
+    return shtg_batch_create_dh_fields(nfields, lmax, samp.data(), norm, csphase, plan);
+}
+
+int shtg_batch_create_dh_fields(int nfields, const int *lmax, const int *sampling, int norm, int csphase,
+                                shtg_batch_plan **plan)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     BatchPlan *b = nullptr;

@@ -108,6 +108,18 @@ def test_argument_checks_before_any_device_work():
     assert lib.shtg_makegrid_glq(dummy, 5, 5, 4, dp(), 5, 1, 1, 4, 0, dummy, 5, 9, nul) == 5
     assert lib.shtg_makegrid_glq(dummy, 5, 5, 4, B._dptr(z), 4, 1, 1, 4, 0, dummy, 5, 9, nul) == 1
     assert lib.shtg_shexpand_glq(dummy, 5, 9, B._dptr(z), 4, B._dptr(z), 5, 1, 1, 4, dummy, 5, 5, nul) == 1
+    # batched plans: sampling (uniform or per field), norm, csphase, degree, field count
+    ci = ctypes.c_int
+    Ls = (ci * 3)(4, 7, 4)
+    plan = ctypes.c_void_p(16)
+    assert lib.shtg_batch_create_dh(3, Ls, 3, 1, 1, ctypes.byref(plan)) == 2 and not plan.value
+    assert lib.shtg_batch_create_dh(0, Ls, 2, 1, 1, ctypes.byref(plan)) == 1 and not plan.value
+    assert lib.shtg_batch_create_dh_fields(3, Ls, (ci * 3)(1, 2, 0), 1, 1, ctypes.byref(plan)) == 2
+    assert lib.shtg_batch_create_dh_fields(3, Ls, (ci * 3)(1, 2, 1), 5, 1, ctypes.byref(plan)) == 2
+    assert lib.shtg_batch_create_dh_fields(3, Ls, (ci * 3)(1, 2, 1), 1, 0, ctypes.byref(plan)) == 2
+    assert lib.shtg_batch_create_dh_fields(3, (ci * 3)(4, -1, 4), (ci * 3)(1, 2, 1), 1, 1, ctypes.byref(plan)) == 2
+    assert lib.shtg_batch_create_dh_fields(3, Ls, None, 1, 1, ctypes.byref(plan)) == 1
+    assert not plan.value
 
 
 def test_normal_gravity_matches_oracle():

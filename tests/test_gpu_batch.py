@@ -91,3 +91,46 @@ def test_repeated_degrees_scales_and_zero_field_match_plain(torch, norm, cs, sam
     assert float(back[3].abs().max()) == 0.0
     for i in (0, 5, 9, 22):
         assert coef_err(back[i].cpu().numpy(), cs_host[i]) <= 1e-11
+
+
+def test_sampling_per_field_matches_plain(torch):
+    """A plan mixing DH1 and DH2 fields (the same degree under both
+    samplings too): every field equals the plain path of its own sampling
+    (bitwise, except the synthesis adjoint: see below) and the oracle to
+    1e-12."""
+    from shtools_amd import backend as B
+    from shtools_amd.batch import DHBatch
+    print("case batch sampling per field", flush=True)
+    Ls = [40, 40, 17, 96, 5, 96, 63, 0, 130]
+    ss = [1, 2, 2, 1, 1, 2, 1, 2, 1]
+    rng = np.random.default_rng(77)
+    cs_host = [rand_cilm(rng, L) for L in Ls]
+    cilm = [torch.from_numpy(c).cuda() for c in cs_host]
+    b = DHBatch(Ls, sampling=ss, norm=1, csphase=-1)
+    grids = b.makegrid(cilm)
+    back = b.expand(grids)
+    gbar = [torch.from_numpy(rng.standard_normal(tuple(g.shape))).cuda() for g in grids]
+    adj_s = b.makegrid_adjoint(gbar)
+    adj_a = b.expand_adjoint(cilm)
+    torch.cuda.synchronize()
+    for i, (L, s) in enumerate(zip(Ls, ss)):
+        assert tuple(grids[i].shape) == (2 * L + 2, s * (2 * L + 2))
+        g_plain = B.MakeGridDH(cilm[i], sampling=s, norm=1, csphase=-1)
+        c_plain = B.SHExpandDH(grids[i], sampling=s, norm=1, csphase=-1)
+        s_plain = B.MakeGridDH_adjoint(gbar[i], tuple(cilm[i].shape), sampling=s, norm=1, csphase=-1)
+        a_plain = B.SHExpandDH_adjoint(cilm[i], tuple(grids[i].shape), sampling=s, norm=1, csphase=-1)
+        torch.cuda.synchronize()
+        assert torch.equal(g_plain, grids[i]), (i, L, s)
+        assert torch.equal(c_plain, back[i]), (i, L, s)
+        # The synthesis adjoint sums over every slot, the pole's included; at
+        # L = 63 (65 slots) the plain launch takes them as 2 ring pairs per
+        # lane in one pass and the batched one, whose work items are sized
+        # for the 64 slots of the analysis, as 1 per lane in two passes: the
+        # same sums in another order.
+        assert coef_err(adj_s[i].cpu().numpy(), s_plain.cpu().numpy()) <= 1e-14, (i, L, s)
+        assert torch.equal(a_plain, adj_a[i]), (i, L, s)
+        g_ref = oracle.MakeGridDH(cs_host[i], sampling=s, norm=1, csphase=-1)
+        assert grid_err(grids[i].cpu().numpy(), g_ref) <= 1e-12, (i, L, s)
+        assert coef_err(back[i].cpu().numpy(), cs_host[i]) <= 1e-11, (i, L, s)
+    with pytest.raises(B.SHToolsError):
+        DHBatch(Ls, sampling=ss[:-1])

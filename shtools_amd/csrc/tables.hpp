@@ -31,43 +31,41 @@ struct RecurrenceTables {
     std::vector<double> post_c; // tri: the same for the complex analysis
 };
 
-// ff1/ff2 of the reference for one order m, l = m..L (index l).
-// src/MakeGridDH.f95:321-381 (norm 1,4 / 2 / 3 branches).
-inline void ref_ff_column(int norm, int m, int L, const std::vector<double> &sqr,
-                          std::vector<double> &ff1, std::vector<double> &ff2)
+// Recursion coefficients of one order m, l = m+1..L (index l), in extended
+// precision: P_l = ff1 z P_{l-1} - ff2 P_{l-2} with the reference's
+// coefficients (src/MakeGridDH.f95:321-381: norm 1,4 / 2 / 3 branches,
+// the m = l-1 entry with ff2 = 0), each from exact integer ratios with
+// one long-double square root.
+inline void ff_column(int norm, int m, int L, std::vector<long double> &ff1, std::vector<long double> &ff2)
 {
-    ff1.assign(L + 2, 0.0);
-    ff2.assign(L + 2, 0.0);
+    typedef long double R;
+    ff1.assign(L + 2, 0.0L);
+    ff2.assign(L + 2, 0.0L);
     for (int l = m + 1; l <= L; ++l) {
-        double f1 = 0.0, f2 = 0.0;
+        const R lp = (R)(l + m), lm = (R)(l - m), t1 = (R)(2 * l - 1), t3 = (R)(2 * l - 3), t2 = (R)(2 * l + 1);
+        R f1 = 0.0L, f2 = 0.0L;
         if (norm == 1 || norm == 4) {
             if (m == 0) {
-                if (l == 1) { f1 = sqr[3]; f2 = 0.0; }
+                if (l == 1) f1 = std::sqrt(3.0L);
                 else {
-                    f1 = sqr[2 * l - 1] * sqr[2 * l + 1] / (double)l;
-                    f2 = (double)(l - 1) * sqr[2 * l + 1] / sqr[2 * l - 3] / (double)l;
+                    f1 = std::sqrt(t1 * t2) / (R)l;
+                    f2 = (R)(l - 1) * std::sqrt(t2 / t3) / (R)l;
                 }
-            } else if (m <= l - 2) {
-                f1 = sqr[2 * l + 1] * sqr[2 * l - 1] / sqr[l + m] / sqr[l - m];
-                f2 = sqr[2 * l + 1] * sqr[l - m - 1] * sqr[l + m - 1] / sqr[2 * l - 3] / sqr[l + m] / sqr[l - m];
-            } else {  // m = l - 1
-                f1 = sqr[2 * l + 1] * sqr[2 * l - 1] / sqr[l + m] / sqr[l - m];
-                f2 = 0.0;
+            } else {
+                f1 = std::sqrt(t2 * t1 / (lp * lm));
+                if (m <= l - 2) f2 = std::sqrt(t2 * (lm - 1) * (lp - 1) / (t3 * lp * lm));
             }
         } else if (norm == 2) {
             if (m == 0) {
-                if (l == 1) { f1 = 1.0; f2 = 0.0; }
-                else { f1 = (double)(2 * l - 1) / (double)l; f2 = (double)(l - 1) / (double)l; }
-            } else if (m <= l - 2) {
-                f1 = (double)(2 * l - 1) / sqr[l + m] / sqr[l - m];
-                f2 = sqr[l - m - 1] * sqr[l + m - 1] / sqr[l + m] / sqr[l - m];
+                if (l == 1) f1 = 1.0L;
+                else { f1 = t1 / (R)l; f2 = (R)(l - 1) / (R)l; }
             } else {
-                f1 = (double)(2 * l - 1) / sqr[l + m] / sqr[l - m];
-                f2 = 0.0;
+                f1 = t1 / std::sqrt(lp * lm);
+                if (m <= l - 2) f2 = std::sqrt((lm - 1) * (lp - 1) / (lp * lm));
             }
         } else {  // norm 3
-            if (m == 0) { f1 = (double)(2 * l - 1) / (double)l; f2 = (double)(l - 1) / (double)l; }
-            else { f1 = (double)(2 * l - 1) / (double)(l - m); f2 = (double)(l + m - 1) / (double)(l - m); }
+            if (m == 0) { f1 = t1 / (R)l; f2 = (R)(l - 1) / (R)l; }
+            else { f1 = t1 / lm; f2 = (lp - 1) / lm; }
         }
         ff1[l] = f1;
         ff2[l] = f2;
@@ -110,25 +108,30 @@ inline RecurrenceTables build_recurrence(int norm, int L)
     t.s.assign(n, 0.0);
     t.post.assign(n, 0.0);
     t.post_c.assign(n, 0.0);
-    std::vector<double> sqr(2 * L + 4, 0.0);
-    for (int k = 1; k < (int)sqr.size(); ++k) sqr[k] = std::sqrt((double)k);
     std::vector<double> epi, epi_c;
     epilogue_factors(norm, L, epi);
     epilogue_factors(norm, L, epi_c, true);
-    std::vector<double> ff1, ff2;
+    // The running products s and the ratios alpha are formed in long double
+    // and rounded once: alpha and s then carry at most half an ulp each.
+    // Rounded along the way in double (about 30 ulps after a few thousand
+    // degrees), their perturbations add up near the poles, where the
+    // recursion's solutions hardly separate: at z = cos(pi/5600), degree
+    // 2800, order 0, the functions were 1.5e-11 off against 2.3e-12 now (the
+    // reference's own double recursion: 1.2e-12).
+    std::vector<long double> ff1, ff2;
     for (int m = 0; m <= L; ++m) {
-        ref_ff_column(norm, m, L, sqr, ff1, ff2);
+        ff_column(norm, m, L, ff1, ff2);
         int64_t o = tri_off(m, L);
-        double s_lm2 = 1.0, s_lm1 = 1.0;  // s_{l-2}, s_{l-1}
+        long double s_lm2 = 1.0L, s_lm1 = 1.0L;  // s_{l-2}, s_{l-1}
         for (int l = m; l <= L; ++l) {
-            double s, a;
-            if (l == m) { s = 1.0; a = 0.0; }
-            else if (l == m + 1) { s = 1.0; a = ff1[l]; }
+            long double s, a;
+            if (l == m) { s = 1.0L; a = 0.0L; }
+            else if (l == m + 1) { s = 1.0L; a = ff1[l]; }
             else { s = ff2[l] * s_lm2; a = ff1[l] * s_lm1 / s; }
-            t.alpha[o + (l - m)] = a;
-            t.s[o + (l - m)] = s;
-            t.post[o + (l - m)] = s * epi[(size_t)l * (L + 1) + m];
-            t.post_c[o + (l - m)] = s * epi_c[(size_t)l * (L + 1) + m];
+            t.alpha[o + (l - m)] = (double)a;
+            t.s[o + (l - m)] = (double)s;
+            t.post[o + (l - m)] = (double)(s * (long double)epi[(size_t)l * (L + 1) + m]);
+            t.post_c[o + (l - m)] = (double)(s * (long double)epi_c[(size_t)l * (L + 1) + m]);
             s_lm2 = s_lm1;
             s_lm1 = s;
         }

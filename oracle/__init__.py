@@ -41,8 +41,88 @@ _MESSAGES = {
 }
 
 
+_REFSRC = "/root/reference/src"
+_REFLIB = os.path.join(_HERE, "_ref", "libshtools_ref.so")
+_FLANG = "/opt/rocm/lib/llvm/bin/flang"
+_ref = None
+
+
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    # the reference's own helper routines, where its sources and flang exist
+    if os.path.isdir(_REFSRC) and os.path.exists(_FLANG):
+        r = subprocess.run(["make", "-s", "-C", _HERE, "ref"], capture_output=True, text=True)
+        if r.returncode != 0:
+            print("oracle: building the reference helpers failed:\n" + r.stdout + r.stderr)
+
+
+def ref_lib():
+    """The reference's helper routines compiled from its sources (DHaj,
+    PreGLQ, SHGLQ, GLQGridCoord, PlmBar, PlmSchmidt, PlmON, PLegendreA,
+    NormalGravity) behind the C entry points of ref_glue.f95, or None where
+    they have not been built."""
+    global _ref
+    if _ref is None and os.path.exists(_REFLIB):
+        L = ctypes.CDLL(_REFLIB)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        i, d = ctypes.c_int, ctypes.c_double
+        L.ref_dhaj.argtypes = [i, i, dp, ip]
+        L.ref_preglq.argtypes = [d, d, i, dp, dp, ip]
+        L.ref_shglq.argtypes = [i, dp, dp, ip]
+        L.ref_glqgridcoord.argtypes = [i, i, dp, dp, ip, ip, ip]
+        L.ref_plm.argtypes = [i, i, d, i, i, dp, ip]
+        L.ref_normalgravity.argtypes = [d, d, d, d, d]
+        L.ref_normalgravity.restype = d
+        _ref = L
+    return _ref
+
+
+def _ref_call(fn, *args):
+    st = ctypes.c_int(-1)
+    fn(*args, ctypes.byref(st))
+    return st.value
+
+
+def ref_DHaj(n, extend=0):
+    """(aj, exit status) of the reference's DHaj."""
+    aj = np.zeros(n + extend)
+    return aj, _ref_call(ref_lib().ref_dhaj, n, extend, _p(aj))
+
+
+def ref_SHGLQ(lmax):
+    """(zero, w, exit status) of the reference's SHGLQ (no plx table)."""
+    zero, w = np.zeros(lmax + 1), np.zeros(lmax + 1)
+    return zero, w, _ref_call(ref_lib().ref_shglq, lmax, _p(zero), _p(w))
+
+
+def ref_PreGLQ(x1, x2, n):
+    zero, w = np.zeros(n), np.zeros(n)
+    return zero, w, _ref_call(ref_lib().ref_preglq, float(x1), float(x2), n, _p(zero), _p(w))
+
+
+def ref_GLQGridCoord(lmax, extend=0):
+    """(lat, lon, nlat, nlong, exit status) of the reference's GLQGridCoord."""
+    lat, lon = np.zeros(lmax + 1), np.zeros(2 * lmax + 1 + extend)
+    nlat, nlong, st = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(-1)
+    ref_lib().ref_glqgridcoord(lmax, extend, _p(lat), _p(lon), ctypes.byref(nlat), ctypes.byref(nlong),
+                               ctypes.byref(st))
+    return lat, lon, nlat.value, nlong.value, st.value
+
+
+# transform normalisation -> the reference's standalone routine (kind of ref_plm)
+_PLM_KIND = {1: 1, 2: 2, 3: 4, 4: 3}
+
+
+def ref_plm(lmax, z, norm=1, csphase=1, cnorm=0):
+    """(p, exit status): the reference's PlmBar / PlmSchmidt / PLegendreA /
+    PlmON (norm 1 / 2 / 3 / 4) at z, packed by l(l+1)/2 + m."""
+    p = np.zeros((lmax + 1) * (lmax + 2) // 2)
+    return p, _ref_call(ref_lib().ref_plm, _PLM_KIND[norm], lmax, float(z), csphase, cnorm, _p(p))
+
+
+def ref_NormalGravity(geocentric_lat, gm, omega, a, b):
+    return ref_lib().ref_normalgravity(float(geocentric_lat), float(gm), float(omega), float(a), float(b))
 
 
 def lib():

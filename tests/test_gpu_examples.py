@@ -90,3 +90,62 @@ def test_example_fields_in_one_batch(B, fields):
         torch.cuda.synchronize()
         assert torch.equal(g_plain, grids[i]), names[i]
         assert coef_err(back[i].cpu().numpy(), c) <= 1e-11, names[i]
+
+
+def test_known_answers_normalisation_notebook(B):
+    """examples/notebooks/spherical-harmonic-normalizations.ipynb cells 4, 9,
+    11: a single 4pi-normalised Y_{5,2} at lmax 100 has N = 4 pi under both
+    quadratures (printed 12.566370614359176 for GLQ and 12.566370614359178
+    for DH); here on the HIP grids, with the library's nodes and weights."""
+    print("case notebook known answers", flush=True)
+    lmax = 100
+    c = np.zeros((2, lmax + 1, lmax + 1))
+    c[0, 5, 2] = 1.0
+    z, w = B.SHGLQ(lmax)
+    g = B.MakeGridGLQ(c, z, extend=1)
+    nlon = g.shape[1] - 1
+    N = ((g[:, :nlon] ** 2) * w[:, None]).sum() * (2.0 * np.pi / nlon)
+    assert abs(N - 12.566370614359176) < 4e-15 * 12.6
+    g = B.MakeGridDH(c, extend=1)
+    n = g.shape[0] - 1
+    aj = B.DHaj(n)
+    N = ((g[:n, :n] ** 2) * aj[:, None]).sum() * 2.0 * np.sqrt(2.0) * np.pi / n
+    assert abs(N - 12.566370614359178) < 4e-15 * 12.6
+    # and the analysis returns the single coefficient
+    back = B.SHExpandDH(g[:n, :n])
+    assert abs(back[0, 5, 2] - 1.0) < 1e-14 and np.abs(back).sum() - abs(back[0, 5, 2]) < 1e-12
+
+
+@pytest.mark.parametrize("sampling", [1, 2])
+def test_reference_accuracy_program(B, sampling):
+    """examples/python/TimingAccuracy/TimingAccuracyDH.py on the HIP path:
+    normal random coefficients (seed 0) scaled to the power 1/(1+l)^1.5 per
+    degree, synthesis and analysis at lmax 2, 4, ..., 2048, the relative
+    error of every coefficient.  The program fails above a maximum relative
+    error of 100; here the median grows like rounding over the degrees
+    (measured 6e-16 at lmax 2 to 1.1e-13 at 2048) and the coefficient
+    vector's relative error stays within 1e-11."""
+    maxdeg = 2048
+    ls = np.arange(maxdeg + 1)
+    mask = np.zeros((2, maxdeg + 1, maxdeg + 1), dtype=bool)
+    for l in ls:
+        mask[:, l, : l + 1] = True
+    mask[1, :, 0] = False
+    cilm = np.random.RandomState(0).normal(loc=0.0, scale=1.0, size=(2, maxdeg + 1, maxdeg + 1))
+    # power per degree (4pi normalisation): sum over m <= l of C^2 + S^2
+    old_power = np.array([(cilm[:, l, : l + 1] ** 2).sum() for l in ls])
+    cilm *= np.sqrt((1.0 / (1.0 + ls) ** 1.5) / old_power)[None, :, None]
+    cilm[~mask] = 0.0
+    lmax = 2
+    while lmax <= maxdeg:
+        print(f"case accuracy program sampling {sampling} lmax {lmax}", flush=True)
+        c = np.ascontiguousarray(cilm[:, : lmax + 1, : lmax + 1])
+        m = mask[:, : lmax + 1, : lmax + 1]
+        c2 = B.SHExpandDH(B.MakeGridDH(c, sampling=sampling), sampling=sampling)
+        err = np.abs(c[m] - c2[m]) / np.abs(c[m])
+        print(f"  maxerror {err.max():.2e} mean square {np.mean(err ** 2):.2e} median {np.median(err):.2e}",
+              flush=True)
+        assert err.max() <= 100.0
+        assert np.median(err) <= 3e-16 * max(lmax, 8)
+        assert coef_err(c2, c) <= 1e-11
+        lmax *= 2

@@ -109,9 +109,20 @@ def _stream(device):
     return ctypes.c_void_p(_torch().cuda.current_stream(device).cuda_stream)
 
 
-def _finish(out, as_tensor):
-    if as_tensor:
+def _out_kind(*inputs):
+    """How results go back: "cuda" (device tensors) if any input is a device
+    tensor, else "cpu" for host tensors, else None (numpy arrays)."""
+    kinds = [("cuda" if x.is_cuda else "cpu") for x in inputs if _is_tensor(x)]
+    if "cuda" in kinds:
+        return "cuda"
+    return "cpu" if kinds else None
+
+
+def _finish(out, kind):
+    if kind == "cuda":
         return out
+    if kind == "cpu":
+        return out.cpu()
     return out.cpu().numpy()
 
 
@@ -120,7 +131,7 @@ def _finish(out, as_tensor):
 def MakeGridDH(cilm, lmax=None, norm=1, sampling=1, csphase=1, lmax_calc=None, extend=0):
     """Evaluate real coefficients on a Driscoll-Healy grid (src/MakeGridDH.f95)."""
     torch = _torch()
-    as_tensor = _is_tensor(cilm)
+    as_tensor = _out_kind(cilm)
     dev = _device_of(cilm)
     c = _to_dev(cilm, dev)
     if c.dim() != 3 or c.shape[0] < 2:
@@ -145,7 +156,7 @@ def MakeGridDH(cilm, lmax=None, norm=1, sampling=1, csphase=1, lmax_calc=None, e
 def SHExpandDH(griddh, norm=1, sampling=1, csphase=1, lmax_calc=None):
     """Expand a Driscoll-Healy grid into real coefficients (src/SHExpandDH.f95)."""
     torch = _torch()
-    as_tensor = _is_tensor(griddh)
+    as_tensor = _out_kind(griddh)
     dev = _device_of(griddh)
     g = _to_dev(griddh, dev)
     if g.dim() != 2:
@@ -169,7 +180,7 @@ def SHExpandDH(griddh, norm=1, sampling=1, csphase=1, lmax_calc=None):
 def MakeGridGLQ(cilm, zero, lmax=None, norm=1, csphase=1, lmax_calc=None, extend=0):
     """Evaluate real coefficients on a Gauss-Legendre grid (src/MakeGridGLQ.f95)."""
     torch = _torch()
-    as_tensor = _is_tensor(cilm)
+    as_tensor = _out_kind(cilm)
     dev = _device_of(cilm)
     c = _to_dev(cilm, dev)
     if c.dim() != 3 or c.shape[0] < 2:
@@ -194,7 +205,7 @@ def MakeGridGLQ(cilm, zero, lmax=None, norm=1, csphase=1, lmax_calc=None, extend
 def SHExpandGLQ(gridglq, w, zero, norm=1, csphase=1, lmax_calc=None):
     """Expand a Gauss-Legendre grid into real coefficients (src/SHExpandGLQ.f95)."""
     torch = _torch()
-    as_tensor = _is_tensor(gridglq)
+    as_tensor = _out_kind(gridglq)
     dev = _device_of(gridglq)
     g = _to_dev(gridglq, dev)
     if g.dim() != 2:
@@ -230,7 +241,7 @@ def MakeGridDHC(cilm, lmax=None, norm=1, sampling=1, csphase=1, lmax_calc=None, 
     """Evaluate complex coefficients on a Driscoll-Healy grid (src/MakeGridDHC.f95).
     cilm[0] holds the orders +m, cilm[1] the orders -m."""
     torch = _torch()
-    as_tensor = _is_tensor(cilm)
+    as_tensor = _out_kind(cilm)
     dev = _device_of(cilm)
     c = _to_dev_c(cilm, dev)
     if c.dim() != 3 or c.shape[0] < 2:
@@ -255,7 +266,7 @@ def MakeGridDHC(cilm, lmax=None, norm=1, sampling=1, csphase=1, lmax_calc=None, 
 def SHExpandDHC(griddh, norm=1, sampling=1, csphase=1, lmax_calc=None):
     """Expand a complex Driscoll-Healy grid into complex coefficients (src/SHExpandDHC.f95)."""
     torch = _torch()
-    as_tensor = _is_tensor(griddh)
+    as_tensor = _out_kind(griddh)
     dev = _device_of(griddh)
     g = _to_dev_c(griddh, dev)
     if g.dim() != 2:
@@ -277,7 +288,7 @@ def SHExpandDHC(griddh, norm=1, sampling=1, csphase=1, lmax_calc=None):
 def MakeGridGLQC(cilm, zero, lmax=None, norm=1, csphase=1, lmax_calc=None, extend=0):
     """Evaluate complex coefficients on a Gauss-Legendre grid (src/MakeGridGLQC.f95)."""
     torch = _torch()
-    as_tensor = _is_tensor(cilm)
+    as_tensor = _out_kind(cilm)
     dev = _device_of(cilm)
     c = _to_dev_c(cilm, dev)
     if c.dim() != 3 or c.shape[0] < 2:
@@ -302,7 +313,7 @@ def MakeGridGLQC(cilm, zero, lmax=None, norm=1, csphase=1, lmax_calc=None, exten
 def SHExpandGLQC(gridglq, w, zero, norm=1, csphase=1, lmax_calc=None):
     """Expand a complex Gauss-Legendre grid into complex coefficients (src/SHExpandGLQC.f95)."""
     torch = _torch()
-    as_tensor = _is_tensor(gridglq)
+    as_tensor = _out_kind(gridglq)
     dev = _device_of(gridglq)
     g = _to_dev_c(gridglq, dev)
     if g.dim() != 2:
@@ -329,7 +340,7 @@ def MakeGradientDH(cilm, lmax=None, sampling=2, lmax_calc=None, extend=0, radius
     """Horizontal gradient (theta, phi components) of a 4pi-normalised real
     field on DH grids (src/MakeGradientDH.f95; f2py src/pyshtools.pyf:2021-2043)."""
     torch = _torch()
-    as_tensor = _is_tensor(cilm)
+    as_tensor = _out_kind(cilm)
     dev = _device_of(cilm)
     c = _to_dev(cilm, dev)
     if c.dim() != 3 or c.shape[0] < 2:
@@ -360,7 +371,7 @@ def MakeGravGridDH(cilm, gm, r0, lmax=None, a=None, f=0.0, sampling=2, lmax_calc
     defaults these are: a = r0, f = 0, sampling 2, omega 0, normal gravity
     removed).  Returns (rad, theta, phi, total, pot)."""
     torch = _torch()
-    as_tensor = _is_tensor(cilm)
+    as_tensor = _out_kind(cilm)
     dev = _device_of(cilm)
     c = _to_dev(cilm, dev)
     if c.dim() != 3 or c.shape[0] < 2:
@@ -401,7 +412,7 @@ def SHMultiply(sh1, sh2, lmax1=None, lmax2=None, norm=1, csphase=1):
     """Coefficients of degree lmax1 + lmax2 of the product of two real fields
     (src/SHMultiply.f95; f2py src/pyshtools.pyf:596-620)."""
     torch = _torch()
-    as_tensor = _is_tensor(sh1) or _is_tensor(sh2)
+    as_tensor = _out_kind(sh1, sh2)
     dev = _device_of(sh1, sh2)
     a = _to_dev(sh1, dev)
     b = _to_dev(sh2, dev)

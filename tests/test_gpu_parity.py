@@ -104,6 +104,16 @@ def test_torch_tensors_in_and_out(B):
     torch.cuda.synchronize()
     assert coef_err(c2.cpu().numpy(), c) <= 1e-12
     assert grid_err(g.cpu().numpy(), oracle.MakeGridDH(c, sampling=2)) <= TOL
+    # host tensors come back as host tensors (computed on the device)
+    gh = B.MakeGridDH(torch.from_numpy(c), sampling=2)
+    assert isinstance(gh, torch.Tensor) and not gh.is_cuda
+    assert torch.equal(gh, g.cpu())
+    ch = B.SHExpandDH(gh, sampling=2)
+    assert isinstance(ch, torch.Tensor) and not ch.is_cuda and torch.equal(ch, c2.cpu())
+    z, w = B.SHGLQ(40)
+    gq = B.MakeGridGLQ(torch.from_numpy(c), z)
+    assert isinstance(gq, torch.Tensor) and not gq.is_cuda
+    assert torch.equal(gq, B.MakeGridGLQ(ct, torch.from_numpy(z).cuda()).cpu())
 
 
 def test_errors_match_reference(B):

@@ -68,3 +68,47 @@ def test_captured_plain_calls_replay(torch):
         # eager calls in between, the larger ones growing the shared buffer
         B.SHExpandDH(B.MakeGridDH(big, sampling=2), sampling=2)
         torch.cuda.synchronize()
+
+
+def test_host_threads_on_their_own_streams(torch):
+    """Four host threads, each on its own stream, call the plain transforms
+    at interleaved degrees (the library serialises the calls on the host and
+    orders its shared intermediates on the device): every result equals the
+    single-threaded one bitwise."""
+    import threading
+    from shtools_amd import backend as B
+    print("case host threads", flush=True)
+    rng = np.random.default_rng(33)
+    Ls = [31, 255, 64, 600, 5, 129, 400, 90]
+    cs = [torch.from_numpy(rand_cilm(rng, L)).cuda() for L in Ls]
+    ref_g = [B.MakeGridDH(c, sampling=2) for c in cs]
+    ref_c = [B.SHExpandDH(g, sampling=2) for g in ref_g]
+    torch.cuda.synchronize()
+    outs, errors = [[] for _ in range(4)], []
+
+    def work(k):
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                for rep in range(3):
+                    for j in range(len(Ls)):
+                        i = (j + 3 * k + rep) % len(Ls)
+                        g = B.MakeGridDH(cs[i], sampling=2)
+                        outs[k].append((i, g, B.SHExpandDH(g, sampling=2)))
+            s.synchronize()
+        except Exception as e:  # reported by the main thread
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=60)
+    assert not any(t.is_alive() for t in threads)
+    assert not errors, errors
+    torch.cuda.synchronize()
+    for k in range(4):
+        assert len(outs[k]) == 3 * len(Ls)
+        for i, g, c2 in outs[k]:
+            assert torch.equal(g, ref_g[i]), (k, i)
+            assert torch.equal(c2, ref_c[i]), (k, i)

@@ -489,8 +489,24 @@ def main():
         cpu = {"value": len(sample) / (c1 - c0), "unit": "fields/s", "cores": nthreads, "kind": "port",
                "sample": f"first {len(sample)} of the {len(Ls)} fields (lmax {min(Ls[i] for i in idx)}.."
                          f"{max(Ls[i] for i in idx)}), MakeGridDH+SHExpandDH per field, OpenMP over fields, "
-                         f"{c1 - c0:.1f} s wall, round-trip err {cerr:.1e}"}
+                         f"{c1 - c0:.1f} s wall, round-trip err {cerr:.1e}",
+               "model_gflops": acc / (c1 - c0) / 1e9}
         print(f"[bench] cpu baseline {cpu['value']:.3f} fields/s on {nthreads} threads", file=sys.stderr, flush=True)
+        # the same loop on one thread, over a quarter of the time budget
+        idx1, acc1 = [], 0.0
+        for i in idx:
+            if idx1 and acc1 >= args.cpu_seconds / 4 * 1.5e9:
+                break
+            idx1.append(i)
+            acc1 += 2 * sum(dh_flops(Ls[i]))
+        c0 = time.perf_counter()
+        oracle.roundtrip_dh_batch([host_cilm[i] for i in idx1], norm=1, sampling=2, csphase=1, nthreads=1)
+        c1 = time.perf_counter()
+        cpu["one_thread"] = {"value": len(idx1) / (c1 - c0), "unit": "fields/s", "cores": 1,
+                             "model_gflops": acc1 / (c1 - c0) / 1e9,
+                             "sample": f"first {len(idx1)} fields of the same sample, {c1 - c0:.1f} s"}
+        print(f"[bench] cpu baseline {cpu['model_gflops']:.1f} model GFLOP/s on {nthreads} threads, "
+              f"{cpu['one_thread']['model_gflops']:.2f} on one", file=sys.stderr, flush=True)
 
     line = {
         "metric": "batched DH synth+analysis fields/s, FP64, batched vs per-field, rel err <=1e-12",

@@ -176,3 +176,17 @@ def test_reference_accuracy_programs(B, program, maxdeg):
         assert np.median(err) <= max(3e-16 * lmax, 5e-15)
         assert coef_err(c2, c) <= 1e-11
         lmax *= 2
+
+
+@pytest.mark.parametrize("name,radius", [("mars90", 3389500.0), ("srtm300", 6371000.0)])
+def test_topography_gradient(B, fields, name, radius):
+    """MakeGradientDH of the example topographies (DH2 with the extra row
+    and column) against the oracle, on the gradient's bar (max|d|/max|ref|
+    <= 1e-11: both evaluate a difference that cancels near the poles)."""
+    print(f"case gradient {name}", flush=True)
+    c = fields[name]
+    kw = dict(sampling=2, extend=1, radius=radius)
+    th, ph = B.MakeGradientDH(c, **kw)
+    th_ref, ph_ref = oracle.MakeGradientDH(c, **kw)
+    assert th.shape == th_ref.shape and ph.shape == ph_ref.shape
+    assert grid_err(th, th_ref) <= 1e-11 and grid_err(ph, ph_ref) <= 1e-11

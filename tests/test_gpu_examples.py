@@ -190,3 +190,20 @@ def test_topography_gradient(B, fields, name, radius):
     th_ref, ph_ref = oracle.MakeGradientDH(c, **kw)
     assert th.shape == th_ref.shape and ph.shape == ph_ref.shape
     assert grid_err(th, th_ref) <= 1e-11 and grid_err(ph, ph_ref) <= 1e-11
+
+
+def test_product_of_example_fields(B, fields):
+    """SHMultiply of Mars topography (degree 90) with the GMM-3 gravity
+    coefficients (degree 120) against the oracle (1e-12 of the largest
+    product coefficient), and the product's DH synthesis against the
+    pointwise product of the factors' syntheses."""
+    print("case SHMultiply example fields", flush=True)
+    a, b = fields["mars90"], fields["gmm3_cilm"]
+    out = B.SHMultiply(a, b)
+    ref = oracle.SHMultiply(a, b)
+    assert out.shape == ref.shape == (2, 211, 211)
+    assert np.abs(out - ref).max() <= 1e-12 * np.abs(ref).max()
+    L = 210
+    pad = lambda c: np.pad(c, ((0, 0), (0, L + 1 - c.shape[1]), (0, L + 1 - c.shape[2])))
+    ga, gb = B.MakeGridDH(pad(a), sampling=2), B.MakeGridDH(pad(b), sampling=2)
+    assert grid_err(B.MakeGridDH(out, sampling=2), ga * gb) <= 1e-12

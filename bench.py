@@ -25,25 +25,34 @@ KERNEL_NAMES = {0: "k_pack_synth", 1: "k_synth_leg", 2: "k_synth_fft", 3: "k_ana
 BATCH_KERNELS = {"k_pack_synth": "kb_pack", "k_synth_leg": "kb_synth_leg", "k_synth_fft": "kb_synth_fft",
                  "k_anal_fft": "kb_anal_fft", "k_anal_leg": "kb_anal_leg", "k_unpack_anal": "kb_unpack"}
 CALIB_BYTES = 1 << 30     # plain device-to-device copy used to scale the HBM counters
+PROF_STEPS = 3            # eager steps averaged for the per-stage HIP-event durations
 
 
 def stage_model(L, sampling=2):
-    """Algorithmic work of each stage for one DH field of degree L, one
-    direction: (FP64 flops, HBM bytes).  Legendre stages are priced by the
-    recursion + accumulation flops; FFT stages by the grid they write (read)
-    plus the order-major ring sums they read (write); pack/unpack by the
-    coefficient array and the packed triangle."""
+    """Work of each stage for one DH field of degree L, one direction:
+    (algorithmic FP64 flops, algorithmic HBM bytes, intermediate bytes).
+
+    The algorithmic figures split the transform's work model (per direction
+    F = 8 P (L+1)(L+2)/2 + 2.5 nlat nlon log2 nlon flops and B = 8 (2 (L+1)^2
+    + nlat nlon) bytes: coefficients once, grid once) over the stages that do
+    it: the Legendre stages the recursion and accumulation flops, the ring
+    FFTs the FFT flops and the grid, pack/unpack the coefficient array.  The
+    intermediate bytes are what the split into stages adds on top (the packed
+    coefficient triangle and the order-major ring sums written by one stage
+    and read by the next); they are reported as overhead, not credited."""
     from shtools_amd.synthetic import dh_flops
     n = 2 * L + 2
     nlon = sampling * n
     P = n // 2 + 1
     leg, fft = dh_flops(L, sampling)
     tri = (L + 1) * (L + 2) // 2
-    pack_b = 8.0 * 2 * (L + 1) ** 2 + 16.0 * tri
-    fft_b = 8.0 * n * nlon + 32.0 * P * (L + 1)
-    leg_b = 16.0 * tri + 32.0 * P * (L + 1)
-    return {"k_pack_synth": (0.0, pack_b), "k_synth_leg": (leg, leg_b), "k_synth_fft": (fft, fft_b),
-            "k_anal_fft": (fft, fft_b), "k_anal_leg": (leg, leg_b), "k_unpack_anal": (0.0, pack_b)}
+    coef_b = 8.0 * 2 * (L + 1) ** 2
+    grid_b = 8.0 * n * nlon
+    ring_sums = 32.0 * P * (L + 1)
+    packed = 16.0 * tri
+    return {"k_pack_synth": (0.0, coef_b, packed), "k_synth_leg": (leg, 0.0, packed + ring_sums),
+            "k_synth_fft": (fft, grid_b, ring_sums), "k_anal_fft": (fft, grid_b, ring_sums),
+            "k_anal_leg": (leg, 0.0, packed + ring_sums), "k_unpack_anal": (0.0, coef_b, packed)}
 
 
 def parse():
@@ -412,38 +421,43 @@ def main():
     print(f"[bench] worst round-trip err {rt:.2e}", file=sys.stderr, flush=True)
 
     # per-stage durations with HIP events recorded on the launch stream
+    # around every launch, averaged over PROF_STEPS eager steps
     lib = _lib.lib()
-    lib.shtg_profile_enable(1)
-    step()
-    torch.cuda.synchronize()
     nk = 16
     ms = (ctypes.c_double * nk)()
     cnt = (ctypes.c_longlong * nk)()
+    lib.shtg_profile_enable(1)
+    for _ in range(PROF_STEPS):
+        step()
+    torch.cuda.synchronize()
     lib.shtg_profile_collect(ms, cnt, nk)
     lib.shtg_profile_enable(0)
-    kern = {KERNEL_NAMES.get(k, str(k)): ms[k] for k in range(nk) if cnt[k]}
+    kern = {KERNEL_NAMES.get(k, str(k)): ms[k] / PROF_STEPS for k in range(nk) if cnt[k]}
+    launches = {KERNEL_NAMES.get(k, str(k)): cnt[k] / PROF_STEPS for k in range(nk) if cnt[k]}
     print("[bench] stage ms: " + ", ".join(f"{k}={v:.2f}" for k, v in kern.items()), file=sys.stderr, flush=True)
 
     # algorithmic work per stage over the batch, and the roofline of each
     work = {}
     for L in Ls:
-        for k, (f, b) in stage_model(L).items():
-            w = work.setdefault(k, [0.0, 0.0])
-            w[0] += f
-            w[1] += b
+        for k, fbi in stage_model(L).items():
+            w = work.setdefault(k, [0.0, 0.0, 0.0])
+            for j in range(3):
+                w[j] += fbi[j]
     stages = {}
     for k, t in kern.items():
-        f, b = work.get(k, (0.0, 0.0))
-        bound = "mfma" if k in ("k_synth_leg", "k_anal_leg") else "hbm"
-        if bound == "mfma":
+        f, b, inter = work.get(k, (0.0, 0.0, 0.0))
+        bound = "fp64" if k in ("k_synth_leg", "k_anal_leg") else "hbm"
+        if bound == "fp64":
             ach, peak, unit = f / (t * 1e-3) / 1e12, PEAK_FP64_TFLOPS, "TFLOP/s"
         else:
             ach, peak, unit = b / (t * 1e-3) / 1e9, PEAK_HBM_GBS, "GB/s"
-        st = {"ms": t, "bound": bound, "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak,
-              "algo_flops": f, "algo_bytes": b}
+        st = {"ms": t, "launches": launches[k], "bound": bound, "achieved": ach, "peak": peak, "unit": unit,
+              "frac": ach / peak, "algo_flops": f, "algo_bytes": b, "intermediate_bytes": inter}
         if traffic:
             st["hbm_bytes"] = (traffic["FETCH_SIZE"]["kernels"][k] + traffic["WRITE_SIZE"]["kernels"][k])
             st["hbm_gbs_counters"] = st["hbm_bytes"] / (t * 1e-3) / 1e9
+            if b > 0:
+                st["hbm_over_algo"] = st["hbm_bytes"] / b
         if fp64:
             st["fp64_flops_counters"] = fp64[k]
             st["fp64_tflops_counters"] = fp64[k] / (t * 1e-3) / 1e12
@@ -452,7 +466,10 @@ def main():
     d = stages[dom]
     roofline = {"bound": d["bound"], "achieved": d["achieved"], "peak": d["peak"], "unit": d["unit"],
                 "frac": d["frac"], "traffic": d.get("hbm_bytes"), "kernel": BATCH_KERNELS.get(dom, dom),
-                "launch": "one step's dispatches of the stage (FFT stages: one per LDS class)"}
+                "ms": d["ms"], "algo_bytes": d["algo_bytes"], "algo_flops": d["algo_flops"],
+                "intermediate_bytes": d["intermediate_bytes"],
+                "launch": "one step's dispatches of the stage (FFT stages: one per LDS class); achieved = "
+                          "algorithmic bytes (grid once) or flops of those dispatches / their summed duration"}
     if traffic_err:
         roofline["traffic_note"] = traffic_err
     if traffic:
@@ -468,12 +485,19 @@ def main():
         configs = other_configs(dev)
 
     cpu = None
+    parity = None
     if not args.no_cpu:
         import oracle
+        nproc = os.cpu_count() or 1
         try:
-            nthreads = min(16, len(os.sched_getaffinity(0)))
+            naff = len(os.sched_getaffinity(0))
         except AttributeError:
-            nthreads = min(16, os.cpu_count() or 1)
+            naff = nproc
+        # every core of this process's affinity mask, unless the host
+        # limits the process's thread count (OMP_NUM_THREADS, the CPU share
+        # a shared GPU host gives one GPU's jobs)
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        nthreads = min(naff, omp) if omp > 0 else naff
         budget = args.cpu_seconds * 1.5e9 * nthreads   # ~1.5 GFLOP/s per core for the Fortran-style loops
         idx, acc = [], 0.0
         for i, L in enumerate(Ls):
@@ -483,13 +507,30 @@ def main():
             acc += 2 * sum(dh_flops(L))
         sample = [host_cilm[i] for i in idx]
         c0 = time.perf_counter()
-        outc = oracle.roundtrip_dh_batch(sample, norm=1, sampling=2, csphase=1, nthreads=nthreads)
+        gref, outc = oracle.roundtrip_dh_batch(sample, norm=1, sampling=2, csphase=1, nthreads=nthreads,
+                                               return_grids=True)
         c1 = time.perf_counter()
         cerr = max(float(np.linalg.norm(o - c) / np.linalg.norm(c)) for o, c in zip(outc, sample))
+        # the oracle's grids and coefficients of these fields against the GPU
+        # batched outputs of the timed step (same inputs)
+        if not args.loop:
+            g_errs = [float(np.abs(grids[i].cpu().numpy() - g).max() / np.abs(g).max())
+                      for i, g in zip(idx, gref)]
+            c_errs = [float(np.linalg.norm(outs[i].cpu().numpy() - c) / np.linalg.norm(c))
+                      for i, c in zip(idx, outc)]
+            parity = {"grid_err_max": max(g_errs), "coef_err_max": max(c_errs), "fields": len(idx),
+                      "measure": "per field: grids max|gpu-oracle|/max|oracle|, coefficients "
+                                 "||gpu-oracle||/||oracle||; the oracle is the C restatement of the Fortran "
+                                 "MakeGridDH/SHExpandDH, run on the CPU-baseline sample"}
+            print(f"[bench] parity vs oracle on {len(idx)} fields: grid {parity['grid_err_max']:.2e}, "
+                  f"coef {parity['coef_err_max']:.2e}", file=sys.stderr, flush=True)
         cpu = {"value": len(sample) / (c1 - c0), "unit": "fields/s", "cores": nthreads, "kind": "port",
-               "sample": f"first {len(sample)} of the {len(Ls)} fields (lmax {min(Ls[i] for i in idx)}.."
-                         f"{max(Ls[i] for i in idx)}), MakeGridDH+SHExpandDH per field, OpenMP over fields, "
-                         f"{c1 - c0:.1f} s wall, round-trip err {cerr:.1e}",
+               "nproc": nproc, "affinity_cores": naff, "omp_num_threads": omp or None,
+               "sample": f"first {len(sample)} of the {len(Ls)} fields in workload order, as many as ~"
+                         f"{args.cpu_seconds:.0f} s of CPU work at 1.5 GFLOP/s per thread allow (lmax "
+                         f"{min(Ls[i] for i in idx)}..{max(Ls[i] for i in idx)}), MakeGridDH+SHExpandDH per "
+                         f"field, OpenMP over fields on {nthreads} threads, {c1 - c0:.1f} s wall, round-trip "
+                         f"err {cerr:.1e}",
                "model_gflops": acc / (c1 - c0) / 1e9}
         print(f"[bench] cpu baseline {cpu['value']:.3f} fields/s on {nthreads} threads", file=sys.stderr, flush=True)
         # the same loop on one thread, over a quarter of the time budget
@@ -529,6 +570,8 @@ def main():
         "roofline": roofline,
         "model_tflops": total_flops * args.steps / (t1 - t0) / 1e12,
         "roundtrip_err": rt,
+        "parity_err": max(parity["grid_err_max"], parity["coef_err_max"]) if parity else None,
+        "parity": parity,
         "loop_fields_per_s": loop_rate,
         "batched_vs_loop": (value / loop_rate) if loop_rate else None,
         "stages": stages,

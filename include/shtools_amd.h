@@ -15,7 +15,10 @@
  *     the call returns before it completes;
  *   - the return value is the reference's exitstatus: 0 ok, 1 improper array
  *     dimensions, 2 improper bounds of an input variable, 3 allocation
- *     failure, 5 other (e.g. neither zero nor plx given, no GPU);
+ *     failure, 5 other (e.g. neither zero nor plx given, no GPU); and one
+ *     of this library's own: 6, the call would have to build device tables
+ *     or bind new arrays while its stream is captured into a HIP graph
+ *     (make one eager call of the same shape on the same arrays first);
  *   - norm 1 = 4pi (geodesy), 2 = Schmidt, 3 = unnormalised, 4 = orthonormal;
  *     csphase 1 omits and -1 applies the Condon-Shortley phase (-1)^m;
  *   - calls are thread-safe; host-side tables are cached per (norm, csphase,
@@ -200,6 +203,11 @@ int shtg_batch_tune(shtg_batch_plan *plan, int reps, void *stream);
 /* Workgroup sizes of the plan's ring-FFT groups (up to n of them); returns
  * the number of groups. */
 int shtg_batch_get_shapes(shtg_batch_plan *plan, int *threads_synth, int *threads_anal, int n);
+/* The ring-FFT group of each field (group[0 .. nfields), n >= nfields):
+ * fields of one group share a launch and an LDS size class; the last group,
+ * numbered after the shapes of shtg_batch_get_shapes, holds the fields whose
+ * rings run on the global workspace.  Returns the number of groups. */
+int shtg_batch_field_groups(shtg_batch_plan *plan, int *group, int n);
 
 /* Host helpers. */
 /* SHGLQ nodes and weights (include/shtools.h:142-148; src/SHGLQ.f95:187-192 ->

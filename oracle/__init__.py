@@ -139,6 +139,8 @@ def lib():
         L.or_plm.argtypes = [i, i, i, ctypes.c_double, dp]
         L.or_makegriddh.argtypes = [dp, i, i, i, i, i, i, i, i, dp, i, i]
         L.or_shexpanddh.argtypes = [dp, i, i, i, i, i, i, dp, i, i]
+        L.or_shexpanddh_par.argtypes = [dp, i, i, i, i, i, i, dp, i, i]
+        L.or_shexpandglq_par.argtypes = [dp, i, i, i, dp, i, dp, i, i, i, i, dp, i, i]
         L.or_makegridglq.argtypes = [dp, i, i, i, dp, i, i, i, i, i, dp, i, i]
         L.or_shexpandglq.argtypes = [dp, i, i, i, dp, i, dp, i, i, i, i, dp, i, i]
         L.or_makegriddhc.argtypes = [dp, i, i, i, i, i, i, i, i, dp, i, i]
@@ -210,14 +212,17 @@ def MakeGridDH(cilm, lmax=None, norm=1, sampling=1, csphase=1, lmax_calc=None, e
     return grid
 
 
-def SHExpandDH(griddh, norm=1, sampling=1, csphase=1, lmax_calc=None):
+def SHExpandDH(griddh, norm=1, sampling=1, csphase=1, lmax_calc=None, parallel=False):
+    """parallel: sum the orders on all OpenMP threads (or_shexpanddh_par,
+    bitwise the same coefficients; for high degrees)."""
     grid = np.ascontiguousarray(griddh, dtype=np.float64)
     n = grid.shape[0]
     if lmax_calc is None:
         lmax_calc = n // 2 - 1
     cilm = np.zeros((2, lmax_calc + 1, lmax_calc + 1))
-    _check(lib().or_shexpanddh(_p(grid), n, grid.shape[1], norm, sampling, csphase, lmax_calc,
-                               _p(cilm), lmax_calc + 1, lmax_calc + 1))
+    fn = lib().or_shexpanddh_par if parallel else lib().or_shexpanddh
+    _check(fn(_p(grid), n, grid.shape[1], norm, sampling, csphase, lmax_calc,
+              _p(cilm), lmax_calc + 1, lmax_calc + 1))
     return cilm
 
 
@@ -235,7 +240,8 @@ def MakeGridGLQ(cilm, zero, lmax=None, norm=1, csphase=1, lmax_calc=None, extend
     return grid
 
 
-def SHExpandGLQ(gridglq, w, zero, norm=1, csphase=1, lmax_calc=None):
+def SHExpandGLQ(gridglq, w, zero, norm=1, csphase=1, lmax_calc=None, parallel=False):
+    """parallel: as for SHExpandDH (or_shexpandglq_par)."""
     grid = np.ascontiguousarray(gridglq, dtype=np.float64)
     w = np.ascontiguousarray(w, dtype=np.float64)
     zero = np.ascontiguousarray(zero, dtype=np.float64)
@@ -243,9 +249,10 @@ def SHExpandGLQ(gridglq, w, zero, norm=1, csphase=1, lmax_calc=None):
     if lmax_calc is None:
         lmax_calc = lmax
     cilm = np.zeros((2, lmax_calc + 1, lmax_calc + 1))
-    _check(lib().or_shexpandglq(_p(grid), grid.shape[0], grid.shape[1], lmax, _p(w), w.size,
-                                _p(zero), zero.size, norm, csphase, lmax_calc, _p(cilm),
-                                lmax_calc + 1, lmax_calc + 1))
+    fn = lib().or_shexpandglq_par if parallel else lib().or_shexpandglq
+    _check(fn(_p(grid), grid.shape[0], grid.shape[1], lmax, _p(w), w.size,
+              _p(zero), zero.size, norm, csphase, lmax_calc, _p(cilm),
+              lmax_calc + 1, lmax_calc + 1))
     return cilm
 
 
@@ -379,8 +386,9 @@ def irfft(c, n):
     return out
 
 
-def roundtrip_dh_batch(cilms, norm=1, sampling=2, csphase=1, nthreads=1):
-    """MakeGridDH -> SHExpandDH for a list of cilm arrays, OpenMP over fields."""
+def roundtrip_dh_batch(cilms, norm=1, sampling=2, csphase=1, nthreads=1, return_grids=False):
+    """MakeGridDH -> SHExpandDH for a list of cilm arrays, OpenMP over fields.
+    Returns the coefficients (and, with return_grids, the grids first)."""
     lmaxs = np.array([c.shape[1] - 1 for c in cilms], dtype=np.int32)
     csz = np.array([2 * (L + 1) ** 2 for L in lmaxs], dtype=np.int64)
     gsz = np.array([(2 * L + 2) * sampling * (2 * L + 2) for L in lmaxs], dtype=np.int64)
@@ -395,4 +403,8 @@ def roundtrip_dh_batch(cilms, norm=1, sampling=2, csphase=1, nthreads=1):
         goff.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
         norm, sampling, csphase, _p(cin), _p(grid), _p(out), nthreads)
     _check(st)
-    return [out[o:o + s].reshape(2, L + 1, L + 1) for o, s, L in zip(coff, csz, lmaxs)]
+    coefs = [out[o:o + s].reshape(2, L + 1, L + 1) for o, s, L in zip(coff, csz, lmaxs)]
+    if not return_grids:
+        return coefs
+    grids = [grid[o:o + s].reshape(2 * L + 2, sampling * (2 * L + 2)) for o, s, L in zip(goff, gsz, lmaxs)]
+    return grids, coefs

@@ -876,6 +876,258 @@ int or_shexpanddh(const double *grid, int n, int g1, int norm, int sampling, int
     return 0;
 }
 
+/* ------------------------------------- order-parallel analysis (same sums)
+ *
+ * The sequential analysis above visits the ring pairs in order and, per
+ * ring, every order m; a coefficient (l, m) therefore accumulates its ring
+ * contributions in ring order, and nothing of order m depends on another
+ * order except two chains that do not depend on the ring data: the seeds
+ * pmm (pmm_step) and, per ring, the iterated rescalem = u^m / 1e-280.  So the
+ * orders can be summed in parallel, each over the rings in the same order,
+ * from precomputed Fourier coefficients, seeds and rescale factors: every
+ * coefficient comes out bitwise as in the sequential loop (checked by the
+ * tests).  Rings r = 0 .. nr-1 are the ring pairs in processing order
+ * (z[r], u[r], north/south coefficients f1/f2 + r * 2 nf), and fe, if not
+ * NULL, the equator processed after them. */
+static void anal_orders(const tables *t, int norm, int phase, int nr, const double *z, const double *u,
+                        const double *f1, const double *f2, const double *fe, int nf, double *cf)
+{
+    const double *sqr = t->sqr;
+    const int lc = t->lc;
+    double *seed = (double *)malloc(sizeof(double) * (size_t)(lc + 2));
+    double *resc = (double *)malloc(sizeof(double) * (size_t)(nr > 0 ? nr : 1) * (lc + 1));
+    double pmm = pmm_start(norm, sqr), pmm_fin;
+    for (int m = 1; m <= lc - 1; m++) seed[m] = pmm_step(norm, phase, &pmm, m, sqr);
+    pmm_fin = pmm;
+    for (int r = 0; r < nr; r++) {
+        double rescalem = 1.0 / 1.0e-280;
+        for (int m = 1; m <= lc; m++) {
+            rescalem = rescalem * u[r];
+            resc[(size_t)r * (lc + 1) + m] = rescalem;
+        }
+    }
+#define F1(r, i, m) f1[((size_t)(r) * 2 + (i)) * nf + (m)]
+#define F2(r, i, m) f2[((size_t)(r) * 2 + (i)) * nf + (m)]
+#define FE(i, m) fe[(size_t)(i) * nf + (m)]
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int m = 0; m <= lc; m++) {
+        double pm2, pm1, p, ffc[2][2];
+        if (m == 0) {
+            for (int r = 0; r < nr; r++) {
+                pm2 = pm00(norm);
+                CF(0, 0, 0) += pm2 * (F1(r, 0, 0) + F2(r, 0, 0));
+                if (lc == 0) continue;
+                pm1 = FF(t, ff1, 1, 0) * z[r] * pm2;
+                CF(0, 1, 0) += pm1 * (F1(r, 0, 0) - F2(r, 0, 0));
+                ffc[0][0] = F1(r, 0, 0) - F2(r, 0, 0);
+                ffc[0][1] = F1(r, 0, 0) + F2(r, 0, 0);
+                for (int l = 2; l <= lc; l++) {
+                    p = FF(t, ff1, l, 0) * z[r] * pm1 - FF(t, ff2, l, 0) * pm2;
+                    pm2 = pm1; pm1 = p;
+                    CF(0, l, 0) += p * ffc[0][fsym(l, 0) > 0];
+                }
+            }
+            if (fe) {
+                pm2 = pm00(norm);
+                CF(0, 0, 0) += pm2 * FE(0, 0);
+                for (int l = 2; l <= lc; l += 2) {
+                    p = -FF(t, ff2, l, 0) * pm2;
+                    pm2 = p;
+                    CF(0, l, 0) += p * FE(0, 0);
+                }
+            }
+        } else if (m <= lc - 1) {
+            for (int r = 0; r < nr; r++) {
+                const double rescalem = resc[(size_t)r * (lc + 1) + m];
+                double a1[2], a2[2];
+                for (int i = 0; i < 2; i++) { a1[i] = F1(r, i, m) * rescalem; a2[i] = F2(r, i, m) * rescalem; }
+                pm2 = seed[m];
+                for (int i = 0; i < 2; i++) CF(i, m, m) += pm2 * (a1[i] + a2[i]);
+                pm1 = z[r] * FF(t, ff1, m + 1, m) * pm2;
+                for (int i = 0; i < 2; i++) CF(i, m + 1, m) += pm1 * (a1[i] - a2[i]);
+                for (int i = 0; i < 2; i++) {
+                    ffc[i][0] = a1[i] - a2[i];
+                    ffc[i][1] = a1[i] + a2[i];
+                }
+                for (int l = m + 2; l <= lc; l++) {
+                    p = z[r] * FF(t, ff1, l, m) * pm1 - FF(t, ff2, l, m) * pm2;
+                    pm2 = pm1; pm1 = p;
+                    int s = ((l - m) % 2 == 0);
+                    CF(0, l, m) += p * ffc[0][s];
+                    CF(1, l, m) += p * ffc[1][s];
+                }
+            }
+            if (fe) {
+                const double rescalem = 1.0 / 1.0e-280;
+                double a[2];
+                for (int i = 0; i < 2; i++) a[i] = FE(i, m) * rescalem;
+                pm2 = seed[m];
+                for (int i = 0; i < 2; i++) CF(i, m, m) += pm2 * a[i];
+                for (int l = m + 2; l <= lc; l += 2) {
+                    p = -FF(t, ff2, l, m) * pm2;
+                    pm2 = p;
+                    for (int i = 0; i < 2; i++) CF(i, l, m) += p * a[i];
+                }
+            }
+        } else {  /* m == lc >= 1: the last sectoral term */
+            for (int r = 0; r < nr; r++) {
+                const double pl = pmm_last(norm, phase, pmm_fin, lc, sqr, resc[(size_t)r * (lc + 1) + lc]);
+                for (int i = 0; i < 2; i++) CF(i, lc, lc) += pl * (F1(r, i, lc) + F2(r, i, lc));
+            }
+            if (fe) {
+                const double pl = pmm_last(norm, phase, pmm_fin, lc, sqr, 1.0 / 1.0e-280);
+                for (int i = 0; i < 2; i++) CF(i, lc, lc) += pl * FE(i, lc);
+            }
+        }
+    }
+#undef F1
+#undef F2
+#undef FE
+    free(seed);
+    free(resc);
+}
+
+/* SHExpandDH with the orders summed in parallel (anal_orders): the same
+ * coefficients, bitwise, as or_shexpanddh, in a fraction of the time on
+ * several cores.  Ring Fourier coefficients are computed in parallel too,
+ * each thread with its own FFT plan. */
+int or_shexpanddh_par(const double *grid, int n, int g1, int norm, int sampling, int csphase,
+                      int lmax_calc, double *cilm, int d1, int d2)
+{
+    int lmax = n / 2 - 1, lc, nlong;
+    if (lmax_calc > lmax) return 2;
+    lc = lmax_calc < lmax ? lmax_calc : lmax;
+    if (sampling != 1 && sampling != 2) return 2;
+    if (n % 2 != 0) return 2;
+    if (d1 < lc + 1 || d2 < lc + 1) return 1;
+    nlong = (sampling == 1) ? n : 2 * n;
+    if (g1 < nlong) return 1;
+    if (csphase != -1 && csphase != 1) return 2;
+    if (norm < 1 || norm > 4) return 2;
+    double pi = acos(-1.0);
+    for (size_t k = 0; k < (size_t)2 * d1 * d2; k++) cilm[k] = 0.0;
+    double *aj = (double *)malloc(sizeof(double) * (n + 1));
+    or_dhaj(n, 0, aj);
+    for (int j = 0; j < n; j++) aj[j] = aj[j] * sqrt(4.0 * pi);
+    tables t;
+    int st = build_tables(&t, lc, norm);
+    const int nf = n / 2 + 1, i_eq = n / 2 + 1, nr = i_eq - 2;  /* pairs i = 2 .. i_eq - 1 */
+    double *cf = (double *)calloc((size_t)2 * (lc + 1) * (lc + 1), sizeof(double));
+    double *z = (double *)malloc(sizeof(double) * (size_t)(nr + 1));
+    double *u = (double *)malloc(sizeof(double) * (size_t)(nr + 1));
+    double *f1 = (double *)calloc((size_t)2 * nf * (nr + 1), sizeof(double));
+    double *f2 = (double *)calloc((size_t)2 * nf * (nr + 1), sizeof(double));
+    double *fe = (double *)calloc((size_t)2 * nf, sizeof(double));
+    if (st || !cf || !z || !u || !f1 || !f2 || !fe) {
+        free_tables(&t); free(aj); free(cf); free(z); free(u); free(f1); free(f2); free(fe);
+        return 3;
+    }
+#define GR(i, k) grid[(size_t)(i) * g1 + (k)]
+#pragma omp parallel
+    {
+        fftplan *plan = plan_create(nlong);
+        cplx *buf = (cplx *)malloc(sizeof(cplx) * nlong);
+        cplx *cc = (cplx *)malloc(sizeof(cplx) * (nlong / 2 + 1));
+#pragma omp for schedule(dynamic, 4)
+        for (int i = 2; i <= i_eq; i++) {
+            const int r = i - 2;
+            double *a = (i == i_eq) ? fe : f1 + (size_t)r * 2 * nf;
+            r2c(plan, &GR(i - 1, 0), buf, cc);
+            for (int m = 0; m < n / 2; m++) {
+                a[m] = sqrt(2 * pi) * aj[i - 1] * creal(cc[m]) / (double)nlong;
+                a[nf + m] = -sqrt(2 * pi) * aj[i - 1] * cimag(cc[m]) / (double)nlong;
+            }
+            if (i == i_eq) continue;
+            double theta = (i - 1) * pi / (double)n;
+            z[r] = cos(theta);
+            u[r] = sqrt((1.0 - z[r]) * (1.0 + z[r]));
+            int i_s = 2 * i_eq - i;
+            double *b = f2 + (size_t)r * 2 * nf;
+            r2c(plan, &GR(i_s - 1, 0), buf, cc);
+            for (int m = 0; m < n / 2; m++) {
+                b[m] = sqrt(2 * pi) * aj[i_s - 1] * creal(cc[m]) / (double)nlong;
+                b[nf + m] = -sqrt(2 * pi) * aj[i_s - 1] * cimag(cc[m]) / (double)nlong;
+            }
+        }
+        free(buf); free(cc);
+        plan_destroy(plan);
+    }
+#undef GR
+    anal_orders(&t, norm, csphase, nr, z, u, f1, f2, fe, nf, cf);
+    anal_epilogue(norm, lc, cf, cilm, d1, d2);
+    free(aj); free(cf); free(z); free(u); free(f1); free(f2); free(fe);
+    free_tables(&t);
+    return 0;
+}
+
+/* SHExpandGLQ with the orders summed in parallel (as or_shexpanddh_par). */
+int or_shexpandglq_par(const double *grid, int g0, int g1, int lmax, const double *w, int nw,
+                       const double *zero, int nzero, int norm, int csphase, int lmax_calc,
+                       double *cilm, int d1, int d2)
+{
+    int lc, nlong = 2 * lmax + 1, nlat = lmax + 1;
+    if (lmax_calc > lmax) return 2;
+    lc = lmax_calc < lmax ? lmax_calc : lmax;
+    if (d1 < lc + 1 || d2 < lc + 1) return 1;
+    if (g1 < 2 * lmax + 1 || g0 < lmax + 1) return 1;
+    if (nw < lmax + 1) return 1;
+    if (!zero) return 5;
+    if (nzero < lmax + 1) return 1;
+    if (norm < 1 || norm > 4) return 2;
+    if (csphase != -1 && csphase != 1) return 2;
+    double pi = acos(-1.0);
+    for (size_t k = 0; k < (size_t)2 * d1 * d2; k++) cilm[k] = 0.0;
+    tables t;
+    int st = build_tables(&t, lc, norm);
+    const int nf = lmax + 1, half = (nlat + 1) / 2, has_eq = nlat % 2 != 0;
+    const int nr = has_eq ? half - 1 : half;
+    double *cf = (double *)calloc((size_t)2 * (lc + 1) * (lc + 1), sizeof(double));
+    double *z = (double *)malloc(sizeof(double) * (size_t)(nr + 1));
+    double *u = (double *)malloc(sizeof(double) * (size_t)(nr + 1));
+    double *f1 = (double *)calloc((size_t)2 * nf * (nr + 1), sizeof(double));
+    double *f2 = (double *)calloc((size_t)2 * nf * (nr + 1), sizeof(double));
+    double *fe = (double *)calloc((size_t)2 * nf, sizeof(double));
+    if (st || !cf || !z || !u || !f1 || !f2 || !fe) {
+        free_tables(&t); free(cf); free(z); free(u); free(f1); free(f2); free(fe);
+        return 3;
+    }
+#define GR(i, k) grid[(size_t)(i) * g1 + (k)]
+#pragma omp parallel
+    {
+        fftplan *plan = plan_create(nlong);
+        cplx *buf = (cplx *)malloc(sizeof(cplx) * nlong);
+        cplx *cc = (cplx *)malloc(sizeof(cplx) * (nlong / 2 + 1));
+#pragma omp for schedule(dynamic, 4)
+        for (int i = 1; i <= half; i++) {
+            const int r = i - 1, eq = has_eq && i == half;
+            double *a = eq ? fe : f1 + (size_t)r * 2 * nf;
+            r2c(plan, &GR(i - 1, 0), buf, cc);
+            for (int m = 0; m <= lmax; m++) {
+                a[m] = 2 * w[i - 1] * pi * creal(cc[m]) / (double)nlong;
+                a[nf + m] = -2 * w[i - 1] * pi * cimag(cc[m]) / (double)nlong;
+            }
+            if (eq) continue;
+            z[r] = zero[i - 1];
+            u[r] = sqrt((1.0 - z[r]) * (1.0 + z[r]));
+            int i_s = nlat + 1 - i;
+            double *b = f2 + (size_t)r * 2 * nf;
+            r2c(plan, &GR(i_s - 1, 0), buf, cc);
+            for (int m = 0; m <= lmax; m++) {
+                b[m] = 2 * w[i_s - 1] * pi * creal(cc[m]) / (double)nlong;
+                b[nf + m] = -2 * w[i_s - 1] * pi * cimag(cc[m]) / (double)nlong;
+            }
+        }
+        free(buf); free(cc);
+        plan_destroy(plan);
+    }
+#undef GR
+    anal_orders(&t, norm, csphase, nr, z, u, f1, f2, has_eq ? fe : NULL, nf, cf);
+    anal_epilogue(norm, lc, cf, cilm, d1, d2);
+    free(cf); free(z); free(u); free(f1); free(f2); free(fe);
+    free_tables(&t);
+    return 0;
+}
+
 int or_shexpandglq(const double *grid, int g0, int g1, int lmax, const double *w, int nw,
                    const double *zero, int nzero, int norm, int csphase, int lmax_calc,
                    double *cilm, int d1, int d2)

@@ -60,6 +60,7 @@ SIGNATURES = {
     "shtg_batch_destroy": [_vp],
     "shtg_batch_tune": [_vp, _c_int, _vp],
     "shtg_batch_get_shapes": [_vp, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), _c_int],
+    "shtg_batch_field_groups": [_vp, ctypes.POINTER(_c_int), _c_int],
     "shtg_profile_enable": [_c_int],
     "shtg_profile_collect": [_dp, ctypes.POINTER(ctypes.c_longlong), _c_int],
     "shtg_device_count": [],

@@ -55,6 +55,10 @@ def _status_message(status):
         return "Error allocating memory."
     if status == 4:
         return "File IO error."
+    if status == 6:
+        # this library's own status: no counterpart in the reference
+        return ("Call not allowed under HIP graph capture: it needs device tables or array bindings that "
+                "do not exist yet; make one eager call of the same shape on the same arrays first.")
     return "Unhandled Fortran 95 error."
 
 

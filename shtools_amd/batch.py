@@ -167,6 +167,16 @@ class DHBatch:
         self.expand(grids, out=outs)
         return self.shapes()
 
+    def field_groups(self):
+        """The ring-FFT group of each field (fields of one group share a
+        launch and an LDS size class; the highest number is the group whose
+        rings run on the global workspace, if any)."""
+        arr = (ctypes.c_int * len(self.lmax))()
+        ng = _lib.lib().shtg_batch_field_groups(self._plan, arr, len(self.lmax))
+        if ng < 0:
+            raise SHToolsError(_status_message(5))
+        return [arr[i] for i in range(len(self.lmax))]
+
     def shapes(self):
         """(synthesis, analysis) threads per workgroup of each ring-FFT group."""
         n = 64

@@ -23,7 +23,7 @@
 
 namespace shtg {
 
-enum { ST_OK = 0, ST_DIMS = 1, ST_BOUNDS = 2, ST_ALLOC = 3, ST_OTHER = 5 };
+enum { ST_OK = 0, ST_DIMS = 1, ST_BOUNDS = 2, ST_ALLOC = 3, ST_OTHER = 5, ST_CAPTURE = 6 };
 
 // Largest LDS footprint a ring-FFT workgroup may use (gfx950: 160 KiB), and
 // the elements of a workspace FFT's LDS chunk of that size.
@@ -279,12 +279,17 @@ static uint64_t fnv1a(const void *p, size_t nbytes, uint64_t h = 146959810393466
 // GLQ geometry (src/MakeGridGLQ.f95:467-480,546-553): slot s pairs row s
 // (z = zero(s)) with row nlat-1-s; for odd nlat the middle row is an exact
 // equator (z = 0).  Weights 2 pi w / nlong (src/SHExpandGLQ.f95:535-547).
+static uint64_t glq_key(int L, const double *zero, const double *wq)
+{
+    const int nlat = L + 1;
+    const uint64_t h = fnv1a(zero, sizeof(double) * nlat);
+    return wq ? fnv1a(wq, sizeof(double) * nlat, h) : fnv1a("nw", 2, h);
+}
+
 static Geometry *get_glq(int dev, int L, const double *zero, const double *wq)
 {
     const int nlat = L + 1;
-    uint64_t h = fnv1a(zero, sizeof(double) * nlat);
-    h = wq ? fnv1a(wq, sizeof(double) * nlat, h) : fnv1a("nw", 2, h);
-    auto &slot = g_glq[{dev, L, h}];
+    auto &slot = g_glq[{dev, L, glq_key(L, zero, wq)}];
     if (slot) return slot.get();
     slot.reset(new Geometry());
     Geometry &g = *slot;
@@ -786,6 +791,7 @@ struct BatchPlan {
     int n_unpack = 0;
     std::vector<FftGroup> fft;
     FftGroup fft_ws;              // fields whose ring FFTs run on the global workspace
+    std::vector<int> fft_group_of;  // per field: its index in fft (fft.size(): the workspace group)
     double2 *fws = nullptr;       // that workspace: kWsGroups slices of fws_stride
     long long fws_stride = 0;
     int fft_slots = 2;
@@ -798,15 +804,29 @@ struct BatchPlan {
     double2 *cbuf = nullptr;  // packed coefficients (synthesis) / sums (analysis)
     double4 *fbuf = nullptr;  // per-order ring sums
     // Field pointers bound at slots 0..3 (synthesis in, synthesis out,
-    // analysis in, analysis out): a device table per slot and its host copy.
-    // A table a graph capture used is pinned: rebinding the slot afterwards
-    // uploads to a new table, so replays of that graph keep their pointers.
-    struct PtrSlot {
+    // analysis in, analysis out).  Each slot keeps a ring of kPtrRing device
+    // tables; a new pointer list goes to the next table of the ring, copied
+    // in stream order (hipMemcpyAsync on the call's stream from the table's
+    // pinned host copy) after the host has waited for that table's previous
+    // use (an event recorded after the launches that read it), so work still
+    // queued on the stream keeps reading the table it was launched with.  A
+    // table a graph capture used is pinned: it is never written again, and
+    // the ring replaces it by a new table when it comes round.
+    static constexpr int kPtrRing = 4;
+    struct PtrTable {
         const void **dev = nullptr;
-        std::vector<const void *> host;
+        const void **host = nullptr;  // pinned staging of the upload
+        hipEvent_t done = nullptr;    // after the last launches that read it
+        bool recorded = false;
         bool pinned = false;
+        std::vector<const void *> ptrs;
+    };
+    struct PtrSlot {
+        PtrTable ring[kPtrRing];
+        int cur = -1;  // the bound table
+        const void **dev() const { return cur >= 0 ? ring[cur].dev : nullptr; }
     } slot[4];
-    std::vector<void *> old_tables;  // pinned tables replaced by a rebind
+    std::vector<PtrTable> old_tables;  // pinned tables replaced in their ring
     BTables T{};
 };
 
@@ -817,9 +837,21 @@ static T *dev_upload_n(const T *v, size_t n)
     return dev_upload(tmp);
 }
 
+static void free_ptr_table(BatchPlan::PtrTable &t)
+{
+    if (t.done) (void)hipEventSynchronize(t.done);
+    (void)hipFree((void *)t.dev);
+    if (t.host) (void)hipHostFree((void *)t.host);
+    if (t.done) (void)hipEventDestroy(t.done);
+    t = BatchPlan::PtrTable{};
+}
+
 static void batch_free(BatchPlan *b)
 {
     if (!b) return;
+    for (auto &s : b->slot)
+        for (auto &t : s.ring) free_ptr_table(t);
+    for (auto &t : b->old_tables) free_ptr_table(t);
     (void)hipFree(b->fields);
     (void)hipFree(b->items_pack);
     (void)hipFree(b->items_sleg);
@@ -839,8 +871,6 @@ static void batch_free(BatchPlan *b)
     for (auto &p : b->hplans) free_fft(p);
     (void)hipFree(b->cbuf);
     (void)hipFree(b->fbuf);
-    for (auto &t : b->slot) (void)hipFree((void *)t.dev);
-    for (auto p : b->old_tables) (void)hipFree(p);
     delete b;
 }
 
@@ -1007,7 +1037,9 @@ static int batch_create_dh(int nfields, const int *lmax, const int *sampling, in
         group_lds[key] = std::max(group_lds[key], need);
         for (int s = 0; s < f.P; s += b->fft_slots) groups[key].push_back({(double)need, i, s, 0});
     }
+    b->fft_group_of.assign(nfields, (int)groups.size());
     for (auto &kv : groups) {
+        for (auto &x : kv.second) b->fft_group_of[x.a] = (int)b->fft.size();
         std::stable_sort(kv.second.begin(), kv.second.end(), by_work);
         std::vector<int2> it;
         for (auto &x : kv.second) it.push_back(make_int2(x.a, x.b));
@@ -1054,30 +1086,57 @@ static bool capturing(hipStream_t st)
 }
 
 // Bind field pointers p[0..nf) to slot k (0: synthesis in, 1: synthesis
-// out, 2: analysis in, 3: analysis out) for launches on stream st.  A new
-// list is uploaded (synchronously, so it cannot be bound under graph
-// capture: capture after an eager call with the same arrays); a list bound
-// under capture pins its table.
+// out, 2: analysis in, 3: analysis out) for launches on stream st.  An
+// unchanged list keeps its table.  A new list goes to the next table of the
+// slot's ring (see BatchPlan::PtrTable), uploaded in stream order; it cannot
+// be bound under graph capture (ST_CAPTURE: capture after an eager call with
+// the same arrays), and a list bound under capture pins its table.
 static int batch_bind(BatchPlan *b, int k, const void *const *p, hipStream_t st)
 {
     for (int i = 0; i < b->nf; ++i)
         if (!p[i]) return ST_DIMS;
-    BatchPlan::PtrSlot &t = b->slot[k];
+    BatchPlan::PtrSlot &s = b->slot[k];
     const bool cap = capturing(st);
-    if (t.dev && std::equal(p, p + b->nf, t.host.begin())) {
-        if (cap) t.pinned = true;
+    if (s.cur >= 0 && std::equal(p, p + b->nf, s.ring[s.cur].ptrs.begin())) {
+        if (cap) s.ring[s.cur].pinned = true;
         return ST_OK;
     }
-    if (cap) return ST_OTHER;
-    if (!t.dev || t.pinned) {
-        if (t.dev) b->old_tables.push_back((void *)t.dev);
-        t.dev = nullptr;
-        t.pinned = false;
-        if (!HIP_OK(hipMalloc((void **)&t.dev, sizeof(void *) * b->nf))) return ST_ALLOC;
+    if (cap) return ST_CAPTURE;
+    const int next = (s.cur + 1) % BatchPlan::kPtrRing;
+    BatchPlan::PtrTable &t = s.ring[next];
+    if (t.pinned) {
+        // a captured graph reads this table: keep it until the plan goes
+        b->old_tables.push_back(t);
+        t = BatchPlan::PtrTable{};
     }
-    if (!HIP_OK(hipMemcpy((void *)t.dev, p, sizeof(void *) * b->nf, hipMemcpyHostToDevice))) return ST_OTHER;
-    t.host.assign(p, p + b->nf);
+    const size_t bytes = sizeof(void *) * (size_t)b->nf;
+    if (!t.dev) {
+        if (!HIP_OK(hipMalloc((void **)&t.dev, bytes))) { t.dev = nullptr; return ST_ALLOC; }
+        if (!HIP_OK(hipHostMalloc((void **)&t.host, bytes, hipHostMallocDefault))) { t.host = nullptr; return ST_ALLOC; }
+        if (!HIP_OK(hipEventCreateWithFlags(&t.done, hipEventDisableTiming))) { t.done = nullptr; return ST_OTHER; }
+    }
+    // the table's previous launches (kPtrRing binds ago) must be done with it
+    // and with its staging copy
+    if (t.recorded && !HIP_OK(hipEventSynchronize(t.done))) return ST_OTHER;
+    std::copy(p, p + b->nf, t.host);
+    if (!HIP_OK(hipMemcpyAsync((void *)t.dev, t.host, bytes, hipMemcpyHostToDevice, st))) return ST_OTHER;
+    t.ptrs.assign(p, p + b->nf);
+    t.recorded = false;
+    s.cur = next;
     return ST_OK;
+}
+
+// After the launches of a call that read slots k0 and k1: mark their bound
+// tables' last use (not under capture: those tables are pinned instead).
+static void batch_mark_used(BatchPlan *b, int k0, int k1, hipStream_t st)
+{
+    if (capturing(st)) return;
+    for (int k : {k0, k1}) {
+        BatchPlan::PtrSlot &s = b->slot[k];
+        if (s.cur < 0) continue;
+        BatchPlan::PtrTable &t = s.ring[s.cur];
+        t.recorded = HIP_OK(hipEventRecord(t.done, st));
+    }
 }
 
 // workgroups covering n items under xcd_item's mapping (multiple of 32)
@@ -1087,8 +1146,8 @@ static int xcd_blocks(int n) { return ((n + 31) / 32) * 32; }
 // scales the coefficients by post and the rows by the analysis weights).
 static int batch_synthesis(BatchPlan *b, hipStream_t st, const BTables &T)
 {
-    const double *const *cin = (const double *const *)b->slot[0].dev;
-    double *const *gout = (double *const *)b->slot[1].dev;
+    const double *const *cin = (const double *const *)b->slot[0].dev();
+    double *const *gout = (double *const *)b->slot[1].dev();
     {
         ProfScope ps(SHTG_K_PACK, st);
         hipLaunchKernelGGL(kb_pack, dim3(b->n_pack), dim3(kThreads), 0, st, b->fields, b->items_pack, T, cin,
@@ -1109,6 +1168,7 @@ static int batch_synthesis(BatchPlan *b, hipStream_t st, const BTables &T)
                                kMaxLds, st, b->fields, b->fft_ws.items, b->fft_ws.nitems, b->fft_slots, T, b->fbuf,
                                gout, b->fws, b->fws_stride, kChunkCap);
     }
+    batch_mark_used(b, 0, 1, st);
     return HIP_OK(hipGetLastError()) ? ST_OK : ST_OTHER;
 }
 
@@ -1116,8 +1176,8 @@ static int batch_synthesis(BatchPlan *b, hipStream_t st, const BTables &T)
 // uses unit weights, every slot and the recursion's s as epilogue).
 static int batch_analysis(BatchPlan *b, hipStream_t st, const BTables &T)
 {
-    const double *const *gin = (const double *const *)b->slot[2].dev;
-    double *const *cout = (double *const *)b->slot[3].dev;
+    const double *const *gin = (const double *const *)b->slot[2].dev();
+    double *const *cout = (double *const *)b->slot[3].dev();
     {
         ProfScope ps(SHTG_K_ANAL_FFT, st);
         for (auto &g : b->fft)
@@ -1140,6 +1200,7 @@ static int batch_analysis(BatchPlan *b, hipStream_t st, const BTables &T)
         hipLaunchKernelGGL(kb_unpack, dim3(b->n_unpack), dim3(kThreads), 0, st, b->fields, b->items_unpack, T,
                            b->cbuf, cout);
     }
+    batch_mark_used(b, 2, 3, st);
     return HIP_OK(hipGetLastError()) ? ST_OK : ST_OTHER;
 }
 
@@ -1152,8 +1213,8 @@ static int batch_analysis(BatchPlan *b, hipStream_t st, const BTables &T)
 // and the plan's workspace; the caller reruns its transforms afterwards.
 static int batch_tune(BatchPlan *b, hipStream_t st, int reps)
 {
-    const double *const *gin = (const double *const *)b->slot[2].dev;
-    double *const *gout = (double *const *)b->slot[1].dev;
+    const double *const *gin = (const double *const *)b->slot[2].dev();
+    double *const *gout = (double *const *)b->slot[1].dev();
     if (!gin || !gout) return ST_OTHER;  // needs a prior synthesis and analysis
     hipEvent_t e0, e1;
     if (!HIP_OK(hipEventCreate(&e0)) || !HIP_OK(hipEventCreate(&e1))) return ST_OTHER;
@@ -1312,11 +1373,33 @@ struct CallTables {
     const DevSeeds *sd = nullptr;
 };
 
+// Whether a call's tables exist already (nothing to build or upload).
+static bool tables_ready(int dev, bool dh, int lmax, int sampling, const double *zero, const double *wq, int norm,
+                         int csphase, int Lc, bool cplx)
+{
+    if (!kernels_configured) return false;
+    auto r = g_rec.find({dev, norm});
+    if (r == g_rec.end() || r->second.L < Lc) return false;
+    auto s = g_seeds.find({dev, norm + (cplx ? 8 : 0), csphase});
+    if (s == g_seeds.end() || s->second.L < Lc) return false;
+    if (dh) {
+        auto g = g_dh.find({dev, lmax, sampling});
+        return g != g_dh.end() && g->second && g->second->ok;
+    }
+    auto g = g_glq.find({dev, lmax, glq_key(lmax, zero, wq)});
+    return g != g_glq.end() && g->second && g->second->ok;
+}
+
+// Under graph capture a call may not build tables (device allocations and
+// synchronous uploads are not allowed there): it needs an eager call of the
+// same degree, grid and normalisation before the capture, else ST_CAPTURE.
 static int get_tables(bool dh, int lmax, int sampling, const double *zero, const double *wq, int norm,
-                      int csphase, int Lc, CallTables &t, bool cplx = false)
+                      int csphase, int Lc, CallTables &t, hipStream_t st, bool cplx = false)
 {
     const int dev = current_device();
     if (dev < 0) return ST_OTHER;
+    if (capturing(st) && !tables_ready(dev, dh, lmax, sampling, zero, wq, norm, csphase, Lc, cplx))
+        return ST_CAPTURE;
     configure_kernels();
     t.g = dh ? get_dh(dev, lmax, sampling) : get_glq(dev, lmax, zero, wq);
     t.rt = get_recurrence(dev, norm, Lc);
@@ -1362,7 +1445,8 @@ int shtg_makegrid_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax, int
     if (!cilm || !grid) return ST_DIMS;
     std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
-    if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t))) return st;
+    if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t,
+                         (hipStream_t)stream))) return st;
     return launch_synthesis(*t.g, *t.rt, *t.sd, cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend, grid, grid_d1,
                             (hipStream_t)stream);
 }
@@ -1376,7 +1460,8 @@ int shtg_shexpand_dh(const double *grid, int grid_d0, int grid_d1, int norm, int
     if (!cilm || !grid) return ST_DIMS;
     std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
-    if ((st = get_tables(true, sh.lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t))) return st;
+    if ((st = get_tables(true, sh.lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t,
+                         (hipStream_t)stream))) return st;
     return launch_analysis(*t.g, *t.rt, *t.sd, grid, grid_d1, sh.Lc, cilm, cilm_d1, cilm_d2, (hipStream_t)stream);
 }
 
@@ -1391,7 +1476,7 @@ int shtg_makegrid_glq(const double *cilm, int cilm_d1, int cilm_d2, int lmax, co
     if (!cilm || !grid) return ST_DIMS;
     std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
-    if ((st = get_tables(false, lmax, 0, zero, nullptr, norm, csphase, sh.Lc, t))) return st;
+    if ((st = get_tables(false, lmax, 0, zero, nullptr, norm, csphase, sh.Lc, t, (hipStream_t)stream))) return st;
     return launch_synthesis(*t.g, *t.rt, *t.sd, cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend, grid, grid_d1,
                             (hipStream_t)stream);
 }
@@ -1407,7 +1492,7 @@ int shtg_shexpand_glq(const double *grid, int grid_d0, int grid_d1, const double
     if (!cilm || !grid) return ST_DIMS;
     std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
-    if ((st = get_tables(false, sh.lmax, 0, zero, w, norm, csphase, sh.Lc, t))) return st;
+    if ((st = get_tables(false, sh.lmax, 0, zero, w, norm, csphase, sh.Lc, t, (hipStream_t)stream))) return st;
     return launch_analysis(*t.g, *t.rt, *t.sd, grid, grid_d1, sh.Lc, cilm, cilm_d1, cilm_d2, (hipStream_t)stream);
 }
 
@@ -1422,7 +1507,8 @@ int shtg_makegrid_dh_adjoint(const double *grid_bar, int grid_d0, int grid_d1, i
     if (!cilm_bar || !grid_bar) return ST_DIMS;
     std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
-    if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t))) return st;
+    if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t,
+                         (hipStream_t)stream))) return st;
     return adjoint_of_synthesis(t, sh, grid_bar, grid_d1, cilm_bar, cilm_d1, cilm_d2, (hipStream_t)stream);
 }
 
@@ -1435,7 +1521,8 @@ int shtg_shexpand_dh_adjoint(const double *cilm_bar, int cilm_d1, int cilm_d2, i
     if (!cilm_bar || !grid_bar) return ST_DIMS;
     std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
-    if ((st = get_tables(true, sh.lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t))) return st;
+    if ((st = get_tables(true, sh.lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t,
+                         (hipStream_t)stream))) return st;
     return adjoint_of_analysis(t, sh, cilm_bar, cilm_d1, cilm_d2, grid_bar, grid_d0, grid_d1, (hipStream_t)stream);
 }
 
@@ -1450,7 +1537,7 @@ int shtg_makegrid_glq_adjoint(const double *grid_bar, int grid_d0, int grid_d1, 
     if (!cilm_bar || !grid_bar) return ST_DIMS;
     std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
-    if ((st = get_tables(false, lmax, 0, zero, nullptr, norm, csphase, sh.Lc, t))) return st;
+    if ((st = get_tables(false, lmax, 0, zero, nullptr, norm, csphase, sh.Lc, t, (hipStream_t)stream))) return st;
     return adjoint_of_synthesis(t, sh, grid_bar, grid_d1, cilm_bar, cilm_d1, cilm_d2, (hipStream_t)stream);
 }
 
@@ -1465,7 +1552,7 @@ int shtg_shexpand_glq_adjoint(const double *cilm_bar, int cilm_d1, int cilm_d2, 
     if (!cilm_bar || !grid_bar) return ST_DIMS;
     std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
-    if ((st = get_tables(false, sh.lmax, 0, zero, w, norm, csphase, sh.Lc, t))) return st;
+    if ((st = get_tables(false, sh.lmax, 0, zero, w, norm, csphase, sh.Lc, t, (hipStream_t)stream))) return st;
     return adjoint_of_analysis(t, sh, cilm_bar, cilm_d1, cilm_d2, grid_bar, grid_d0, grid_d1, (hipStream_t)stream);
 }
 
@@ -1479,7 +1566,8 @@ int shtg_makegrid_dhc(const double *cilm, int cilm_d1, int cilm_d2, int lmax, in
     if (!cilm || !grid) return ST_DIMS;
     std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
-    if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t, true))) return st;
+    if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t,
+                         (hipStream_t)stream, true))) return st;
     return launch_synthesis_c(*t.g, *t.rt, *t.sd, (const double2 *)cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend,
                               (double2 *)grid, grid_d1, (hipStream_t)stream);
 }
@@ -1493,7 +1581,8 @@ int shtg_shexpand_dhc(const double *grid, int grid_d0, int grid_d1, int norm, in
     if (!cilm || !grid) return ST_DIMS;
     std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
-    if ((st = get_tables(true, sh.lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t, true))) return st;
+    if ((st = get_tables(true, sh.lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t,
+                         (hipStream_t)stream, true))) return st;
     return launch_analysis_c(*t.g, *t.rt, *t.sd, (const double2 *)grid, grid_d1, sh.Lc, (double2 *)cilm, cilm_d1,
                              cilm_d2, (hipStream_t)stream);
 }
@@ -1509,7 +1598,7 @@ int shtg_makegrid_glqc(const double *cilm, int cilm_d1, int cilm_d2, int lmax, c
     if (!cilm || !grid) return ST_DIMS;
     std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
-    if ((st = get_tables(false, lmax, 0, zero, nullptr, norm, csphase, sh.Lc, t, true))) return st;
+    if ((st = get_tables(false, lmax, 0, zero, nullptr, norm, csphase, sh.Lc, t, (hipStream_t)stream, true))) return st;
     return launch_synthesis_c(*t.g, *t.rt, *t.sd, (const double2 *)cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend,
                               (double2 *)grid, grid_d1, (hipStream_t)stream);
 }
@@ -1525,7 +1614,7 @@ int shtg_shexpand_glqc(const double *grid, int grid_d0, int grid_d1, const doubl
     if (!cilm || !grid) return ST_DIMS;
     std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
-    if ((st = get_tables(false, sh.lmax, 0, zero, w, norm, csphase, sh.Lc, t, true))) return st;
+    if ((st = get_tables(false, sh.lmax, 0, zero, w, norm, csphase, sh.Lc, t, (hipStream_t)stream, true))) return st;
     return launch_analysis_c(*t.g, *t.rt, *t.sd, (const double2 *)grid, grid_d1, sh.Lc, (double2 *)cilm, cilm_d1,
                              cilm_d2, (hipStream_t)stream);
 }
@@ -1541,7 +1630,7 @@ int shtg_makegradient_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax,
     if (!cilm || !theta || !phi) return ST_DIMS;
     std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
-    if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, 1, 1, sh.Lc, t))) return st;
+    if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, 1, 1, sh.Lc, t, (hipStream_t)stream))) return st;
     hipStream_t hs = (hipStream_t)stream;
     const int L1 = sh.Lc + 1, n = 2 * lmax + 2;
     const size_t plane = (size_t)L1 * L1;
@@ -1680,7 +1769,7 @@ int shtg_makegravgrid_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax,
     if (!cilm || !rad || !theta || !phi || !total || !pot) return ST_DIMS;
     std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
-    if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, 1, 1, sh.Lc, t))) return st;
+    if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, 1, 1, sh.Lc, t, (hipStream_t)stream))) return st;
     const GravTables *gt = get_grav(current_device(), lmax, sampling, extend, r0, a, f, omega, normal_gravity_flag,
                                     normal_gravity_gm);
     if (!gt) return ST_ALLOC;
@@ -1752,7 +1841,8 @@ int shtg_shmultiply(double *cilmout, int out_d1, int out_d2, const double *cilm1
     const auto *zw = glq_cached(nlat);
     if (!zw) return ST_OTHER;
     CallTables t;
-    int st = get_tables(false, lmaxout, 0, zw->first.data(), zw->second.data(), norm, csphase, lmaxout, t);
+    int st = get_tables(false, lmaxout, 0, zw->first.data(), zw->second.data(), norm, csphase, lmaxout, t,
+                        (hipStream_t)stream);
     if (st) return st;
     hipStream_t hs = (hipStream_t)stream;
     const size_t ng = (size_t)nlat * nlong;
@@ -1972,6 +2062,16 @@ int shtg_batch_get_shapes(shtg_batch_plan *plan, int *threads_synth, int *thread
         if (threads_anal) threads_anal[i] = b->fft[i].threads_anal;
     }
     return ng;
+}
+
+int shtg_batch_field_groups(shtg_batch_plan *plan, int *group, int n)
+{
+    if (!plan || !group) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    if (n < b->nf) return ST_DIMS;
+    for (int i = 0; i < b->nf; ++i) group[i] = b->fft_group_of[i];
+    return (int)b->fft.size() + (b->fft_ws.nitems ? 1 : 0);
 }
 
 int shtg_profile_enable(int on)

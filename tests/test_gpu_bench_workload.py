@@ -1,0 +1,101 @@
+"""The batched benchmark workload itself: 512 DH2 fields, lmax ~ U{63..1023}
+(seed 3), spectral slopes ~ U[-4, -2], 4pi normalisation -- the fields, the
+tuned plan and the captured graph the benchmark times.
+
+Every field's grid and coefficients are bitwise those of the plain path
+(MakeGridDH / SHExpandDH one field per call, which the parity tests compare
+with the oracle up to lmax 2799), and in every ring-FFT group of the plan
+(one launch and LDS size class each; the 1024-thread class of the largest
+rings included) the fields of the lowest and highest degree are compared
+with the oracle: grids max|d|/max|ref| <= 1e-12, coefficients
+||d|| / ||ref|| <= 1e-12 (src/MakeGridDH.f95:504-643, src/SHExpandDH.f95:
+463-702)."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import coef_err, grid_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def workload(torch):
+    from shtools_amd.batch import DHBatch
+    from shtools_amd.synthetic import mixed_lmax_workload, power_law_cilm
+    spec, rng = mixed_lmax_workload(nfields=512)
+    Ls = [L for L, _ in spec]
+    host = [power_law_cilm(rng, L, s) for L, s in spec]
+    cilm = [torch.from_numpy(c).cuda() for c in host]
+    batch = DHBatch(Ls, sampling=2, norm=1, csphase=1)
+    grids = [torch.empty(batch.grid_shape(i), dtype=torch.float64, device="cuda") for i in range(len(Ls))]
+    outs = [torch.empty(batch.cilm_shape(i), dtype=torch.float64, device="cuda") for i in range(len(Ls))]
+    print(f"case bench workload: {len(Ls)} fields, lmax {min(Ls)}..{max(Ls)}", flush=True)
+    batch.tune(cilm, grids, outs, reps=1)
+    graph = batch.capture(cilm, grids, outs)
+    for g, c in zip(grids, outs):
+        g.fill_(np.nan)
+        c.fill_(np.nan)
+    torch.cuda.synchronize()
+    graph.replay()
+    torch.cuda.synchronize()
+    return Ls, host, cilm, batch, grids, outs
+
+
+def test_every_field_equals_plain_path(torch, workload):
+    """Grids bitwise.  Coefficients bitwise where both launches sum a field's
+    ring pairs in the same passes (fields up to lmax ~250); above that the
+    plain call, which has only its own orders to fill the GPU with, splits
+    each order's ring pairs into more blocks than the batched launch does, so
+    the same sums are associated differently: equal to 1e-14 there."""
+    from shtools_amd import backend as B
+    Ls, _, cilm, _, grids, outs = workload
+    print("case bench workload vs plain path", flush=True)
+    nbit = 0
+    for i, L in enumerate(Ls):
+        g = B.MakeGridDH(cilm[i], sampling=2)
+        c = B.SHExpandDH(grids[i], sampling=2)
+        torch.cuda.synchronize()
+        assert torch.equal(g, grids[i]), (i, L)
+        if torch.equal(c, outs[i]):
+            nbit += 1
+        else:
+            assert L > 250, (i, L)
+            assert float(torch.linalg.norm(c - outs[i]) / torch.linalg.norm(c)) <= 1e-14, (i, L)
+        del g, c
+    print(f"  coefficients bitwise equal for {nbit} of {len(Ls)} fields", flush=True)
+
+
+def test_fields_of_every_fft_group_against_oracle(torch, workload):
+    Ls, host, _, batch, grids, outs = workload
+    groups = batch.field_groups()
+    picks = set()
+    for grp in sorted(set(groups)):
+        members = [i for i in range(len(Ls)) if groups[i] == grp]
+        picks.add(min(members, key=lambda i: Ls[i]))
+        picks.add(max(members, key=lambda i: Ls[i]))
+    # at least two fields of the largest rings (the 1024-thread class)
+    big = sorted((i for i in range(len(Ls)) if Ls[i] > 850), key=lambda i: -Ls[i])
+    assert len(big) >= 2
+    picks.update(big[:2])
+    shapes = batch.shapes()
+    assert any(t == 1024 for pair in shapes for t in pair) or max(Ls) <= 850
+    for i in sorted(picks, key=lambda i: Ls[i]):
+        L = Ls[i]
+        print(f"case bench workload field {i} lmax {L} group {groups[i]}", flush=True)
+        g_ref = oracle.MakeGridDH(host[i], sampling=2)
+        ge = grid_err(grids[i].cpu().numpy(), g_ref)
+        c_ref = oracle.SHExpandDH(g_ref, sampling=2, parallel=True)
+        ce = coef_err(outs[i].cpu().numpy(), c_ref)
+        print(f"  grid err {ge:.2e} coef err {ce:.2e}", flush=True)
+        assert ge <= 1e-12, (i, L)
+        assert ce <= 1e-12, (i, L)
+        assert coef_err(outs[i].cpu().numpy(), host[i]) <= 1e-11, (i, L)

@@ -112,3 +112,63 @@ def test_host_threads_on_their_own_streams(torch):
         for i, g, c2 in outs[k]:
             assert torch.equal(g, ref_g[i]), (k, i)
             assert torch.equal(c2, ref_c[i]), (k, i)
+
+
+def test_batched_calls_on_a_side_stream_keep_their_arrays(torch):
+    """Back-to-back batched calls on one plan with different arrays on a
+    non-blocking stream, without any synchronisation in between: each call
+    reads and writes the arrays it was given (its pointer table is written in
+    stream order, never under work still queued), more calls than the ring of
+    tables holds included."""
+    from shtools_amd import backend as B
+    from shtools_amd.batch import DHBatch
+    print("case batched side stream rebinding", flush=True)
+    rng = np.random.default_rng(34)
+    Ls = [700, 31, 255, 5, 500, 64]
+    batch = DHBatch(Ls, sampling=2)
+    sets = [[torch.from_numpy(rand_cilm(rng, L)).cuda() for L in Ls] for _ in range(7)]
+    refs = [[B.MakeGridDH(c, sampling=2) for c in cs] for cs in sets]
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    grids, backs = [], []
+    with torch.cuda.stream(s):
+        for cs in sets:
+            g = batch.makegrid(cs)          # new output arrays each call
+            grids.append(g)
+            backs.append(batch.expand(g))
+    s.synchronize()
+    for k, cs in enumerate(sets):
+        for i in range(len(Ls)):
+            assert torch.equal(grids[k][i], refs[k][i]), (k, i)
+            assert float(torch.linalg.norm(backs[k][i] - cs[i]) / torch.linalg.norm(cs[i])) <= 1e-11, (k, i)
+
+
+def test_cold_call_under_capture_raises_cleanly(torch):
+    """A plain call whose grid tables do not exist yet (degree 337, sampling
+    1: no other test uses it) cannot build them under graph capture: it
+    raises (status 6) without enqueuing anything; after one eager call at
+    that degree the same call captures and replays."""
+    from shtools_amd import backend as B
+    print("case cold capture", flush=True)
+    rng = np.random.default_rng(35)
+    B.MakeGridDH(torch.from_numpy(rand_cilm(rng, 20)).cuda(), sampling=2)  # library initialised
+    torch.cuda.synchronize()
+    L = 337
+    c = torch.from_numpy(rand_cilm(rng, L)).cuda()
+    graph = torch.cuda.CUDAGraph()
+    with pytest.raises(B.SHToolsError, match="graph capture"):
+        with torch.cuda.graph(graph):
+            B.MakeGridDH(c, sampling=1, norm=2)
+    torch.cuda.synchronize()
+    ref = B.MakeGridDH(c, sampling=1, norm=2)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        g = B.MakeGridDH(c, sampling=1, norm=2)
+    for _ in range(2):
+        g.fill_(np.nan)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(g, ref)

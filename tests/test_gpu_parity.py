@@ -180,3 +180,29 @@ def test_dh_l2047_roundtrip(B):
     g = B.MakeGridDH(ct, norm=4, sampling=1)
     c2 = B.SHExpandDH(g, norm=4, sampling=1)
     assert coef_err(c2.cpu().numpy(), c) <= 1e-11
+
+
+@pytest.mark.parametrize("L", [0, 1, 2, 5, 12])
+@pytest.mark.parametrize("kind", ["dh1", "dh2", "glq"])
+def test_analysis_of_arbitrary_grids_vs_brute_force_quadrature(B, L, kind):
+    """The HIP analysis of grids with independent random values at every
+    node (not band-limited) against the mpmath quadrature fixtures
+    (tests/golden/make_brute_analysis.py), every normalisation and phase:
+    max error in the orthonormal scale <= 1e-13 of the largest coefficient."""
+    import os
+    from test_oracle import _orth_scale
+    print(f"case brute-force analysis {kind} L={L}", flush=True)
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "brute_analysis.npz"),
+                allow_pickle=False)
+    g = d[f"{kind}_L{L}_grid"]
+    for norm in (1, 2, 3, 4):
+        for cs in (1, -1):
+            ref = d[f"{kind}_L{L}_n{norm}_cs{cs}"]
+            if kind == "glq":
+                zero, w = B.SHGLQ(L)
+                c = B.SHExpandGLQ(g, w, zero, norm=norm, csphase=cs)
+            else:
+                c = B.SHExpandDH(g, norm=norm, sampling=1 if kind == "dh1" else 2, csphase=cs)
+            s = _orth_scale(L, norm)
+            err = np.abs((c - ref) * s).max() / np.abs(ref * s).max()
+            assert err <= 1e-13, (norm, cs, err)

@@ -19,6 +19,7 @@
  *     of this library's own: 6, the call would have to build device tables
  *     or bind new arrays while its stream is captured into a HIP graph
  *     (make one eager call of the same shape on the same arrays first);
+ *     7 and 8 come from shtg_makegrid_dh_noalloc only (see there);
  *   - norm 1 = 4pi (geodesy), 2 = Schmidt, 3 = unnormalised, 4 = orthonormal;
  *     csphase 1 omits and -1 applies the Condon-Shortley phase (-1)^m;
  *   - calls are thread-safe; host-side tables are cached per (norm, csphase,
@@ -40,6 +41,20 @@ extern "C" {
 int shtg_makegrid_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax, int norm,
                      int sampling, int csphase, int lmax_calc, int extend,
                      double *grid, int grid_d0, int grid_d1, void *stream);
+
+/* MakeGridDH without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Arguments
+ * and results are those of shtg_makegrid_dh.  The intermediates live in a
+ * 64 MiB device array of the library that each GPU holds until the process
+ * ends (enough for lmax up to about 1290); a call that needs more returns 7.
+ * The call builds no tables either: the degree, sampling and normalisation
+ * must have been used by an ordinary call on this device first, else it
+ * returns 8.  All calls on one GPU share that GPU's array: calls that
+ * overlap in time on one GPU (several streams or host threads) race. */
+int shtg_makegrid_dh_noalloc(const double *cilm, int cilm_d1, int cilm_d2, int lmax, int norm,
+                             int sampling, int csphase, int lmax_calc, int extend,
+                             double *grid, int grid_d0, int grid_d1, void *stream);
 
 /* SHExpandDH: src/SHExpandDH.f95:1-2 (include/shtools.h:92-102;
  * src/pyshtools.pyf:235-254).  n = grid_d0 must be even; the grid has at

@@ -17,6 +17,8 @@ _vp = ctypes.c_void_p
 
 SIGNATURES = {
     "shtg_makegrid_dh": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _vp],
+    "shtg_makegrid_dh_noalloc": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp,
+                                 _c_int, _c_int, _vp],
     "shtg_shexpand_dh": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _vp],
     "shtg_makegrid_glq": [_vp, _c_int, _c_int, _c_int, _dp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int,
                           _c_int, _vp],

@@ -59,6 +59,11 @@ def _status_message(status):
         # this library's own status: no counterpart in the reference
         return ("Call not allowed under HIP graph capture: it needs device tables or array bindings that "
                 "do not exist yet; make one eager call of the same shape on the same arrays first.")
+    if status == 7:
+        return "The no-allocation synthesis needs more scratch than its fixed device array (64 MiB) holds."
+    if status == 8:
+        return ("The no-allocation synthesis found no device tables for this degree, sampling and "
+                "normalisation: make one ordinary MakeGridDH call with them on this device first.")
     return "Unhandled Fortran 95 error."
 
 

@@ -23,7 +23,8 @@
 
 namespace shtg {
 
-enum { ST_OK = 0, ST_DIMS = 1, ST_BOUNDS = 2, ST_ALLOC = 3, ST_OTHER = 5, ST_CAPTURE = 6 };
+enum { ST_OK = 0, ST_DIMS = 1, ST_BOUNDS = 2, ST_ALLOC = 3, ST_OTHER = 5, ST_CAPTURE = 6, ST_SCRATCH = 7,
+       ST_NOTABLES = 8 };
 
 // Largest LDS footprint a ring-FFT workgroup may use (gfx950: 160 KiB), and
 // the elements of a workspace FFT's LDS chunk of that size.
@@ -558,10 +559,27 @@ struct RhoRings {
     const double *z = nullptr, *u = nullptr, *r2 = nullptr;
 };
 
+// Intermediates of a synthesis supplied by the caller (the no-allocation
+// entry point's scratch) instead of the per-device buffer.
+struct ExtWorkspace {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+// Bytes of a synthesis' intermediates: packed coefficients, per-order ring
+// sums and, for rings beyond the LDS, the FFT workspace.
+static size_t synthesis_bytes(const Geometry &g, int Lc)
+{
+    const size_t npack = (size_t)tri_size(Lc);
+    const size_t nf = (size_t)(Lc + 1) * g.P;
+    const size_t nws = fft_in_lds(g.fft.m) ? 0 : (size_t)std::min(g.P, kWsGroups) * ws_stride(g.fft.m);
+    return npack * sizeof(double2) + nf * sizeof(double4) + nws * sizeof(double2);
+}
+
 static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const DevSeeds &sd, const double *cilm,
                             int d1, int d2, int Lc, int nrows_out, int ext_col, double *grid, long long ld,
                             hipStream_t st, const double *ctab = nullptr, const double2 *roww = nullptr,
-                            const RhoRings &ring = RhoRings())
+                            const RhoRings &ring = RhoRings(), const ExtWorkspace &ext = ExtWorkspace())
 {
     if (!ctab) ctab = rt.s;
     const double *rz = ring.r2 ? ring.z : g.z, *ru = ring.r2 ? ring.u : g.u;
@@ -572,10 +590,14 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
     const size_t nf = (size_t)(Lc + 1) * g.P;
     const bool lds_fft = fft_in_lds(g.fft.m);
     const int nwg = std::min(g.P, kWsGroups);
-    const size_t nws = lds_fft ? 0 : (size_t)nwg * ws_stride(g.fft.m);
     WsLease lease;
-    const size_t bytes = npack * sizeof(double2) + nf * sizeof(double4) + nws * sizeof(double2);
-    if (int rc = ws_acquire(lease, bytes, st)) return rc;
+    const size_t bytes = synthesis_bytes(g, Lc);
+    if (ext.p) {
+        if (bytes > ext.cap) return ST_SCRATCH;
+        lease.p = ext.p;
+    } else if (int rc = ws_acquire(lease, bytes, st)) {
+        return rc;
+    }
     void *ws = lease.p;
     double2 *cpack = (double2 *)ws;
     double4 *fbuf = (double4 *)((char *)ws + npack * sizeof(double2));
@@ -605,7 +627,7 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
                                g.rows, nrows_out, grid, ld, ext_col, roww, fws, ws_stride(g.fft.m), kmax, kChunkCap);
     }
     const bool ok = HIP_OK(hipGetLastError());
-    ws_release(lease, st);
+    if (!ext.p) ws_release(lease, st);
     if (!ok) return ST_OTHER;
     return ST_OK;
 }
@@ -1449,6 +1471,66 @@ int shtg_makegrid_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax, int
                          (hipStream_t)stream))) return st;
     return launch_synthesis(*t.g, *t.rt, *t.sd, cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend, grid, grid_d1,
                             (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ no-allocation synthesis
+//
+// shtg_makegrid_dh_noalloc keeps a synthesis' intermediates in `scratch`, a
+// device array of kScratchBytes that every GPU the library runs on holds
+// until the process ends (each GPU its own copy at its own address).  In
+// host code the array's name stands for no device address: its address on
+// a device comes from hipGetSymbolAddress, taken once per device with that
+// device current (on the entry point's first call there) and kept in
+// g_scratch, and only that address is passed to kernels.  Every call on a
+// GPU shares that GPU's array: calls that overlap in time on one GPU (other
+// streams, other host threads) race on it.  The synthesis writes every
+// scratch element it later reads, so nothing needs clearing between calls.
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static void *g_scratch[kMaxDevices] = {};
+
+static void *scratch_address(int dev)
+{
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        if (!HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) || !p) return nullptr;
+        g_scratch[dev] = p;
+    }
+    return g_scratch[dev];
+}
+
+extern "C" {
+
+int shtg_makegrid_dh_noalloc(const double *cilm, int cilm_d1, int cilm_d2, int lmax, int norm, int sampling,
+                             int csphase, int lmax_calc, int extend, double *grid, int grid_d0, int grid_d1,
+                             void *stream)
+{
+    Shape sh;
+    int st = check_makegrid_dh(cilm_d1, cilm_d2, lmax, norm, sampling, csphase, lmax_calc, extend, grid_d0, grid_d1,
+                               sh);
+    if (st) return st;
+    if (!cilm || !grid) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int dev = 0;
+    if (!HIP_OK(hipGetDevice(&dev))) return ST_OTHER;
+    // only tables an ordinary call has built already (building them would
+    // allocate)
+    if (!tables_ready(dev, true, lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, false)) return ST_NOTABLES;
+    void *ws = scratch_address(dev);
+    if (!ws) return ST_OTHER;
+    const Geometry &g = *g_dh.find({dev, lmax, sampling})->second;
+    if (synthesis_bytes(g, sh.Lc) > kScratchBytes) return ST_SCRATCH;
+    const DevRecurrence &rt = g_rec.find({dev, norm})->second;
+    const DevSeeds &sd = g_seeds.find({dev, norm, csphase})->second;
+    ExtWorkspace ext;
+    ext.p = ws;
+    ext.cap = kScratchBytes;
+    return launch_synthesis(g, rt, sd, cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend, grid, grid_d1,
+                            (hipStream_t)stream, nullptr, nullptr, RhoRings(), ext);
 }
 
 int shtg_shexpand_dh(const double *grid, int grid_d0, int grid_d1, int norm, int sampling, int csphase,

@@ -378,7 +378,7 @@ def main():
     if not args.loop and not args.no_tune:
         t0 = time.perf_counter()
         shapes = batch.tune(cilm, grids, outs)
-        print(f"[bench] tuned ring-FFT workgroups in {time.perf_counter() - t0:.1f} s: {shapes}",
+        print(f"[bench] tuned launches in {time.perf_counter() - t0:.1f} s: {shapes}",
               file=sys.stderr, flush=True)
 
     # the per-field loop of the plain kernels, for the batched-vs-loop ratio
@@ -575,7 +575,7 @@ def main():
         "loop_fields_per_s": loop_rate,
         "batched_vs_loop": (value / loop_rate) if loop_rate else None,
         "stages": stages,
-        "fft_workgroups": shapes,
+        "tuning": shapes,
         "other_configs": configs,
         "cpu_baseline": cpu,
     }

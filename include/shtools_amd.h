@@ -207,14 +207,39 @@ int shtg_batch_makegrid_dh_adjoint(shtg_batch_plan *plan, const double *const *g
 int shtg_batch_shexpand_dh_adjoint(shtg_batch_plan *plan, const double *const *cilm_bar,
                                    double *const *grid_bar, void *stream);
 int shtg_batch_destroy(shtg_batch_plan *plan);
+/* Same-degree buckets on the matrix cores (no counterpart in the
+ * reference): fields of one degree and sampling, in groups of up to 32, whose
+ * group has at least min_bucket fields have their synthesis Legendre sums
+ * computed as FP64 MFMA products (one recursion per ring pair and order
+ * shared by the group); min_bucket = 0 turns this off (the default).  The
+ * grids are bitwise those of the per-field path (the FP64 MFMA accumulates
+ * its products as an in-order chain of fused multiply-adds, the order of
+ * the per-field sums).  `stream` is unused (kept for symmetry). */
+int shtg_batch_set_mfma(shtg_batch_plan *plan, int min_bucket, void *stream);
 /* Launch-shape autotuner (no counterpart in the reference): times each
  * ring-FFT group of the plan at 256, 512 and 1024 threads per workgroup
- * (best of `reps` runs, synthesis and analysis separately) on the arrays
- * bound by the plan's last makegrid and shexpand calls, and keeps the
- * fastest.  Synchronises `stream`; overwrites those grids and the plan's
- * workspace, so rerun the transforms afterwards.  Returns 5 if the plan has
- * not been called in both directions yet. */
-int shtg_batch_tune(shtg_batch_plan *plan, int reps, void *stream);
+ * (best of `reps` runs, synthesis and analysis separately), then the
+ * Legendre launches: ring pairs per lane of the synthesis waves (2, 3, 4)
+ * and the same-degree bucket size from which the synthesis goes to the MFMA
+ * kernel (off, 2, 4, 8, 16, where the plan has such buckets), on the arrays
+ * bound by the plan's last makegrid and shexpand calls, keeping the fastest
+ * of each; none of these changes any result.  flags bit 0 also tunes ring
+ * pairs per lane of the analysis waves (4, 6, 8), which regroups the
+ * analysis sums (coefficients then change by rounding).  Synchronises
+ * `stream`; overwrites those grids and the plan's workspace, so rerun the
+ * transforms afterwards.  Returns 5 if the plan has not been called in both
+ * directions yet. */
+int shtg_batch_tune(shtg_batch_plan *plan, int reps, int flags, void *stream);
+/* The Legendre launch choices of a plan (from shtg_batch_tune, or set here
+ * to reuse an earlier tuning): ring pairs per lane of the synthesis waves
+ * (1 .. 4) and of the analysis waves (1 .. 8), and the MFMA bucket size
+ * (0: off, else >= 2; see shtg_batch_set_mfma).  Results do not depend on
+ * them beyond rounding (analysis sums are associated by pass). */
+int shtg_batch_get_tuning(shtg_batch_plan *plan, int *synth_kmax, int *anal_kmax, int *mfma_min);
+int shtg_batch_set_tuning(shtg_batch_plan *plan, int synth_kmax, int anal_kmax, int mfma_min);
+/* Set the ring-FFT workgroup sizes (256, 512 or 1024 per group; n = the
+ * number of groups shtg_batch_get_shapes returns). */
+int shtg_batch_set_shapes(shtg_batch_plan *plan, const int *threads_synth, const int *threads_anal, int n);
 /* Workgroup sizes of the plan's ring-FFT groups (up to n of them); returns
  * the number of groups. */
 int shtg_batch_get_shapes(shtg_batch_plan *plan, int *threads_synth, int *threads_anal, int n);

@@ -60,9 +60,13 @@ SIGNATURES = {
     "shtg_batch_makegrid_dh_adjoint": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp],
     "shtg_batch_shexpand_dh_adjoint": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp],
     "shtg_batch_destroy": [_vp],
-    "shtg_batch_tune": [_vp, _c_int, _vp],
+    "shtg_batch_tune": [_vp, _c_int, _c_int, _vp],
     "shtg_batch_get_shapes": [_vp, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), _c_int],
     "shtg_batch_field_groups": [_vp, ctypes.POINTER(_c_int), _c_int],
+    "shtg_batch_set_mfma": [_vp, _c_int, _vp],
+    "shtg_batch_get_tuning": [_vp, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)],
+    "shtg_batch_set_tuning": [_vp, _c_int, _c_int, _c_int],
+    "shtg_batch_set_shapes": [_vp, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), _c_int],
     "shtg_profile_enable": [_c_int],
     "shtg_profile_collect": [_dp, ctypes.POINTER(ctypes.c_longlong), _c_int],
     "shtg_device_count": [],

@@ -19,6 +19,29 @@ def _ptr_array(tensors):
     return arr
 
 
+def _read_cache(path):
+    import json
+    import os
+    if not path or not os.path.exists(path):
+        return {}
+    with open(path) as fh:
+        try:
+            return json.load(fh)
+        except ValueError:
+            return {}
+
+
+def _write_cache(path, key, value):
+    import json
+    import os
+    data = _read_cache(path)
+    data[key] = value
+    tmp = f"{path}.{os.getpid()}.tmp"
+    with open(tmp, "w") as fh:
+        json.dump(data, fh, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
 class DHBatch:
     """Plan for the fields of degrees ``lmax``; ``sampling`` is one value for
     every field or a sequence with one per field."""
@@ -151,21 +174,77 @@ class DHBatch:
             self.expand(grids, out=outs)
         return graph
 
-    def tune(self, cilms, grids, outs, reps=3):
-        """Autotune the ring-FFT workgroup sizes of this plan on these arrays
-        (timed on the device), then recompute grids and outs.  Returns the
-        chosen (synthesis, analysis) threads per FFT group."""
+    def tune(self, cilms, grids, outs, reps=3, cache=None, analysis=False):
+        """Autotune this plan's launches on these arrays (timed on the
+        device): ring-FFT workgroup sizes, ring pairs per lane of the
+        synthesis Legendre waves and the MFMA bucket size (shtg_batch_tune;
+        none of these changes any result), and with analysis=True ring pairs
+        per lane of the analysis waves (coefficients then change by
+        rounding); then recompute grids and outs.  Returns the choice (see
+        tuning()).
+
+        cache: path of a JSON file (default: $SHTOOLS_AMD_TUNE_CACHE, if set)
+        in which choices are kept per GPU architecture and plan (the fields'
+        degrees and samplings); a plan found there takes the stored choice
+        instead of being timed."""
+        import os
         torch = self.torch
+        cache = cache or os.environ.get("SHTOOLS_AMD_TUNE_CACHE")
+        key = self._tune_key() + (":a" if analysis else "") if cache else None
+        stored = _read_cache(cache).get(key) if cache else None
+        if stored is not None:
+            self.set_tuning(stored)
+        else:
+            self.makegrid(cilms, out=grids)
+            self.expand(grids, out=outs)
+            stream = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+            with torch.cuda.device(self.device):
+                st = _lib.lib().shtg_batch_tune(self._plan, int(reps), 1 if analysis else 0, stream)
+            if st != 0:
+                raise SHToolsError(_status_message(st))
+            if cache:
+                _write_cache(cache, key, self.tuning())
         self.makegrid(cilms, out=grids)
         self.expand(grids, out=outs)
-        stream = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
-        with torch.cuda.device(self.device):
-            st = _lib.lib().shtg_batch_tune(self._plan, int(reps), stream)
+        return self.tuning()
+
+    def _tune_key(self):
+        import hashlib
+        arch = self.torch.cuda.get_device_properties(self.device).gcnArchName.split(":")[0]
+        sig = ",".join(f"{L}/{s}" for L, s in zip(self.lmax, self.sampling))
+        return arch + ":" + hashlib.sha1(sig.encode()).hexdigest()[:16]
+
+    def tuning(self):
+        """The plan's launch choices: {"fft_threads": [(synthesis, analysis)
+        threads per ring-FFT group], "synth_kmax", "anal_kmax": ring pairs per
+        lane of the Legendre waves, "mfma_min": bucket size sent to the MFMA
+        kernel (0: none)}."""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        st = _lib.lib().shtg_batch_get_tuning(self._plan, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
         if st != 0:
             raise SHToolsError(_status_message(st))
-        self.makegrid(cilms, out=grids)
-        self.expand(grids, out=outs)
-        return self.shapes()
+        return {"fft_threads": self.shapes(), "synth_kmax": a.value, "anal_kmax": b.value, "mfma_min": c.value}
+
+    def set_tuning(self, t):
+        """Apply a choice returned by tuning() (for instance one stored by an
+        earlier run)."""
+        lib = _lib.lib()
+        st = lib.shtg_batch_set_tuning(self._plan, int(t["synth_kmax"]), int(t["anal_kmax"]), int(t["mfma_min"]))
+        if st == 0 and t.get("fft_threads"):
+            n = len(t["fft_threads"])
+            ts = (ctypes.c_int * n)(*[int(x[0]) for x in t["fft_threads"]])
+            ta = (ctypes.c_int * n)(*[int(x[1]) for x in t["fft_threads"]])
+            st = lib.shtg_batch_set_shapes(self._plan, ts, ta, n)
+        if st != 0:
+            raise SHToolsError(_status_message(st))
+
+    def set_mfma(self, min_bucket):
+        """Compute the synthesis Legendre sums of same-degree buckets of at
+        least ``min_bucket`` fields as FP64 MFMA products (0: off)."""
+        stream = ctypes.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
+        st = _lib.lib().shtg_batch_set_mfma(self._plan, int(min_bucket), stream)
+        if st != 0:
+            raise SHToolsError(_status_message(st))
 
     def field_groups(self):
         """The ring-FFT group of each field (fields of one group share a

@@ -1073,6 +1073,160 @@ __global__ __launch_bounds__(kWave) void kb_synth_leg(const BField *__restrict__
                                    fbuf + f.foff + (size_t)m * f.P);
 }
 
+// ---------------------------------------------- same-degree buckets on MFMA
+//
+// Fields of one degree and sampling (a bucket) share their rings and their
+// Legendre functions, so for an order m the synthesis sums of all of them
+// are one product
+//     F[s][col] = sum_j Q[s][j] W[j][col],  Q[s][j] = q_{m+j}(z_s),
+//     W[j][col] = packed coefficient (C s or S s) of field col / 2,
+// for the ring-pair slots s, split by the parity of j into the even and odd
+// sums whose sum and difference are the north and south rows.  One wave
+// takes 16 slots and every column of the bucket, 4 degrees of one parity at
+// a time on v_mfma_f64_16x16x4f64 (operand layouts: A[i][k] in lane
+// i + 16k, B[k][j] in lane j + 16k, D[i][j] in register i / 4 of lane
+// j + 16 (i % 4)).  Lane i computes the recursion of slot s0 + i % 16 shifted
+// by 2 (i / 16) degrees, so that at block t it holds q_{8t + 2g} and
+// q_{8t + 2g + 1} (g = i / 16): the A operands of the even and odd products
+// of that block.  The recursion is thus computed 4 times per slot (2 FP64
+// VALU operations per degree and lane), and the products run on the matrix
+// pipe: 2 x ceil(2B / 16) MFMAs per block of 8 degrees and 16 slots.
+//
+// The same sums as the VALU tile (synth_leg_tile), and bitwise the same
+// results: the recursion runs the same operations per slot, and the FP64
+// MFMA accumulates its 4 products per output as a chain of fused
+// multiply-adds in k order, which is the order of the VALU tile's even (odd)
+// degree sums.
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int kBucketMaxCols = 64;  // up to 32 fields per MFMA bucket wave (4 column tiles)
+
+// items: (bucket, m, first slot, 0); bf: the bucket's nbf fields.
+//
+// Every lane runs the recursion of its slot over the same degrees (the
+// alpha row is wave-uniform), 8 per block, and lane group g takes q_{8t+2g}
+// and q_{8t+2g+1} of the block as its A operands.  The coefficient operands
+// of the next block are loaded while the current block's products run.
+// (Two 16-slot tiles per wave, sharing each coefficient operand, measured
+// slower: more registers, fewer waves.)
+constexpr int kMfmaSlots = 16;  // slots per MFMA synthesis wave
+template <int NT>
+__device__ __forceinline__ void synth_leg_mfma_tile(const BField *__restrict__ fields, const int *__restrict__ bf,
+                                                    int nbf, const BField &f0, int m, int s0, const BTables &T,
+                                                    const double2 *__restrict__ cpack, double4 *__restrict__ fbuf)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int r = lane & 15, g = lane >> 4;
+    const bool g1 = (g & 1) != 0, g2 = (g & 2) != 0;
+    const int jmax = f0.L - m;
+    const double *__restrict__ al = T.alpha + tri_offd(m, T.Lt);
+    const int s = s0 + r;
+    const double z = (s < f0.P) ? T.z[f0.geo + s] : 0.0;
+    // this lane's B-operand column in each column tile: a field's packed C or
+    // S row of order m (nullptr past the bucket's fields)
+    const double *wp[NT];
+#pragma unroll
+    for (int c = 0; c < NT; ++c) {
+        const int col = 16 * c + r, k = col >> 1;
+        wp[c] = (k < nbf) ? reinterpret_cast<const double *>(cpack + fields[bf[k]].coff + tri_offd(m, f0.L)) + (col & 1)
+                          : nullptr;
+    }
+    auto load_w = [&](int t, double (&we)[NT], double (&wo)[NT]) {
+        const int je = 8 * t + 2 * g;
+#pragma unroll
+        for (int c = 0; c < NT; ++c) {
+            we[c] = (wp[c] && je <= jmax) ? wp[c][2 * je] : 0.0;
+            wo[c] = (wp[c] && je + 1 <= jmax) ? wp[c][2 * je + 2] : 0.0;
+        }
+    };
+    double q1 = 0.0, q2 = -T.seed[m];  // the first step yields the seed
+    v4d accE[NT], accO[NT];
+#pragma unroll
+    for (int c = 0; c < NT; ++c) {
+        accE[c] = v4d{0.0, 0.0, 0.0, 0.0};
+        accO[c] = v4d{0.0, 0.0, 0.0, 0.0};
+    }
+    const int nblk = (jmax + 8) / 8, nfull = (jmax + 1) / 8;
+    double we[NT], wo[NT];
+    load_w(0, we, wo);
+    for (int t = 0; t < nblk; ++t) {
+        const int j0 = 8 * t;
+        double q[8];
+        if (t < nfull) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const double qa = k == 0 ? q1 : q[k - 1], qb = k == 0 ? q2 : (k == 1 ? q1 : q[k - 2]);
+                q[k] = fma(al[j0 + k] * z, qa, -qb);
+            }
+        } else {
+            // the last, partial block: degrees past jmax continue bounded
+            // (alpha 0) and meet zero coefficients
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const double qa = k == 0 ? q1 : q[k - 1], qb = k == 0 ? q2 : (k == 1 ? q1 : q[k - 2]);
+                q[k] = fma((j0 + k <= jmax ? al[j0 + k] : 0.0) * z, qa, -qb);
+            }
+        }
+        q1 = q[7];
+        q2 = q[6];
+        const double qe = g2 ? (g1 ? q[6] : q[4]) : (g1 ? q[2] : q[0]);
+        const double qo = g2 ? (g1 ? q[7] : q[5]) : (g1 ? q[3] : q[1]);
+        double ne[NT], no[NT];
+        if (t + 1 < nblk) load_w(t + 1, ne, no);
+#pragma unroll
+        for (int c = 0; c < NT; ++c) {
+            accE[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(qe, we[c], accE[c], 0, 0, 0);
+            accO[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(qo, wo[c], accO[c], 0, 0, 0);
+        }
+        if (t + 1 < nblk) {
+#pragma unroll
+            for (int c = 0; c < NT; ++c) {
+                we[c] = ne[c];
+                wo[c] = no[c];
+            }
+        }
+    }
+    // D[i][j]: slot s0 + 4 * reg + g, column 16 c + r
+#pragma unroll
+    for (int c = 0; c < NT; ++c) {
+        const int col = 16 * c + r, k = col >> 1;
+        if (k >= nbf) continue;
+        const BField &f = fields[bf[k]];
+        double *base = reinterpret_cast<double *>(fbuf + f.foff + (size_t)m * f.P) + (col & 1);
+#pragma unroll
+        for (int reg = 0; reg < 4; ++reg) {
+            const int ss = s0 + 4 * reg + g;
+            if (ss < f.P) {
+                const double e = accE[c][reg], o = accO[c][reg];
+                base[4 * (size_t)ss] = e + o;
+                base[4 * (size_t)ss + 2] = e - o;
+            }
+        }
+    }
+}
+
+// one wave per item (bucket, m, first slot, -); bfields: per bucket
+// kBucketMaxCols / 2 field indices, nbf: fields per bucket
+__global__ __launch_bounds__(kWave) void kb_synth_leg_mfma(const BField *__restrict__ fields,
+                                                           const int4 *__restrict__ items,
+                                                           const int *__restrict__ bfields,
+                                                           const int *__restrict__ nbf, BTables T,
+                                                           const double2 *__restrict__ cpack,
+                                                           double4 *__restrict__ fbuf)
+{
+    const int4 it = items[blockIdx.x];
+    const int *bf = bfields + it.x * (kBucketMaxCols / 2);
+    const int n = nbf[it.x];
+    const BField f0 = fields[bf[0]];
+    if (n <= 8)
+        synth_leg_mfma_tile<1>(fields, bf, n, f0, it.y, it.z, T, cpack, fbuf);
+    else if (n <= 16)
+        synth_leg_mfma_tile<2>(fields, bf, n, f0, it.y, it.z, T, cpack, fbuf);
+    else if (n <= 24)
+        synth_leg_mfma_tile<3>(fields, bf, n, f0, it.y, it.z, T, cpack, fbuf);
+    else
+        synth_leg_mfma_tile<4>(fields, bf, n, f0, it.y, it.z, T, cpack, fbuf);
+}
+
 // Ring FFT items (field, first slot): each workgroup transforms nslots
 // consecutive ring pairs in turn.  Workgroups are mapped so that groups of 4
 // consecutive items run on the same XCD (shared L2 lines of fbuf/gbuf).

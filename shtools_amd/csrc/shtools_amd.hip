@@ -807,6 +807,17 @@ struct BatchPlan {
     int n_pack = 0;
     int4 *items_sleg = nullptr;
     int n_sleg = 0;
+    // synthesis of same-degree buckets on MFMA (kb_synth_leg_mfma): buckets of
+    // at least mfma_min fields (0: off) leave the VALU items
+    int mfma_min = 0;
+    int synth_kmax = kSynthKMax;  // ring pairs per lane of the synthesis Legendre waves (1 .. kSynthKMax)
+    int anal_kmax = kAnalKMax;    // ring pairs per lane and pass of the analysis waves (1 .. kAnalKMax)
+    int4 *items_smfma = nullptr;
+    int n_smfma = 0;
+    int *bfields = nullptr, *bnf = nullptr;  // per bucket: field indices, count
+    std::vector<BField> hfields;             // host copies of fields and polar-skip rows
+    std::vector<int> hsmin;
+    std::vector<void *> retired;             // replaced item lists (freed with the plan)
     int2 *items_aleg[kAnalKMax] = {};  // by ring pairs per lane K = index + 1
     int n_aleg[kAnalKMax] = {};
     int4 *items_unpack = nullptr;
@@ -877,6 +888,10 @@ static void batch_free(BatchPlan *b)
     (void)hipFree(b->fields);
     (void)hipFree(b->items_pack);
     (void)hipFree(b->items_sleg);
+    (void)hipFree(b->items_smfma);
+    (void)hipFree(b->bfields);
+    (void)hipFree(b->bnf);
+    for (void *p : b->retired) (void)hipFree(p);
     for (auto p : b->items_aleg) (void)hipFree(p);
     (void)hipFree(b->items_unpack);
     for (auto &g : b->fft) (void)hipFree(g.items);
@@ -894,6 +909,107 @@ static void batch_free(BatchPlan *b)
     (void)hipFree(b->cbuf);
     (void)hipFree(b->fbuf);
     delete b;
+}
+
+struct W {
+    double w;
+    int a, b, c;
+};
+static bool by_work(const W &x, const W &y) { return x.w > y.w; }
+
+// Synthesis Legendre work items of a plan, longest first: buckets of at least
+// mfma_min fields of one degree and sampling (mfma_min >= 2; 0: none) as
+// (bucket, m, kMfmaSlots-slot tile) items of kb_synth_leg_mfma, every other field as
+// (field, m, slot block, ring pairs per lane) items of kb_synth_leg.
+static int batch_synth_items(BatchPlan *b, int mfma_min)
+{
+    const std::vector<BField> &fields = b->hfields;
+    const int nfields = (int)fields.size();
+    std::map<std::pair<int, int>, std::vector<int>> groups;  // (L, nlong) -> fields
+    for (int i = 0; i < nfields; ++i) groups[{fields[i].L, fields[i].nlong}].push_back(i);
+    std::vector<int> in_bucket(nfields, -1);
+    std::vector<int> bf, bn;
+    if (mfma_min >= 2)
+        for (auto &kv : groups) {
+            const std::vector<int> &mem = kv.second;
+            for (size_t k0 = 0; k0 + (size_t)mfma_min <= mem.size(); k0 += kBucketMaxCols / 2) {
+                const int cnt = (int)std::min(mem.size() - k0, (size_t)(kBucketMaxCols / 2));
+                if (cnt < mfma_min) break;
+                const int id = (int)bn.size();
+                bn.push_back(cnt);
+                for (int k = 0; k < kBucketMaxCols / 2; ++k) bf.push_back(k < cnt ? mem[k0 + k] : mem[k0]);
+                for (int k = 0; k < cnt; ++k) in_bucket[mem[k0 + k]] = id;
+            }
+        }
+    std::vector<W> ws, wm;
+    for (int i = 0; i < nfields; ++i) {
+        const BField &f = fields[i];
+        const int *sm = b->hsmin.data() + f.sm;
+        if (in_bucket[i] >= 0) {
+            if (bf[(size_t)in_bucket[i] * (kBucketMaxCols / 2)] != i) continue;  // the bucket's first field
+            for (int m = 0; m <= f.L; ++m)
+                for (int s0 = sm[m] / 16 * 16; s0 < f.P; s0 += kMfmaSlots)
+                    wm.push_back({(double)(f.L - m + 1) * bn[in_bucket[i]], in_bucket[i], m, s0});
+            continue;
+        }
+        for (int m = 0; m <= f.L; ++m) {
+            // blocks of kSynthKMax*64 slots from the order's first slot with
+            // non-negligible functions; the last one packs its lanes tightly
+            for (int s0 = sm[m]; s0 < f.P; s0 += b->synth_kmax * kWave) {
+                const int kk = std::min(b->synth_kmax, (f.P - s0 + kWave - 1) / kWave);
+                ws.push_back({(double)(f.L - m + 1) * std::min(kk * kWave, f.P - s0), i, m, s0 + (kk << 24)});
+            }
+        }
+    }
+    std::stable_sort(ws.begin(), ws.end(), by_work);
+    std::stable_sort(wm.begin(), wm.end(), by_work);
+    std::vector<int4> is, im;
+    for (auto &x : ws) is.push_back(make_int4(x.a, x.b, x.c & 0xFFFFFF, x.c >> 24));
+    for (auto &x : wm) im.push_back(make_int4(x.a, x.b, x.c, 0));
+    // the replaced lists may still be read by queued work or captured graphs:
+    // they are kept until the plan is destroyed
+    for (void *p : {(void *)b->items_sleg, (void *)b->items_smfma, (void *)b->bfields, (void *)b->bnf})
+        if (p) b->retired.push_back(p);
+    b->items_sleg = nullptr;
+    b->items_smfma = nullptr;
+    b->bfields = b->bnf = nullptr;
+    b->n_sleg = (int)is.size();
+    b->n_smfma = (int)im.size();
+    if (b->n_sleg && !(b->items_sleg = dev_upload(is))) return ST_ALLOC;
+    if (b->n_smfma) {
+        b->items_smfma = dev_upload(im);
+        b->bfields = dev_upload(bf);
+        b->bnf = dev_upload(bn);
+        if (!b->items_smfma || !b->bfields || !b->bnf) return ST_ALLOC;
+    }
+    b->mfma_min = mfma_min;
+    return ST_OK;
+}
+
+// Analysis Legendre work items, longest first: one single-wave item (field,
+// m) per order, grouped by ring pairs per lane (at most b->anal_kmax) into
+// one launch each.  Replaced lists are kept until the plan is destroyed.
+static int batch_anal_items(BatchPlan *b)
+{
+    std::vector<W> wa;
+    for (int i = 0; i < (int)b->hfields.size(); ++i) {
+        const BField &f = b->hfields[i];
+        const int *sm = b->hsmin.data() + f.sm;
+        for (int m = 0; m <= f.L; ++m) {
+            const int sa = std::max(1, sm[m]);
+            wa.push_back({(double)(f.L - m + 1) * (f.P - sa), i, m, anal_lanes_per_slot(f.P - sa, b->anal_kmax)});
+        }
+    }
+    std::stable_sort(wa.begin(), wa.end(), by_work);
+    std::vector<int2> ia[kAnalKMax];
+    for (auto &x : wa) ia[x.c - 1].push_back(make_int2(x.a, x.b));
+    for (int k = 0; k < kAnalKMax; ++k) {
+        if (b->items_aleg[k]) b->retired.push_back(b->items_aleg[k]);
+        b->items_aleg[k] = nullptr;
+        b->n_aleg[k] = (int)ia[k].size();
+        if (b->n_aleg[k] && !(b->items_aleg[k] = dev_upload(ia[k]))) return ST_ALLOC;
+    }
+    return ST_OK;
 }
 
 // sampling: per field (1 or 2).
@@ -996,8 +1112,9 @@ static int batch_create_dh(int nfields, const int *lmax, const int *sampling, in
     if (!HIP_OK(hipMalloc(&b->fbuf, (size_t)foff * sizeof(double4)))) return ST_ALLOC;
 
     // work items, longest first
-    struct W { double w; int a, b, c; };
-    std::vector<W> wp, ws, wa, wu;
+    b->hfields = fields;
+    b->hsmin = smin;
+    std::vector<W> wp, wu;
     for (int i = 0; i < nfields; ++i) {
         const BField &f = fields[i];
         for (int m0 = 0; m0 <= f.L; m0 += kTM)
@@ -1005,37 +1122,16 @@ static int batch_create_dh(int nfields, const int *lmax, const int *sampling, in
                 wu.push_back({(double)f.L, i, m0, l0});
                 if (l0 + kTL - 1 >= m0) wp.push_back({(double)f.L, i, m0, l0});
             }
-        const std::vector<int> &sm = smin_of[key_of(i)];
-        for (int m = 0; m <= f.L; ++m) {
-            // blocks of kSynthKMax*64 slots from the order's first slot with
-            // non-negligible functions; the last one packs its lanes tightly
-            for (int s0 = sm[m]; s0 < f.P; s0 += kSynthKMax * kWave) {
-                const int kk = std::min(kSynthKMax, (f.P - s0 + kWave - 1) / kWave);
-                ws.push_back({(double)(f.L - m + 1) * std::min(kk * kWave, f.P - s0), i, m, s0 + (kk << 24)});
-            }
-            const int sa = std::max(1, sm[m]);
-            wa.push_back({(double)(f.L - m + 1) * (f.P - sa), i, m, anal_lanes_per_slot(f.P - sa)});
-        }
     }
-    auto by_work = [](const W &x, const W &y) { return x.w > y.w; };
     std::stable_sort(wp.begin(), wp.end(), by_work);
-    std::stable_sort(ws.begin(), ws.end(), by_work);
-    std::stable_sort(wa.begin(), wa.end(), by_work);
-    std::vector<int4> ip, iu, is;
-    std::vector<int2> ia[kAnalKMax];
+    std::vector<int4> ip, iu;
     for (auto &x : wp) ip.push_back(make_int4(x.a, x.b, x.c, 0));
-    for (auto &x : ws) is.push_back(make_int4(x.a, x.b, x.c & 0xFFFFFF, x.c >> 24));
-    for (auto &x : wa) ia[x.c - 1].push_back(make_int2(x.a, x.b));
     for (auto &x : wu) iu.push_back(make_int4(x.a, x.b, x.c, 0));
     b->items_pack = dev_upload(ip);
-    b->items_sleg = dev_upload(is);
-    for (int k = 0; k < kAnalKMax; ++k) {
-        b->n_aleg[k] = (int)ia[k].size();
-        if (b->n_aleg[k] && !(b->items_aleg[k] = dev_upload(ia[k]))) return ST_ALLOC;
-    }
+    if (int rc = batch_synth_items(b.get(), 0)) return rc;
+    if (int rc = batch_anal_items(b.get())) return rc;
     b->items_unpack = dev_upload(iu);
     b->n_pack = (int)ip.size();
-    b->n_sleg = (int)is.size();
     b->n_unpack = (int)iu.size();
     if (!b->items_pack || !b->items_sleg || !b->items_unpack) return ST_ALLOC;
 
@@ -1177,8 +1273,12 @@ static int batch_synthesis(BatchPlan *b, hipStream_t st, const BTables &T)
     }
     {
         ProfScope ps(SHTG_K_SYNTH_LEG, st);
-        hipLaunchKernelGGL(kb_synth_leg, dim3(b->n_sleg), dim3(kWave), 0, st, b->fields, b->items_sleg, T,
-                           b->cbuf, b->fbuf);
+        if (b->n_smfma)
+            hipLaunchKernelGGL(kb_synth_leg_mfma, dim3(b->n_smfma), dim3(kWave), 0, st, b->fields, b->items_smfma,
+                               b->bfields, b->bnf, T, b->cbuf, b->fbuf);
+        if (b->n_sleg)
+            hipLaunchKernelGGL(kb_synth_leg, dim3(b->n_sleg), dim3(kWave), 0, st, b->fields, b->items_sleg, T,
+                               b->cbuf, b->fbuf);
     }
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
@@ -1228,12 +1328,86 @@ static int batch_analysis(BatchPlan *b, hipStream_t st, const BTables &T)
 
 // ------------------------------------------------------------ autotuning
 
+// The Legendre launches of one direction on the plan's current items and
+// workspace (synthesis: packed coefficients in cbuf -> fbuf; analysis: fbuf
+// -> cbuf); best of reps, in ms (negative on a launch error).
+static float time_legendre(BatchPlan *b, hipStream_t st, int reps, bool synth, hipEvent_t e0, hipEvent_t e1)
+{
+    float best = 1e30f;
+    for (int r = 0; r < reps; ++r) {
+        (void)hipEventRecord(e0, st);
+        if (synth) {
+            if (b->n_smfma)
+                hipLaunchKernelGGL(kb_synth_leg_mfma, dim3(b->n_smfma), dim3(kWave), 0, st, b->fields, b->items_smfma,
+                                   b->bfields, b->bnf, b->T, b->cbuf, b->fbuf);
+            if (b->n_sleg)
+                hipLaunchKernelGGL(kb_synth_leg, dim3(b->n_sleg), dim3(kWave), 0, st, b->fields, b->items_sleg, b->T,
+                                   b->cbuf, b->fbuf);
+        } else {
+            for (int k = 0; k < kAnalKMax; ++k)
+                if (b->n_aleg[k])
+                    hipLaunchKernelGGL(batch_anal_leg_kernel(k + 1), dim3(b->n_aleg[k]), dim3(kWave), 0, st,
+                                       b->fields, b->items_aleg[k], b->T, b->fbuf, b->cbuf);
+        }
+        (void)hipEventRecord(e1, st);
+        float ms = 0.0f;
+        if (!HIP_OK(hipGetLastError()) || !HIP_OK(hipEventSynchronize(e1)) || !HIP_OK(hipEventElapsedTime(&ms, e0, e1)))
+            return -1.0f;
+        best = std::min(best, ms);
+    }
+    return best;
+}
+
+// Legendre part of the autotuner: ring pairs per lane of the synthesis
+// waves (2 .. kSynthKMax), then, when the plan has fields of equal degree
+// and sampling, the smallest bucket that goes to the MFMA kernel (off, 2, 4,
+// 8, 16 -- only thresholds that change which buckets qualify); each time the
+// fastest launch is kept.  These choices leave every result bitwise as it
+// was.  With tune_analysis, also ring pairs per lane of the analysis waves
+// (4, 6, 8), which regroups the sums over ring pairs (results change by
+// rounding).
+static int batch_tune_legendre(BatchPlan *b, hipStream_t st, int reps, bool tune_analysis)
+{
+    hipEvent_t e0, e1;
+    if (!HIP_OK(hipEventCreate(&e0)) || !HIP_OK(hipEventCreate(&e1))) return ST_OTHER;
+    int rc = ST_OK;
+    auto pick = [&](std::vector<int> cands, bool synth, std::function<int(int)> apply, int &keep) {
+        float best = 1e30f;
+        int best_v = keep;
+        for (int v : cands) {
+            if ((rc = apply(v))) return;
+            const float ms = time_legendre(b, st, reps, synth, e0, e1);
+            if (ms < 0.0f) { rc = ST_OTHER; return; }
+            if (ms < best) { best = ms; best_v = v; }
+        }
+        keep = best_v;
+        rc = apply(best_v);
+    };
+    int sk = b->synth_kmax, mm = b->mfma_min, ak = b->anal_kmax;
+    pick({2, 3, kSynthKMax}, true, [&](int v) { b->synth_kmax = v; return batch_synth_items(b, mm); }, sk);
+    // largest same-degree group of the plan
+    std::map<std::pair<int, int>, int> cnt;
+    int biggest = 0;
+    for (const BField &f : b->hfields) biggest = std::max(biggest, ++cnt[{f.L, f.nlong}]);
+    if (!rc && biggest >= 2) {
+        std::vector<int> thr = {0};
+        for (int t : {2, 4, 8, 16})
+            if (t <= biggest) thr.push_back(t);
+        pick(thr, true, [&](int v) { return batch_synth_items(b, v); }, mm);
+    }
+    if (!rc && tune_analysis)
+        pick({4, 6, kAnalKMax}, false, [&](int v) { b->anal_kmax = v; return batch_anal_items(b); }, ak);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
 // Launch-shape autotuner of a batch plan: for every ring-FFT group, times the
 // synthesis and the analysis kernels at 256, 512 and 1024 threads per
 // workgroup (the group's items and LDS are unchanged) on the plan's bound
 // arrays, and keeps the fastest of each.  The runs overwrite the bound grids
 // and the plan's workspace; the caller reruns its transforms afterwards.
-static int batch_tune(BatchPlan *b, hipStream_t st, int reps)
+static int batch_tune(BatchPlan *b, hipStream_t st, int reps, int flags)
 {
     const double *const *gin = (const double *const *)b->slot[2].dev();
     double *const *gout = (double *const *)b->slot[1].dev();
@@ -1277,7 +1451,8 @@ static int batch_tune(BatchPlan *b, hipStream_t st, int reps)
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    return st_out;
+    if (st_out) return st_out;
+    return batch_tune_legendre(b, st, reps, (flags & 1) != 0);
 }
 
 // ------------------------------------------------------------ argument checks
@@ -2126,11 +2301,11 @@ int shtg_batch_shexpand_dh_adjoint(shtg_batch_plan *plan, const double *const *c
     return batch_synthesis(b, (hipStream_t)stream, T);
 }
 
-int shtg_batch_tune(shtg_batch_plan *plan, int reps, void *stream)
+int shtg_batch_tune(shtg_batch_plan *plan, int reps, int flags, void *stream)
 {
-    if (!plan || reps < 1) return ST_BOUNDS;
+    if (!plan || reps < 1 || (flags & ~1)) return ST_BOUNDS;
     std::lock_guard<std::mutex> lk(g_mu);
-    return batch_tune(reinterpret_cast<BatchPlan *>(plan), (hipStream_t)stream, reps);
+    return batch_tune(reinterpret_cast<BatchPlan *>(plan), (hipStream_t)stream, reps, flags);
 }
 
 int shtg_batch_get_shapes(shtg_batch_plan *plan, int *threads_synth, int *threads_anal, int n)
@@ -2144,6 +2319,56 @@ int shtg_batch_get_shapes(shtg_batch_plan *plan, int *threads_synth, int *thread
         if (threads_anal) threads_anal[i] = b->fft[i].threads_anal;
     }
     return ng;
+}
+
+int shtg_batch_set_mfma(shtg_batch_plan *plan, int min_bucket, void *stream)
+{
+    if (!plan || min_bucket < 0 || min_bucket == 1) return ST_BOUNDS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    (void)stream;
+    return batch_synth_items(reinterpret_cast<BatchPlan *>(plan), min_bucket);
+}
+
+int shtg_batch_get_tuning(shtg_batch_plan *plan, int *synth_kmax, int *anal_kmax, int *mfma_min)
+{
+    if (!plan) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    if (synth_kmax) *synth_kmax = b->synth_kmax;
+    if (anal_kmax) *anal_kmax = b->anal_kmax;
+    if (mfma_min) *mfma_min = b->mfma_min;
+    return ST_OK;
+}
+
+int shtg_batch_set_tuning(shtg_batch_plan *plan, int synth_kmax, int anal_kmax, int mfma_min)
+{
+    if (!plan) return ST_DIMS;
+    if (synth_kmax < 1 || synth_kmax > kSynthKMax || anal_kmax < 1 || anal_kmax > kAnalKMax || mfma_min < 0 ||
+        mfma_min == 1)
+        return ST_BOUNDS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    b->synth_kmax = synth_kmax;
+    b->anal_kmax = anal_kmax;
+    int st = batch_synth_items(b, mfma_min);
+    if (!st) st = batch_anal_items(b);
+    return st;
+}
+
+int shtg_batch_set_shapes(shtg_batch_plan *plan, const int *threads_synth, const int *threads_anal, int n)
+{
+    if (!plan || !threads_synth || !threads_anal) return ST_DIMS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    if (n != (int)b->fft.size()) return ST_DIMS;
+    for (int i = 0; i < n; ++i)
+        for (int t : {threads_synth[i], threads_anal[i]})
+            if (t != kFftMaxThreads / 4 && t != kFftMaxThreads / 2 && t != kFftMaxThreads) return ST_BOUNDS;
+    for (int i = 0; i < n; ++i) {
+        b->fft[i].threads = threads_synth[i];
+        b->fft[i].threads_anal = threads_anal[i];
+    }
+    return ST_OK;
 }
 
 int shtg_batch_field_groups(shtg_batch_plan *plan, int *group, int n)

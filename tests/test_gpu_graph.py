@@ -52,8 +52,10 @@ def test_batch_tune_keeps_results():
     ref_c = [c.clone() for c in batch.expand(ref_g)]
     grids = [torch.empty_like(g) for g in ref_g]
     outs = [torch.empty_like(c) for c in ref_c]
-    shapes = batch.tune(cilms, grids, outs, reps=2)
-    assert shapes and all(t in (256, 512, 1024) for pair in shapes for t in pair)
+    t = batch.tune(cilms, grids, outs, reps=2)
+    shapes = t["fft_threads"]
+    assert shapes and all(x in (256, 512, 1024) for pair in shapes for x in pair)
+    assert 1 <= t["synth_kmax"] <= 4 and 1 <= t["anal_kmax"] <= 8 and t["mfma_min"] in (0, 2, 4, 8, 16)
     for i in range(len(Ls)):
         assert torch.equal(grids[i], ref_g[i])
         assert torch.equal(outs[i], ref_c[i])
@@ -95,3 +97,49 @@ def test_graph_keeps_its_arrays_after_rebinding():
             assert torch.equal(outs[i], ref_c[i]), (rep, i)
             assert not torch.equal(og[i], ref_g[i])
         del og, oc
+
+
+def test_tune_legendre_shapes_mfma_and_cache(tmp_path):
+    """The tuner's Legendre choices (synthesis ring pairs per lane, MFMA
+    buckets) on a plan with same-degree buckets keep every result bitwise;
+    with analysis=True the analysis regrouping changes coefficients by
+    rounding only.  A tuning stored in the cache file is applied to a new
+    plan of the same degrees without timing it again."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from shtools_amd.batch import DHBatch
+    print("case tune legendre", flush=True)
+    rng = np.random.default_rng(24)
+    Ls = [200] * 12 + [64] * 5 + [500, 31]
+    cilms = [torch.from_numpy(rand_cilm(rng, L)).cuda() for L in Ls]
+    cache = str(tmp_path / "tune.json")
+    batch = DHBatch(Ls, sampling=2)
+    ref_g = [g.clone() for g in batch.makegrid(cilms)]
+    ref_c = [c.clone() for c in batch.expand(ref_g)]
+    grids = [torch.empty_like(g) for g in ref_g]
+    outs = [torch.empty_like(c) for c in ref_c]
+    t = batch.tune(cilms, grids, outs, reps=1, cache=cache)
+    print(f"  tuned {t}", flush=True)
+    for i in range(len(Ls)):
+        assert torch.equal(grids[i], ref_g[i]), i
+        assert torch.equal(outs[i], ref_c[i]), i
+    # forcing each choice keeps the results bitwise too
+    for sk, mm in ((2, 0), (3, 2), (4, 8), (1, 16)):
+        batch.set_tuning(dict(t, synth_kmax=sk, mfma_min=mm))
+        g = batch.makegrid(cilms)
+        torch.cuda.synchronize()
+        for i in range(len(Ls)):
+            assert torch.equal(g[i], ref_g[i]), (sk, mm, i)
+    # the cache: a new plan of the same degrees takes the stored choice
+    other = DHBatch(Ls, sampling=2)
+    t2 = other.tune(cilms, grids, outs, reps=1, cache=cache)
+    assert t2 == t
+    for i in range(len(Ls)):
+        assert torch.equal(grids[i], ref_g[i]), i
+    # the analysis waves too: coefficients equal to rounding
+    ta = other.tune(cilms, grids, outs, reps=1, analysis=True)
+    assert ta["anal_kmax"] in (4, 6, 8)
+    for i in range(len(Ls)):
+        assert torch.equal(grids[i], ref_g[i]), i
+        assert float(torch.linalg.norm(outs[i] - ref_c[i]) / torch.linalg.norm(ref_c[i])) <= 1e-14, i

@@ -35,6 +35,7 @@ constexpr int kRegRadices[] = {2, 3, 4, 5, 8};
 struct FftPlanHost {
     int n = 0, m = 0, nst = 0, blue = 0;
     int lc = -1;                // pruning: orders |k| <= lc only (-1: none)
+    int half = 0;               // split plan: the ring's columns in two halves of n / 2 (see make_fft_plan)
     int radix[kMaxStages] = {0};
     int stride[kMaxStages] = {0};
     int twoff[kMaxStages] = {0};  // offset of each stage's twiddles in twst
@@ -44,6 +45,8 @@ struct FftPlanHost {
     std::vector<double> chirp;  // 2*n (Bluestein)
     std::vector<double> bhat;   // 2*m (Bluestein), DIF output order, /m
     std::vector<double> bhat2;  // pruned plans: analysis spectrum (bhat is synthesis)
+    std::vector<double> bhat3;  // split plans: synthesis spectrum of the second half
+    std::vector<double> bhat4;  // split plans: analysis spectrum of the second half
 };
 
 // Fewest-stage factorization of m over the register radices (and radix 7
@@ -200,14 +203,15 @@ inline void set_stages(FftPlanHost &p, const std::vector<int> &rad)
     if (p.twst.empty()) p.twst.assign(2, 0.0);
 }
 
-// Spectrum (DIF order, scaled by 1/m) of the conjugate chirp placed at the
-// positions p of signed lags t: p = t for t in [0, tpos], p = m + t for
-// t in [-tneg, -1].
-inline std::vector<double> chirp_spectrum(const FftPlanHost &p, int tpos, int tneg)
+// Spectrum (DIF order, scaled by 1/m) of the conjugate chirp of lag
+// t + shift placed at the positions p of signed lags t: p = t for t in
+// [0, tpos], p = m + t for t in [-tneg, -1].
+inline std::vector<double> chirp_spectrum(const FftPlanHost &p, int tpos, int tneg, int shift = 0)
 {
     const long double pi = 3.14159265358979323846264338327950288L;
     const int n = p.n;
     auto wconj = [&](long long t) {
+        t += shift;
         long long q = ((t % (2LL * n)) + 2LL * n) % (2LL * n);
         q = (q * q) % (2LL * n);
         long double a = pi * (long double)q / (long double)n;
@@ -225,11 +229,34 @@ inline std::vector<double> chirp_spectrum(const FftPlanHost &p, int tpos, int tn
     return out;
 }
 
+// Work length of the split plan of a pruned Bluestein ring (0 if there is
+// none): m >= n / 2 + 2 lc.
+inline int split_length(int n, int lc)
+{
+    if (lc < 0 || n % 2 != 0) return 0;
+    std::vector<int> rad;
+    if (factor_stages(n, true, rad)) return 0;  // a direct plan needs no Bluestein
+    return bluestein_length(n / 2 + 2 * lc);
+}
+
 // Plan for length n.  lc >= 0 (even n only) declares that the synthesis
 // input and the analysis output occupy only the orders |k| <= lc; Bluestein
 // then needs m >= n + 2 lc instead of 2n - 1 (signed indices: negative
 // orders sit at the top of the work array).
-inline FftPlanHost make_fft_plan(int n, int lc = -1)
+//
+// split (Bluestein rings with lc >= 0 only): the ring's n columns are taken
+// as two halves [0, n/2) and [n/2, n), each needing only a convolution of
+// n/2 + 2 lc lags, so m >= n/2 + 2 lc (about 2/3 of the above for
+// n = 4 (lc + 1)).  Synthesis transforms the chirped orders once and runs
+// the turn and the inverse stages once per half of the output columns, with
+// the spectrum kept in registers in between; analysis transforms each half
+// of the input columns, sums the two products with their chirp spectra in
+// the turn and runs the inverse stages once.  Three transforms of the
+// shorter length for two of the longer: the same work on 2/3 of the LDS.
+// bhat / bhat3 are the synthesis spectra of the two output halves (lags
+// shifted by 0 and n/2), bhat2 / bhat4 the analysis spectra of the two input
+// halves (shifted by 0 and -n/2).
+inline FftPlanHost make_fft_plan(int n, int lc = -1, bool split = false)
 {
     FftPlanHost p;
     p.n = n;
@@ -247,7 +274,8 @@ inline FftPlanHost make_fft_plan(int n, int lc = -1)
     p.blue = 1;
     const bool prune = lc >= 0 && (n % 2 == 0) && n + 2 * lc < 2 * n - 1;
     p.lc = prune ? lc : -1;
-    p.m = bluestein_length(prune ? n + 2 * lc : 2 * n - 1);
+    p.half = (split && prune) ? 1 : 0;
+    p.m = bluestein_length(p.half ? n / 2 + 2 * lc : prune ? n + 2 * lc : 2 * n - 1);
     factor_stages(p.m, false, rad);
     set_stages(p, rad);
     const long double pi = 3.14159265358979323846264338327950288L;
@@ -258,7 +286,13 @@ inline FftPlanHost make_fft_plan(int n, int lc = -1)
         p.chirp[2 * j] = (double)std::cos(a);
         p.chirp[2 * j + 1] = (double)std::sin(a);
     }
-    if (prune) {
+    if (p.half) {
+        const int h = n / 2;
+        p.bhat = chirp_spectrum(p, h - 1 + lc, lc, 0);    // synthesis, outputs [0, h) at 0 ..
+        p.bhat3 = chirp_spectrum(p, h - 1 + lc, lc, h);   // .. and [h, n) at 0
+        p.bhat2 = chirp_spectrum(p, lc, h - 1 + lc, 0);   // analysis, inputs [0, h) at 0 ..
+        p.bhat4 = chirp_spectrum(p, lc, h - 1 + lc, -h);  // .. and [h, n) at 0
+    } else if (prune) {
         p.bhat = chirp_spectrum(p, n - 1 + lc, lc);   // synthesis: inputs |j| <= lc, outputs [0, n)
         p.bhat2 = chirp_spectrum(p, lc, n - 1 + lc);  // analysis: inputs [0, n), outputs |k| <= lc
     } else {

@@ -381,10 +381,53 @@ __device__ __forceinline__ void synth_fft_fill(const FftDev &F, const SynthRing 
 // chunk != nullptr: lds is a work array in global memory and the DFT runs
 // its inner stages in the LDS chunk (chunk_cap elements) when a group of the
 // plan fits there.
+// Split plans (FftDev::half): the inverse stages run once per half of the
+// output columns, the forward spectrum kept in registers in between.
+__device__ __forceinline__ void synth_fft_run_split(const FftDev &F, int2 rw, int nrows, double *__restrict__ grid,
+                                                    long long ld, int ext_col, double2 rwt, double2 *lds)
+{
+    const int n = F.n, h = n / 2;
+    const bool north = rw.x >= 0 && rw.x < nrows;
+    const bool south = rw.y >= 0 && rw.y < nrows;
+    TurnStash keep;
+    fft_dif_front(lds, F);
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        if (half == 0)
+            fft_run_turn_split<kSynth1>(lds, F, F.bhat, keep);
+        else
+            fft_run_turn_split<kSynth2>(lds, F, F.bhat3, keep);
+        __syncthreads();
+        fft_dit_back(lds, F);
+        // columns j = half h + p at work positions p
+        for (int p = threadIdx.x; p < h; p += blockDim.x) {
+            const int j = half * h + p;
+            const double2 v = fft_get_c(lds, F, j, p, F.chirp[j]);
+            const double gn = v.x * rwt.x, gs = -v.y * rwt.y;
+            if (north) grid[rw.x * ld + j] = gn;
+            if (south) grid[rw.y * ld + j] = gs;
+            if (ext_col && j == 0) {
+                if (north) grid[rw.x * ld + n] = gn;
+                if (south) grid[rw.y * ld + n] = gs;
+            }
+        }
+        if (half == 0) __syncthreads();  // the second turn overwrites the array
+    }
+}
+
+//
+// kSplit: the kernel instantiation for split plans (FftDev::half); kept
+// apart so that the other instantiations do not carry the split flow's
+// registers.
+template <bool kSplit = false>
 __device__ __forceinline__ void synth_fft_run(const FftDev &F, int2 rw, int nrows, double *__restrict__ grid,
                                               long long ld, int ext_col, double2 rwt, double2 *lds,
                                               double2 *chunk = nullptr, int chunk_cap = 0)
 {
+    if constexpr (kSplit) {
+        synth_fft_run_split(F, rw, nrows, grid, ld, ext_col, rwt, lds);
+        return;
+    }
     const int n = F.n;
     if (chunk && F.nst >= 2 && F.m / F.radix[0] <= chunk_cap)
         fft_forward_chunked(lds, F, F.bhat, chunk, chunk_cap);
@@ -412,6 +455,7 @@ __device__ __forceinline__ void synth_fft_run(const FftDev &F, int2 rw, int nrow
     }
 }
 
+template <bool kSplit = false>
 __device__ __forceinline__ void synth_fft_tile(const FftDev &F, const double4 *__restrict__ fbuf, int P, int Lc,
                                                int s, double u, int2 rw, int nrows, double *__restrict__ grid,
                                                long long ld, int ext_col, double2 rwt, double2 *lds,
@@ -421,7 +465,7 @@ __device__ __forceinline__ void synth_fft_tile(const FftDev &F, const double4 *_
     synth_fft_load(fbuf, P, Lc, s, in);
     synth_fft_fill(F, in, fbuf, P, Lc, s, u, lds);
     __syncthreads();
-    synth_fft_run(F, rw, nrows, grid, ld, ext_col, rwt, lds, chunk, chunk_cap);
+    synth_fft_run<kSplit>(F, rw, nrows, grid, ld, ext_col, rwt, lds, chunk, chunk_cap);
 }
 
 // Analysis inputs of one thread: ring values at the column pairs q and
@@ -488,20 +532,14 @@ __device__ __forceinline__ void anal_fft_fill(const FftDev &F, const AnalRing &i
     for (int q = n + threadIdx.x; q < m; q += nt) lds[fpad(q)] = make_double2(0.0, 0.0);
 }
 
-// DFT of the filled array; w = (w_north, w_south) include the quadrature
-// weight and the reference's constant factors; the outputs carry the
-// order's rescale factor for the ring (sin(theta) = u).
-__device__ __forceinline__ void anal_fft_run(const FftDev &F, int2 rw, int nrows, double2 w, double u, int P, int Lc,
-                                             int s, double4 *__restrict__ gbuf, double2 *lds,
-                                             double2 *chunk = nullptr, int chunk_cap = 0)
+// The orders 0 .. Lc of the transformed ring pair: split into the two rings,
+// weighted, times the order's rescale factor, into gbuf.
+__device__ __forceinline__ void anal_fft_extract(const FftDev &F, int2 rw, int nrows, double2 w, double u, int P,
+                                                 int Lc, int s, double4 *__restrict__ gbuf, const double2 *lds)
 {
     const int n = F.n;
     const bool north = rw.x >= 0 && rw.x < nrows;
     const bool south = rw.y >= 0 && rw.y < nrows;
-    if (chunk && F.nst >= 2 && F.m / F.radix[0] <= chunk_cap)
-        fft_forward_chunked(lds, F, F.bhat2, chunk, chunk_cap);
-    else
-        fft_forward(lds, F, F.bhat2);
     const double wn = north ? w.x : 0.0, ws = south ? w.y : 0.0;
     // orders interleave over the threads (a wave reads consecutive elements)
     for (int mm = threadIdx.x; mm <= Lc; mm += blockDim.x) {
@@ -517,11 +555,66 @@ __device__ __forceinline__ void anal_fft_run(const FftDev &F, int2 rw, int nrows
     }
 }
 
+// DFT of the filled array; w = (w_north, w_south) include the quadrature
+// weight and the reference's constant factors; the outputs carry the
+// order's rescale factor for the ring (sin(theta) = u).
+__device__ __forceinline__ void anal_fft_run(const FftDev &F, int2 rw, int nrows, double2 w, double u, int P, int Lc,
+                                             int s, double4 *__restrict__ gbuf, double2 *lds,
+                                             double2 *chunk = nullptr, int chunk_cap = 0)
+{
+    if (chunk && F.nst >= 2 && F.m / F.radix[0] <= chunk_cap)
+        fft_forward_chunked(lds, F, F.bhat2, chunk, chunk_cap);
+    else
+        fft_forward(lds, F, F.bhat2);
+    anal_fft_extract(F, rw, nrows, w, u, P, Lc, s, gbuf, lds);
+}
+
+// Split plans (FftDev::half): each half of the ring's columns is filled and
+// transformed in turn; the first half's product with its chirp spectrum is
+// kept in registers and added in the second turn, then the inverse stages
+// run once and the orders are extracted as for the whole ring.
+__device__ __forceinline__ void anal_fft_tile_split(const FftDev &F, const double *__restrict__ grid, long long ld,
+                                                    int2 rw, int nrows, double2 w, double u, int P, int Lc, int s,
+                                                    double4 *__restrict__ gbuf, int ext_col, double2 *lds)
+{
+    const int n = F.n, h = n / 2, m = F.m;
+    const bool north = rw.x >= 0 && rw.x < nrows;
+    const bool south = rw.y >= 0 && rw.y < nrows;
+    TurnStash keep;
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        for (int q = threadIdx.x; q < m; q += blockDim.x) {
+            double2 v = make_double2(0.0, 0.0);
+            if (q < h) {
+                const int col = half * h + q;
+                v = fft_prep_in_c(F, anal_column(grid, ld, rw, north, south, n, ext_col, col), F.chirp[col]);
+            }
+            lds[fpad(q)] = v;
+        }
+        __syncthreads();
+        fft_dif_front(lds, F);
+        if (half == 0) {
+            fft_run_turn_split<kAnal1>(lds, F, F.bhat2, keep);
+            __syncthreads();  // the second half's fill overwrites the array
+        } else {
+            fft_run_turn_split<kAnal2>(lds, F, F.bhat4, keep);
+            __syncthreads();
+        }
+    }
+    fft_dit_back(lds, F);
+    anal_fft_extract(F, rw, nrows, w, u, P, Lc, s, gbuf, lds);
+}
+
+template <bool kSplit = false>
 __device__ __forceinline__ void anal_fft_tile(const FftDev &F, const double *__restrict__ grid, long long ld,
                                               int2 rw, int nrows, double2 w, double u, int P, int Lc, int s,
                                               double4 *__restrict__ gbuf, int ext_col, double2 *lds,
                                               double2 *chunk = nullptr, int chunk_cap = 0)
 {
+    if constexpr (kSplit) {
+        anal_fft_tile_split(F, grid, ld, rw, nrows, w, u, P, Lc, s, gbuf, ext_col, lds);
+        return;
+    }
     AnalRing in;
     anal_fft_load(F, grid, ld, rw, nrows, ext_col, in);
     anal_fft_fill(F, in, grid, ld, rw, nrows, ext_col, lds);
@@ -571,6 +664,7 @@ __global__ __launch_bounds__(kWave) void k_synth_leg(const double *__restrict__ 
 
 // rows[s] = (north row, south row or -1); rows >= nrows are skipped;
 // roww[s]: row weights (nullptr: 1).
+template <bool kSplit>
 __global__ __launch_bounds__(kFftMaxThreads) void k_synth_fft(FftDev F, const double4 *__restrict__ fbuf, int P, int Lc,
                                                         const double *__restrict__ ringu,
                                                         const int2 *__restrict__ rows, int nrows,
@@ -580,10 +674,11 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_synth_fft(FftDev F, const do
 {
     extern __shared__ double2 lds[];
     const int s = blockIdx.x;
-    synth_fft_tile(F, fbuf, P, slot_orders(kmax, s, Lc), s, ringu[s], rows[s], nrows, grid, ld, ext_col,
-                   roww ? roww[s] : make_double2(1.0, 1.0), lds);
+    synth_fft_tile<kSplit>(F, fbuf, P, slot_orders(kmax, s, Lc), s, ringu[s], rows[s], nrows, grid, ld, ext_col,
+                           roww ? roww[s] : make_double2(1.0, 1.0), lds);
 }
 
+template <bool kSplit>
 __global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft(FftDev F, const double *__restrict__ grid, long long ld,
                                                        const double *__restrict__ ringu,
                                                        const int2 *__restrict__ rows, int nrows,
@@ -593,8 +688,8 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft(FftDev F, const dou
 {
     extern __shared__ double2 lds[];
     const int s = blockIdx.x;
-    anal_fft_tile(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, slot_orders(kmax, s, Lc), s, gbuf, ext_col,
-                  lds);
+    anal_fft_tile<kSplit>(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, slot_orders(kmax, s, Lc), s, gbuf,
+                          ext_col, lds);
 }
 
 // Rings whose FFT work array does not fit the LDS of a CU: the same tiles
@@ -773,6 +868,7 @@ __device__ __forceinline__ double msign(int m) { return (m & 1) ? -1.0 : 1.0; }
 // Complex synthesis of slot s: for the north (ring 0) and south (ring 1)
 // row, X_k = F_k and X_{n-k} = (-1)^k G_k (F, G the sums of orders +k, -k,
 // times the order's rescale), inverse DFT via conj(DFT(conj X)).
+template <bool kSplit>
 __device__ __forceinline__ void synth_fft_c_tile(const FftDev &F, const double4 *__restrict__ fb_p,
                                                  const double4 *__restrict__ fb_m, int P, int Lc, int s, double u,
                                                  int2 rw, int nrows, double2 *__restrict__ grid, long long ld,
@@ -800,6 +896,29 @@ __device__ __forceinline__ void synth_fft_c_tile(const FftDev &F, const double4 
             }
         }
         __syncthreads();
+        if constexpr (kSplit) {
+            // split plan: the inverse stages once per half of the columns
+            TurnStash keep;
+            const int h = n / 2;
+            fft_dif_front(x, F);
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                if (half == 0)
+                    fft_run_turn_split<kSynth1>(x, F, F.bhat, keep);
+                else
+                    fft_run_turn_split<kSynth2>(x, F, F.bhat3, keep);
+                __syncthreads();
+                fft_dit_back(x, F);
+                for (int p = threadIdx.x; p < h; p += blockDim.x) {
+                    const int j = half * h + p;
+                    const double2 v = conjd(fft_get(x, F, j, p));
+                    grid[row * ld + j] = v;
+                    if (ext_col && j == 0) grid[row * ld + n] = v;
+                }
+                __syncthreads();
+            }
+            continue;
+        }
         if (chunked)
             fft_forward_chunked(x, F, F.bhat, chunk, chunk_cap);
         else
@@ -816,6 +935,7 @@ __device__ __forceinline__ void synth_fft_c_tile(const FftDev &F, const double4 
 // Complex analysis of slot s: DFT of each ring, weighted, orders +k and -k
 // (the latter with (-1)^k), times the order's rescale; the north pass
 // stores (f, f), the south pass turns them into (f_N + f_S, f_N - f_S).
+template <bool kSplit>
 __device__ __forceinline__ void anal_fft_c_tile(const FftDev &F, const double2 *__restrict__ grid, long long ld,
                                                 int2 rw, int nrows, double2 w, double u, int P, int Lc, int s,
                                                 double4 *__restrict__ gb_p, double4 *__restrict__ gb_m, double2 *x,
@@ -827,13 +947,35 @@ __device__ __forceinline__ void anal_fft_c_tile(const FftDev &F, const double2 *
         const int row = ring ? rw.y : rw.x;
         if (row < 0 || row >= nrows) continue;
         const double wr = ring ? w.y : w.x;
-        for (int q = threadIdx.x; q < m; q += blockDim.x)
-            x[fpad(q)] = (q < n) ? fft_prep_in(F, q, grid[row * ld + q]) : make_double2(0.0, 0.0);
-        __syncthreads();
-        if (chunked)
-            fft_forward_chunked(x, F, F.bhat2, chunk, chunk_cap);
-        else
-            fft_forward(x, F, F.bhat2);
+        if constexpr (kSplit) {
+            // split plan: each half of the columns filled and transformed in
+            // turn, the products with their spectra summed in the turn
+            TurnStash keep;
+            const int h = n / 2;
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                for (int q = threadIdx.x; q < m; q += blockDim.x) {
+                    const int col = half * h + q;
+                    x[fpad(q)] = (q < h) ? fft_prep_in(F, col, grid[row * ld + col]) : make_double2(0.0, 0.0);
+                }
+                __syncthreads();
+                fft_dif_front(x, F);
+                if (half == 0)
+                    fft_run_turn_split<kAnal1>(x, F, F.bhat2, keep);
+                else
+                    fft_run_turn_split<kAnal2>(x, F, F.bhat4, keep);
+                __syncthreads();
+            }
+            fft_dit_back(x, F);
+        } else {
+            for (int q = threadIdx.x; q < m; q += blockDim.x)
+                x[fpad(q)] = (q < n) ? fft_prep_in(F, q, grid[row * ld + q]) : make_double2(0.0, 0.0);
+            __syncthreads();
+            if (chunked)
+                fft_forward_chunked(x, F, F.bhat2, chunk, chunk_cap);
+            else
+                fft_forward(x, F, F.bhat2);
+        }
         for (int k = threadIdx.x; k <= Lc; k += blockDim.x) {
             const double r = order_rescale(u, k) * wr;
             const double2 zp = fft_get(x, F, k, k);
@@ -860,7 +1002,7 @@ __device__ __forceinline__ void anal_fft_c_tile(const FftDev &F, const double2 *
 }
 
 // One workgroup per slot (LDS) or per slice of a global workspace (WS).
-template <bool WS>
+template <bool WS, bool kSplit>
 __global__ __launch_bounds__(kFftMaxThreads) void k_synth_fft_c(FftDev F, const double4 *__restrict__ fb_p,
                                                           const double4 *__restrict__ fb_m, int P, int Lc,
                                                           const double *__restrict__ ringu,
@@ -873,11 +1015,11 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_synth_fft_c(FftDev F, const 
     if constexpr (WS) x = ws + blockIdx.x * ws_stride;
     else x = lds;
     for (int s = blockIdx.x; s < P; s += gridDim.x)
-        synth_fft_c_tile(F, fb_p, fb_m, P, Lc, s, ringu[s], rows[s], nrows, grid, ld, ext_col, x,
+        synth_fft_c_tile<kSplit>(F, fb_p, fb_m, P, Lc, s, ringu[s], rows[s], nrows, grid, ld, ext_col, x,
                          WS ? lds : nullptr, chunk_cap);
 }
 
-template <bool WS>
+template <bool WS, bool kSplit>
 __global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft_c(FftDev F, const double2 *__restrict__ grid,
                                                          long long ld, const double *__restrict__ ringu,
                                                          const int2 *__restrict__ rows, int nrows,
@@ -890,7 +1032,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft_c(FftDev F, const d
     if constexpr (WS) x = ws + blockIdx.x * ws_stride;
     else x = lds;
     for (int s = blockIdx.x; s < P; s += gridDim.x)
-        anal_fft_c_tile(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, Lc, s, gb_p, gb_m, x,
+        anal_fft_c_tile<kSplit>(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, Lc, s, gb_p, gb_m, x,
                         WS ? lds : nullptr, chunk_cap);
 }
 
@@ -1238,6 +1380,7 @@ __device__ __forceinline__ int xcd_item(int b, int n)
     return it < n ? it : -1;
 }
 
+template <bool kSplit>
 __global__ __launch_bounds__(kFftMaxThreads) void kb_synth_fft(const BField *__restrict__ fields,
                                                          const int2 *__restrict__ items, int nitems, int nslots,
                                                          BTables T, const double4 *__restrict__ fbuf,
@@ -1259,12 +1402,13 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_synth_fft(const BField *__r
         synth_fft_fill(F, in, fb, f.P, slot_orders(T.kmax, f.geo + s, f.L), s, T.u[f.geo + s], lds);
         __syncthreads();
         if (s + 1 < send) synth_fft_load(fb, f.P, slot_orders(T.kmax, f.geo + s + 1, f.L), s + 1, in);
-        synth_fft_run(F, T.rows[f.geo + s], f.nlat, grid[it.x], f.nlong, 0,
+        synth_fft_run<kSplit>(F, T.rows[f.geo + s], f.nlat, grid[it.x], f.nlong, 0,
                       T.roww ? T.roww[f.geo + s] : make_double2(1.0, 1.0), lds);
         __syncthreads();
     }
 }
 
+template <bool kSplit>
 __global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft(const BField *__restrict__ fields,
                                                         const int2 *__restrict__ items, int nitems, int nslots,
                                                         BTables T, const double *const *grid,
@@ -1278,6 +1422,14 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft(const BField *__re
     const FftDev &F = T.plans[f.fft];
     const double *g = grid[it.x];
     const int send = min(it.y + nslots, f.P);
+    if constexpr (kSplit) {
+        for (int s = it.y; s < send; ++s) {
+            anal_fft_tile_split(F, g, f.nlong, T.rows[f.geo + s], f.nlat, T.w[f.geo + s], T.u[f.geo + s], f.P,
+                                slot_orders(T.kmax, f.geo + s, f.L), s, gbuf + f.foff, 0, lds);
+            __syncthreads();
+        }
+        return;
+    }
     // the next slot's rows are loaded while the current slot is in its FFT
     // stages
     AnalRing in;

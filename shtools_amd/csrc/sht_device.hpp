@@ -60,6 +60,7 @@ __device__ __forceinline__ double scaled_upow(double u, int m)
 struct FftDev {
     int n, m, nst, blue;
     int lc;                // pruned Bluestein: orders |k| <= lc (-1: unpruned)
+    int half;              // split Bluestein plan (the ring's columns in two halves; see make_fft_plan)
     int radix[kMaxStages];
     int stride[kMaxStages];
     int twoff[kMaxStages];
@@ -69,6 +70,8 @@ struct FftDev {
     const double2 *chirp;  // [n]
     const double2 *bhat;   // [m] (pruned plans: synthesis spectrum)
     const double2 *bhat2;  // [m] pruned plans: analysis spectrum (else = bhat)
+    const double2 *bhat3;  // [m] split plans: synthesis spectrum of the second output half
+    const double2 *bhat4;  // [m] split plans: analysis spectrum of the second input half
 };
 
 // Work-array position of signed order k (|k| <= lc) of a pruned plan, or of
@@ -84,7 +87,7 @@ __device__ __forceinline__ int fft_pos(const FftDev &P, int k)
 // strides, which spread over the banks; padding (one slot per 16 elements)
 // broke those patterns and measured 6% slower.
 __device__ __forceinline__ int fpad(int e) { return e; }
-__host__ __device__ __forceinline__ int fft_lds_slots(int m) { return m + 1; }
+__host__ __device__ __forceinline__ int fft_lds_slots(int m) { return m; }
 
 // Fully unrolled forward DFTs, y_k = sum_j v_j e^{-2 pi i jk/R}, in place.
 __device__ __forceinline__ void dft2(double2 &a, double2 &b)
@@ -381,6 +384,97 @@ __device__ __forceinline__ void fft_forward_chunked(double2 *x, const FftDev &P,
         fft_run_stage<true>(x, P, 0, P.m);
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------- split Bluestein plans
+//
+// A split plan (FftDev::half) runs the pieces of fft_forward separately: the
+// DIF stages before the turn, the turn once per half of the ring's columns
+// (with the spectrum, or the first half's product, kept in registers in
+// between) and the DIT stages after it.
+
+// DIF stages of a Bluestein plan before the turn; ends with a barrier.
+__device__ __forceinline__ void fft_dif_front(double2 *x, const FftDev &P)
+{
+    for (int st = 0; st < P.nst - 1; ++st) {
+        fft_run_stage<false>(x, P, st, P.m);
+        __syncthreads();
+    }
+}
+
+// DIT stages of a Bluestein plan after the turn; ends with a barrier.
+__device__ __forceinline__ void fft_dit_back(double2 *x, const FftDev &P)
+{
+    for (int st = P.nst - 2; st >= 0; --st) {
+        fft_run_stage<true>(x, P, st, P.m);
+        __syncthreads();
+    }
+}
+
+// Values a thread keeps between the two turns of a split plan: its turn
+// butterflies' R values each, at most kTurnStash of them (the launch gives
+// every thread at most kTurnStash / R butterflies; see split_turn_threads).
+constexpr int kTurnStash = 10;
+struct TurnStash {
+    double2 v[kTurnStash];
+};
+
+// The turns of a split plan (R the last radix, one butterfly of R contiguous
+// elements per step; the spectrum S is the last DIF stage's output):
+//   kSynth1: S from x, kept; x <- DIT(conj(S bhat))     (first output half)
+//   kSynth2: S from the stash; x <- DIT(conj(S bhat))   (second output half)
+//   kAnal1:  S from x; keeps S bhat; x is not written   (first input half)
+//   kAnal2:  S from x; x <- DIT(conj(S bhat + kept))    (second input half)
+enum { kSynth1 = 0, kSynth2 = 1, kAnal1 = 2, kAnal2 = 3 };
+template <int R, int MODE>
+__device__ __forceinline__ void fft_turn_split(double2 *x, int m, const double2 *__restrict__ bhat, TurnStash &keep)
+{
+    // butterfly k of this thread keeps keep.v[k R .. k R + R) (compile-time
+    // indices: the values stay in registers)
+    constexpr int NB = kTurnStash / R;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const int b = threadIdx.x + k * blockDim.x;
+        if (b >= m / R) break;
+        const int base = b * R, i = k * R;
+        double2 v[R];
+        if constexpr (MODE == kSynth2) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] = keep.v[i + r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] = x[fpad(base + r)];
+            dft_unrolled<R>(v);
+        }
+        if constexpr (MODE == kSynth1) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) keep.v[i + r] = v[r];
+        }
+        if constexpr (MODE == kAnal1) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) keep.v[i + r] = cmul(v[r], bhat[base + r]);
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                double2 t = cmul(v[r], bhat[base + r]);
+                if constexpr (MODE == kAnal2) t = cadd(t, keep.v[i + r]);
+                v[r] = conjd(t);
+            }
+            dft_unrolled<R>(v);
+#pragma unroll
+            for (int r = 0; r < R; ++r) x[fpad(base + r)] = v[r];
+        }
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void fft_run_turn_split(double2 *x, const FftDev &P, const double2 *__restrict__ bhat,
+                                                   TurnStash &keep)
+{
+    if (P.radix[P.nst - 1] == 3)
+        fft_turn_split<3, MODE>(x, P.m, bhat, keep);
+    else
+        fft_turn_split<5, MODE>(x, P.m, bhat, keep);
 }
 
 // X_k after fft_forward; for Bluestein, pos is the work-array position

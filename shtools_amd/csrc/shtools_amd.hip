@@ -85,12 +85,14 @@ static bool kernels_configured = false;
 static void configure_kernels()
 {
     if (kernels_configured) return;
-    (void)hipFuncSetAttribute((const void *)k_synth_fft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
-    (void)hipFuncSetAttribute((const void *)k_anal_fft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
-    for (const void *k : {(const void *)k_synth_fft_ws, (const void *)k_anal_fft_ws, (const void *)kb_synth_fft,
-                          (const void *)kb_anal_fft, (const void *)kb_synth_fft_ws, (const void *)kb_anal_fft_ws,
-                          (const void *)k_synth_fft_c<false>, (const void *)k_synth_fft_c<true>,
-                          (const void *)k_anal_fft_c<false>, (const void *)k_anal_fft_c<true>})
+    for (const void *k : {(const void *)k_synth_fft<false>, (const void *)k_synth_fft<true>,
+                          (const void *)k_anal_fft<false>, (const void *)k_anal_fft<true>,
+                          (const void *)k_synth_fft_ws, (const void *)k_anal_fft_ws, (const void *)kb_synth_fft<false>,
+                          (const void *)kb_synth_fft<true>, (const void *)kb_anal_fft<false>,
+                          (const void *)kb_anal_fft<true>, (const void *)kb_synth_fft_ws, (const void *)kb_anal_fft_ws,
+                          (const void *)k_synth_fft_c<false, false>, (const void *)k_synth_fft_c<false, true>,
+                          (const void *)k_synth_fft_c<true, false>, (const void *)k_anal_fft_c<false, false>,
+                          (const void *)k_anal_fft_c<false, true>, (const void *)k_anal_fft_c<true, false>})
         (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
     kernels_configured = true;
 }
@@ -155,11 +157,41 @@ static long long ws_stride(int m) { return ((long long)fft_lds_slots(m) + 15) / 
 
 static bool upload_fft_arrays(const FftPlanHost &h, FftDev &d);
 
+// Workgroups of a ring FFT with work length m that share a CU's LDS (0: the
+// array does not fit, the ring runs on the global workspace).
+static int lds_per_cu(int m)
+{
+    if (!fft_in_lds(m)) return 0;
+    return std::min(4, (int)(kMaxLds / ((size_t)fft_lds_slots(m) * sizeof(double2))));
+}
+
+// Whether a ring of length n with band limit lc runs the split Bluestein
+// plan (make_fft_plan): when its shorter work array brings the ring from the
+// global workspace into the LDS, for at most 15% more transform work (3
+// transforms of the split length against 2 of the whole one).  Plain and
+// batched transforms decide alike, so they stay bitwise equal.
+//
+// Rings that fit the LDS either way keep the whole plan: the split array is
+// about 2/3 as long but has as many stages, and its second inverse pass adds
+// more LDS traffic than the extra workgroups per CU recover (measured on the
+// 512-field benchmark: synthesis ring FFTs 13.9 -> 17.1 ms with split plans
+// wherever they raised the workgroups per CU, analysis 13.2 -> 13.4 ms).
+static bool ring_split(int n, int lc)
+{
+    const int mh = split_length(n, lc);
+    if (mh <= 0 || !fft_in_lds(mh)) return false;
+    const int mf = bluestein_length(n + 2 * lc);
+    if (3.0 * mh > 1.15 * 2.0 * mf) return false;
+    return lds_per_cu(mf) == 0;
+}
+
+static FftPlanHost ring_plan(int n, int lc) { return make_fft_plan(n, lc, lc >= 0 && ring_split(n, lc)); }
+
 // Device arrays of an FFT plan that no work has been enqueued on.
 static void free_fft(FftDev &d)
 {
     for (const void *q : {(const void *)d.tw, (const void *)d.twst, (const void *)d.perm, (const void *)d.chirp,
-                          (const void *)d.bhat})
+                          (const void *)d.bhat, (const void *)d.bhat3, (const void *)d.bhat4})
         if (q) (void)hipFree(const_cast<void *>(q));
     if (d.bhat2 && d.bhat2 != d.bhat) (void)hipFree(const_cast<double2 *>(d.bhat2));
     std::memset(&d, 0, sizeof(d));
@@ -181,6 +213,7 @@ static bool upload_fft_arrays(const FftPlanHost &h, FftDev &d)
     d.nst = h.nst;
     d.blue = h.blue;
     d.lc = h.lc;
+    d.half = h.half;
     for (int i = 0; i < h.nst; ++i) {
         d.radix[i] = h.radix[i];
         d.stride[i] = h.stride[i];
@@ -196,6 +229,11 @@ static bool upload_fft_arrays(const FftPlanHost &h, FftDev &d)
         d.bhat = reinterpret_cast<const double2 *>(dev_upload(h.bhat));
         d.bhat2 = h.bhat2.empty() ? d.bhat : reinterpret_cast<const double2 *>(dev_upload(h.bhat2));
         if (!d.chirp || !d.bhat || !d.bhat2) return false;
+        if (h.half) {
+            d.bhat3 = reinterpret_cast<const double2 *>(dev_upload(h.bhat3));
+            d.bhat4 = reinterpret_cast<const double2 *>(dev_upload(h.bhat4));
+            if (!d.bhat3 || !d.bhat4) return false;
+        }
     }
     return true;
 }
@@ -224,7 +262,7 @@ static bool finish_geometry(Geometry &g, const std::vector<double> &z, const std
     g.kmax = dev_upload(km);
     g.smin = dev_upload(polar_smin(km, L));
     if (!g.z || !g.u || !g.w || !g.rows || !g.ones || !g.kmax || !g.smin) return false;
-    FftPlanHost fp = make_fft_plan(g.nlong, lc);
+    FftPlanHost fp = lc >= 0 ? ring_plan(g.nlong, lc) : make_fft_plan(g.nlong, lc);
     if (!upload_fft(fp, g.fft)) return false;
     g.ok = true;
     return true;
@@ -495,6 +533,25 @@ static int fft_threads(int m)
     return std::min(t, cap);
 }
 
+// Threads per workgroup of a plan's ring FFTs: fft_threads(m), and for a
+// split plan at least as many as give every thread at most kTurnStash / R
+// turn butterflies (R the last radix), and 512 for the largest arrays (two
+// such workgroups share a CU, each with up to 128 registers per thread).
+static int plan_min_threads(const FftDev &F)
+{
+    if (!F.half) return 1;
+    const int R = F.radix[F.nst - 1], nb = kTurnStash / R;
+    const int need = (F.m / R + nb - 1) / nb;
+    return (need + kWave - 1) / kWave * kWave;
+}
+
+static int plan_threads(const FftDev &F)
+{
+    int t = fft_threads(F.m);
+    if (F.half && F.m > 4096) t = std::min(t, kFftMaxThreads / 2);
+    return std::max(t, plan_min_threads(F));
+}
+
 static int fft_slots_per_wg()
 {
     static const int v = std::max(1, std::min(16, env_int("SHTG_FFT_SLOTS", 2)));
@@ -620,7 +677,8 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         if (lds_fft)
-            hipLaunchKernelGGL(k_synth_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, fbuf, g.P, Lc, ru,
+            hipLaunchKernelGGL(g.fft.half ? k_synth_fft<true> : k_synth_fft<false>, dim3(g.P),
+                               dim3(plan_threads(g.fft)), lds, st, g.fft, fbuf, g.P, Lc, ru,
                                g.rows, nrows_out, grid, ld, ext_col, roww, kmax);
         else
             hipLaunchKernelGGL(k_synth_fft_ws, dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft, fbuf, g.P, Lc, ru,
@@ -667,7 +725,8 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
         ProfScope ps(SHTG_K_ANAL_FFT, st);
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         if (lds_fft)
-            hipLaunchKernelGGL(k_anal_fft, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, grid, ld, g.u, g.rows,
+            hipLaunchKernelGGL(g.fft.half ? k_anal_fft<true> : k_anal_fft<false>, dim3(g.P),
+                               dim3(plan_threads(g.fft)), lds, st, g.fft, grid, ld, g.u, g.rows,
                                nrows, w, g.P, Lc, gbuf, ext_col, (const int *)g.kmax);
         else
             hipLaunchKernelGGL(k_anal_fft_ws, dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft, grid, ld, g.u, g.rows,
@@ -726,10 +785,11 @@ static int launch_synthesis_c(const Geometry &g, const DevRecurrence &rt, const 
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         if (lds_fft)
-            hipLaunchKernelGGL(k_synth_fft_c<false>, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, fb_p, fb_m,
+            hipLaunchKernelGGL((g.fft.half ? k_synth_fft_c<false, true> : k_synth_fft_c<false, false>), dim3(g.P),
+                               dim3(plan_threads(g.fft)), lds, st, g.fft, fb_p, fb_m,
                                g.P, Lc, g.u, g.rows, nrows_out, grid, ld, ext_col, (double2 *)nullptr, 0LL, 0);
         else
-            hipLaunchKernelGGL(k_synth_fft_c<true>, dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft, fb_p, fb_m,
+            hipLaunchKernelGGL((k_synth_fft_c<true, false>), dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft, fb_p, fb_m,
                                g.P, Lc, g.u, g.rows, nrows_out, grid, ld, ext_col, fws, ws_stride(g.fft.m),
                                kChunkCap);
     }
@@ -759,10 +819,11 @@ static int launch_analysis_c(const Geometry &g, const DevRecurrence &rt, const D
         ProfScope ps(SHTG_K_ANAL_FFT, st);
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         if (lds_fft)
-            hipLaunchKernelGGL(k_anal_fft_c<false>, dim3(g.P), dim3(fft_threads(g.fft.m)), lds, st, g.fft, grid, ld,
+            hipLaunchKernelGGL((g.fft.half ? k_anal_fft_c<false, true> : k_anal_fft_c<false, false>), dim3(g.P),
+                               dim3(plan_threads(g.fft)), lds, st, g.fft, grid, ld,
                                g.u, g.rows, g.nlat, g.w, g.P, Lc, gb_p, gb_m, (double2 *)nullptr, 0LL, 0);
         else
-            hipLaunchKernelGGL(k_anal_fft_c<true>, dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft, grid, ld,
+            hipLaunchKernelGGL((k_anal_fft_c<true, false>), dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft, grid, ld,
                                g.u, g.rows, g.nlat, g.w, g.P, Lc, gb_p, gb_m, fws, ws_stride(g.fft.m), kChunkCap);
     }
     {
@@ -797,6 +858,8 @@ struct FftGroup {
     size_t lds = 0;
     int threads = 256;       // synthesis workgroup size (heuristic or tuned)
     int threads_anal = 256;  // analysis workgroup size
+    int min_threads = 1;     // the least its split plans allow (plan_min_threads)
+    bool half = false;       // its fields' rings run split plans (the kernels' split instantiation)
 };
 
 struct BatchPlan {
@@ -1071,7 +1134,7 @@ static int batch_create_dh(int nfields, const int *lmax, const int *sampling, in
         smin_of[key] = polar_smin(km, L);
         sm_of[key] = (int)smin.size();
         smin.insert(smin.end(), smin_of[key].begin(), smin_of[key].end());
-        FftPlanHost fp = make_fft_plan(nlong, L);
+        FftPlanHost fp = ring_plan(nlong, L);
         FftDev fd;
         if (!upload_fft(fp, fd)) return ST_OTHER;
         b->hplans.push_back(fd);
@@ -1138,8 +1201,10 @@ static int batch_create_dh(int nfields, const int *lmax, const int *sampling, in
     b->fft_slots = fft_slots_per_wg();
     // FFT items grouped by how many workgroups fit the LDS of a CU (the
     // group's LDS is its largest need), then by threads per workgroup
-    std::map<std::pair<int, int>, std::vector<W>> groups;  // (-(WGs per CU), -threads)
-    std::map<std::pair<int, int>, size_t> group_lds;
+    typedef std::tuple<int, int, int> GKey;  // (-(WGs per CU), -threads, split)
+    std::map<GKey, std::vector<W>> groups;
+    std::map<GKey, size_t> group_lds;
+    std::map<GKey, int> group_min;
     std::vector<W> wsi;
     for (int i = 0; i < nfields; ++i) {
         const BField &f = fields[i];
@@ -1150,9 +1215,10 @@ static int batch_create_dh(int nfields, const int *lmax, const int *sampling, in
             continue;
         }
         const int per_cu = std::max(1, (int)(kMaxLds / need));
-        const int thr = fft_threads(b->hplans[f.fft].m);
-        const auto key = std::make_pair(-std::min(per_cu, 4), -thr);
+        const int thr = plan_threads(b->hplans[f.fft]);
+        const GKey key(-std::min(per_cu, 4), -thr, b->hplans[f.fft].half);
         group_lds[key] = std::max(group_lds[key], need);
+        group_min[key] = std::max(group_min[key], plan_min_threads(b->hplans[f.fft]));
         for (int s = 0; s < f.P; s += b->fft_slots) groups[key].push_back({(double)need, i, s, 0});
     }
     b->fft_group_of.assign(nfields, (int)groups.size());
@@ -1165,7 +1231,9 @@ static int batch_create_dh(int nfields, const int *lmax, const int *sampling, in
         g.items = dev_upload(it);
         g.nitems = (int)it.size();
         g.lds = group_lds[kv.first];
-        g.threads = g.threads_anal = -kv.first.second;
+        g.threads = g.threads_anal = -std::get<1>(kv.first);
+        g.min_threads = group_min[kv.first];
+        g.half = std::get<2>(kv.first) != 0;
         if (!g.items) return ST_ALLOC;
         b->fft.push_back(g);
     }
@@ -1283,7 +1351,8 @@ static int batch_synthesis(BatchPlan *b, hipStream_t st, const BTables &T)
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
         for (auto &g : b->fft)
-            hipLaunchKernelGGL(kb_synth_fft, dim3(xcd_blocks(g.nitems)), dim3(g.threads), g.lds, st, b->fields,
+            hipLaunchKernelGGL(g.half ? kb_synth_fft<true> : kb_synth_fft<false>, dim3(xcd_blocks(g.nitems)),
+                               dim3(g.threads), g.lds, st, b->fields,
                                g.items, g.nitems, b->fft_slots, T, b->fbuf, gout);
         if (b->fft_ws.nitems)
             hipLaunchKernelGGL(kb_synth_fft_ws, dim3(std::min(b->fft_ws.nitems, kWsGroups)), dim3(kFftMaxThreads),
@@ -1303,7 +1372,8 @@ static int batch_analysis(BatchPlan *b, hipStream_t st, const BTables &T)
     {
         ProfScope ps(SHTG_K_ANAL_FFT, st);
         for (auto &g : b->fft)
-            hipLaunchKernelGGL(kb_anal_fft, dim3(xcd_blocks(g.nitems)), dim3(g.threads_anal), g.lds, st, b->fields,
+            hipLaunchKernelGGL(g.half ? kb_anal_fft<true> : kb_anal_fft<false>, dim3(xcd_blocks(g.nitems)),
+                               dim3(g.threads_anal), g.lds, st, b->fields,
                                g.items, g.nitems, b->fft_slots, T, gin, b->fbuf);
         if (b->fft_ws.nitems)
             hipLaunchKernelGGL(kb_anal_fft_ws, dim3(std::min(b->fft_ws.nitems, kWsGroups)), dim3(kFftMaxThreads),
@@ -1420,14 +1490,17 @@ static int batch_tune(BatchPlan *b, hipStream_t st, int reps, int flags)
             float best = 1e30f;
             int best_t = dir ? g.threads_anal : g.threads;
             for (int t : {kFftMaxThreads / 4, kFftMaxThreads / 2, kFftMaxThreads}) {
+                if (t < g.min_threads) continue;
                 float tmin = 1e30f;
                 for (int r = 0; r < reps; ++r) {
                     (void)hipEventRecord(e0, st);
                     if (dir == 0)
-                        hipLaunchKernelGGL(kb_synth_fft, dim3(xcd_blocks(g.nitems)), dim3(t), g.lds, st, b->fields,
+                        hipLaunchKernelGGL(g.half ? kb_synth_fft<true> : kb_synth_fft<false>,
+                                           dim3(xcd_blocks(g.nitems)), dim3(t), g.lds, st, b->fields,
                                            g.items, g.nitems, b->fft_slots, b->T, b->fbuf, gout);
                     else
-                        hipLaunchKernelGGL(kb_anal_fft, dim3(xcd_blocks(g.nitems)), dim3(t), g.lds, st, b->fields,
+                        hipLaunchKernelGGL(g.half ? kb_anal_fft<true> : kb_anal_fft<false>,
+                                           dim3(xcd_blocks(g.nitems)), dim3(t), g.lds, st, b->fields,
                                            g.items, g.nitems, b->fft_slots, b->T, gin, b->fbuf);
                     (void)hipEventRecord(e1, st);
                     float ms = 0.0f;
@@ -2363,7 +2436,9 @@ int shtg_batch_set_shapes(shtg_batch_plan *plan, const int *threads_synth, const
     if (n != (int)b->fft.size()) return ST_DIMS;
     for (int i = 0; i < n; ++i)
         for (int t : {threads_synth[i], threads_anal[i]})
-            if (t != kFftMaxThreads / 4 && t != kFftMaxThreads / 2 && t != kFftMaxThreads) return ST_BOUNDS;
+            if ((t != kFftMaxThreads / 4 && t != kFftMaxThreads / 2 && t != kFftMaxThreads) ||
+                t < b->fft[i].min_threads)
+                return ST_BOUNDS;
     for (int i = 0; i < n; ++i) {
         b->fft[i].threads = threads_synth[i];
         b->fft[i].threads_anal = threads_anal[i];

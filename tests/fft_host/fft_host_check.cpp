@@ -40,6 +40,9 @@ static FftDev to_dev(const FftPlanHost &h)
     d.chirp = h.blue ? reinterpret_cast<const double2 *>(h.chirp.data()) : nullptr;
     d.bhat = h.blue ? reinterpret_cast<const double2 *>(h.bhat.data()) : nullptr;
     d.bhat2 = h.blue ? reinterpret_cast<const double2 *>((h.bhat2.empty() ? h.bhat : h.bhat2).data()) : nullptr;
+    d.half = h.half;
+    d.bhat3 = h.half ? reinterpret_cast<const double2 *>(h.bhat3.data()) : nullptr;
+    d.bhat4 = h.half ? reinterpret_cast<const double2 *>(h.bhat4.data()) : nullptr;
     return d;
 }
 
@@ -56,6 +59,97 @@ static std::complex<long double> direct(const std::vector<std::pair<int, double2
     return acc;
 }
 
+// The turn of a split plan run by T virtual threads (each its own stash),
+// T the fewest that give every thread at most kTurnStash / R butterflies,
+// as the launches do.
+template <int MODE>
+static void turn_threads(double2 *x, const FftDev &F, const double2 *bhat, std::vector<TurnStash> &keep)
+{
+    const int R = F.radix[F.nst - 1], nb = kTurnStash / R;
+    const int T = (F.m / R + nb - 1) / nb;
+    keep.resize(T);
+    blockDim.x = T;
+    for (int t = 0; t < T; ++t) {
+        threadIdx.x = t;
+        fft_run_turn_split<MODE>(x, F, bhat, keep[t]);
+    }
+    blockDim.x = 1;
+    threadIdx.x = 0;
+}
+
+// Split plan (make_fft_plan(n, lc, true)): synthesis (orders |k| <= lc in,
+// every column out, by halves) and analysis (every column in, by halves;
+// orders |k| <= lc out) as the kernels run them, against direct sums.
+static double run_split(const FftPlanHost &h, std::mt19937_64 &rng, int nsample)
+{
+    const FftDev F = to_dev(h);
+    const int n = h.n, hh = n / 2;
+    std::normal_distribution<double> g;
+    std::vector<std::complex<long double>> root(n);
+    for (int k = 0; k < n; ++k) root[k] = root_ld(k, n);
+    double maxerr = 0.0, maxref = 0.0;
+    auto check = [&](const std::vector<std::pair<int, double2>> &in, int k, double2 y) {
+        const auto r = direct(in, n, k, root);
+        maxerr = std::max(maxerr, (double)std::abs(std::complex<long double>(y.x, y.y) - r));
+        maxref = std::max(maxref, (double)std::abs(r));
+    };
+    std::uniform_int_distribution<int> u(0, n - 1);
+    // synthesis
+    {
+        std::vector<double2> x((size_t)h.m, make_double2(0.0, 0.0));
+        std::vector<std::pair<int, double2>> in;
+        for (int k = -h.lc; k <= h.lc; ++k) {
+            const int j = (k + n) % n;
+            const double2 v = make_double2(g(rng), g(rng));
+            in.push_back({j, v});
+            x[fft_pos(F, j)] = fft_prep_in(F, j, v);
+        }
+        std::vector<TurnStash> keep;
+        fft_dif_front(x.data(), F);
+        for (int half = 0; half < 2; ++half) {
+            if (half == 0)
+                turn_threads<kSynth1>(x.data(), F, F.bhat, keep);
+            else
+                turn_threads<kSynth2>(x.data(), F, F.bhat3, keep);
+            fft_dit_back(x.data(), F);
+            std::vector<int> ps = {0, hh - 1};
+            for (int t = 0; t < nsample && (int)ps.size() < hh; ++t) ps.push_back(u(rng) % hh);
+            if (hh <= nsample) {
+                ps.clear();
+                for (int p = 0; p < hh; ++p) ps.push_back(p);
+            }
+            for (int p : ps) check(in, half * hh + p, fft_get(x.data(), F, half * hh + p, p));
+        }
+    }
+    // analysis
+    {
+        std::vector<double2> x((size_t)h.m), val(n);
+        std::vector<std::pair<int, double2>> in;
+        for (int j = 0; j < n; ++j) {
+            val[j] = make_double2(g(rng), g(rng));
+            in.push_back({j, val[j]});
+        }
+        std::vector<TurnStash> keep;
+        for (int half = 0; half < 2; ++half) {
+            for (int q = 0; q < h.m; ++q)
+                x[q] = q < hh ? fft_prep_in(F, half * hh + q, val[half * hh + q]) : make_double2(0.0, 0.0);
+            fft_dif_front(x.data(), F);
+            if (half == 0)
+                turn_threads<kAnal1>(x.data(), F, F.bhat2, keep);
+            else
+                turn_threads<kAnal2>(x.data(), F, F.bhat4, keep);
+        }
+        fft_dit_back(x.data(), F);
+        for (int k = -h.lc; k <= h.lc; ++k) {
+            if (2 * h.lc + 1 > nsample && k != -h.lc && k != h.lc && k != 0 && u(rng) % (2 * h.lc + 1) >= nsample)
+                continue;
+            const int kk = (k + n) % n;
+            check(in, kk, fft_get(x.data(), F, kk, fft_pos(F, kk)));
+        }
+    }
+    return maxerr / maxref;
+}
+
 // mode 0: full input and output; 1: synthesis of a pruned plan (orders
 // |k| <= lc in, all n out); 2: analysis of a pruned plan (all n in, orders
 // |k| <= lc out).
@@ -64,7 +158,7 @@ static double run_case(const FftPlanHost &h, int mode, std::mt19937_64 &rng, int
     const FftDev F = to_dev(h);
     const int n = h.n;
     std::normal_distribution<double> g;
-    std::vector<double2> x((size_t)h.m + 1, make_double2(0.0, 0.0));
+    std::vector<double2> x((size_t)h.m, make_double2(0.0, 0.0));
     std::vector<std::pair<int, double2>> in;
     if (mode == 1) {
         for (int k = -h.lc; k <= h.lc; ++k) {
@@ -86,7 +180,7 @@ static double run_case(const FftPlanHost &h, int mode, std::mt19937_64 &rng, int
     if (h.nst >= 2) {
         const int gsz = h.m / h.radix[0];
         for (int cap : {gsz, 3 * gsz}) {
-            std::vector<double2> y = x, lds((size_t)cap + 1);
+            std::vector<double2> y = x, lds((size_t)cap);
             fft_forward_chunked(y.data(), F, mode == 2 ? F.bhat2 : F.bhat, lds.data(), cap);
             xc.push_back(y);
         }
@@ -139,13 +233,23 @@ int main(int argc, char **argv)
         const int lc = (n % 2 == 0) ? n / 4 - 1 : -1;  // the DH ring: lmax = n/4 - 1
         FftPlanHost full = make_fft_plan(n);
         double err = run_case(full, 0, rng, nsample);
+        int nsplit = 0;
         if (n % 2 == 0) {
             FftPlanHost pr = make_fft_plan(n, lc);
             if (pr.lc >= 0) {
                 err = std::max(err, run_case(pr, 1, rng, nsample));
                 err = std::max(err, run_case(pr, 2, rng, nsample));
             }
+            // split plans of the DH2 (lc = n/4 - 1) and DH1 (lc = n/2 - 1) rings
+            for (int l : {lc, n / 2 - 1}) {
+                FftPlanHost sp = make_fft_plan(n, l, true);
+                if (sp.half) {
+                    err = std::max(err, run_split(sp, rng, nsample));
+                    ++nsplit;
+                }
+            }
         }
+        if (std::getenv("SPLIT_COUNT")) std::printf("split %d ", nsplit);
         std::printf("%d %d %d %d", n, full.m, full.blue, full.nst);
         for (int s = 0; s < full.nst; ++s) std::printf(" %d", full.radix[s]);
         std::printf(" %.3e\n", err);

@@ -44,7 +44,6 @@ constexpr int kWave = 64;      // Legendre workgroups are single waves
 constexpr int kSynthKMax = 4;  // ring pairs per lane in synthesis (1 .. 4)
 constexpr int kAnalKMax = 8;   // ring pairs per lane and pass in analysis (1 .. 8)
 constexpr int kAnalLC = 8;     // degrees per reduction chunk in analysis
-constexpr int kRedStride = 65; // LDS row stride (doubles) of the analysis reduction
 
 // ====================================================================== tiles
 
@@ -195,16 +194,84 @@ __device__ __forceinline__ void synth_leg_dispatch(int kk, const double *__restr
     else synth_leg_tile<4, kRho>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow);
 }
 
+// ------------------------------------------------- cross-lane reductions
+//
+// 64-bit lane moves and swaps from 32-bit ones (gfx950 v_permlane32_swap,
+// v_permlane16_swap and DPP row moves applied to both halves).
+__device__ __forceinline__ unsigned lo32(double v) { return (unsigned)__double_as_longlong(v); }
+__device__ __forceinline__ unsigned hi32(double v) { return (unsigned)(__double_as_longlong(v) >> 32); }
+__device__ __forceinline__ double from32(unsigned lo, unsigned hi)
+{
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// a = [a lanes 0-31, b lanes 0-31], b = [a lanes 32-63, b lanes 32-63]
+__device__ __forceinline__ void swap32_f64(double &a, double &b)
+{
+    const auto l = __builtin_amdgcn_permlane32_swap(lo32(a), lo32(b), false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(hi32(a), hi32(b), false, false);
+    a = from32(l[0], h[0]);
+    b = from32(l[1], h[1]);
+}
+// per 32 lanes: a = [a lanes 0-15, b lanes 0-15], b = [a lanes 16-31, b lanes 16-31]
+__device__ __forceinline__ void swap16_f64(double &a, double &b)
+{
+    const auto l = __builtin_amdgcn_permlane16_swap(lo32(a), lo32(b), false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap(hi32(a), hi32(b), false, false);
+    a = from32(l[0], h[0]);
+    b = from32(l[1], h[1]);
+}
+// DPP row moves: 0x140 row_mirror (lane i of a row of 16 reads 15 - i),
+// 0x141 row_half_mirror (7 - i within 8), 0x4E / 0xB1 quad_perm xor 2 / 1
+template <int kCtrl>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const int l = __builtin_amdgcn_update_dpp(0, (int)lo32(v), kCtrl, 0xF, 0xF, false);
+    const int h = __builtin_amdgcn_update_dpp(0, (int)hi32(v), kCtrl, 0xF, 0xF, false);
+    return from32((unsigned)l, (unsigned)h);
+}
+
+// Sums over the 64 lanes of a wave of 16 values per lane v[0..15], by a
+// transpose-reduce in registers: each step pairs a lane with one that differs
+// in one lane bit (5, 4: permlane swaps; 3, 2: row mirrors, which flip that
+// bit; then quad moves for bits 1, 0), keeps half of its values and adds the
+// partner's copies of them.  Returns the sum of value vi = (lane bit 2) +
+// 2 (bit 3) + 4 (bit 4) + 8 (bit 5), which the 4 lanes of each quad hold.
+__device__ __forceinline__ double wave_transpose_sum16(double (&v)[16], int lane)
+{
+    double a[8], b[4], c[2];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        swap32_f64(v[i], v[i + 8]);
+        a[i] = v[i] + v[i + 8];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        swap16_f64(a[i], a[i + 4]);
+        b[i] = a[i] + a[i + 4];
+    }
+    const bool b3 = (lane & 8) != 0, b2 = (lane & 4) != 0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const double send = b3 ? b[i] : b[i + 2], keep = b3 ? b[i + 2] : b[i];
+        c[i] = keep + dpp_f64<0x140>(send);
+    }
+    const double send = b2 ? c[0] : c[1], keep = b2 ? c[1] : c[0];
+    double d = keep + dpp_f64<0x141>(send);
+    d += dpp_f64<0x4E>(d);
+    d += dpp_f64<0xB1>(d);
+    return d;
+}
+
 // Analysis sums of order m over ring pairs [s0, P) (one wave):
 // out[j] = sum over slots of q_{m+j}(z) * (E or O by parity of j), as (C, S)
 // pairs.  grow holds the order's (E_C, E_S, O_C, O_S) already multiplied by
 // the order's rescale factor.  Passes of K * 64 slots; the first pass
 // stores, later passes accumulate.  Each chunk of kAnalLC degrees is reduced
-// over the wave through red (16 x kRedStride doubles of LDS).
+// over the wave in registers (wave_transpose_sum16).
 template <int K>
 __device__ __forceinline__ void anal_leg_tile(const double *__restrict__ al, double sd,
                                               const double *__restrict__ ringz, const double4 *__restrict__ grow,
-                                              int jmax, int s0, int P, double2 *__restrict__ out, double *red)
+                                              int jmax, int s0, int P, double2 *__restrict__ out)
 {
     constexpr int LC = kAnalLC;
     const int lane = threadIdx.x & (kWave - 1);
@@ -233,23 +300,18 @@ __device__ __forceinline__ void anal_leg_tile(const double *__restrict__ al, dou
         // Each chunk of LC degrees: K recursion steps per lane and degree, the
         // first ring's product initialising the accumulators.  Full chunks
         // run without guards so the recursion state rotates in registers.
+        static_assert(2 * LC == 16, "the register reduction takes 16 values per lane");
         auto reduce_chunk = [&](double (&acc)[LC][2], int j0) {
-            // transpose through LDS: lane (v, part) sums 16 lanes of value v
+            double v[16];
 #pragma unroll
-            for (int v = 0; v < 2 * LC; ++v) red[v * kRedStride + lane] = acc[v >> 1][v & 1];
-            __syncthreads();
-            const int v = lane & 15, part = lane >> 4;
-            double sum = 0.0;
-#pragma unroll
-            for (int i = 0; i < 16; ++i) sum += red[v * kRedStride + part * 16 + i];
-            sum += __shfl_xor(sum, 16);
-            sum += __shfl_xor(sum, 32);
-            const int jj = v >> 1;
-            if (part == 0 && j0 + jj <= jmax) {
-                double *o = reinterpret_cast<double *>(out + j0 + jj) + (v & 1);
+            for (int i = 0; i < 16; ++i) v[i] = acc[i >> 1][i & 1];
+            const double sum = wave_transpose_sum16(v, lane);
+            const int vi = ((lane >> 2) & 1) | ((lane >> 2) & 2) | ((lane >> 2) & 4) | ((lane >> 2) & 8);
+            const int jj = vi >> 1;
+            if ((lane & 3) == 0 && j0 + jj <= jmax) {
+                double *o = reinterpret_cast<double *>(out + j0 + jj) + (vi & 1);
                 *o = (p0 == s0) ? sum : *o + sum;
             }
-            __syncthreads();
         };
         int j0 = 0;
         for (; j0 + LC - 1 <= jmax; j0 += LC) {
@@ -745,7 +807,6 @@ __global__ __launch_bounds__(kWave) void k_anal_leg(const double *__restrict__ a
                                                     double2 *__restrict__ cbuf, const int *__restrict__ smin,
                                                     int nblk, int bslots, long long cstride)
 {
-    __shared__ double red[2 * kAnalLC * kRedStride];
     const int m = blockIdx.x / nblk, b = blockIdx.x - m * nblk;
     const int sa = (smin ? max(s0, smin[m]) : s0) + b * bslots;
     const int se = nblk > 1 ? min(P, sa + bslots) : P;
@@ -754,7 +815,7 @@ __global__ __launch_bounds__(kWave) void k_anal_leg(const double *__restrict__ a
         for (int j = threadIdx.x; j <= Lc - m; j += kWave) out[j] = make_double2(0.0, 0.0);
         return;
     }
-    anal_leg_tile<K>(alpha + tri_offd(m, Lt), seed[m], ringz, gbuf + (size_t)m * P, Lc - m, sa, se, out, red);
+    anal_leg_tile<K>(alpha + tri_offd(m, Lt), seed[m], ringz, gbuf + (size_t)m * P, Lc - m, sa, se, out);
 }
 
 // grid: ntm x ntl tiles over the whole d1 x d2 array
@@ -1497,13 +1558,11 @@ __global__ __launch_bounds__(kWave) void kb_anal_leg(const BField *__restrict__ 
                                                      const int2 *__restrict__ items, BTables T,
                                                      const double4 *__restrict__ gbuf, double2 *__restrict__ cbuf)
 {
-    __shared__ double red[2 * kAnalLC * kRedStride];
     const int2 it = items[blockIdx.x];
     const BField f = fields[it.x];
     const int m = it.y;
     anal_leg_tile<K>(T.alpha + tri_offd(m, T.Lt), T.seed[m], T.z + f.geo, gbuf + f.foff + (size_t)m * f.P,
-                     f.L - m, T.smin ? max(T.s0, T.smin[f.sm + m]) : T.s0, f.P, cbuf + f.coff + tri_offd(m, f.L),
-                     red);
+                     f.L - m, T.smin ? max(T.s0, T.smin[f.sm + m]) : T.s0, f.P, cbuf + f.coff + tri_offd(m, f.L));
 }
 
 // items: (field, first order, first degree) of a tile of the field's

@@ -254,6 +254,14 @@ int shtg_batch_field_groups(shtg_batch_plan *plan, int *group, int n);
  * src/PreGLQ.f95:93-149):
  * zero[lmax+1] descending from ~+1, w[lmax+1]. */
 int shtg_shglq(int lmax, double *zero, double *w);
+/* NGLQ, NGLQSH, NGLQSHN (include/shtools.h:1031-1033; src/PreGLQ.f95:153-233):
+ * Gauss-Legendre points needed to integrate exactly a polynomial of degree
+ * `degree` (ceiling((degree + 1) / 2)), a field of degree `degree`
+ * (degree + 1), and that field raised to the power n (ceiling(((n + 1)
+ * degree + 1) / 2)).  -1 for a negative degree (the reference stops). */
+int shtg_nglq(int degree);
+int shtg_nglqsh(int degree);
+int shtg_nglqshn(int degree, int n);
 /* DHaj (include/shtools.h:1034; src/DHaj.f95:91-105): aj[n+extend]
  * Driscoll-Healy weights. */
 int shtg_dhaj(int n, int extend, double *aj);

@@ -74,6 +74,9 @@ def ref_lib():
         L.ref_plm.argtypes = [i, i, d, i, i, dp, ip]
         L.ref_normalgravity.argtypes = [d, d, d, d, d]
         L.ref_normalgravity.restype = d
+        for name, nargs in (("ref_nglq", 1), ("ref_nglqsh", 1), ("ref_nglqshn", 2)):
+            getattr(L, name).argtypes = [i] * nargs
+            getattr(L, name).restype = i
         _ref = L
     return _ref
 

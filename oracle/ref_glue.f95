@@ -87,3 +87,31 @@ function ref_normalgravity(lat, gm, omega, a, b) bind(C, name="ref_normalgravity
     real(c_double) :: ref_normalgravity
     ref_normalgravity = NormalGravity(lat, gm, omega, a, b)
 end function ref_normalgravity
+
+function ref_nglq(degree) bind(C, name="ref_nglq")
+    ! src/PreGLQ.f95: NGLQ, NGLQSH, NGLQSHN (degree >= 0: the reference stops otherwise)
+    use iso_c_binding
+    use SHTOOLS, only: NGLQ
+    implicit none
+    integer(c_int), value :: degree
+    integer(c_int) :: ref_nglq
+    ref_nglq = NGLQ(degree)
+end function ref_nglq
+
+function ref_nglqsh(degree) bind(C, name="ref_nglqsh")
+    use iso_c_binding
+    use SHTOOLS, only: NGLQSH
+    implicit none
+    integer(c_int), value :: degree
+    integer(c_int) :: ref_nglqsh
+    ref_nglqsh = NGLQSH(degree)
+end function ref_nglqsh
+
+function ref_nglqshn(degree, n) bind(C, name="ref_nglqshn")
+    use iso_c_binding
+    use SHTOOLS, only: NGLQSHN
+    implicit none
+    integer(c_int), value :: degree, n
+    integer(c_int) :: ref_nglqshn
+    ref_nglqshn = NGLQSHN(degree, n)
+end function ref_nglqshn

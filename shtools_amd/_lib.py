@@ -48,6 +48,9 @@ SIGNATURES = {
                                   _c_int, _c_int, _vp],
     "shtg_shexpand_glq_adjoint": [_vp, _c_int, _c_int, _dp, _c_int, _dp, _c_int, _c_int, _c_int, _c_int, _vp,
                                   _c_int, _c_int, _vp],
+    "shtg_nglq": [_c_int],
+    "shtg_nglqsh": [_c_int],
+    "shtg_nglqshn": [_c_int, _c_int],
     "shtg_shglq": [_c_int, _dp, _dp],
     "shtg_dhaj": [_c_int, _c_int, _dp],
     "shtg_glqgridcoord": [_c_int, _c_int, _dp, _dp],

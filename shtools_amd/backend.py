@@ -35,6 +35,7 @@ from . import _lib
 
 __all__ = [
     "SHToolsError", "MakeGridDH", "SHExpandDH", "MakeGridGLQ", "SHExpandGLQ", "SHGLQ", "GLQGridCoord", "DHaj",
+    "NGLQ", "NGLQSH", "NGLQSHN",
     "PlmBar", "PlmSchmidt", "PlmON", "PLegendreA",
     "SHExpandDHC", "MakeGridDHC", "SHExpandGLQC", "MakeGridGLQC", "MakeGradientDH", "SHRotateRealCoef",
     "MakeGridDH_adjoint", "SHExpandDH_adjoint", "MakeGridGLQ_adjoint", "SHExpandGLQ_adjoint",
@@ -444,6 +445,31 @@ def SHMultiply(sh1, sh2, lmax1=None, lmax2=None, norm=1, csphase=1):
 
 
 # ------------------------------------------------------------ host helpers
+
+def NGLQ(degree):
+    """Gauss-Legendre points integrating a polynomial of this degree exactly
+    (src/PreGLQ.f95:153-178)."""
+    return _nglq_call("shtg_nglq", degree)
+
+
+def NGLQSH(degree):
+    """Gauss-Legendre points integrating a field of this degree exactly
+    (src/PreGLQ.f95:181-205)."""
+    return _nglq_call("shtg_nglqsh", degree)
+
+
+def NGLQSHN(degree, n):
+    """Gauss-Legendre points integrating a field of this degree raised to the
+    power n exactly (src/PreGLQ.f95:208-233)."""
+    return _nglq_call("shtg_nglqshn", degree, n)
+
+
+def _nglq_call(name, *args):
+    r = getattr(_lib.lib(), name)(*[int(a) for a in args])
+    if r < 0:
+        raise SHToolsError(_status_message(2))
+    return r
+
 
 def SHGLQ(lmax):
     """Gauss-Legendre nodes (descending) and weights for degree lmax (src/SHGLQ.f95)."""

@@ -2209,6 +2209,19 @@ int shtg_dh_polar_orders(int lmax, int *kmax)
     return ST_OK;
 }
 
+// src/PreGLQ.f95:153-233: Gauss-Legendre points that integrate exactly a
+// polynomial of the given degree, a field of degree `degree`, and a field of
+// that degree raised to the power n.  The reference stops the program for a
+// negative degree; these return -1 instead.
+int shtg_nglq(int degree) { return degree < 0 ? -1 : (degree + 2) / 2; }
+int shtg_nglqsh(int degree) { return degree < 0 ? -1 : degree + 1; }
+int shtg_nglqshn(int degree, int n)
+{
+    if (degree < 0) return -1;
+    const long long d = (long long)(n + 1) * degree + 1;  // ceiling(d / 2), d of either sign
+    return (int)(d >= 0 ? (d + 1) / 2 : -((-d) / 2));
+}
+
 int shtg_shglq(int lmax, double *zero, double *w)
 {
     if (lmax < 0) return ST_BOUNDS;

@@ -137,3 +137,28 @@ def test_recursion_tables_near_poles_as_accurate_as_reference(z):
     rms = lambda e: float(np.sqrt(np.mean(np.square(e))))
     assert rms(ours) <= 1.2 * rms(ref), (rms(ours), rms(ref))
     assert max(np.abs(ours)) <= 1e-11
+
+
+def test_nglq_counts_equal_to_reference():
+    """NGLQ, NGLQSH, NGLQSHN of the library against the reference's own
+    (src/PreGLQ.f95:153-233), and the count's meaning: that many
+    Gauss-Legendre nodes integrate z^degree exactly, one fewer does not."""
+    lib = oracle.ref_lib()
+    for d in list(range(0, 200)) + [511, 1023, 2047, 2799, 4000]:
+        assert B.NGLQ(d) == lib.ref_nglq(d)
+        assert B.NGLQSH(d) == lib.ref_nglqsh(d)
+        for n in range(0, 6):
+            assert B.NGLQSHN(d, n) == lib.ref_nglqshn(d, n), (d, n)
+    for d in range(0, 40):
+        for npt, exact in ((B.NGLQ(d), True), (B.NGLQ(d) - 1, False)):
+            if npt < 1:
+                continue
+            x, w = np.polynomial.legendre.leggauss(npt)
+            val = float(np.sum(w * x ** d))
+            ref = 0.0 if d % 2 else 2.0 / (d + 1)
+            if exact:
+                assert abs(val - ref) <= 1e-13, (d, npt)
+            elif d % 2 == 0:
+                assert abs(val - ref) > 1e-12, (d, npt)
+    with pytest.raises(B.SHToolsError):
+        B.NGLQ(-1)

@@ -32,6 +32,14 @@ constexpr int kMaxStages = 24;
 #define SHTG_FFT_RADICES(X) X(2) X(3) X(4) X(5) X(8)
 constexpr int kRegRadices[] = {2, 3, 4, 5, 8};
 
+// LDS (or workspace) elements of a ring FFT's work array of length m: the
+// array is unpadded (see fpad in sht_device.hpp).
+constexpr int fft_lds_slots(int m) { return m; }
+
+// Values a thread of a split plan keeps between its two turns (see
+// sht_device.hpp): up to kTurnStash / R turn butterflies of R values.
+constexpr int kTurnStash = 10;
+
 struct FftPlanHost {
     int n = 0, m = 0, nst = 0, blue = 0;
     int lc = -1;                // pruning: orders |k| <= lc only (-1: none)

@@ -30,27 +30,16 @@
 
 #include <hip/hip_runtime.h>
 
+#include "layout.hpp"
 #include "sht_device.hpp"
 
 namespace shtg {
-
-constexpr int kThreads = 256;  // generic workgroups
-// Ring-FFT workgroups have 256, 512 or 1024 threads.  The 1024-thread bound
-// caps the kernels at 128 VGPRs, i.e. up to 4 waves per SIMD: the ring FFTs
-// are latency-bound, so the stage radices stay small enough (up to 8) to
-// keep that occupancy.
-constexpr int kFftMaxThreads = 1024;
-constexpr int kWave = 64;      // Legendre workgroups are single waves
-constexpr int kSynthKMax = 4;  // ring pairs per lane in synthesis (1 .. 4)
-constexpr int kAnalKMax = 8;   // ring pairs per lane and pass in analysis (1 .. 8)
-constexpr int kAnalLC = 8;     // degrees per reduction chunk in analysis
 
 // ====================================================================== tiles
 
 // Coefficient tiles of kTL degrees x kTM orders, transposed through LDS so
 // that both the [l][m] arrays and the per-order packed arrays are accessed
 // contiguously.
-constexpr int kTM = 64, kTL = 32;
 typedef double2 CoefTile[kTL][kTM + 1];
 
 // cpack[tri(m) + l - m] = (C s, S s) for the tile's m <= l <= L.
@@ -1226,19 +1215,6 @@ __global__ __launch_bounds__(kThreads) void k_gradient_combine(double *__restric
 
 // ============================================================= batched kernels
 
-// Per-field description of a batch (all fields: lmax_calc = lmax).
-struct BField {
-    int L;           // degree (grid and expansion)
-    int P;           // ring-pair slots
-    int nlat;        // grid rows
-    int nlong;       // grid columns (FFT length)
-    int geo;         // offset of the field's slots in the concatenated ring tables
-    int fft;         // index of the field's FFT plan
-    long long coff;  // offset of the field in cpack/cbuf (double2 units)
-    long long foff;  // offset of the field in fbuf/gbuf (double4 units)
-    int sm;          // offset of the field's per-order first slots in T.smin
-};
-
 struct BTables {
     const double *alpha, *s, *post, *seed;  // s: pack scale, post: unpack scale
     int Lt;
@@ -1301,7 +1277,6 @@ __global__ __launch_bounds__(kWave) void kb_synth_leg(const BField *__restrict__
 // multiply-adds in k order, which is the order of the VALU tile's even (odd)
 // degree sums.
 typedef double v4d __attribute__((ext_vector_type(4)));
-constexpr int kBucketMaxCols = 64;  // up to 32 fields per MFMA bucket wave (4 column tiles)
 
 // items: (bucket, m, first slot, 0); bf: the bucket's nbf fields.
 //
@@ -1311,7 +1286,6 @@ constexpr int kBucketMaxCols = 64;  // up to 32 fields per MFMA bucket wave (4 c
 // of the next block are loaded while the current block's products run.
 // (Two 16-slot tiles per wave, sharing each coefficient operand, measured
 // slower: more registers, fewer waves.)
-constexpr int kMfmaSlots = 16;  // slots per MFMA synthesis wave
 template <int NT>
 __device__ __forceinline__ void synth_leg_mfma_tile(const BField *__restrict__ fields, const int *__restrict__ bf,
                                                     int nbf, const BField &f0, int m, int s0, const BTables &T,

@@ -87,7 +87,6 @@ __device__ __forceinline__ int fft_pos(const FftDev &P, int k)
 // strides, which spread over the banks; padding (one slot per 16 elements)
 // broke those patterns and measured 6% slower.
 __device__ __forceinline__ int fpad(int e) { return e; }
-__host__ __device__ __forceinline__ int fft_lds_slots(int m) { return m; }
 
 // Fully unrolled forward DFTs, y_k = sum_j v_j e^{-2 pi i jk/R}, in place.
 __device__ __forceinline__ void dft2(double2 &a, double2 &b)
@@ -413,8 +412,7 @@ __device__ __forceinline__ void fft_dit_back(double2 *x, const FftDev &P)
 
 // Values a thread keeps between the two turns of a split plan: its turn
 // butterflies' R values each, at most kTurnStash of them (the launch gives
-// every thread at most kTurnStash / R butterflies; see split_turn_threads).
-constexpr int kTurnStash = 10;
+// every thread at most kTurnStash / R butterflies; see plan_min_threads).
 struct TurnStash {
     double2 v[kTurnStash];
 };

@@ -18,18 +18,15 @@
 
 #include "../../include/shtools_amd.h"
 #include "fftplan.hpp"
+#include "checks.hpp"
 #include "kernels.hpp"
 #include "tables.hpp"
 
 namespace shtg {
 
-enum { ST_OK = 0, ST_DIMS = 1, ST_BOUNDS = 2, ST_ALLOC = 3, ST_OTHER = 5, ST_CAPTURE = 6, ST_SCRATCH = 7,
-       ST_NOTABLES = 8 };
-
-// Largest LDS footprint a ring-FFT workgroup may use (gfx950: 160 KiB), and
-// the elements of a workspace FFT's LDS chunk of that size.
-constexpr size_t kMaxLds = 160 * 1024;
+// Elements of a workspace FFT's LDS chunk of the largest LDS footprint.
 constexpr int kChunkCap = (int)(kMaxLds / sizeof(double2));
+static_assert(sizeof(Item4) == sizeof(int4) && sizeof(Item2) == sizeof(int2), "work items as int4 / int2");
 
 static std::mutex g_mu;
 
@@ -149,43 +146,7 @@ static const DevSeeds *get_seeds(int dev, int norm, int csphase, int L, bool cpl
     return &r;
 }
 
-// Ring FFTs run in LDS when their work array fits a CU's LDS, otherwise on
-// a global workspace of kWsGroups slices.
-constexpr int kWsGroups = 512;
-static bool fft_in_lds(int m) { return (size_t)fft_lds_slots(m) * sizeof(double2) <= kMaxLds; }
-static long long ws_stride(int m) { return ((long long)fft_lds_slots(m) + 15) / 16 * 16; }
-
 static bool upload_fft_arrays(const FftPlanHost &h, FftDev &d);
-
-// Workgroups of a ring FFT with work length m that share a CU's LDS (0: the
-// array does not fit, the ring runs on the global workspace).
-static int lds_per_cu(int m)
-{
-    if (!fft_in_lds(m)) return 0;
-    return std::min(4, (int)(kMaxLds / ((size_t)fft_lds_slots(m) * sizeof(double2))));
-}
-
-// Whether a ring of length n with band limit lc runs the split Bluestein
-// plan (make_fft_plan): when its shorter work array brings the ring from the
-// global workspace into the LDS, for at most 15% more transform work (3
-// transforms of the split length against 2 of the whole one).  Plain and
-// batched transforms decide alike, so they stay bitwise equal.
-//
-// Rings that fit the LDS either way keep the whole plan: the split array is
-// about 2/3 as long but has as many stages, and its second inverse pass adds
-// more LDS traffic than the extra workgroups per CU recover (measured on the
-// 512-field benchmark: synthesis ring FFTs 13.9 -> 17.1 ms with split plans
-// wherever they raised the workgroups per CU, analysis 13.2 -> 13.4 ms).
-static bool ring_split(int n, int lc)
-{
-    const int mh = split_length(n, lc);
-    if (mh <= 0 || !fft_in_lds(mh)) return false;
-    const int mf = bluestein_length(n + 2 * lc);
-    if (3.0 * mh > 1.15 * 2.0 * mf) return false;
-    return lds_per_cu(mf) == 0;
-}
-
-static FftPlanHost ring_plan(int n, int lc) { return make_fft_plan(n, lc, lc >= 0 && ring_split(n, lc)); }
 
 // Device arrays of an FFT plan that no work has been enqueued on.
 static void free_fft(FftDev &d)
@@ -451,16 +412,6 @@ static int current_device()
     return d;
 }
 
-// Ring pairs per lane of an analysis wave for P slots (1 .. kAnalKMax): the
-// fewest passes of at most kAnalKMax * 64 slots, with lanes packed as tightly
-// as those passes allow.
-static int anal_lanes_per_slot(int P, int kmax = kAnalKMax)
-{
-    const int per = kmax * kWave;
-    const int np = std::max(1, (P + per - 1) / per);
-    return std::max(1, (P + np * kWave - 1) / (np * kWave));
-}
-
 // Waves a plain transform's Legendre launch aims for (8 per SIMD), and the
 // fewest ring pairs per lane the plain kernels go down to for it.
 constexpr int kLegWaves = 8192;
@@ -511,51 +462,6 @@ static BatchAnalLegKernel batch_anal_leg_kernel(int K)
     static const BatchAnalLegKernel k[kAnalKMax] = {kb_anal_leg<1>, kb_anal_leg<2>, kb_anal_leg<3>, kb_anal_leg<4>,
                                                     kb_anal_leg<5>, kb_anal_leg<6>, kb_anal_leg<7>, kb_anal_leg<8>};
     return k[std::min(std::max(K, 1), kAnalKMax) - 1];
-}
-
-// Tuning of the ring-FFT launches: threads per workgroup by work length and
-// ring pairs per batched workgroup.  SHTG_FFT_MAXTHREADS (256/512/1024) and
-// SHTG_FFT_SLOTS override the defaults.
-static int env_int(const char *name, int dflt)
-{
-    const char *v = std::getenv(name);
-    if (!v || !*v) return dflt;
-    int x = std::atoi(v);
-    return x > 0 ? x : dflt;
-}
-
-static int fft_threads(int m)
-{
-    static const int cap = std::min(kFftMaxThreads, env_int("SHTG_FFT_MAXTHREADS", kFftMaxThreads));
-    int t = 256;
-    if (m > 1024) t = 512;
-    if (m > 4096) t = 1024;
-    return std::min(t, cap);
-}
-
-// Threads per workgroup of a plan's ring FFTs: fft_threads(m), and for a
-// split plan at least as many as give every thread at most kTurnStash / R
-// turn butterflies (R the last radix), and 512 for the largest arrays (two
-// such workgroups share a CU, each with up to 128 registers per thread).
-static int plan_min_threads(const FftDev &F)
-{
-    if (!F.half) return 1;
-    const int R = F.radix[F.nst - 1], nb = kTurnStash / R;
-    const int need = (F.m / R + nb - 1) / nb;
-    return (need + kWave - 1) / kWave * kWave;
-}
-
-static int plan_threads(const FftDev &F)
-{
-    int t = fft_threads(F.m);
-    if (F.half && F.m > 4096) t = std::min(t, kFftMaxThreads / 2);
-    return std::max(t, plan_min_threads(F));
-}
-
-static int fft_slots_per_wg()
-{
-    static const int v = std::max(1, std::min(16, env_int("SHTG_FFT_SLOTS", 2)));
-    return v;
 }
 
 static int grid_blocks(long long work)
@@ -974,61 +880,12 @@ static void batch_free(BatchPlan *b)
     delete b;
 }
 
-struct W {
-    double w;
-    int a, b, c;
-};
-static bool by_work(const W &x, const W &y) { return x.w > y.w; }
-
-// Synthesis Legendre work items of a plan, longest first: buckets of at least
-// mfma_min fields of one degree and sampling (mfma_min >= 2; 0: none) as
-// (bucket, m, kMfmaSlots-slot tile) items of kb_synth_leg_mfma, every other field as
-// (field, m, slot block, ring pairs per lane) items of kb_synth_leg.
+// Synthesis Legendre work items of a plan (build_synth_items): same-degree
+// buckets of at least mfma_min fields (0: none) to the MFMA kernel, the rest
+// to the VALU kernel with b->synth_kmax ring pairs per lane.
 static int batch_synth_items(BatchPlan *b, int mfma_min)
 {
-    const std::vector<BField> &fields = b->hfields;
-    const int nfields = (int)fields.size();
-    std::map<std::pair<int, int>, std::vector<int>> groups;  // (L, nlong) -> fields
-    for (int i = 0; i < nfields; ++i) groups[{fields[i].L, fields[i].nlong}].push_back(i);
-    std::vector<int> in_bucket(nfields, -1);
-    std::vector<int> bf, bn;
-    if (mfma_min >= 2)
-        for (auto &kv : groups) {
-            const std::vector<int> &mem = kv.second;
-            for (size_t k0 = 0; k0 + (size_t)mfma_min <= mem.size(); k0 += kBucketMaxCols / 2) {
-                const int cnt = (int)std::min(mem.size() - k0, (size_t)(kBucketMaxCols / 2));
-                if (cnt < mfma_min) break;
-                const int id = (int)bn.size();
-                bn.push_back(cnt);
-                for (int k = 0; k < kBucketMaxCols / 2; ++k) bf.push_back(k < cnt ? mem[k0 + k] : mem[k0]);
-                for (int k = 0; k < cnt; ++k) in_bucket[mem[k0 + k]] = id;
-            }
-        }
-    std::vector<W> ws, wm;
-    for (int i = 0; i < nfields; ++i) {
-        const BField &f = fields[i];
-        const int *sm = b->hsmin.data() + f.sm;
-        if (in_bucket[i] >= 0) {
-            if (bf[(size_t)in_bucket[i] * (kBucketMaxCols / 2)] != i) continue;  // the bucket's first field
-            for (int m = 0; m <= f.L; ++m)
-                for (int s0 = sm[m] / 16 * 16; s0 < f.P; s0 += kMfmaSlots)
-                    wm.push_back({(double)(f.L - m + 1) * bn[in_bucket[i]], in_bucket[i], m, s0});
-            continue;
-        }
-        for (int m = 0; m <= f.L; ++m) {
-            // blocks of kSynthKMax*64 slots from the order's first slot with
-            // non-negligible functions; the last one packs its lanes tightly
-            for (int s0 = sm[m]; s0 < f.P; s0 += b->synth_kmax * kWave) {
-                const int kk = std::min(b->synth_kmax, (f.P - s0 + kWave - 1) / kWave);
-                ws.push_back({(double)(f.L - m + 1) * std::min(kk * kWave, f.P - s0), i, m, s0 + (kk << 24)});
-            }
-        }
-    }
-    std::stable_sort(ws.begin(), ws.end(), by_work);
-    std::stable_sort(wm.begin(), wm.end(), by_work);
-    std::vector<int4> is, im;
-    for (auto &x : ws) is.push_back(make_int4(x.a, x.b, x.c & 0xFFFFFF, x.c >> 24));
-    for (auto &x : wm) im.push_back(make_int4(x.a, x.b, x.c, 0));
+    const SynthItems it = build_synth_items(b->hfields, b->hsmin, b->synth_kmax, mfma_min);
     // the replaced lists may still be read by queued work or captured graphs:
     // they are kept until the plan is destroyed
     for (void *p : {(void *)b->items_sleg, (void *)b->items_smfma, (void *)b->bfields, (void *)b->bnf})
@@ -1036,41 +893,30 @@ static int batch_synth_items(BatchPlan *b, int mfma_min)
     b->items_sleg = nullptr;
     b->items_smfma = nullptr;
     b->bfields = b->bnf = nullptr;
-    b->n_sleg = (int)is.size();
-    b->n_smfma = (int)im.size();
-    if (b->n_sleg && !(b->items_sleg = dev_upload(is))) return ST_ALLOC;
+    b->n_sleg = (int)it.valu.size();
+    b->n_smfma = (int)it.mfma.size();
+    if (b->n_sleg && !(b->items_sleg = reinterpret_cast<int4 *>(dev_upload(it.valu)))) return ST_ALLOC;
     if (b->n_smfma) {
-        b->items_smfma = dev_upload(im);
-        b->bfields = dev_upload(bf);
-        b->bnf = dev_upload(bn);
+        b->items_smfma = reinterpret_cast<int4 *>(dev_upload(it.mfma));
+        b->bfields = dev_upload(it.bfields);
+        b->bnf = dev_upload(it.bnf);
         if (!b->items_smfma || !b->bfields || !b->bnf) return ST_ALLOC;
     }
     b->mfma_min = mfma_min;
     return ST_OK;
 }
 
-// Analysis Legendre work items, longest first: one single-wave item (field,
-// m) per order, grouped by ring pairs per lane (at most b->anal_kmax) into
-// one launch each.  Replaced lists are kept until the plan is destroyed.
+// Analysis Legendre work items (build_anal_items) with at most b->anal_kmax
+// ring pairs per lane.  Replaced lists are kept until the plan is destroyed.
 static int batch_anal_items(BatchPlan *b)
 {
-    std::vector<W> wa;
-    for (int i = 0; i < (int)b->hfields.size(); ++i) {
-        const BField &f = b->hfields[i];
-        const int *sm = b->hsmin.data() + f.sm;
-        for (int m = 0; m <= f.L; ++m) {
-            const int sa = std::max(1, sm[m]);
-            wa.push_back({(double)(f.L - m + 1) * (f.P - sa), i, m, anal_lanes_per_slot(f.P - sa, b->anal_kmax)});
-        }
-    }
-    std::stable_sort(wa.begin(), wa.end(), by_work);
-    std::vector<int2> ia[kAnalKMax];
-    for (auto &x : wa) ia[x.c - 1].push_back(make_int2(x.a, x.b));
+    std::vector<Item2> ia[kAnalKMax];
+    build_anal_items(b->hfields, b->hsmin, b->anal_kmax, ia);
     for (int k = 0; k < kAnalKMax; ++k) {
         if (b->items_aleg[k]) b->retired.push_back(b->items_aleg[k]);
         b->items_aleg[k] = nullptr;
         b->n_aleg[k] = (int)ia[k].size();
-        if (b->n_aleg[k] && !(b->items_aleg[k] = dev_upload(ia[k]))) return ST_ALLOC;
+        if (b->n_aleg[k] && !(b->items_aleg[k] = reinterpret_cast<int2 *>(dev_upload(ia[k])))) return ST_ALLOC;
     }
     return ST_OK;
 }
@@ -1083,7 +929,7 @@ static int batch_create_dh(int nfields, const int *lmax, const int *sampling, in
     if (norm < 1 || norm > 4 || (csphase != 1 && csphase != -1)) return ST_BOUNDS;
     int Lmax = 0;
     for (int i = 0; i < nfields; ++i) {
-        if (lmax[i] < 0 || (sampling[i] != 1 && sampling[i] != 2)) return ST_BOUNDS;
+        if (lmax[i] < 0 || lmax[i] > kMaxArgDegree || (sampling[i] != 1 && sampling[i] != 2)) return ST_BOUNDS;
         Lmax = std::max(Lmax, lmax[i]);
     }
     const int dev = current_device();
@@ -1100,149 +946,55 @@ static int batch_create_dh(int nfields, const int *lmax, const int *sampling, in
     const DevSeeds *sd = get_seeds(dev, norm, csphase, Lmax);
     if (!rt || !sd) return ST_ALLOC;
 
-    // ring geometry and FFT plans per distinct (degree, sampling)
-    const double pi = std::acos(-1.0);
-    auto key_of = [&](int i) { return std::make_pair(lmax[i], sampling[i]); };
-    typedef std::pair<int, int> Key;
-    std::map<Key, std::pair<int, int>> geo_of;  // -> (geo offset, plan index)
-    std::map<Key, int> sm_of;                   // -> offset of its smin row
-    std::map<Key, std::vector<int>> smin_of;    // -> smin row
-    std::vector<double> z, u, w;
-    std::vector<int> rows, kmax, smin;
-    std::vector<size_t> plan_lds;
-    for (int i = 0; i < nfields; ++i) {
-        const int L = lmax[i];
-        const Key key = key_of(i);
-        if (geo_of.count(key)) continue;
-        const int n = 2 * L + 2, nlong = sampling[i] * n, P = n / 2 + 1;
-        const int off = (int)z.size();
-        std::vector<double> aj;
-        dh_weights(n, aj);
-        const double c = std::sqrt(2 * pi) * std::sqrt(4.0 * pi) / (double)nlong;
-        for (int s = 0; s < P; ++s) {
-            const double zz = (s == n / 2) ? 0.0 : std::cos(pi * (double)s / (double)n);
-            z.push_back(zz);
-            u.push_back((s == n / 2) ? 1.0 : std::sqrt((1.0 - zz) * (1.0 + zz)));
-            rows.push_back(s);
-            rows.push_back((s == n / 2) ? -1 : n - s);
-            w.push_back(c * aj[s]);
-            w.push_back((s == 0 || s == n / 2) ? 0.0 : c * aj[n - s]);
-        }
-        const std::vector<double> zl(z.begin() + off, z.end()), ul(u.begin() + off, u.end());
-        const std::vector<int> km = polar_kmax(zl, ul, L);
-        kmax.insert(kmax.end(), km.begin(), km.end());
-        smin_of[key] = polar_smin(km, L);
-        sm_of[key] = (int)smin.size();
-        smin.insert(smin.end(), smin_of[key].begin(), smin_of[key].end());
-        FftPlanHost fp = ring_plan(nlong, L);
+    // ring geometry, FFT plans, field offsets and work items (layout.hpp)
+    BatchLayout lay;
+    build_batch_layout(nfields, lmax, sampling, lay);
+    for (const FftPlanHost &fp : lay.plans) {
         FftDev fd;
         if (!upload_fft(fp, fd)) return ST_OTHER;
         b->hplans.push_back(fd);
-        plan_lds.push_back((size_t)fft_lds_slots(fp.m) * sizeof(double2));
-        geo_of[key] = {off, (int)b->hplans.size() - 1};
     }
-    b->z = dev_upload(z);
-    b->u = dev_upload(u);
-    b->w = reinterpret_cast<double2 *>(dev_upload(w));
-    b->ones = reinterpret_cast<double2 *>(dev_upload(std::vector<double>(w.size(), 1.0)));
-    b->rows = reinterpret_cast<int2 *>(dev_upload(rows));
-    b->kmax = dev_upload(kmax);
-    b->smin = dev_upload(smin);
+    b->z = dev_upload(lay.z);
+    b->u = dev_upload(lay.u);
+    b->w = reinterpret_cast<double2 *>(dev_upload(lay.w));
+    b->ones = reinterpret_cast<double2 *>(dev_upload(std::vector<double>(lay.w.size(), 1.0)));
+    b->rows = reinterpret_cast<int2 *>(dev_upload(lay.rows));
+    b->kmax = dev_upload(lay.kmax);
+    b->smin = dev_upload(lay.smin);
     b->plans = dev_upload(b->hplans);
     if (!b->z || !b->u || !b->w || !b->ones || !b->rows || !b->kmax || !b->smin || !b->plans) return ST_ALLOC;
-
-    // fields and workspace offsets
-    std::vector<BField> fields(nfields);
-    long long coff = 0, foff = 0;
-    for (int i = 0; i < nfields; ++i) {
-        const int L = lmax[i], n = 2 * L + 2;
-        BField &f = fields[i];
-        f.L = L;
-        f.P = n / 2 + 1;
-        f.nlat = n;
-        f.nlong = sampling[i] * n;
-        f.geo = geo_of[key_of(i)].first;
-        f.fft = geo_of[key_of(i)].second;
-        f.coff = coff;
-        f.foff = foff;
-        f.sm = sm_of[key_of(i)];
-        coff += tri_size(L);
-        foff += (long long)(L + 1) * f.P;
-    }
-    b->fields = dev_upload(fields);
+    b->fields = dev_upload(lay.fields);
     if (!b->fields) return ST_ALLOC;
-    if (!HIP_OK(hipMalloc(&b->cbuf, (size_t)coff * sizeof(double2)))) return ST_ALLOC;
-    if (!HIP_OK(hipMalloc(&b->fbuf, (size_t)foff * sizeof(double4)))) return ST_ALLOC;
-
-    // work items, longest first
-    b->hfields = fields;
-    b->hsmin = smin;
-    std::vector<W> wp, wu;
-    for (int i = 0; i < nfields; ++i) {
-        const BField &f = fields[i];
-        for (int m0 = 0; m0 <= f.L; m0 += kTM)
-            for (int l0 = 0; l0 <= f.L; l0 += kTL) {
-                wu.push_back({(double)f.L, i, m0, l0});
-                if (l0 + kTL - 1 >= m0) wp.push_back({(double)f.L, i, m0, l0});
-            }
-    }
-    std::stable_sort(wp.begin(), wp.end(), by_work);
-    std::vector<int4> ip, iu;
-    for (auto &x : wp) ip.push_back(make_int4(x.a, x.b, x.c, 0));
-    for (auto &x : wu) iu.push_back(make_int4(x.a, x.b, x.c, 0));
-    b->items_pack = dev_upload(ip);
+    if (!HIP_OK(hipMalloc(&b->cbuf, (size_t)lay.ncpack * sizeof(double2)))) return ST_ALLOC;
+    if (!HIP_OK(hipMalloc(&b->fbuf, (size_t)lay.nfbuf * sizeof(double4)))) return ST_ALLOC;
+    b->hfields = lay.fields;
+    b->hsmin = lay.smin;
+    b->items_pack = reinterpret_cast<int4 *>(dev_upload(lay.pack));
+    b->items_unpack = reinterpret_cast<int4 *>(dev_upload(lay.unpack));
+    b->n_pack = (int)lay.pack.size();
+    b->n_unpack = (int)lay.unpack.size();
+    if (!b->items_pack || !b->items_unpack) return ST_ALLOC;
     if (int rc = batch_synth_items(b.get(), 0)) return rc;
     if (int rc = batch_anal_items(b.get())) return rc;
-    b->items_unpack = dev_upload(iu);
-    b->n_pack = (int)ip.size();
-    b->n_unpack = (int)iu.size();
-    if (!b->items_pack || !b->items_sleg || !b->items_unpack) return ST_ALLOC;
 
     b->fft_slots = fft_slots_per_wg();
-    // FFT items grouped by how many workgroups fit the LDS of a CU (the
-    // group's LDS is its largest need), then by threads per workgroup
-    typedef std::tuple<int, int, int> GKey;  // (-(WGs per CU), -threads, split)
-    std::map<GKey, std::vector<W>> groups;
-    std::map<GKey, size_t> group_lds;
-    std::map<GKey, int> group_min;
-    std::vector<W> wsi;
-    for (int i = 0; i < nfields; ++i) {
-        const BField &f = fields[i];
-        const size_t need = plan_lds[f.fft];
-        if (!fft_in_lds(b->hplans[f.fft].m)) {
-            for (int s = 0; s < f.P; s += b->fft_slots) wsi.push_back({(double)need, i, s, 0});
-            b->fws_stride = std::max(b->fws_stride, ws_stride(b->hplans[f.fft].m));
-            continue;
-        }
-        const int per_cu = std::max(1, (int)(kMaxLds / need));
-        const int thr = plan_threads(b->hplans[f.fft]);
-        const GKey key(-std::min(per_cu, 4), -thr, b->hplans[f.fft].half);
-        group_lds[key] = std::max(group_lds[key], need);
-        group_min[key] = std::max(group_min[key], plan_min_threads(b->hplans[f.fft]));
-        for (int s = 0; s < f.P; s += b->fft_slots) groups[key].push_back({(double)need, i, s, 0});
-    }
-    b->fft_group_of.assign(nfields, (int)groups.size());
-    for (auto &kv : groups) {
-        for (auto &x : kv.second) b->fft_group_of[x.a] = (int)b->fft.size();
-        std::stable_sort(kv.second.begin(), kv.second.end(), by_work);
-        std::vector<int2> it;
-        for (auto &x : kv.second) it.push_back(make_int2(x.a, x.b));
+    const FftLayout fl = build_fft_groups(lay.fields, lay.plans, b->fft_slots);
+    b->fft_group_of = fl.group_of;
+    for (const FftGroupLayout &gl : fl.groups) {
         FftGroup g;
-        g.items = dev_upload(it);
-        g.nitems = (int)it.size();
-        g.lds = group_lds[kv.first];
-        g.threads = g.threads_anal = -std::get<1>(kv.first);
-        g.min_threads = group_min[kv.first];
-        g.half = std::get<2>(kv.first) != 0;
+        g.items = reinterpret_cast<int2 *>(dev_upload(gl.items));
+        g.nitems = (int)gl.items.size();
+        g.lds = gl.lds;
+        g.threads = g.threads_anal = gl.threads;
+        g.min_threads = gl.min_threads;
+        g.half = gl.half;
         if (!g.items) return ST_ALLOC;
         b->fft.push_back(g);
     }
-    if (!wsi.empty()) {
-        std::stable_sort(wsi.begin(), wsi.end(), by_work);
-        std::vector<int2> it;
-        for (auto &x : wsi) it.push_back(make_int2(x.a, x.b));
-        b->fft_ws.items = dev_upload(it);
-        b->fft_ws.nitems = (int)it.size();
+    if (!fl.ws_items.empty()) {
+        b->fws_stride = fl.ws_stride;
+        b->fft_ws.items = reinterpret_cast<int2 *>(dev_upload(fl.ws_items));
+        b->fft_ws.nitems = (int)fl.ws_items.size();
         b->fft_ws.threads = kFftMaxThreads;
         if (!b->fft_ws.items) return ST_ALLOC;
         if (!HIP_OK(hipMalloc(&b->fws, (size_t)kWsGroups * b->fws_stride * sizeof(double2)))) return ST_ALLOC;
@@ -1526,114 +1278,6 @@ static int batch_tune(BatchPlan *b, hipStream_t st, int reps, int flags)
     (void)hipEventDestroy(e1);
     if (st_out) return st_out;
     return batch_tune_legendre(b, st, reps, (flags & 1) != 0);
-}
-
-// ------------------------------------------------------------ argument checks
-
-// Shapes of one plain transform after the reference's argument checks.
-struct Shape {
-    int lmax = 0;      // grid degree
-    int Lc = 0;        // degrees computed
-    int nrows = 0;     // grid rows written by synthesis (with extend)
-    int ext = 0;       // extend
-};
-
-// src/MakeGridDH.f95:107-251
-static int check_makegrid_dh(int cilm_d1, int cilm_d2, int lmax, int norm, int sampling, int csphase,
-                             int lmax_calc, int extend, int grid_d0, int grid_d1, Shape &sh)
-{
-    if (lmax < 0) return ST_BOUNDS;
-    const int n = 2 * lmax + 2;
-    int nlong;
-    if (sampling == 1) nlong = n;
-    else if (sampling == 2) nlong = 2 * n;
-    else return ST_BOUNDS;
-    if (extend != 0 && extend != 1) return ST_BOUNDS;
-    const int nlat_out = n + extend, nlong_out = nlong + extend;
-    if (grid_d0 < nlat_out || grid_d1 < nlong_out) return ST_DIMS;
-    if (norm < 1 || norm > 4) return ST_BOUNDS;
-    if (csphase != 1 && csphase != -1) return ST_BOUNDS;
-    if (lmax_calc > lmax) return ST_BOUNDS;
-    if (lmax_calc < 0 || cilm_d1 < 1 || cilm_d2 < 1) return ST_BOUNDS;
-    int Lc = lmax;
-    if (cilm_d2 - 1 < Lc) Lc = cilm_d2 - 1;
-    if (cilm_d1 - 1 < Lc) Lc = cilm_d1 - 1;
-    if (lmax_calc < Lc) Lc = lmax_calc;
-    sh.lmax = lmax;
-    sh.Lc = Lc;
-    sh.nrows = nlat_out;
-    sh.ext = extend;
-    return ST_OK;
-}
-
-// src/SHExpandDH.f95:118-270
-static int check_shexpand_dh(int grid_d0, int grid_d1, int norm, int sampling, int csphase, int lmax_calc,
-                             int cilm_d1, int cilm_d2, Shape &sh)
-{
-    const int n = grid_d0;
-    const int lmax = n / 2 - 1;
-    if (lmax_calc > lmax) return ST_BOUNDS;
-    if (lmax_calc < 0) return ST_BOUNDS;
-    const int Lc = lmax_calc < lmax ? lmax_calc : lmax;
-    if (sampling != 1 && sampling != 2) return ST_BOUNDS;
-    if (n % 2 != 0 || n < 2) return ST_BOUNDS;
-    if (cilm_d1 < Lc + 1 || cilm_d2 < Lc + 1) return ST_DIMS;
-    const int nlong = sampling * n;
-    if (grid_d1 < nlong) return ST_DIMS;
-    if (csphase != 1 && csphase != -1) return ST_BOUNDS;
-    if (norm < 1 || norm > 4) return ST_BOUNDS;
-    sh.lmax = lmax;
-    sh.Lc = Lc;
-    sh.nrows = n;
-    return ST_OK;
-}
-
-// src/MakeGridGLQ.f95:108-270
-static int check_makegrid_glq(int cilm_d1, int cilm_d2, int lmax, int nzero, const double *zero, int norm,
-                              int csphase, int lmax_calc, int extend, int grid_d0, int grid_d1, Shape &sh)
-{
-    if (lmax < 0) return ST_BOUNDS;
-    const int nlong = 2 * lmax + 1, nlat = lmax + 1;
-    if (extend != 0 && extend != 1) return ST_BOUNDS;
-    const int nlong_out = nlong + extend;
-    if (grid_d1 < nlong_out || grid_d0 < nlat) return ST_DIMS;
-    if (!zero) return ST_OTHER;
-    if (nzero < lmax + 1) return ST_DIMS;
-    if (norm < 1 || norm > 4) return ST_BOUNDS;
-    if (csphase != 1 && csphase != -1) return ST_BOUNDS;
-    if (lmax_calc > lmax) return ST_BOUNDS;
-    if (lmax_calc < 0 || cilm_d1 < 1 || cilm_d2 < 1) return ST_BOUNDS;
-    int Lc = lmax;
-    if (cilm_d2 - 1 < Lc) Lc = cilm_d2 - 1;
-    if (cilm_d1 - 1 < Lc) Lc = cilm_d1 - 1;
-    if (lmax_calc < Lc) Lc = lmax_calc;
-    sh.lmax = lmax;
-    sh.Lc = Lc;
-    sh.nrows = nlat;
-    sh.ext = extend;
-    return ST_OK;
-}
-
-// src/SHExpandGLQ.f95:100-250
-static int check_shexpand_glq(int grid_d0, int grid_d1, const double *w, int nw, const double *zero, int nzero,
-                              int norm, int csphase, int lmax_calc, int cilm_d1, int cilm_d2, Shape &sh)
-{
-    const int lmax = grid_d0 - 1;
-    if (lmax < 0) return ST_DIMS;
-    if (lmax_calc > lmax) return ST_BOUNDS;
-    if (lmax_calc < 0) return ST_BOUNDS;
-    const int Lc = lmax_calc < lmax ? lmax_calc : lmax;
-    if (cilm_d1 < Lc + 1 || cilm_d2 < Lc + 1) return ST_DIMS;
-    if (grid_d1 < 2 * lmax + 1 || grid_d0 < lmax + 1) return ST_DIMS;
-    if (!w || nw < lmax + 1) return ST_DIMS;
-    if (!zero) return ST_OTHER;
-    if (nzero < lmax + 1) return ST_DIMS;
-    if (norm < 1 || norm > 4) return ST_BOUNDS;
-    if (csphase != 1 && csphase != -1) return ST_BOUNDS;
-    sh.lmax = lmax;
-    sh.Lc = Lc;
-    sh.nrows = lmax + 1;
-    return ST_OK;
 }
 
 // Device tables of one call, under g_mu.
@@ -2158,7 +1802,7 @@ int shtg_shmultiply(double *cilmout, int out_d1, int out_d2, const double *cilm1
                     const double *cilm2, int c2_d1, int c2_d2, int lmax2, int norm, int csphase, void *stream)
 {
     // argument checks in the order of src/SHMultiply.f95:61-180
-    if (lmax1 < 0 || lmax2 < 0) return ST_BOUNDS;
+    if (lmax1 < 0 || lmax2 < 0 || lmax1 > kMaxArgDegree || lmax2 > kMaxArgDegree - lmax1) return ST_BOUNDS;
     if (c1_d1 < lmax1 + 1 || c1_d2 < lmax1 + 1) return ST_DIMS;
     if (c2_d1 < lmax2 + 1 || c2_d2 < lmax2 + 1) return ST_DIMS;
     const int lmaxout = lmax1 + lmax2;
@@ -2195,7 +1839,7 @@ int shtg_shmultiply(double *cilmout, int out_d1, int out_d2, const double *cilm1
 
 int shtg_dh_polar_orders(int lmax, int *kmax)
 {
-    if (lmax < 0) return ST_BOUNDS;
+    if (lmax < 0 || lmax > kMaxArgDegree) return ST_BOUNDS;
     if (!kmax) return ST_DIMS;
     const int n = 2 * lmax + 2, P = n / 2 + 1;
     const double pi = std::acos(-1.0);
@@ -2218,13 +1862,13 @@ int shtg_nglqsh(int degree) { return degree < 0 ? -1 : degree + 1; }
 int shtg_nglqshn(int degree, int n)
 {
     if (degree < 0) return -1;
-    const long long d = (long long)(n + 1) * degree + 1;  // ceiling(d / 2), d of either sign
+    const long long d = ((long long)n + 1) * degree + 1;  // ceiling(d / 2), d of either sign
     return (int)(d >= 0 ? (d + 1) / 2 : -((-d) / 2));
 }
 
 int shtg_shglq(int lmax, double *zero, double *w)
 {
-    if (lmax < 0) return ST_BOUNDS;
+    if (lmax < 0 || lmax > kMaxArgDegree) return ST_BOUNDS;
     if (!zero || !w) return ST_DIMS;
     return glq_nodes(lmax + 1, zero, w);
 }
@@ -2243,7 +1887,7 @@ int shtg_dhaj(int n, int extend, double *aj)
 
 int shtg_glqgridcoord(int lmax, int extend, double *lat, double *lon)
 {
-    if (lmax < 0) return ST_BOUNDS;
+    if (lmax < 0 || lmax > kMaxArgDegree) return ST_BOUNDS;
     if (extend != 0 && extend != 1) return ST_BOUNDS;
     if (!lat || !lon) return ST_DIMS;
     const int nlat = lmax + 1, nlong = 2 * lmax + 1;
@@ -2262,12 +1906,13 @@ int shtg_glqgridcoord(int lmax, int extend, double *lat, double *lon)
 // PLegendreA/PlmON family for the transforms' normalisations).
 int shtg_plm(int lmax, int norm, int csphase, double z, double *p)
 {
-    if (lmax < 0 || norm < 1 || norm > 4 || (csphase != 1 && csphase != -1)) return ST_BOUNDS;
+    if (lmax < 0 || lmax > kMaxArgDegree || norm < 1 || norm > 4 || (csphase != 1 && csphase != -1))
+        return ST_BOUNDS;
     if (!p) return ST_DIMS;
     RecurrenceTables t = build_recurrence(norm, lmax);
     std::vector<double> seed = build_seeds(norm, csphase, lmax);
     const double u = std::sqrt((1.0 - z) * (1.0 + z));
-    for (int i = 0; i < (lmax + 1) * (lmax + 1); ++i) p[i] = 0.0;
+    for (size_t i = 0; i < (size_t)(lmax + 1) * (lmax + 1); ++i) p[i] = 0.0;
     for (int m = 0; m <= lmax; ++m) {
         double r = 1.0;
         if (m > 0) {

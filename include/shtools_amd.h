@@ -196,6 +196,16 @@ int shtg_batch_create_dh(int nfields, const int *lmax, int sampling, int norm, i
  * [2lmax[i]+2][sampling[i](2lmax[i]+2)]. */
 int shtg_batch_create_dh_fields(int nfields, const int *lmax, const int *sampling, int norm, int csphase,
                                 shtg_batch_plan **plan);
+/* Batched GLQ transforms: per field MakeGridGLQ / SHExpandGLQ
+ * (src/MakeGridGLQ.f95:108-480, src/SHExpandGLQ.f95:100-649; f2py
+ * src/pyshtools.pyf:334-377) with lmax_calc = lmax, extend = 0,
+ * on the nodes and weights of shtg_shglq (SHGLQ), which the plan computes
+ * itself: grid[i] is a device [lmax[i]+1][2lmax[i]+1] array.  The batched
+ * calls, adjoints, tuning and field groups below apply unchanged. */
+int shtg_batch_create_glq(int nfields, const int *lmax, int norm, int csphase, shtg_batch_plan **plan);
+/* A plan mixing grids: grid[i] = 1 or 2 (DH with that sampling) or 3 (GLQ). */
+int shtg_batch_create_fields(int nfields, const int *lmax, const int *grid, int norm, int csphase,
+                             shtg_batch_plan **plan);
 int shtg_batch_makegrid_dh(shtg_batch_plan *plan, const double *const *cilm, double *const *grid,
                            void *stream);
 int shtg_batch_shexpand_dh(shtg_batch_plan *plan, const double *const *grid, double *const *cilm,

@@ -58,6 +58,9 @@ SIGNATURES = {
     "shtg_batch_create_dh": [_c_int, ctypes.POINTER(_c_int), _c_int, _c_int, _c_int, ctypes.POINTER(_vp)],
     "shtg_batch_create_dh_fields": [_c_int, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), _c_int, _c_int,
                                     ctypes.POINTER(_vp)],
+    "shtg_batch_create_glq": [_c_int, ctypes.POINTER(_c_int), _c_int, _c_int, ctypes.POINTER(_vp)],
+    "shtg_batch_create_fields": [_c_int, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), _c_int, _c_int,
+                                 ctypes.POINTER(_vp)],
     "shtg_batch_makegrid_dh": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp],
     "shtg_batch_shexpand_dh": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp],
     "shtg_batch_makegrid_dh_adjoint": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp],

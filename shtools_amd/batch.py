@@ -1,10 +1,13 @@
-"""Batched Driscoll-Healy transforms of many fields of uneven degree.
+"""Batched spherical-harmonic transforms of many fields of uneven degree.
 
-One plan per set of degrees; every call covers all fields with a handful of
-launches (pack, Legendre, ring FFTs grouped by size, unpack) whose work
-items are ordered longest first.  Per field the results are those of
-``backend.MakeGridDH(c, sampling=s, norm=n, csphase=p)`` and
-``backend.SHExpandDH(g, sampling=s, norm=n, csphase=p)``.
+One plan per set of degrees and grids; every call covers all fields with a
+handful of launches (pack, Legendre, ring FFTs grouped by size, unpack)
+whose work items are ordered longest first.  Per field the results are
+those of ``backend.MakeGridDH(c, sampling=s, norm=n, csphase=p)`` and
+``backend.SHExpandDH(g, sampling=s, norm=n, csphase=p)`` for DH fields, and
+of ``backend.MakeGridGLQ(c, zero, norm=n, csphase=p)`` and
+``backend.SHExpandGLQ(g, w, zero, norm=n, csphase=p)`` with ``zero, w =
+backend.SHGLQ(lmax)`` for GLQ fields.
 """
 import ctypes
 
@@ -42,28 +45,31 @@ def _write_cache(path, key, value):
     os.replace(tmp, path)
 
 
-class DHBatch:
-    """Plan for the fields of degrees ``lmax``; ``sampling`` is one value for
-    every field or a sequence with one per field."""
+# grid codes of the C interface (shtg_batch_create_fields)
+_GRID_CODES = {"DH1": 1, "DH2": 2, "GLQ": 3, 1: 1, 2: 2, 3: 3}
 
-    def __init__(self, lmax, sampling=2, norm=1, csphase=1, device=None):
+
+class SHBatch:
+    """Plan for the fields of degrees ``lmax`` on grids ``grid``: one of
+    "DH1", "DH2" (DH with sampling 1 or 2) or "GLQ" for every field, or a
+    sequence with one per field."""
+
+    def __init__(self, lmax, grid="DH2", norm=1, csphase=1, device=None):
         import torch
         self.torch = torch
         self.lmax = [int(L) for L in lmax]
-        if isinstance(sampling, (list, tuple)):
-            if len(sampling) != len(self.lmax):
-                raise SHToolsError(_status_message(1))
-            self.sampling = [int(x) for x in sampling]
-        else:
-            self.sampling = [int(sampling)] * len(self.lmax)
+        grids = list(grid) if isinstance(grid, (list, tuple)) else [grid] * len(self.lmax)
+        if len(grids) != len(self.lmax) or any(g not in _GRID_CODES for g in grids):
+            raise SHToolsError(_status_message(1 if len(grids) != len(self.lmax) else 2))
+        self.grid = [_GRID_CODES[g] for g in grids]
         self.norm, self.csphase = int(norm), int(csphase)
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         arr = (ctypes.c_int * len(self.lmax))(*self.lmax)
-        samp = (ctypes.c_int * len(self.lmax))(*self.sampling)
+        codes = (ctypes.c_int * len(self.lmax))(*self.grid)
         plan = ctypes.c_void_p()
         with torch.cuda.device(self.device):
-            st = _lib.lib().shtg_batch_create_dh_fields(len(self.lmax), arr, samp, self.norm, self.csphase,
-                                                        ctypes.byref(plan))
+            st = _lib.lib().shtg_batch_create_fields(len(self.lmax), arr, codes, self.norm, self.csphase,
+                                                     ctypes.byref(plan))
         if st != 0:
             raise SHToolsError(_status_message(st))
         self._plan = plan
@@ -79,8 +85,11 @@ class DHBatch:
                 pass
 
     def grid_shape(self, i):
-        n = 2 * self.lmax[i] + 2
-        return (n, self.sampling[i] * n)
+        L = self.lmax[i]
+        if self.grid[i] == 3:
+            return (L + 1, 2 * L + 1)
+        n = 2 * L + 2
+        return (n, self.grid[i] * n)
 
     def cilm_shape(self, i):
         L = self.lmax[i]
@@ -211,7 +220,8 @@ class DHBatch:
     def _tune_key(self):
         import hashlib
         arch = self.torch.cuda.get_device_properties(self.device).gcnArchName.split(":")[0]
-        sig = ",".join(f"{L}/{s}" for L, s in zip(self.lmax, self.sampling))
+        # grid codes 1 and 2 are the DH samplings
+        sig = ",".join(f"{L}/{g}" for L, g in zip(self.lmax, self.grid))
         return arch + ":" + hashlib.sha1(sig.encode()).hexdigest()[:16]
 
     def tuning(self):
@@ -263,3 +273,26 @@ class DHBatch:
         ta = (ctypes.c_int * n)()
         ng = _lib.lib().shtg_batch_get_shapes(self._plan, ts, ta, n)
         return [(ts[i], ta[i]) for i in range(min(ng, n))]
+
+
+class DHBatch(SHBatch):
+    """Plan for DH fields of degrees ``lmax``; ``sampling`` is one value for
+    every field or a sequence with one per field."""
+
+    def __init__(self, lmax, sampling=2, norm=1, csphase=1, device=None):
+        n = len(list(lmax))
+        samp = list(sampling) if isinstance(sampling, (list, tuple)) else [sampling] * n
+        if len(samp) != n:
+            raise SHToolsError(_status_message(1))
+        if any(int(x) not in (1, 2) for x in samp):
+            raise SHToolsError(_status_message(2))
+        self.sampling = [int(x) for x in samp]
+        super().__init__(lmax, grid=self.sampling, norm=norm, csphase=csphase, device=device)
+
+
+class GLQBatch(SHBatch):
+    """Plan for Gauss-Legendre fields of degrees ``lmax`` (grids of
+    lmax+1 x 2lmax+1 on the nodes of SHGLQ)."""
+
+    def __init__(self, lmax, norm=1, csphase=1, device=None):
+        super().__init__(lmax, grid="GLQ", norm=norm, csphase=csphase, device=device)

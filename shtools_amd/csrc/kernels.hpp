@@ -1221,7 +1221,7 @@ struct BTables {
     const double *z, *u;
     const double2 *w;     // analysis ring weights per slot
     const double2 *roww;  // synthesis row weights per slot (nullptr: 1)
-    int s0;               // first slot of the analysis sums
+    int s0;               // 1: analysis sums start at each field's s0; 0: at slot 0 (adjoint of synthesis)
     const int2 *rows;
     const FftDev *plans;
     const int *kmax;      // per slot (indexed like z): orders with non-negligible functions
@@ -1524,9 +1524,10 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft_ws(const BField *_
 }
 
 // items: (field, m); single-wave workgroups, one launch per ring pairs per
-// lane K.  Slots below T.s0 are skipped (the DH pole slot carries no
-// analysis weight; the adjoint of synthesis reads it), and so are the slots
-// below the order's first slot with non-negligible functions.
+// lane K.  Slots below the field's s0 are skipped (the DH pole slot carries
+// no analysis weight) unless T.s0 is 0 (the adjoint of synthesis reads every
+// slot), and so are the slots below the order's first slot with
+// non-negligible functions.
 template <int K>
 __global__ __launch_bounds__(kWave) void kb_anal_leg(const BField *__restrict__ fields,
                                                      const int2 *__restrict__ items, BTables T,
@@ -1535,8 +1536,9 @@ __global__ __launch_bounds__(kWave) void kb_anal_leg(const BField *__restrict__ 
     const int2 it = items[blockIdx.x];
     const BField f = fields[it.x];
     const int m = it.y;
+    const int s0 = min(T.s0, f.s0);
     anal_leg_tile<K>(T.alpha + tri_offd(m, T.Lt), T.seed[m], T.z + f.geo, gbuf + f.foff + (size_t)m * f.P,
-                     f.L - m, T.smin ? max(T.s0, T.smin[f.sm + m]) : T.s0, f.P, cbuf + f.coff + tri_offd(m, f.L));
+                     f.L - m, T.smin ? max(s0, T.smin[f.sm + m]) : s0, f.P, cbuf + f.coff + tri_offd(m, f.L));
 }
 
 // items: (field, first order, first degree) of a tile of the field's

@@ -53,6 +53,14 @@ inline int lds_per_cu(int m)
     return std::min(4, (int)(kMaxLds / ((size_t)fft_lds_slots(m) * kComplexBytes)));
 }
 
+inline int env_int(const char *name, int dflt)
+{
+    const char *v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    int x = std::atoi(v);
+    return x > 0 ? x : dflt;
+}
+
 // Whether a ring of length n with band limit lc runs the split Bluestein
 // plan (make_fft_plan): when its shorter work array brings the ring from the
 // global workspace into the LDS, for at most 15% more transform work (3
@@ -70,7 +78,8 @@ inline bool ring_split(int n, int lc)
     if (mh <= 0 || !fft_in_lds(mh)) return false;
     const int mf = bluestein_length(n + 2 * lc);
     if (3.0 * mh > 1.15 * 2.0 * mf) return false;
-    return lds_per_cu(mf) == 0;
+    static const int below = env_int("SHTG_SPLIT_BELOW", 1);
+    return lds_per_cu(mf) < below;
 }
 
 inline FftPlanHost ring_plan(int n, int lc) { return make_fft_plan(n, lc, lc >= 0 && ring_split(n, lc)); }
@@ -78,13 +87,6 @@ inline FftPlanHost ring_plan(int n, int lc) { return make_fft_plan(n, lc, lc >= 
 // Tuning of the ring-FFT launches: threads per workgroup by work length and
 // ring pairs per batched workgroup.  SHTG_FFT_MAXTHREADS (256/512/1024) and
 // SHTG_FFT_SLOTS override the defaults.
-inline int env_int(const char *name, int dflt)
-{
-    const char *v = std::getenv(name);
-    if (!v || !*v) return dflt;
-    int x = std::atoi(v);
-    return x > 0 ? x : dflt;
-}
 
 inline int fft_threads(int m)
 {
@@ -135,6 +137,10 @@ inline int anal_lanes_per_slot(int P, int kmax = kAnalKMax)
 
 // ------------------------------------------------------------ batched plans
 
+// Grids of batched fields: DH with sampling 1 or 2 (src/MakeGridDH.f95) or
+// Gauss-Legendre (src/MakeGridGLQ.f95, nodes of src/PreGLQ.f95).
+enum { kGridDH1 = 1, kGridDH2 = 2, kGridGLQ = 3 };
+
 // Per-field description of a batch (all fields: lmax_calc = lmax), as the
 // batched kernels read it.
 struct BField {
@@ -147,6 +153,7 @@ struct BField {
     long long coff;  // offset of the field in cpack/cbuf (double2 units)
     long long foff;  // offset of the field in fbuf/gbuf (double4 units)
     int sm;          // offset of the field's per-order first slots in the smin table
+    int s0;          // first slot with an analysis weight (DH: 1, the pole carries none; GLQ: 0)
 };
 
 // Work items as the kernels read them (int4 / int2 on the device).
@@ -157,7 +164,7 @@ struct Item2 {
     int x, y;
 };
 
-// Ring geometry of every distinct (degree, sampling) of a plan, the fields'
+// Ring geometry of every distinct (degree, grid) of a plan, the fields'
 // offsets into the plan's workspaces, and the coefficient-tile items.
 struct BatchLayout {
     std::vector<double> z, u, w;  // per slot: cos and sin of the northern ring, weights (north, south)
@@ -178,60 +185,108 @@ struct WorkItem {
 };
 inline bool by_work(const WorkItem &x, const WorkItem &y) { return x.w > y.w; }
 
-// The layout of a plan of nfields DH fields of degrees lmax[i] and samplings
-// sampling[i] (both already checked: lmax >= 0, sampling 1 or 2).  Ring
-// slot s = 0 .. n/2 of a field of n = 2L+2 rows pairs row s (theta = pi s /
-// n) with row n - s; slot 0 is the north pole with the south pole (row n,
-// present only with extend), slot n/2 the equator (unpaired)
-// (src/MakeGridDH.f95:504-510, src/SHExpandDH.f95:287-294,463-479).
-inline void build_batch_layout(int nfields, const int *lmax, const int *sampling, BatchLayout &out)
+// Rows, slots and ring length of a field of degree L on grid `grid`.
+struct GridShape {
+    int nlat, nlong, P;
+};
+inline GridShape grid_shape(int L, int grid)
+{
+    if (grid == kGridGLQ) return {L + 1, 2 * L + 1, (L + 2) / 2};
+    const int n = 2 * L + 2;
+    return {n, (grid == kGridDH1 ? 1 : 2) * n, n / 2 + 1};
+}
+
+// Appends the ring slots of one geometry (z, u, rows, weights) to out.
+//
+// DH (src/MakeGridDH.f95:504-510, src/SHExpandDH.f95:287-294,463-479): slot
+// s = 0 .. n/2 of n = 2L+2 rows pairs row s (theta = pi s / n) with row
+// n - s; slot 0 is the north pole with the south pole (row n, present only
+// with extend, so its weight is 0), slot n/2 the equator (unpaired).
+// Weights sqrt(2 pi) sqrt(4 pi) aj / nlong.
+//
+// GLQ (src/MakeGridGLQ.f95:467-480, src/SHExpandGLQ.f95:535-547): slot s
+// pairs row s (z = zero(s), the nodes of glq_nodes in descending order) with
+// row nlat-1-s; for odd nlat the middle row is an exact equator.  Weights
+// 2 pi w / nlong.  Returns false when the nodes do not converge.
+inline bool append_geometry(int L, int grid, BatchLayout &out)
 {
     const double pi = std::acos(-1.0);
+    const GridShape g = grid_shape(L, grid);
+    if (grid == kGridGLQ) {
+        std::vector<double> zero(g.nlat), wq(g.nlat);
+        if (glq_nodes(g.nlat, zero.data(), wq.data()) != 0) return false;
+        for (int s = 0; s < g.P; ++s) {
+            const bool eq = (g.nlat % 2 != 0) && (s == g.P - 1);
+            const double zz = eq ? 0.0 : zero[s];
+            out.z.push_back(zz);
+            out.u.push_back(eq ? 1.0 : std::sqrt((1.0 - zz) * (1.0 + zz)));
+            out.rows.push_back(s);
+            out.rows.push_back(eq ? -1 : g.nlat - 1 - s);
+            out.w.push_back(2.0 * pi * wq[s] / (double)g.nlong);
+            out.w.push_back(eq ? 0.0 : 2.0 * pi * wq[g.nlat - 1 - s] / (double)g.nlong);
+        }
+        return true;
+    }
+    const int n = g.nlat;
+    std::vector<double> aj;
+    dh_weights(n, aj);
+    const double c = std::sqrt(2 * pi) * std::sqrt(4.0 * pi) / (double)g.nlong;
+    for (int s = 0; s < g.P; ++s) {
+        const double zz = (s == n / 2) ? 0.0 : std::cos(pi * (double)s / (double)n);
+        out.z.push_back(zz);
+        out.u.push_back((s == n / 2) ? 1.0 : std::sqrt((1.0 - zz) * (1.0 + zz)));
+        out.rows.push_back(s);
+        out.rows.push_back((s == n / 2) ? -1 : n - s);
+        out.w.push_back(c * aj[s]);
+        out.w.push_back((s == 0 || s == n / 2) ? 0.0 : c * aj[n - s]);
+    }
+    return true;
+}
+
+// The layout of a plan of nfields fields of degrees lmax[i] on grids
+// grid[i] (kGridDH1, kGridDH2 or kGridGLQ; both already checked).  Every
+// distinct (degree, grid) gets one geometry and one ring-FFT plan, built as
+// the plain transforms build theirs (pruned for DH rings, full for GLQ
+// rings), so that batched and plain results agree.  Returns false when a
+// GLQ geometry cannot be built.
+inline bool build_batch_layout(int nfields, const int *lmax, const int *grid, BatchLayout &out)
+{
     typedef std::pair<int, int> Key;
     std::map<Key, std::pair<int, int>> geo_of;  // -> (geo offset, plan index)
     std::map<Key, int> sm_of;                   // -> offset of its smin row
     for (int i = 0; i < nfields; ++i) {
         const int L = lmax[i];
-        const Key key(L, sampling[i]);
+        const Key key(L, grid[i]);
         if (geo_of.count(key)) continue;
-        const int n = 2 * L + 2, nlong = sampling[i] * n, P = n / 2 + 1;
         const int off = (int)out.z.size();
-        std::vector<double> aj;
-        dh_weights(n, aj);
-        const double c = std::sqrt(2 * pi) * std::sqrt(4.0 * pi) / (double)nlong;
-        for (int s = 0; s < P; ++s) {
-            const double zz = (s == n / 2) ? 0.0 : std::cos(pi * (double)s / (double)n);
-            out.z.push_back(zz);
-            out.u.push_back((s == n / 2) ? 1.0 : std::sqrt((1.0 - zz) * (1.0 + zz)));
-            out.rows.push_back(s);
-            out.rows.push_back((s == n / 2) ? -1 : n - s);
-            out.w.push_back(c * aj[s]);
-            out.w.push_back((s == 0 || s == n / 2) ? 0.0 : c * aj[n - s]);
-        }
+        if (!append_geometry(L, grid[i], out)) return false;
         const std::vector<double> zl(out.z.begin() + off, out.z.end()), ul(out.u.begin() + off, out.u.end());
         const std::vector<int> km = polar_kmax(zl, ul, L);
         out.kmax.insert(out.kmax.end(), km.begin(), km.end());
         const std::vector<int> sm = polar_smin(km, L);
         sm_of[key] = (int)out.smin.size();
         out.smin.insert(out.smin.end(), sm.begin(), sm.end());
-        out.plans.push_back(ring_plan(nlong, L));
+        const GridShape g = grid_shape(L, grid[i]);
+        out.plans.push_back(grid[i] == kGridGLQ ? make_fft_plan(g.nlong, -1) : ring_plan(g.nlong, L));
         geo_of[key] = {off, (int)out.plans.size() - 1};
     }
     out.fields.assign(nfields, BField{});
     long long coff = 0, foff = 0;
     for (int i = 0; i < nfields; ++i) {
-        const int L = lmax[i], n = 2 * L + 2;
-        const Key key(L, sampling[i]);
+        const int L = lmax[i];
+        const Key key(L, grid[i]);
+        const GridShape g = grid_shape(L, grid[i]);
         BField &f = out.fields[i];
         f.L = L;
-        f.P = n / 2 + 1;
-        f.nlat = n;
-        f.nlong = sampling[i] * n;
+        f.P = g.P;
+        f.nlat = g.nlat;
+        f.nlong = g.nlong;
         f.geo = geo_of[key].first;
         f.fft = geo_of[key].second;
         f.coff = coff;
         f.foff = foff;
         f.sm = sm_of[key];
+        f.s0 = grid[i] == kGridGLQ ? 0 : 1;
         coff += tri_size(L);
         foff += (long long)(L + 1) * f.P;
     }
@@ -251,6 +306,7 @@ inline void build_batch_layout(int nfields, const int *lmax, const int *sampling
     out.unpack.clear();
     for (auto &x : wp) out.pack.push_back({x.a, x.b, x.c, 0});
     for (auto &x : wu) out.unpack.push_back({x.a, x.b, x.c, 0});
+    return true;
 }
 
 // Synthesis Legendre work items, longest first: buckets of at least
@@ -313,8 +369,8 @@ inline SynthItems build_synth_items(const std::vector<BField> &fields, const std
 
 // Analysis Legendre work items, longest first: one single-wave item (field,
 // m) per order, grouped by ring pairs per lane (at most anal_kmax) into one
-// launch each (out[K - 1]).  The DH pole slot carries no analysis weight, so
-// the sums start at slot max(1, smin).
+// launch each (out[K - 1]).  The sums start at slot max(s0, smin): the DH
+// pole slot carries no analysis weight.
 inline void build_anal_items(const std::vector<BField> &fields, const std::vector<int> &smin, int anal_kmax,
                              std::vector<Item2> (&out)[kAnalKMax])
 {
@@ -323,7 +379,7 @@ inline void build_anal_items(const std::vector<BField> &fields, const std::vecto
         const BField &f = fields[i];
         const int *sm = smin.data() + f.sm;
         for (int m = 0; m <= f.L; ++m) {
-            const int sa = std::max(1, sm[m]);
+            const int sa = std::max(f.s0, sm[m]);
             wa.push_back({(double)(f.L - m + 1) * (f.P - sa), i, m, anal_lanes_per_slot(f.P - sa, anal_kmax)});
         }
     }

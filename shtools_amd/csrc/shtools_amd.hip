@@ -922,14 +922,14 @@ static int batch_anal_items(BatchPlan *b)
 }
 
 // sampling: per field (1 or 2).
-static int batch_create_dh(int nfields, const int *lmax, const int *sampling, int norm, int csphase,
-                           BatchPlan **out)
+static int batch_create(int nfields, const int *lmax, const int *grid, int norm, int csphase, BatchPlan **out)
 {
-    if (nfields <= 0 || !lmax || !sampling || !out) return ST_DIMS;
+    if (nfields <= 0 || !lmax || !grid || !out) return ST_DIMS;
     if (norm < 1 || norm > 4 || (csphase != 1 && csphase != -1)) return ST_BOUNDS;
     int Lmax = 0;
     for (int i = 0; i < nfields; ++i) {
-        if (lmax[i] < 0 || lmax[i] > kMaxArgDegree || (sampling[i] != 1 && sampling[i] != 2)) return ST_BOUNDS;
+        if (lmax[i] < 0 || lmax[i] > kMaxArgDegree) return ST_BOUNDS;
+        if (grid[i] != kGridDH1 && grid[i] != kGridDH2 && grid[i] != kGridGLQ) return ST_BOUNDS;
         Lmax = std::max(Lmax, lmax[i]);
     }
     const int dev = current_device();
@@ -948,7 +948,7 @@ static int batch_create_dh(int nfields, const int *lmax, const int *sampling, in
 
     // ring geometry, FFT plans, field offsets and work items (layout.hpp)
     BatchLayout lay;
-    build_batch_layout(nfields, lmax, sampling, lay);
+    if (!build_batch_layout(nfields, lmax, grid, lay)) return ST_OTHER;
     for (const FftPlanHost &fp : lay.plans) {
         FftDev fd;
         if (!upload_fft(fp, fd)) return ST_OTHER;
@@ -1965,9 +1965,30 @@ int shtg_batch_create_dh(int nfields, const int *lmax, int sampling, int norm, i
 int shtg_batch_create_dh_fields(int nfields, const int *lmax, const int *sampling, int norm, int csphase,
                                 shtg_batch_plan **plan)
 {
+    if (plan) *plan = nullptr;
+    if (nfields <= 0 || !sampling) return ST_DIMS;
+    for (int i = 0; i < nfields; ++i)
+        if (sampling[i] != 1 && sampling[i] != 2) return ST_BOUNDS;
+    // sampling 1 and 2 are the grid codes kGridDH1 and kGridDH2
+    return shtg_batch_create_fields(nfields, lmax, sampling, norm, csphase, plan);
+}
+
+int shtg_batch_create_glq(int nfields, const int *lmax, int norm, int csphase, shtg_batch_plan **plan)
+{
+    if (nfields <= 0) {
+        if (plan) *plan = nullptr;
+        return ST_DIMS;
+    }
+    const std::vector<int> grid((size_t)nfields, (int)kGridGLQ);
+    return shtg_batch_create_fields(nfields, lmax, grid.data(), norm, csphase, plan);
+}
+
+int shtg_batch_create_fields(int nfields, const int *lmax, const int *grid, int norm, int csphase,
+                             shtg_batch_plan **plan)
+{
     std::lock_guard<std::mutex> lk(g_mu);
     BatchPlan *b = nullptr;
-    int st = batch_create_dh(nfields, lmax, sampling, norm, csphase, &b);
+    int st = batch_create(nfields, lmax, grid, norm, csphase, &b);
     if (plan) *plan = reinterpret_cast<shtg_batch_plan *>(b);
     return st;
 }

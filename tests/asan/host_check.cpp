@@ -222,11 +222,12 @@ static void check_plans()
 
 // ------------------------------------------------------------ batched layouts
 
+// samp: grid codes per field (kGridDH1, kGridDH2, kGridGLQ)
 static void check_layout_case(const std::vector<int> &lmax, const std::vector<int> &samp, const char *name)
 {
     const int nf = (int)lmax.size();
     BatchLayout lay;
-    build_batch_layout(nf, lmax.data(), samp.data(), lay);
+    CHECK(build_batch_layout(nf, lmax.data(), samp.data(), lay), "%s: layout", name);
     const size_t nslots = lay.z.size();
     CHECK(lay.u.size() == nslots && lay.w.size() == 2 * nslots && lay.rows.size() == 2 * nslots &&
               lay.kmax.size() == nslots,
@@ -235,9 +236,11 @@ static void check_layout_case(const std::vector<int> &lmax, const std::vector<in
     long long coff = 0, foff = 0;
     for (int i = 0; i < nf; ++i) {
         const BField &f = lay.fields[i];
-        const int n = 2 * lmax[i] + 2;
-        CHECK(f.L == lmax[i] && f.P == n / 2 + 1 && f.nlat == n && f.nlong == samp[i] * n, "%s: field %d", name,
-              i);
+        const bool glq = samp[i] == kGridGLQ;
+        const int n = glq ? lmax[i] + 1 : 2 * lmax[i] + 2;  // grid rows
+        CHECK(f.L == lmax[i] && f.P == (glq ? (n + 1) / 2 : n / 2 + 1) && f.nlat == n &&
+                  f.nlong == (glq ? 2 * lmax[i] + 1 : samp[i] * n) && f.s0 == (glq ? 0 : 1),
+              "%s: field %d", name, i);
         CHECK(f.geo >= 0 && (size_t)f.geo + f.P <= nslots, "%s: field %d geo", name, i);
         CHECK(f.sm >= 0 && (size_t)f.sm + f.L + 1 <= lay.smin.size(), "%s: field %d smin", name, i);
         CHECK(f.fft >= 0 && f.fft < (int)lay.plans.size() && lay.plans[f.fft].n == f.nlong, "%s: field %d plan",
@@ -247,7 +250,14 @@ static void check_layout_case(const std::vector<int> &lmax, const std::vector<in
         foff += (long long)(f.L + 1) * f.P;
         for (int s = 0; s < f.P; ++s) {
             const int north = lay.rows[2 * (size_t)(f.geo + s)], south = lay.rows[2 * (size_t)(f.geo + s) + 1];
-            CHECK(north == s && (s == n / 2 ? south == -1 : south == n - s), "%s: field %d rows s %d", name, i, s);
+            // DH: row s with n - s, the equator n/2 alone; GLQ: row s with
+            // nlat-1-s, the middle row of odd nlat alone
+            const bool alone = glq ? (n % 2 == 1 && s == f.P - 1) : (s == n / 2);
+            CHECK(north == s && (alone ? south == -1 : south == (glq ? n - 1 - s : n - s)),
+                  "%s: field %d rows s %d", name, i, s);
+            CHECK(!glq || (lay.w[2 * (size_t)(f.geo + s)] > 0.0 && lay.z[f.geo + s] >= 0.0 &&
+                           (s == 0 || lay.z[f.geo + s] < lay.z[f.geo + s - 1])),
+                  "%s: GLQ slot %d", name, s);
             CHECK(std::isfinite(lay.w[2 * (size_t)(f.geo + s)]) && std::isfinite(lay.w[2 * (size_t)(f.geo + s) + 1]),
                   "%s: weights", name);
         }
@@ -439,6 +449,8 @@ static void check_layouts()
     run({0, 1, 2, 3, 4, 5}, {2, 1, 2, 1, 2, 1}, "smallest degrees");
     run(std::vector<int>(40, 100), std::vector<int>(40, 2), "one degree, 40 fields (buckets)");
     run({1023, 1290, 1100}, {2, 2, 1}, "largest rings (workspace and split)");
+    run({0, 1, 2, 3, 4, 5, 6}, std::vector<int>(7, kGridGLQ), "smallest GLQ degrees (odd and even nlat)");
+    run({1023, 2047, 100, 100, 101}, {kGridGLQ, kGridGLQ, kGridGLQ, kGridDH2, kGridGLQ}, "GLQ with DH");
     // mixed workloads like the batched benchmark: degrees across the LDS classes
     for (int seed = 1; seed <= 4; ++seed) {
         Lcg r(seed);
@@ -447,7 +459,8 @@ static void check_layouts()
         for (int i = 0; i < nf; ++i) {
             L[i] = (r.next() % 4 == 0) ? r.uniform(0, 40) : r.uniform(41, 1100);
             if (r.next() % 5 == 0 && i > 0) L[i] = L[i - 1];  // repeated degrees form buckets
-            S[i] = (r.next() % 3 == 0) ? 1 : 2;
+            const uint32_t k = r.next() % 4;
+            S[i] = k == 0 ? kGridDH1 : k == 1 ? kGridGLQ : kGridDH2;
         }
         run(L, S, "random mix seed " + std::to_string(seed));
     }

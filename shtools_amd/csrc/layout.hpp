@@ -71,7 +71,11 @@ inline int env_int(const char *name, int dflt)
 // about 2/3 as long but has as many stages, and its second inverse pass adds
 // more LDS traffic than the extra workgroups per CU recover (measured on the
 // 512-field benchmark: synthesis ring FFTs 13.9 -> 17.1 ms with split plans
-// wherever they raised the workgroups per CU, analysis 13.2 -> 13.4 ms).
+// wherever they raised the workgroups per CU, analysis 13.2 -> 13.4 ms; for
+// the one-workgroup-per-CU class alone, 14.0 -> 14.0 and 13.1 -> 12.6 ms).
+// SHTG_SPLIT_BELOW = k splits every ring whose whole plan fits fewer than k
+// workgroups per CU (default 1: only rings beyond the LDS); it changes
+// results by rounding.
 inline bool ring_split(int n, int lc)
 {
     const int mh = split_length(n, lc);

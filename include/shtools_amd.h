@@ -23,7 +23,10 @@
  *   - norm 1 = 4pi (geodesy), 2 = Schmidt, 3 = unnormalised, 4 = orthonormal;
  *     csphase 1 omits and -1 applies the Condon-Shortley phase (-1)^m;
  *   - calls are thread-safe; host-side tables are cached per (norm, csphase,
- *     degree) and per grid.
+ *     degree) and per grid.  A lock is held only while a call looks its
+ *     tables up (or builds them); host threads calling on different streams
+ *     enqueue their work concurrently.  Calls on one batched plan take the
+ *     plan's own lock.
  */
 #ifndef SHTOOLS_AMD_H
 #define SHTOOLS_AMD_H
@@ -187,8 +190,10 @@ int shtg_shexpand_glq_adjoint(const double *cilm_bar, int cilm_d1, int cilm_d2, 
  * first.  cilm[i] points to a device [2][lmax[i]+1][lmax[i]+1] array and
  * grid[i] to a device [2lmax[i]+2][sampling(2lmax[i]+2)] array; the arrays of
  * pointers are host arrays (uploaded when they change).  Calls on one plan
- * must not overlap in time on different streams (they share its
- * workspace). */
+ * are serialised on the host by the plan's lock, but their work must not
+ * overlap in time on different streams (they share its workspace); plans
+ * are independent of each other.  shtg_batch_destroy must not run while
+ * another call on the plan does. */
 typedef struct shtg_batch_plan shtg_batch_plan;
 int shtg_batch_create_dh(int nfields, const int *lmax, int sampling, int norm, int csphase,
                          shtg_batch_plan **plan);

@@ -12,6 +12,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <atomic>
 #include <mutex>
 #include <tuple>
 #include <vector>
@@ -28,7 +29,17 @@ namespace shtg {
 constexpr int kChunkCap = (int)(kMaxLds / sizeof(double2));
 static_assert(sizeof(Item4) == sizeof(int4) && sizeof(Item2) == sizeof(int2), "work items as int4 / int2");
 
+// Locking: g_mu guards the table caches (recursion, seeds, geometries,
+// gravity rows, Gauss-Legendre nodes) while a call looks its tables up or
+// builds them; the launches run outside it, on copies of the table
+// descriptors (a table that grows is replaced, and the outgrown arrays stay
+// allocated).  g_ws_mu guards the per-device intermediates of the plain
+// launchers, g_prof_mu the profiling records, and each batched plan has its
+// own mutex for its bindings and workspace.  Host threads calling on
+// different streams therefore issue their launches concurrently.
 static std::mutex g_mu;
+static std::mutex g_ws_mu;
+static std::mutex g_prof_mu;
 
 #define HIP_OK(x) (hipSuccess == (x))
 
@@ -338,8 +349,9 @@ static bool capturing(hipStream_t st);
 static int current_device();
 
 // Intermediates of the plain launchers: one buffer per device, reused by
-// consecutive calls (the callers hold g_mu, so calls do not interleave on
-// the host).  Every call first makes its stream wait, on the device, for
+// consecutive calls; a call that finds it held by another host thread's
+// call takes its own block from the pool.  Every call first makes its
+// stream wait, on the device, for
 // the previous user's work (an event recorded at release; on the same
 // stream the wait is already met, and a stream handle reused after a
 // destroy is covered too).  A larger request replaces the buffer by one of the
@@ -352,13 +364,15 @@ struct WsCache {
     void *buf = nullptr;
     size_t size = 0;
     hipEvent_t ev = nullptr;
-    bool used = false;
+    bool used = false;    // ev marks the end of the last user's work
+    bool leased = false;  // held by a call between ws_acquire and ws_release
 };
-static std::map<int, WsCache> g_ws;
+static std::map<int, WsCache> g_ws;  // under g_ws_mu
 
 struct WsLease {
     void *p = nullptr;
     bool pooled = false;
+    int dev = -1;
 };
 
 static int ws_acquire(WsLease &l, size_t bytes, hipStream_t st)
@@ -369,7 +383,14 @@ static int ws_acquire(WsLease &l, size_t bytes, hipStream_t st)
     }
     const int dev = current_device();
     if (dev < 0) return ST_OTHER;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
     WsCache &c = g_ws[dev];
+    if (c.leased) {
+        // another host thread's call holds the buffer: this one takes a
+        // block of its own from the pool
+        l.pooled = true;
+        return HIP_OK(pool_alloc(&l.p, bytes, st)) ? ST_OK : ST_ALLOC;
+    }
     if (!c.ev && !HIP_OK(hipEventCreateWithFlags(&c.ev, hipEventDisableTiming))) return ST_OTHER;
     if (c.used && !HIP_OK(hipStreamWaitEvent(st, c.ev, 0))) return ST_OTHER;
     if (bytes > c.size) {
@@ -381,8 +402,10 @@ static int ws_acquire(WsLease &l, size_t bytes, hipStream_t st)
         c.buf = nb;
         c.size = b;
     }
+    c.leased = true;
     l.p = c.buf;
     l.pooled = false;
+    l.dev = dev;
     return ST_OK;
 }
 
@@ -392,9 +415,11 @@ static void ws_release(const WsLease &l, hipStream_t st)
         (void)hipFreeAsync(l.p, st);
         return;
     }
-    WsCache &c = g_ws[current_device()];
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    WsCache &c = g_ws[l.dev];
     (void)hipEventRecord(c.ev, st);
     c.used = true;
+    c.leased = false;
 }
 
 static int current_device()
@@ -484,7 +509,7 @@ static void zero_outside(double *a, int rows, int cols, long long ld, int r0, in
 
 // When enabled, every kernel launch is bracketed by HIP events recorded on
 // the launch stream; shtg_profile_collect() reduces them per kernel id.
-static bool g_prof = false;
+static std::atomic<bool> g_prof{false};
 struct ProfRec {
     int kid;
     hipEvent_t a, b;
@@ -505,6 +530,7 @@ struct ProfScope {
     {
         if (!g_prof || !a) return;
         (void)hipEventRecord(b, st);
+        std::lock_guard<std::mutex> lk(g_prof_mu);
         g_prof_recs.push_back({kid, a, b});
         a = b = nullptr;
     }
@@ -769,6 +795,7 @@ struct FftGroup {
 };
 
 struct BatchPlan {
+    std::mutex mu;  // calls on the plan (bindings, workspace, tuning)
     int dev = -1, nf = 0, norm = 1, csphase = 1, Lmax = 0;
     std::vector<int> L;
     BField *fields = nullptr;
@@ -1280,11 +1307,12 @@ static int batch_tune(BatchPlan *b, hipStream_t st, int reps, int flags)
     return batch_tune_legendre(b, st, reps, (flags & 1) != 0);
 }
 
-// Device tables of one call, under g_mu.
+// Device tables of one call: the geometry (never changed once built) and
+// copies of the recursion and seed descriptors, taken under g_mu.
 struct CallTables {
     const Geometry *g = nullptr;
-    const DevRecurrence *rt = nullptr;
-    const DevSeeds *sd = nullptr;
+    DevRecurrence rt;
+    DevSeeds sd;
 };
 
 // Whether a call's tables exist already (nothing to build or upload).
@@ -1312,13 +1340,16 @@ static int get_tables(bool dh, int lmax, int sampling, const double *zero, const
 {
     const int dev = current_device();
     if (dev < 0) return ST_OTHER;
+    std::lock_guard<std::mutex> lk(g_mu);
     if (capturing(st) && !tables_ready(dev, dh, lmax, sampling, zero, wq, norm, csphase, Lc, cplx))
         return ST_CAPTURE;
     configure_kernels();
     t.g = dh ? get_dh(dev, lmax, sampling) : get_glq(dev, lmax, zero, wq);
-    t.rt = get_recurrence(dev, norm, Lc);
-    t.sd = get_seeds(dev, norm, csphase, Lc, cplx);
-    if (!t.g || !t.rt || !t.sd) return ST_ALLOC;
+    const DevRecurrence *rt = get_recurrence(dev, norm, Lc);
+    const DevSeeds *sd = get_seeds(dev, norm, csphase, Lc, cplx);
+    if (!t.g || !rt || !sd) return ST_ALLOC;
+    t.rt = *rt;
+    t.sd = *sd;
     return ST_OK;
 }
 
@@ -1328,7 +1359,7 @@ static int get_tables(bool dh, int lmax, int sampling, const double *zero, const
 static int adjoint_of_synthesis(const CallTables &t, const Shape &sh, const double *grid_bar, long long ld,
                                 double *cilm_bar, int d1, int d2, hipStream_t st)
 {
-    return launch_analysis(*t.g, *t.rt, *t.sd, grid_bar, ld, sh.Lc, cilm_bar, d1, d2, st, t.g->ones, 0, t.rt->s,
+    return launch_analysis(*t.g, t.rt, t.sd, grid_bar, ld, sh.Lc, cilm_bar, d1, d2, st, t.g->ones, 0, t.rt.s,
                            sh.nrows, sh.ext);
 }
 
@@ -1339,7 +1370,7 @@ static int adjoint_of_analysis(const CallTables &t, const Shape &sh, const doubl
 {
     const Geometry &g = *t.g;
     zero_outside(grid_bar, gd0, gd1, gd1, g.nlat, g.nlong, st);
-    return launch_synthesis(g, *t.rt, *t.sd, cilm_bar, d1, d2, sh.Lc, g.nlat, 0, grid_bar, gd1, st, t.rt->post,
+    return launch_synthesis(g, t.rt, t.sd, cilm_bar, d1, d2, sh.Lc, g.nlat, 0, grid_bar, gd1, st, t.rt.post,
                             g.w);
 }
 
@@ -1357,11 +1388,10 @@ int shtg_makegrid_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax, int
                                sh);
     if (st) return st;
     if (!cilm || !grid) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
     if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t,
                          (hipStream_t)stream))) return st;
-    return launch_synthesis(*t.g, *t.rt, *t.sd, cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend, grid, grid_d1,
+    return launch_synthesis(*t.g, t.rt, t.sd, cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend, grid, grid_d1,
                             (hipStream_t)stream);
 }
 
@@ -1406,18 +1436,26 @@ int shtg_makegrid_dh_noalloc(const double *cilm, int cilm_d1, int cilm_d2, int l
                                sh);
     if (st) return st;
     if (!cilm || !grid) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     int dev = 0;
     if (!HIP_OK(hipGetDevice(&dev))) return ST_OTHER;
-    // only tables an ordinary call has built already (building them would
-    // allocate)
-    if (!tables_ready(dev, true, lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, false)) return ST_NOTABLES;
-    void *ws = scratch_address(dev);
-    if (!ws) return ST_OTHER;
-    const Geometry &g = *g_dh.find({dev, lmax, sampling})->second;
+    const Geometry *gp;
+    DevRecurrence rt;
+    DevSeeds sd;
+    void *ws;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        // only tables an ordinary call has built already (building them would
+        // allocate)
+        if (!tables_ready(dev, true, lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, false))
+            return ST_NOTABLES;
+        ws = scratch_address(dev);
+        if (!ws) return ST_OTHER;
+        gp = g_dh.find({dev, lmax, sampling})->second.get();
+        rt = g_rec.find({dev, norm})->second;
+        sd = g_seeds.find({dev, norm, csphase})->second;
+    }
+    const Geometry &g = *gp;
     if (synthesis_bytes(g, sh.Lc) > kScratchBytes) return ST_SCRATCH;
-    const DevRecurrence &rt = g_rec.find({dev, norm})->second;
-    const DevSeeds &sd = g_seeds.find({dev, norm, csphase})->second;
     ExtWorkspace ext;
     ext.p = ws;
     ext.cap = kScratchBytes;
@@ -1432,11 +1470,10 @@ int shtg_shexpand_dh(const double *grid, int grid_d0, int grid_d1, int norm, int
     int st = check_shexpand_dh(grid_d0, grid_d1, norm, sampling, csphase, lmax_calc, cilm_d1, cilm_d2, sh);
     if (st) return st;
     if (!cilm || !grid) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
     if ((st = get_tables(true, sh.lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t,
                          (hipStream_t)stream))) return st;
-    return launch_analysis(*t.g, *t.rt, *t.sd, grid, grid_d1, sh.Lc, cilm, cilm_d1, cilm_d2, (hipStream_t)stream);
+    return launch_analysis(*t.g, t.rt, t.sd, grid, grid_d1, sh.Lc, cilm, cilm_d1, cilm_d2, (hipStream_t)stream);
 }
 
 int shtg_makegrid_glq(const double *cilm, int cilm_d1, int cilm_d2, int lmax, const double *zero, int nzero,
@@ -1448,10 +1485,9 @@ int shtg_makegrid_glq(const double *cilm, int cilm_d1, int cilm_d2, int lmax, co
                                 grid_d1, sh);
     if (st) return st;
     if (!cilm || !grid) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
     if ((st = get_tables(false, lmax, 0, zero, nullptr, norm, csphase, sh.Lc, t, (hipStream_t)stream))) return st;
-    return launch_synthesis(*t.g, *t.rt, *t.sd, cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend, grid, grid_d1,
+    return launch_synthesis(*t.g, t.rt, t.sd, cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend, grid, grid_d1,
                             (hipStream_t)stream);
 }
 
@@ -1464,10 +1500,9 @@ int shtg_shexpand_glq(const double *grid, int grid_d0, int grid_d1, const double
                                 sh);
     if (st) return st;
     if (!cilm || !grid) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
     if ((st = get_tables(false, sh.lmax, 0, zero, w, norm, csphase, sh.Lc, t, (hipStream_t)stream))) return st;
-    return launch_analysis(*t.g, *t.rt, *t.sd, grid, grid_d1, sh.Lc, cilm, cilm_d1, cilm_d2, (hipStream_t)stream);
+    return launch_analysis(*t.g, t.rt, t.sd, grid, grid_d1, sh.Lc, cilm, cilm_d1, cilm_d2, (hipStream_t)stream);
 }
 
 int shtg_makegrid_dh_adjoint(const double *grid_bar, int grid_d0, int grid_d1, int lmax, int norm, int sampling,
@@ -1479,7 +1514,6 @@ int shtg_makegrid_dh_adjoint(const double *grid_bar, int grid_d0, int grid_d1, i
                                sh);
     if (st) return st;
     if (!cilm_bar || !grid_bar) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
     if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t,
                          (hipStream_t)stream))) return st;
@@ -1493,7 +1527,6 @@ int shtg_shexpand_dh_adjoint(const double *cilm_bar, int cilm_d1, int cilm_d2, i
     int st = check_shexpand_dh(grid_d0, grid_d1, norm, sampling, csphase, lmax_calc, cilm_d1, cilm_d2, sh);
     if (st) return st;
     if (!cilm_bar || !grid_bar) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
     if ((st = get_tables(true, sh.lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t,
                          (hipStream_t)stream))) return st;
@@ -1509,7 +1542,6 @@ int shtg_makegrid_glq_adjoint(const double *grid_bar, int grid_d0, int grid_d1, 
                                 grid_d1, sh);
     if (st) return st;
     if (!cilm_bar || !grid_bar) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
     if ((st = get_tables(false, lmax, 0, zero, nullptr, norm, csphase, sh.Lc, t, (hipStream_t)stream))) return st;
     return adjoint_of_synthesis(t, sh, grid_bar, grid_d1, cilm_bar, cilm_d1, cilm_d2, (hipStream_t)stream);
@@ -1524,7 +1556,6 @@ int shtg_shexpand_glq_adjoint(const double *cilm_bar, int cilm_d1, int cilm_d2, 
                                 sh);
     if (st) return st;
     if (!cilm_bar || !grid_bar) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
     if ((st = get_tables(false, sh.lmax, 0, zero, w, norm, csphase, sh.Lc, t, (hipStream_t)stream))) return st;
     return adjoint_of_analysis(t, sh, cilm_bar, cilm_d1, cilm_d2, grid_bar, grid_d0, grid_d1, (hipStream_t)stream);
@@ -1538,11 +1569,10 @@ int shtg_makegrid_dhc(const double *cilm, int cilm_d1, int cilm_d2, int lmax, in
                                sh);
     if (st) return st;
     if (!cilm || !grid) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
     if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t,
                          (hipStream_t)stream, true))) return st;
-    return launch_synthesis_c(*t.g, *t.rt, *t.sd, (const double2 *)cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend,
+    return launch_synthesis_c(*t.g, t.rt, t.sd, (const double2 *)cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend,
                               (double2 *)grid, grid_d1, (hipStream_t)stream);
 }
 
@@ -1553,11 +1583,10 @@ int shtg_shexpand_dhc(const double *grid, int grid_d0, int grid_d1, int norm, in
     int st = check_shexpand_dh(grid_d0, grid_d1, norm, sampling, csphase, lmax_calc, cilm_d1, cilm_d2, sh);
     if (st) return st;
     if (!cilm || !grid) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
     if ((st = get_tables(true, sh.lmax, sampling, nullptr, nullptr, norm, csphase, sh.Lc, t,
                          (hipStream_t)stream, true))) return st;
-    return launch_analysis_c(*t.g, *t.rt, *t.sd, (const double2 *)grid, grid_d1, sh.Lc, (double2 *)cilm, cilm_d1,
+    return launch_analysis_c(*t.g, t.rt, t.sd, (const double2 *)grid, grid_d1, sh.Lc, (double2 *)cilm, cilm_d1,
                              cilm_d2, (hipStream_t)stream);
 }
 
@@ -1570,10 +1599,9 @@ int shtg_makegrid_glqc(const double *cilm, int cilm_d1, int cilm_d2, int lmax, c
                                 grid_d1, sh);
     if (st) return st;
     if (!cilm || !grid) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
     if ((st = get_tables(false, lmax, 0, zero, nullptr, norm, csphase, sh.Lc, t, (hipStream_t)stream, true))) return st;
-    return launch_synthesis_c(*t.g, *t.rt, *t.sd, (const double2 *)cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend,
+    return launch_synthesis_c(*t.g, t.rt, t.sd, (const double2 *)cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend,
                               (double2 *)grid, grid_d1, (hipStream_t)stream);
 }
 
@@ -1586,10 +1614,9 @@ int shtg_shexpand_glqc(const double *grid, int grid_d0, int grid_d1, const doubl
                                 sh);
     if (st) return st;
     if (!cilm || !grid) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
     if ((st = get_tables(false, sh.lmax, 0, zero, w, norm, csphase, sh.Lc, t, (hipStream_t)stream, true))) return st;
-    return launch_analysis_c(*t.g, *t.rt, *t.sd, (const double2 *)grid, grid_d1, sh.Lc, (double2 *)cilm, cilm_d1,
+    return launch_analysis_c(*t.g, t.rt, t.sd, (const double2 *)grid, grid_d1, sh.Lc, (double2 *)cilm, cilm_d1,
                              cilm_d2, (hipStream_t)stream);
 }
 
@@ -1602,7 +1629,6 @@ int shtg_makegradient_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax,
     int st = check_makegrid_dh(cilm_d1, cilm_d2, lmax, 1, sampling, 1, lmax_calc, extend, grid_d0, grid_d1, sh);
     if (st) return st;
     if (!cilm || !theta || !phi) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
     if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, 1, 1, sh.Lc, t, (hipStream_t)stream))) return st;
     hipStream_t hs = (hipStream_t)stream;
@@ -1614,10 +1640,10 @@ int shtg_makegradient_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax,
     double *coefs = (double *)ws, *gb = coefs + 6 * plane;
     hipLaunchKernelGGL(k_gradient_coefs, dim3(grid_blocks((long long)plane)), dim3(kThreads), 0, hs, cilm, cilm_d1,
                        cilm_d2, sh.Lc, coefs);
-    st = launch_synthesis(*t.g, *t.rt, *t.sd, coefs, L1, L1, sh.Lc, sh.nrows, extend, theta, grid_d1, hs);
-    if (!st) st = launch_synthesis(*t.g, *t.rt, *t.sd, coefs + 2 * plane, L1, L1, sh.Lc, sh.nrows, extend, gb,
+    st = launch_synthesis(*t.g, t.rt, t.sd, coefs, L1, L1, sh.Lc, sh.nrows, extend, theta, grid_d1, hs);
+    if (!st) st = launch_synthesis(*t.g, t.rt, t.sd, coefs + 2 * plane, L1, L1, sh.Lc, sh.nrows, extend, gb,
                                    grid_d1, hs);
-    if (!st) st = launch_synthesis(*t.g, *t.rt, *t.sd, coefs + 4 * plane, L1, L1, sh.Lc, sh.nrows, extend, phi,
+    if (!st) st = launch_synthesis(*t.g, t.rt, t.sd, coefs + 4 * plane, L1, L1, sh.Lc, sh.nrows, extend, phi,
                                    grid_d1, hs);
     if (!st) {
         const int ncols = sampling * n + extend;
@@ -1741,11 +1767,14 @@ int shtg_makegravgrid_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax,
     if (st) return st;
     if (normal_gravity_flag != 0 && normal_gravity_flag != 1) return ST_BOUNDS;
     if (!cilm || !rad || !theta || !phi || !total || !pot) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     CallTables t;
     if ((st = get_tables(true, lmax, sampling, nullptr, nullptr, 1, 1, sh.Lc, t, (hipStream_t)stream))) return st;
-    const GravTables *gt = get_grav(current_device(), lmax, sampling, extend, r0, a, f, omega, normal_gravity_flag,
-                                    normal_gravity_gm);
+    const GravTables *gt;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        gt = get_grav(current_device(), lmax, sampling, extend, r0, a, f, omega, normal_gravity_flag,
+                      normal_gravity_gm);
+    }
     if (!gt) return ST_ALLOC;
     hipStream_t hs = (hipStream_t)stream;
     const int L1 = sh.Lc + 1, n = 2 * lmax + 2;
@@ -1760,13 +1789,13 @@ int shtg_makegravgrid_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax,
     ring.u = gt->u;
     ring.r2 = gt->r2;
     const Geometry &g = *t.g;
-    st = launch_synthesis(g, *t.rt, *t.sd, cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend, pot, grid_d1, hs,
+    st = launch_synthesis(g, t.rt, t.sd, cilm, cilm_d1, cilm_d2, sh.Lc, sh.nrows, extend, pot, grid_d1, hs,
                           nullptr, nullptr, ring);
-    if (!st) st = launch_synthesis(g, *t.rt, *t.sd, coefs + 2 * plane, L1, L1, sh.Lc, sh.nrows, extend, rad, grid_d1,
+    if (!st) st = launch_synthesis(g, t.rt, t.sd, coefs + 2 * plane, L1, L1, sh.Lc, sh.nrows, extend, rad, grid_d1,
                                    hs, nullptr, nullptr, ring);
-    if (!st) st = launch_synthesis(g, *t.rt, *t.sd, coefs, L1, L1, sh.Lc, sh.nrows, extend, theta, grid_d1, hs,
+    if (!st) st = launch_synthesis(g, t.rt, t.sd, coefs, L1, L1, sh.Lc, sh.nrows, extend, theta, grid_d1, hs,
                                    nullptr, nullptr, ring);
-    if (!st) st = launch_synthesis(g, *t.rt, *t.sd, coefs + 4 * plane, L1, L1, sh.Lc, sh.nrows, extend, phi, grid_d1,
+    if (!st) st = launch_synthesis(g, t.rt, t.sd, coefs + 4 * plane, L1, L1, sh.Lc, sh.nrows, extend, phi, grid_d1,
                                    hs, nullptr, nullptr, ring);
     if (!st) {
         const int ncols = sampling * n + extend;
@@ -1810,9 +1839,12 @@ int shtg_shmultiply(double *cilmout, int out_d1, int out_d2, const double *cilm1
     if (csphase != 1 && csphase != -1) return ST_BOUNDS;
     if (norm < 1 || norm > 4) return ST_BOUNDS;
     if (!cilmout || !cilm1 || !cilm2) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     const int nlat = lmaxout + 1, nlong = 2 * lmaxout + 1;
-    const auto *zw = glq_cached(nlat);
+    const std::pair<std::vector<double>, std::vector<double>> *zw;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        zw = glq_cached(nlat);  // entries are never changed once made
+    }
     if (!zw) return ST_OTHER;
     CallTables t;
     int st = get_tables(false, lmaxout, 0, zw->first.data(), zw->second.data(), norm, csphase, lmaxout, t,
@@ -1825,14 +1857,14 @@ int shtg_shmultiply(double *cilmout, int out_d1, int out_d2, const double *cilm1
     double *g1 = (double *)ws, *g2 = g1 + ng;
     // each factor synthesised on the product's grid from its leading
     // (lmax+1)^2 block (src/SHMultiply.f95:330-350)
-    st = launch_synthesis(*t.g, *t.rt, *t.sd, cilm1, c1_d1, c1_d2, lmax1, nlat, 0, g1, nlong, hs);
-    if (!st) st = launch_synthesis(*t.g, *t.rt, *t.sd, cilm2, c2_d1, c2_d2, lmax2, nlat, 0, g2, nlong, hs);
+    st = launch_synthesis(*t.g, t.rt, t.sd, cilm1, c1_d1, c1_d2, lmax1, nlat, 0, g1, nlong, hs);
+    if (!st) st = launch_synthesis(*t.g, t.rt, t.sd, cilm2, c2_d1, c2_d2, lmax2, nlat, 0, g2, nlong, hs);
     if (!st) {
         hipLaunchKernelGGL(k_mul_inplace, dim3(grid_blocks((long long)ng)), dim3(kThreads), 0, hs, g1, g2,
                            (long long)ng);
         if (!HIP_OK(hipGetLastError())) st = ST_OTHER;
     }
-    if (!st) st = launch_analysis(*t.g, *t.rt, *t.sd, g1, nlong, lmaxout, cilmout, out_d1, out_d2, hs);
+    if (!st) st = launch_analysis(*t.g, t.rt, t.sd, g1, nlong, lmaxout, cilmout, out_d1, out_d2, hs);
     (void)hipFreeAsync(ws, hs);
     return st;
 }
@@ -1995,7 +2027,7 @@ int shtg_batch_create_fields(int nfields, const int *lmax, const int *grid, int 
 
 int shtg_batch_destroy(shtg_batch_plan *plan)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
+    // no call on the plan may be in progress (the caller's responsibility)
     batch_free(reinterpret_cast<BatchPlan *>(plan));
     return ST_OK;
 }
@@ -2003,8 +2035,8 @@ int shtg_batch_destroy(shtg_batch_plan *plan)
 int shtg_batch_makegrid_dh(shtg_batch_plan *plan, const double *const *cilm, double *const *grid, void *stream)
 {
     if (!plan || !cilm || !grid) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    std::lock_guard<std::mutex> lk(b->mu);
     int st = batch_bind(b, 0, (const void *const *)cilm, (hipStream_t)stream);
     if (!st) st = batch_bind(b, 1, (const void *const *)grid, (hipStream_t)stream);
     if (st) return st;
@@ -2014,8 +2046,8 @@ int shtg_batch_makegrid_dh(shtg_batch_plan *plan, const double *const *cilm, dou
 int shtg_batch_shexpand_dh(shtg_batch_plan *plan, const double *const *grid, double *const *cilm, void *stream)
 {
     if (!plan || !cilm || !grid) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    std::lock_guard<std::mutex> lk(b->mu);
     int st = batch_bind(b, 2, (const void *const *)grid, (hipStream_t)stream);
     if (!st) st = batch_bind(b, 3, (const void *const *)cilm, (hipStream_t)stream);
     if (st) return st;
@@ -2026,8 +2058,8 @@ int shtg_batch_makegrid_dh_adjoint(shtg_batch_plan *plan, const double *const *g
                                    double *const *cilm_bar, void *stream)
 {
     if (!plan || !cilm_bar || !grid_bar) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    std::lock_guard<std::mutex> lk(b->mu);
     int st = batch_bind(b, 2, (const void *const *)grid_bar, (hipStream_t)stream);
     if (!st) st = batch_bind(b, 3, (const void *const *)cilm_bar, (hipStream_t)stream);
     if (st) return st;
@@ -2042,8 +2074,8 @@ int shtg_batch_shexpand_dh_adjoint(shtg_batch_plan *plan, const double *const *c
                                    double *const *grid_bar, void *stream)
 {
     if (!plan || !cilm_bar || !grid_bar) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    std::lock_guard<std::mutex> lk(b->mu);
     int st = batch_bind(b, 0, (const void *const *)cilm_bar, (hipStream_t)stream);
     if (!st) st = batch_bind(b, 1, (const void *const *)grid_bar, (hipStream_t)stream);
     if (st) return st;
@@ -2056,15 +2088,16 @@ int shtg_batch_shexpand_dh_adjoint(shtg_batch_plan *plan, const double *const *c
 int shtg_batch_tune(shtg_batch_plan *plan, int reps, int flags, void *stream)
 {
     if (!plan || reps < 1 || (flags & ~1)) return ST_BOUNDS;
-    std::lock_guard<std::mutex> lk(g_mu);
-    return batch_tune(reinterpret_cast<BatchPlan *>(plan), (hipStream_t)stream, reps, flags);
+    BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    std::lock_guard<std::mutex> lk(b->mu);
+    return batch_tune(b, (hipStream_t)stream, reps, flags);
 }
 
 int shtg_batch_get_shapes(shtg_batch_plan *plan, int *threads_synth, int *threads_anal, int n)
 {
     if (!plan) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    std::lock_guard<std::mutex> lk(b->mu);
     const int ng = (int)b->fft.size();
     for (int i = 0; i < n && i < ng; ++i) {
         if (threads_synth) threads_synth[i] = b->fft[i].threads;
@@ -2076,16 +2109,17 @@ int shtg_batch_get_shapes(shtg_batch_plan *plan, int *threads_synth, int *thread
 int shtg_batch_set_mfma(shtg_batch_plan *plan, int min_bucket, void *stream)
 {
     if (!plan || min_bucket < 0 || min_bucket == 1) return ST_BOUNDS;
-    std::lock_guard<std::mutex> lk(g_mu);
     (void)stream;
-    return batch_synth_items(reinterpret_cast<BatchPlan *>(plan), min_bucket);
+    BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    std::lock_guard<std::mutex> lk(b->mu);
+    return batch_synth_items(b, min_bucket);
 }
 
 int shtg_batch_get_tuning(shtg_batch_plan *plan, int *synth_kmax, int *anal_kmax, int *mfma_min)
 {
     if (!plan) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    std::lock_guard<std::mutex> lk(b->mu);
     if (synth_kmax) *synth_kmax = b->synth_kmax;
     if (anal_kmax) *anal_kmax = b->anal_kmax;
     if (mfma_min) *mfma_min = b->mfma_min;
@@ -2098,8 +2132,8 @@ int shtg_batch_set_tuning(shtg_batch_plan *plan, int synth_kmax, int anal_kmax, 
     if (synth_kmax < 1 || synth_kmax > kSynthKMax || anal_kmax < 1 || anal_kmax > kAnalKMax || mfma_min < 0 ||
         mfma_min == 1)
         return ST_BOUNDS;
-    std::lock_guard<std::mutex> lk(g_mu);
     BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    std::lock_guard<std::mutex> lk(b->mu);
     b->synth_kmax = synth_kmax;
     b->anal_kmax = anal_kmax;
     int st = batch_synth_items(b, mfma_min);
@@ -2110,8 +2144,8 @@ int shtg_batch_set_tuning(shtg_batch_plan *plan, int synth_kmax, int anal_kmax, 
 int shtg_batch_set_shapes(shtg_batch_plan *plan, const int *threads_synth, const int *threads_anal, int n)
 {
     if (!plan || !threads_synth || !threads_anal) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    std::lock_guard<std::mutex> lk(b->mu);
     if (n != (int)b->fft.size()) return ST_DIMS;
     for (int i = 0; i < n; ++i)
         for (int t : {threads_synth[i], threads_anal[i]})
@@ -2128,8 +2162,8 @@ int shtg_batch_set_shapes(shtg_batch_plan *plan, const int *threads_synth, const
 int shtg_batch_field_groups(shtg_batch_plan *plan, int *group, int n)
 {
     if (!plan || !group) return ST_DIMS;
-    std::lock_guard<std::mutex> lk(g_mu);
     BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);
+    std::lock_guard<std::mutex> lk(b->mu);
     if (n < b->nf) return ST_DIMS;
     for (int i = 0; i < b->nf; ++i) group[i] = b->fft_group_of[i];
     return (int)b->fft.size() + (b->fft_ws.nitems ? 1 : 0);
@@ -2137,14 +2171,14 @@ int shtg_batch_field_groups(shtg_batch_plan *plan, int *group, int n)
 
 int shtg_profile_enable(int on)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
+    std::lock_guard<std::mutex> lk(g_prof_mu);
     g_prof = on != 0;
     return ST_OK;
 }
 
 int shtg_profile_collect(double *ms, long long *count, int nk)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
+    std::lock_guard<std::mutex> lk(g_prof_mu);
     for (int k = 0; k < nk; ++k) { ms[k] = 0.0; count[k] = 0; }
     int st = ST_OK;
     for (auto &r : g_prof_recs) {

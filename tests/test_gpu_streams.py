@@ -72,9 +72,9 @@ def test_captured_plain_calls_replay(torch):
 
 def test_host_threads_on_their_own_streams(torch):
     """Four host threads, each on its own stream, call the plain transforms
-    at interleaved degrees (the library serialises the calls on the host and
-    orders its shared intermediates on the device): every result equals the
-    single-threaded one bitwise."""
+    at interleaved degrees (the library locks only its table lookups, and a
+    call finding the shared intermediates held by another thread's call takes
+    its own): every result equals the single-threaded one bitwise."""
     import threading
     from shtools_amd import backend as B
     print("case host threads", flush=True)
@@ -112,6 +112,77 @@ def test_host_threads_on_their_own_streams(torch):
         for i, g, c2 in outs[k]:
             assert torch.equal(g, ref_g[i]), (k, i)
             assert torch.equal(c2, ref_c[i]), (k, i)
+
+
+def test_host_threads_cold_tables_and_own_plans(torch):
+    """Host threads that start on tables nobody has built (a normalisation
+    not used before, degrees that make the recursion tables grow while other
+    threads' launches are in flight) and threads that each run their own
+    batched plan (DH and GLQ): every result equals the one computed
+    afterwards on one thread, bitwise."""
+    import threading
+    from shtools_amd import backend as B
+    from shtools_amd.batch import DHBatch, GLQBatch
+    print("case host threads cold tables + plans", flush=True)
+    rng = np.random.default_rng(34)
+    Ls = [5, 700, 64, 1100, 300, 17, 900, 129]
+    cs = [torch.from_numpy(rand_cilm(rng, L)).cuda() for L in Ls]
+    plan_Ls = [[40, 200, 7], [300, 31, 31, 90], [128, 12, 500]]
+    plan_cs = [[torch.from_numpy(rand_cilm(rng, L)).cuda() for L in pl] for pl in plan_Ls]
+    plans = [DHBatch(plan_Ls[0], sampling=2, norm=2, csphase=-1),
+             GLQBatch(plan_Ls[1], norm=2, csphase=-1), DHBatch(plan_Ls[2], sampling=1, norm=2, csphase=-1)]
+    outs, pouts, errors = [[] for _ in range(4)], [None] * 3, []
+
+    def plain(k):
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                for rep in range(2):
+                    for j in range(len(Ls)):
+                        i = (j + 2 * k + rep) % len(Ls)
+                        g = B.MakeGridDH(cs[i], sampling=2, norm=2, csphase=-1)
+                        outs[k].append((i, g, B.SHExpandDH(g, sampling=2, norm=2, csphase=-1)))
+            s.synchronize()
+        except Exception as e:  # reported by the main thread
+            errors.append(repr(e))
+
+    def batched(k):
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                res = []
+                for rep in range(3):
+                    g = plans[k].makegrid(plan_cs[k])
+                    res.append((g, plans[k].expand(g)))
+                pouts[k] = res
+            s.synchronize()
+        except Exception as e:
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=plain, args=(k,)) for k in range(4)]
+    threads += [threading.Thread(target=batched, args=(k,)) for k in range(3)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in threads)
+    assert not errors, errors
+    torch.cuda.synchronize()
+    ref_g = [B.MakeGridDH(c, sampling=2, norm=2, csphase=-1) for c in cs]
+    ref_c = [B.SHExpandDH(g, sampling=2, norm=2, csphase=-1) for g in ref_g]
+    torch.cuda.synchronize()
+    for k in range(4):
+        assert len(outs[k]) == 2 * len(Ls)
+        for i, g, c2 in outs[k]:
+            assert torch.equal(g, ref_g[i]), (k, i)
+            assert torch.equal(c2, ref_c[i]), (k, i)
+    for k in range(3):
+        g1 = plans[k].makegrid(plan_cs[k])
+        c1 = plans[k].expand(g1)
+        torch.cuda.synchronize()
+        for g, c2 in pouts[k]:
+            assert all(torch.equal(a, b) for a, b in zip(g, g1)), k
+            assert all(torch.equal(a, b) for a, b in zip(c2, c1)), k
 
 
 def test_batched_calls_on_a_side_stream_keep_their_arrays(torch):

@@ -232,7 +232,7 @@ int shtg_batch_destroy(shtg_batch_plan *plan);
  * the per-field sums).  `stream` is unused (kept for symmetry). */
 int shtg_batch_set_mfma(shtg_batch_plan *plan, int min_bucket, void *stream);
 /* Launch-shape autotuner (no counterpart in the reference): times each
- * ring-FFT group of the plan at 256, 512 and 1024 threads per workgroup
+ * ring-FFT group of the plan at 256, 384, 512, 768 and 1024 threads per workgroup
  * (best of `reps` runs, synthesis and analysis separately), then the
  * Legendre launches: ring pairs per lane of the synthesis waves (2, 3, 4)
  * and the same-degree bucket size from which the synthesis goes to the MFMA
@@ -252,8 +252,9 @@ int shtg_batch_tune(shtg_batch_plan *plan, int reps, int flags, void *stream);
  * them beyond rounding (analysis sums are associated by pass). */
 int shtg_batch_get_tuning(shtg_batch_plan *plan, int *synth_kmax, int *anal_kmax, int *mfma_min);
 int shtg_batch_set_tuning(shtg_batch_plan *plan, int synth_kmax, int anal_kmax, int mfma_min);
-/* Set the ring-FFT workgroup sizes (256, 512 or 1024 per group; n = the
- * number of groups shtg_batch_get_shapes returns). */
+/* Set the ring-FFT workgroup sizes (a multiple of 64 from 256 to 1024 per
+ * group, at least what the group's plans need; n = the number of groups
+ * shtg_batch_get_shapes returns). */
 int shtg_batch_set_shapes(shtg_batch_plan *plan, const int *threads_synth, const int *threads_anal, int n);
 /* Workgroup sizes of the plan's ring-FFT groups (up to n of them); returns
  * the number of groups. */

@@ -2,9 +2,10 @@
 //
 // Every ring (pair) transform is a complex DFT of length n, forward
 // X_k = sum_j x_j e^{-2 pi i jk/n}; inverse transforms are taken as
-// conj(F(conj(.))).  When n has no prime factor above 7 the kernel runs
-// in-place decimation-in-frequency stages (radices up to 8, see
-// factor_stages) on the length-n array and reads results through a
+// conj(F(conj(.))).  When n has no prime factor above 7, apart from at most
+// one of the prime radices 11 .. 23 (kPrimeRadices), the kernel runs
+// in-place decimation-in-frequency stages (radices up to 8 and that prime,
+// see direct_stages) on the length-n array and reads results through a
 // digit-reversal table.  Otherwise the DFT is computed by Bluestein's chirp-z
 // method on a 5-smooth work length m >= 2n-1 (bluestein_length):
 // the chirp-modulated input is transformed by DIF, multiplied by the
@@ -57,6 +58,30 @@ struct FftPlanHost {
     std::vector<double> bhat4;  // split plans: analysis spectrum of the second half
 };
 
+// Prime radices a direct plan may end with (one of them, once): a ring
+// whose length has such a prime factor runs a direct DFT of that length in
+// its last stage (stride 1, no twiddles) instead of Bluestein's three
+// transforms of about 1.5 times its length.  Only rings whose array fits
+// the LDS (n <= kPrimeMaxLen) take such plans: they run in kernel
+// instantiations of their own (kVarPrime), which the workspace kernels for
+// longer rings do not have.  X(R) is applied to each.
+#define SHTG_PRIME_RADICES(X) X(11) X(13) X(17) X(19) X(23)
+#define SHTG_LIST_ITEM(R) R,
+constexpr int kPrimeRadices[] = {SHTG_PRIME_RADICES(SHTG_LIST_ITEM)};
+#undef SHTG_LIST_ITEM
+constexpr int kPrimeMaxLen = 160 * 1024 / 16;
+
+// Kernel instantiations by plan kind: plain plans, split Bluestein plans
+// (FftPlanHost::half) and direct plans ending with a prime radix.
+enum { kVarPlain = 0, kVarSplit = 1, kVarPrime = 2 };
+
+inline bool is_prime_radix(int p)
+{
+    for (int q : kPrimeRadices)
+        if (p == q) return true;
+    return false;
+}
+
 // Fewest-stage factorization of m over the register radices (and radix 7
 // when allow7).  Even radices run first, at the large strides; odd radices
 // run last, so that the stride-1 stage has an odd radix whenever m has an odd
@@ -107,6 +132,23 @@ inline bool factor_stages(int m, bool allow7, std::vector<int> &rad)
     rad = even;
     rad.insert(rad.end(), odd.begin(), odd.end());
     return (int)rad.size() <= kMaxStages;
+}
+
+// Stages of a direct plan of length n: the register radices and radix 7,
+// then (n <= kPrimeMaxLen) at most one prime radix of kPrimeRadices as the
+// last stage.
+inline bool direct_stages(int n, std::vector<int> &rad)
+{
+    if (factor_stages(n, true, rad)) return true;
+    if (n > kPrimeMaxLen) return false;
+    for (int p : kPrimeRadices) {
+        if (n % p != 0 || (n / p) % p == 0) continue;
+        if (!factor_stages(n / p, true, rad)) continue;
+        rad.push_back(p);
+        return (int)rad.size() <= kMaxStages;
+    }
+    rad.clear();
+    return false;
 }
 
 inline bool is_5smooth(int m)
@@ -243,7 +285,7 @@ inline int split_length(int n, int lc)
 {
     if (lc < 0 || n % 2 != 0) return 0;
     std::vector<int> rad;
-    if (factor_stages(n, true, rad)) return 0;  // a direct plan needs no Bluestein
+    if (direct_stages(n, rad)) return 0;  // a direct plan needs no Bluestein
     return bluestein_length(n / 2 + 2 * lc);
 }
 
@@ -274,7 +316,7 @@ inline FftPlanHost make_fft_plan(int n, int lc = -1, bool split = false)
         set_stages(p, rad);
         return p;
     }
-    if (factor_stages(n, true, rad)) {
+    if (direct_stages(n, rad)) {
         p.m = n;
         set_stages(p, rad);
         return p;
@@ -307,6 +349,14 @@ inline FftPlanHost make_fft_plan(int n, int lc = -1, bool split = false)
         p.bhat = chirp_spectrum(p, n - 1, n - 1);
     }
     return p;
+}
+
+// The kernel instantiation a plan needs (P: FftPlanHost or the device plan).
+template <class P>
+inline int plan_variant(const P &F)
+{
+    if (F.half) return kVarSplit;
+    return (F.nst > 0 && F.radix[F.nst - 1] > 8) ? kVarPrime : kVarPlain;
 }
 
 }  // namespace shtg

@@ -35,6 +35,11 @@
 
 namespace shtg {
 
+// Largest workgroup of a ring-FFT kernel instantiation: 512 threads for the
+// prime-radix plans (up to 256 registers per thread for the prime
+// butterflies of up to 23 values), 1024 otherwise.
+constexpr int fft_max_threads(int var) { return var == kVarPrime ? kFftMaxThreads / 2 : kFftMaxThreads; }
+
 // ====================================================================== tiles
 
 // Coefficient tiles of kTL degrees x kTM orders, transposed through LDS so
@@ -453,13 +458,13 @@ __device__ __forceinline__ void synth_fft_run_split(const FftDev &F, int2 rw, in
         // columns j = half h + p at work positions p
         for (int p = threadIdx.x; p < h; p += blockDim.x) {
             const int j = half * h + p;
-            const double2 v = fft_get_c(lds, F, j, p, F.chirp[j]);
+            const double2 v = fft_get_c(lds, F, j, p, gld(F.chirp + j));
             const double gn = v.x * rwt.x, gs = -v.y * rwt.y;
-            if (north) grid[rw.x * ld + j] = gn;
-            if (south) grid[rw.y * ld + j] = gs;
+            if (north) gst(grid + rw.x * ld + j, gn);
+            if (south) gst(grid + rw.y * ld + j, gs);
             if (ext_col && j == 0) {
-                if (north) grid[rw.x * ld + n] = gn;
-                if (south) grid[rw.y * ld + n] = gs;
+                if (north) gst(grid + rw.x * ld + n, gn);
+                if (south) gst(grid + rw.y * ld + n, gs);
             }
         }
         if (half == 0) __syncthreads();  // the second turn overwrites the array
@@ -467,15 +472,15 @@ __device__ __forceinline__ void synth_fft_run_split(const FftDev &F, int2 rw, in
 }
 
 //
-// kSplit: the kernel instantiation for split plans (FftDev::half); kept
-// apart so that the other instantiations do not carry the split flow's
-// registers.
-template <bool kSplit = false>
+// kVar: the kernel instantiation by plan kind (kVarPlain, kVarSplit for
+// split plans, kVarPrime for direct plans ending with a prime radix); each
+// kind's code and registers stay out of the other instantiations.
+template <int kVar = kVarPlain>
 __device__ __forceinline__ void synth_fft_run(const FftDev &F, int2 rw, int nrows, double *__restrict__ grid,
                                               long long ld, int ext_col, double2 rwt, double2 *lds,
                                               double2 *chunk = nullptr, int chunk_cap = 0)
 {
-    if constexpr (kSplit) {
+    if constexpr (kVar == kVarSplit) {
         synth_fft_run_split(F, rw, nrows, grid, ld, ext_col, rwt, lds);
         return;
     }
@@ -483,7 +488,7 @@ __device__ __forceinline__ void synth_fft_run(const FftDev &F, int2 rw, int nrow
     if (chunk && F.nst >= 2 && F.m / F.radix[0] <= chunk_cap)
         fft_forward_chunked(lds, F, F.bhat, chunk, chunk_cap);
     else
-        fft_forward(lds, F, F.bhat);
+        fft_forward<kVar == kVarPrime>(lds, F, F.bhat);
     const bool north = rw.x >= 0 && rw.x < nrows;
     const bool south = rw.y >= 0 && rw.y < nrows;
     // columns j and n - j (1 <= j < n - j) together: one chirp read
@@ -491,22 +496,22 @@ __device__ __forceinline__ void synth_fft_run(const FftDev &F, int2 rw, int nrow
         const double2 c = fft_chirp(F, j);
         const double2 v = fft_get_c(lds, F, j, j, c);
         const double gn = v.x * rwt.x, gs = -v.y * rwt.y;
-        if (north) grid[rw.x * ld + j] = gn;
-        if (south) grid[rw.y * ld + j] = gs;
+        if (north) gst(grid + rw.x * ld + j, gn);
+        if (south) gst(grid + rw.y * ld + j, gs);
         if (ext_col && j == 0) {
-            if (north) grid[rw.x * ld + n] = gn;
-            if (south) grid[rw.y * ld + n] = gs;
+            if (north) gst(grid + rw.x * ld + n, gn);
+            if (south) gst(grid + rw.y * ld + n, gs);
         }
         const int j2 = n - j;
         if (j > 0 && j2 > j) {
             const double2 v2 = fft_get_c(lds, F, j2, j2, fft_chirp_mirror(F, c));
-            if (north) grid[rw.x * ld + j2] = v2.x * rwt.x;
-            if (south) grid[rw.y * ld + j2] = -v2.y * rwt.y;
+            if (north) gst(grid + rw.x * ld + j2, v2.x * rwt.x);
+            if (south) gst(grid + rw.y * ld + j2, -v2.y * rwt.y);
         }
     }
 }
 
-template <bool kSplit = false>
+template <int kVar = kVarPlain>
 __device__ __forceinline__ void synth_fft_tile(const FftDev &F, const double4 *__restrict__ fbuf, int P, int Lc,
                                                int s, double u, int2 rw, int nrows, double *__restrict__ grid,
                                                long long ld, int ext_col, double2 rwt, double2 *lds,
@@ -516,7 +521,7 @@ __device__ __forceinline__ void synth_fft_tile(const FftDev &F, const double4 *_
     synth_fft_load(fbuf, P, Lc, s, in);
     synth_fft_fill(F, in, fbuf, P, Lc, s, u, lds);
     __syncthreads();
-    synth_fft_run<kSplit>(F, rw, nrows, grid, ld, ext_col, rwt, lds, chunk, chunk_cap);
+    synth_fft_run<kVar>(F, rw, nrows, grid, ld, ext_col, rwt, lds, chunk, chunk_cap);
 }
 
 // Analysis inputs of one thread: ring values at the column pairs q and
@@ -533,11 +538,11 @@ __device__ __forceinline__ double2 anal_column(const double *__restrict__ grid, 
                                                bool south, int n, int ext_col, int k)
 {
     double2 v;
-    v.x = north ? grid[rw.x * ld + k] : 0.0;
-    v.y = south ? grid[rw.y * ld + k] : 0.0;
+    v.x = north ? gld(grid + rw.x * ld + k) : 0.0;
+    v.y = south ? gld(grid + rw.y * ld + k) : 0.0;
     if (ext_col && k == 0) {
-        if (north) v.x += grid[rw.x * ld + n];
-        if (south) v.y += grid[rw.y * ld + n];
+        if (north) v.x += gld(grid + rw.x * ld + n);
+        if (south) v.y += gld(grid + rw.y * ld + n);
     }
     return v;
 }
@@ -609,6 +614,7 @@ __device__ __forceinline__ void anal_fft_extract(const FftDev &F, int2 rw, int n
 // DFT of the filled array; w = (w_north, w_south) include the quadrature
 // weight and the reference's constant factors; the outputs carry the
 // order's rescale factor for the ring (sin(theta) = u).
+template <int kVar = kVarPlain>
 __device__ __forceinline__ void anal_fft_run(const FftDev &F, int2 rw, int nrows, double2 w, double u, int P, int Lc,
                                              int s, double4 *__restrict__ gbuf, double2 *lds,
                                              double2 *chunk = nullptr, int chunk_cap = 0)
@@ -616,7 +622,7 @@ __device__ __forceinline__ void anal_fft_run(const FftDev &F, int2 rw, int nrows
     if (chunk && F.nst >= 2 && F.m / F.radix[0] <= chunk_cap)
         fft_forward_chunked(lds, F, F.bhat2, chunk, chunk_cap);
     else
-        fft_forward(lds, F, F.bhat2);
+        fft_forward<kVar == kVarPrime>(lds, F, F.bhat2);
     anal_fft_extract(F, rw, nrows, w, u, P, Lc, s, gbuf, lds);
 }
 
@@ -638,7 +644,7 @@ __device__ __forceinline__ void anal_fft_tile_split(const FftDev &F, const doubl
             double2 v = make_double2(0.0, 0.0);
             if (q < h) {
                 const int col = half * h + q;
-                v = fft_prep_in_c(F, anal_column(grid, ld, rw, north, south, n, ext_col, col), F.chirp[col]);
+                v = fft_prep_in_c(F, anal_column(grid, ld, rw, north, south, n, ext_col, col), gld(F.chirp + col));
             }
             lds[fpad(q)] = v;
         }
@@ -656,13 +662,13 @@ __device__ __forceinline__ void anal_fft_tile_split(const FftDev &F, const doubl
     anal_fft_extract(F, rw, nrows, w, u, P, Lc, s, gbuf, lds);
 }
 
-template <bool kSplit = false>
+template <int kVar = kVarPlain>
 __device__ __forceinline__ void anal_fft_tile(const FftDev &F, const double *__restrict__ grid, long long ld,
                                               int2 rw, int nrows, double2 w, double u, int P, int Lc, int s,
                                               double4 *__restrict__ gbuf, int ext_col, double2 *lds,
                                               double2 *chunk = nullptr, int chunk_cap = 0)
 {
-    if constexpr (kSplit) {
+    if constexpr (kVar == kVarSplit) {
         anal_fft_tile_split(F, grid, ld, rw, nrows, w, u, P, Lc, s, gbuf, ext_col, lds);
         return;
     }
@@ -670,7 +676,7 @@ __device__ __forceinline__ void anal_fft_tile(const FftDev &F, const double *__r
     anal_fft_load(F, grid, ld, rw, nrows, ext_col, in);
     anal_fft_fill(F, in, grid, ld, rw, nrows, ext_col, lds);
     __syncthreads();
-    anal_fft_run(F, rw, nrows, w, u, P, Lc, s, gbuf, lds, chunk, chunk_cap);
+    anal_fft_run<kVar>(F, rw, nrows, w, u, P, Lc, s, gbuf, lds, chunk, chunk_cap);
 }
 
 // =============================================================== plain kernels
@@ -715,8 +721,8 @@ __global__ __launch_bounds__(kWave) void k_synth_leg(const double *__restrict__ 
 
 // rows[s] = (north row, south row or -1); rows >= nrows are skipped;
 // roww[s]: row weights (nullptr: 1).
-template <bool kSplit>
-__global__ __launch_bounds__(kFftMaxThreads) void k_synth_fft(FftDev F, const double4 *__restrict__ fbuf, int P, int Lc,
+template <int kVar>
+__global__ __launch_bounds__(fft_max_threads(kVar)) void k_synth_fft(FftDev F, const double4 *__restrict__ fbuf, int P, int Lc,
                                                         const double *__restrict__ ringu,
                                                         const int2 *__restrict__ rows, int nrows,
                                                         double *__restrict__ grid, long long ld, int ext_col,
@@ -725,12 +731,12 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_synth_fft(FftDev F, const do
 {
     extern __shared__ double2 lds[];
     const int s = blockIdx.x;
-    synth_fft_tile<kSplit>(F, fbuf, P, slot_orders(kmax, s, Lc), s, ringu[s], rows[s], nrows, grid, ld, ext_col,
+    synth_fft_tile<kVar>(F, fbuf, P, slot_orders(kmax, s, Lc), s, ringu[s], rows[s], nrows, grid, ld, ext_col,
                            roww ? roww[s] : make_double2(1.0, 1.0), lds);
 }
 
-template <bool kSplit>
-__global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft(FftDev F, const double *__restrict__ grid, long long ld,
+template <int kVar>
+__global__ __launch_bounds__(fft_max_threads(kVar)) void k_anal_fft(FftDev F, const double *__restrict__ grid, long long ld,
                                                        const double *__restrict__ ringu,
                                                        const int2 *__restrict__ rows, int nrows,
                                                        const double2 *__restrict__ wts, int P, int Lc,
@@ -739,7 +745,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft(FftDev F, const dou
 {
     extern __shared__ double2 lds[];
     const int s = blockIdx.x;
-    anal_fft_tile<kSplit>(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, slot_orders(kmax, s, Lc), s, gbuf,
+    anal_fft_tile<kVar>(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, slot_orders(kmax, s, Lc), s, gbuf,
                           ext_col, lds);
 }
 
@@ -918,7 +924,7 @@ __device__ __forceinline__ double msign(int m) { return (m & 1) ? -1.0 : 1.0; }
 // Complex synthesis of slot s: for the north (ring 0) and south (ring 1)
 // row, X_k = F_k and X_{n-k} = (-1)^k G_k (F, G the sums of orders +k, -k,
 // times the order's rescale), inverse DFT via conj(DFT(conj X)).
-template <bool kSplit>
+template <int kVar>
 __device__ __forceinline__ void synth_fft_c_tile(const FftDev &F, const double4 *__restrict__ fb_p,
                                                  const double4 *__restrict__ fb_m, int P, int Lc, int s, double u,
                                                  int2 rw, int nrows, double2 *__restrict__ grid, long long ld,
@@ -946,7 +952,7 @@ __device__ __forceinline__ void synth_fft_c_tile(const FftDev &F, const double4 
             }
         }
         __syncthreads();
-        if constexpr (kSplit) {
+        if constexpr (kVar == kVarSplit) {
             // split plan: the inverse stages once per half of the columns
             TurnStash keep;
             const int h = n / 2;
@@ -972,7 +978,7 @@ __device__ __forceinline__ void synth_fft_c_tile(const FftDev &F, const double4 
         if (chunked)
             fft_forward_chunked(x, F, F.bhat, chunk, chunk_cap);
         else
-            fft_forward(x, F, F.bhat);
+            fft_forward<kVar == kVarPrime>(x, F, F.bhat);
         for (int j = threadIdx.x; j < n; j += blockDim.x) {
             const double2 v = conjd(fft_get(x, F, j, j));
             grid[row * ld + j] = v;
@@ -985,7 +991,7 @@ __device__ __forceinline__ void synth_fft_c_tile(const FftDev &F, const double4 
 // Complex analysis of slot s: DFT of each ring, weighted, orders +k and -k
 // (the latter with (-1)^k), times the order's rescale; the north pass
 // stores (f, f), the south pass turns them into (f_N + f_S, f_N - f_S).
-template <bool kSplit>
+template <int kVar>
 __device__ __forceinline__ void anal_fft_c_tile(const FftDev &F, const double2 *__restrict__ grid, long long ld,
                                                 int2 rw, int nrows, double2 w, double u, int P, int Lc, int s,
                                                 double4 *__restrict__ gb_p, double4 *__restrict__ gb_m, double2 *x,
@@ -997,7 +1003,7 @@ __device__ __forceinline__ void anal_fft_c_tile(const FftDev &F, const double2 *
         const int row = ring ? rw.y : rw.x;
         if (row < 0 || row >= nrows) continue;
         const double wr = ring ? w.y : w.x;
-        if constexpr (kSplit) {
+        if constexpr (kVar == kVarSplit) {
             // split plan: each half of the columns filled and transformed in
             // turn, the products with their spectra summed in the turn
             TurnStash keep;
@@ -1024,7 +1030,7 @@ __device__ __forceinline__ void anal_fft_c_tile(const FftDev &F, const double2 *
             if (chunked)
                 fft_forward_chunked(x, F, F.bhat2, chunk, chunk_cap);
             else
-                fft_forward(x, F, F.bhat2);
+                fft_forward<kVar == kVarPrime>(x, F, F.bhat2);
         }
         for (int k = threadIdx.x; k <= Lc; k += blockDim.x) {
             const double r = order_rescale(u, k) * wr;
@@ -1052,8 +1058,8 @@ __device__ __forceinline__ void anal_fft_c_tile(const FftDev &F, const double2 *
 }
 
 // One workgroup per slot (LDS) or per slice of a global workspace (WS).
-template <bool WS, bool kSplit>
-__global__ __launch_bounds__(kFftMaxThreads) void k_synth_fft_c(FftDev F, const double4 *__restrict__ fb_p,
+template <bool WS, int kVar>
+__global__ __launch_bounds__(fft_max_threads(kVar)) void k_synth_fft_c(FftDev F, const double4 *__restrict__ fb_p,
                                                           const double4 *__restrict__ fb_m, int P, int Lc,
                                                           const double *__restrict__ ringu,
                                                           const int2 *__restrict__ rows, int nrows,
@@ -1065,12 +1071,12 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_synth_fft_c(FftDev F, const 
     if constexpr (WS) x = ws + blockIdx.x * ws_stride;
     else x = lds;
     for (int s = blockIdx.x; s < P; s += gridDim.x)
-        synth_fft_c_tile<kSplit>(F, fb_p, fb_m, P, Lc, s, ringu[s], rows[s], nrows, grid, ld, ext_col, x,
+        synth_fft_c_tile<kVar>(F, fb_p, fb_m, P, Lc, s, ringu[s], rows[s], nrows, grid, ld, ext_col, x,
                          WS ? lds : nullptr, chunk_cap);
 }
 
-template <bool WS, bool kSplit>
-__global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft_c(FftDev F, const double2 *__restrict__ grid,
+template <bool WS, int kVar>
+__global__ __launch_bounds__(fft_max_threads(kVar)) void k_anal_fft_c(FftDev F, const double2 *__restrict__ grid,
                                                          long long ld, const double *__restrict__ ringu,
                                                          const int2 *__restrict__ rows, int nrows,
                                                          const double2 *__restrict__ wts, int P, int Lc,
@@ -1082,7 +1088,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft_c(FftDev F, const d
     if constexpr (WS) x = ws + blockIdx.x * ws_stride;
     else x = lds;
     for (int s = blockIdx.x; s < P; s += gridDim.x)
-        anal_fft_c_tile<kSplit>(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, Lc, s, gb_p, gb_m, x,
+        anal_fft_c_tile<kVar>(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, Lc, s, gb_p, gb_m, x,
                         WS ? lds : nullptr, chunk_cap);
 }
 
@@ -1415,8 +1421,8 @@ __device__ __forceinline__ int xcd_item(int b, int n)
     return it < n ? it : -1;
 }
 
-template <bool kSplit>
-__global__ __launch_bounds__(kFftMaxThreads) void kb_synth_fft(const BField *__restrict__ fields,
+template <int kVar>
+__global__ __launch_bounds__(fft_max_threads(kVar)) void kb_synth_fft(const BField *__restrict__ fields,
                                                          const int2 *__restrict__ items, int nitems, int nslots,
                                                          BTables T, const double4 *__restrict__ fbuf,
                                                          double *const *grid)
@@ -1437,14 +1443,14 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_synth_fft(const BField *__r
         synth_fft_fill(F, in, fb, f.P, slot_orders(T.kmax, f.geo + s, f.L), s, T.u[f.geo + s], lds);
         __syncthreads();
         if (s + 1 < send) synth_fft_load(fb, f.P, slot_orders(T.kmax, f.geo + s + 1, f.L), s + 1, in);
-        synth_fft_run<kSplit>(F, T.rows[f.geo + s], f.nlat, grid[it.x], f.nlong, 0,
+        synth_fft_run<kVar>(F, T.rows[f.geo + s], f.nlat, grid[it.x], f.nlong, 0,
                       T.roww ? T.roww[f.geo + s] : make_double2(1.0, 1.0), lds);
         __syncthreads();
     }
 }
 
-template <bool kSplit>
-__global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft(const BField *__restrict__ fields,
+template <int kVar>
+__global__ __launch_bounds__(fft_max_threads(kVar)) void kb_anal_fft(const BField *__restrict__ fields,
                                                         const int2 *__restrict__ items, int nitems, int nslots,
                                                         BTables T, const double *const *grid,
                                                         double4 *__restrict__ gbuf)
@@ -1457,7 +1463,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft(const BField *__re
     const FftDev &F = T.plans[f.fft];
     const double *g = grid[it.x];
     const int send = min(it.y + nslots, f.P);
-    if constexpr (kSplit) {
+    if constexpr (kVar == kVarSplit) {
         for (int s = it.y; s < send; ++s) {
             anal_fft_tile_split(F, g, f.nlong, T.rows[f.geo + s], f.nlat, T.w[f.geo + s], T.u[f.geo + s], f.P,
                                 slot_orders(T.kmax, f.geo + s, f.L), s, gbuf + f.foff, 0, lds);
@@ -1474,7 +1480,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void kb_anal_fft(const BField *__re
         anal_fft_fill(F, in, g, f.nlong, rw, f.nlat, 0, lds);
         __syncthreads();
         if (s + 1 < send) anal_fft_load(F, g, f.nlong, T.rows[f.geo + s + 1], f.nlat, 0, in);
-        anal_fft_run(F, rw, f.nlat, T.w[f.geo + s], T.u[f.geo + s], f.P, slot_orders(T.kmax, f.geo + s, f.L), s,
+        anal_fft_run<kVar>(F, rw, f.nlat, T.w[f.geo + s], T.u[f.geo + s], f.P, slot_orders(T.kmax, f.geo + s, f.L), s,
                      gbuf + f.foff, lds);
         __syncthreads();
     }

@@ -115,11 +115,21 @@ inline int plan_min_threads(const P &F)
     return (need + kWave - 1) / kWave * kWave;
 }
 
+// Largest workgroup of the kernel instantiation a plan runs in (512 for
+// prime-radix plans, whose kernels are built for up to 256 registers per
+// thread).
+template <class P>
+inline int plan_max_threads(const P &F)
+{
+    return plan_variant(F) == kVarPrime ? kFftMaxThreads / 2 : kFftMaxThreads;
+}
+
 template <class P>
 inline int plan_threads(const P &F)
 {
     int t = fft_threads(F.m);
     if (F.half && F.m > 4096) t = std::min(t, kFftMaxThreads / 2);
+    t = std::min(t, plan_max_threads(F));
     return std::max(t, plan_min_threads(F));
 }
 
@@ -394,14 +404,16 @@ inline void build_anal_items(const std::vector<BField> &fields, const std::vecto
 
 // Ring-FFT work items: (field, first slot) of fft_slots consecutive ring
 // pairs, grouped by how many workgroups fit the LDS of a CU (the group's LDS
-// is its largest need), then by threads per workgroup and split plans;
-// fields whose rings exceed the LDS form the workspace group.
+// is its largest need), then by threads per workgroup and plan variant
+// (kVarPlain, kVarSplit, kVarPrime: one kernel instantiation each); fields
+// whose rings exceed the LDS form the workspace group.
 struct FftGroupLayout {
     std::vector<Item2> items;
     size_t lds = 0;
     int threads = 256;
     int min_threads = 1;
-    bool half = false;
+    int max_threads = 1024;  // the instantiation's largest workgroup
+    int variant = kVarPlain;
 };
 struct FftLayout {
     std::vector<FftGroupLayout> groups;
@@ -413,7 +425,7 @@ struct FftLayout {
 inline FftLayout build_fft_groups(const std::vector<BField> &fields, const std::vector<FftPlanHost> &plans,
                                   int fft_slots)
 {
-    typedef std::tuple<int, int, int> GKey;  // (-(WGs per CU), -threads, split)
+    typedef std::tuple<int, int, int> GKey;  // (-(WGs per CU), -threads, plan variant)
     std::map<GKey, std::vector<WorkItem>> groups;
     std::map<GKey, size_t> group_lds;
     std::map<GKey, int> group_min;
@@ -429,7 +441,7 @@ inline FftLayout build_fft_groups(const std::vector<BField> &fields, const std::
             continue;
         }
         const int per_cu = std::max(1, (int)(kMaxLds / need));
-        const GKey key(-std::min(per_cu, 4), -plan_threads(p), p.half);
+        const GKey key(-std::min(per_cu, 4), -plan_threads(p), plan_variant(p));
         group_lds[key] = std::max(group_lds[key], need);
         group_min[key] = std::max(group_min[key], plan_min_threads(p));
         for (int s = 0; s < f.P; s += fft_slots) groups[key].push_back({(double)need, i, s, 0});
@@ -443,7 +455,8 @@ inline FftLayout build_fft_groups(const std::vector<BField> &fields, const std::
         g.lds = group_lds[kv.first];
         g.threads = -std::get<1>(kv.first);
         g.min_threads = group_min[kv.first];
-        g.half = std::get<2>(kv.first) != 0;
+        g.variant = std::get<2>(kv.first);
+        g.max_threads = g.variant == kVarPrime ? kFftMaxThreads / 2 : kFftMaxThreads;
         out.groups.push_back(std::move(g));
     }
     std::stable_sort(wsi.begin(), wsi.end(), by_work);

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "fftplan.hpp"
+#include "prime_roots.hpp"
 
 namespace shtg {
 
@@ -13,6 +14,30 @@ namespace shtg {
 // Complex product with the fused multiply-adds written out, so that its
 // rounding does not depend on which product the compiler would contract in
 // a given kernel (the plain and the batched kernels then agree bit for bit).
+// Loads through the global address space.  Pointers the batched kernels
+// read from tables in memory (plans, field arrays) are generic to the
+// compiler, and its flat loads count against the LDS wait counter as well:
+// each wait for LDS data would then also wait for a twiddle or spectrum
+// load from the caches.  T must live in device global memory.
+// (Vector types are loaded by component: their copy constructors take
+// generic references, which would turn the load back into a flat one.)
+typedef const __attribute__((address_space(1))) double gdouble;
+typedef const __attribute__((address_space(1))) int gint;
+__device__ __forceinline__ double gld(const double *p) { return *(gdouble *)p; }
+__device__ __forceinline__ int gld(const int *p) { return *(gint *)p; }
+__device__ __forceinline__ double2 gld(const double2 *p)
+{
+    gdouble *q = (gdouble *)p;
+    return make_double2(q[0], q[1]);
+}
+__device__ __forceinline__ double4 gld(const double4 *p)
+{
+    gdouble *q = (gdouble *)p;
+    return make_double4(q[0], q[1], q[2], q[3]);
+}
+// Stores through the global address space (see gld).
+__device__ __forceinline__ void gst(double *p, double v) { *(__attribute__((address_space(1))) double *)p = v; }
+
 __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 {
     return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
@@ -154,6 +179,52 @@ __device__ __forceinline__ void dft_unrolled(double2 (&v)[R])
     }
 }
 
+// DFT of prime length R in registers (the last stage of direct plans whose
+// length has a prime factor among kPrimeRadices): with t_j = v_j + v_{R-j}
+// and d_j = v_j - v_{R-j}, j = 1 .. (R-1)/2,
+//     X_k     = v_0 + sum_j t_j cos(2 pi jk/R) - i sum_j d_j sin(2 pi jk/R),
+//     X_{R-k} = v_0 + sum_j t_j cos(2 pi jk/R) + i sum_j d_j sin(2 pi jk/R),
+// (R-1)^2 fused multiply-adds per butterfly in place of 4 (R-1)^2 for the
+// plain sums.  The roots come from the constant table kPrimeRoots
+// (prime_roots.hpp) at indices that are the same for every lane, so they
+// arrive as scalar operands; the loop over k stays rolled to keep the
+// registers for t and d.  Outputs go to x[base + k*S] as they are formed.
+template <int R>
+__device__ __forceinline__ void dft_prime_store(double2 (&v)[R], double2 *x, int base, int S)
+{
+    constexpr int H = (R - 1) / 2;
+    constexpr int off = prime_roots_offset(R);
+    static_assert(off >= 0, "a prime radix with a root table");
+    double2 t[H], d[H];
+    const double2 x0 = v[0];
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+        t[j - 1] = cadd(v[j], v[R - j]);
+        d[j - 1] = csub(v[j], v[R - j]);
+    }
+    double2 sum = x0;
+#pragma unroll
+    for (int j = 0; j < H; ++j) sum = cadd(sum, t[j]);
+    x[fpad(base)] = sum;
+#pragma unroll 1
+    for (int k = 1; k <= H; ++k) {
+        double2 a = x0, b = make_double2(0.0, 0.0);
+        int q = 0;
+#pragma unroll
+        for (int j = 1; j <= H; ++j) {
+            q += k;
+            if (q >= R) q -= R;  // q = jk mod R
+            const double c = kPrimeRoots[off + 2 * q], sn = kPrimeRoots[off + 2 * q + 1];
+            a.x = fma(t[j - 1].x, c, a.x);
+            a.y = fma(t[j - 1].y, c, a.y);
+            b.x = fma(d[j - 1].x, sn, b.x);
+            b.y = fma(d[j - 1].y, sn, b.y);
+        }
+        x[fpad(base + k * S)] = make_double2(a.x + b.y, a.y - b.x);
+        x[fpad(base + (R - k) * S)] = make_double2(a.x - b.y, a.y + b.x);
+    }
+}
+
 // v[r] *= w^r for r = 1..R-1 (R <= 8), w = e^{-2 pi i i/(R S)} of the
 // butterfly: the powers come from a product tree of depth 3.
 template <int R>
@@ -175,8 +246,9 @@ __device__ __forceinline__ void apply_twiddles(double2 (&v)[R], double2 w1)
 // One in-place stage of radix R over x[0..m): butterflies over elements
 // base + r*S; DIF applies twiddles after the butterfly, DIT (the transposed
 // stage) before it.  Radices 2, 3, 4, 5 and 8 are unrolled in registers;
-// radix 7 uses a rolled loop over outputs that stores each output as it is
-// formed.
+// radix 7 (and a prime radix at a stride other than 1) uses a rolled loop
+// over outputs that stores each output as it is formed; the prime radices
+// at stride 1 use dft_prime_store.
 //
 // x holds mloc elements (mloc = m, or a run of whole groups of R*S elements
 // of the m-element transform: the butterflies and twiddles are the same in
@@ -186,6 +258,7 @@ __device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double
                                           const double2 *__restrict__ tws, int mloc)
 {
     constexpr bool kUnrolled = (R == 2 || R == 3 || R == 4 || R == 5 || R == 8);
+    constexpr bool kPrime = R >= 11 && !DIT;
     const int Lsub = R * S, step = m / Lsub, nb = mloc / R, nt = blockDim.x;
     // butterfly b = grp * S + i starts at element base = grp * Lsub + i; the
     // thread's first (grp, i) come from one division, later ones by stepping
@@ -206,16 +279,19 @@ __device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double
         for (int r = 0; r < R; ++r) v[r] = x[fpad(base + r * S)];
         if constexpr (kUnrolled) {
             // twiddles w^r, w = e^{-2 pi i i/Lsub}
-            const double2 w1 = tws[i];
+            const double2 w1 = gld(tws + i);
             if constexpr (DIT) apply_twiddles<R>(v, w1);
             dft_unrolled<R>(v);
             if constexpr (!DIT) apply_twiddles<R>(v, w1);
 #pragma unroll
             for (int r = 0; r < R; ++r) x[fpad(base + r * S)] = v[r];
+        } else if (kPrime && S == 1) {
+            // the last DIF stage of a direct plan: no twiddles
+            if constexpr (kPrime) dft_prime_store<R>(v, x, base, S);
         } else {
             if constexpr (DIT) {
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[i * r * step]);
+                for (int r = 1; r < R; ++r) v[r] = cmul(v[r], gld(tw + i * r * step));
             }
             const int st = m / R;
             for (int k = 0; k < R; ++k) {
@@ -225,12 +301,12 @@ __device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double
                 for (int j = 1; j < R; ++j) {
                     jk += k;
                     if (jk >= R) jk -= R;
-                    const double2 c = tw[jk * st];
+                    const double2 c = gld(tw + jk * st);
                     acc.x += v[j].x * c.x - v[j].y * c.y;
                     acc.y += v[j].x * c.y + v[j].y * c.x;
                 }
                 if constexpr (!DIT) {
-                    if (k > 0) acc = cmul(acc, tw[i * k * step]);
+                    if (k > 0) acc = cmul(acc, gld(tw + i * k * step));
                 }
                 x[fpad(base + k * S)] = acc;
             }
@@ -244,7 +320,10 @@ __device__ __forceinline__ void fft_stage(double2 *x, int m, int S, const double
     }
 }
 
-template <bool DIT>
+// kPrimes: the instantiation for direct plans ending with a prime radix
+// (kVarPrime), which alone carries the prime butterflies and their
+// registers.  Such plans have no DIT stages.
+template <bool DIT, bool kPrimes = false>
 __device__ __forceinline__ void fft_run_stage(double2 *x, const FftDev &P, int st, int mloc)
 {
     const int S = P.stride[st];
@@ -255,7 +334,17 @@ __device__ __forceinline__ void fft_run_stage(double2 *x, const FftDev &P, int s
         SHTG_FFT_RADICES(SHTG_STAGE_CASE)
 #undef SHTG_STAGE_CASE
     case 7: fft_stage<7, DIT>(x, P.m, S, P.tw, tws, mloc); break;
-    default: break;
+    default:
+        if constexpr (kPrimes && !DIT) {
+            switch (P.radix[st]) {
+#define SHTG_PRIME_CASE(R) \
+    case R: fft_stage<R, false>(x, P.m, S, P.tw, tws, mloc); break;
+                SHTG_PRIME_RADICES(SHTG_PRIME_CASE)
+#undef SHTG_PRIME_CASE
+            default: break;
+            }
+        }
+        break;
     }
 }
 
@@ -263,7 +352,7 @@ __device__ __forceinline__ void fft_run_stage(double2 *x, const FftDev &P, int s
 // before fft_forward.
 __device__ __forceinline__ double2 fft_prep_in(const FftDev &P, int j, double2 v)
 {
-    return P.blue ? cmul(v, P.chirp[j]) : v;
+    return P.blue ? cmul(v, gld(P.chirp + j)) : v;
 }
 
 // The chirp e^{-i pi j^2/n} of natural index j (Bluestein plans).  Indices j
@@ -271,7 +360,7 @@ __device__ __forceinline__ double2 fft_prep_in(const FftDev &P, int j, double2 v
 // that handle both read the table once.
 __device__ __forceinline__ double2 fft_chirp(const FftDev &P, int j)
 {
-    return P.blue ? P.chirp[j] : make_double2(1.0, 0.0);
+    return P.blue ? gld(P.chirp + j) : make_double2(1.0, 0.0);
 }
 __device__ __forceinline__ double2 fft_chirp_mirror(const FftDev &P, double2 c)
 {
@@ -301,7 +390,7 @@ __device__ __forceinline__ void fft_turn(double2 *x, int mloc, const double2 *__
         for (int r = 0; r < R; ++r) v[r] = x[fpad(base + r)];
         dft_unrolled<R>(v);
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = conjd(cmul(v[r], bhat[base + r]));
+        for (int r = 0; r < R; ++r) v[r] = conjd(cmul(v[r], gld(bhat + base + r)));
         dft_unrolled<R>(v);
 #pragma unroll
         for (int r = 0; r < R; ++r) x[fpad(base + r)] = v[r];
@@ -326,12 +415,13 @@ __device__ __forceinline__ void fft_run_turn(double2 *x, const FftDev &P, const 
 // One call site per stage direction (the last DIF stage of a plain plan runs
 // in the same loop as the others), which keeps the kernels' code and
 // register use down.
+template <bool kPrimes = false>
 __device__ __forceinline__ void fft_forward(double2 *x, const FftDev &P, const double2 *__restrict__ bhat)
 {
     const int last = P.nst - 1;
     const int ndif = P.blue ? last : last + 1;  // Bluestein: the last stage is part of the turn
     for (int st = 0; st < ndif; ++st) {
-        fft_run_stage<false>(x, P, st, P.m);
+        fft_run_stage<false, kPrimes>(x, P, st, P.m);
         __syncthreads();
     }
     if (!P.blue) return;
@@ -450,11 +540,11 @@ __device__ __forceinline__ void fft_turn_split(double2 *x, int m, const double2 
         }
         if constexpr (MODE == kAnal1) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) keep.v[i + r] = cmul(v[r], bhat[base + r]);
+            for (int r = 0; r < R; ++r) keep.v[i + r] = cmul(v[r], gld(bhat + base + r));
         } else {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                double2 t = cmul(v[r], bhat[base + r]);
+                double2 t = cmul(v[r], gld(bhat + base + r));
                 if constexpr (MODE == kAnal2) t = cadd(t, keep.v[i + r]);
                 v[r] = conjd(t);
             }
@@ -479,15 +569,15 @@ __device__ __forceinline__ void fft_run_turn_split(double2 *x, const FftDev &P, 
 // holding order k (fft_pos for signed orders of pruned plans).
 __device__ __forceinline__ double2 fft_get(const double2 *x, const FftDev &P, int k, int pos)
 {
-    if (P.blue) return cmul(conjd(x[fpad(pos)]), P.chirp[k]);
-    return x[fpad(P.perm[k])];
+    if (P.blue) return cmul(conjd(x[fpad(pos)]), gld(P.chirp + k));
+    return x[fpad(gld(P.perm + k))];
 }
 
 // fft_get with the chirp c of k given (fft_chirp / fft_chirp_mirror).
 __device__ __forceinline__ double2 fft_get_c(const double2 *x, const FftDev &P, int k, int pos, double2 c)
 {
     if (P.blue) return cmul(conjd(x[fpad(pos)]), c);
-    return x[fpad(P.perm[k])];
+    return x[fpad(gld(P.perm + k))];
 }
 
 }  // namespace shtg

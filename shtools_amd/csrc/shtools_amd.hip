@@ -43,6 +43,16 @@ static std::mutex g_prof_mu;
 
 #define HIP_OK(x) (hipSuccess == (x))
 
+// The ring-FFT kernel instantiation for a plan variant (kVarPlain,
+// kVarSplit, kVarPrime).
+template <class K>
+static K by_variant(int v, K plain, K split, K prime)
+{
+    return v == kVarSplit ? split : v == kVarPrime ? prime : plain;
+}
+#define SHTG_FFT_KERNEL(k, v) by_variant((v), k<kVarPlain>, k<kVarSplit>, k<kVarPrime>)
+
+
 template <class T>
 static T *dev_upload(const std::vector<T> &v)
 {
@@ -93,14 +103,18 @@ static bool kernels_configured = false;
 static void configure_kernels()
 {
     if (kernels_configured) return;
-    for (const void *k : {(const void *)k_synth_fft<false>, (const void *)k_synth_fft<true>,
-                          (const void *)k_anal_fft<false>, (const void *)k_anal_fft<true>,
-                          (const void *)k_synth_fft_ws, (const void *)k_anal_fft_ws, (const void *)kb_synth_fft<false>,
-                          (const void *)kb_synth_fft<true>, (const void *)kb_anal_fft<false>,
-                          (const void *)kb_anal_fft<true>, (const void *)kb_synth_fft_ws, (const void *)kb_anal_fft_ws,
-                          (const void *)k_synth_fft_c<false, false>, (const void *)k_synth_fft_c<false, true>,
-                          (const void *)k_synth_fft_c<true, false>, (const void *)k_anal_fft_c<false, false>,
-                          (const void *)k_anal_fft_c<false, true>, (const void *)k_anal_fft_c<true, false>})
+    for (const void *k :
+         {(const void *)k_synth_fft<kVarPlain>, (const void *)k_synth_fft<kVarSplit>,
+          (const void *)k_synth_fft<kVarPrime>, (const void *)k_anal_fft<kVarPlain>,
+          (const void *)k_anal_fft<kVarSplit>, (const void *)k_anal_fft<kVarPrime>, (const void *)k_synth_fft_ws,
+          (const void *)k_anal_fft_ws, (const void *)kb_synth_fft<kVarPlain>, (const void *)kb_synth_fft<kVarSplit>,
+          (const void *)kb_synth_fft<kVarPrime>, (const void *)kb_anal_fft<kVarPlain>,
+          (const void *)kb_anal_fft<kVarSplit>, (const void *)kb_anal_fft<kVarPrime>, (const void *)kb_synth_fft_ws,
+          (const void *)kb_anal_fft_ws, (const void *)k_synth_fft_c<false, kVarPlain>,
+          (const void *)k_synth_fft_c<false, kVarSplit>, (const void *)k_synth_fft_c<false, kVarPrime>,
+          (const void *)k_synth_fft_c<true, kVarPlain>, (const void *)k_anal_fft_c<false, kVarPlain>,
+          (const void *)k_anal_fft_c<false, kVarSplit>, (const void *)k_anal_fft_c<false, kVarPrime>,
+          (const void *)k_anal_fft_c<true, kVarPlain>})
         (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
     kernels_configured = true;
 }
@@ -609,7 +623,7 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         if (lds_fft)
-            hipLaunchKernelGGL(g.fft.half ? k_synth_fft<true> : k_synth_fft<false>, dim3(g.P),
+            hipLaunchKernelGGL(SHTG_FFT_KERNEL(k_synth_fft, plan_variant(g.fft)), dim3(g.P),
                                dim3(plan_threads(g.fft)), lds, st, g.fft, fbuf, g.P, Lc, ru,
                                g.rows, nrows_out, grid, ld, ext_col, roww, kmax);
         else
@@ -657,7 +671,7 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
         ProfScope ps(SHTG_K_ANAL_FFT, st);
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         if (lds_fft)
-            hipLaunchKernelGGL(g.fft.half ? k_anal_fft<true> : k_anal_fft<false>, dim3(g.P),
+            hipLaunchKernelGGL(SHTG_FFT_KERNEL(k_anal_fft, plan_variant(g.fft)), dim3(g.P),
                                dim3(plan_threads(g.fft)), lds, st, g.fft, grid, ld, g.u, g.rows,
                                nrows, w, g.P, Lc, gbuf, ext_col, (const int *)g.kmax);
         else
@@ -717,11 +731,11 @@ static int launch_synthesis_c(const Geometry &g, const DevRecurrence &rt, const 
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         if (lds_fft)
-            hipLaunchKernelGGL((g.fft.half ? k_synth_fft_c<false, true> : k_synth_fft_c<false, false>), dim3(g.P),
+            hipLaunchKernelGGL((by_variant(plan_variant(g.fft), k_synth_fft_c<false, kVarPlain>, k_synth_fft_c<false, kVarSplit>, k_synth_fft_c<false, kVarPrime>)), dim3(g.P),
                                dim3(plan_threads(g.fft)), lds, st, g.fft, fb_p, fb_m,
                                g.P, Lc, g.u, g.rows, nrows_out, grid, ld, ext_col, (double2 *)nullptr, 0LL, 0);
         else
-            hipLaunchKernelGGL((k_synth_fft_c<true, false>), dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft, fb_p, fb_m,
+            hipLaunchKernelGGL((k_synth_fft_c<true, kVarPlain>), dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft, fb_p, fb_m,
                                g.P, Lc, g.u, g.rows, nrows_out, grid, ld, ext_col, fws, ws_stride(g.fft.m),
                                kChunkCap);
     }
@@ -751,11 +765,11 @@ static int launch_analysis_c(const Geometry &g, const DevRecurrence &rt, const D
         ProfScope ps(SHTG_K_ANAL_FFT, st);
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         if (lds_fft)
-            hipLaunchKernelGGL((g.fft.half ? k_anal_fft_c<false, true> : k_anal_fft_c<false, false>), dim3(g.P),
+            hipLaunchKernelGGL((by_variant(plan_variant(g.fft), k_anal_fft_c<false, kVarPlain>, k_anal_fft_c<false, kVarSplit>, k_anal_fft_c<false, kVarPrime>)), dim3(g.P),
                                dim3(plan_threads(g.fft)), lds, st, g.fft, grid, ld,
                                g.u, g.rows, g.nlat, g.w, g.P, Lc, gb_p, gb_m, (double2 *)nullptr, 0LL, 0);
         else
-            hipLaunchKernelGGL((k_anal_fft_c<true, false>), dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft, grid, ld,
+            hipLaunchKernelGGL((k_anal_fft_c<true, kVarPlain>), dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft, grid, ld,
                                g.u, g.rows, g.nlat, g.w, g.P, Lc, gb_p, gb_m, fws, ws_stride(g.fft.m), kChunkCap);
     }
     {
@@ -791,7 +805,8 @@ struct FftGroup {
     int threads = 256;       // synthesis workgroup size (heuristic or tuned)
     int threads_anal = 256;  // analysis workgroup size
     int min_threads = 1;     // the least its split plans allow (plan_min_threads)
-    bool half = false;       // its fields' rings run split plans (the kernels' split instantiation)
+    int max_threads = kFftMaxThreads;  // the largest its kernel instantiation allows
+    int variant = kVarPlain;           // its plans' kind: the kernels' instantiation
 };
 
 struct BatchPlan {
@@ -1014,7 +1029,8 @@ static int batch_create(int nfields, const int *lmax, const int *grid, int norm,
         g.lds = gl.lds;
         g.threads = g.threads_anal = gl.threads;
         g.min_threads = gl.min_threads;
-        g.half = gl.half;
+        g.variant = gl.variant;
+        g.max_threads = gl.max_threads;
         if (!g.items) return ST_ALLOC;
         b->fft.push_back(g);
     }
@@ -1130,7 +1146,7 @@ static int batch_synthesis(BatchPlan *b, hipStream_t st, const BTables &T)
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
         for (auto &g : b->fft)
-            hipLaunchKernelGGL(g.half ? kb_synth_fft<true> : kb_synth_fft<false>, dim3(xcd_blocks(g.nitems)),
+            hipLaunchKernelGGL(SHTG_FFT_KERNEL(kb_synth_fft, g.variant), dim3(xcd_blocks(g.nitems)),
                                dim3(g.threads), g.lds, st, b->fields,
                                g.items, g.nitems, b->fft_slots, T, b->fbuf, gout);
         if (b->fft_ws.nitems)
@@ -1151,7 +1167,7 @@ static int batch_analysis(BatchPlan *b, hipStream_t st, const BTables &T)
     {
         ProfScope ps(SHTG_K_ANAL_FFT, st);
         for (auto &g : b->fft)
-            hipLaunchKernelGGL(g.half ? kb_anal_fft<true> : kb_anal_fft<false>, dim3(xcd_blocks(g.nitems)),
+            hipLaunchKernelGGL(SHTG_FFT_KERNEL(kb_anal_fft, g.variant), dim3(xcd_blocks(g.nitems)),
                                dim3(g.threads_anal), g.lds, st, b->fields,
                                g.items, g.nitems, b->fft_slots, T, gin, b->fbuf);
         if (b->fft_ws.nitems)
@@ -1256,6 +1272,18 @@ static int batch_tune_legendre(BatchPlan *b, hipStream_t st, int reps, bool tune
 // workgroup (the group's items and LDS are unchanged) on the plan's bound
 // arrays, and keeps the fastest of each.  The runs overwrite the bound grids
 // and the plan's workspace; the caller reruns its transforms afterwards.
+// Workgroup sizes the ring-FFT autotuner tries.  Besides powers of two, 384
+// and 768 threads: a stage of B butterflies takes ceil(B / threads)
+// iterations between barriers, and for the Bluestein lengths of the largest
+// rings (m = 5760 .. 6144, one workgroup per CU) 768 threads leave fewer
+// lanes idle than 1024.
+constexpr int kFftShapes[] = {256, 384, 512, 768, 1024};
+
+static bool valid_fft_shape(int t, int min_threads, int max_threads)
+{
+    return t % kWave == 0 && t >= kFftShapes[0] && t <= max_threads && t >= min_threads;
+}
+
 static int batch_tune(BatchPlan *b, hipStream_t st, int reps, int flags)
 {
     const double *const *gin = (const double *const *)b->slot[2].dev();
@@ -1268,17 +1296,17 @@ static int batch_tune(BatchPlan *b, hipStream_t st, int reps, int flags)
         for (int dir = 0; dir < 2; ++dir) {
             float best = 1e30f;
             int best_t = dir ? g.threads_anal : g.threads;
-            for (int t : {kFftMaxThreads / 4, kFftMaxThreads / 2, kFftMaxThreads}) {
-                if (t < g.min_threads) continue;
+            for (int t : kFftShapes) {
+                if (t < g.min_threads || t > g.max_threads) continue;
                 float tmin = 1e30f;
                 for (int r = 0; r < reps; ++r) {
                     (void)hipEventRecord(e0, st);
                     if (dir == 0)
-                        hipLaunchKernelGGL(g.half ? kb_synth_fft<true> : kb_synth_fft<false>,
+                        hipLaunchKernelGGL(SHTG_FFT_KERNEL(kb_synth_fft, g.variant),
                                            dim3(xcd_blocks(g.nitems)), dim3(t), g.lds, st, b->fields,
                                            g.items, g.nitems, b->fft_slots, b->T, b->fbuf, gout);
                     else
-                        hipLaunchKernelGGL(g.half ? kb_anal_fft<true> : kb_anal_fft<false>,
+                        hipLaunchKernelGGL(SHTG_FFT_KERNEL(kb_anal_fft, g.variant),
                                            dim3(xcd_blocks(g.nitems)), dim3(t), g.lds, st, b->fields,
                                            g.items, g.nitems, b->fft_slots, b->T, gin, b->fbuf);
                     (void)hipEventRecord(e1, st);
@@ -2149,9 +2177,7 @@ int shtg_batch_set_shapes(shtg_batch_plan *plan, const int *threads_synth, const
     if (n != (int)b->fft.size()) return ST_DIMS;
     for (int i = 0; i < n; ++i)
         for (int t : {threads_synth[i], threads_anal[i]})
-            if ((t != kFftMaxThreads / 4 && t != kFftMaxThreads / 2 && t != kFftMaxThreads) ||
-                t < b->fft[i].min_threads)
-                return ST_BOUNDS;
+            if (!valid_fft_shape(t, b->fft[i].min_threads, b->fft[i].max_threads)) return ST_BOUNDS;
     for (int i = 0; i < n; ++i) {
         b->fft[i].threads = threads_synth[i];
         b->fft[i].threads_anal = threads_anal[i];

@@ -153,7 +153,9 @@ static void check_plan(const FftPlanHost &p, int n, int lc, bool split)
     long long prod = 1;
     for (int s = 0; s < p.nst; ++s) {
         const int R = p.radix[s];
-        CHECK(R >= 2 && R <= 8 && R != 6, "radix %d n %d stage %d", R, n, s);
+        // register radices and 7; a prime radix only as a direct plan's last stage
+        CHECK((R >= 2 && R <= 8 && R != 6) || (is_prime_radix(R) && !p.blue && s == p.nst - 1 && n <= kPrimeMaxLen),
+              "radix %d n %d stage %d", R, n, s);
         prod *= R;
         CHECK(p.stride[s] * (long long)prod == p.m, "stride n %d stage %d", n, s);
         CHECK(p.twoff[s] >= 0 && 2 * ((size_t)p.twoff[s] + p.stride[s]) <= p.twst.size(), "twiddles n %d stage %d",
@@ -422,9 +424,12 @@ static void check_layout_case(const std::vector<int> &lmax, const std::vector<in
                 CHECK(fft_in_lds(p.m) && g.lds >= (size_t)fft_lds_slots(p.m) * kComplexBytes && g.lds <= kMaxLds,
                       "%s: group %d LDS", name, group);
                 CHECK(g.threads >= plan_min_threads(p) && g.min_threads >= plan_min_threads(p) &&
-                          g.threads <= kFftMaxThreads && g.threads % kWave == 0,
+                          g.threads <= g.max_threads && g.max_threads == plan_max_threads(p) &&
+                          g.threads % kWave == 0,
                       "%s: group %d threads", name, group);
-                CHECK(g.half == (p.half != 0), "%s: group %d split", name, group);
+                CHECK(g.variant == plan_variant(p), "%s: group %d plan variant", name, group);
+                // prime-radix plans run in the LDS only
+                CHECK(plan_variant(p) != kVarPrime || p.n <= kPrimeMaxLen, "%s: prime plan length", name);
             } else {
                 CHECK(!fft_in_lds(p.m) && fl.ws_stride >= ws_stride(p.m), "%s: workspace stride", name);
             }

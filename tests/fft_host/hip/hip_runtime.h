@@ -8,12 +8,17 @@
 
 #define __device__
 #define __host__
+#define __constant__
 #define __forceinline__ inline
 
 struct double2 {
     double x, y;
 };
 inline double2 make_double2(double a, double b) { return {a, b}; }
+struct double4 {
+    double x, y, z, w;
+};
+inline double4 make_double4(double a, double b, double c, double d) { return {a, b, c, d}; }
 
 struct HostDim3 {
     unsigned x, y, z;

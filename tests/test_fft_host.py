@@ -23,7 +23,7 @@ TOL = 1e-13  # max |X - X_direct| / max |X_direct|, FP64 FFT
 @pytest.fixture(scope="module")
 def checker(tmp_path_factory):
     exe = str(tmp_path_factory.mktemp("fft_host") / "fft_host_check")
-    r = subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(HERE, "fft_host"),
+    r = subprocess.run(["g++", "-O2", "-std=c++17", "-Wno-attributes", "-I", os.path.join(HERE, "fft_host"),
                         "-Wno-unknown-pragmas", "-o", exe, SRC], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     return exe
@@ -68,3 +68,19 @@ def test_radix_coverage(checker):
 def test_high_degree_rings(checker):
     """Ring lengths of degrees 2047..2800 (the high-degree workload)."""
     _check(_run(checker, [4 * (L + 1) for L in (2047, 2048, 2399, 2400, 2799)]))
+
+
+def test_prime_radix_plans(checker):
+    """Direct plans ending with a prime radix (11 .. 23, the last stage, run
+    as symmetric sums over compile-time roots): ring lengths p * k for
+    smooth k, DH and GLQ alike, and their Bluestein fallbacks (two such
+    primes, or rings beyond the LDS)."""
+    primes = [11, 13, 17, 19, 23]
+    lengths = [p * k for p in primes for k in (1, 2, 3, 4, 6, 8, 12, 20, 64, 100, 420)]
+    rows = _run(checker, lengths + [11 * 13 * 4, 23 * 512 * 2])
+    _check(rows)
+    for row in rows[:len(lengths)]:
+        n, blue, last = int(row[0]), int(row[2]), int(row[-2])
+        if n <= 10240:
+            assert blue == 0 and last in primes, row
+    assert int(rows[-2][2]) == 1 and int(rows[-1][2]) == 1

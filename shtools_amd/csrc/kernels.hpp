@@ -1436,13 +1436,15 @@ __global__ __launch_bounds__(fft_max_threads(kVar)) void kb_synth_fft(const BFie
     const double4 *fb = fbuf + f.foff;
     const int send = min(it.y + nslots, f.P);
     // the next slot's ring sums are loaded while the current slot is in its
-    // FFT stages
+    // FFT stages (not for split plans, whose turns keep values in registers)
     SynthRing in;
-    synth_fft_load(fb, f.P, slot_orders(T.kmax, f.geo + it.y, f.L), it.y, in);
+    if constexpr (kVar != kVarSplit) synth_fft_load(fb, f.P, slot_orders(T.kmax, f.geo + it.y, f.L), it.y, in);
     for (int s = it.y; s < send; ++s) {
+        if constexpr (kVar == kVarSplit) synth_fft_load(fb, f.P, slot_orders(T.kmax, f.geo + s, f.L), s, in);
         synth_fft_fill(F, in, fb, f.P, slot_orders(T.kmax, f.geo + s, f.L), s, T.u[f.geo + s], lds);
         __syncthreads();
-        if (s + 1 < send) synth_fft_load(fb, f.P, slot_orders(T.kmax, f.geo + s + 1, f.L), s + 1, in);
+        if constexpr (kVar != kVarSplit)
+            if (s + 1 < send) synth_fft_load(fb, f.P, slot_orders(T.kmax, f.geo + s + 1, f.L), s + 1, in);
         synth_fft_run<kVar>(F, T.rows[f.geo + s], f.nlat, grid[it.x], f.nlong, 0,
                       T.roww ? T.roww[f.geo + s] : make_double2(1.0, 1.0), lds);
         __syncthreads();

@@ -62,19 +62,22 @@ inline int env_int(const char *name, int dflt)
 }
 
 // Whether a ring of length n with band limit lc runs the split Bluestein
-// plan (make_fft_plan): when its shorter work array brings the ring from the
-// global workspace into the LDS, for at most 15% more transform work (3
+// plan (make_fft_plan): when its whole plan fits fewer than two workgroups
+// per CU (one workgroup per CU, or the global workspace beyond the LDS) and
+// the split plan fits the LDS, for at most 15% more transform work (3
 // transforms of the split length against 2 of the whole one).  Plain and
 // batched transforms decide alike, so they stay bitwise equal.
 //
-// Rings that fit the LDS either way keep the whole plan: the split array is
-// about 2/3 as long but has as many stages, and its second inverse pass adds
-// more LDS traffic than the extra workgroups per CU recover (measured on the
-// 512-field benchmark: synthesis ring FFTs 13.9 -> 17.1 ms with split plans
-// wherever they raised the workgroups per CU, analysis 13.2 -> 13.4 ms; for
-// the one-workgroup-per-CU class alone, 14.0 -> 14.0 and 13.1 -> 12.6 ms).
+// Splitting the one-workgroup-per-CU class (rings of DH2 fields above degree
+// ~ 850) gives two workgroups per CU: measured on the 512-field benchmark,
+// tuned, synthesis ring FFTs 13.3 -> 12.9 ms and analysis 12.4 -> 12.0 ms
+// (whole step 48.9-49.4 -> 47.8 ms).  Rings that fit two or more workgroups
+// per CU keep the whole plan: the split array is about 2/3 as long but has as
+// many stages, and its second inverse pass adds more LDS traffic than the
+// extra workgroups per CU recover (also splitting the two-workgroup class:
+// 13.6 and 12.8 ms; splitting every in-LDS ring: synthesis 13.9 -> 17.1 ms).
 // SHTG_SPLIT_BELOW = k splits every ring whose whole plan fits fewer than k
-// workgroups per CU (default 1: only rings beyond the LDS); it changes
+// workgroups per CU (default 2; 1: only rings beyond the LDS); it changes
 // results by rounding.
 inline bool ring_split(int n, int lc)
 {
@@ -82,7 +85,7 @@ inline bool ring_split(int n, int lc)
     if (mh <= 0 || !fft_in_lds(mh)) return false;
     const int mf = bluestein_length(n + 2 * lc);
     if (3.0 * mh > 1.15 * 2.0 * mf) return false;
-    static const int below = env_int("SHTG_SPLIT_BELOW", 1);
+    static const int below = env_int("SHTG_SPLIT_BELOW", 2);
     return lds_per_cu(mf) < below;
 }
 

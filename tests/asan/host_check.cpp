@@ -204,8 +204,8 @@ static void check_plans()
         for (int nlong : {4 * (L + 1), 2 * (L + 1)}) {
             const FftPlanHost p = ring_plan(nlong, L);
             check_plan(p, nlong, L, ring_split(nlong, L));
-            CHECK(!p.half || (!fft_in_lds(bluestein_length(nlong + 2 * L)) && fft_in_lds(p.m)),
-                  "split only for rings beyond the LDS: n %d", nlong);
+            CHECK(!p.half || (lds_per_cu(bluestein_length(nlong + 2 * L)) < 2 && fft_in_lds(p.m)),
+                  "split only for rings of fewer than two workgroups per CU: n %d", nlong);
             ++ncases;
         }
         const FftPlanHost g = make_fft_plan(2 * L + 1);

@@ -75,6 +75,28 @@ int shtg_makegrid_glq(const double *cilm, int cilm_d1, int cilm_d2, int lmax,
                       const double *zero, int nzero, int norm, int csphase, int lmax_calc,
                       int extend, double *grid, int grid_d0, int grid_d1, void *stream);
 
+/* MakeGridGLQ with a table of Legendre functions (the plx branch):
+ * src/MakeGridGLQ.f95:1-2,408-420,441-465 (include/shtools.h:161-172 with
+ * plx present).  plx is a DEVICE array of (lmax+1) * (lmax+1)(lmax+2)/2
+ * doubles laid out as the reference's Fortran array plx(lmax+1, k):
+ * plx[k*(lmax+1) + i] = P_lm at node i, k = l(l+1)/2 + m, as
+ * shtg_shglq_plx writes it.  The table decides normalisation and phase;
+ * norm and csphase are checked as the reference checks them, and norm sets
+ * the value of a degree-0 field.  plx == NULL is exit status 5 (neither
+ * table nor nodes). */
+int shtg_makegrid_glq_plx(const double *cilm, int cilm_d1, int cilm_d2, int lmax,
+                          const double *plx, int norm, int csphase, int lmax_calc, int extend,
+                          double *grid, int grid_d0, int grid_d1, void *stream);
+
+/* SHExpandGLQ with a table of Legendre functions (the plx branch):
+ * src/SHExpandGLQ.f95:1-2,419-441,659-689 (include/shtools.h:150-159 with
+ * plx present).  lmax = grid_d0 - 1; `w` is a HOST array of >= lmax+1
+ * weights; plx a DEVICE table as for shtg_makegrid_glq_plx; norm selects
+ * the normalisation epilogue. */
+int shtg_shexpand_glq_plx(const double *grid, int grid_d0, int grid_d1, const double *w, int nw,
+                          const double *plx, int norm, int csphase, int lmax_calc, double *cilm,
+                          int cilm_d1, int cilm_d2, void *stream);
+
 /* SHExpandGLQ, on-the-fly branch: src/SHExpandGLQ.f95:1-2,445-649
  * (include/shtools.h:150-159; src/pyshtools.pyf:334-355).  lmax =
  * grid_d0 - 1; `w` and `zero` are HOST arrays of >= lmax+1 weights and
@@ -270,6 +292,15 @@ int shtg_batch_field_groups(shtg_batch_plan *plan, int *group, int n);
  * src/PreGLQ.f95:93-149):
  * zero[lmax+1] descending from ~+1, w[lmax+1]. */
 int shtg_shglq(int lmax, double *zero, double *w);
+/* SHGLQ with its table plx (include/shtools.h:142-148 with plx present;
+ * src/SHGLQ.f95:202-257): zero and w as shtg_shglq (HOST arrays), plx a
+ * DEVICE array of (lmax+1) * (lmax+1)(lmax+2)/2 doubles receiving the
+ * Legendre functions of normalisation norm and phase csphase at the nodes
+ * (plx[k*(lmax+1) + i], k = l(l+1)/2 + m; cnorm = 1: the complex
+ * normalisation, which omits the sqrt(2) of orders m >= 1; norm 3 has
+ * none), computed on the stream.  Not under graph capture (status 6). */
+int shtg_shglq_plx(int lmax, double *zero, double *w, double *plx, int norm, int csphase, int cnorm,
+                   void *stream);
 /* NGLQ, NGLQSH, NGLQSHN (include/shtools.h:1031-1033; src/PreGLQ.f95:153-233):
  * Gauss-Legendre points needed to integrate exactly a polynomial of degree
  * `degree` (ceiling((degree + 1) / 2)), a field of degree `degree`

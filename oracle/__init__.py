@@ -70,6 +70,7 @@ def ref_lib():
         L.ref_dhaj.argtypes = [i, i, dp, ip]
         L.ref_preglq.argtypes = [d, d, i, dp, dp, ip]
         L.ref_shglq.argtypes = [i, dp, dp, ip]
+        L.ref_shglq_plx.argtypes = [i, dp, dp, dp, i, i, i, ip]
         L.ref_glqgridcoord.argtypes = [i, i, dp, dp, ip, ip, ip]
         L.ref_plm.argtypes = [i, i, d, i, i, dp, ip]
         L.ref_normalgravity.argtypes = [d, d, d, d, d]
@@ -97,6 +98,15 @@ def ref_SHGLQ(lmax):
     """(zero, w, exit status) of the reference's SHGLQ (no plx table)."""
     zero, w = np.zeros(lmax + 1), np.zeros(lmax + 1)
     return zero, w, _ref_call(ref_lib().ref_shglq, lmax, _p(zero), _p(w))
+
+
+def ref_SHGLQ_plx(lmax, norm=1, csphase=1, cnorm=0):
+    """(zero, w, plx, exit status) of the reference's SHGLQ with its plx
+    table, plx as an array [k, i] (the Fortran plx(i, k) in C order)."""
+    zero, w = np.zeros(lmax + 1), np.zeros(lmax + 1)
+    plx = np.zeros(((lmax + 1) * (lmax + 2) // 2, lmax + 1))
+    st = _ref_call(ref_lib().ref_shglq_plx, lmax, _p(zero), _p(w), _p(plx), norm, csphase, cnorm)
+    return zero, w, plx, st
 
 
 def ref_PreGLQ(x1, x2, n):
@@ -146,6 +156,9 @@ def lib():
         L.or_shexpandglq_par.argtypes = [dp, i, i, i, dp, i, dp, i, i, i, i, dp, i, i]
         L.or_makegridglq.argtypes = [dp, i, i, i, dp, i, i, i, i, i, dp, i, i]
         L.or_shexpandglq.argtypes = [dp, i, i, i, dp, i, dp, i, i, i, i, dp, i, i]
+        L.or_plx_table.argtypes = [i, dp, i, i, i, i, dp]
+        L.or_makegridglq_plx.argtypes = [dp, i, i, i, dp, i, i, i, i, dp, i, i]
+        L.or_shexpandglq_plx.argtypes = [dp, i, i, i, dp, i, dp, i, i, i, dp, i, i]
         L.or_makegriddhc.argtypes = [dp, i, i, i, i, i, i, i, i, dp, i, i]
         L.or_shexpanddhc.argtypes = [dp, i, i, i, i, i, i, dp, i, i]
         L.or_makegridglqc.argtypes = [dp, i, i, i, dp, i, i, i, i, i, dp, i, i]
@@ -256,6 +269,49 @@ def SHExpandGLQ(gridglq, w, zero, norm=1, csphase=1, lmax_calc=None, parallel=Fa
     _check(fn(_p(grid), grid.shape[0], grid.shape[1], lmax, _p(w), w.size,
               _p(zero), zero.size, norm, csphase, lmax_calc, _p(cilm),
               lmax_calc + 1, lmax_calc + 1))
+    return cilm
+
+
+def plx_table(lmax, zero, norm=1, csphase=1, cnorm=0):
+    """SHGLQ's plx table at the nodes zero (src/SHGLQ.f95:202-257), as the
+    array plx[k, i] = P_lm(zero[i]), k = l(l+1)/2 + m (the Fortran array
+    plx(i, k) in C order)."""
+    zero = np.ascontiguousarray(zero, dtype=np.float64)
+    plx = np.zeros(((lmax + 1) * (lmax + 2) // 2, lmax + 1))
+    _check(lib().or_plx_table(lmax, _p(zero), zero.size, norm, csphase, cnorm, _p(plx)))
+    return plx
+
+
+def MakeGridGLQ_plx(cilm, plx, lmax=None, norm=1, csphase=1, lmax_calc=None, extend=0):
+    """MakeGridGLQ with the table plx in place of the nodes
+    (src/MakeGridGLQ.f95:441-465)."""
+    cilm = np.ascontiguousarray(cilm, dtype=np.float64)
+    plx = np.ascontiguousarray(plx, dtype=np.float64)
+    if lmax is None:
+        lmax = cilm.shape[1] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    assert plx.shape == ((lmax + 1) * (lmax + 2) // 2, lmax + 1)
+    g0, g1 = lmax + 1, 2 * lmax + 1 + extend
+    grid = np.zeros((g0, g1))
+    _check(lib().or_makegridglq_plx(_p(cilm), cilm.shape[1], cilm.shape[2], lmax, _p(plx), norm, csphase,
+                                    lmax_calc, extend, _p(grid), g0, g1))
+    return grid
+
+
+def SHExpandGLQ_plx(gridglq, w, plx, norm=1, csphase=1, lmax_calc=None):
+    """SHExpandGLQ with the table plx in place of the nodes
+    (src/SHExpandGLQ.f95:419-441)."""
+    grid = np.ascontiguousarray(gridglq, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    plx = np.ascontiguousarray(plx, dtype=np.float64)
+    lmax = grid.shape[0] - 1
+    assert plx.shape == ((lmax + 1) * (lmax + 2) // 2, lmax + 1)
+    if lmax_calc is None:
+        lmax_calc = lmax
+    cilm = np.zeros((2, lmax_calc + 1, lmax_calc + 1))
+    _check(lib().or_shexpandglq_plx(_p(grid), grid.shape[0], grid.shape[1], lmax, _p(w), w.size, _p(plx), norm,
+                                    csphase, lmax_calc, _p(cilm), lmax_calc + 1, lmax_calc + 1))
     return cilm
 
 

@@ -39,6 +39,20 @@ subroutine ref_shglq(lmax, zero, w, status) bind(C, name="ref_shglq")
     call SHGLQ(lmax, zero(1:lmax + 1), w(1:lmax + 1), exitstatus=status)
 end subroutine ref_shglq
 
+subroutine ref_shglq_plx(lmax, zero, w, plx, norm, csphase, cnorm, status) bind(C, name="ref_shglq_plx")
+    ! src/SHGLQ.f95 with the plx table: plx(lmax + 1, (lmax + 1)(lmax + 2)/2)
+    ! in Fortran order
+    use, intrinsic :: iso_c_binding
+    use SHTOOLS, only: SHGLQ
+    implicit none
+    integer(c_int), value :: lmax, norm, csphase, cnorm
+    real(c_double), intent(out) :: zero(*), w(*)
+    real(c_double), intent(out) :: plx(lmax + 1, *)
+    integer(c_int), intent(out) :: status
+    call SHGLQ(lmax, zero(1:lmax + 1), w(1:lmax + 1), plx=plx(:, 1:(lmax + 1) * (lmax + 2) / 2), norm=norm, &
+               csphase=csphase, cnorm=cnorm, exitstatus=status)
+end subroutine ref_shglq_plx
+
 subroutine ref_glqgridcoord(lmax, extend, lat, lon, nlat, nlong, status) bind(C, name="ref_glqgridcoord")
     ! src/GLQGridCoord.f95: lat(1 : lmax + 1), lon(1 : 2 lmax + 1 + extend)
     use iso_c_binding

@@ -363,9 +363,10 @@ static double pmm_last(int norm, int phase, double pmm, int lc, const double *sq
 }
 
 /* All P_lm(z), l,m <= lmax, with the transforms' recursion, seeds and
- * 1e-280 scaling; p[l*(lmax+1)+m].  Used to pin the recursion against
- * closed forms. */
-int or_plm(int lmax, int norm, int csphase, double z, double *p)
+ * 1e-280 scaling; p[l*(lmax+1)+m].  cplx: the complex normalisation, whose
+ * sectoral chain starts without sqrt(2) (src/PlmBar.f95:242-253; norm 3 has
+ * none either way). */
+static int plm_eval(int lmax, int norm, int csphase, int cplx, double z, double *p)
 {
     if (norm < 1 || norm > 4 || (csphase != 1 && csphase != -1) || lmax < 0) return 2;
     tables t;
@@ -384,7 +385,7 @@ int or_plm(int lmax, int norm, int csphase, double z, double *p)
         pv = FF(&t, ff1, l, 0) * z * pm1 - FF(&t, ff2, l, 0) * pm2;
         p[l * L1] = pv; pm2 = pm1; pm1 = pv;
     }
-    double pmm = pmm_start(norm, sqr), rescalem = 1.0 / 1.0e-280;
+    double pmm = cplx ? pmm_start_c(norm) : pmm_start(norm, sqr), rescalem = 1.0 / 1.0e-280;
     for (int m = 1; m <= lmax - 1; m++) {
         rescalem = rescalem * u;
         pm2 = pmm_step(norm, csphase, &pmm, m, sqr);
@@ -400,6 +401,45 @@ int or_plm(int lmax, int norm, int csphase, double z, double *p)
     rescalem = rescalem * u;
     p[lmax * L1 + lmax] = pmm_last(norm, csphase, pmm, lmax, sqr, rescalem);
     free_tables(&t);
+    return 0;
+}
+
+/* Used to pin the recursion against closed forms. */
+int or_plm(int lmax, int norm, int csphase, double z, double *p)
+{
+    return plm_eval(lmax, norm, csphase, 0, z, p);
+}
+
+/* The table of SHGLQ's optional argument plx (src/SHGLQ.f95:202-257): the
+ * Legendre functions at the Gauss-Legendre nodes zero[0..lmax], stored as
+ * the Fortran array plx(lmax+1, (lmax+1)(lmax+2)/2) in C order,
+ * plx[k * (lmax+1) + i] with k = l(l+1)/2 + m.  Nodes i < (n+1)/2 are
+ * evaluated (the reference calls PlmBar / PlmSchmidt / PLegendreA / PlmON,
+ * cnorm passed to all but PLegendreA); node n-1-i takes them times
+ * (-1)^(l-m).  The functions here are those of the transforms' recursion,
+ * which the tests compare with the reference's own table. */
+int or_plx_table(int lmax, const double *zero, int nzero, int norm, int csphase, int cnorm, double *plx)
+{
+    if (lmax < 0) return 2;
+    if (nzero < lmax + 1) return 1;
+    if (norm < 1 || norm > 4) return 2;
+    if (csphase != 1 && csphase != -1) return 2;
+    const int n = lmax + 1, L1 = lmax + 1;
+    double *p = (double *)malloc(sizeof(double) * (size_t)L1 * L1);
+    if (!p) return 3;
+    for (int i = 1; i <= (n + 1) / 2; i++) {
+        int st = plm_eval(lmax, norm, csphase, norm != 3 && cnorm == 1, zero[i - 1], p);
+        if (st) { free(p); return st; }
+        const int mirror = !(i == (n + 1) / 2 && n % 2 != 0);
+        const int i_s = n + 1 - i;
+        for (int l = 0; l <= lmax; l++)
+            for (int m = 0; m <= l; m++) {
+                const size_t k = (size_t)l * (l + 1) / 2 + m;
+                plx[k * n + (i - 1)] = p[(size_t)l * L1 + m];
+                if (mirror) plx[k * n + (i_s - 1)] = plx[k * n + (i - 1)] * (double)fsym(l, m);
+            }
+    }
+    free(p);
     return 0;
 }
 
@@ -1180,6 +1220,120 @@ int or_shexpandglq(const double *grid, int g0, int g1, int lmax, const double *w
     free(buf); free(cc); free(f1); free(f2); free(cf);
     plan_destroy(plan);
     free_tables(&t);
+    return 0;
+}
+
+/* ------------------------------------------- GLQ with a plx table */
+
+/* MakeGridGLQ with plx (src/MakeGridGLQ.f95:107-270 checks, :408-420 for
+ * lmax_comp = 0, :441-465): every row from the table, degrees summed from
+ * lmax_comp down to 0.  plx as or_plx_table lays it out (first dimension
+ * lmax + 1). */
+int or_makegridglq_plx(const double *cilm, int d1, int d2, int lmax, const double *plx, int norm, int csphase,
+                       int lmax_calc, int extend, double *grid, int g0, int g1)
+{
+    int nlong = 2 * lmax + 1, nlat = lmax + 1, nlong_out;
+    if (extend == 0) nlong_out = nlong; else if (extend == 1) nlong_out = nlong + 1; else return 2;
+    if (g1 < nlong_out || g0 < nlat) return 1;
+    if (!plx) return 5;
+    if (norm < 1 || norm > 4) return 2;
+    if (csphase != -1 && csphase != 1) return 2;
+    if (lmax_calc > lmax) return 2;
+    int lc = lmax;
+    if (d2 - 1 < lc) lc = d2 - 1;
+    if (d1 - 1 < lc) lc = d1 - 1;
+    if (lmax_calc < lc) lc = lmax_calc;
+    double pi = acos(-1.0);
+#define G(i, k) grid[(size_t)(i) * g1 + (k)]
+    if (lc == 0) {
+        double pm2 = (norm == 4) ? 1.0 / sqrt(4.0 * pi) : 1.0;
+        for (int i = 0; i < nlat; i++)
+            for (int k = 0; k < nlong_out; k++) G(i, k) = cilm[0] * pm2;
+        return 0;
+    }
+    double *cf = to_fortran_order(cilm, d1, d2, lc);
+    if (!cf) return 3;
+    int ncl = nlong / 2 + 1;   /* == lmax + 1 */
+#pragma omp parallel
+    {
+    fftplan *plan = plan_create(nlong);
+    cplx *coef = (cplx *)malloc(sizeof(cplx) * ncl);
+    cplx *buf = (cplx *)malloc(sizeof(cplx) * nlong);
+    double *row = (double *)malloc(sizeof(double) * nlong);
+    double *cr = (double *)malloc(sizeof(double) * 2 * (lmax + 2));
+    double *ci = cr + (lmax + 2);
+#pragma omp for schedule(dynamic, 1)
+    for (int i = 1; i <= nlat; i++) {
+        double coef0 = 0.0;
+        for (int k = 0; k < 2 * (lmax + 2); k++) cr[k] = 0.0;
+        for (int l = lc; l >= 0; l--) {
+            size_t k = (size_t)l * (l + 1) / 2;   /* m = 0 */
+            coef0 = coef0 + CF(0, l, 0) * plx[k * nlat + (i - 1)];
+            for (int m = 1; m <= l; m++) {
+                k = (size_t)l * (l + 1) / 2 + m;
+                const double p = plx[k * nlat + (i - 1)];
+                cr[m] = cr[m] + CF(0, l, m) * p / 2.0;
+                ci[m] = ci[m] + (-CF(1, l, m)) * p / 2.0;
+            }
+        }
+        cr[0] = coef0; ci[0] = 0.0;
+        pack_half(coef, cr, ci, lmax + 1, ncl);
+        c2r(plan, coef, buf, row);
+        for (int k = 0; k < nlong; k++) G(i - 1, k) = row[k];
+    }
+    free(coef); free(buf); free(row); free(cr);
+    plan_destroy(plan);
+    }
+    if (extend == 1)
+        for (int i = 0; i < nlat; i++) G(i, nlong_out - 1) = G(i, 0);
+#undef G
+    free(cf);
+    return 0;
+}
+
+/* SHExpandGLQ with plx (src/SHExpandGLQ.f95:100-250 checks, :419-441 and the
+ * epilogue :674-702): every ring's weighted Fourier coefficients times the
+ * table, rings in order, then the normalisation. */
+int or_shexpandglq_plx(const double *grid, int g0, int g1, int lmax, const double *w, int nw,
+                       const double *plx, int norm, int csphase, int lmax_calc, double *cilm, int d1, int d2)
+{
+    int lc, nlong = 2 * lmax + 1, nlat = lmax + 1;
+    if (lmax_calc > lmax) return 2;
+    lc = lmax_calc < lmax ? lmax_calc : lmax;
+    if (d1 < lc + 1 || d2 < lc + 1) return 1;
+    if (g1 < 2 * lmax + 1 || g0 < lmax + 1) return 1;
+    if (nw < lmax + 1) return 1;
+    if (!plx) return 5;
+    if (norm < 1 || norm > 4) return 2;
+    if (csphase != -1 && csphase != 1) return 2;
+    double pi = acos(-1.0);
+    for (size_t k = 0; k < (size_t)2 * d1 * d2; k++) cilm[k] = 0.0;
+    double *cf = (double *)calloc((size_t)2 * (lc + 1) * (lc + 1), sizeof(double));
+    if (!cf) return 3;
+    fftplan *plan = plan_create(nlong);
+    cplx *buf = (cplx *)malloc(sizeof(cplx) * nlong);
+    cplx *cc = (cplx *)malloc(sizeof(cplx) * (nlong / 2 + 1));
+    int nf = lmax + 1;
+    double *f1 = (double *)calloc((size_t)2 * nf, sizeof(double));
+#define GR(i, k) grid[(size_t)(i) * g1 + (k)]
+    for (int i = 1; i <= nlat; i++) {
+        r2c(plan, &GR(i - 1, 0), buf, cc);
+        for (int m = 0; m <= lmax; m++) {
+            f1[m] = 2 * w[i - 1] * pi * creal(cc[m]) / (double)nlong;
+            f1[nf + m] = -2 * w[i - 1] * pi * cimag(cc[m]) / (double)nlong;
+        }
+        size_t k = 0;
+        for (int l = 0; l <= lc; l++)
+            for (int m = 0; m <= l; m++, k++) {
+                const double p = plx[k * nlat + (i - 1)];
+                CF(0, l, m) = CF(0, l, m) + p * f1[m];
+                CF(1, l, m) = CF(1, l, m) + p * f1[nf + m];
+            }
+    }
+#undef GR
+    anal_epilogue(norm, lc, cf, cilm, d1, d2);
+    free(buf); free(cc); free(f1); free(cf);
+    plan_destroy(plan);
     return 0;
 }
 

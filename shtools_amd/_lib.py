@@ -52,6 +52,11 @@ SIGNATURES = {
     "shtg_nglqsh": [_c_int],
     "shtg_nglqshn": [_c_int, _c_int],
     "shtg_shglq": [_c_int, _dp, _dp],
+    "shtg_shglq_plx": [_c_int, _dp, _dp, _vp, _c_int, _c_int, _c_int, _vp],
+    "shtg_makegrid_glq_plx": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int,
+                              _vp],
+    "shtg_shexpand_glq_plx": [_vp, _c_int, _c_int, _dp, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _c_int, _c_int,
+                              _vp],
     "shtg_dhaj": [_c_int, _c_int, _dp],
     "shtg_glqgridcoord": [_c_int, _c_int, _dp, _dp],
     "shtg_plm": [_c_int, _c_int, _c_int, ctypes.c_double, _dp],

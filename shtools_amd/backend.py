@@ -39,6 +39,7 @@ __all__ = [
     "PlmBar", "PlmSchmidt", "PlmON", "PLegendreA",
     "SHExpandDHC", "MakeGridDHC", "SHExpandGLQC", "MakeGridGLQC", "MakeGradientDH", "SHRotateRealCoef",
     "MakeGridDH_adjoint", "SHExpandDH_adjoint", "MakeGridGLQ_adjoint", "SHExpandGLQ_adjoint",
+    "SHGLQ_plx", "MakeGridGLQ_plx", "SHExpandGLQ_plx",
 ]
 
 
@@ -187,6 +188,37 @@ def SHExpandDH(griddh, norm=1, sampling=1, csphase=1, lmax_calc=None):
 
 # ------------------------------------------------------------ GLQ
 
+def _plx_dev(plx, lmax, dev):
+    """The table plx of degree lmax as a contiguous float64 tensor on dev,
+    shape ((lmax+1)(lmax+2)/2, lmax+1) (the reference's Fortran array
+    plx(lmax+1, k) in C order)."""
+    torch = _torch()
+    t = plx.to(dev) if _is_tensor(plx) else torch.from_numpy(np.asarray(plx, dtype=np.float64)).to(dev)
+    t = t.to(torch.float64).contiguous()
+    if tuple(t.shape) != ((lmax + 1) * (lmax + 2) // 2, lmax + 1):
+        raise SHToolsError(_status_message(1))
+    return t
+
+
+def SHGLQ_plx(lmax, norm=1, csphase=1, cnorm=0, device=None):
+    """SHGLQ with its table of Legendre functions (src/SHGLQ.f95:202-257):
+    (zero, w, plx), zero and w numpy arrays as SHGLQ returns them, plx a
+    device tensor of shape ((lmax+1)(lmax+2)/2, lmax+1) holding P_lm at node
+    i in plx[l(l+1)/2 + m, i], for MakeGridGLQ(..., plx=plx) and
+    SHExpandGLQ(..., plx=plx)."""
+    torch = _torch()
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    if lmax < 0:
+        raise SHToolsError(_status_message(2))
+    zero = np.zeros(lmax + 1)
+    w = np.zeros(lmax + 1)
+    plx = torch.empty(((lmax + 1) * (lmax + 2) // 2, lmax + 1), dtype=torch.float64, device=dev)
+    with torch.cuda.device(dev):
+        _check(_lib.lib().shtg_shglq_plx(int(lmax), _dptr(zero), _dptr(w), plx.data_ptr(), int(norm), int(csphase),
+                                         int(cnorm), _stream(dev)))
+    return zero, w, plx
+
+
 def MakeGridGLQ(cilm, zero, lmax=None, norm=1, csphase=1, lmax_calc=None, extend=0):
     """Evaluate real coefficients on a Gauss-Legendre grid (src/MakeGridGLQ.f95)."""
     torch = _torch()
@@ -210,6 +242,60 @@ def MakeGridGLQ(cilm, zero, lmax=None, norm=1, csphase=1, lmax_calc=None, extend
                                           g0, g1, _stream(dev))
     _check(st)
     return _finish(grid, as_tensor)
+
+
+def MakeGridGLQ_plx(cilm, plx, lmax=None, norm=1, csphase=1, lmax_calc=None, extend=0):
+    """MakeGridGLQ with the table plx of SHGLQ_plx in place of the nodes (the
+    reference's plx branch, src/MakeGridGLQ.f95:441-465): the functions are
+    read from the table, which fixes normalisation and phase (norm sets the
+    value of a degree-0 field)."""
+    torch = _torch()
+    as_tensor = _out_kind(cilm)
+    dev = _device_of(cilm)
+    c = _to_dev(cilm, dev)
+    if c.dim() != 3 or c.shape[0] < 2:
+        raise SHToolsError(_status_message(1))
+    if lmax is None:
+        lmax = c.shape[1] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    if extend not in (0, 1) or lmax < 0:
+        raise SHToolsError(_status_message(2))
+    p = _plx_dev(plx, lmax, dev)
+    g0, g1 = lmax + 1, 2 * lmax + 1 + extend
+    grid = torch.empty((g0, g1), dtype=torch.float64, device=dev)
+    with torch.cuda.device(dev):
+        st = _lib.lib().shtg_makegrid_glq_plx(c.data_ptr(), c.shape[1], c.shape[2], int(lmax), p.data_ptr(),
+                                              int(norm), int(csphase), int(lmax_calc), int(extend), grid.data_ptr(),
+                                              g0, g1, _stream(dev))
+    _check(st)
+    return _finish(grid, as_tensor)
+
+
+def SHExpandGLQ_plx(gridglq, w, plx, norm=1, csphase=1, lmax_calc=None):
+    """SHExpandGLQ with the table plx of SHGLQ_plx in place of the nodes (the
+    reference's plx branch, src/SHExpandGLQ.f95:419-441); norm selects the
+    normalisation epilogue."""
+    torch = _torch()
+    as_tensor = _out_kind(gridglq)
+    dev = _device_of(gridglq)
+    g = _to_dev(gridglq, dev)
+    if g.dim() != 2:
+        raise SHToolsError(_status_message(1))
+    wq = _host_f64(w)
+    lmax = g.shape[0] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    if lmax_calc < 0:
+        raise SHToolsError(_status_message(2))
+    p = _plx_dev(plx, lmax, dev)
+    cilm = torch.empty((2, lmax_calc + 1, lmax_calc + 1), dtype=torch.float64, device=dev)
+    with torch.cuda.device(dev):
+        st = _lib.lib().shtg_shexpand_glq_plx(g.data_ptr(), g.shape[0], g.shape[1], _dptr(wq), wq.size, p.data_ptr(),
+                                              int(norm), int(csphase), int(lmax_calc), cilm.data_ptr(),
+                                              lmax_calc + 1, lmax_calc + 1, _stream(dev))
+    _check(st)
+    return _finish(cilm, as_tensor)
 
 
 def SHExpandGLQ(gridglq, w, zero, norm=1, csphase=1, lmax_calc=None):

@@ -124,4 +124,23 @@ inline int check_shexpand_glq(int grid_d0, int grid_d1, const double *w, int nw,
     return ST_OK;
 }
 
+// The same checks with the table plx in place of the nodes (the
+// reference's plx branch): the table's size follows from lmax, and no table
+// at all is exit status 5 as for missing nodes (src/MakeGridGLQ.f95:162-190,
+// src/SHExpandGLQ.f95:166-193).
+inline int check_makegrid_glq_plx(int cilm_d1, int cilm_d2, int lmax, const double *plx, int norm, int csphase,
+                                  int lmax_calc, int extend, int grid_d0, int grid_d1, Shape &sh)
+{
+    if (lmax < 0 || lmax > kMaxArgDegree) return ST_BOUNDS;
+    return check_makegrid_glq(cilm_d1, cilm_d2, lmax, lmax + 1, plx, norm, csphase, lmax_calc, extend, grid_d0,
+                              grid_d1, sh);
+}
+
+inline int check_shexpand_glq_plx(int grid_d0, int grid_d1, const double *w, int nw, const double *plx, int norm,
+                                  int csphase, int lmax_calc, int cilm_d1, int cilm_d2, Shape &sh)
+{
+    return check_shexpand_glq(grid_d0, grid_d1, w, nw, plx, grid_d0, norm, csphase, lmax_calc, cilm_d1, cilm_d2,
+                              sh);
+}
+
 }  // namespace shtg

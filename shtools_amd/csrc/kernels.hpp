@@ -109,6 +109,11 @@ __device__ __forceinline__ void unpack_tile(const double2 *__restrict__ cbuf, co
 // (1 for m = 0, where the recursion is unscaled: src/MakeGridDH.f95:516-540).
 __device__ __forceinline__ double order_rescale(double u, int m) { return m == 0 ? 1.0 : scaled_upow(u, m); }
 
+// The rescale a ring FFT applies to order k of a slot with sin(theta) = u:
+// order_rescale, or 1 where u < 0 marks sums of actual function values
+// (tables of the functions, no extended-range scaling).
+__device__ __forceinline__ double ring_rescale(double u, int k) { return u < 0.0 ? 1.0 : order_rescale(u, k); }
+
 // Synthesis sums of order m for the ring pairs slot0 + k*64 + lane (one
 // wave).  al: recursion row of order m (al[j] = alpha_{m+j}); cp: packed
 // coefficients of order m; frow: fbuf row of order m.  The sums are stored
@@ -424,10 +429,10 @@ __device__ __forceinline__ void synth_fft_fill(const FftDev &F, const SynthRing 
 #pragma unroll
     for (int j = 0; j < kSynthPre; ++j) {
         const int k = threadIdx.x + j * blockDim.x;
-        if (k <= Lc) put(k, k == 0 ? 1.0 : scaled_upow(u, k), in.f[j]);
+        if (k <= Lc) put(k, ring_rescale(u, k), in.f[j]);
     }
     for (int k = threadIdx.x + kSynthPre * blockDim.x; k <= Lc; k += blockDim.x)
-        put(k, scaled_upow(u, k), fbuf[(size_t)k * P + s]);
+        put(k, ring_rescale(u, k), fbuf[(size_t)k * P + s]);
 }
 
 // DFT of the filled array and the rows out; the north and south rows are
@@ -589,7 +594,8 @@ __device__ __forceinline__ void anal_fft_fill(const FftDev &F, const AnalRing &i
 }
 
 // The orders 0 .. Lc of the transformed ring pair: split into the two rings,
-// weighted, times the order's rescale factor, into gbuf.
+// weighted, times the order's rescale factor, into gbuf as the even and odd
+// combinations of the two rings (u < 0: no rescale, the two rings apart).
 __device__ __forceinline__ void anal_fft_extract(const FftDev &F, int2 rw, int nrows, double2 w, double u, int P,
                                                  int Lc, int s, double4 *__restrict__ gbuf, const double2 *lds)
 {
@@ -599,7 +605,7 @@ __device__ __forceinline__ void anal_fft_extract(const FftDev &F, int2 rw, int n
     const double wn = north ? w.x : 0.0, ws = south ? w.y : 0.0;
     // orders interleave over the threads (a wave reads consecutive elements)
     for (int mm = threadIdx.x; mm <= Lc; mm += blockDim.x) {
-        const double r = (mm == 0) ? 1.0 : scaled_upow(u, mm);
+        const double r = ring_rescale(u, mm);
         const double2 c = fft_chirp(F, mm);
         const double2 zm = fft_get_c(lds, F, mm, mm, c);
         const int nm = mm == 0 ? 0 : n - mm;
@@ -607,7 +613,11 @@ __device__ __forceinline__ void anal_fft_extract(const FftDev &F, int2 rw, int n
         const double xr = 0.5 * (zm.x + zn.x), xi = 0.5 * (zm.y - zn.y);
         const double yr = 0.5 * (zm.y + zn.y), yi = 0.5 * (zn.x - zm.x);
         const double f1c = wn * xr, f1s = -wn * xi, f2c = ws * yr, f2s = -ws * yi;
-        gbuf[(size_t)mm * P + s] = make_double4((f1c + f2c) * r, (f1s + f2s) * r, (f1c - f2c) * r, (f1s - f2s) * r);
+        if (u < 0.0)
+            gbuf[(size_t)mm * P + s] = make_double4(f1c, f1s, f2c, f2s);
+        else
+            gbuf[(size_t)mm * P + s] =
+                make_double4((f1c + f2c) * r, (f1s + f2s) * r, (f1c - f2c) * r, (f1s - f2s) * r);
     }
 }
 
@@ -731,8 +741,8 @@ __global__ __launch_bounds__(fft_max_threads(kVar)) void k_synth_fft(FftDev F, c
 {
     extern __shared__ double2 lds[];
     const int s = blockIdx.x;
-    synth_fft_tile<kVar>(F, fbuf, P, slot_orders(kmax, s, Lc), s, ringu[s], rows[s], nrows, grid, ld, ext_col,
-                           roww ? roww[s] : make_double2(1.0, 1.0), lds);
+    synth_fft_tile<kVar>(F, fbuf, P, slot_orders(kmax, s, Lc), s, ringu ? ringu[s] : -1.0, rows[s], nrows, grid, ld,
+                         ext_col, roww ? roww[s] : make_double2(1.0, 1.0), lds);
 }
 
 template <int kVar>
@@ -745,8 +755,8 @@ __global__ __launch_bounds__(fft_max_threads(kVar)) void k_anal_fft(FftDev F, co
 {
     extern __shared__ double2 lds[];
     const int s = blockIdx.x;
-    anal_fft_tile<kVar>(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, slot_orders(kmax, s, Lc), s, gbuf,
-                          ext_col, lds);
+    anal_fft_tile<kVar>(F, grid, ld, rows[s], nrows, wts[s], ringu ? ringu[s] : -1.0, P, slot_orders(kmax, s, Lc), s,
+                        gbuf, ext_col, lds);
 }
 
 // Rings whose FFT work array does not fit the LDS of a CU: the same tiles
@@ -763,8 +773,8 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_synth_fft_ws(FftDev F, const
     extern __shared__ double2 chunk[];
     double2 *x = ws + blockIdx.x * ws_stride;
     for (int s = blockIdx.x; s < P; s += gridDim.x) {
-        synth_fft_tile(F, fbuf, P, slot_orders(kmax, s, Lc), s, ringu[s], rows[s], nrows, grid, ld, ext_col,
-                       roww ? roww[s] : make_double2(1.0, 1.0), x, chunk, chunk_cap);
+        synth_fft_tile(F, fbuf, P, slot_orders(kmax, s, Lc), s, ringu ? ringu[s] : -1.0, rows[s], nrows, grid, ld,
+                       ext_col, roww ? roww[s] : make_double2(1.0, 1.0), x, chunk, chunk_cap);
         __syncthreads();
     }
 }
@@ -780,8 +790,8 @@ __global__ __launch_bounds__(kFftMaxThreads) void k_anal_fft_ws(FftDev F, const 
     extern __shared__ double2 chunk[];
     double2 *x = ws + blockIdx.x * ws_stride;
     for (int s = blockIdx.x; s < P; s += gridDim.x) {
-        anal_fft_tile(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, slot_orders(kmax, s, Lc), s, gbuf, ext_col,
-                      x, chunk, chunk_cap);
+        anal_fft_tile(F, grid, ld, rows[s], nrows, wts[s], ringu ? ringu[s] : -1.0, P, slot_orders(kmax, s, Lc), s,
+                      gbuf, ext_col, x, chunk, chunk_cap);
         __syncthreads();
     }
 }
@@ -835,6 +845,146 @@ __global__ __launch_bounds__(kThreads) void k_zero_outside(double *__restrict__ 
         const int r = (int)(e / cols), c = (int)(e - (long long)r * cols);
         if (r >= r0 || c >= c0) a[r * ld + c] = 0.0;
     }
+}
+
+// Set every element of a rows x cols array (leading dimension ld) to
+// src[0] * v (src == nullptr: v).
+__global__ __launch_bounds__(kThreads) void k_fill(double *__restrict__ a, int rows, int cols, long long ld,
+                                                   const double *__restrict__ src, double v)
+{
+    if (src) v = src[0] * v;
+    const long long total = (long long)rows * cols;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int r = (int)(e / cols), c = (int)(e - (long long)r * cols);
+        a[r * ld + c] = v;
+    }
+}
+
+// ===================================================== precomputed tables
+//
+// The plx branch of the GLQ routines: a table of the Legendre functions at
+// the Gauss-Legendre nodes, the Fortran array plx(n, (L+1)(L+2)/2) of
+// src/SHGLQ.f95, here plx[k * ld + i] for node i and k = l(l+1)/2 + m
+// (ld = n = L + 1), read by the synthesis and analysis in place of the
+// on-the-fly recursion.  The ring FFTs are the transforms' own, told by a
+// negative ring u that the sums are actual function values (no order
+// rescale) and, in analysis, to keep the two rings of a slot apart.
+
+// The table (src/SHGLQ.f95:202-257): one wave per (m, block of 64 northern
+// nodes i < (n+1)/2), each lane the recursion at its node; node n-1-i takes
+// the values times (-1)^(l-m).  nblk blocks per order.
+__global__ __launch_bounds__(kWave) void k_plx_table(const double *__restrict__ alpha,
+                                                     const double *__restrict__ stab, int Lt,
+                                                     const double *__restrict__ seed,
+                                                     const double *__restrict__ zero, int n, int L, int nblk,
+                                                     double *__restrict__ plx)
+{
+    const int m = blockIdx.x / nblk, b = blockIdx.x - m * nblk;
+    const int i = b * kWave + (int)threadIdx.x;
+    if (i >= (n + 1) / 2) return;
+    const double z = zero[i];
+    const double u = sqrt((1.0 - z) * (1.0 + z));
+    const double r = order_rescale(u, m);
+    const double zz = (r == 0.0) ? 0.0 : z;
+    const double *__restrict__ al = alpha + tri_offd(m, Lt);
+    const double *__restrict__ sl = stab + tri_offd(m, Lt);
+    const int is = n - 1 - i;
+    double q1 = 0.0, q2 = -seed[m];  // the first step yields the seed
+    for (int l = m; l <= L; ++l) {
+        const double q = (al[l - m] * zz) * q1 - q2;
+        q2 = q1;
+        q1 = q;
+        const double p = (r == 0.0) ? 0.0 : sl[l - m] * q * r;
+        const size_t k = (size_t)l * (l + 1) / 2 + m;
+        plx[k * n + i] = p;
+        if (is != i) plx[k * n + is] = ((l - m) & 1) ? -p : p;
+    }
+}
+
+// Synthesis sums from the table (src/MakeGridGLQ.f95:441-465): one wave per
+// (m, block of 64 slots), lanes = slots; fbuf[m][s] = (sum C p, sum S p) of
+// the slot's north row and of its south row, degrees from Lc down to m.
+__global__ __launch_bounds__(kWave) void k_plx_synth_leg(const double *__restrict__ cilm, int d1, int d2, int Lc,
+                                                         const double *__restrict__ plx, int ld,
+                                                         const int2 *__restrict__ rows, int P, int nblk,
+                                                         double4 *__restrict__ fbuf)
+{
+    const int m = blockIdx.x / nblk, b = blockIdx.x - m * nblk;
+    const int s = b * kWave + (int)threadIdx.x;
+    if (s >= P) return;
+    const int2 rw = rows[s];
+    const bool south = rw.y >= 0;
+    const double *__restrict__ cc = cilm + m;
+    const double *__restrict__ cs = cilm + (size_t)d1 * d2 + m;
+    double cn = 0.0, sn = 0.0, csum = 0.0, ssum = 0.0;
+    for (int l = Lc; l >= m; --l) {
+        const size_t k = (size_t)l * (l + 1) / 2 + m;
+        const double C = cc[(size_t)l * d2], S = (m > 0) ? cs[(size_t)l * d2] : 0.0;
+        const double pn = plx[k * ld + rw.x];
+        cn = fma(C, pn, cn);
+        sn = fma(S, pn, sn);
+        if (south) {
+            const double ps = plx[k * ld + rw.y];
+            csum = fma(C, ps, csum);
+            ssum = fma(S, ps, ssum);
+        }
+    }
+    fbuf[(size_t)m * P + s] = make_double4(cn, sn, csum, ssum);
+}
+
+// Analysis from the table (src/SHExpandGLQ.f95:419-441 and the epilogue
+// :659-689): one wave per (m, block of 64 degrees), lanes = degrees l;
+// gbuf[m][s] holds the weighted Fourier coefficients of the slot's north
+// and south rows apart, summed over the rows in order as the reference does,
+// then divided by the normalisation's integral.  nblk blocks per order;
+// cilm is zero outside what this writes.
+__global__ __launch_bounds__(kWave) void k_plx_anal_leg(const double4 *__restrict__ gbuf, int P, int nlat, int Lc,
+                                                        const double *__restrict__ plx, int ld, int norm,
+                                                        int nblk, double *__restrict__ cilm, int d1, int d2)
+{
+    const int m = blockIdx.x / nblk, b = blockIdx.x - m * nblk;
+    const int l = m + b * kWave + (int)threadIdx.x;
+    if (l > Lc) return;
+    const double *__restrict__ row = plx + ((size_t)l * (l + 1) / 2 + m) * ld;
+    const double4 *__restrict__ g = gbuf + (size_t)m * P;
+    double c = 0.0, sv = 0.0;
+    for (int r = 0; r < nlat; ++r) {
+        // row r is the north row of slot r or the south row of slot nlat-1-r
+        const bool north = r < P;
+        const double4 f = g[north ? r : nlat - 1 - r];
+        const double p = row[r];
+        c = fma(p, north ? f.x : f.z, c);
+        sv = fma(p, north ? f.y : f.w, sv);
+    }
+    const double pi = 3.14159265358979323846;
+    switch (norm) {
+    case 1:
+        c = c / (4 * pi);
+        sv = sv / (4 * pi);
+        break;
+    case 2:
+        c = (2 * l + 1) * c / (4 * pi);
+        sv = (2 * l + 1) * sv / (4 * pi);
+        break;
+    case 3: {
+        double prod = 4 * pi / (double)(2 * l + 1);
+        if (m == 0) {
+            c = c / prod;
+        } else {
+            prod = prod / 2.0;
+            for (int j = 1; j <= m && j <= l - 1; ++j) prod = prod * (l + j) * (l - j + 1);
+            const double d = (m == l) ? prod * 2 * l : prod;
+            c = c / d;
+            sv = sv / d;
+        }
+        break;
+    }
+    default:
+        break;
+    }
+    cilm[(size_t)l * d2 + m] = c;
+    cilm[((size_t)d1 + l) * d2 + m] = sv;
 }
 
 // ============================================================ complex kernels

@@ -1897,6 +1897,188 @@ int shtg_shmultiply(double *cilmout, int out_d1, int out_d2, const double *cilm1
     return st;
 }
 
+// ------------------------------------------------------------ plx tables
+//
+// The GLQ routines' optional table of Legendre functions at the nodes (the
+// reference's plx argument): SHGLQ builds it, MakeGridGLQ and SHExpandGLQ
+// read it in place of the recursion.  The table is device memory of
+// (lmax+1) (lmax+1)(lmax+2)/2 doubles, plx[k (lmax+1) + i] (the Fortran
+// array plx(i, k) in C order); a transform streams the part of degrees up
+// to lmax_calc once.
+
+}  // extern "C"
+
+namespace shtg {
+
+// Geometry of a plx call (ring FFT plan and, for analysis, the caller's
+// weights wq) on the library's own Gauss-Legendre nodes.
+static int plx_tables(int lmax, const double *wq, int Lc, CallTables &t, hipStream_t st)
+{
+    const std::pair<std::vector<double>, std::vector<double>> *zw;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        zw = glq_cached(lmax + 1);  // entries are never changed once made
+    }
+    if (!zw) return ST_OTHER;
+    return get_tables(false, lmax, 0, zw->first.data(), wq, 1, 1, Lc, t, st);
+}
+
+static int launch_plx_synthesis(const Geometry &g, const double *cilm, int d1, int d2, int Lc, const double *plx,
+                                int ext_col, double *grid, long long ld, hipStream_t st)
+{
+    const size_t nf = (size_t)(Lc + 1) * g.P;
+    const bool lds_fft = fft_in_lds(g.fft.m);
+    const int nwg = std::min(g.P, kWsGroups);
+    const size_t nws = lds_fft ? 0 : (size_t)nwg * ws_stride(g.fft.m);
+    WsLease lease;
+    if (int rc = ws_acquire(lease, nf * sizeof(double4) + nws * sizeof(double2), st)) return rc;
+    double4 *fbuf = (double4 *)lease.p;
+    double2 *fws = (double2 *)((char *)fbuf + nf * sizeof(double4));
+    {
+        ProfScope ps(SHTG_K_SYNTH_LEG, st);
+        const int nblk = (g.P + kWave - 1) / kWave;
+        hipLaunchKernelGGL(k_plx_synth_leg, dim3((Lc + 1) * nblk), dim3(kWave), 0, st, cilm, d1, d2, Lc, plx, g.nlat,
+                           g.rows, g.P, nblk, fbuf);
+    }
+    {
+        // no ring u: the sums are function values (no order rescale)
+        ProfScope ps(SHTG_K_SYNTH_FFT, st);
+        const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
+        if (lds_fft)
+            hipLaunchKernelGGL(SHTG_FFT_KERNEL(k_synth_fft, plan_variant(g.fft)), dim3(g.P),
+                               dim3(plan_threads(g.fft)), lds, st, g.fft, fbuf, g.P, Lc, (const double *)nullptr,
+                               g.rows, g.nlat, grid, ld, ext_col, (const double2 *)nullptr, (const int *)nullptr);
+        else
+            hipLaunchKernelGGL(k_synth_fft_ws, dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft, fbuf, g.P, Lc,
+                               (const double *)nullptr, g.rows, g.nlat, grid, ld, ext_col, (const double2 *)nullptr,
+                               fws, ws_stride(g.fft.m), (const int *)nullptr, kChunkCap);
+    }
+    const bool ok = HIP_OK(hipGetLastError());
+    ws_release(lease, st);
+    return ok ? ST_OK : ST_OTHER;
+}
+
+static int launch_plx_analysis(const Geometry &g, const double *grid, long long ld, int Lc, const double *plx,
+                               int norm, double *cilm, int d1, int d2, hipStream_t st)
+{
+    const size_t nf = (size_t)(Lc + 1) * g.P;
+    const bool lds_fft = fft_in_lds(g.fft.m);
+    const int nwg = std::min(g.P, kWsGroups);
+    const size_t nws = lds_fft ? 0 : (size_t)nwg * ws_stride(g.fft.m);
+    WsLease lease;
+    if (int rc = ws_acquire(lease, nf * sizeof(double4) + nws * sizeof(double2), st)) return rc;
+    double4 *gbuf = (double4 *)lease.p;
+    double2 *fws = (double2 *)((char *)gbuf + nf * sizeof(double4));
+    zero_outside(cilm, 2 * d1, d2, d2, 0, 0, st);
+    {
+        // no ring u: the rows' coefficients unscaled and apart
+        ProfScope ps(SHTG_K_ANAL_FFT, st);
+        const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
+        if (lds_fft)
+            hipLaunchKernelGGL(SHTG_FFT_KERNEL(k_anal_fft, plan_variant(g.fft)), dim3(g.P),
+                               dim3(plan_threads(g.fft)), lds, st, g.fft, grid, ld, (const double *)nullptr, g.rows,
+                               g.nlat, g.w, g.P, Lc, gbuf, 0, (const int *)nullptr);
+        else
+            hipLaunchKernelGGL(k_anal_fft_ws, dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft, grid, ld,
+                               (const double *)nullptr, g.rows, g.nlat, g.w, g.P, Lc, gbuf, 0, fws,
+                               ws_stride(g.fft.m), (const int *)nullptr, kChunkCap);
+    }
+    {
+        ProfScope ps(SHTG_K_ANAL_LEG, st);
+        const int nblk = (Lc + kWave) / kWave;
+        hipLaunchKernelGGL(k_plx_anal_leg, dim3((Lc + 1) * nblk), dim3(kWave), 0, st, gbuf, g.P, g.nlat, Lc, plx,
+                           g.nlat, norm, nblk, cilm, d1, d2);
+    }
+    const bool ok = HIP_OK(hipGetLastError());
+    ws_release(lease, st);
+    return ok ? ST_OK : ST_OTHER;
+}
+
+}  // namespace shtg
+
+extern "C" {
+
+int shtg_shglq_plx(int lmax, double *zero, double *w, double *plx, int norm, int csphase, int cnorm,
+                   void *stream)
+{
+    // argument checks in the order of src/SHGLQ.f95:64-150
+    if (lmax < 0 || lmax > kMaxArgDegree) return ST_BOUNDS;
+    if (!zero || !w || !plx) return ST_DIMS;
+    if (norm < 1 || norm > 4) return ST_BOUNDS;
+    if (csphase != 1 && csphase != -1) return ST_BOUNDS;
+    const int n = lmax + 1;
+    hipStream_t hs = (hipStream_t)stream;
+    const int dev = current_device();
+    if (dev < 0) return ST_OTHER;
+    const std::pair<std::vector<double>, std::vector<double>> *zw;
+    DevRecurrence rt;
+    DevSeeds sd;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        zw = glq_cached(n);
+        if (!zw) return ST_OTHER;
+        if (capturing(hs)) return ST_CAPTURE;  // table uploads and a host-to-device copy follow
+        const DevRecurrence *r = get_recurrence(dev, norm, lmax);
+        // PLegendreA (norm 3) takes no cnorm (src/SHGLQ.f95:225-227)
+        const DevSeeds *s = get_seeds(dev, norm, csphase, lmax, cnorm == 1 && norm != 3);
+        if (!r || !s) return ST_ALLOC;
+        rt = *r;
+        sd = *s;
+    }
+    std::copy(zw->first.begin(), zw->first.end(), zero);
+    std::copy(zw->second.begin(), zw->second.end(), w);
+    // the nodes on the device, from the cache's host copy (kept for the
+    // process), in a block of the per-device intermediates
+    WsLease lease;
+    if (int rc = ws_acquire(lease, (size_t)n * sizeof(double), hs)) return rc;
+    bool ok = HIP_OK(hipMemcpyAsync(lease.p, zw->first.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice,
+                                    hs));
+    if (ok) {
+        const int nblk = ((n + 1) / 2 + kWave - 1) / kWave;
+        hipLaunchKernelGGL(k_plx_table, dim3((lmax + 1) * nblk), dim3(kWave), 0, hs, rt.alpha, rt.s, rt.L, sd.seed,
+                           (const double *)lease.p, n, lmax, nblk, plx);
+        ok = HIP_OK(hipGetLastError());
+    }
+    ws_release(lease, hs);
+    return ok ? ST_OK : ST_OTHER;
+}
+
+int shtg_makegrid_glq_plx(const double *cilm, int cilm_d1, int cilm_d2, int lmax, const double *plx, int norm,
+                          int csphase, int lmax_calc, int extend, double *grid, int grid_d0, int grid_d1,
+                          void *stream)
+{
+    Shape sh;
+    int st = check_makegrid_glq_plx(cilm_d1, cilm_d2, lmax, plx, norm, csphase, lmax_calc, extend, grid_d0, grid_d1,
+                                    sh);
+    if (st) return st;
+    if (!cilm || !grid) return ST_DIMS;
+    hipStream_t hs = (hipStream_t)stream;
+    if (sh.Lc == 0) {
+        // a degree-0 field: every node cilm(1,1,1) P_00 (src/MakeGridGLQ.f95:408-420)
+        const double pm2 = (norm == 4) ? 1.0 / std::sqrt(4.0 * std::acos(-1.0)) : 1.0;
+        const int nlong_out = 2 * lmax + 1 + extend;
+        hipLaunchKernelGGL(k_fill, dim3(grid_blocks((long long)(lmax + 1) * nlong_out)), dim3(kThreads), 0, hs, grid,
+                           lmax + 1, nlong_out, (long long)grid_d1, cilm, pm2);
+        return HIP_OK(hipGetLastError()) ? ST_OK : ST_OTHER;
+    }
+    CallTables t;
+    if ((st = plx_tables(lmax, nullptr, sh.Lc, t, hs))) return st;
+    return launch_plx_synthesis(*t.g, cilm, cilm_d1, cilm_d2, sh.Lc, plx, extend, grid, grid_d1, hs);
+}
+
+int shtg_shexpand_glq_plx(const double *grid, int grid_d0, int grid_d1, const double *w, int nw, const double *plx,
+                          int norm, int csphase, int lmax_calc, double *cilm, int cilm_d1, int cilm_d2, void *stream)
+{
+    Shape sh;
+    int st = check_shexpand_glq_plx(grid_d0, grid_d1, w, nw, plx, norm, csphase, lmax_calc, cilm_d1, cilm_d2, sh);
+    if (st) return st;
+    if (!cilm || !grid) return ST_DIMS;
+    hipStream_t hs = (hipStream_t)stream;
+    CallTables t;
+    if ((st = plx_tables(sh.lmax, w, sh.Lc, t, hs))) return st;
+    return launch_plx_analysis(*t.g, grid, grid_d1, sh.Lc, plx, norm, cilm, cilm_d1, cilm_d2, hs);
+}
+
 int shtg_dh_polar_orders(int lmax, int *kmax)
 {
     if (lmax < 0 || lmax > kMaxArgDegree) return ST_BOUNDS;

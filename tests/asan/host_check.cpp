@@ -521,7 +521,18 @@ static void check_arguments()
     CHECK(check_shexpand_glq(11, 20, w, 11, zero, 11, 1, 1, 10, 11, 11, sh) == ST_DIMS, "SHExpandGLQ short rows");
     CHECK(check_shexpand_glq(11, 21, nullptr, 11, zero, 11, 1, 1, 10, 11, 11, sh) == ST_DIMS, "SHExpandGLQ no weights");
     CHECK(check_shexpand_glq(0, 21, w, 11, zero, 11, 1, 1, 10, 11, 11, sh) == ST_DIMS, "SHExpandGLQ empty grid");
-    ncases += 14;
+    // the plx branch: the table in place of the nodes
+    const double plx[4] = {0};
+    CHECK(check_makegrid_glq_plx(11, 11, 10, plx, 1, 1, 10, 0, 11, 21, sh) == ST_OK && sh.Lc == 10,
+          "MakeGridGLQ plx");
+    CHECK(check_makegrid_glq_plx(11, 11, 10, nullptr, 1, 1, 10, 0, 11, 21, sh) == ST_OTHER, "MakeGridGLQ no table");
+    CHECK(check_makegrid_glq_plx(11, 11, -1, plx, 1, 1, 10, 0, 11, 21, sh) == ST_BOUNDS, "MakeGridGLQ plx lmax");
+    CHECK(check_makegrid_glq_plx(11, 11, 10, plx, 5, 1, 10, 0, 11, 21, sh) == ST_BOUNDS, "MakeGridGLQ plx norm");
+    CHECK(check_shexpand_glq_plx(11, 21, w, 11, plx, 1, 1, 4, 5, 5, sh) == ST_OK && sh.Lc == 4, "SHExpandGLQ plx");
+    CHECK(check_shexpand_glq_plx(11, 21, w, 11, nullptr, 1, 1, 10, 11, 11, sh) == ST_OTHER, "SHExpandGLQ no table");
+    CHECK(check_shexpand_glq_plx(11, 21, w, 10, plx, 1, 1, 10, 11, 11, sh) == ST_DIMS, "SHExpandGLQ plx weights");
+    CHECK(check_shexpand_glq_plx(11, 21, w, 11, plx, 1, 0, 10, 11, 11, sh) == ST_BOUNDS, "SHExpandGLQ plx phase");
+    ncases += 22;
     // extreme integers: every check returns a status, and a grid it accepts
     // has dimensions that exist (no overflow in the shape arithmetic)
     const int ext[] = {INT_MIN, INT_MIN + 1, -2, -1, 0, 1, 2, 3, 1000, (INT_MAX - 4) / 4, (INT_MAX - 4) / 4 + 1,
@@ -539,7 +550,11 @@ static void check_arguments()
             CHECK(st != ST_OK || (long long)b >= 2LL * a + 2, "MakeGridGLQ lmax %d dims %d", a, b);
             st = check_shexpand_glq(a, b, w, INT_MAX, zero, INT_MAX, 1, 1, INT_MAX, INT_MAX, INT_MAX, sh);
             CHECK(st != ST_OK || (a >= 1 && (long long)b >= 2LL * a - 1), "SHExpandGLQ rows %d cols %d", a, b);
-            ncases += 6;
+            st = check_makegrid_glq_plx(b, b, a, plx, 1, 1, a, 1, b, b, sh);
+            CHECK(st != ST_OK || (long long)b >= 2LL * a + 2, "MakeGridGLQ plx lmax %d dims %d", a, b);
+            st = check_shexpand_glq_plx(a, b, w, INT_MAX, plx, 1, 1, INT_MAX, INT_MAX, INT_MAX, sh);
+            CHECK(st != ST_OK || (a >= 1 && (long long)b >= 2LL * a - 1), "SHExpandGLQ plx rows %d cols %d", a, b);
+            ncases += 8;
         }
     std::printf("ok checks %d\n", ncases);
 }

@@ -97,6 +97,19 @@ int shtg_shexpand_glq_plx(const double *grid, int grid_d0, int grid_d1, const do
                           const double *plx, int norm, int csphase, int lmax_calc, double *cilm,
                           int cilm_d1, int cilm_d2, void *stream);
 
+/* MakeGridGLQC and SHExpandGLQC with a table of Legendre functions (their
+ * plx branches: src/MakeGridGLQC.f95:411-421,443-468 and src/SHExpandGLQC.f95,
+ * include/shtools.h:174-195 with plx present): as shtg_makegrid_glq_plx and
+ * shtg_shexpand_glq_plx for complex coefficients and grids (interleaved
+ * (re, im) doubles, the layouts of shtg_makegrid_glqc / shtg_shexpand_glqc),
+ * with a table of the complex normalisation (shtg_shglq_plx, cnorm = 1). */
+int shtg_makegrid_glqc_plx(const double *cilm, int cilm_d1, int cilm_d2, int lmax,
+                           const double *plx, int norm, int csphase, int lmax_calc, int extend,
+                           double *grid, int grid_d0, int grid_d1, void *stream);
+int shtg_shexpand_glqc_plx(const double *grid, int grid_d0, int grid_d1, const double *w, int nw,
+                           const double *plx, int norm, int csphase, int lmax_calc, double *cilm,
+                           int cilm_d1, int cilm_d2, void *stream);
+
 /* SHExpandGLQ, on-the-fly branch: src/SHExpandGLQ.f95:1-2,445-649
  * (include/shtools.h:150-159; src/pyshtools.pyf:334-355).  lmax =
  * grid_d0 - 1; `w` and `zero` are HOST arrays of >= lmax+1 weights and

@@ -159,6 +159,8 @@ def lib():
         L.or_plx_table.argtypes = [i, dp, i, i, i, i, dp]
         L.or_makegridglq_plx.argtypes = [dp, i, i, i, dp, i, i, i, i, dp, i, i]
         L.or_shexpandglq_plx.argtypes = [dp, i, i, i, dp, i, dp, i, i, i, dp, i, i]
+        L.or_makegridglqc_plx.argtypes = [dp, i, i, i, dp, i, i, i, i, dp, i, i]
+        L.or_shexpandglqc_plx.argtypes = [dp, i, i, i, dp, i, dp, i, i, i, dp, i, i]
         L.or_makegriddhc.argtypes = [dp, i, i, i, i, i, i, i, i, dp, i, i]
         L.or_shexpanddhc.argtypes = [dp, i, i, i, i, i, i, dp, i, i]
         L.or_makegridglqc.argtypes = [dp, i, i, i, dp, i, i, i, i, i, dp, i, i]
@@ -365,6 +367,39 @@ def SHExpandGLQC(gridglq, w, zero, norm=1, csphase=1, lmax_calc=None):
     _check(lib().or_shexpandglqc(_p(grid), grid.shape[0], grid.shape[1], lmax, _p(w), w.size,
                                  _p(zero), zero.size, norm, csphase, lmax_calc, _p(cilm),
                                  lmax_calc + 1, lmax_calc + 1))
+    return cilm
+
+
+def MakeGridGLQC_plx(cilm, plx, lmax=None, norm=1, csphase=1, lmax_calc=None, extend=0):
+    """MakeGridGLQC with the table plx (of complex normalisation) in place of
+    the nodes."""
+    cilm = np.ascontiguousarray(cilm, dtype=np.complex128)
+    plx = np.ascontiguousarray(plx, dtype=np.float64)
+    if lmax is None:
+        lmax = cilm.shape[1] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    assert plx.shape == ((lmax + 1) * (lmax + 2) // 2, lmax + 1)
+    g0, g1 = lmax + 1, 2 * lmax + 1 + extend
+    grid = np.zeros((g0, g1), dtype=np.complex128)
+    _check(lib().or_makegridglqc_plx(_p(cilm), cilm.shape[1], cilm.shape[2], lmax, _p(plx), norm, csphase,
+                                     lmax_calc, extend, _p(grid), g0, g1))
+    return grid
+
+
+def SHExpandGLQC_plx(gridglq, w, plx, norm=1, csphase=1, lmax_calc=None):
+    """SHExpandGLQC with the table plx (of complex normalisation) in place of
+    the nodes."""
+    grid = np.ascontiguousarray(gridglq, dtype=np.complex128)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    plx = np.ascontiguousarray(plx, dtype=np.float64)
+    lmax = grid.shape[0] - 1
+    assert plx.shape == ((lmax + 1) * (lmax + 2) // 2, lmax + 1)
+    if lmax_calc is None:
+        lmax_calc = lmax
+    cilm = np.zeros((2, lmax_calc + 1, lmax_calc + 1), dtype=np.complex128)
+    _check(lib().or_shexpandglqc_plx(_p(grid), grid.shape[0], grid.shape[1], lmax, _p(w), w.size, _p(plx), norm,
+                                     csphase, lmax_calc, _p(cilm), lmax_calc + 1, lmax_calc + 1))
     return cilm
 
 

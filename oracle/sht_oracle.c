@@ -1809,6 +1809,106 @@ int or_shexpandglqc(const double *grid_ri, int g0, int g1, int lmax, const doubl
     return 0;
 }
 
+/* MakeGridGLQC with plx (src/MakeGridGLQC.f95:411-421, :443-468): every row
+ * from the table (of complex normalisation, SHGLQ's cnorm = 1), degrees
+ * summed from lmax_comp down; the order -m takes cilm(2,l,m) (-1)^m. */
+int or_makegridglqc_plx(const double *cilm_ri, int d1, int d2, int lmax, const double *plx, int norm, int csphase,
+                        int lmax_calc, int extend, double *grid_ri, int g0, int g1)
+{
+    const cplx *cilm = (const cplx *)cilm_ri;
+    cplx *grid = (cplx *)grid_ri;
+    int nlong = 2 * lmax + 1, nlat = lmax + 1, nlong_out;
+    if (extend == 0) nlong_out = nlong; else if (extend == 1) nlong_out = nlong + 1; else return 2;
+    if (g1 < nlong_out || g0 < nlat) return 1;
+    if (!plx) return 5;
+    if (norm < 1 || norm > 4) return 2;
+    if (csphase != -1 && csphase != 1) return 2;
+    if (lmax_calc > lmax) return 2;
+    int lc = lc_of_makegrid(d1, d2, lmax, lmax_calc);
+#define G(i, k) grid[(size_t)(i) * g1 + (k)]
+    if (lc == 0) {
+        double pm2 = pm00(norm);
+        for (int i = 0; i < nlat; i++)
+            for (int k = 0; k < nlong_out; k++) G(i, k) = cilm[0] * pm2;
+        return 0;
+    }
+    cplx *cc = to_fortran_order_c(cilm, d1, d2, lc);
+    if (!cc) return 3;
+#pragma omp parallel
+    {
+        fftplan *plan = plan_create(nlong);
+        cplx *coef = (cplx *)malloc(sizeof(cplx) * nlong);
+#pragma omp for schedule(dynamic, 1)
+        for (int i = 1; i <= nlat; i++) {
+            for (int k = 0; k < nlong; k++) coef[k] = 0;
+            for (int l = lc; l >= 0; l--) {
+                size_t k = (size_t)l * (l + 1) / 2;   /* m = 0 */
+                coef[0] = coef[0] + CCF(0, l, 0) * plx[k * nlat + (i - 1)];
+                for (int m = 1; m <= l; m++) {
+                    k = (size_t)l * (l + 1) / 2 + m;
+                    const double p = plx[k * nlat + (i - 1)];
+                    coef[m] = coef[m] + CCF(0, l, m) * p;
+                    coef[nlong - m] = coef[nlong - m] + CCF(1, l, m) * p * msign(m);
+                }
+            }
+            plan_exec(plan, coef, 1);
+            for (int k = 0; k < nlong; k++) G(i - 1, k) = coef[k];
+        }
+        free(coef);
+        plan_destroy(plan);
+    }
+    if (extend == 1)
+        for (int i = 0; i < nlat; i++) G(i, nlong_out - 1) = G(i, 0);
+#undef G
+    free(cc);
+    return 0;
+}
+
+/* SHExpandGLQC with plx (src/SHExpandGLQC.f95 plx branch and epilogue):
+ * every ring's weighted DFT times the table, rings in order. */
+int or_shexpandglqc_plx(const double *grid_ri, int g0, int g1, int lmax, const double *w, int nw,
+                        const double *plx, int norm, int csphase, int lmax_calc, double *cilm_ri, int d1, int d2)
+{
+    const cplx *grid = (const cplx *)grid_ri;
+    cplx *cilm = (cplx *)cilm_ri;
+    int lc, nlong = 2 * lmax + 1, nlat = lmax + 1;
+    if (lmax_calc > lmax) return 2;
+    lc = lmax_calc < lmax ? lmax_calc : lmax;
+    if (d1 < lc + 1 || d2 < lc + 1) return 1;
+    if (g1 < 2 * lmax + 1 || g0 < lmax + 1) return 1;
+    if (nw < lmax + 1) return 1;
+    if (!plx) return 5;
+    if (norm < 1 || norm > 4) return 2;
+    if (csphase != -1 && csphase != 1) return 2;
+    double pi = acos(-1.0);
+    for (size_t k = 0; k < (size_t)2 * d1 * d2; k++) cilm[k] = 0;
+    cplx *cc = (cplx *)calloc((size_t)2 * (lc + 1) * (lc + 1), sizeof(cplx));
+    if (!cc) return 3;
+    fftplan *plan = plan_create(nlong);
+    cplx *f1 = (cplx *)malloc(sizeof(cplx) * nlong);
+#define GR(i, k) grid[(size_t)(i) * g1 + (k)]
+    for (int i = 1; i <= nlat; i++) {
+        for (int k = 0; k < nlong; k++) f1[k] = GR(i - 1, k);
+        plan_exec(plan, f1, 0);
+        for (int k = 0; k < nlong; k++) f1[k] = 2 * f1[k] * w[i - 1] * pi / (double)nlong;
+        size_t k = 0;
+        for (int l = 0; l <= lc; l++) {
+            CCF(0, l, 0) = CCF(0, l, 0) + plx[k * nlat + (i - 1)] * f1[0];
+            k++;
+            for (int m = 1; m <= l; m++, k++) {
+                const double p = plx[k * nlat + (i - 1)];
+                CCF(0, l, m) = CCF(0, l, m) + p * f1[m];
+                CCF(1, l, m) = CCF(1, l, m) + msign(m) * p * f1[nlong - m];
+            }
+        }
+    }
+#undef GR
+    anal_epilogue_c(norm, lc, cc, cilm, d1, d2);
+    free(f1); free(cc);
+    plan_destroy(plan);
+    return 0;
+}
+
 /* ------------------------------------------------------ gradient */
 
 /* Horizontal gradient of a 4pi-normalised real field (no Condon-Shortley

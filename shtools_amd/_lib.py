@@ -57,6 +57,10 @@ SIGNATURES = {
                               _vp],
     "shtg_shexpand_glq_plx": [_vp, _c_int, _c_int, _dp, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _c_int, _c_int,
                               _vp],
+    "shtg_makegrid_glqc_plx": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int,
+                               _vp],
+    "shtg_shexpand_glqc_plx": [_vp, _c_int, _c_int, _dp, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _c_int, _c_int,
+                               _vp],
     "shtg_dhaj": [_c_int, _c_int, _dp],
     "shtg_glqgridcoord": [_c_int, _c_int, _dp, _dp],
     "shtg_plm": [_c_int, _c_int, _c_int, ctypes.c_double, _dp],

@@ -39,7 +39,7 @@ __all__ = [
     "PlmBar", "PlmSchmidt", "PlmON", "PLegendreA",
     "SHExpandDHC", "MakeGridDHC", "SHExpandGLQC", "MakeGridGLQC", "MakeGradientDH", "SHRotateRealCoef",
     "MakeGridDH_adjoint", "SHExpandDH_adjoint", "MakeGridGLQ_adjoint", "SHExpandGLQ_adjoint",
-    "SHGLQ_plx", "MakeGridGLQ_plx", "SHExpandGLQ_plx",
+    "SHGLQ_plx", "MakeGridGLQ_plx", "SHExpandGLQ_plx", "MakeGridGLQC_plx", "SHExpandGLQC_plx",
 ]
 
 
@@ -426,6 +426,57 @@ def SHExpandGLQC(gridglq, w, zero, norm=1, csphase=1, lmax_calc=None):
         st = _lib.lib().shtg_shexpand_glqc(g.data_ptr(), g.shape[0], g.shape[1], _dptr(wq), wq.size, _dptr(z),
                                            z.size, int(norm), int(csphase), int(lmax_calc), cilm.data_ptr(),
                                            lmax_calc + 1, lmax_calc + 1, _stream(dev))
+    _check(st)
+    return _finish(cilm, as_tensor)
+
+
+def MakeGridGLQC_plx(cilm, plx, lmax=None, norm=1, csphase=1, lmax_calc=None, extend=0):
+    """MakeGridGLQC with the table plx of SHGLQ_plx (cnorm=1) in place of the
+    nodes (the reference's plx branch, src/MakeGridGLQC.f95:443-468)."""
+    torch = _torch()
+    as_tensor = _out_kind(cilm)
+    dev = _device_of(cilm)
+    c = _to_dev_c(cilm, dev)
+    if c.dim() != 3 or c.shape[0] < 2:
+        raise SHToolsError(_status_message(1))
+    if lmax is None:
+        lmax = c.shape[1] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    if extend not in (0, 1) or lmax < 0:
+        raise SHToolsError(_status_message(2))
+    p = _plx_dev(plx, lmax, dev)
+    g0, g1 = lmax + 1, 2 * lmax + 1 + extend
+    grid = torch.empty((g0, g1), dtype=torch.complex128, device=dev)
+    with torch.cuda.device(dev):
+        st = _lib.lib().shtg_makegrid_glqc_plx(c.data_ptr(), c.shape[1], c.shape[2], int(lmax), p.data_ptr(),
+                                               int(norm), int(csphase), int(lmax_calc), int(extend), grid.data_ptr(),
+                                               g0, g1, _stream(dev))
+    _check(st)
+    return _finish(grid, as_tensor)
+
+
+def SHExpandGLQC_plx(gridglq, w, plx, norm=1, csphase=1, lmax_calc=None):
+    """SHExpandGLQC with the table plx of SHGLQ_plx (cnorm=1) in place of the
+    nodes (the reference's plx branch of src/SHExpandGLQC.f95)."""
+    torch = _torch()
+    as_tensor = _out_kind(gridglq)
+    dev = _device_of(gridglq)
+    g = _to_dev_c(gridglq, dev)
+    if g.dim() != 2:
+        raise SHToolsError(_status_message(1))
+    wq = _host_f64(w)
+    lmax = g.shape[0] - 1
+    if lmax_calc is None:
+        lmax_calc = lmax
+    if lmax_calc < 0:
+        raise SHToolsError(_status_message(2))
+    p = _plx_dev(plx, lmax, dev)
+    cilm = torch.empty((2, lmax_calc + 1, lmax_calc + 1), dtype=torch.complex128, device=dev)
+    with torch.cuda.device(dev):
+        st = _lib.lib().shtg_shexpand_glqc_plx(g.data_ptr(), g.shape[0], g.shape[1], _dptr(wq), wq.size,
+                                               p.data_ptr(), int(norm), int(csphase), int(lmax_calc), cilm.data_ptr(),
+                                               lmax_calc + 1, lmax_calc + 1, _stream(dev))
     _check(st)
     return _finish(cilm, as_tensor)
 

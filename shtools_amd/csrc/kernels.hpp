@@ -933,6 +933,30 @@ __global__ __launch_bounds__(kWave) void k_plx_synth_leg(const double *__restric
     fbuf[(size_t)m * P + s] = make_double4(cn, sn, csum, ssum);
 }
 
+// The analysis epilogue of the table's coefficient (l, m): divided by the
+// integral of the squared function (src/SHExpandGLQ.f95:659-689; the
+// complex transforms, src/SHExpandGLQC.f95, omit the halving of the
+// unnormalised product for m >= 1), in the reference's operations.
+__device__ __forceinline__ double plx_normalise(double c, int norm, int l, int m, bool cplx)
+{
+    const double pi = 3.14159265358979323846;
+    switch (norm) {
+    case 1:
+        return c / (4 * pi);
+    case 2:
+        return (2 * l + 1) * c / (4 * pi);
+    case 3: {
+        double prod = 4 * pi / (double)(2 * l + 1);
+        if (m == 0) return c / prod;
+        if (!cplx) prod = prod / 2.0;
+        for (int j = 1; j <= m && j <= l - 1; ++j) prod = prod * (l + j) * (l - j + 1);
+        return c / ((m == l) ? prod * 2 * l : prod);
+    }
+    default:
+        return c;
+    }
+}
+
 // Analysis from the table (src/SHExpandGLQ.f95:419-441 and the epilogue
 // :659-689): one wave per (m, block of 64 degrees), lanes = degrees l;
 // gbuf[m][s] holds the weighted Fourier coefficients of the slot's north
@@ -957,34 +981,97 @@ __global__ __launch_bounds__(kWave) void k_plx_anal_leg(const double4 *__restric
         c = fma(p, north ? f.x : f.z, c);
         sv = fma(p, north ? f.y : f.w, sv);
     }
-    const double pi = 3.14159265358979323846;
-    switch (norm) {
-    case 1:
-        c = c / (4 * pi);
-        sv = sv / (4 * pi);
-        break;
-    case 2:
-        c = (2 * l + 1) * c / (4 * pi);
-        sv = (2 * l + 1) * sv / (4 * pi);
-        break;
-    case 3: {
-        double prod = 4 * pi / (double)(2 * l + 1);
-        if (m == 0) {
-            c = c / prod;
-        } else {
-            prod = prod / 2.0;
-            for (int j = 1; j <= m && j <= l - 1; ++j) prod = prod * (l + j) * (l - j + 1);
-            const double d = (m == l) ? prod * 2 * l : prod;
-            c = c / d;
-            sv = sv / d;
+    cilm[(size_t)l * d2 + m] = plx_normalise(c, norm, l, m, false);
+    // S_l0 stays as summed: the reference divides only C_l0 for norm 3
+    cilm[((size_t)d1 + l) * d2 + m] = (m == 0 && norm == 3) ? sv : plx_normalise(sv, norm, l, m, false);
+}
+
+// MakeGridGLQC with the table (src/MakeGridGLQC.f95:443-468): as
+// k_plx_synth_leg for complex coefficients cilm[2][d1][d2]; fb_p[m][s] the
+// sums of plane 0 (orders +m), fb_m[m][s] of plane 1 (orders -m, m >= 1;
+// the ring FFT applies their (-1)^m), each (Re, Im) of the north row and of
+// the south row.
+__global__ __launch_bounds__(kWave) void k_plx_synth_leg_c(const double2 *__restrict__ cilm, int d1, int d2, int Lc,
+                                                           const double *__restrict__ plx, int ld,
+                                                           const int2 *__restrict__ rows, int P, int nblk,
+                                                           double4 *__restrict__ fb_p, double4 *__restrict__ fb_m)
+{
+    const int m = blockIdx.x / nblk, b = blockIdx.x - m * nblk;
+    const int s = b * kWave + (int)threadIdx.x;
+    if (s >= P) return;
+    const int2 rw = rows[s];
+    const bool south = rw.y >= 0;
+    const double2 *__restrict__ cp = cilm + m;
+    const double2 *__restrict__ cm = cilm + (size_t)d1 * d2 + m;
+    double4 ap = make_double4(0.0, 0.0, 0.0, 0.0), am = ap;
+    for (int l = Lc; l >= m; --l) {
+        const size_t k = (size_t)l * (l + 1) / 2 + m;
+        const double2 A = cp[(size_t)l * d2];
+        const double2 Bm = (m > 0) ? cm[(size_t)l * d2] : make_double2(0.0, 0.0);
+        const double pn = plx[k * ld + rw.x];
+        ap.x = fma(A.x, pn, ap.x);
+        ap.y = fma(A.y, pn, ap.y);
+        am.x = fma(Bm.x, pn, am.x);
+        am.y = fma(Bm.y, pn, am.y);
+        if (south) {
+            const double ps = plx[k * ld + rw.y];
+            ap.z = fma(A.x, ps, ap.z);
+            ap.w = fma(A.y, ps, ap.w);
+            am.z = fma(Bm.x, ps, am.z);
+            am.w = fma(Bm.y, ps, am.w);
         }
-        break;
     }
-    default:
-        break;
+    fb_p[(size_t)m * P + s] = ap;
+    fb_m[(size_t)m * P + s] = am;
+}
+
+// SHExpandGLQC with the table (src/SHExpandGLQC.f95 plx branch and the
+// epilogue): as k_plx_anal_leg for the complex DFTs of the rows, gb_p /
+// gb_m holding orders +m and -m (the latter times (-1)^m) of the slot's
+// north and south rows apart; complex coefficients cilm[2][d1][d2].
+__global__ __launch_bounds__(kWave) void k_plx_anal_leg_c(const double4 *__restrict__ gb_p,
+                                                          const double4 *__restrict__ gb_m, int P, int nlat, int Lc,
+                                                          const double *__restrict__ plx, int ld, int norm,
+                                                          int nblk, double2 *__restrict__ cilm, int d1, int d2)
+{
+    const int m = blockIdx.x / nblk, b = blockIdx.x - m * nblk;
+    const int l = m + b * kWave + (int)threadIdx.x;
+    if (l > Lc) return;
+    const double *__restrict__ row = plx + ((size_t)l * (l + 1) / 2 + m) * ld;
+    const double4 *__restrict__ gp = gb_p + (size_t)m * P;
+    const double4 *__restrict__ gm = gb_m + (size_t)m * P;
+    double2 cp = make_double2(0.0, 0.0), cm = cp;
+    for (int r = 0; r < nlat; ++r) {
+        const bool north = r < P;
+        const int s = north ? r : nlat - 1 - r;
+        const double p = row[r];
+        const double4 fp = gp[s];
+        cp.x = fma(p, north ? fp.x : fp.z, cp.x);
+        cp.y = fma(p, north ? fp.y : fp.w, cp.y);
+        if (m > 0) {
+            const double4 fm = gm[s];
+            cm.x = fma(p, north ? fm.x : fm.z, cm.x);
+            cm.y = fma(p, north ? fm.y : fm.w, cm.y);
+        }
     }
-    cilm[(size_t)l * d2 + m] = c;
-    cilm[((size_t)d1 + l) * d2 + m] = sv;
+    cilm[(size_t)l * d2 + m] = make_double2(plx_normalise(cp.x, norm, l, m, true), plx_normalise(cp.y, norm, l, m, true));
+    if (m > 0)
+        cilm[((size_t)d1 + l) * d2 + m] =
+            make_double2(plx_normalise(cm.x, norm, l, m, true), plx_normalise(cm.y, norm, l, m, true));
+}
+
+// A degree-0 complex field: every element of a rows x cols array (leading
+// dimension ld) src[0] * v.
+__global__ __launch_bounds__(kThreads) void k_fill_c(double2 *__restrict__ a, int rows, int cols, long long ld,
+                                                     const double2 *__restrict__ src, double v)
+{
+    const double2 x = make_double2(src[0].x * v, src[0].y * v);
+    const long long total = (long long)rows * cols;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int r = (int)(e / cols), c = (int)(e - (long long)r * cols);
+        a[r * ld + c] = x;
+    }
 }
 
 // ============================================================ complex kernels
@@ -1090,7 +1177,7 @@ __device__ __forceinline__ void synth_fft_c_tile(const FftDev &F, const double4 
         for (int q = threadIdx.x; q < m; q += blockDim.x)
             if ((q > Lc && q < top - Lc) || q >= top) x[fpad(q)] = make_double2(0.0, 0.0);
         for (int k = threadIdx.x; k <= Lc; k += blockDim.x) {
-            const double r = order_rescale(u, k);
+            const double r = ring_rescale(u, k);
             const double4 fp = fb_p[(size_t)k * P + s];
             const double2 a = ring ? make_double2(fp.z * r, fp.w * r) : make_double2(fp.x * r, fp.y * r);
             x[fpad(k)] = fft_prep_in(F, k, conjd(a));
@@ -1183,7 +1270,7 @@ __device__ __forceinline__ void anal_fft_c_tile(const FftDev &F, const double2 *
                 fft_forward<kVar == kVarPrime>(x, F, F.bhat2);
         }
         for (int k = threadIdx.x; k <= Lc; k += blockDim.x) {
-            const double r = order_rescale(u, k) * wr;
+            const double r = ring_rescale(u, k) * wr;
             const double2 zp = fft_get(x, F, k, k);
             const double2 fp = make_double2(zp.x * r, zp.y * r);
             double2 fm = make_double2(0.0, 0.0);
@@ -1197,6 +1284,12 @@ __device__ __forceinline__ void anal_fft_c_tile(const FftDev &F, const double2 *
             if (ring == 0) {
                 op = make_double4(fp.x, fp.y, fp.x, fp.y);
                 om = make_double4(fm.x, fm.y, fm.x, fm.y);
+            } else if (u < 0.0) {
+                // unscaled sums: the two rings kept apart
+                op.z = fp.x;
+                op.w = fp.y;
+                om.z = fm.x;
+                om.w = fm.y;
             } else {
                 const double4 a = op, b = om;
                 op = make_double4(a.x + fp.x, a.y + fp.y, a.x - fp.x, a.y - fp.y);
@@ -1221,8 +1314,8 @@ __global__ __launch_bounds__(fft_max_threads(kVar)) void k_synth_fft_c(FftDev F,
     if constexpr (WS) x = ws + blockIdx.x * ws_stride;
     else x = lds;
     for (int s = blockIdx.x; s < P; s += gridDim.x)
-        synth_fft_c_tile<kVar>(F, fb_p, fb_m, P, Lc, s, ringu[s], rows[s], nrows, grid, ld, ext_col, x,
-                         WS ? lds : nullptr, chunk_cap);
+        synth_fft_c_tile<kVar>(F, fb_p, fb_m, P, Lc, s, ringu ? ringu[s] : -1.0, rows[s], nrows, grid, ld, ext_col, x,
+                               WS ? lds : nullptr, chunk_cap);
 }
 
 template <bool WS, int kVar>
@@ -1238,8 +1331,8 @@ __global__ __launch_bounds__(fft_max_threads(kVar)) void k_anal_fft_c(FftDev F, 
     if constexpr (WS) x = ws + blockIdx.x * ws_stride;
     else x = lds;
     for (int s = blockIdx.x; s < P; s += gridDim.x)
-        anal_fft_c_tile<kVar>(F, grid, ld, rows[s], nrows, wts[s], ringu[s], P, Lc, s, gb_p, gb_m, x,
-                        WS ? lds : nullptr, chunk_cap);
+        anal_fft_c_tile<kVar>(F, grid, ld, rows[s], nrows, wts[s], ringu ? ringu[s] : -1.0, P, Lc, s, gb_p, gb_m, x,
+                              WS ? lds : nullptr, chunk_cap);
 }
 
 // ================================================================== gradient

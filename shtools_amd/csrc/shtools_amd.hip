@@ -1994,6 +1994,78 @@ static int launch_plx_analysis(const Geometry &g, const double *grid, long long 
     return ok ? ST_OK : ST_OTHER;
 }
 
+// Complex counterparts (MakeGridGLQC / SHExpandGLQC with plx).
+static int launch_plx_synthesis_c(const Geometry &g, const double2 *cilm, int d1, int d2, int Lc, const double *plx,
+                                  int ext_col, double2 *grid, long long ld, hipStream_t st)
+{
+    const size_t nf = (size_t)(Lc + 1) * g.P;
+    const bool lds_fft = fft_in_lds(g.fft.m);
+    const int nwg = std::min(g.P, kWsGroups);
+    const size_t nws = lds_fft ? 0 : (size_t)nwg * ws_stride(g.fft.m);
+    WsLease lease;
+    if (int rc = ws_acquire(lease, 2 * nf * sizeof(double4) + nws * sizeof(double2), st)) return rc;
+    double4 *fb_p = (double4 *)lease.p, *fb_m = fb_p + nf;
+    double2 *fws = (double2 *)(fb_m + nf);
+    {
+        ProfScope ps(SHTG_K_SYNTH_LEG, st);
+        const int nblk = (g.P + kWave - 1) / kWave;
+        hipLaunchKernelGGL(k_plx_synth_leg_c, dim3((Lc + 1) * nblk), dim3(kWave), 0, st, cilm, d1, d2, Lc, plx,
+                           g.nlat, g.rows, g.P, nblk, fb_p, fb_m);
+    }
+    {
+        ProfScope ps(SHTG_K_SYNTH_FFT, st);
+        const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
+        if (lds_fft)
+            hipLaunchKernelGGL((by_variant(plan_variant(g.fft), k_synth_fft_c<false, kVarPlain>,
+                                           k_synth_fft_c<false, kVarSplit>, k_synth_fft_c<false, kVarPrime>)),
+                               dim3(g.P), dim3(plan_threads(g.fft)), lds, st, g.fft, fb_p, fb_m, g.P, Lc,
+                               (const double *)nullptr, g.rows, g.nlat, grid, ld, ext_col, (double2 *)nullptr, 0LL, 0);
+        else
+            hipLaunchKernelGGL((k_synth_fft_c<true, kVarPlain>), dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft,
+                               fb_p, fb_m, g.P, Lc, (const double *)nullptr, g.rows, g.nlat, grid, ld, ext_col, fws,
+                               ws_stride(g.fft.m), kChunkCap);
+    }
+    const bool ok = HIP_OK(hipGetLastError());
+    ws_release(lease, st);
+    return ok ? ST_OK : ST_OTHER;
+}
+
+static int launch_plx_analysis_c(const Geometry &g, const double2 *grid, long long ld, int Lc, const double *plx,
+                                 int norm, double2 *cilm, int d1, int d2, hipStream_t st)
+{
+    const size_t nf = (size_t)(Lc + 1) * g.P;
+    const bool lds_fft = fft_in_lds(g.fft.m);
+    const int nwg = std::min(g.P, kWsGroups);
+    const size_t nws = lds_fft ? 0 : (size_t)nwg * ws_stride(g.fft.m);
+    WsLease lease;
+    if (int rc = ws_acquire(lease, 2 * nf * sizeof(double4) + nws * sizeof(double2), st)) return rc;
+    double4 *gb_p = (double4 *)lease.p, *gb_m = gb_p + nf;
+    double2 *fws = (double2 *)(gb_m + nf);
+    zero_outside((double *)cilm, 2 * d1, 2 * d2, 2 * d2, 0, 0, st);
+    {
+        ProfScope ps(SHTG_K_ANAL_FFT, st);
+        const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
+        if (lds_fft)
+            hipLaunchKernelGGL((by_variant(plan_variant(g.fft), k_anal_fft_c<false, kVarPlain>,
+                                           k_anal_fft_c<false, kVarSplit>, k_anal_fft_c<false, kVarPrime>)),
+                               dim3(g.P), dim3(plan_threads(g.fft)), lds, st, g.fft, grid, ld, (const double *)nullptr,
+                               g.rows, g.nlat, g.w, g.P, Lc, gb_p, gb_m, (double2 *)nullptr, 0LL, 0);
+        else
+            hipLaunchKernelGGL((k_anal_fft_c<true, kVarPlain>), dim3(nwg), dim3(kFftMaxThreads), kMaxLds, st, g.fft,
+                               grid, ld, (const double *)nullptr, g.rows, g.nlat, g.w, g.P, Lc, gb_p, gb_m, fws,
+                               ws_stride(g.fft.m), kChunkCap);
+    }
+    {
+        ProfScope ps(SHTG_K_ANAL_LEG, st);
+        const int nblk = (Lc + kWave) / kWave;
+        hipLaunchKernelGGL(k_plx_anal_leg_c, dim3((Lc + 1) * nblk), dim3(kWave), 0, st, gb_p, gb_m, g.P, g.nlat, Lc,
+                           plx, g.nlat, norm, nblk, cilm, d1, d2);
+    }
+    const bool ok = HIP_OK(hipGetLastError());
+    ws_release(lease, st);
+    return ok ? ST_OK : ST_OTHER;
+}
+
 }  // namespace shtg
 
 extern "C" {
@@ -2077,6 +2149,44 @@ int shtg_shexpand_glq_plx(const double *grid, int grid_d0, int grid_d1, const do
     CallTables t;
     if ((st = plx_tables(sh.lmax, w, sh.Lc, t, hs))) return st;
     return launch_plx_analysis(*t.g, grid, grid_d1, sh.Lc, plx, norm, cilm, cilm_d1, cilm_d2, hs);
+}
+
+int shtg_makegrid_glqc_plx(const double *cilm, int cilm_d1, int cilm_d2, int lmax, const double *plx, int norm,
+                           int csphase, int lmax_calc, int extend, double *grid, int grid_d0, int grid_d1,
+                           void *stream)
+{
+    Shape sh;
+    int st = check_makegrid_glq_plx(cilm_d1, cilm_d2, lmax, plx, norm, csphase, lmax_calc, extend, grid_d0, grid_d1,
+                                    sh);
+    if (st) return st;
+    if (!cilm || !grid) return ST_DIMS;
+    hipStream_t hs = (hipStream_t)stream;
+    if (sh.Lc == 0) {
+        // a degree-0 field: every node cilm(1,1,1) P_00 (src/MakeGridGLQC.f95:411-421)
+        const double pm2 = (norm == 4) ? 1.0 / std::sqrt(4.0 * std::acos(-1.0)) : 1.0;
+        const int nlong_out = 2 * lmax + 1 + extend;
+        hipLaunchKernelGGL(k_fill_c, dim3(grid_blocks((long long)(lmax + 1) * nlong_out)), dim3(kThreads), 0, hs,
+                           (double2 *)grid, lmax + 1, nlong_out, (long long)grid_d1, (const double2 *)cilm, pm2);
+        return HIP_OK(hipGetLastError()) ? ST_OK : ST_OTHER;
+    }
+    CallTables t;
+    if ((st = plx_tables(lmax, nullptr, sh.Lc, t, hs))) return st;
+    return launch_plx_synthesis_c(*t.g, (const double2 *)cilm, cilm_d1, cilm_d2, sh.Lc, plx, extend, (double2 *)grid,
+                                  grid_d1, hs);
+}
+
+int shtg_shexpand_glqc_plx(const double *grid, int grid_d0, int grid_d1, const double *w, int nw, const double *plx,
+                           int norm, int csphase, int lmax_calc, double *cilm, int cilm_d1, int cilm_d2, void *stream)
+{
+    Shape sh;
+    int st = check_shexpand_glq_plx(grid_d0, grid_d1, w, nw, plx, norm, csphase, lmax_calc, cilm_d1, cilm_d2, sh);
+    if (st) return st;
+    if (!cilm || !grid) return ST_DIMS;
+    hipStream_t hs = (hipStream_t)stream;
+    CallTables t;
+    if ((st = plx_tables(sh.lmax, w, sh.Lc, t, hs))) return st;
+    return launch_plx_analysis_c(*t.g, (const double2 *)grid, grid_d1, sh.Lc, plx, norm, (double2 *)cilm, cilm_d1,
+                                 cilm_d2, hs);
 }
 
 int shtg_dh_polar_orders(int lmax, int *kmax)

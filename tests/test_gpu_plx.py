@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import coef_err, for_norm, grid_err, rand_cilm
+from helpers import coef_err, for_norm, grid_err, rand_cilm, rand_cilm_c
 
 pytestmark = pytest.mark.gpu
 
@@ -146,6 +146,46 @@ def test_high_degree_against_on_the_fly(B, L):
     c2 = B.SHExpandGLQ(g2, w, zero)
     assert coef_err(c1, c2) <= TOL
     assert coef_err(c1, c) <= 1e-11
+
+
+CPLX_CASES = [(L, norm, cs, e) for L in (0, 1, 2, 7, 12, 33, 100) for norm in (1, 2, 3, 4) for cs in (1, -1)
+              for e in (0, 1) if not (norm == 3 and L > 12)]
+
+
+@pytest.mark.parametrize("L,norm,cs,extend", CPLX_CASES)
+def test_complex_transforms_against_oracle(B, L, norm, cs, extend):
+    """MakeGridGLQC / SHExpandGLQC with a table of the complex normalisation
+    (cnorm = 1) against the oracle's plx branch on the same table."""
+    print(f"case plx complex L={L} norm={norm} cs={cs} e={extend}", flush=True)
+    rng = np.random.default_rng(4000 + 10 * L + norm)
+    c = rand_cilm_c(rng, L, norm)
+    zero, w, plx = B.SHGLQ_plx(L, norm=norm, csphase=cs, cnorm=1)
+    t = plx.cpu().numpy()
+    g_ref = oracle.MakeGridGLQC_plx(c, t, norm=norm, csphase=cs, extend=extend)
+    g = B.MakeGridGLQC_plx(c, plx, norm=norm, csphase=cs, extend=extend)
+    assert g.shape == g_ref.shape
+    assert grid_err(g, g_ref) <= TOL
+    gin = g_ref[:, : 2 * L + 1]
+    c_ref = oracle.SHExpandGLQC_plx(gin, w, t, norm=norm, csphase=cs)
+    c2 = B.SHExpandGLQC_plx(gin, w, plx, norm=norm, csphase=cs)
+    assert c2.shape == c_ref.shape
+    assert coef_err(c2, c_ref) <= (TOL if norm != 3 else 1e-10)
+    if L >= 2:
+        lc = L // 2
+        gr = rng.standard_normal(gin.shape) + 1j * rng.standard_normal(gin.shape)
+        assert coef_err(B.SHExpandGLQC_plx(gr, w, plx, norm=norm, csphase=cs, lmax_calc=lc),
+                        oracle.SHExpandGLQC_plx(gr, w, t, norm=norm, csphase=cs, lmax_calc=lc)) <= TOL
+
+
+def test_complex_high_degree_against_on_the_fly(B):
+    L = 700
+    rng = np.random.default_rng(L)
+    c = rand_cilm_c(rng, L)
+    zero, w, plx = B.SHGLQ_plx(L, norm=1, cnorm=1)
+    g = B.MakeGridGLQC_plx(c, plx)
+    g2 = B.MakeGridGLQC(c, zero)
+    assert grid_err(g, g2) <= TOL
+    assert coef_err(B.SHExpandGLQC_plx(g2, w, plx), B.SHExpandGLQC(g2, w, zero)) <= TOL
 
 
 def test_exit_statuses(B):

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import coef_err, for_norm, grid_err, rand_cilm
+from helpers import coef_err, for_norm, grid_err, rand_cilm, rand_cilm_c
 
 NEEDS_REF = pytest.mark.skipif(oracle.ref_lib() is None, reason="reference helpers not built (oracle/_ref)")
 # the oracle's functions recur as the transforms do; PlmSchmidt recurs
@@ -94,6 +94,32 @@ def test_transforms_with_table_equal_on_the_fly(norm, cs):
             g1 = oracle.MakeGridGLQ(c, zero, norm=norm, csphase=cs, lmax_calc=lc)
             g2 = oracle.MakeGridGLQ_plx(c, plx, norm=norm, csphase=cs, lmax_calc=lc)
             assert grid_err(g2, g1) <= 1e-13, L
+
+
+@pytest.mark.parametrize("norm", [1, 2, 3, 4])
+@pytest.mark.parametrize("cs", [1, -1])
+def test_complex_transforms_with_table_equal_on_the_fly(norm, cs):
+    """MakeGridGLQC / SHExpandGLQC with a table of the complex normalisation
+    (cnorm = 1) against their on-the-fly branches."""
+    rng = np.random.default_rng(20 * norm + (cs > 0))
+    for L in ((0, 1, 2, 7, 12) if norm == 3 else (0, 1, 2, 7, 40)):
+        c = rand_cilm_c(rng, L, norm)
+        zero, w = oracle.SHGLQ(L)
+        plx = oracle.plx_table(L, zero, norm, cs, 1)
+        for ext in (0, 1):
+            g1 = oracle.MakeGridGLQC(c, zero, norm=norm, csphase=cs, extend=ext)
+            g2 = oracle.MakeGridGLQC_plx(c, plx, norm=norm, csphase=cs, extend=ext)
+            assert grid_err(g2, g1) <= 1e-13, (L, ext)
+        g = g1[:, : 2 * L + 1]
+        c1 = oracle.SHExpandGLQC(g, w, zero, norm=norm, csphase=cs)
+        c2 = oracle.SHExpandGLQC_plx(g, w, plx, norm=norm, csphase=cs)
+        assert coef_err(c2, c1) <= (1e-12 if norm == 3 else 1e-13), L
+        if L >= 2:
+            gr = rng.standard_normal(g.shape) + 1j * rng.standard_normal(g.shape)
+            lc = L // 2
+            c1 = oracle.SHExpandGLQC(gr, w, zero, norm=norm, csphase=cs, lmax_calc=lc)
+            c2 = oracle.SHExpandGLQC_plx(gr, w, plx, norm=norm, csphase=cs, lmax_calc=lc)
+            assert coef_err(c2, c1) <= (1e-12 if norm == 3 else 1e-13), L
 
 
 def test_plx_branch_exit_codes():

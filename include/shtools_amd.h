@@ -171,16 +171,20 @@ int shtg_makegravgrid_dh(const double *cilm, int cilm_d1, int cilm_d2, int lmax,
  * geocentric_lat (degrees), by Somigliana's closed form; into *out. */
 int shtg_normal_gravity(double geocentric_lat, double gm, double omega, double a, double b, double *out);
 
-/* SHMultiply: src/SHMultiply.f95:1-368 (f2py src/pyshtools.pyf:596-620, which
- * hides precomp = 0).  Coefficients [2][out_d1][out_d2] (at least
- * lmax1+lmax2+1 square) of the pointwise product of the fields of the
- * leading (lmax1+1)^2 and (lmax2+1)^2 blocks of cilm1 and cilm2: both
- * synthesised on the Gauss-Legendre grid of degree lmax1+lmax2, multiplied,
- * and expanded.  The reference's optional precomputed Legendre table (plx)
- * has no counterpart: the GPU recomputes the functions in every pass. */
+/* SHMultiply: src/SHMultiply.f95:1-368, include/shtools.h:254-265 (f2py
+ * src/pyshtools.pyf:596-620 fixes precomp = 0).  Coefficients
+ * [2][out_d1][out_d2] (at least lmax1+lmax2+1 square) of the pointwise
+ * product of the fields of the leading (lmax1+1)^2 and (lmax2+1)^2 blocks of
+ * cilm1 and cilm2: both synthesised on the Gauss-Legendre grid of degree
+ * lmax1+lmax2, multiplied, and expanded.  precomp = 0 recurs the Legendre
+ * functions in every pass; precomp = 1 first builds SHGLQ's table of them at
+ * the nodes (as shtg_shglq_plx, (lmax1+lmax2+1)^2 (lmax1+lmax2+2)/2 doubles
+ * of stream-ordered device memory for the call) and reads it in every pass
+ * (src/SHMultiply.f95:281-325); other values are exit status 2, and
+ * precomp = 1 under stream capture is status 6. */
 int shtg_shmultiply(double *cilmout, int out_d1, int out_d2, const double *cilm1, int c1_d1, int c1_d2,
-                    int lmax1, const double *cilm2, int c2_d1, int c2_d2, int lmax2, int norm, int csphase,
-                    void *stream);
+                    int lmax1, const double *cilm2, int c2_d1, int c2_d2, int lmax2, int precomp, int norm,
+                    int csphase, void *stream);
 
 /* The polar skip of the DH transforms (host; no counterpart in the
  * reference, which sums every term): kmax[s] for the ring-pair slots

@@ -168,7 +168,7 @@ def lib():
         L.or_makegradientdh.argtypes = [dp, i, i, i, i, i, i, ctypes.c_double, dp, dp, i, i]
         d = ctypes.c_double
         L.or_makegravgriddh.argtypes = [dp, i, i, i, d, d, d, d, i, i, d, i, d, i, dp, dp, dp, dp, dp, i, i]
-        L.or_shmultiply.argtypes = [dp, i, i, dp, i, i, i, dp, i, i, i, i, i]
+        L.or_shmultiply.argtypes = [dp, i, i, dp, i, i, i, dp, i, i, i, i, i, i]
         L.or_normalgravity.argtypes = [d, d, d, d, d]
         L.or_normalgravity.restype = d
         L.or_r2c.argtypes = [i, dp, dp]
@@ -440,9 +440,10 @@ def MakeGravGridDH(cilm, gm, r0, lmax=None, a=None, f=0.0, sampling=2, lmax_calc
     return tuple(out)
 
 
-def SHMultiply(sh1, sh2, lmax1=None, lmax2=None, norm=1, csphase=1):
+def SHMultiply(sh1, sh2, lmax1=None, lmax2=None, norm=1, csphase=1, precomp=0):
     """Coefficients of the product of two fields (src/SHMultiply.f95;
-    f2py src/pyshtools.pyf:596-620)."""
+    f2py src/pyshtools.pyf:596-620, which fixes precomp = 0; precomp = 1 is
+    the table branch, src/SHMultiply.f95:281-325)."""
     sh1 = np.ascontiguousarray(sh1, dtype=np.float64)
     sh2 = np.ascontiguousarray(sh2, dtype=np.float64)
     if lmax1 is None:
@@ -452,7 +453,7 @@ def SHMultiply(sh1, sh2, lmax1=None, lmax2=None, norm=1, csphase=1):
     n = lmax1 + lmax2 + 1
     out = np.zeros((2, n, n))
     _check(lib().or_shmultiply(_p(out), n, n, _p(sh1), sh1.shape[1], sh1.shape[2], lmax1, _p(sh2), sh2.shape[1],
-                               sh2.shape[2], lmax2, norm, csphase))
+                               sh2.shape[2], lmax2, precomp, norm, csphase))
     return out
 
 

@@ -2418,11 +2418,13 @@ int or_makegravgriddh(const double *cilm, int d1, int d2, int lmax, double gm, d
 /* Spherical harmonic coefficients (degree lmax1 + lmax2) of the pointwise
  * product of two real fields: both synthesised on the Gauss-Legendre grid of
  * degree lmax1 + lmax2, multiplied and expanded.  Restates
- * src/SHMultiply.f95:1-368 for precomp = 0 (the f2py interface hides precomp,
- * src/pyshtools.pyf:596-620): argument checks :61-180, nodes :182-231, the
- * leading (2, lmax+1, lmax+1) blocks of the inputs :330-350. */
+ * src/SHMultiply.f95:1-368: argument checks :61-180 (precomp :137-151),
+ * nodes :182-231, the leading (2, lmax+1, lmax+1) blocks of the inputs; with
+ * precomp = 0 (the only mode of the f2py interface, src/pyshtools.pyf:596-620)
+ * the transforms recur the functions (:256-280, :330-350), with precomp = 1
+ * they read SHGLQ's table of them at the nodes (:281-325). */
 int or_shmultiply(double *out, int od1, int od2, const double *c1, int c1d1, int c1d2, int lmax1,
-                  const double *c2, int c2d1, int c2d2, int lmax2, int norm, int csphase)
+                  const double *c2, int c2d1, int c2d2, int lmax2, int precomp, int norm, int csphase)
 {
     if (lmax1 < 0 || lmax2 < 0) return 2;
     if (c1d1 < lmax1 + 1 || c1d2 < lmax1 + 1) return 1;
@@ -2430,15 +2432,21 @@ int or_shmultiply(double *out, int od1, int od2, const double *c1, int c1d1, int
     int lmaxout = lmax1 + lmax2;
     if (od1 < lmaxout + 1 || od2 < lmaxout + 1) return 1;
     if (csphase != 1 && csphase != -1) return 2;
+    if (precomp != 0 && precomp != 1) return 2;
     if (norm < 1 || norm > 4) return 2;
     int nlat = lmaxout + 1, nlong = 2 * lmaxout + 1;
     double *zero = (double *)malloc(sizeof(double) * nlat), *w = (double *)malloc(sizeof(double) * nlat);
+    double *plx = NULL;
     double *s1 = (double *)calloc((size_t)2 * (lmax1 + 1) * (lmax1 + 1), sizeof(double));
     double *s2 = (double *)calloc((size_t)2 * (lmax2 + 1) * (lmax2 + 1), sizeof(double));
     double *g1 = (double *)malloc(sizeof(double) * (size_t)nlat * nlong);
     double *g2 = (double *)malloc(sizeof(double) * (size_t)nlat * nlong);
     int st = (zero && w && s1 && s2 && g1 && g2) ? 0 : 3;
     if (!st) st = or_preglq(nlat, zero, w);
+    if (!st && precomp == 1) {
+        plx = (double *)malloc(sizeof(double) * (size_t)nlat * ((size_t)nlat * (nlat + 1) / 2));
+        st = plx ? or_plx_table(lmaxout, zero, nlat, norm, csphase, 0, plx) : 3;
+    }
     if (!st) {
         for (int i = 0; i < 2; i++)
             for (int l = 0; l <= lmax1; l++)
@@ -2448,17 +2456,23 @@ int or_shmultiply(double *out, int od1, int od2, const double *c1, int c1d1, int
             for (int l = 0; l <= lmax2; l++)
                 for (int m = 0; m <= lmax2; m++)
                     s2[((size_t)i * (lmax2 + 1) + l) * (lmax2 + 1) + m] = c2[((size_t)i * c2d1 + l) * c2d2 + m];
-        st = or_makegridglq(s1, lmax1 + 1, lmax1 + 1, lmaxout, zero, nlat, norm, csphase, lmaxout, 0, g1, nlat,
-                            nlong);
+        st = plx ? or_makegridglq_plx(s1, lmax1 + 1, lmax1 + 1, lmaxout, plx, norm, csphase, lmaxout, 0, g1, nlat,
+                                      nlong)
+                 : or_makegridglq(s1, lmax1 + 1, lmax1 + 1, lmaxout, zero, nlat, norm, csphase, lmaxout, 0, g1, nlat,
+                                  nlong);
     }
     if (!st)
-        st = or_makegridglq(s2, lmax2 + 1, lmax2 + 1, lmaxout, zero, nlat, norm, csphase, lmaxout, 0, g2, nlat,
-                            nlong);
+        st = plx ? or_makegridglq_plx(s2, lmax2 + 1, lmax2 + 1, lmaxout, plx, norm, csphase, lmaxout, 0, g2, nlat,
+                                      nlong)
+                 : or_makegridglq(s2, lmax2 + 1, lmax2 + 1, lmaxout, zero, nlat, norm, csphase, lmaxout, 0, g2, nlat,
+                                  nlong);
     if (!st) {
         for (size_t e = 0; e < (size_t)nlat * nlong; e++) g1[e] *= g2[e];
-        st = or_shexpandglq(g1, nlat, nlong, lmaxout, w, nlat, zero, nlat, norm, csphase, lmaxout, out, od1, od2);
+        st = plx ? or_shexpandglq_plx(g1, nlat, nlong, lmaxout, w, nlat, plx, norm, csphase, lmaxout, out, od1, od2)
+                 : or_shexpandglq(g1, nlat, nlong, lmaxout, w, nlat, zero, nlat, norm, csphase, lmaxout, out, od1,
+                                  od2);
     }
-    free(zero); free(w); free(s1); free(s2); free(g1); free(g2);
+    free(zero); free(w); free(s1); free(s2); free(g1); free(g2); free(plx);
     return st;
 }
 

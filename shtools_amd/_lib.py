@@ -40,7 +40,7 @@ SIGNATURES = {
                             _dp],
     "shtg_dh_polar_orders": [_c_int, ctypes.POINTER(ctypes.c_int)],
     "shtg_shmultiply": [_vp, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _c_int,
-                        _c_int, _vp],
+                        _c_int, _c_int, _vp],
     "shtg_makegrid_dh_adjoint": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int,
                                  _c_int, _vp],
     "shtg_shexpand_dh_adjoint": [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _vp],

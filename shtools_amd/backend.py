@@ -555,9 +555,12 @@ def NormalGravity(geocentric_lat, gm, omega, a, b):
 
 # ------------------------------------------------------------ products
 
-def SHMultiply(sh1, sh2, lmax1=None, lmax2=None, norm=1, csphase=1):
+def SHMultiply(sh1, sh2, lmax1=None, lmax2=None, norm=1, csphase=1, precomp=0):
     """Coefficients of degree lmax1 + lmax2 of the product of two real fields
-    (src/SHMultiply.f95; f2py src/pyshtools.pyf:596-620)."""
+    (src/SHMultiply.f95; f2py src/pyshtools.pyf:596-620).  precomp, which the
+    f2py interface fixes at 0, is the Fortran routine's optional argument:
+    1 builds SHGLQ's table of the Legendre functions at the nodes and reads
+    it in the three transforms (src/SHMultiply.f95:281-325)."""
     torch = _torch()
     as_tensor = _out_kind(sh1, sh2)
     dev = _device_of(sh1, sh2)
@@ -575,8 +578,8 @@ def SHMultiply(sh1, sh2, lmax1=None, lmax2=None, norm=1, csphase=1):
     out = torch.empty((2, n, n), dtype=torch.float64, device=dev)
     with torch.cuda.device(dev):
         st = _lib.lib().shtg_shmultiply(out.data_ptr(), n, n, a.data_ptr(), a.shape[1], a.shape[2], int(lmax1),
-                                        b.data_ptr(), b.shape[1], b.shape[2], int(lmax2), int(norm), int(csphase),
-                                        _stream(dev))
+                                        b.data_ptr(), b.shape[1], b.shape[2], int(lmax2), int(precomp), int(norm),
+                                        int(csphase), _stream(dev))
     _check(st)
     return _finish(out, as_tensor)
 

@@ -1855,48 +1855,6 @@ static const std::pair<std::vector<double>, std::vector<double>> *glq_cached(int
     return &(cache[nlat] = {std::move(z), std::move(w)});
 }
 
-int shtg_shmultiply(double *cilmout, int out_d1, int out_d2, const double *cilm1, int c1_d1, int c1_d2, int lmax1,
-                    const double *cilm2, int c2_d1, int c2_d2, int lmax2, int norm, int csphase, void *stream)
-{
-    // argument checks in the order of src/SHMultiply.f95:61-180
-    if (lmax1 < 0 || lmax2 < 0 || lmax1 > kMaxArgDegree || lmax2 > kMaxArgDegree - lmax1) return ST_BOUNDS;
-    if (c1_d1 < lmax1 + 1 || c1_d2 < lmax1 + 1) return ST_DIMS;
-    if (c2_d1 < lmax2 + 1 || c2_d2 < lmax2 + 1) return ST_DIMS;
-    const int lmaxout = lmax1 + lmax2;
-    if (out_d1 < lmaxout + 1 || out_d2 < lmaxout + 1) return ST_DIMS;
-    if (csphase != 1 && csphase != -1) return ST_BOUNDS;
-    if (norm < 1 || norm > 4) return ST_BOUNDS;
-    if (!cilmout || !cilm1 || !cilm2) return ST_DIMS;
-    const int nlat = lmaxout + 1, nlong = 2 * lmaxout + 1;
-    const std::pair<std::vector<double>, std::vector<double>> *zw;
-    {
-        std::lock_guard<std::mutex> lk(g_mu);
-        zw = glq_cached(nlat);  // entries are never changed once made
-    }
-    if (!zw) return ST_OTHER;
-    CallTables t;
-    int st = get_tables(false, lmaxout, 0, zw->first.data(), zw->second.data(), norm, csphase, lmaxout, t,
-                        (hipStream_t)stream);
-    if (st) return st;
-    hipStream_t hs = (hipStream_t)stream;
-    const size_t ng = (size_t)nlat * nlong;
-    void *ws = nullptr;
-    if (!HIP_OK(pool_alloc(&ws, 2 * ng * sizeof(double), hs))) return ST_ALLOC;
-    double *g1 = (double *)ws, *g2 = g1 + ng;
-    // each factor synthesised on the product's grid from its leading
-    // (lmax+1)^2 block (src/SHMultiply.f95:330-350)
-    st = launch_synthesis(*t.g, t.rt, t.sd, cilm1, c1_d1, c1_d2, lmax1, nlat, 0, g1, nlong, hs);
-    if (!st) st = launch_synthesis(*t.g, t.rt, t.sd, cilm2, c2_d1, c2_d2, lmax2, nlat, 0, g2, nlong, hs);
-    if (!st) {
-        hipLaunchKernelGGL(k_mul_inplace, dim3(grid_blocks((long long)ng)), dim3(kThreads), 0, hs, g1, g2,
-                           (long long)ng);
-        if (!HIP_OK(hipGetLastError())) st = ST_OTHER;
-    }
-    if (!st) st = launch_analysis(*t.g, t.rt, t.sd, g1, nlong, lmaxout, cilmout, out_d1, out_d2, hs);
-    (void)hipFreeAsync(ws, hs);
-    return st;
-}
-
 // ------------------------------------------------------------ plx tables
 //
 // The GLQ routines' optional table of Legendre functions at the nodes (the
@@ -2066,6 +2024,38 @@ static int launch_plx_analysis_c(const Geometry &g, const double2 *grid, long lo
     return ok ? ST_OK : ST_OTHER;
 }
 
+// SHGLQ's table for degree lmax at the nodes zero (host, lmax + 1 of them)
+// into plx, from the recursion tables of its normalisation.
+static int launch_plx_table(const DevRecurrence &rt, const DevSeeds &sd, const std::vector<double> &zero, int lmax,
+                            double *plx, hipStream_t st)
+{
+    const int n = lmax + 1;
+    // the nodes on the device, from the caller's host copy (which outlives
+    // the copy), in a block of the per-device intermediates
+    WsLease lease;
+    if (int rc = ws_acquire(lease, (size_t)n * sizeof(double), st)) return rc;
+    bool ok = HIP_OK(hipMemcpyAsync(lease.p, zero.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+    if (ok) {
+        const int nblk = ((n + 1) / 2 + kWave - 1) / kWave;
+        hipLaunchKernelGGL(k_plx_table, dim3((lmax + 1) * nblk), dim3(kWave), 0, st, rt.alpha, rt.s, rt.L, sd.seed,
+                           (const double *)lease.p, n, lmax, nblk, plx);
+        ok = HIP_OK(hipGetLastError());
+    }
+    ws_release(lease, st);
+    return ok ? ST_OK : ST_OTHER;
+}
+
+// A degree-0 field on the Gauss-Legendre grid: every node cilm(1,1,1) P_00
+// (src/MakeGridGLQ.f95:408-420, which precedes the plx branch).
+static int fill_degree0(const double *cilm, int norm, int nlat, int nlong_out, double *grid, long long ld,
+                        hipStream_t st)
+{
+    const double pm2 = (norm == 4) ? 1.0 / std::sqrt(4.0 * std::acos(-1.0)) : 1.0;
+    hipLaunchKernelGGL(k_fill, dim3(grid_blocks((long long)nlat * nlong_out)), dim3(kThreads), 0, st, grid, nlat,
+                       nlong_out, ld, cilm, pm2);
+    return HIP_OK(hipGetLastError()) ? ST_OK : ST_OTHER;
+}
+
 }  // namespace shtg
 
 extern "C" {
@@ -2099,20 +2089,8 @@ int shtg_shglq_plx(int lmax, double *zero, double *w, double *plx, int norm, int
     }
     std::copy(zw->first.begin(), zw->first.end(), zero);
     std::copy(zw->second.begin(), zw->second.end(), w);
-    // the nodes on the device, from the cache's host copy (kept for the
-    // process), in a block of the per-device intermediates
-    WsLease lease;
-    if (int rc = ws_acquire(lease, (size_t)n * sizeof(double), hs)) return rc;
-    bool ok = HIP_OK(hipMemcpyAsync(lease.p, zw->first.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice,
-                                    hs));
-    if (ok) {
-        const int nblk = ((n + 1) / 2 + kWave - 1) / kWave;
-        hipLaunchKernelGGL(k_plx_table, dim3((lmax + 1) * nblk), dim3(kWave), 0, hs, rt.alpha, rt.s, rt.L, sd.seed,
-                           (const double *)lease.p, n, lmax, nblk, plx);
-        ok = HIP_OK(hipGetLastError());
-    }
-    ws_release(lease, hs);
-    return ok ? ST_OK : ST_OTHER;
+    // the cache's host copy of the nodes is kept for the process
+    return launch_plx_table(rt, sd, zw->first, lmax, plx, hs);
 }
 
 int shtg_makegrid_glq_plx(const double *cilm, int cilm_d1, int cilm_d2, int lmax, const double *plx, int norm,
@@ -2125,14 +2103,7 @@ int shtg_makegrid_glq_plx(const double *cilm, int cilm_d1, int cilm_d2, int lmax
     if (st) return st;
     if (!cilm || !grid) return ST_DIMS;
     hipStream_t hs = (hipStream_t)stream;
-    if (sh.Lc == 0) {
-        // a degree-0 field: every node cilm(1,1,1) P_00 (src/MakeGridGLQ.f95:408-420)
-        const double pm2 = (norm == 4) ? 1.0 / std::sqrt(4.0 * std::acos(-1.0)) : 1.0;
-        const int nlong_out = 2 * lmax + 1 + extend;
-        hipLaunchKernelGGL(k_fill, dim3(grid_blocks((long long)(lmax + 1) * nlong_out)), dim3(kThreads), 0, hs, grid,
-                           lmax + 1, nlong_out, (long long)grid_d1, cilm, pm2);
-        return HIP_OK(hipGetLastError()) ? ST_OK : ST_OTHER;
-    }
+    if (sh.Lc == 0) return fill_degree0(cilm, norm, lmax + 1, 2 * lmax + 1 + extend, grid, grid_d1, hs);
     CallTables t;
     if ((st = plx_tables(lmax, nullptr, sh.Lc, t, hs))) return st;
     return launch_plx_synthesis(*t.g, cilm, cilm_d1, cilm_d2, sh.Lc, plx, extend, grid, grid_d1, hs);
@@ -2187,6 +2158,69 @@ int shtg_shexpand_glqc_plx(const double *grid, int grid_d0, int grid_d1, const d
     if ((st = plx_tables(sh.lmax, w, sh.Lc, t, hs))) return st;
     return launch_plx_analysis_c(*t.g, (const double2 *)grid, grid_d1, sh.Lc, plx, norm, (double2 *)cilm, cilm_d1,
                                  cilm_d2, hs);
+}
+
+int shtg_shmultiply(double *cilmout, int out_d1, int out_d2, const double *cilm1, int c1_d1, int c1_d2, int lmax1,
+                    const double *cilm2, int c2_d1, int c2_d2, int lmax2, int precomp, int norm, int csphase,
+                    void *stream)
+{
+    // argument checks in the order of src/SHMultiply.f95:61-180
+    if (lmax1 < 0 || lmax2 < 0 || lmax1 > kMaxArgDegree || lmax2 > kMaxArgDegree - lmax1) return ST_BOUNDS;
+    if (c1_d1 < lmax1 + 1 || c1_d2 < lmax1 + 1) return ST_DIMS;
+    if (c2_d1 < lmax2 + 1 || c2_d2 < lmax2 + 1) return ST_DIMS;
+    const int lmaxout = lmax1 + lmax2;
+    if (out_d1 < lmaxout + 1 || out_d2 < lmaxout + 1) return ST_DIMS;
+    if (csphase != 1 && csphase != -1) return ST_BOUNDS;
+    if (precomp != 0 && precomp != 1) return ST_BOUNDS;
+    if (norm < 1 || norm > 4) return ST_BOUNDS;
+    if (!cilmout || !cilm1 || !cilm2) return ST_DIMS;
+    const int nlat = lmaxout + 1, nlong = 2 * lmaxout + 1;
+    const std::pair<std::vector<double>, std::vector<double>> *zw;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        zw = glq_cached(nlat);  // entries are never changed once made
+    }
+    if (!zw) return ST_OTHER;
+    CallTables t;
+    int st = get_tables(false, lmaxout, 0, zw->first.data(), zw->second.data(), norm, csphase, lmaxout, t,
+                        (hipStream_t)stream);
+    if (st) return st;
+    hipStream_t hs = (hipStream_t)stream;
+    if (precomp == 1 && capturing(hs)) return ST_CAPTURE;  // the table's nodes are copied from the host
+    const size_t ng = (size_t)nlat * nlong;
+    // precomp = 1: SHGLQ's table of the functions at the nodes, nlat values
+    // per (l, m), read by the transforms' table branches
+    // (src/SHMultiply.f95:281-325)
+    const size_t nplx = (precomp == 1) ? (size_t)nlat * ((size_t)nlat * (nlat + 1) / 2) : 0;
+    void *ws = nullptr;
+    if (!HIP_OK(pool_alloc(&ws, (2 * ng + nplx) * sizeof(double), hs))) return ST_ALLOC;
+    double *g1 = (double *)ws, *g2 = g1 + ng, *plx = (precomp == 1) ? g2 + ng : nullptr;
+    if (plx) {
+        st = launch_plx_table(t.rt, t.sd, zw->first, lmaxout, plx, hs);
+        // each factor synthesised on the product's grid from its leading
+        // (lmax+1)^2 block, the table read up to the factor's degree
+        auto factor = [&](const double *c, int d1, int d2, int L, double *g) {
+            return L == 0 ? fill_degree0(c, norm, nlat, nlong, g, nlong, hs)
+                          : launch_plx_synthesis(*t.g, c, d1, d2, L, plx, 0, g, nlong, hs);
+        };
+        if (!st) st = factor(cilm1, c1_d1, c1_d2, lmax1, g1);
+        if (!st) st = factor(cilm2, c2_d1, c2_d2, lmax2, g2);
+    } else {
+        // each factor synthesised on the product's grid from its leading
+        // (lmax+1)^2 block (src/SHMultiply.f95:330-350)
+        st = launch_synthesis(*t.g, t.rt, t.sd, cilm1, c1_d1, c1_d2, lmax1, nlat, 0, g1, nlong, hs);
+        if (!st) st = launch_synthesis(*t.g, t.rt, t.sd, cilm2, c2_d1, c2_d2, lmax2, nlat, 0, g2, nlong, hs);
+    }
+    if (!st) {
+        hipLaunchKernelGGL(k_mul_inplace, dim3(grid_blocks((long long)ng)), dim3(kThreads), 0, hs, g1, g2,
+                           (long long)ng);
+        if (!HIP_OK(hipGetLastError())) st = ST_OTHER;
+    }
+    if (!st)
+        st = plx ? launch_plx_analysis(*t.g, g1, nlong, lmaxout, plx, norm, cilmout, out_d1, out_d2, hs)
+                 : launch_analysis(*t.g, t.rt, t.sd, g1, nlong, lmaxout, cilmout, out_d1, out_d2, hs);
+    (void)hipFreeAsync(ws, hs);
+    return st;
 }
 
 int shtg_dh_polar_orders(int lmax, int *kmax)

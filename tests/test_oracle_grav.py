@@ -14,8 +14,9 @@ the restated code paths:
   the oracle's MakeGridDH and MakeGradientDH of the continued coefficients
   (r0/a)^l C_lm, which are pinned on their own in test_oracle.py;
 * SHMultiply: the square of P_10 in every normalisation (closed form), the
-  unit field as identity, exact commutation, and the product's DH synthesis
-  against the pointwise product of the factors' syntheses.
+  unit field as identity, exact commutation, the product's DH synthesis
+  against the pointwise product of the factors' syntheses, and its table
+  branch (precomp = 1) against the recursion branch.
 """
 import numpy as np
 import pytest
@@ -197,6 +198,26 @@ def test_shmultiply_is_the_pointwise_product(norm, csphase):
     assert np.abs(go - ga * gb).max() <= 1e-12 * np.abs(ga * gb).max()
 
 
+@pytest.mark.parametrize("norm", [1, 2, 3, 4])
+@pytest.mark.parametrize("csphase", [1, -1])
+def test_shmultiply_table_branch_equals_recursion_branch(norm, csphase):
+    """precomp = 1 (SHGLQ's table of the functions at the nodes, read by the
+    transforms' plx branches; src/SHMultiply.f95:281-325) is the same product
+    as precomp = 0: a table of the wrong degree, node order or normalisation,
+    or a factor read with the wrong lmax, fails."""
+    rng = np.random.default_rng(40 + norm)
+    # the unnormalised functions overflow the product's degrees beyond ~15
+    # (src/MakeGridDH.f95:17-18)
+    cases = [(0, 0), (0, 3), (1, 0), (5, 3), (8, 6)] + ([] if norm == 3 else [(12, 9), (24, 17)])
+    for l1, l2 in cases:
+        a, b = rand_cilm(rng, l1), rand_cilm(rng, l2)
+        o0 = oracle.SHMultiply(a, b, norm=norm, csphase=csphase)
+        o1 = oracle.SHMultiply(a, b, norm=norm, csphase=csphase, precomp=1)
+        assert o1.shape == o0.shape
+        tol = 1e-12 if norm == 3 else 1e-13
+        assert np.abs(o1 - o0).max() <= tol * np.abs(o0).max(), (l1, l2)
+
+
 def test_shmultiply_errors():
     c = np.zeros((2, 3, 3))
     with pytest.raises(Exception):
@@ -205,3 +226,5 @@ def test_shmultiply_errors():
         oracle.SHMultiply(c, c, norm=5)
     with pytest.raises(Exception):
         oracle.SHMultiply(c, c, csphase=0)
+    with pytest.raises(Exception):
+        oracle.SHMultiply(c, c, precomp=2)

@@ -708,6 +708,20 @@ __global__ __launch_bounds__(kThreads) void k_pack_synth(const double *__restric
 // pairs per lane as its slots need.
 // ringr2: per-slot rho^2 of a radially continued synthesis (nullptr: none);
 // smin: per order the first slot with non-negligible functions (nullptr: 0).
+// Workgroup b of a plain Legendre launch: order m and slot block sb, with
+// the nb blocks of an order on one XCD (workgroups go to the 8 XCDs in turn,
+// b mod 8), so that their reads of the order's recursion and coefficient
+// rows share that XCD's L2; orders are dealt to the XCDs in turn, lowest
+// (longest) first.  Grid: 8 nb ceil((Lc+1)/8) workgroups; false for the
+// padding ones (m > Lc).
+__device__ __forceinline__ bool order_block(int b, int nb, int Lc, int &m, int &sb)
+{
+    const int t = b >> 3;
+    sb = t % nb;
+    m = 8 * (t / nb) + (b & 7);
+    return m <= Lc;
+}
+
 __global__ __launch_bounds__(kWave) void k_synth_leg(const double *__restrict__ alpha, int Lt,
                                                      const double *__restrict__ seed,
                                                      const double *__restrict__ ringz,
@@ -716,8 +730,8 @@ __global__ __launch_bounds__(kWave) void k_synth_leg(const double *__restrict__ 
                                                      const double *__restrict__ ringr2,
                                                      const int *__restrict__ smin, int kb)
 {
-    const int m = blockIdx.x / nsb;
-    const int sb = blockIdx.x - m * nsb;
+    int m, sb;
+    if (!order_block(blockIdx.x, nsb, Lc, m, sb)) return;
     const int slot0 = (smin ? smin[m] : 0) + sb * kb * kWave;
     if (slot0 >= P) return;
     const int kk = min(kb, (P - slot0 + kWave - 1) / kWave);
@@ -812,7 +826,8 @@ __global__ __launch_bounds__(kWave) void k_anal_leg(const double *__restrict__ a
                                                     double2 *__restrict__ cbuf, const int *__restrict__ smin,
                                                     int nblk, int bslots, long long cstride)
 {
-    const int m = blockIdx.x / nblk, b = blockIdx.x - m * nblk;
+    int m, b;
+    if (!order_block(blockIdx.x, nblk, Lc, m, b)) return;
     const int sa = (smin ? max(s0, smin[m]) : s0) + b * bslots;
     const int se = nblk > 1 ? min(P, sa + bslots) : P;
     double2 *out = cbuf + b * cstride + tri_offd(m, Lc);

@@ -487,6 +487,10 @@ static int anal_blocks(int Lc, int passes)
     return std::max(1, std::min(passes, (kLegWaves + Lc) / (Lc + 1)));
 }
 
+// workgroups of a plain Legendre launch over orders 0..Lc in nb blocks each
+// (order_block's mapping)
+static unsigned leg_grid(int Lc, int nb) { return 8u * (unsigned)nb * (unsigned)((Lc + 8) / 8); }
+
 // the analysis kernels by ring pairs per lane
 typedef decltype(&k_anal_leg<1>) AnalLegKernel;
 typedef decltype(&kb_anal_leg<1>) BatchAnalLegKernel;
@@ -616,7 +620,7 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
         ProfScope ps(SHTG_K_SYNTH_LEG, st);
         const int kb = synth_lanes(Lc, g.P);
         const int nsb = (g.P + kb * kWave - 1) / (kb * kWave);
-        hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nsb), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, rz, cpack,
+        hipLaunchKernelGGL(k_synth_leg, dim3(leg_grid(Lc, nsb)), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, rz, cpack,
                            Lc, g.P, nsb, fbuf, ring.r2, smin, kb);
     }
     {
@@ -681,7 +685,7 @@ static int launch_analysis(const Geometry &g, const DevRecurrence &rt, const Dev
     }
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);
-        hipLaunchKernelGGL(anal_leg_kernel(K), dim3((Lc + 1) * nblk), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed,
+        hipLaunchKernelGGL(anal_leg_kernel(K), dim3(leg_grid(Lc, nblk)), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed,
                            g.z, gbuf, Lc, s0, g.P, cbuf, (const int *)g.smin, nblk, bslots, (long long)npack);
     }
     ProfScope ps_un(SHTG_K_UNPACK, st);
@@ -722,9 +726,9 @@ static int launch_synthesis_c(const Geometry &g, const DevRecurrence &rt, const 
         ProfScope ps(SHTG_K_SYNTH_LEG, st);
         const int kb = synth_lanes(Lc, g.P);
         const int nsb = (g.P + kb * kWave - 1) / (kb * kWave);
-        hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nsb), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, cp_p,
+        hipLaunchKernelGGL(k_synth_leg, dim3(leg_grid(Lc, nsb)), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, cp_p,
                            Lc, g.P, nsb, fb_p, (const double *)nullptr, (const int *)nullptr, kb);
-        hipLaunchKernelGGL(k_synth_leg, dim3((Lc + 1) * nsb), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, cp_m,
+        hipLaunchKernelGGL(k_synth_leg, dim3(leg_grid(Lc, nsb)), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, cp_m,
                            Lc, g.P, nsb, fb_m, (const double *)nullptr, (const int *)nullptr, kb);
     }
     {
@@ -775,9 +779,9 @@ static int launch_analysis_c(const Geometry &g, const DevRecurrence &rt, const D
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);
         const AnalLegKernel kern = anal_leg_kernel(anal_lanes_per_slot(g.P - g.s0));
-        hipLaunchKernelGGL(kern, dim3(Lc + 1), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gb_p, Lc, g.s0, g.P,
+        hipLaunchKernelGGL(kern, dim3(leg_grid(Lc, 1)), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gb_p, Lc, g.s0, g.P,
                            cb_p, (const int *)nullptr, 1, 0, 0LL);
-        hipLaunchKernelGGL(kern, dim3(Lc + 1), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gb_m, Lc, g.s0, g.P,
+        hipLaunchKernelGGL(kern, dim3(leg_grid(Lc, 1)), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, gb_m, Lc, g.s0, g.P,
                            cb_m, (const int *)nullptr, 1, 0, 0LL);
     }
     {

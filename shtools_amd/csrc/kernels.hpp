@@ -123,7 +123,17 @@ __device__ __forceinline__ double ring_rescale(double u, int k) { return u < 0.0
 // continuation factor r0/r of the ring): the rescaled recursion
 // q_l = alpha_l z q_{l-1} - q_{l-2} becomes, for rho^(l-m) q_l,
 // q_l = alpha_l (rho z) q_{l-1} - rho^2 q_{l-2}, with ringz holding rho z.
-template <int K, bool kRho>
+//
+// kStage: the recursion and coefficient rows reach the wave in blocks of 64
+// degrees through LDS instead of as wave-uniform scalar loads: one vector
+// load per lane fetches a block while the previous one is in use (vector
+// loads complete in order, so waiting for one block does not wait for the
+// next; scalar loads are waited for all together, which leaves their
+// latency on every few degrees of a wave whose rows are not cached), and
+// the degree loop reads the block back as wave-wide broadcasts.  For the
+// launches of one field, whose longest waves bound them; workgroups are one
+// wave.
+template <int K, bool kRho, bool kStage>
 __device__ __forceinline__ void synth_leg_tile(const double *__restrict__ al, double sd,
                                                const double *__restrict__ ringz, const double *__restrict__ ringr2,
                                                const double2 *__restrict__ cp, int jmax, int P, int slot0,
@@ -145,32 +155,60 @@ __device__ __forceinline__ void synth_leg_tile(const double *__restrict__ al, do
     auto step = [&](double a, double z_, double qa, double qb, double r2_) {
         return kRho ? fma(a * z_, qa, -(r2_ * qb)) : fma(a * z_, qa, -qb);
     };
+    // the rows of degrees [jb, je] (jb even) from a, c at offset jb, then
+    // the last degree if it is even; j runs on across calls
     int j = 0;
-#pragma unroll 2
-    for (; j + 1 <= jmax; j += 2) {
-        const double a0 = al[j], a1 = al[j + 1];
-        const double2 c0 = cp[j], c1 = cp[j + 1];
+    auto degrees = [&](const double *__restrict__ a, const double2 *__restrict__ c, int jb, int je) {
+#pragma unroll 4
+        for (; j + 1 <= je; j += 2) {
+            const double a0 = a[j - jb], a1 = a[j - jb + 1];
+            const double2 c0 = c[j - jb], c1 = c[j - jb + 1];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const double q = step(a0, z[k], q1[k], q2[k], r2[k]);
-            eC[k] = fma(q, c0.x, eC[k]);
-            eS[k] = fma(q, c0.y, eS[k]);
-            const double q3 = step(a1, z[k], q, q1[k], r2[k]);
-            oC[k] = fma(q3, c1.x, oC[k]);
-            oS[k] = fma(q3, c1.y, oS[k]);
-            q2[k] = q;
-            q1[k] = q3;
+            for (int k = 0; k < K; ++k) {
+                const double q = step(a0, z[k], q1[k], q2[k], r2[k]);
+                eC[k] = fma(q, c0.x, eC[k]);
+                eS[k] = fma(q, c0.y, eS[k]);
+                const double q3 = step(a1, z[k], q, q1[k], r2[k]);
+                oC[k] = fma(q3, c1.x, oC[k]);
+                oS[k] = fma(q3, c1.y, oS[k]);
+                q2[k] = q;
+                q1[k] = q3;
+            }
         }
-    }
-    if (j == jmax) {
-        const double a0 = al[j];
-        const double2 c0 = cp[j];
+        if (je == jmax && j == jmax) {
+            const double a0 = a[j - jb];
+            const double2 c0 = c[j - jb];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const double q = step(a0, z[k], q1[k], q2[k], r2[k]);
-            eC[k] = fma(q, c0.x, eC[k]);
-            eS[k] = fma(q, c0.y, eS[k]);
+            for (int k = 0; k < K; ++k) {
+                const double q = step(a0, z[k], q1[k], q2[k], r2[k]);
+                eC[k] = fma(q, c0.x, eC[k]);
+                eS[k] = fma(q, c0.y, eS[k]);
+            }
         }
+    };
+    if constexpr (kStage) {
+        __shared__ double sal[kWave];
+        __shared__ double2 scp[kWave];
+        double pa = 0.0;
+        double2 pc = make_double2(0.0, 0.0);
+        if (lane <= jmax) {
+            pa = al[lane];
+            pc = cp[lane];
+        }
+        for (int jb = 0; jb <= jmax; jb += kWave) {
+            __syncthreads();  // the previous block's reads are done
+            sal[lane] = pa;
+            scp[lane] = pc;
+            __syncthreads();
+            const int jn = jb + kWave + lane;
+            if (jn <= jmax) {
+                pa = al[jn];
+                pc = cp[jn];
+            }
+            degrees(sal, scp, jb, min(jmax, jb + kWave - 1));
+        }
+    } else {
+        degrees(al, cp, 0, jmax);
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -179,7 +217,7 @@ __device__ __forceinline__ void synth_leg_tile(const double *__restrict__ al, do
     }
 }
 
-template <int K, bool kRho = false>
+template <int K, bool kRho = false, bool kStage = false>
 __device__ __forceinline__ void synth_leg_dispatch(int kk, const double *__restrict__ al, double sd,
                                                    const double *__restrict__ ringz,
                                                    const double2 *__restrict__ cp, int jmax, int P, int slot0,
@@ -187,10 +225,10 @@ __device__ __forceinline__ void synth_leg_dispatch(int kk, const double *__restr
                                                    const double *__restrict__ ringr2 = nullptr)
 {
     static_assert(K == 4, "synthesis tiles are instantiated for 1 .. 4 ring pairs per lane");
-    if (kk <= 1) synth_leg_tile<1, kRho>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow);
-    else if (kk == 2) synth_leg_tile<2, kRho>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow);
-    else if (kk == 3) synth_leg_tile<3, kRho>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow);
-    else synth_leg_tile<4, kRho>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow);
+    if (kk <= 1) synth_leg_tile<1, kRho, kStage>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow);
+    else if (kk == 2) synth_leg_tile<2, kRho, kStage>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow);
+    else if (kk == 3) synth_leg_tile<3, kRho, kStage>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow);
+    else synth_leg_tile<4, kRho, kStage>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow);
 }
 
 // ------------------------------------------------- cross-lane reductions
@@ -267,14 +305,35 @@ __device__ __forceinline__ double wave_transpose_sum16(double (&v)[16], int lane
 // the order's rescale factor.  Passes of K * 64 slots; the first pass
 // stores, later passes accumulate.  Each chunk of kAnalLC degrees is reduced
 // over the wave in registers (wave_transpose_sum16).
-template <int K>
+template <int K, bool kStage>
 __device__ __forceinline__ void anal_leg_tile(const double *__restrict__ al, double sd,
                                               const double *__restrict__ ringz, const double4 *__restrict__ grow,
                                               int jmax, int s0, int P, double2 *__restrict__ out)
 {
     constexpr int LC = kAnalLC;
+    static_assert(kWave % LC == 0, "a chunk of degrees lies in one staged block");
     const int lane = threadIdx.x & (kWave - 1);
+    // kStage: the recursion row in blocks of 64 degrees through LDS, as in
+    // synth_leg_tile (one-wave workgroups)
+    __shared__ double sal[kWave];
+    double pre = 0.0;
+    // the recursion coefficients of the chunk of degrees starting at j0
+    auto chunk_row = [&](int j0) -> const double * {
+        if constexpr (kStage) {
+            if ((j0 & (kWave - 1)) == 0) {
+                __syncthreads();  // the previous block's reads are done
+                sal[lane] = pre;
+                __syncthreads();
+                const int jn = j0 + kWave + lane;
+                pre = (jn <= jmax) ? al[jn] : 0.0;
+            }
+            return sal + (j0 & (kWave - 1));
+        } else {
+            return al + j0;
+        }
+    };
     for (int p0 = s0; p0 < P; p0 += K * kWave) {
+        if constexpr (kStage) pre = (lane <= jmax) ? al[lane] : 0.0;
         double z[K], q1[K], q2[K], eC[K], eS[K], oC[K], oS[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -314,10 +373,11 @@ __device__ __forceinline__ void anal_leg_tile(const double *__restrict__ al, dou
         };
         int j0 = 0;
         for (; j0 + LC - 1 <= jmax; j0 += LC) {
+            const double *ab = chunk_row(j0);
             double acc[LC][2];
 #pragma unroll
             for (int jj = 0; jj < LC; ++jj) {
-                const double a = al[j0 + jj];
+                const double a = ab[jj];
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     // explicit fused multiply-adds, as in the synthesis tile
@@ -338,13 +398,14 @@ __device__ __forceinline__ void anal_leg_tile(const double *__restrict__ al, dou
             reduce_chunk(acc, j0);
         }
         if (j0 <= jmax) {
+            const double *ab = chunk_row(j0);
             double acc[LC][2];
 #pragma unroll
             for (int jj = 0; jj < LC; ++jj) acc[jj][0] = acc[jj][1] = 0.0;
 #pragma unroll
             for (int jj = 0; jj < LC; ++jj) {
                 if (j0 + jj <= jmax) {
-                    const double a = al[j0 + jj];
+                    const double a = ab[jj];
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
                         const double q = fma(a * z[k], q1[k], -q2[k]);
@@ -722,6 +783,9 @@ __device__ __forceinline__ bool order_block(int b, int nb, int Lc, int &m, int &
     return m <= Lc;
 }
 
+// kStage: the rows through LDS (synth_leg_tile), for launches that bring
+// few ring pairs per lane.
+template <bool kStage>
 __global__ __launch_bounds__(kWave) void k_synth_leg(const double *__restrict__ alpha, int Lt,
                                                      const double *__restrict__ seed,
                                                      const double *__restrict__ ringz,
@@ -735,12 +799,14 @@ __global__ __launch_bounds__(kWave) void k_synth_leg(const double *__restrict__ 
     const int slot0 = (smin ? smin[m] : 0) + sb * kb * kWave;
     if (slot0 >= P) return;
     const int kk = min(kb, (P - slot0 + kWave - 1) / kWave);
+    const double *al = alpha + tri_offd(m, Lt);
+    const double2 *cp = cpack + tri_offd(m, Lc);
     if (ringr2)
-        synth_leg_dispatch<kSynthKMax, true>(kk, alpha + tri_offd(m, Lt), seed[m], ringz, cpack + tri_offd(m, Lc),
-                                             Lc - m, P, slot0, fbuf + (size_t)m * P, ringr2);
+        synth_leg_dispatch<kSynthKMax, true, kStage>(kk, al, seed[m], ringz, cp, Lc - m, P, slot0,
+                                                     fbuf + (size_t)m * P, ringr2);
     else
-        synth_leg_dispatch<kSynthKMax>(kk, alpha + tri_offd(m, Lt), seed[m], ringz, cpack + tri_offd(m, Lc), Lc - m,
-                                       P, slot0, fbuf + (size_t)m * P);
+        synth_leg_dispatch<kSynthKMax, false, kStage>(kk, al, seed[m], ringz, cp, Lc - m, P, slot0,
+                                                      fbuf + (size_t)m * P);
 }
 
 // rows[s] = (north row, south row or -1); rows >= nrows are skipped;
@@ -835,7 +901,7 @@ __global__ __launch_bounds__(kWave) void k_anal_leg(const double *__restrict__ a
         for (int j = threadIdx.x; j <= Lc - m; j += kWave) out[j] = make_double2(0.0, 0.0);
         return;
     }
-    anal_leg_tile<K>(alpha + tri_offd(m, Lt), seed[m], ringz, gbuf + (size_t)m * P, Lc - m, sa, se, out);
+    anal_leg_tile<K, true>(alpha + tri_offd(m, Lt), seed[m], ringz, gbuf + (size_t)m * P, Lc - m, sa, se, out);
 }
 
 // grid: ntm x ntl tiles over the whole d1 x d2 array
@@ -1803,7 +1869,7 @@ __global__ __launch_bounds__(kWave) void kb_anal_leg(const BField *__restrict__ 
     const BField f = fields[it.x];
     const int m = it.y;
     const int s0 = min(T.s0, f.s0);
-    anal_leg_tile<K>(T.alpha + tri_offd(m, T.Lt), T.seed[m], T.z + f.geo, gbuf + f.foff + (size_t)m * f.P,
+    anal_leg_tile<K, false>(T.alpha + tri_offd(m, T.Lt), T.seed[m], T.z + f.geo, gbuf + f.foff + (size_t)m * f.P,
                      f.L - m, T.smin ? max(s0, T.smin[f.sm + m]) : s0, f.P, cbuf + f.coff + tri_offd(m, f.L));
 }
 

@@ -491,6 +491,14 @@ static int anal_blocks(int Lc, int passes)
 // (order_block's mapping)
 static unsigned leg_grid(int Lc, int nb) { return 8u * (unsigned)nb * (unsigned)((Lc + 8) / 8); }
 
+// The plain synthesis kernel for kb ring pairs per lane: fewer than the most
+// mean a launch of few waves, bounded by its longest waves' reads of their
+// rows, which then come through LDS (DH2 lmax 511 synthesis Legendre 44 ->
+// 27 us); with the most, the launch is bound by arithmetic and the LDS path
+// costs registers and occupancy (DH1 lmax 2047 513 -> 564 us).
+typedef decltype(&k_synth_leg<false>) SynthLegKernel;
+static SynthLegKernel synth_leg_kernel(int kb) { return kb < kSynthKMax ? k_synth_leg<true> : k_synth_leg<false>; }
+
 // the analysis kernels by ring pairs per lane
 typedef decltype(&k_anal_leg<1>) AnalLegKernel;
 typedef decltype(&kb_anal_leg<1>) BatchAnalLegKernel;
@@ -620,8 +628,8 @@ static int launch_synthesis(const Geometry &g, const DevRecurrence &rt, const De
         ProfScope ps(SHTG_K_SYNTH_LEG, st);
         const int kb = synth_lanes(Lc, g.P);
         const int nsb = (g.P + kb * kWave - 1) / (kb * kWave);
-        hipLaunchKernelGGL(k_synth_leg, dim3(leg_grid(Lc, nsb)), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, rz, cpack,
-                           Lc, g.P, nsb, fbuf, ring.r2, smin, kb);
+        hipLaunchKernelGGL(synth_leg_kernel(kb), dim3(leg_grid(Lc, nsb)), dim3(kWave), 0, st, rt.alpha, rt.L,
+                           sd.seed, rz, cpack, Lc, g.P, nsb, fbuf, ring.r2, smin, kb);
     }
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
@@ -726,10 +734,10 @@ static int launch_synthesis_c(const Geometry &g, const DevRecurrence &rt, const 
         ProfScope ps(SHTG_K_SYNTH_LEG, st);
         const int kb = synth_lanes(Lc, g.P);
         const int nsb = (g.P + kb * kWave - 1) / (kb * kWave);
-        hipLaunchKernelGGL(k_synth_leg, dim3(leg_grid(Lc, nsb)), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, cp_p,
-                           Lc, g.P, nsb, fb_p, (const double *)nullptr, (const int *)nullptr, kb);
-        hipLaunchKernelGGL(k_synth_leg, dim3(leg_grid(Lc, nsb)), dim3(kWave), 0, st, rt.alpha, rt.L, sd.seed, g.z, cp_m,
-                           Lc, g.P, nsb, fb_m, (const double *)nullptr, (const int *)nullptr, kb);
+        hipLaunchKernelGGL(synth_leg_kernel(kb), dim3(leg_grid(Lc, nsb)), dim3(kWave), 0, st, rt.alpha, rt.L,
+                           sd.seed, g.z, cp_p, Lc, g.P, nsb, fb_p, (const double *)nullptr, (const int *)nullptr, kb);
+        hipLaunchKernelGGL(synth_leg_kernel(kb), dim3(leg_grid(Lc, nsb)), dim3(kWave), 0, st, rt.alpha, rt.L,
+                           sd.seed, g.z, cp_m, Lc, g.P, nsb, fb_m, (const double *)nullptr, (const int *)nullptr, kb);
     }
     {
         ProfScope ps(SHTG_K_SYNTH_FFT, st);

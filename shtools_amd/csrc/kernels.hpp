@@ -130,14 +130,15 @@ __device__ __forceinline__ double ring_rescale(double u, int k) { return u < 0.0
 // loads complete in order, so waiting for one block does not wait for the
 // next; scalar loads are waited for all together, which leaves their
 // latency on every few degrees of a wave whose rows are not cached), and
-// the degree loop reads the block back as wave-wide broadcasts.  For the
-// launches of one field, whose longest waves bound them; workgroups are one
-// wave.
+// the degree loop reads the block back as wave-wide broadcasts (sal, scp:
+// the kernel's LDS blocks of 64 entries).  For the launches of one field,
+// whose longest waves bound them; workgroups are one wave.
 template <int K, bool kRho, bool kStage>
 __device__ __forceinline__ void synth_leg_tile(const double *__restrict__ al, double sd,
                                                const double *__restrict__ ringz, const double *__restrict__ ringr2,
                                                const double2 *__restrict__ cp, int jmax, int P, int slot0,
-                                               double4 *__restrict__ frow)
+                                               double4 *__restrict__ frow, double *sal = nullptr,
+                                               double2 *scp = nullptr)
 {
     const int lane = threadIdx.x & (kWave - 1);
     double z[K], q1[K], q2[K], eC[K], eS[K], oC[K], oS[K], r2[K];
@@ -187,8 +188,6 @@ __device__ __forceinline__ void synth_leg_tile(const double *__restrict__ al, do
         }
     };
     if constexpr (kStage) {
-        __shared__ double sal[kWave];
-        __shared__ double2 scp[kWave];
         double pa = 0.0;
         double2 pc = make_double2(0.0, 0.0);
         if (lane <= jmax) {
@@ -222,13 +221,14 @@ __device__ __forceinline__ void synth_leg_dispatch(int kk, const double *__restr
                                                    const double *__restrict__ ringz,
                                                    const double2 *__restrict__ cp, int jmax, int P, int slot0,
                                                    double4 *__restrict__ frow,
-                                                   const double *__restrict__ ringr2 = nullptr)
+                                                   const double *__restrict__ ringr2 = nullptr,
+                                                   double *sal = nullptr, double2 *scp = nullptr)
 {
     static_assert(K == 4, "synthesis tiles are instantiated for 1 .. 4 ring pairs per lane");
-    if (kk <= 1) synth_leg_tile<1, kRho, kStage>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow);
-    else if (kk == 2) synth_leg_tile<2, kRho, kStage>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow);
-    else if (kk == 3) synth_leg_tile<3, kRho, kStage>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow);
-    else synth_leg_tile<4, kRho, kStage>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow);
+    if (kk <= 1) synth_leg_tile<1, kRho, kStage>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow, sal, scp);
+    else if (kk == 2) synth_leg_tile<2, kRho, kStage>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow, sal, scp);
+    else if (kk == 3) synth_leg_tile<3, kRho, kStage>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow, sal, scp);
+    else synth_leg_tile<4, kRho, kStage>(al, sd, ringz, ringr2, cp, jmax, P, slot0, frow, sal, scp);
 }
 
 // ------------------------------------------------- cross-lane reductions
@@ -801,12 +801,15 @@ __global__ __launch_bounds__(kWave) void k_synth_leg(const double *__restrict__ 
     const int kk = min(kb, (P - slot0 + kWave - 1) / kWave);
     const double *al = alpha + tri_offd(m, Lt);
     const double2 *cp = cpack + tri_offd(m, Lc);
+    // one pair of staging blocks for every tile instantiation of the kernel
+    __shared__ double sal[kStage ? kWave : 1];
+    __shared__ double2 scp[kStage ? kWave : 1];
     if (ringr2)
         synth_leg_dispatch<kSynthKMax, true, kStage>(kk, al, seed[m], ringz, cp, Lc - m, P, slot0,
-                                                     fbuf + (size_t)m * P, ringr2);
+                                                     fbuf + (size_t)m * P, ringr2, sal, scp);
     else
         synth_leg_dispatch<kSynthKMax, false, kStage>(kk, al, seed[m], ringz, cp, Lc - m, P, slot0,
-                                                      fbuf + (size_t)m * P);
+                                                      fbuf + (size_t)m * P, nullptr, sal, scp);
 }
 
 // rows[s] = (north row, south row or -1); rows >= nrows are skipped;

@@ -459,12 +459,16 @@ constexpr int kPlainAnalKMin = 4;
 
 // Ring pairs per lane of a plain synthesis: kSynthKMax, or fewer when the
 // orders times the blocks of ring pairs would give the launch fewer than
-// kLegWaves waves.
+// kLegWaves waves; but 2 rather than 1 while 2 still gives kLegWaves / 4
+// waves, since the staged tile (kb < kSynthKMax) runs 2 ring pairs per lane
+// faster than twice the waves of 1 (GLQ lmax 1023: 54 vs 74 us, DH2 lmax
+// 511: 22.5 vs 25.4 us).
 static int synth_lanes(int Lc, int P)
 {
+    auto waves = [&](int kb) { return (long long)(Lc + 1) * ((P + kb * kWave - 1) / (kb * kWave)); };
     for (int kb = kSynthKMax; kb > kPlainSynthKMin; --kb)
-        if ((long long)(Lc + 1) * ((P + kb * kWave - 1) / (kb * kWave)) >= kLegWaves) return kb;
-    return kPlainSynthKMin;
+        if (waves(kb) >= kLegWaves) return kb;
+    return waves(2) >= kLegWaves / 4 ? 2 : kPlainSynthKMin;
 }
 
 // Largest ring pairs per lane of a plain analysis over nslots slots: fewer

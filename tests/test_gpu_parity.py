@@ -143,6 +143,30 @@ def test_dh_medium_vs_oracle(B, L, sampling):
     assert coef_err(B.SHExpandDH(g_ref, sampling=sampling), c_ref) <= TOL
 
 
+@pytest.mark.parametrize("L", [63, 64, 65, 128, 129, 511])
+@pytest.mark.parametrize("kind", ["dh2", "glq"])
+def test_block_edge_degrees_vs_oracle(B, L, kind):
+    """Degrees at the edges of the 64-degree blocks in which the plain
+    Legendre kernels read their recursion and coefficient rows (an order's
+    degree count L - m + 1 runs through every block edge), with one and two
+    ring pairs per lane (lmax 511 DH2 synthesis: 2)."""
+    print(f"case block edges {kind} L={L}", flush=True)
+    rng = np.random.default_rng(600 + L)
+    c = rand_cilm(rng, L)
+    for norm in (1, 4):
+        if kind == "dh2":
+            g_ref = oracle.MakeGridDH(c, sampling=2, norm=norm)
+            assert grid_err(B.MakeGridDH(c, sampling=2, norm=norm), g_ref) <= TOL
+            c_ref = oracle.SHExpandDH(g_ref, sampling=2, norm=norm)
+            assert coef_err(B.SHExpandDH(g_ref, sampling=2, norm=norm), c_ref) <= TOL
+        else:
+            z, w = B.SHGLQ(L)
+            g_ref = oracle.MakeGridGLQ(c, z, norm=norm)
+            assert grid_err(B.MakeGridGLQ(c, z, norm=norm), g_ref) <= TOL
+            c_ref = oracle.SHExpandGLQ(g_ref, w, z, norm=norm)
+            assert coef_err(B.SHExpandGLQ(g_ref, w, z, norm=norm), c_ref) <= TOL
+
+
 def test_dh_l1023_vs_oracle(B):
     """DH2 at the top degree of the batched configuration (lmax = 1023)."""
     print("case DH L=1023", flush=True)

@@ -72,8 +72,9 @@ constexpr int kPrimeRadices[] = {SHTG_PRIME_RADICES(SHTG_LIST_ITEM)};
 constexpr int kPrimeMaxLen = 160 * 1024 / 16;
 
 // Kernel instantiations by plan kind: plain plans, split Bluestein plans
-// (FftPlanHost::half) and direct plans ending with a prime radix.
-enum { kVarPlain = 0, kVarSplit = 1, kVarPrime = 2 };
+// (FftPlanHost::half), direct plans ending with a prime radix, and
+// Bluestein plans of a power-of-two length (their turn runs radix 8, 4 or 2).
+enum { kVarPlain = 0, kVarSplit = 1, kVarPrime = 2, kVarPow2 = 3 };
 
 inline bool is_prime_radix(int p)
 {
@@ -160,18 +161,23 @@ inline bool is_5smooth(int m)
 
 // Bluestein work length: the 5-smooth m >= lo (up to 1.35 lo) with the
 // fewest LDS passes, m (2 stages - 1) (the last DIF and first DIT stage run
-// together in registers), plus one pass when the last radix is even (its
-// stride-1 accesses collide on the LDS banks).  Powers of two are skipped,
-// so that the last radix is odd (3 or 5), the two radices the device runs
-// the Bluestein turn for.
-inline int bluestein_length(int lo)
+// together in registers).  The last radix is odd (3 or 5: m has an odd
+// factor, which factor_stages orders last), except for powers of two when
+// allow_pow2 (plans that are not split and whose array fits the LDS: they
+// run in kernel instantiations of their own, kVarPow2, whose turn is radix
+// 8, 4 or 2; GLQ lmax 1023 rings, n = 2047: m = 4096 in 4 stages instead of
+// 4608 in 5).  Other even last radices are skipped (their stride-1 accesses
+// collide on the LDS banks).
+constexpr int kPow2MaxLen = 160 * 1024 / 16;
+inline int bluestein_length(int lo, bool allow_pow2 = false)
 {
     int best_m = 0;
     double best_cost = 0.0;
     std::vector<int> rad;
     for (int m = lo; m <= lo + lo / 3 + 8; ++m) {
         if (!is_5smooth(m) || !factor_stages(m, false, rad)) continue;
-        if ((m & (m - 1)) == 0 || rad.back() % 2 == 0) continue;
+        const bool pow2 = (m & (m - 1)) == 0;
+        if (pow2 ? !(allow_pow2 && m <= kPow2MaxLen) : rad.back() % 2 == 0) continue;
         const double cost = (double)m * (2.0 * rad.size() - 1.0);
         if (best_m == 0 || cost < best_cost) {
             best_m = m;
@@ -325,7 +331,7 @@ inline FftPlanHost make_fft_plan(int n, int lc = -1, bool split = false)
     const bool prune = lc >= 0 && (n % 2 == 0) && n + 2 * lc < 2 * n - 1;
     p.lc = prune ? lc : -1;
     p.half = (split && prune) ? 1 : 0;
-    p.m = bluestein_length(p.half ? n / 2 + 2 * lc : prune ? n + 2 * lc : 2 * n - 1);
+    p.m = bluestein_length(p.half ? n / 2 + 2 * lc : prune ? n + 2 * lc : 2 * n - 1, !p.half);
     factor_stages(p.m, false, rad);
     set_stages(p, rad);
     const long double pi = 3.14159265358979323846264338327950288L;
@@ -356,6 +362,7 @@ template <class P>
 inline int plan_variant(const P &F)
 {
     if (F.half) return kVarSplit;
+    if (F.blue && (F.m & (F.m - 1)) == 0) return kVarPow2;
     return (F.nst > 0 && F.radix[F.nst - 1] > 8) ? kVarPrime : kVarPlain;
 }
 

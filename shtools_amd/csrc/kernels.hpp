@@ -554,7 +554,7 @@ __device__ __forceinline__ void synth_fft_run(const FftDev &F, int2 rw, int nrow
     if (chunk && F.nst >= 2 && F.m / F.radix[0] <= chunk_cap)
         fft_forward_chunked(lds, F, F.bhat, chunk, chunk_cap);
     else
-        fft_forward<kVar == kVarPrime>(lds, F, F.bhat);
+        fft_forward<kVar == kVarPrime, kVar == kVarPow2>(lds, F, F.bhat);
     const bool north = rw.x >= 0 && rw.x < nrows;
     const bool south = rw.y >= 0 && rw.y < nrows;
     // columns j and n - j (1 <= j < n - j) together: one chirp read
@@ -693,7 +693,7 @@ __device__ __forceinline__ void anal_fft_run(const FftDev &F, int2 rw, int nrows
     if (chunk && F.nst >= 2 && F.m / F.radix[0] <= chunk_cap)
         fft_forward_chunked(lds, F, F.bhat2, chunk, chunk_cap);
     else
-        fft_forward<kVar == kVarPrime>(lds, F, F.bhat2);
+        fft_forward<kVar == kVarPrime, kVar == kVarPow2>(lds, F, F.bhat2);
     anal_fft_extract(F, rw, nrows, w, u, P, Lc, s, gbuf, lds);
 }
 
@@ -1299,7 +1299,7 @@ __device__ __forceinline__ void synth_fft_c_tile(const FftDev &F, const double4 
         if (chunked)
             fft_forward_chunked(x, F, F.bhat, chunk, chunk_cap);
         else
-            fft_forward<kVar == kVarPrime>(x, F, F.bhat);
+            fft_forward<kVar == kVarPrime, kVar == kVarPow2>(x, F, F.bhat);
         for (int j = threadIdx.x; j < n; j += blockDim.x) {
             const double2 v = conjd(fft_get(x, F, j, j));
             grid[row * ld + j] = v;
@@ -1351,7 +1351,7 @@ __device__ __forceinline__ void anal_fft_c_tile(const FftDev &F, const double2 *
             if (chunked)
                 fft_forward_chunked(x, F, F.bhat2, chunk, chunk_cap);
             else
-                fft_forward<kVar == kVarPrime>(x, F, F.bhat2);
+                fft_forward<kVar == kVarPrime, kVar == kVarPow2>(x, F, F.bhat2);
         }
         for (int k = threadIdx.x; k <= Lc; k += blockDim.x) {
             const double r = ring_rescale(u, k) * wr;

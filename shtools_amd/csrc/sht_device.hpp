@@ -397,15 +397,24 @@ __device__ __forceinline__ void fft_turn(double2 *x, int mloc, const double2 *__
     }
 }
 
-// Bluestein work lengths are 5-smooth and not powers of two
-// (bluestein_length), and factor_stages orders odd radices last, so the
-// last radix of a Bluestein plan is 3 or 5.
+// The last radix of a Bluestein plan is 3 or 5, or, for the power-of-two
+// lengths of kVarPow2 plans, 8, 4 or 2 (bluestein_length).
+template <bool kPow2 = false>
 __device__ __forceinline__ void fft_run_turn(double2 *x, const FftDev &P, const double2 *__restrict__ bhat, int mloc)
 {
-    if (P.radix[P.nst - 1] == 3)
-        fft_turn<3>(x, mloc, bhat);
-    else
-        fft_turn<5>(x, mloc, bhat);
+    if constexpr (kPow2) {
+        if (P.radix[P.nst - 1] == 8)
+            fft_turn<8>(x, mloc, bhat);
+        else if (P.radix[P.nst - 1] == 4)
+            fft_turn<4>(x, mloc, bhat);
+        else
+            fft_turn<2>(x, mloc, bhat);
+    } else {
+        if (P.radix[P.nst - 1] == 3)
+            fft_turn<3>(x, mloc, bhat);
+        else
+            fft_turn<5>(x, mloc, bhat);
+    }
 }
 
 // In-place forward DFT of the values stored in x (x has P.m slots; for
@@ -415,7 +424,7 @@ __device__ __forceinline__ void fft_run_turn(double2 *x, const FftDev &P, const 
 // One call site per stage direction (the last DIF stage of a plain plan runs
 // in the same loop as the others), which keeps the kernels' code and
 // register use down.
-template <bool kPrimes = false>
+template <bool kPrimes = false, bool kPow2 = false>
 __device__ __forceinline__ void fft_forward(double2 *x, const FftDev &P, const double2 *__restrict__ bhat)
 {
     const int last = P.nst - 1;
@@ -425,7 +434,7 @@ __device__ __forceinline__ void fft_forward(double2 *x, const FftDev &P, const d
         __syncthreads();
     }
     if (!P.blue) return;
-    fft_run_turn(x, P, bhat, P.m);
+    fft_run_turn<kPow2>(x, P, bhat, P.m);
     __syncthreads();
     for (int st = last - 1; st >= 0; --st) {
         fft_run_stage<true>(x, P, st, P.m);

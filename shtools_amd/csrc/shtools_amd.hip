@@ -44,13 +44,13 @@ static std::mutex g_prof_mu;
 #define HIP_OK(x) (hipSuccess == (x))
 
 // The ring-FFT kernel instantiation for a plan variant (kVarPlain,
-// kVarSplit, kVarPrime).
+// kVarSplit, kVarPrime, kVarPow2).
 template <class K>
-static K by_variant(int v, K plain, K split, K prime)
+static K by_variant(int v, K plain, K split, K prime, K pow2)
 {
-    return v == kVarSplit ? split : v == kVarPrime ? prime : plain;
+    return v == kVarSplit ? split : v == kVarPrime ? prime : v == kVarPow2 ? pow2 : plain;
 }
-#define SHTG_FFT_KERNEL(k, v) by_variant((v), k<kVarPlain>, k<kVarSplit>, k<kVarPrime>)
+#define SHTG_FFT_KERNEL(k, v) by_variant((v), k<kVarPlain>, k<kVarSplit>, k<kVarPrime>, k<kVarPow2>)
 
 
 template <class T>
@@ -114,7 +114,9 @@ static void configure_kernels()
           (const void *)k_synth_fft_c<false, kVarSplit>, (const void *)k_synth_fft_c<false, kVarPrime>,
           (const void *)k_synth_fft_c<true, kVarPlain>, (const void *)k_anal_fft_c<false, kVarPlain>,
           (const void *)k_anal_fft_c<false, kVarSplit>, (const void *)k_anal_fft_c<false, kVarPrime>,
-          (const void *)k_anal_fft_c<true, kVarPlain>})
+          (const void *)k_anal_fft_c<true, kVarPlain>, (const void *)k_synth_fft<kVarPow2>,
+          (const void *)k_anal_fft<kVarPow2>, (const void *)kb_synth_fft<kVarPow2>, (const void *)kb_anal_fft<kVarPow2>,
+          (const void *)k_synth_fft_c<false, kVarPow2>, (const void *)k_anal_fft_c<false, kVarPow2>})
         (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
     kernels_configured = true;
 }
@@ -747,7 +749,7 @@ static int launch_synthesis_c(const Geometry &g, const DevRecurrence &rt, const 
         ProfScope ps(SHTG_K_SYNTH_FFT, st);
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         if (lds_fft)
-            hipLaunchKernelGGL((by_variant(plan_variant(g.fft), k_synth_fft_c<false, kVarPlain>, k_synth_fft_c<false, kVarSplit>, k_synth_fft_c<false, kVarPrime>)), dim3(g.P),
+            hipLaunchKernelGGL((by_variant(plan_variant(g.fft), k_synth_fft_c<false, kVarPlain>, k_synth_fft_c<false, kVarSplit>, k_synth_fft_c<false, kVarPrime>, k_synth_fft_c<false, kVarPow2>)), dim3(g.P),
                                dim3(plan_threads(g.fft)), lds, st, g.fft, fb_p, fb_m,
                                g.P, Lc, g.u, g.rows, nrows_out, grid, ld, ext_col, (double2 *)nullptr, 0LL, 0);
         else
@@ -781,7 +783,7 @@ static int launch_analysis_c(const Geometry &g, const DevRecurrence &rt, const D
         ProfScope ps(SHTG_K_ANAL_FFT, st);
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         if (lds_fft)
-            hipLaunchKernelGGL((by_variant(plan_variant(g.fft), k_anal_fft_c<false, kVarPlain>, k_anal_fft_c<false, kVarSplit>, k_anal_fft_c<false, kVarPrime>)), dim3(g.P),
+            hipLaunchKernelGGL((by_variant(plan_variant(g.fft), k_anal_fft_c<false, kVarPlain>, k_anal_fft_c<false, kVarSplit>, k_anal_fft_c<false, kVarPrime>, k_anal_fft_c<false, kVarPow2>)), dim3(g.P),
                                dim3(plan_threads(g.fft)), lds, st, g.fft, grid, ld,
                                g.u, g.rows, g.nlat, g.w, g.P, Lc, gb_p, gb_m, (double2 *)nullptr, 0LL, 0);
         else
@@ -1991,7 +1993,7 @@ static int launch_plx_synthesis_c(const Geometry &g, const double2 *cilm, int d1
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         if (lds_fft)
             hipLaunchKernelGGL((by_variant(plan_variant(g.fft), k_synth_fft_c<false, kVarPlain>,
-                                           k_synth_fft_c<false, kVarSplit>, k_synth_fft_c<false, kVarPrime>)),
+                                           k_synth_fft_c<false, kVarSplit>, k_synth_fft_c<false, kVarPrime>, k_synth_fft_c<false, kVarPow2>)),
                                dim3(g.P), dim3(plan_threads(g.fft)), lds, st, g.fft, fb_p, fb_m, g.P, Lc,
                                (const double *)nullptr, g.rows, g.nlat, grid, ld, ext_col, (double2 *)nullptr, 0LL, 0);
         else
@@ -2021,7 +2023,7 @@ static int launch_plx_analysis_c(const Geometry &g, const double2 *grid, long lo
         const size_t lds = (size_t)fft_lds_slots(g.fft.m) * sizeof(double2);
         if (lds_fft)
             hipLaunchKernelGGL((by_variant(plan_variant(g.fft), k_anal_fft_c<false, kVarPlain>,
-                                           k_anal_fft_c<false, kVarSplit>, k_anal_fft_c<false, kVarPrime>)),
+                                           k_anal_fft_c<false, kVarSplit>, k_anal_fft_c<false, kVarPrime>, k_anal_fft_c<false, kVarPow2>)),
                                dim3(g.P), dim3(plan_threads(g.fft)), lds, st, g.fft, grid, ld, (const double *)nullptr,
                                g.rows, g.nlat, g.w, g.P, Lc, gb_p, gb_m, (double2 *)nullptr, 0LL, 0);
         else

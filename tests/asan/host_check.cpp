@@ -176,7 +176,9 @@ static void check_plan(const FftPlanHost &p, int n, int lc, bool split)
     const int h = n / 2;
     const long long need = p.half ? h + 2LL * lc : p.lc >= 0 ? n + 2LL * lc : 2LL * n - 1;
     CHECK(p.m >= need && p.m <= need + need / 3 + 8, "Bluestein length n %d lc %d m %d", n, lc, p.m);
-    CHECK(p.radix[p.nst - 1] % 2 == 1, "odd last radix n %d", n);
+    CHECK(p.radix[p.nst - 1] % 2 == 1 || (!p.half && (p.m & (p.m - 1)) == 0 && p.m <= kPow2MaxLen &&
+                                         plan_variant(p) == kVarPow2),
+          "odd last radix, or a power-of-two length in its own instantiation n %d", n);
     CHECK(p.chirp.size() == 2 * (size_t)n && p.bhat.size() == 2 * (size_t)p.m, "chirp arrays n %d", n);
     CHECK(p.lc < 0 || p.bhat2.size() == 2 * (size_t)p.m, "analysis spectrum n %d", n);
     CHECK(!p.half || (split && p.bhat3.size() == 2 * (size_t)p.m && p.bhat4.size() == 2 * (size_t)p.m),

@@ -177,8 +177,9 @@ static double run_case(const FftPlanHost &h, int mode, std::mt19937_64 &rng, int
     // the chunked variant (work array outside the LDS, inner stages on
     // chunks of one and of three groups) must give the same bits
     std::vector<std::vector<double2>> xc;
-    // (plans ending with a prime radix run in the LDS only, never chunked)
-    if (h.nst >= 2 && plan_variant(h) != kVarPrime) {
+    // (plans ending with a prime radix and power-of-two Bluestein plans run
+    // in the LDS only, never chunked)
+    if (h.nst >= 2 && plan_variant(h) != kVarPrime && plan_variant(h) != kVarPow2) {
         const int gsz = h.m / h.radix[0];
         for (int cap : {gsz, 3 * gsz}) {
             std::vector<double2> y = x, lds((size_t)cap);
@@ -186,7 +187,10 @@ static double run_case(const FftPlanHost &h, int mode, std::mt19937_64 &rng, int
             xc.push_back(y);
         }
     }
-    fft_forward<true>(x.data(), F, mode == 2 ? F.bhat2 : F.bhat);
+    if (plan_variant(h) == kVarPow2)
+        fft_forward<true, true>(x.data(), F, mode == 2 ? F.bhat2 : F.bhat);
+    else
+        fft_forward<true>(x.data(), F, mode == 2 ? F.bhat2 : F.bhat);
     for (const auto &y : xc)
         for (int e = 0; e < h.m; ++e)
             if (y[e].x != x[e].x || y[e].y != x[e].y) return 2.0;

@@ -163,11 +163,11 @@ inline bool is_5smooth(int m)
 // fewest LDS passes, m (2 stages - 1) (the last DIF and first DIT stage run
 // together in registers).  The last radix is odd (3 or 5: m has an odd
 // factor, which factor_stages orders last), except for powers of two when
-// allow_pow2 (plans that are not split and whose array fits the LDS: they
-// run in kernel instantiations of their own, kVarPow2, whose turn is radix
-// 8, 4 or 2; GLQ lmax 1023 rings, n = 2047: m = 4096 in 4 stages instead of
-// 4608 in 5).  Other even last radices are skipped (their stride-1 accesses
-// collide on the LDS banks).
+// allow_pow2 (unpruned plans whose array fits the LDS: they run in kernel
+// instantiations of their own, kVarPow2, whose turn is radix 8, 4 or 2; GLQ
+// lmax 1023 rings, n = 2047: m = 4096 in 4 stages instead of 4608 in 5).
+// Other even last radices are skipped (their stride-1 accesses collide on
+// the LDS banks).
 constexpr int kPow2MaxLen = 160 * 1024 / 16;
 inline int bluestein_length(int lo, bool allow_pow2 = false)
 {
@@ -331,7 +331,9 @@ inline FftPlanHost make_fft_plan(int n, int lc = -1, bool split = false)
     const bool prune = lc >= 0 && (n % 2 == 0) && n + 2 * lc < 2 * n - 1;
     p.lc = prune ? lc : -1;
     p.half = (split && prune) ? 1 : 0;
-    p.m = bluestein_length(p.half ? n / 2 + 2 * lc : prune ? n + 2 * lc : 2 * n - 1, !p.half);
+    // powers of two for unpruned plans only (the GLQ rings): pruned DH plans
+    // measured slower with them, split ones more so
+    p.m = bluestein_length(p.half ? n / 2 + 2 * lc : prune ? n + 2 * lc : 2 * n - 1, !p.half && !prune);
     factor_stages(p.m, false, rad);
     set_stages(p, rad);
     const long double pi = 3.14159265358979323846264338327950288L;

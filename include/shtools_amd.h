@@ -278,7 +278,7 @@ int shtg_batch_set_mfma(shtg_batch_plan *plan, int min_bucket, void *stream);
  * kernel (off, 2, 4, 8, 16, where the plan has such buckets), on the arrays
  * bound by the plan's last makegrid and shexpand calls, keeping the fastest
  * of each; none of these changes any result.  flags bit 0 also tunes ring
- * pairs per lane of the analysis waves (4, 6, 8), which regroups the
+ * pairs per lane of the analysis waves (4, 6, 8, 12), which regroups the
  * analysis sums (coefficients then change by rounding).  Synchronises
  * `stream`; overwrites those grids and the plan's workspace, so rerun the
  * transforms afterwards.  Returns 5 if the plan has not been called in both
@@ -286,7 +286,7 @@ int shtg_batch_set_mfma(shtg_batch_plan *plan, int min_bucket, void *stream);
 int shtg_batch_tune(shtg_batch_plan *plan, int reps, int flags, void *stream);
 /* The Legendre launch choices of a plan (from shtg_batch_tune, or set here
  * to reuse an earlier tuning): ring pairs per lane of the synthesis waves
- * (1 .. 4) and of the analysis waves (1 .. 8), and the MFMA bucket size
+ * (1 .. 4) and of the analysis waves (1 .. 12), and the MFMA bucket size
  * (0: off, else >= 2; see shtg_batch_set_mfma).  Results do not depend on
  * them beyond rounding (analysis sums are associated by pass). */
 int shtg_batch_get_tuning(shtg_batch_plan *plan, int *synth_kmax, int *anal_kmax, int *mfma_min);

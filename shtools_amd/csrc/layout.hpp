@@ -29,6 +29,10 @@ constexpr int kFftMaxThreads = 1024;
 constexpr int kWave = 64;      // Legendre workgroups are single waves
 constexpr int kSynthKMax = 4;  // ring pairs per lane in synthesis (1 .. 4)
 constexpr int kAnalKMax = 8;   // ring pairs per lane and pass in analysis (1 .. 8)
+// Batched analysis waves go up to 12 ring pairs per lane (246 VGPRs, still
+// 2 waves per SIMD like 8): one pass over up to 768 slots instead of two,
+// one cross-lane reduction per chunk of degrees instead of two.
+constexpr int kBatchAnalKMax = 12;
 constexpr int kAnalLC = 8;     // degrees per reduction chunk in analysis
 // Coefficient tiles of kTL degrees x kTM orders (pack / unpack).
 constexpr int kTM = 64, kTL = 32;
@@ -389,7 +393,7 @@ inline SynthItems build_synth_items(const std::vector<BField> &fields, const std
 // launch each (out[K - 1]).  The sums start at slot max(s0, smin): the DH
 // pole slot carries no analysis weight.
 inline void build_anal_items(const std::vector<BField> &fields, const std::vector<int> &smin, int anal_kmax,
-                             std::vector<Item2> (&out)[kAnalKMax])
+                             std::vector<Item2> (&out)[kBatchAnalKMax])
 {
     std::vector<WorkItem> wa;
     for (int i = 0; i < (int)fields.size(); ++i) {

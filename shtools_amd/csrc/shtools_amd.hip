@@ -516,9 +516,10 @@ static AnalLegKernel anal_leg_kernel(int K)
 }
 static BatchAnalLegKernel batch_anal_leg_kernel(int K)
 {
-    static const BatchAnalLegKernel k[kAnalKMax] = {kb_anal_leg<1>, kb_anal_leg<2>, kb_anal_leg<3>, kb_anal_leg<4>,
-                                                    kb_anal_leg<5>, kb_anal_leg<6>, kb_anal_leg<7>, kb_anal_leg<8>};
-    return k[std::min(std::max(K, 1), kAnalKMax) - 1];
+    static const BatchAnalLegKernel k[kBatchAnalKMax] = {
+        kb_anal_leg<1>, kb_anal_leg<2>, kb_anal_leg<3>, kb_anal_leg<4>,  kb_anal_leg<5>,  kb_anal_leg<6>,
+        kb_anal_leg<7>, kb_anal_leg<8>, kb_anal_leg<9>, kb_anal_leg<10>, kb_anal_leg<11>, kb_anal_leg<12>};
+    return k[std::min(std::max(K, 1), kBatchAnalKMax) - 1];
 }
 
 static int grid_blocks(long long work)
@@ -840,15 +841,15 @@ struct BatchPlan {
     // at least mfma_min fields (0: off) leave the VALU items
     int mfma_min = 0;
     int synth_kmax = kSynthKMax;  // ring pairs per lane of the synthesis Legendre waves (1 .. kSynthKMax)
-    int anal_kmax = kAnalKMax;    // ring pairs per lane and pass of the analysis waves (1 .. kAnalKMax)
+    int anal_kmax = kBatchAnalKMax;  // ring pairs per lane and pass of the analysis waves (1 .. kBatchAnalKMax)
     int4 *items_smfma = nullptr;
     int n_smfma = 0;
     int *bfields = nullptr, *bnf = nullptr;  // per bucket: field indices, count
     std::vector<BField> hfields;             // host copies of fields and polar-skip rows
     std::vector<int> hsmin;
     std::vector<void *> retired;             // replaced item lists (freed with the plan)
-    int2 *items_aleg[kAnalKMax] = {};  // by ring pairs per lane K = index + 1
-    int n_aleg[kAnalKMax] = {};
+    int2 *items_aleg[kBatchAnalKMax] = {};  // by ring pairs per lane K = index + 1
+    int n_aleg[kBatchAnalKMax] = {};
     int4 *items_unpack = nullptr;
     int n_unpack = 0;
     std::vector<FftGroup> fft;
@@ -970,9 +971,9 @@ static int batch_synth_items(BatchPlan *b, int mfma_min)
 // ring pairs per lane.  Replaced lists are kept until the plan is destroyed.
 static int batch_anal_items(BatchPlan *b)
 {
-    std::vector<Item2> ia[kAnalKMax];
+    std::vector<Item2> ia[kBatchAnalKMax];
     build_anal_items(b->hfields, b->hsmin, b->anal_kmax, ia);
-    for (int k = 0; k < kAnalKMax; ++k) {
+    for (int k = 0; k < kBatchAnalKMax; ++k) {
         if (b->items_aleg[k]) b->retired.push_back(b->items_aleg[k]);
         b->items_aleg[k] = nullptr;
         b->n_aleg[k] = (int)ia[k].size();
@@ -1195,7 +1196,7 @@ static int batch_analysis(BatchPlan *b, hipStream_t st, const BTables &T)
     }
     {
         ProfScope ps(SHTG_K_ANAL_LEG, st);
-        for (int k = 0; k < kAnalKMax; ++k)
+        for (int k = 0; k < kBatchAnalKMax; ++k)
             if (b->n_aleg[k])
                 hipLaunchKernelGGL(batch_anal_leg_kernel(k + 1), dim3(b->n_aleg[k]), dim3(kWave), 0, st, b->fields,
                                    b->items_aleg[k], T, b->fbuf, b->cbuf);
@@ -1227,7 +1228,7 @@ static float time_legendre(BatchPlan *b, hipStream_t st, int reps, bool synth, h
                 hipLaunchKernelGGL(kb_synth_leg, dim3(b->n_sleg), dim3(kWave), 0, st, b->fields, b->items_sleg, b->T,
                                    b->cbuf, b->fbuf);
         } else {
-            for (int k = 0; k < kAnalKMax; ++k)
+            for (int k = 0; k < kBatchAnalKMax; ++k)
                 if (b->n_aleg[k])
                     hipLaunchKernelGGL(batch_anal_leg_kernel(k + 1), dim3(b->n_aleg[k]), dim3(kWave), 0, st,
                                        b->fields, b->items_aleg[k], b->T, b->fbuf, b->cbuf);
@@ -1279,7 +1280,7 @@ static int batch_tune_legendre(BatchPlan *b, hipStream_t st, int reps, bool tune
         pick(thr, true, [&](int v) { return batch_synth_items(b, v); }, mm);
     }
     if (!rc && tune_analysis)
-        pick({4, 6, kAnalKMax}, false, [&](int v) { b->anal_kmax = v; return batch_anal_items(b); }, ak);
+        pick({4, 6, 8, kBatchAnalKMax}, false, [&](int v) { b->anal_kmax = v; return batch_anal_items(b); }, ak);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return rc;
@@ -2501,7 +2502,7 @@ int shtg_batch_get_tuning(shtg_batch_plan *plan, int *synth_kmax, int *anal_kmax
 int shtg_batch_set_tuning(shtg_batch_plan *plan, int synth_kmax, int anal_kmax, int mfma_min)
 {
     if (!plan) return ST_DIMS;
-    if (synth_kmax < 1 || synth_kmax > kSynthKMax || anal_kmax < 1 || anal_kmax > kAnalKMax || mfma_min < 0 ||
+    if (synth_kmax < 1 || synth_kmax > kSynthKMax || anal_kmax < 1 || anal_kmax > kBatchAnalKMax || mfma_min < 0 ||
         mfma_min == 1)
         return ST_BOUNDS;
     BatchPlan *b = reinterpret_cast<BatchPlan *>(plan);

@@ -382,12 +382,12 @@ static void check_layout_case(const std::vector<int> &lmax, const std::vector<in
         }
 
     // analysis Legendre items: every (field, order) once, lanes within the bound
-    for (int ak : {1, 3, kAnalKMax}) {
-        std::vector<Item2> ia[kAnalKMax];
+    for (int ak : {1, 3, kAnalKMax, kBatchAnalKMax}) {
+        std::vector<Item2> ia[kBatchAnalKMax];
         build_anal_items(lay.fields, lay.smin, ak, ia);
         std::vector<std::vector<char>> cov(nf);
         for (int i = 0; i < nf; ++i) cov[i].assign(lay.fields[i].L + 1, 0);
-        for (int k = 0; k < kAnalKMax; ++k) {
+        for (int k = 0; k < kBatchAnalKMax; ++k) {
             CHECK(k < ak || ia[k].empty(), "%s: analysis list %d beyond kmax %d", name, k + 1, ak);
             for (const Item2 &t : ia[k]) {
                 CHECK(t.x >= 0 && t.x < nf, "%s: analysis item field", name);

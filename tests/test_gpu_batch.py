@@ -134,3 +134,37 @@ def test_sampling_per_field_matches_plain(torch):
         assert coef_err(back[i].cpu().numpy(), cs_host[i]) <= 1e-11, (i, L, s)
     with pytest.raises(B.SHToolsError):
         DHBatch(Ls, sampling=ss[:-1])
+
+
+def test_wide_analysis_waves_against_oracle_and_two_passes(torch):
+    """Orders of 513 to 768 analysis slots are summed in one pass of 9 to 12
+    ring pairs per lane (DH2, lmax 520, 600, 680, 760: one field per wave
+    width).  The coefficients agree with the oracle's analysis of the same
+    grids (src/SHExpandDH.f95:463-702) to 1e-12, and with the same plan
+    capped at 8 ring pairs per lane (two passes, the sums associated
+    differently) and the plain path to 1e-14."""
+    from shtools_amd import backend as B
+    from shtools_amd.batch import DHBatch
+    Ls = [520, 600, 680, 760]
+    print(f"case wide analysis waves lmax {Ls}", flush=True)
+    rng = np.random.default_rng(71)
+    cs_host = [rand_cilm(rng, L) for L in Ls]
+    cilm = [torch.from_numpy(c).cuda() for c in cs_host]
+    b = DHBatch(Ls, sampling=2)
+    assert b.tuning()["anal_kmax"] == 12
+    grids = [g.clone() for g in b.makegrid(cilm)]
+    wide = [c.clone() for c in b.expand(grids)]
+    t = b.tuning()
+    b.set_tuning(dict(t, anal_kmax=8))
+    narrow = b.expand(grids)
+    torch.cuda.synchronize()
+    for i, L in enumerate(Ls):
+        g_host = grids[i].cpu().numpy()
+        c_ref = oracle.SHExpandDH(g_host, sampling=2, parallel=True)
+        ce = coef_err(wide[i].cpu().numpy(), c_ref)
+        print(f"  lmax {L}: coef err {ce:.2e}", flush=True)
+        assert ce <= 1e-12, L
+        assert coef_err(wide[i].cpu().numpy(), narrow[i].cpu().numpy()) <= 1e-14, L
+        c_plain = B.SHExpandDH(grids[i], sampling=2)
+        assert coef_err(wide[i].cpu().numpy(), c_plain.cpu().numpy()) <= 1e-14, L
+        assert coef_err(wide[i].cpu().numpy(), cs_host[i]) <= 1e-11, L
